@@ -1,0 +1,200 @@
+"""Throughput of the canonical k-mer counting path on one MI355X.
+
+One step = one pass of the hot path over the whole read set, which is
+already resident in HBM as the framed "seq\\n" byte stream: clear the table,
+fused parse + canonical 31-mers + Bloom singleton split + table count, then
+pack the reference's high/low export records into device buffers.
+value = k-mers counted per second over the timed steps.
+
+Default workload: repeat_100mbp (100 Mbp genome with 30% diverged repeat
+copies, 20M x 150 bp reads at 30x, 0.5% error, k=31, Bloom split on).
+The CPU baseline times the reference binary (oracle/_ref/kmc_ref, built from
+the reference sources) on a bounded prefix of the same reads, on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse_args():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="repeat_100mbp", choices=["repeat_100mbp", "count_5mbp"])
+    ap.add_argument("--reads", type=int, default=0, help="override the read count (smaller runs)")
+    ap.add_argument("--no-bloom", action="store_true")
+    ap.add_argument("--cpu-sample-reads", type=int, default=1_000_000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    return ap.parse_args()
+
+
+def cpu_baseline(ws, sample_reads, k, genome):
+    """Reference Tree binary on the first sample_reads reads (FASTQ)."""
+    from ktcountermemorygreedy_amd import kc  # noqa: F401
+    ref = os.path.join(ROOT, "oracle", "_ref", "kmc_ref")
+    threads = max(6, min(16, os.cpu_count() or 6))
+    sample = ws.scaled(sample_reads)
+    fq = sample.reads(fastq=True, genome=genome)
+    if os.path.exists(ref):
+        tmp = tempfile.mkdtemp(prefix="kcbench_")
+        path = os.path.join(tmp, "sample.fastq")
+        with open(path, "wb") as f:
+            f.write(fq.tobytes())
+        os.makedirs(os.path.join(tmp, "out"))
+        t0 = time.time()
+        out = subprocess.run([ref, path, str(k), str(threads), "16", os.path.join(tmp, "out") + "/", "1024", "2",
+                              "4294967295", "4294967295"], stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                             text=True, timeout=600).stdout
+        wall = time.time() - t0
+        subprocess.run(["rm", "-rf", tmp])
+        kmers = int(re.search(r"Total read k-mer count: (\d+)", out).group(1))
+        us = int(re.search(r"Total elapsed us: (\d+)", out).group(1))
+        return {"value": kmers / (us * 1e-6), "unit": "k-mers/s", "cores": threads, "kind": "reference",
+                "sample": "first %d reads of the workload as FASTQ (%d k-mers), Tree <fastq> %d %d 16 <tmp> "
+                          "1024 2 4294967295 4294967295; elapsed %.2f s (wall %.2f s)"
+                          % (sample_reads, kmers, k, threads, us * 1e-6, wall)}
+    # no reference build on this host: time the oracle port (single thread)
+    from oracle import oracle as O
+    seq = O.fastq_frame(fq.tobytes())
+    t0 = time.time()
+    keys = O.extract(seq, k)
+    O.count(keys, O.words_for_k(k))
+    dt = time.time() - t0
+    return {"value": keys.shape[0] / dt, "unit": "k-mers/s", "cores": 1, "kind": "port",
+            "sample": "first %d reads, oracle extract+count" % sample_reads}
+
+
+def main():
+    args = parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every box in this setup has one MI355X; --gpus %d is not "
+              "supported (the multi-GPU design is described in DESIGN.md)" % args.gpus)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from ktcountermemorygreedy_amd import kc
+    from ktcountermemorygreedy_amd import workloads as W
+
+    ws = {"repeat_100mbp": W.REPEAT_100MBP, "count_5mbp": W.COUNT_5MBP}[args.workload]
+    if args.reads:
+        ws = ws.scaled(args.reads, ws.name)
+    k = 31
+    bloom = args.workload == "repeat_100mbp" and not args.no_bloom
+
+    t0 = time.time()
+    genome = ws.genome()
+    seq_host = ws.reads(genome=genome)
+    gen_s = time.time() - t0
+    S = seq_host.size
+    dev = torch.device("cuda", 0)
+    d_seq = torch.empty(S + 16, dtype=torch.uint8, device=dev)
+    d_seq[:S].copy_(torch.from_numpy(seq_host))
+    torch.cuda.synchronize()
+
+    ctx = kc.Context(0)
+    # table sized for the workload: keys reaching the table (Bloom on: k-mers
+    # seen at least twice, plus false positives) and the FIRST list
+    est_distinct = int(ws.genome_len * 1.1 + S * ws.sub_rate * k * 1.1)
+    cap = 1 << max(20, int(np.ceil(np.log2(est_distinct * (0.5 if bloom else 1.0) * 1.6))))
+    table = kc.Table(ctx, k, count_max=0xFFFFFFFF, capacity=cap, max_distinct=est_distinct, bloom=bloom,
+                     bloom_blocks=(1 << max(12, int(np.ceil(np.log2(est_distinct / 4))))) if bloom else 0,
+                     low_capacity=est_distinct if bloom else 0)
+    stream = torch.cuda.ExternalStream(ctx.stream_ptr, device=dev)
+
+    # export buffers sized once (the export pass writes into them each step)
+    def one_step(ev=None):
+        table.clear()
+        if ev:
+            ev[0].record(stream)
+        table.count_seq(d_seq, S)
+        if ev:
+            ev[1].record(stream)
+        hb, lb = table_export_sizes()
+        return hb, lb
+
+    import ctypes as C
+    bufs = {}
+
+    def table_export_sizes():
+        hb, lb = C.c_uint64(), C.c_uint64()
+        if not bufs:
+            kc.check(kc.lib().kc_table_export_device(table.h, None, 0, C.byref(hb), None, 0, C.byref(lb)),
+                     "export sizes")
+            bufs["h"] = torch.empty(int(hb.value * 1.05) + 4096, dtype=torch.uint8, device=dev)
+            bufs["l"] = torch.empty(int(lb.value * 1.05) + 4096, dtype=torch.uint8, device=dev)
+        kc.check(kc.lib().kc_table_export_device(table.h, kc._ptr(bufs["h"]), bufs["h"].numel(), C.byref(hb),
+                                                 kc._ptr(bufs["l"]), bufs["l"].numel(), C.byref(lb)), "export")
+        return hb.value, lb.value
+
+    for _ in range(args.warmup):
+        one_step()
+    ctx.sync()
+    torch.cuda.synchronize()
+
+    events = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+              for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        hb, lb = one_step(events[i])
+    ctx.sync()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    count_ms = [a.elapsed_time(b) for a, b in events]
+
+    st = table.stats()
+    assert st["errors"] == 0, st
+    K = st["kmers"]
+    # size-independent checks: every window counted once; merged histogram
+    # mass equals the number of windows (no count exceeds the bin range here)
+    bins = table.histogram(1, 100000)
+    mass = int((bins * np.arange(1, bins.size + 1)).sum())
+    windows = int(np.sum(np.maximum(np.diff(np.flatnonzero(np.concatenate(
+        [[True], seq_host == 10]))) - 1 - (k - 1), 0))) if ws.n_rate == 0 else K
+
+    k_table = K - st["low"]
+    alg_bytes = S + 24 * k_table + (32 * K if bloom else 0)
+    avg_count_s = float(np.mean(count_ms)) * 1e-3
+    achieved = alg_bytes / avg_count_s / 1e9
+    step_s = elapsed / args.steps
+    value = K / step_s
+
+    cpu = None if args.no_cpu_baseline else cpu_baseline(ws, min(args.cpu_sample_reads, ws.n_reads), k, genome)
+    line = {
+        "metric": "canonical 31-mers counted/s on 1 MI355X vs host CPU",
+        "value": value, "unit": "k-mers/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": step_s * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u64", "data": "synthetic",
+        "config": {"workload": ws.name, "reads": ws.n_reads, "read_len": ws.read_len, "k": k,
+                   "bloom": bloom, "seq_bytes": S, "kmers": K, "table_keys": st["distinct"],
+                   "first_occurrences": st["low"], "table_slots": st["capacity"],
+                   "export_high_bytes": hb, "export_low_bytes": lb,
+                   "step": "clear + fused parse/canonical/Bloom/count + export packing (device)"},
+        "roofline": {"bound": "hbm", "kernel": "k_count_seq", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "alg_bytes_per_launch": alg_bytes, "avg_launch_ms": avg_count_s * 1e3},
+        "cpu_baseline": cpu,
+        "checks": {"windows_match": windows == K, "hist_mass_equals_kmers": mass == K},
+        "gen_seconds": gen_s,
+    }
+    print(json.dumps(line), flush=True)
+    table.close()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,139 @@
+/*
+ * kc_api.h — C ABI of the MI355X k-mer counting engine (libkc.so).
+ *
+ * The engine is a drop-in for KTCounterMemoryGreedy's counting path.  The
+ * reference is header-only C++ with no FFI; each entry point below names the
+ * reference interface it replaces (file:line).  Plain pointers and sizes
+ * only.  Pointers documented as "device" must point to HBM (hipMalloc or a
+ * torch CUDA tensor); "host" pointers are ordinary process memory.
+ *
+ * Status: every call returns an int, KC_OK (0) or a negative KC_E* code.
+ * Nothing calls exit().  Calls on one kc_ctx are issued in order on that
+ * context's HIP stream; a kc_ctx must be used by one host thread at a time.
+ *
+ * Key layout (identical to kmer<N>, kmer.h:11-119 and GetKmer.h:71-77):
+ * nw = 1 (k<=32) or 2 (k<=63) little-endian u64 words, word 0 most
+ * significant, string position j at bit 64*nw-2k+2j; low padding bits zero.
+ */
+#ifndef KC_API_H
+#define KC_API_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KC_OK 0
+#define KC_EINVAL (-1)   /* bad argument */
+#define KC_ENOMEM (-2)   /* device or host allocation failed */
+#define KC_EFULL (-3)    /* table or FIRST-occurrence list overflowed */
+#define KC_EDEVICE (-4)  /* HIP runtime error */
+#define KC_EIO (-5)      /* file error */
+#define KC_ENOTSUP (-6)  /* configuration not supported by this build */
+
+typedef struct kc_ctx kc_ctx;
+typedef struct kc_table kc_table;
+
+/* Table configuration.  Mirrors the Tree CLI knobs (main.cpp:484-554):
+ * k, filter_min, filter_max, count_max; capacity replaces map_capacity. */
+typedef struct kc_table_config {
+    uint32_t k;            /* 1..63 */
+    uint32_t count_max;    /* saturation value (main.cpp:599-615), >= 1 */
+    uint32_t filter_min;   /* high.bin filter (FinalDrainWriter.h:101-104) */
+    uint32_t filter_max;
+    uint64_t capacity;     /* table slots, power of two; 0 = derive from max_distinct */
+    uint64_t max_distinct; /* sizing hint when capacity == 0 */
+    uint32_t bloom;        /* 1 = Bloom singleton split (BloomFilter.h:149-298) */
+    uint32_t reserved;
+    uint64_t bloom_blocks; /* {BF1,BF2} blocks, power of two; 0 = derive */
+    uint64_t low_capacity; /* FIRST-occurrence list capacity; 0 = derive */
+} kc_table_config;
+
+typedef struct kc_stats {
+    uint64_t kmers;        /* windows counted (main.cpp:418 "Total read k-mer count") */
+    uint64_t distinct;     /* keys stored in the table */
+    uint64_t low;          /* FIRST occurrences routed to low.bin (Bloom on) */
+    uint64_t capacity;     /* table slots */
+    uint64_t errors;       /* nonzero: overflow or failed wait; results invalid */
+} kc_stats;
+
+const char *kc_strerror(int status);
+
+/* ---- context ---------------------------------------------------------- */
+int kc_ctx_create(int device, kc_ctx **out);
+void kc_ctx_destroy(kc_ctx *ctx);
+int kc_ctx_sync(kc_ctx *ctx);
+/* hipStream_t of the context, for callers that record events on it. */
+void *kc_ctx_stream(kc_ctx *ctx);
+
+/* ---- counting table: replaces ConcurrentCountingHashMap<N>
+ *      (ConcurrentCountingHashMap.h:211-312) and, on the production path,
+ *      ConcurrentMap<N> (ConcurrentMap.h:231-300) fed by KmerTree. ---- */
+int kc_table_create(kc_ctx *ctx, const kc_table_config *cfg, kc_table **out);
+void kc_table_destroy(kc_table *t);
+/* Empties the table, Bloom filter, FIRST list and counters (device fill). */
+int kc_table_clear(kc_table *t);
+
+/* Fused parse + canonical extraction + (Bloom split) + count over a
+ * framed byte stream in device memory: sequence lines each followed by
+ * '\n' (FastqReader.h:657-687), any byte outside ACGTUacgtu resetting the
+ * window (GetKmer.h:58-82, FastqParser.h:245-518).  Asynchronous. */
+int kc_table_count_seq(kc_table *t, const uint8_t *d_seq, uint64_t nbytes);
+/* Same from host memory: staged through pinned buffers, H2D overlapped
+ * with counting; chunks cut with a k-1 byte halo (FastqReader.h:542-557). */
+int kc_table_count_seq_host(kc_table *t, const uint8_t *h_seq, uint64_t nbytes);
+/* increment(key, inc) for n keys (ConcurrentCountingHashMap.h:238);
+ * d_inc may be NULL for +1 each.  Keys: n*nw device words. */
+int kc_table_increment(kc_table *t, const uint64_t *d_keys, uint64_t n,
+                       const uint32_t *d_inc);
+/* Point lookup: merged count min(c, count_max) of each key, 0 if absent
+ * (FlatConcurrentHashMap::find_prepared, FlatConcurrentHashMap.h:142-205). */
+int kc_table_query(kc_table *t, const uint64_t *d_keys, uint64_t n,
+                   uint32_t *d_counts);
+int kc_table_stats(kc_table *t, kc_stats *out);
+
+/* Export in the reference's on-disk format: infos.bin (main.cpp:590-597),
+ * high.bin (FinalDrainWriter.h:71-146), low.bin (ExportWriter.h:296-313). */
+int kc_table_export_dir(kc_table *t, const char *dir);
+/* Device-side export: packed high/low records into device buffers.
+ * Either buffer may be NULL to query the required sizes. */
+int kc_table_export_device(kc_table *t, uint8_t *d_high, uint64_t high_cap,
+                           uint64_t *high_bytes, uint8_t *d_low,
+                           uint64_t low_cap, uint64_t *low_bytes);
+/* histogram_tool precise over the table's exported view
+ * (histogram_tool.cpp:734-790): bins[f-min] for f=min..max, host int64. */
+int kc_table_histogram(kc_table *t, uint32_t min_freq, uint32_t max_freq,
+                       int64_t *h_bins);
+
+/* ---- sorts: ExportRecordRadixSort (ExportRecordRadixSort.h:256-269) and
+ *      radix_sort_export_records_12bit (ExportRecordRadixSort12.h:639-646).
+ * Records are {nw u64 key words, u32 count} at record_bytes stride
+ * (12 packed, or 16/24 as ExportRecord<N> in memory).  digit_bits 8 gives
+ * the 8-bit sort's key range, 12 the 12-bit sort's (common_prefix_bases
+ * used only there).  The result lands in d_recs or d_scratch;
+ * *result_in_scratch says which.  Stable, like the reference. ---- */
+int kc_sort_records(kc_ctx *ctx, void *d_recs, void *d_scratch, uint64_t n,
+                    uint32_t nw, uint32_t k, uint32_t common_prefix_bases,
+                    uint32_t digit_bits, uint32_t record_bytes,
+                    int *result_in_scratch);
+
+/* ---- hashes of the raw key bytes, bit-exact with the vendored headers ---- */
+#define KC_HASH_XXH3_64 0      /* XXH3_64bits (xxhash.h:6428) */
+#define KC_HASH_XXH3_128_LO 1  /* XXH3_128bits low64 (xxhash.h:7249) */
+#define KC_HASH_XXH3_128_HI 2  /* XXH3_128bits high64 */
+#define KC_HASH_RAPIDHASH 3    /* rapidhash (rapidhash.h:500) */
+#define KC_HASH_KOMIHASH 4     /* komihash seed 0 (komihash.h:763) */
+#define KC_HASH_CONCURRENT_MAP 5 /* ConcurrentMap::hash_func (ConcurrentMap.h:141-148) */
+#define KC_HASH_FUNC 6         /* hash_func<N> (HashFunction.h:44-63) */
+int kc_hash_keys(kc_ctx *ctx, int kind, const uint64_t *d_keys, uint64_t n,
+                 uint32_t nw, uint64_t *d_out);
+/* ConcurrentBloomFilter::prepare_insert (BloomFilter.h:215-222). */
+int kc_bloom_probe_keys(kc_ctx *ctx, const uint64_t *d_keys, uint64_t n,
+                        uint32_t nw, uint64_t capacity, uint64_t *d_block,
+                        uint64_t *d_mask);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

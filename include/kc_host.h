@@ -1,0 +1,61 @@
+/*
+ * kc_host.h — host-side ingest and synthetic-input helpers of libkc.so.
+ *
+ * FASTQ framing keeps line 2 of every 4-line record followed by '\n', the
+ * byte stream the reference's reader hands to its parsers
+ * (FastqReader.h:478-487, 657-687); gzip input is detected by the ".gz"
+ * suffix as in FastqReader.h:423,575 and inflated with zlib on the host.
+ *
+ * The synthetic generator has no counterpart in the reference; it produces
+ * the seeded workloads the engine is measured and checked on.
+ */
+#ifndef KC_HOST_H
+#define KC_HOST_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Frames an in-memory FASTQ buffer; out must hold len bytes. Returns bytes. */
+uint64_t kc_fastq_frame(const uint8_t *fq, uint64_t len, uint8_t *out);
+/* Reads and frames one FASTQ file (plain or .gz).  *out is malloc'ed and
+ * released with kc_host_free.  Returns KC_OK or KC_EIO. */
+int kc_fastq_frame_file(const char *path, uint8_t **out, uint64_t *out_len);
+void kc_host_free(void *p);
+
+typedef struct kc_genome_spec {
+    uint64_t seed;
+    uint64_t length;            /* bases */
+    uint32_t n_segments;        /* repeat families (0 = uniform genome) */
+    uint32_t segment_len;
+    double repeat_fraction;     /* share of the genome written as copies */
+    double segment_divergence;  /* per-base substitution rate of a copy */
+} kc_genome_spec;
+
+typedef struct kc_read_spec {
+    uint64_t seed;
+    uint64_t n_reads;
+    uint32_t read_len;          /* short-read length */
+    uint32_t both_strands;      /* 1: reverse-complement half the reads */
+    double sub_rate;            /* short-read substitution rate */
+    double n_rate;              /* per-base N rate */
+    double long_fraction;       /* share of reads (by count) that are long */
+    double long_mu, long_sigma; /* lognormal parameters of long-read length */
+    double long_sub_rate;
+    uint32_t min_len;           /* lower clamp of long-read length */
+    uint32_t threads;           /* worker threads (0 = hardware) */
+} kc_read_spec;
+
+/* Writes `length` ACGT bytes. */
+int kc_synth_genome(const kc_genome_spec *spec, uint8_t *out);
+/* Bytes kc_synth_reads will write (fastq = 1: FASTQ records; 0: framed
+ * "seq\n" stream).  Negative on error. */
+int64_t kc_synth_reads_size(const uint8_t *genome, uint64_t glen, const kc_read_spec *spec, int fastq);
+int64_t kc_synth_reads(const uint8_t *genome, uint64_t glen, const kc_read_spec *spec, int fastq,
+                       uint8_t *out, uint64_t cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1048 @@
+// MI355X (gfx950) k-mer counting engine: context, counting table, fused
+// parse/canonical/Bloom/count kernel, point queries, export packing and the
+// histogram merge.  All integer work: no MFMA.  Bound: HBM (random 16-32 B
+// read-modify-writes on the table and Bloom filter, streaming input bytes).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "kc_hash_dev.h"
+#include "kc_internal.h"
+
+namespace {
+thread_local char g_last_error[256];
+constexpr int kBlock = 256;
+constexpr int kSeg = 64;                       // bytes of window ends per thread
+constexpr int kTile = kBlock * kSeg;           // bytes per block
+constexpr uint32_t kValidMask = (1u << 0) | (1u << 2) | (1u << 6) | (1u << 19) | (1u << 20);
+}  // namespace
+
+void kc_set_error(const char *what, hipError_t e)
+{
+    snprintf(g_last_error, sizeof g_last_error, "%s: %s", what, hipGetErrorString(e));
+}
+
+extern "C" const char *kc_strerror(int s)
+{
+    switch (s) {
+    case KC_OK: return "ok";
+    case KC_EINVAL: return "invalid argument";
+    case KC_ENOMEM: return "out of memory";
+    case KC_EFULL: return "table or FIRST list full";
+    case KC_EDEVICE: return g_last_error[0] ? g_last_error : "HIP error";
+    case KC_EIO: return "I/O error";
+    case KC_ENOTSUP: return "not supported";
+    default: return "unknown status";
+    }
+}
+
+namespace kc {
+
+struct TableView {
+    uint64_t *slots;
+    uint64_t mask;
+    uint64_t *bloom;       // nullptr when the Bloom split is off
+    uint64_t bloom_mask;
+    uint64_t *low;
+    uint64_t low_cap;
+    unsigned long long *counters;
+    uint32_t k;
+    uint32_t count_max;
+    uint32_t filter_min;
+    uint32_t filter_max;
+};
+
+template <int NW> constexpr int slot_words() { return NW == 1 ? 2 : 4; }
+
+__device__ __forceinline__ uint64_t lanemask_lt()
+{
+    return (1ULL << __lane_id()) - 1ULL;
+}
+
+__device__ __forceinline__ uint64_t ld_agent(const uint64_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void add_agent(uint64_t *p, uint64_t v)
+{
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Wave sum of a 32-bit value, then one device atomic per wave.
+__device__ __forceinline__ void wave_flush(unsigned long long *ctr, uint32_t v)
+{
+    uint64_t s = v;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (__lane_id() == 0 && s) atomicAdd(ctr, (unsigned long long)s);
+}
+
+// ---- table insert / add ------------------------------------------------
+// Open addressing, linear probing.  A slot is claimed with a 64-bit CAS on
+// its first key word; counts are 64-bit atomic adds (saturation is applied
+// when results are read, so min(true count, count_max) is exact).
+template <int NW>
+__device__ __forceinline__ bool table_add(uint64_t *slots, uint64_t mask, const uint64_t *key,
+                                          uint64_t inc, uint32_t &n_new, uint32_t &n_err)
+{
+    uint64_t idx = xxh3_64<NW>(key) & mask;
+    for (uint64_t probe = 0; probe <= mask; ++probe) {
+        uint64_t *s = slots + idx * slot_words<NW>();
+        const uint64_t prev = atomicCAS((unsigned long long *)s, (unsigned long long)KC_EMPTY,
+                                        (unsigned long long)key[0]);
+        if constexpr (NW == 1) {
+            if (prev == KC_EMPTY || prev == key[0]) {
+                n_new += prev == KC_EMPTY;
+                add_agent(s + 1, inc);
+                return true;
+            }
+        } else {
+            // Two-word key: the CAS winner publishes word 1; a thread that
+            // finds word 0 equal waits for word 1 (bounded) and compares.
+            const bool owner = prev == KC_EMPTY;
+            if (owner) __hip_atomic_store(s + 1, key[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __builtin_amdgcn_wave_barrier();
+            if (owner) {
+                ++n_new;
+                add_agent(s + 2, inc);
+                return true;
+            }
+            if (prev == key[0]) {
+                uint64_t w1 = ld_agent(s + 1);
+                for (int spin = 0; w1 == KC_EMPTY && spin < (1 << 16); ++spin) {
+                    __builtin_amdgcn_s_sleep(1);
+                    w1 = ld_agent(s + 1);
+                }
+                if (w1 == KC_EMPTY) {
+                    ++n_err;
+                    return false;
+                }
+                if (w1 == key[1]) {
+                    add_agent(s + 2, inc);
+                    return true;
+                }
+            }
+        }
+        idx = (idx + 1) & mask;
+    }
+    ++n_err;
+    return false;
+}
+
+// Lookup: returns the raw stored count (0 when absent).
+template <int NW>
+__device__ __forceinline__ uint64_t table_find(const uint64_t *slots, uint64_t mask, const uint64_t *key)
+{
+    uint64_t idx = xxh3_64<NW>(key) & mask;
+    for (uint64_t probe = 0; probe <= mask; ++probe) {
+        const uint64_t *s = slots + idx * slot_words<NW>();
+        const uint64_t w0 = s[0];
+        if (w0 == KC_EMPTY) return 0;
+        if (w0 == key[0]) {
+            if constexpr (NW == 1) return s[1];
+            else if (s[1] == key[1]) return s[2];
+        }
+        idx = (idx + 1) & mask;
+    }
+    return 0;
+}
+
+// Bloom singleton split state machine (BloomFilter.h:233-273): returns
+// 0 FIRST, 1 SECOND, 2 THIRD_PLUS.  Only atomic results decide a FIRST, so
+// each distinct key gets at most one FIRST.
+template <int NW>
+__device__ __forceinline__ int bloom_classify(uint64_t *bins, uint64_t bmask, const uint64_t *key)
+{
+    uint64_t blk, m;
+    bloom_probe<NW>(key, bmask, blk, m);
+    uint64_t *cell = bins + 2 * blk;
+    if ((ld_agent(cell) & m) == m) return 2;
+    if ((ld_agent(cell + 1) & m) == m) {
+        const uint64_t b1 = __hip_atomic_fetch_or(cell, m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return ((b1 & m) == m) ? 2 : 1;
+    }
+    const uint64_t b2 = __hip_atomic_fetch_or(cell + 1, m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if ((b2 & m) == m) {
+        const uint64_t b1 = __hip_atomic_fetch_or(cell, m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return ((b1 & m) == m) ? 2 : 1;
+    }
+    return 0;
+}
+
+// ---- rolling canonical state (GetKmer.h:58-82) -------------------------
+template <int NW> struct Roll;
+
+template <> struct Roll<1> {
+    uint64_t f = 0, r = 0;
+    uint64_t back_mask;
+    uint32_t rev_shift;
+    __device__ Roll(uint32_t k)
+    {
+        back_mask = (k % 32) ? (((1ULL << (2 * (k % 32))) - 1ULL) << (64 - 2 * (k % 32))) : ~0ULL;
+        rev_shift = (k % 32) ? 2 * (32 - k % 32) : 0;
+    }
+    __device__ __forceinline__ void push(uint64_t code)
+    {
+        f = ((f >> 2) | (code << 62)) & back_mask;
+        r = (r << 2) | ((code ^ 3) << rev_shift);
+    }
+    __device__ __forceinline__ void canon(uint64_t *out) const { out[0] = f < r ? f : r; }
+};
+
+template <> struct Roll<2> {
+    uint64_t f0 = 0, f1 = 0, r0 = 0, r1 = 0;
+    uint64_t back_mask;
+    uint32_t rev_shift;
+    __device__ Roll(uint32_t k)
+    {
+        back_mask = (k % 32) ? (((1ULL << (2 * (k % 32))) - 1ULL) << (64 - 2 * (k % 32))) : ~0ULL;
+        rev_shift = (k % 32) ? 2 * (32 - k % 32) : 0;
+    }
+    __device__ __forceinline__ void push(uint64_t code)
+    {
+        // data[0] most significant; the tail word (data[1]) is masked.
+        f1 = ((f1 >> 2) | (f0 << 62)) & back_mask;
+        f0 = (f0 >> 2) | (code << 62);
+        r0 = (r0 << 2) | (r1 >> 62);
+        r1 = (r1 << 2) | ((code ^ 3) << rev_shift);
+    }
+    __device__ __forceinline__ void canon(uint64_t *out) const
+    {
+        const bool fwd = (f0 < r0) || (f0 == r0 && f1 < r1);
+        out[0] = fwd ? f0 : r0;
+        out[1] = fwd ? f1 : r1;
+    }
+};
+
+// ---- fused count kernel ------------------------------------------------
+// One block stages kTile input bytes (+ the previous segment as halo) in
+// LDS, transposed so that lane t reading its own dword j hits consecutive
+// banks.  Thread t then rolls over halo + its 64-byte segment and handles
+// every window whose last byte lies in the segment (and at or after
+// emit_from, which excludes windows owned by an earlier chunk).
+template <int NW, bool BLOOM>
+__global__ __launch_bounds__(kBlock) void k_count_seq(const uint8_t *__restrict__ seq, uint64_t n,
+                                                      uint64_t emit_from, TableView tv)
+{
+    constexpr int kHaloWords = NW == 1 ? 8 : 16;   // >= k-1 bytes of history
+    __shared__ uint32_t lds[16][kBlock + 2];
+    const uint32_t t = threadIdx.x;
+    const uint64_t base = (uint64_t)blockIdx.x * kTile;
+
+    // Stage: 1024 uint4 of this tile plus the previous block's last segment.
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const uint32_t q = t + kBlock * r;      // uint4 index inside the tile
+        const uint64_t off = base + 16ull * q;
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (off + 16 <= n) {
+            v = *reinterpret_cast<const uint4 *>(seq + off);
+        } else if (off < n) {
+            uint32_t w[4] = {0, 0, 0, 0};
+            for (uint64_t b = off; b < n; ++b) w[(b - off) >> 2] |= (uint32_t)seq[b] << (8 * ((b - off) & 3));
+            v = make_uint4(w[0], w[1], w[2], w[3]);
+        }
+        const uint32_t seg = q >> 2, j0 = (q & 3) * 4;
+        lds[j0 + 0][seg + 1] = v.x;
+        lds[j0 + 1][seg + 1] = v.y;
+        lds[j0 + 2][seg + 1] = v.z;
+        lds[j0 + 3][seg + 1] = v.w;
+    }
+    if (t < 16) {
+        uint32_t v = 0;
+        if (base >= kSeg) {
+            const uint64_t off = base - kSeg + 4ull * t;
+            if (off + 4 <= n) v = *reinterpret_cast<const uint32_t *>(seq + off);
+        }
+        lds[t][0] = v;
+    }
+    __syncthreads();
+
+    const uint64_t start = base + (uint64_t)t * kSeg;
+    uint32_t n_kmers = 0, n_new = 0, n_err = 0;
+    if (start < n) {
+        Roll<NW> roll(tv.k);
+        uint32_t have = 0;
+        // halo: last kHaloWords dwords of the previous segment (column t)
+        for (int j = 16 - kHaloWords; j < 16; ++j) {
+            const uint32_t v = lds[j][t];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const uint32_t c = (v >> (8 * b)) & 0xDF;
+                const uint32_t d = c - 0x41;
+                const bool valid = d < 32 && ((kValidMask >> d) & 1);
+                roll.push(((c >> 1) ^ (c >> 2)) & 3);
+                have = valid ? have + 1 : 0;
+            }
+        }
+        for (int j = 0; j < 16; ++j) {
+            const uint32_t v = lds[j][t + 1];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const uint32_t c = (v >> (8 * b)) & 0xDF;
+                const uint32_t d = c - 0x41;
+                const bool valid = d < 32 && ((kValidMask >> d) & 1);
+                roll.push(((c >> 1) ^ (c >> 2)) & 3);
+                have = valid ? have + 1 : 0;
+                const uint64_t pos = start + 4 * j + b;
+                const bool emit = have >= tv.k && pos >= emit_from;
+                if (emit) {
+                    uint64_t key[NW];
+                    roll.canon(key);
+                    ++n_kmers;
+                    bool to_table = true;
+                    if constexpr (BLOOM) {
+                        const bool first = bloom_classify<NW>(tv.bloom, tv.bloom_mask, key) == 0;
+                        const uint64_t firsts = __ballot(first);
+                        if (firsts) {
+                            const int leader = __ffsll((unsigned long long)firsts) - 1;
+                            unsigned long long b0 = 0;
+                            if (__lane_id() == leader)
+                                b0 = atomicAdd(tv.counters + KC_CNT_LOW, (unsigned long long)__popcll(firsts));
+                            b0 = __shfl(b0, leader, 64);
+                            if (first) {
+                                const uint64_t at = b0 + __popcll(firsts & lanemask_lt());
+                                if (at < tv.low_cap) {
+#pragma unroll
+                                    for (int w = 0; w < NW; ++w) tv.low[at * NW + w] = key[w];
+                                } else {
+                                    ++n_err;
+                                }
+                            }
+                        }
+                        to_table = !first;
+                    }
+                    if (to_table) table_add<NW>(tv.slots, tv.mask, key, 1, n_new, n_err);
+                }
+            }
+        }
+    }
+    wave_flush(tv.counters + KC_CNT_KMERS, n_kmers);
+    wave_flush(tv.counters + KC_CNT_DISTINCT, n_new);
+    wave_flush(tv.counters + KC_CNT_ERRORS, n_err);
+}
+
+// increment(key, inc) for a batch of keys.
+template <int NW>
+__global__ __launch_bounds__(kBlock) void k_increment(const uint64_t *__restrict__ keys, uint64_t n,
+                                                      const uint32_t *__restrict__ inc, TableView tv)
+{
+    uint32_t n_new = 0, n_err = 0;
+    for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) {
+        uint64_t key[NW];
+#pragma unroll
+        for (int w = 0; w < NW; ++w) key[w] = keys[i * NW + w];
+        table_add<NW>(tv.slots, tv.mask, key, inc ? inc[i] : 1, n_new, n_err);
+    }
+    wave_flush(tv.counters + KC_CNT_DISTINCT, n_new);
+    wave_flush(tv.counters + KC_CNT_ERRORS, n_err);
+}
+
+template <int NW>
+__global__ __launch_bounds__(kBlock) void k_query(const uint64_t *__restrict__ keys, uint64_t n,
+                                                  uint32_t *__restrict__ out, TableView tv)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) {
+        uint64_t key[NW];
+#pragma unroll
+        for (int w = 0; w < NW; ++w) key[w] = keys[i * NW + w];
+        const uint64_t c = table_find<NW>(tv.slots, tv.mask, key);
+        out[i] = (uint32_t)(c < tv.count_max ? c : tv.count_max);
+    }
+}
+
+template <int NW>
+__global__ void k_init_slots(uint64_t *slots, uint64_t cap)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < cap; i += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t *s = slots + i * slot_words<NW>();
+        s[0] = KC_EMPTY;
+        s[1] = NW == 1 ? 0 : KC_EMPTY;
+        if constexpr (NW == 2) {
+            s[2] = 0;
+            s[3] = 0;
+        }
+    }
+}
+
+__global__ void k_fill(uint64_t *p, uint64_t n, uint64_t v)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// ---- export ------------------------------------------------------------
+// Packed k-mer bytes (ExportWriter.h:296-313): full words as LE u64, then
+// the top tail_bytes of the masked tail word; high records append the low
+// count_bytes of the count (FinalDrainWriter.h:71-88).
+struct PackSpec {
+    uint32_t full_words, tail_bytes, kmer_bytes, count_bytes;
+    uint64_t tail_mask;
+};
+
+__device__ __forceinline__ void put_bytes(uint8_t *dst, uint64_t v, uint32_t nbytes, uint32_t skip)
+{
+    for (uint32_t b = 0; b < nbytes; ++b) dst[b] = (uint8_t)(v >> (8 * (skip + b)));
+}
+
+template <int NW>
+__device__ __forceinline__ void pack_key(uint8_t *dst, const uint64_t *key, const PackSpec &ps)
+{
+    for (uint32_t w = 0; w < ps.full_words; ++w) put_bytes(dst + 8 * w, key[w], 8, 0);
+    if (ps.tail_bytes) {
+        const uint64_t tail = key[ps.full_words] & ps.tail_mask;
+        put_bytes(dst + 8 * ps.full_words, tail, ps.tail_bytes, 8 - ps.tail_bytes);
+    }
+}
+
+// Wave-aggregated append: returns this lane's record index.
+__device__ __forceinline__ uint64_t wave_append(unsigned long long *ctr, bool want)
+{
+    const uint64_t m = __ballot(want);
+    if (!m) return 0;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    unsigned long long b0 = 0;
+    if (__lane_id() == leader) b0 = atomicAdd(ctr, (unsigned long long)__popcll(m));
+    b0 = __shfl(b0, leader, 64);
+    return b0 + __popcll(m & lanemask_lt());
+}
+
+// Table slots -> high records (and, without an active FIRST list, the
+// singletons -> low records, the simplest valid split: merged views are
+// min(c, count_max) either way).  Records dropped by the high.bin filter
+// follow FinalDrainWriter.h:101-104.  out_* may be null to only count.
+template <int NW>
+__global__ __launch_bounds__(kBlock) void k_export_table(TableView tv, PackSpec ps, bool singles_to_low,
+                                                         uint8_t *out_high, uint64_t high_cap,
+                                                         uint8_t *out_low, uint64_t low_cap)
+{
+    const uint64_t cap = tv.mask + 1;
+    const uint64_t iters = (cap + (uint64_t)gridDim.x * kBlock - 1) / ((uint64_t)gridDim.x * kBlock);
+    for (uint64_t it = 0; it < iters; ++it) {
+        const uint64_t i = (it * gridDim.x + blockIdx.x) * (uint64_t)kBlock + threadIdx.x;
+        bool is_high = false, is_low = false;
+        uint64_t key[NW] = {};
+        uint32_t cc = 0;
+        if (i < cap) {
+            const uint64_t *s = tv.slots + i * slot_words<NW>();
+            if (s[0] != KC_EMPTY) {
+#pragma unroll
+                for (int w = 0; w < NW; ++w) key[w] = s[w];
+                const uint64_t c = s[NW == 1 ? 1 : 2];
+                if (singles_to_low && c == 1) {
+                    is_low = true;
+                } else if (c > 0) {
+                    cc = (uint32_t)(c < tv.count_max ? c : tv.count_max);
+                    is_high = !((uint64_t)cc + 1 < tv.filter_min || cc > tv.filter_max);
+                }
+            }
+        }
+        const uint64_t hi = wave_append(tv.counters + KC_CNT_OUT_HIGH, is_high);
+        const uint64_t lo = wave_append(tv.counters + KC_CNT_OUT_LOW, is_low);
+        if (is_high && out_high && hi < high_cap) {
+            uint8_t *d = out_high + hi * (ps.kmer_bytes + ps.count_bytes);
+            pack_key<NW>(d, key, ps);
+            put_bytes(d + ps.kmer_bytes, cc, ps.count_bytes, 0);
+        }
+        if (is_low && out_low && lo < low_cap) pack_key<NW>(out_low + lo * ps.kmer_bytes, key, ps);
+    }
+}
+
+template <int NW>
+__global__ __launch_bounds__(kBlock) void k_export_low_list(TableView tv, PackSpec ps, uint64_t n,
+                                                            uint8_t *out_low, uint64_t low_base)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) {
+        uint64_t key[NW];
+#pragma unroll
+        for (int w = 0; w < NW; ++w) key[w] = tv.low[i * NW + w];
+        pack_key<NW>(out_low + (low_base + i) * ps.kmer_bytes, key, ps);
+    }
+}
+
+// ---- histogram (histogram_tool precise over the exported view) ----------
+// High pass (HighFrequencyInsertThreadPool.h:150-177): for each exported
+// high record c: skip if c+1 < min; hist[c]++ when c in [min,max].
+// Low pass (LowFrequencyQueryThreadPool.h:186-245): found -> hist[c]--,
+// hist[min(c+1,count_max)]++; missing -> hist[1]++.
+constexpr int kLdsBins = 4096;
+
+struct HistSpec {
+    uint32_t min_freq, max_freq;
+    bool singles_low;  // singletons are low records (no active FIRST list)
+};
+
+__device__ __forceinline__ void hist_add(int *lds_bins, unsigned long long *gbins, const HistSpec &h,
+                                         uint64_t f, int delta)
+{
+    if (f < h.min_freq || f > h.max_freq) return;
+    const uint64_t b = f - h.min_freq;
+    if (b < kLdsBins) atomicAdd(lds_bins + b, delta);
+    else atomicAdd(gbins + b, (unsigned long long)(long long)delta);
+}
+
+__device__ __forceinline__ void hist_flush(int *lds_bins, unsigned long long *gbins, uint64_t nb)
+{
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b < kLdsBins && b < nb; b += kBlock)
+        if (lds_bins[b]) atomicAdd(gbins + b, (unsigned long long)(long long)lds_bins[b]);
+}
+
+// exported high count of a slot, or -1 when the slot yields no high record
+template <int NW>
+__device__ __forceinline__ int64_t high_view(const TableView &tv, uint64_t c, bool singles_low)
+{
+    if (c == 0) return -1;
+    if (singles_low && c == 1) return -1;
+    const uint64_t cc = c < tv.count_max ? c : tv.count_max;
+    if (cc + 1 < tv.filter_min || cc > tv.filter_max) return -1;
+    return (int64_t)cc;
+}
+
+template <int NW>
+__global__ __launch_bounds__(kBlock) void k_hist_table(TableView tv, HistSpec h, unsigned long long *gbins)
+{
+    __shared__ int lds_bins[kLdsBins];
+    for (uint32_t b = threadIdx.x; b < kLdsBins; b += kBlock) lds_bins[b] = 0;
+    __syncthreads();
+    const uint64_t cap = tv.mask + 1;
+    for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < cap; i += (uint64_t)gridDim.x * kBlock) {
+        const uint64_t *s = tv.slots + i * slot_words<NW>();
+        if (s[0] == KC_EMPTY) continue;
+        const uint64_t c = s[NW == 1 ? 1 : 2];
+        if (h.singles_low && c == 1) {
+            // low record whose lookup misses the high set
+            hist_add(lds_bins, gbins, h, 1, 1);
+            continue;
+        }
+        const int64_t cc = high_view<NW>(tv, c, h.singles_low);
+        if (cc < 0) continue;
+        if ((uint64_t)cc + 1 < h.min_freq) continue;
+        hist_add(lds_bins, gbins, h, (uint64_t)cc, 1);
+    }
+    hist_flush(lds_bins, gbins, (uint64_t)h.max_freq - h.min_freq + 1);
+}
+
+template <int NW>
+__global__ __launch_bounds__(kBlock) void k_hist_low_list(TableView tv, HistSpec h, uint64_t n,
+                                                          unsigned long long *gbins)
+{
+    __shared__ int lds_bins[kLdsBins];
+    for (uint32_t b = threadIdx.x; b < kLdsBins; b += kBlock) lds_bins[b] = 0;
+    __syncthreads();
+    for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) {
+        uint64_t key[NW];
+#pragma unroll
+        for (int w = 0; w < NW; ++w) key[w] = tv.low[i * NW + w];
+        const uint64_t c = table_find<NW>(tv.slots, tv.mask, key);
+        const int64_t cc = high_view<NW>(tv, c, false);
+        if (cc >= 0 && (uint64_t)cc + 1 >= h.min_freq) {
+            hist_add(lds_bins, gbins, h, (uint64_t)cc, -1);
+            uint64_t merged = (uint64_t)cc + 1;
+            if (merged > tv.count_max) merged = tv.count_max;
+            hist_add(lds_bins, gbins, h, merged, 1);
+        } else {
+            hist_add(lds_bins, gbins, h, 1, 1);
+        }
+    }
+    hist_flush(lds_bins, gbins, (uint64_t)h.max_freq - h.min_freq + 1);
+}
+
+// ---- hash KAT kernels ----------------------------------------------------
+template <int NW>
+__global__ void k_hash(int kind, const uint64_t *keys, uint64_t n, uint64_t *out)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t w[NW];
+#pragma unroll
+        for (int j = 0; j < NW; ++j) w[j] = keys[i * NW + j];
+        uint64_t r = 0, lo, hi;
+        switch (kind) {
+        case KC_HASH_XXH3_64: r = xxh3_64<NW>(w); break;
+        case KC_HASH_XXH3_128_LO: xxh3_128<NW>(w, lo, hi); r = lo; break;
+        case KC_HASH_XXH3_128_HI: xxh3_128<NW>(w, lo, hi); r = hi; break;
+        case KC_HASH_RAPIDHASH: r = rapidhash<NW>(w); break;
+        case KC_HASH_KOMIHASH: r = komihash<NW>(w); break;
+        case KC_HASH_CONCURRENT_MAP: r = mix_hash(komihash<NW>(w), rapidhash<NW>(w)); break;
+        case KC_HASH_FUNC: r = hash_func<NW>(w); break;
+        }
+        out[i] = r;
+    }
+}
+
+template <int NW>
+__global__ void k_bloom_probe(const uint64_t *keys, uint64_t n, uint64_t cap_mask, uint64_t *blk, uint64_t *msk)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t w[NW];
+#pragma unroll
+        for (int j = 0; j < NW; ++j) w[j] = keys[i * NW + j];
+        bloom_probe<NW>(w, cap_mask, blk[i], msk[i]);
+    }
+}
+
+}  // namespace kc
+
+using namespace kc;
+
+// ======================================================================
+// host side
+// ======================================================================
+namespace {
+
+uint64_t next_pow2(uint64_t v)
+{
+    uint64_t p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+bool is_pow2(uint64_t v) { return v && !(v & (v - 1)); }
+
+uint32_t grid_for(uint64_t n, uint32_t cu)
+{
+    const uint64_t want = (n + kBlock - 1) / kBlock;
+    const uint64_t cap = (uint64_t)cu * 16;
+    return (uint32_t)std::max<uint64_t>(1, std::min(want, cap));
+}
+
+TableView view_of(kc_table *t)
+{
+    TableView v;
+    v.slots = t->slots;
+    v.mask = t->cap - 1;
+    v.bloom = t->bloom;
+    v.bloom_mask = t->bloom_blocks ? t->bloom_blocks - 1 : 0;
+    v.low = t->low;
+    v.low_cap = t->low_cap;
+    v.counters = reinterpret_cast<unsigned long long *>(t->counters);
+    v.k = t->cfg.k;
+    v.count_max = t->cfg.count_max;
+    v.filter_min = t->cfg.filter_min;
+    v.filter_max = t->cfg.filter_max;
+    return v;
+}
+
+PackSpec pack_spec(const kc_table *t)
+{
+    PackSpec p;
+    const uint32_t k = t->cfg.k;
+    p.full_words = k / 32;
+    const uint32_t tail_bits = 2 * (k % 32);
+    p.tail_bytes = (tail_bits + 7) / 8;
+    p.kmer_bytes = p.full_words * 8 + p.tail_bytes;
+    const uint32_t cm = t->cfg.count_max;
+    p.count_bytes = cm <= 0xFF ? 1 : cm <= 0xFFFF ? 2 : cm <= 0xFFFFFF ? 3 : 4;
+    p.tail_mask = tail_bits ? (~0ULL << (64 - tail_bits)) : 0;
+    return p;
+}
+
+bool low_list_active(const kc_table *t) { return t->bloom && !t->sealed; }
+
+int read_counters(kc_table *t, uint64_t *h)
+{
+    KC_HIP(hipMemcpyAsync(h, t->counters, KC_CNT_N * 8, hipMemcpyDeviceToHost, t->ctx->stream));
+    KC_HIP(hipStreamSynchronize(t->ctx->stream));
+    return KC_OK;
+}
+
+}  // namespace
+
+int kc_launch_fill(kc_ctx *ctx, uint64_t *p, uint64_t n, uint64_t v)
+{
+    if (!n) return KC_OK;
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(n, ctx->cu_count)), dim3(kBlock), 0, ctx->stream, p, n, v);
+    KC_HIP(hipGetLastError());
+    return KC_OK;
+}
+
+extern "C" int kc_ctx_create(int device, kc_ctx **out)
+{
+    if (!out) return KC_EINVAL;
+    *out = nullptr;
+    int ndev = 0;
+    KC_HIP(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return KC_EINVAL;
+    KC_HIP(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    KC_HIP(hipGetDeviceProperties(&prop, device));
+    kc_ctx *c = new kc_ctx;
+    c->device = device;
+    c->cu_count = (uint32_t)prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return KC_EDEVICE;
+    }
+    *out = c;
+    return KC_OK;
+}
+
+extern "C" void kc_ctx_destroy(kc_ctx *ctx)
+{
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+extern "C" int kc_ctx_sync(kc_ctx *ctx)
+{
+    if (!ctx) return KC_EINVAL;
+    KC_HIP(hipStreamSynchronize(ctx->stream));
+    return KC_OK;
+}
+
+extern "C" void *kc_ctx_stream(kc_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+static int table_init_device(kc_table *t)
+{
+    kc_ctx *c = t->ctx;
+    if (t->nw == 1)
+        hipLaunchKernelGGL(k_init_slots<1>, dim3(grid_for(t->cap, c->cu_count)), dim3(kBlock), 0, c->stream, t->slots, t->cap);
+    else
+        hipLaunchKernelGGL(k_init_slots<2>, dim3(grid_for(t->cap, c->cu_count)), dim3(kBlock), 0, c->stream, t->slots, t->cap);
+    KC_HIP(hipGetLastError());
+    int rc;
+    if (t->bloom && (rc = kc_launch_fill(c, t->bloom, 2 * t->bloom_blocks, 0)) != KC_OK) return rc;
+    if ((rc = kc_launch_fill(c, t->counters, KC_CNT_N, 0)) != KC_OK) return rc;
+    t->sealed = 0;
+    return KC_OK;
+}
+
+extern "C" int kc_table_create(kc_ctx *ctx, const kc_table_config *cfg, kc_table **out)
+{
+    if (!ctx || !cfg || !out) return KC_EINVAL;
+    *out = nullptr;
+    if (cfg->k < 1 || cfg->k > 63 || cfg->count_max == 0 || cfg->filter_max < cfg->filter_min)
+        return cfg->k > 63 && cfg->k <= 128 ? KC_ENOTSUP : KC_EINVAL;
+    if (cfg->capacity && !is_pow2(cfg->capacity)) return KC_EINVAL;
+    if (cfg->bloom_blocks && !is_pow2(cfg->bloom_blocks)) return KC_EINVAL;
+    if (!cfg->capacity && !cfg->max_distinct) return KC_EINVAL;
+    KC_HIP(hipSetDevice(ctx->device));
+
+    kc_table *t = new kc_table();
+    t->ctx = ctx;
+    t->cfg = *cfg;
+    t->nw = cfg->k <= 32 ? 1 : 2;
+    t->cap = cfg->capacity ? cfg->capacity : std::max<uint64_t>(1024, next_pow2(cfg->max_distinct + cfg->max_distinct / 2));
+    const uint64_t hint = cfg->max_distinct ? cfg->max_distinct : t->cap / 2;
+    if (cfg->bloom) {
+        t->bloom_blocks = cfg->bloom_blocks ? cfg->bloom_blocks : std::max<uint64_t>(4096, next_pow2(hint / 4));
+        t->low_cap = cfg->low_capacity ? cfg->low_capacity : hint;
+    }
+    const uint64_t slot_bytes = t->nw == 1 ? 16 : 32;
+    const uint64_t need = t->cap * slot_bytes + 16 * t->bloom_blocks + 8ull * t->nw * t->low_cap + 8 * KC_CNT_N;
+    size_t free_b = 0, total_b = 0;
+    KC_HIP(hipMemGetInfo(&free_b, &total_b));
+    if (need > free_b) {
+        delete t;
+        return KC_ENOMEM;
+    }
+    bool ok = hipMalloc((void **)&t->slots, t->cap * slot_bytes) == hipSuccess &&
+              hipMalloc((void **)&t->counters, 8 * KC_CNT_N) == hipSuccess;
+    if (ok && t->bloom_blocks) ok = hipMalloc((void **)&t->bloom, 16 * t->bloom_blocks) == hipSuccess;
+    if (ok && t->low_cap) ok = hipMalloc((void **)&t->low, 8ull * t->nw * t->low_cap) == hipSuccess;
+    if (!ok) {
+        kc_table_destroy(t);
+        return KC_ENOMEM;
+    }
+    int rc = table_init_device(t);
+    if (rc != KC_OK) {
+        kc_table_destroy(t);
+        return rc;
+    }
+    *out = t;
+    return KC_OK;
+}
+
+extern "C" void kc_table_destroy(kc_table *t)
+{
+    if (!t) return;
+    (void)hipSetDevice(t->ctx->device);
+    (void)hipStreamSynchronize(t->ctx->stream);
+    if (t->slots) (void)hipFree(t->slots);
+    if (t->bloom) (void)hipFree(t->bloom);
+    if (t->low) (void)hipFree(t->low);
+    if (t->counters) (void)hipFree(t->counters);
+    for (int i = 0; i < 2; ++i) {
+        if (t->pinned[i]) (void)hipHostFree(t->pinned[i]);
+        if (t->dstage[i]) (void)hipFree(t->dstage[i]);
+        if (t->stage_done[i]) (void)hipEventDestroy(t->stage_done[i]);
+    }
+    delete t;
+}
+
+extern "C" int kc_table_clear(kc_table *t)
+{
+    if (!t) return KC_EINVAL;
+    KC_HIP(hipSetDevice(t->ctx->device));
+    return table_init_device(t);
+}
+
+static int launch_count(kc_table *t, const uint8_t *d_seq, uint64_t nbytes, uint64_t emit_from)
+{
+    if (!nbytes) return KC_OK;
+    const uint64_t blocks = (nbytes + kTile - 1) / kTile;
+    const TableView v = view_of(t);
+    hipStream_t s = t->ctx->stream;
+    const bool bloom = low_list_active(t);
+    if (t->nw == 1) {
+        if (bloom) hipLaunchKernelGGL((k_count_seq<1, true>), dim3(blocks), dim3(kBlock), 0, s, d_seq, nbytes, emit_from, v);
+        else hipLaunchKernelGGL((k_count_seq<1, false>), dim3(blocks), dim3(kBlock), 0, s, d_seq, nbytes, emit_from, v);
+    } else {
+        if (bloom) hipLaunchKernelGGL((k_count_seq<2, true>), dim3(blocks), dim3(kBlock), 0, s, d_seq, nbytes, emit_from, v);
+        else hipLaunchKernelGGL((k_count_seq<2, false>), dim3(blocks), dim3(kBlock), 0, s, d_seq, nbytes, emit_from, v);
+    }
+    KC_HIP(hipGetLastError());
+    return KC_OK;
+}
+
+extern "C" int kc_table_count_seq(kc_table *t, const uint8_t *d_seq, uint64_t nbytes)
+{
+    if (!t || (!d_seq && nbytes)) return KC_EINVAL;
+    if (reinterpret_cast<uintptr_t>(d_seq) % 16) return KC_EINVAL;
+    if (t->bloom && t->sealed) return KC_EINVAL;  // FIRST list already folded
+    return launch_count(t, d_seq, nbytes, 0);
+}
+
+extern "C" int kc_table_count_seq_host(kc_table *t, const uint8_t *h_seq, uint64_t nbytes)
+{
+    if (!t || (!h_seq && nbytes)) return KC_EINVAL;
+    if (t->bloom && t->sealed) return KC_EINVAL;
+    if (!nbytes) return KC_OK;
+    constexpr uint64_t kStage = 256ull << 20;
+    constexpr uint64_t kHalo = 64;  // >= k-1, multiple of 16
+    hipStream_t s = t->ctx->stream;
+    if (!t->pinned[0]) {
+        t->stage_bytes = kStage + kHalo;
+        for (int i = 0; i < 2; ++i) {
+            KC_HIP(hipHostMalloc((void **)&t->pinned[i], t->stage_bytes, hipHostMallocDefault));
+            KC_HIP(hipMalloc((void **)&t->dstage[i], t->stage_bytes));
+            KC_HIP(hipEventCreateWithFlags(&t->stage_done[i], hipEventDisableTiming));
+        }
+    }
+    uint64_t pos = 0;
+    int buf = 0;
+    while (pos < nbytes) {
+        const uint64_t chunk = std::min(kStage, nbytes - pos);
+        const uint64_t halo = std::min(kHalo, pos);
+        // the pinned buffer is reused two chunks later: wait for its copy
+        KC_HIP(hipEventSynchronize(t->stage_done[buf]));
+        std::memcpy(t->pinned[buf], h_seq + pos - halo, chunk + halo);
+        KC_HIP(hipMemcpyAsync(t->dstage[buf], t->pinned[buf], chunk + halo, hipMemcpyHostToDevice, s));
+        int rc = launch_count(t, t->dstage[buf], chunk + halo, halo);
+        if (rc != KC_OK) return rc;
+        KC_HIP(hipEventRecord(t->stage_done[buf], s));
+        pos += chunk;
+        buf ^= 1;
+    }
+    return KC_OK;
+}
+
+extern "C" int kc_table_increment(kc_table *t, const uint64_t *d_keys, uint64_t n, const uint32_t *d_inc)
+{
+    if (!t || (!d_keys && n)) return KC_EINVAL;
+    if (!n) return KC_OK;
+    const TableView v = view_of(t);
+    const uint32_t g = grid_for(n, t->ctx->cu_count);
+    if (t->nw == 1) hipLaunchKernelGGL(k_increment<1>, dim3(g), dim3(kBlock), 0, t->ctx->stream, d_keys, n, d_inc, v);
+    else hipLaunchKernelGGL(k_increment<2>, dim3(g), dim3(kBlock), 0, t->ctx->stream, d_keys, n, d_inc, v);
+    KC_HIP(hipGetLastError());
+    return KC_OK;
+}
+
+// Folds the FIRST list into the table so that stored counts are the true
+// counts (needed before point queries of a Bloom-split table).
+static int seal_table(kc_table *t)
+{
+    if (!low_list_active(t)) return KC_OK;
+    uint64_t h[KC_CNT_N];
+    int rc = read_counters(t, h);
+    if (rc != KC_OK) return rc;
+    const uint64_t n = std::min<uint64_t>(h[KC_CNT_LOW], t->low_cap);
+    rc = kc_table_increment(t, t->low, n, nullptr);
+    if (rc != KC_OK) return rc;
+    t->sealed = 1;
+    return KC_OK;
+}
+
+extern "C" int kc_table_query(kc_table *t, const uint64_t *d_keys, uint64_t n, uint32_t *d_counts)
+{
+    if (!t || (n && (!d_keys || !d_counts))) return KC_EINVAL;
+    int rc = seal_table(t);
+    if (rc != KC_OK) return rc;
+    if (!n) return KC_OK;
+    const TableView v = view_of(t);
+    const uint32_t g = grid_for(n, t->ctx->cu_count);
+    if (t->nw == 1) hipLaunchKernelGGL(k_query<1>, dim3(g), dim3(kBlock), 0, t->ctx->stream, d_keys, n, d_counts, v);
+    else hipLaunchKernelGGL(k_query<2>, dim3(g), dim3(kBlock), 0, t->ctx->stream, d_keys, n, d_counts, v);
+    KC_HIP(hipGetLastError());
+    return KC_OK;
+}
+
+extern "C" int kc_table_stats(kc_table *t, kc_stats *out)
+{
+    if (!t || !out) return KC_EINVAL;
+    uint64_t h[KC_CNT_N];
+    int rc = read_counters(t, h);
+    if (rc != KC_OK) return rc;
+    out->kmers = h[KC_CNT_KMERS];
+    out->distinct = h[KC_CNT_DISTINCT];
+    out->low = t->sealed ? 0 : h[KC_CNT_LOW];
+    out->capacity = t->cap;
+    out->errors = h[KC_CNT_ERRORS] + (h[KC_CNT_LOW] > t->low_cap ? 1 : 0);
+    return KC_OK;
+}
+
+extern "C" int kc_table_export_device(kc_table *t, uint8_t *d_high, uint64_t high_cap, uint64_t *high_bytes,
+                                      uint8_t *d_low, uint64_t low_cap, uint64_t *low_bytes)
+{
+    if (!t || !high_bytes || !low_bytes) return KC_EINVAL;
+    kc_ctx *c = t->ctx;
+    uint64_t h[KC_CNT_N];
+    int rc = read_counters(t, h);
+    if (rc != KC_OK) return rc;
+    if (h[KC_CNT_ERRORS] || h[KC_CNT_LOW] > t->low_cap) return KC_EFULL;
+    const PackSpec ps = pack_spec(t);
+    const uint64_t hrec = ps.kmer_bytes + ps.count_bytes;
+    const bool list = low_list_active(t);
+    const uint64_t n_list = list ? h[KC_CNT_LOW] : 0;
+    // reset the output cursors, then a counting (or writing) pass
+    if ((rc = kc_launch_fill(c, t->counters + KC_CNT_OUT_HIGH, 2, 0)) != KC_OK) return rc;
+    const TableView v = view_of(t);
+    const uint32_t g = grid_for(t->cap, c->cu_count);
+    const bool write = d_high && d_low;
+    uint8_t *oh = write ? d_high : nullptr;
+    uint8_t *ol = write ? d_low + n_list * ps.kmer_bytes : nullptr;
+    const uint64_t hcap = write ? high_cap / hrec : 0;
+    const uint64_t lcap = write ? (low_cap / ps.kmer_bytes > n_list ? low_cap / ps.kmer_bytes - n_list : 0) : 0;
+    if (t->nw == 1) hipLaunchKernelGGL(k_export_table<1>, dim3(g), dim3(kBlock), 0, c->stream, v, ps, !list, oh, hcap, ol, lcap);
+    else hipLaunchKernelGGL(k_export_table<2>, dim3(g), dim3(kBlock), 0, c->stream, v, ps, !list, oh, hcap, ol, lcap);
+    KC_HIP(hipGetLastError());
+    if (write && n_list) {
+        if (low_cap < n_list * ps.kmer_bytes) return KC_EINVAL;
+        const uint32_t g2 = grid_for(n_list, c->cu_count);
+        if (t->nw == 1) hipLaunchKernelGGL(k_export_low_list<1>, dim3(g2), dim3(kBlock), 0, c->stream, v, ps, n_list, d_low, 0ull);
+        else hipLaunchKernelGGL(k_export_low_list<2>, dim3(g2), dim3(kBlock), 0, c->stream, v, ps, n_list, d_low, 0ull);
+        KC_HIP(hipGetLastError());
+    }
+    rc = read_counters(t, h);
+    if (rc != KC_OK) return rc;
+    *high_bytes = h[KC_CNT_OUT_HIGH] * hrec;
+    *low_bytes = (h[KC_CNT_OUT_LOW] + n_list) * ps.kmer_bytes;
+    if (write && (*high_bytes > high_cap || *low_bytes > low_cap)) return KC_EINVAL;
+    return KC_OK;
+}
+
+static int write_file(const std::string &path, const void *data, uint64_t n)
+{
+    FILE *f = std::fopen(path.c_str(), "wb");
+    if (!f) return KC_EIO;
+    const bool ok = n == 0 || std::fwrite(data, 1, n, f) == n;
+    return (std::fclose(f) == 0 && ok) ? KC_OK : KC_EIO;
+}
+
+extern "C" int kc_table_export_dir(kc_table *t, const char *dir)
+{
+    if (!t || !dir) return KC_EINVAL;
+    std::string d(dir);
+    if (!d.empty() && d.back() != '/') d += '/';
+    uint64_t hb = 0, lb = 0;
+    int rc = kc_table_export_device(t, nullptr, 0, &hb, nullptr, 0, &lb);
+    if (rc != KC_OK) return rc;
+    uint8_t *dh = nullptr, *dl = nullptr;
+    KC_HIP(hipMalloc((void **)&dh, std::max<uint64_t>(hb, 16)));
+    if (hipMalloc((void **)&dl, std::max<uint64_t>(lb, 16)) != hipSuccess) {
+        (void)hipFree(dh);
+        return KC_ENOMEM;
+    }
+    uint64_t hb2 = 0, lb2 = 0;
+    rc = kc_table_export_device(t, dh, hb, &hb2, dl, lb, &lb2);
+    std::vector<uint8_t> hh, hl;
+    if (rc == KC_OK) {
+        hh.resize(hb2);
+        hl.resize(lb2);
+        if (hipMemcpy(hh.data(), dh, hb2, hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(hl.data(), dl, lb2, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = KC_EDEVICE;
+    }
+    (void)hipFree(dh);
+    (void)hipFree(dl);
+    if (rc != KC_OK) return rc;
+    const uint32_t infos[2] = {t->cfg.k, t->cfg.count_max};
+    if ((rc = write_file(d + "infos.bin", infos, sizeof infos)) != KC_OK) return rc;
+    if ((rc = write_file(d + "high.bin", hh.data(), hh.size())) != KC_OK) return rc;
+    return write_file(d + "low.bin", hl.data(), hl.size());
+}
+
+extern "C" int kc_table_histogram(kc_table *t, uint32_t min_freq, uint32_t max_freq, int64_t *h_bins)
+{
+    if (!t || !h_bins || max_freq < min_freq) return KC_EINVAL;
+    kc_ctx *c = t->ctx;
+    uint64_t h[KC_CNT_N];
+    int rc = read_counters(t, h);
+    if (rc != KC_OK) return rc;
+    if (h[KC_CNT_ERRORS] || h[KC_CNT_LOW] > t->low_cap) return KC_EFULL;
+    const uint64_t nb = (uint64_t)max_freq - min_freq + 1;
+    unsigned long long *gb = nullptr;
+    KC_HIP(hipMalloc((void **)&gb, nb * 8));
+    rc = kc_launch_fill(c, reinterpret_cast<uint64_t *>(gb), nb, 0);
+    const bool list = low_list_active(t);
+    HistSpec hs{min_freq, max_freq, !list};
+    const TableView v = view_of(t);
+    if (rc == KC_OK) {
+        const uint32_t g = grid_for(t->cap, c->cu_count);
+        if (t->nw == 1) hipLaunchKernelGGL(k_hist_table<1>, dim3(g), dim3(kBlock), 0, c->stream, v, hs, gb);
+        else hipLaunchKernelGGL(k_hist_table<2>, dim3(g), dim3(kBlock), 0, c->stream, v, hs, gb);
+        if (list && h[KC_CNT_LOW]) {
+            const uint64_t n = h[KC_CNT_LOW];
+            const uint32_t g2 = grid_for(n, c->cu_count);
+            if (t->nw == 1) hipLaunchKernelGGL(k_hist_low_list<1>, dim3(g2), dim3(kBlock), 0, c->stream, v, hs, n, gb);
+            else hipLaunchKernelGGL(k_hist_low_list<2>, dim3(g2), dim3(kBlock), 0, c->stream, v, hs, n, gb);
+        }
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(h_bins, gb, nb * 8, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess)
+            rc = KC_EDEVICE;
+    }
+    (void)hipFree(gb);
+    return rc;
+}
+
+extern "C" int kc_hash_keys(kc_ctx *ctx, int kind, const uint64_t *d_keys, uint64_t n, uint32_t nw, uint64_t *d_out)
+{
+    if (!ctx || (n && (!d_keys || !d_out)) || kind < 0 || kind > KC_HASH_FUNC) return KC_EINVAL;
+    if (!n) return KC_OK;
+    const uint32_t g = grid_for(n, ctx->cu_count);
+    switch (nw) {
+    case 1: hipLaunchKernelGGL(k_hash<1>, dim3(g), dim3(kBlock), 0, ctx->stream, kind, d_keys, n, d_out); break;
+    case 2: hipLaunchKernelGGL(k_hash<2>, dim3(g), dim3(kBlock), 0, ctx->stream, kind, d_keys, n, d_out); break;
+    case 4: hipLaunchKernelGGL(k_hash<4>, dim3(g), dim3(kBlock), 0, ctx->stream, kind, d_keys, n, d_out); break;
+    default: return KC_EINVAL;
+    }
+    KC_HIP(hipGetLastError());
+    return KC_OK;
+}
+
+extern "C" int kc_bloom_probe_keys(kc_ctx *ctx, const uint64_t *d_keys, uint64_t n, uint32_t nw, uint64_t capacity,
+                                   uint64_t *d_block, uint64_t *d_mask)
+{
+    if (!ctx || !is_pow2(capacity) || (n && (!d_keys || !d_block || !d_mask))) return KC_EINVAL;
+    if (!n) return KC_OK;
+    const uint32_t g = grid_for(n, ctx->cu_count);
+    switch (nw) {
+    case 1: hipLaunchKernelGGL(k_bloom_probe<1>, dim3(g), dim3(kBlock), 0, ctx->stream, d_keys, n, capacity - 1, d_block, d_mask); break;
+    case 2: hipLaunchKernelGGL(k_bloom_probe<2>, dim3(g), dim3(kBlock), 0, ctx->stream, d_keys, n, capacity - 1, d_block, d_mask); break;
+    case 4: hipLaunchKernelGGL(k_bloom_probe<4>, dim3(g), dim3(kBlock), 0, ctx->stream, d_keys, n, capacity - 1, d_block, d_mask); break;
+    default: return KC_EINVAL;
+    }
+    KC_HIP(hipGetLastError());
+    return KC_OK;
+}

@@ -1,0 +1,284 @@
+// Host ingest (FASTQ framing, zlib inflate) and the seeded synthetic
+// workload generator.
+#include <zlib.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/kc_api.h"
+#include "../../include/kc_host.h"
+
+namespace {
+
+struct SplitMix {
+    uint64_t s;
+    uint64_t next()
+    {
+        uint64_t z = (s += 0x9e3779b97f4a7c15ULL);
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+        return z ^ (z >> 31);
+    }
+    double u01() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); }
+    uint64_t below(uint64_t m) { return (uint64_t)(((unsigned __int128)next() * m) >> 64); }
+};
+
+const char kBase[4] = {'A', 'C', 'G', 'T'};
+
+uint64_t threshold(double p)
+{
+    if (p <= 0) return 0;
+    if (p >= 1) return ~0ULL;
+    return (uint64_t)std::ldexp(p, 64);
+}
+
+// Framing state machine over one buffer (FastqReader.h:478-487).
+struct Framer {
+    uint32_t state = 0;  // 0 header, 1 sequence, 2 plus, 3 quality
+    uint64_t feed(const uint8_t *p, uint64_t n, uint8_t *out)
+    {
+        uint64_t w = 0, i = 0;
+        while (i < n) {
+            const uint8_t *nl = static_cast<const uint8_t *>(std::memchr(p + i, '\n', n - i));
+            const uint64_t end = nl ? (uint64_t)(nl - p) : n;
+            if (state == 1) {
+                std::memcpy(out + w, p + i, end - i);
+                w += end - i;
+                if (nl) out[w++] = '\n';
+            }
+            if (!nl) break;
+            i = end + 1;
+            state = (state + 1) & 3;
+        }
+        return w;
+    }
+};
+
+}  // namespace
+
+extern "C" uint64_t kc_fastq_frame(const uint8_t *fq, uint64_t len, uint8_t *out)
+{
+    Framer f;
+    return f.feed(fq, len, out);
+}
+
+extern "C" void kc_host_free(void *p) { std::free(p); }
+
+extern "C" int kc_fastq_frame_file(const char *path, uint8_t **out, uint64_t *out_len)
+{
+    if (!path || !out || !out_len) return KC_EINVAL;
+    const std::string f(path);
+    const bool gz = f.size() >= 3 && f.compare(f.size() - 3, 3, ".gz") == 0;
+    gzFile g = gzopen(path, "rb");  // reads plain files transparently too
+    if (!g) return KC_EIO;
+    gzbuffer(g, 1u << 20);
+    (void)gz;
+    uint64_t cap = 1ull << 24, used = 0;
+    uint8_t *buf = static_cast<uint8_t *>(std::malloc(cap));
+    std::vector<uint8_t> in(1u << 22);
+    Framer fr;
+    for (;;) {
+        const int r = gzread(g, in.data(), (unsigned)in.size());
+        if (r < 0) {
+            gzclose(g);
+            std::free(buf);
+            return KC_EIO;
+        }
+        if (r == 0) break;
+        if (used + (uint64_t)r > cap) {
+            while (used + (uint64_t)r > cap) cap *= 2;
+            uint8_t *nb = static_cast<uint8_t *>(std::realloc(buf, cap));
+            if (!nb) {
+                gzclose(g);
+                std::free(buf);
+                return KC_ENOMEM;
+            }
+            buf = nb;
+        }
+        used += fr.feed(in.data(), (uint64_t)r, buf + used);
+    }
+    gzclose(g);
+    *out = buf;
+    *out_len = used;
+    return KC_OK;
+}
+
+extern "C" int kc_synth_genome(const kc_genome_spec *sp, uint8_t *out)
+{
+    if (!sp || !out || !sp->length) return KC_EINVAL;
+    SplitMix rng{sp->seed};
+    for (uint64_t i = 0; i < sp->length; ++i) out[i] = kBase[rng.next() & 3];
+    if (!sp->n_segments || sp->repeat_fraction <= 0) return KC_OK;
+    if (sp->segment_len == 0 || sp->segment_len > sp->length) return KC_EINVAL;
+    // repeat families, then diverged copies until repeat_fraction is reached
+    std::vector<uint8_t> seg((uint64_t)sp->n_segments * sp->segment_len);
+    for (auto &b : seg) b = (uint8_t)(rng.next() & 3);
+    const uint64_t target = (uint64_t)(sp->repeat_fraction * (double)sp->length);
+    const uint64_t div = threshold(sp->segment_divergence);
+    for (uint64_t copied = 0; copied < target; copied += sp->segment_len) {
+        const uint64_t s = rng.below(sp->n_segments);
+        const uint64_t pos = rng.below(sp->length - sp->segment_len + 1);
+        for (uint32_t j = 0; j < sp->segment_len; ++j) {
+            uint8_t b = seg[s * sp->segment_len + j];
+            const uint64_t r = rng.next();
+            if (r < div) b = (uint8_t)((b + 1 + (r & 0xFFFF) % 3) & 3);
+            out[pos + j] = kBase[b];
+        }
+    }
+    return KC_OK;
+}
+
+namespace {
+
+struct ReadPlan {
+    uint64_t len;
+    uint64_t start;
+    bool rc;
+    bool is_long;
+    SplitMix rng;  // state after the plan draws, continues with bases
+};
+
+ReadPlan plan_read(const kc_read_spec *sp, uint64_t glen, uint64_t i)
+{
+    ReadPlan p;
+    p.rng.s = sp->seed ^ ((i + 1) * 0xD1B54A32D192ED03ULL);
+    p.rng.next();
+    p.is_long = sp->long_fraction > 0 && p.rng.u01() < sp->long_fraction;
+    if (p.is_long) {
+        double u1 = p.rng.u01(), u2 = p.rng.u01();
+        if (u1 < 1e-300) u1 = 1e-300;
+        const double z = std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+        double l = std::exp(sp->long_mu + sp->long_sigma * z);
+        l = std::max<double>(l, (double)std::max<uint32_t>(sp->min_len, 1));
+        p.len = std::min<uint64_t>((uint64_t)std::llround(l), glen);
+    } else {
+        p.len = std::min<uint64_t>(sp->read_len, glen);
+    }
+    p.start = p.rng.below(glen - p.len + 1);
+    p.rc = sp->both_strands ? (p.rng.next() & 1) : false;
+    return p;
+}
+
+uint64_t digits(uint64_t v)
+{
+    uint64_t d = 1;
+    while (v >= 10) {
+        v /= 10;
+        ++d;
+    }
+    return d;
+}
+
+uint64_t record_bytes(const ReadPlan &p, uint64_t i, bool fastq)
+{
+    if (!fastq) return p.len + 1;
+    return 2 + digits(i) + 1 + p.len + 1 + 2 + p.len + 1;  // "@r<i>\n" seq "\n+\n" qual "\n"
+}
+
+void write_read(const uint8_t *genome, const kc_read_spec *sp, ReadPlan p, uint64_t i, bool fastq, uint8_t *o)
+{
+    const uint64_t sub = threshold(p.is_long ? sp->long_sub_rate : sp->sub_rate);
+    const uint64_t nth = sub + threshold(sp->n_rate);
+    if (fastq) {
+        *o++ = '@';
+        *o++ = 'r';
+        char tmp[24];
+        int t = 0;
+        uint64_t v = i;
+        do {
+            tmp[t++] = (char)('0' + v % 10);
+            v /= 10;
+        } while (v);
+        while (t) *o++ = (uint8_t)tmp[--t];
+        *o++ = '\n';
+    }
+    for (uint64_t j = 0; j < p.len; ++j) {
+        uint8_t c = p.rc ? genome[p.start + p.len - 1 - j] : genome[p.start + j];
+        uint32_t b = c == 'A' ? 0 : c == 'C' ? 1 : c == 'G' ? 2 : 3;
+        if (p.rc) b ^= 3;
+        const uint64_t r = p.rng.next();
+        uint8_t out;
+        if (r < sub) out = (uint8_t)kBase[(b + 1 + (r & 0xFFFF) % 3) & 3];
+        else if (r < nth) out = 'N';
+        else out = (uint8_t)kBase[b];
+        o[j] = out;
+    }
+    o += p.len;
+    *o++ = '\n';
+    if (fastq) {
+        *o++ = '+';
+        *o++ = '\n';
+        uint64_t q = 0;
+        for (uint64_t j = 0; j < p.len; ++j) {
+            if ((j & 7) == 0) q = p.rng.next();
+            *o++ = (uint8_t)('5' + ((q >> (8 * (j & 7))) & 0xFF) % 21);  // Phred33 20..40
+        }
+        *o++ = '\n';
+    }
+}
+
+template <typename F>
+void parallel_for(uint64_t n, uint32_t threads, F fn)
+{
+    uint32_t th = threads ? threads : std::max(1u, std::thread::hardware_concurrency());
+    th = (uint32_t)std::min<uint64_t>(th, std::max<uint64_t>(1, n / 4096));
+    if (th <= 1) {
+        fn(0, n);
+        return;
+    }
+    std::vector<std::thread> ws;
+    for (uint32_t t = 0; t < th; ++t)
+        ws.emplace_back([=] { fn(n * t / th, n * (t + 1) / th); });
+    for (auto &w : ws) w.join();
+}
+
+}  // namespace
+
+extern "C" int64_t kc_synth_reads_size(const uint8_t *genome, uint64_t glen, const kc_read_spec *sp, int fastq)
+{
+    if (!genome || !sp || !glen) return KC_EINVAL;
+    uint64_t total = 0;
+    parallel_for(sp->n_reads, sp->threads, [&](uint64_t a, uint64_t b) {
+        uint64_t s = 0;
+        for (uint64_t i = a; i < b; ++i) s += record_bytes(plan_read(sp, glen, i), i, fastq != 0);
+        __atomic_fetch_add(&total, s, __ATOMIC_RELAXED);
+    });
+    return (int64_t)total;
+}
+
+extern "C" int64_t kc_synth_reads(const uint8_t *genome, uint64_t glen, const kc_read_spec *sp, int fastq,
+                                  uint8_t *out, uint64_t cap)
+{
+    if (!genome || !sp || !out || !glen) return KC_EINVAL;
+    const uint64_t n = sp->n_reads;
+    // offsets: per-block sums, then a second pass writes
+    const uint64_t kBlk = 1 << 14;
+    const uint64_t nblk = (n + kBlk - 1) / kBlk;
+    std::vector<uint64_t> boff(nblk + 1, 0);
+    parallel_for(nblk, sp->threads, [&](uint64_t a, uint64_t b) {
+        for (uint64_t bi = a; bi < b; ++bi) {
+            uint64_t s = 0;
+            for (uint64_t i = bi * kBlk; i < std::min(n, (bi + 1) * kBlk); ++i)
+                s += record_bytes(plan_read(sp, glen, i), i, fastq != 0);
+            boff[bi + 1] = s;
+        }
+    });
+    for (uint64_t bi = 0; bi < nblk; ++bi) boff[bi + 1] += boff[bi];
+    if (boff[nblk] > cap) return KC_EINVAL;
+    parallel_for(nblk, sp->threads, [&](uint64_t a, uint64_t b) {
+        for (uint64_t bi = a; bi < b; ++bi) {
+            uint64_t o = boff[bi];
+            for (uint64_t i = bi * kBlk; i < std::min(n, (bi + 1) * kBlk); ++i) {
+                const ReadPlan p = plan_read(sp, glen, i);
+                write_read(genome, sp, p, i, fastq != 0, out + o);
+                o += record_bytes(p, i, fastq != 0);
+            }
+        }
+    });
+    return (int64_t)boff[nblk];
+}

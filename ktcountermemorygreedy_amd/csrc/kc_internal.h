@@ -1,0 +1,64 @@
+// Internal declarations shared by the engine's translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/kc_api.h"
+
+#define KC_HIP(expr)                                                      \
+    do {                                                                  \
+        hipError_t kc_e_ = (expr);                                        \
+        if (kc_e_ != hipSuccess) {                                        \
+            kc_set_error(#expr, kc_e_);                                   \
+            return KC_EDEVICE;                                            \
+        }                                                                 \
+    } while (0)
+
+void kc_set_error(const char *what, hipError_t e);
+
+struct kc_ctx {
+    int device;
+    hipStream_t stream;
+    uint32_t cu_count;
+};
+
+// Slot layout in HBM.
+//  nw = 1: 16 B  {u64 key, u64 count}            key EMPTY = ~0
+//  nw = 2: 32 B  {u64 w0, u64 w1, u64 count, pad} w0/w1 EMPTY = ~0
+// ~0 is never a canonical key: for k <= 31 / 33..63 the padding bits are
+// zero, and for k == 32 all-T's reverse complement (all-A's) is smaller.
+constexpr uint64_t KC_EMPTY = ~0ULL;
+// Bit 63 of a count word marks nothing in the stored data; counts stay
+// below 2^63 by construction (at most 2^64 bytes of input).
+
+enum kc_counter_idx : int {
+    KC_CNT_KMERS = 0,
+    KC_CNT_DISTINCT = 1,
+    KC_CNT_LOW = 2,
+    KC_CNT_ERRORS = 3,
+    KC_CNT_OUT_HIGH = 4,
+    KC_CNT_OUT_LOW = 5,
+    KC_CNT_N = 8,
+};
+
+struct kc_table {
+    kc_ctx *ctx;
+    kc_table_config cfg;
+    uint32_t nw;
+    uint32_t sealed;        // Bloom FIRST list folded into the table
+    uint64_t cap;           // slots
+    uint64_t *slots;        // cap * (nw == 1 ? 2 : 4) words
+    uint64_t bloom_blocks;  // 0 when Bloom is off
+    uint64_t *bloom;        // 2 * bloom_blocks words {BF1, BF2}
+    uint64_t low_cap;
+    uint64_t *low;          // low_cap * nw words
+    uint64_t *counters;     // KC_CNT_N device u64
+    // host staging for kc_table_count_seq_host
+    uint8_t *pinned[2];
+    uint8_t *dstage[2];
+    uint64_t stage_bytes;
+    hipEvent_t stage_done[2];
+};
+
+// Launch helpers implemented in kc_count.hip / kc_sort.hip
+int kc_launch_fill(kc_ctx *ctx, uint64_t *p, uint64_t n_words, uint64_t value);

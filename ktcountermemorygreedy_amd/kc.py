@@ -1,0 +1,296 @@
+"""Python binding of libkc.so (ctypes over the C ABI in include/kc_api.h).
+
+Device buffers are torch CUDA tensors (HBM); the library receives their raw
+pointers.  There is no CPU fallback: importing this module on a machine
+without the built library raises, and every call checks the status code.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libkc.so")
+
+KC_OK, KC_EINVAL, KC_ENOMEM, KC_EFULL, KC_EDEVICE, KC_EIO, KC_ENOTSUP = 0, -1, -2, -3, -4, -5, -6
+HASH_XXH3_64, HASH_XXH3_128_LO, HASH_XXH3_128_HI, HASH_RAPIDHASH, HASH_KOMIHASH, \
+    HASH_CONCURRENT_MAP, HASH_FUNC = range(7)
+
+
+class KCError(RuntimeError):
+    def __init__(self, status, what):
+        super().__init__("%s failed: %s (%d)" % (what, lib().kc_strerror(status).decode(), status))
+        self.status = status
+
+
+class TableConfig(C.Structure):
+    _fields_ = [
+        ("k", C.c_uint32), ("count_max", C.c_uint32), ("filter_min", C.c_uint32),
+        ("filter_max", C.c_uint32), ("capacity", C.c_uint64), ("max_distinct", C.c_uint64),
+        ("bloom", C.c_uint32), ("reserved", C.c_uint32), ("bloom_blocks", C.c_uint64),
+        ("low_capacity", C.c_uint64),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [("kmers", C.c_uint64), ("distinct", C.c_uint64), ("low", C.c_uint64),
+                ("capacity", C.c_uint64), ("errors", C.c_uint64)]
+
+
+class GenomeSpec(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("length", C.c_uint64), ("n_segments", C.c_uint32),
+                ("segment_len", C.c_uint32), ("repeat_fraction", C.c_double),
+                ("segment_divergence", C.c_double)]
+
+
+class ReadSpec(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("n_reads", C.c_uint64), ("read_len", C.c_uint32),
+                ("both_strands", C.c_uint32), ("sub_rate", C.c_double), ("n_rate", C.c_double),
+                ("long_fraction", C.c_double), ("long_mu", C.c_double), ("long_sigma", C.c_double),
+                ("long_sub_rate", C.c_double), ("min_len", C.c_uint32), ("threads", C.c_uint32)]
+
+
+_lib = None
+
+# every symbol include/kc_api.h and include/kc_host.h declare
+EXPORTS = {
+    "kc_strerror": (C.c_char_p, [C.c_int]),
+    "kc_ctx_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "kc_ctx_destroy": (None, [C.c_void_p]),
+    "kc_ctx_sync": (C.c_int, [C.c_void_p]),
+    "kc_ctx_stream": (C.c_void_p, [C.c_void_p]),
+    "kc_table_create": (C.c_int, [C.c_void_p, C.POINTER(TableConfig), C.POINTER(C.c_void_p)]),
+    "kc_table_destroy": (None, [C.c_void_p]),
+    "kc_table_clear": (C.c_int, [C.c_void_p]),
+    "kc_table_count_seq": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
+    "kc_table_count_seq_host": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
+    "kc_table_increment": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
+    "kc_table_query": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
+    "kc_table_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
+    "kc_table_export_dir": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "kc_table_export_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64),
+                                         C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "kc_table_histogram": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]),
+    "kc_sort_records": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32,
+                                  C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(C.c_int)]),
+    "kc_hash_keys": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_uint64, C.c_uint32, C.c_void_p]),
+    "kc_bloom_probe_keys": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint64,
+                                      C.c_void_p, C.c_void_p]),
+    "kc_fastq_frame": (C.c_uint64, [C.c_void_p, C.c_uint64, C.c_void_p]),
+    "kc_fastq_frame_file": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p), C.POINTER(C.c_uint64)]),
+    "kc_host_free": (None, [C.c_void_p]),
+    "kc_synth_genome": (C.c_int, [C.POINTER(GenomeSpec), C.c_void_p]),
+    "kc_synth_reads_size": (C.c_int64, [C.c_void_p, C.c_uint64, C.POINTER(ReadSpec), C.c_int]),
+    "kc_synth_reads": (C.c_int64, [C.c_void_p, C.c_uint64, C.POINTER(ReadSpec), C.c_int,
+                                   C.c_void_p, C.c_uint64]),
+}
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("libkc.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in EXPORTS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status, what):
+    if status != KC_OK:
+        raise KCError(status, what)
+
+
+def words_for_k(k):
+    return 1 if k <= 32 else 2
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    if isinstance(t, np.ndarray):
+        return t.ctypes.data_as(C.c_void_p)
+    if not t.is_cuda:
+        raise ValueError("expected a device (cuda) tensor")
+    if not t.is_contiguous():
+        raise ValueError("expected a contiguous tensor")
+    return C.c_void_p(t.data_ptr())
+
+
+class Context:
+    """kc_ctx: one device and one HIP stream."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        check(lib().kc_ctx_create(device, C.byref(h)), "kc_ctx_create")
+        self.h = h
+        self.device = device
+
+    def sync(self):
+        check(lib().kc_ctx_sync(self.h), "kc_ctx_sync")
+
+    @property
+    def stream_ptr(self):
+        return lib().kc_ctx_stream(self.h)
+
+    def close(self):
+        if self.h:
+            lib().kc_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- stateless device ops --
+    def hash_keys(self, kind, keys, nw):
+        import torch
+        n = keys.numel() // nw
+        out = torch.empty(n, dtype=torch.int64, device=keys.device)
+        check(lib().kc_hash_keys(self.h, kind, _ptr(keys), n, nw, _ptr(out)), "kc_hash_keys")
+        self.sync()
+        return out
+
+    def bloom_probe(self, keys, nw, capacity):
+        import torch
+        n = keys.numel() // nw
+        blk = torch.empty(n, dtype=torch.int64, device=keys.device)
+        msk = torch.empty(n, dtype=torch.int64, device=keys.device)
+        check(lib().kc_bloom_probe_keys(self.h, _ptr(keys), n, nw, capacity, _ptr(blk), _ptr(msk)),
+              "kc_bloom_probe_keys")
+        self.sync()
+        return blk, msk
+
+    def sort_records(self, recs, scratch, n, nw, k, cpb, digit_bits, record_bytes):
+        flag = C.c_int(0)
+        check(lib().kc_sort_records(self.h, _ptr(recs), _ptr(scratch), n, nw, k, cpb, digit_bits,
+                                    record_bytes, C.byref(flag)), "kc_sort_records")
+        return bool(flag.value)
+
+
+class Table:
+    """kc_table: a device-resident counting table (ConcurrentCountingHashMap<N>)."""
+
+    def __init__(self, ctx: Context, k, count_max=0xFFFFFFFF, capacity=0, max_distinct=0,
+                 bloom=False, bloom_blocks=0, low_capacity=0, filter_min=0, filter_max=0xFFFFFFFF):
+        cfg = TableConfig(k=k, count_max=count_max, filter_min=filter_min, filter_max=filter_max,
+                          capacity=capacity, max_distinct=max_distinct, bloom=1 if bloom else 0,
+                          reserved=0, bloom_blocks=bloom_blocks, low_capacity=low_capacity)
+        h = C.c_void_p()
+        check(lib().kc_table_create(ctx.h, C.byref(cfg), C.byref(h)), "kc_table_create")
+        self.h, self.ctx, self.k, self.nw, self.count_max = h, ctx, k, words_for_k(k), count_max
+
+    def close(self):
+        if self.h:
+            lib().kc_table_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def clear(self):
+        check(lib().kc_table_clear(self.h), "kc_table_clear")
+
+    def count_seq(self, d_seq, nbytes=None):
+        n = d_seq.numel() if nbytes is None else nbytes
+        check(lib().kc_table_count_seq(self.h, _ptr(d_seq), n), "kc_table_count_seq")
+
+    def count_seq_host(self, seq: bytes):
+        buf = np.frombuffer(seq, dtype=np.uint8)
+        check(lib().kc_table_count_seq_host(self.h, _ptr(buf), len(seq)), "kc_table_count_seq_host")
+        self.ctx.sync()
+
+    def increment(self, d_keys, d_inc=None):
+        n = d_keys.numel() // self.nw
+        check(lib().kc_table_increment(self.h, _ptr(d_keys), n, _ptr(d_inc)), "kc_table_increment")
+
+    def query(self, d_keys, d_out=None):
+        import torch
+        n = d_keys.numel() // self.nw
+        if d_out is None:
+            d_out = torch.empty(n, dtype=torch.int32, device=d_keys.device)
+        check(lib().kc_table_query(self.h, _ptr(d_keys), n, _ptr(d_out)), "kc_table_query")
+        return d_out
+
+    def stats(self):
+        s = Stats()
+        check(lib().kc_table_stats(self.h, C.byref(s)), "kc_table_stats")
+        return {f: getattr(s, f) for f, _ in Stats._fields_}
+
+    def export_dir(self, path):
+        check(lib().kc_table_export_dir(self.h, path.encode()), "kc_table_export_dir")
+
+    def export_device(self):
+        import torch
+        hb, lb = C.c_uint64(), C.c_uint64()
+        check(lib().kc_table_export_device(self.h, None, 0, C.byref(hb), None, 0, C.byref(lb)),
+              "kc_table_export_device(size)")
+        dev = torch.device("cuda", self.ctx.device)
+        dh = torch.empty(max(hb.value, 16), dtype=torch.uint8, device=dev)
+        dl = torch.empty(max(lb.value, 16), dtype=torch.uint8, device=dev)
+        check(lib().kc_table_export_device(self.h, _ptr(dh), dh.numel(), C.byref(hb), _ptr(dl),
+                                           dl.numel(), C.byref(lb)), "kc_table_export_device")
+        self.ctx.sync()
+        return dh[:hb.value], dl[:lb.value]
+
+    def histogram(self, min_freq=1, max_freq=10000):
+        bins = np.zeros(max_freq - min_freq + 1, dtype=np.int64)
+        check(lib().kc_table_histogram(self.h, min_freq, max_freq, _ptr(bins)), "kc_table_histogram")
+        return bins
+
+
+# ---- host helpers ----------------------------------------------------------
+
+def fastq_frame(fq: bytes) -> bytes:
+    src = np.frombuffer(fq, dtype=np.uint8)
+    out = np.empty(max(1, len(fq)), dtype=np.uint8)
+    n = lib().kc_fastq_frame(_ptr(src), len(fq), _ptr(out))
+    return out[:n].tobytes()
+
+
+def fastq_frame_file(path) -> np.ndarray:
+    p, n = C.c_void_p(), C.c_uint64()
+    check(lib().kc_fastq_frame_file(path.encode(), C.byref(p), C.byref(n)), "kc_fastq_frame_file")
+    try:
+        return np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_uint8)), shape=(n.value,)).copy() \
+            if n.value else np.zeros(0, np.uint8)
+    finally:
+        lib().kc_host_free(p)
+
+
+def synth_genome(seed, length, n_segments=0, segment_len=0, repeat_fraction=0.0,
+                 segment_divergence=0.0) -> np.ndarray:
+    spec = GenomeSpec(seed=seed, length=length, n_segments=n_segments, segment_len=segment_len,
+                      repeat_fraction=repeat_fraction, segment_divergence=segment_divergence)
+    out = np.empty(length, dtype=np.uint8)
+    check(lib().kc_synth_genome(C.byref(spec), _ptr(out)), "kc_synth_genome")
+    return out
+
+
+def synth_reads(genome: np.ndarray, seed, n_reads, read_len=150, both_strands=True, sub_rate=0.005,
+                n_rate=0.0, long_fraction=0.0, long_mu=0.0, long_sigma=0.0, long_sub_rate=0.0,
+                min_len=0, fastq=False, threads=0, out=None) -> np.ndarray:
+    spec = ReadSpec(seed=seed, n_reads=n_reads, read_len=read_len, both_strands=1 if both_strands else 0,
+                    sub_rate=sub_rate, n_rate=n_rate, long_fraction=long_fraction, long_mu=long_mu,
+                    long_sigma=long_sigma, long_sub_rate=long_sub_rate, min_len=min_len, threads=threads)
+    size = lib().kc_synth_reads_size(_ptr(genome), genome.size, C.byref(spec), 1 if fastq else 0)
+    if size < 0:
+        raise KCError(int(size), "kc_synth_reads_size")
+    if out is None:
+        out = np.empty(size, dtype=np.uint8)
+    wrote = lib().kc_synth_reads(_ptr(genome), genome.size, C.byref(spec), 1 if fastq else 0,
+                                 _ptr(out), out.size)
+    if wrote < 0:
+        raise KCError(int(wrote), "kc_synth_reads")
+    return out[:wrote]

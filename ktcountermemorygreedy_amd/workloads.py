@@ -1,0 +1,70 @@
+"""Seeded synthetic workloads the engine is measured and checked on.
+
+Each workload is fully determined by its seeds; the generator is the
+C++ routine in csrc/kc_host.cpp (SplitMix64 streams, one per read).
+
+  count_5mbp      5 Mbp uniform genome (seed 1); 200k x 150 bp reads from both
+                  strands, 0.5% substitutions, 0.01% N, Phred33 20..40; k=31
+  repeat_100mbp   100 Mbp genome (seed 2) with 30% of its bases written as
+                  diverged (1%) copies of 500 random 1 kb segments; 20M x
+                  150 bp reads (30x), 0.5% substitutions; k=31, Bloom split on
+  uneven_50mbp    50 Mbp genome (seed 3); 80% 150 bp reads (0.5% error) and
+                  20% long reads (lognormal mean 8 kb, sd 3 kb, 5% error) by
+                  read count, 40x coverage; k = 21, 31, 63
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+from . import kc
+
+
+@dataclass
+class ReadSet:
+    name: str
+    genome_seed: int
+    genome_len: int
+    read_seed: int
+    n_reads: int
+    read_len: int = 150
+    sub_rate: float = 0.005
+    n_rate: float = 0.0
+    n_segments: int = 0
+    segment_len: int = 0
+    repeat_fraction: float = 0.0
+    segment_divergence: float = 0.0
+    long_fraction: float = 0.0
+    long_mean: float = 0.0
+    long_sd: float = 0.0
+    long_sub_rate: float = 0.0
+
+    def lognormal(self):
+        if not self.long_fraction:
+            return 0.0, 0.0
+        s2 = math.log(1.0 + (self.long_sd / self.long_mean) ** 2)
+        return math.log(self.long_mean) - s2 / 2.0, math.sqrt(s2)
+
+    def scaled(self, n_reads, name=None):
+        d = dict(self.__dict__)
+        d["n_reads"] = n_reads
+        d["name"] = name or "%s_%d" % (self.name, n_reads)
+        return ReadSet(**d)
+
+    def genome(self):
+        return kc.synth_genome(self.genome_seed, self.genome_len, self.n_segments, self.segment_len,
+                               self.repeat_fraction, self.segment_divergence)
+
+    def reads(self, fastq=False, genome=None, threads=0, out=None):
+        g = self.genome() if genome is None else genome
+        mu, sigma = self.lognormal()
+        return kc.synth_reads(g, self.read_seed, self.n_reads, self.read_len, True, self.sub_rate,
+                              self.n_rate, self.long_fraction, mu, sigma, self.long_sub_rate,
+                              min_len=100, fastq=fastq, threads=threads, out=out)
+
+
+COUNT_5MBP = ReadSet("count_5mbp", 1, 5_000_000, 11, 200_000, n_rate=1e-4)
+REPEAT_100MBP = ReadSet("repeat_100mbp", 2, 100_000_000, 12, 20_000_000, n_segments=500,
+                        segment_len=1000, repeat_fraction=0.30, segment_divergence=0.01)
+UNEVEN_50MBP = ReadSet("uneven_50mbp", 3, 50_000_000, 13, 1_163_000, long_fraction=0.2,
+                       long_mean=8000.0, long_sd=3000.0, long_sub_rate=0.05)

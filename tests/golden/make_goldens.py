@@ -1,0 +1,211 @@
+"""Writes the golden fixtures under tests/golden/ by running the reference
+binaries built from source (oracle/_ref, see oracle/Makefile).
+
+Run in the build container (it needs the reference tree to build _ref):
+    make -C oracle ref && python tests/golden/make_goldens.py
+
+Outputs (all data, no reference source):
+  tiny_<name>.fastq / .k<k>.c<count_max>.dump.txt / .hist.txt
+  count_5mbp.json          sha256 of the reference dump/histogram of the
+                           seeded 5 Mbp workload plus its k-mer counts
+  hash_kat.txt             hashes and Bloom probes of fixed keys
+  sort_kat_<nw>_<bits>.npz records before/after the reference sorts
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+REF = os.path.join(ROOT, "oracle", "_ref")
+sys.path.insert(0, ROOT)
+
+
+def ref_views(fastq_path, k, count_max, hist_max=10000, threads=6):
+    """Runs Tree + dump_tool + histogram_tool; returns (dump bytes, hist bytes, log).
+    The reference occasionally fails a run (its tools exit(-1)); such a run
+    is repeated, up to three attempts, and the error is shown."""
+    for attempt in range(3):
+        try:
+            return _ref_views_once(fastq_path, k, count_max, hist_max, threads)
+        except subprocess.CalledProcessError as e:
+            sys.stderr.write("reference run failed (attempt %d): %s\n%s\n" % (attempt, e.cmd, e.output))
+    raise RuntimeError("reference run failed three times")
+
+
+def _ref_views_once(fastq_path, k, count_max, hist_max, threads):
+    tmp = tempfile.mkdtemp(prefix="kcgold_")
+    try:
+        log = subprocess.run([os.path.join(REF, "kmc_ref"), fastq_path, str(k), str(threads), "4",
+                              tmp + "/", "1024", "2", "4294967295", str(count_max)],
+                             stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, check=True).stdout
+        subprocess.run([os.path.join(REF, "dump_tool_ref"), "precise", tmp + "/", str(threads), "4",
+                        tmp + "/dump.txt", "1", "4294967295", "--sort"],
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, check=True)
+        subprocess.run([os.path.join(REF, "histogram_tool_ref"), "precise", tmp + "/", str(threads), "4",
+                        tmp + "/hist.txt", "1", str(hist_max)],
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, check=True)
+        with open(tmp + "/dump.txt", "rb") as f:
+            dump = f.read()
+        with open(tmp + "/hist.txt", "rb") as f:
+            hist = f.read()
+        return dump, hist, log
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+def tiny_inputs():
+    """Small FASTQ texts exercising the framing and window-reset rules."""
+    rng = np.random.default_rng(20240601)
+
+    def rand_seq(n, alphabet="ACGT"):
+        return "".join(alphabet[i] for i in rng.integers(0, len(alphabet), n))
+
+    out = {}
+    # mixed case, U, N, other bytes, reads shorter than / equal to k
+    recs = [
+        rand_seq(150), rand_seq(31), rand_seq(30), rand_seq(64),
+        rand_seq(40).lower(), rand_seq(50, "ACGU"), rand_seq(120, "ACGTN"),
+        "ACGT" * 20, "A" * 70, "T" * 70, rand_seq(33) + "N" + rand_seq(33),
+        rand_seq(90, "ACGTacgtRY"), rand_seq(63), rand_seq(100) + "\r",
+    ]
+    out["mixed"] = "".join("@r%d extra text\n%s\n+\n%s\n" % (i, s, "I" * len(s)) for i, s in enumerate(recs))
+    # one read much longer than the reference's 32 KiB reader block
+    long_read = rand_seq(100_000)
+    out["long"] = "@long\n%s\n+\n%s\n@short\n%s\n+\n%s\n" % (long_read, "5" * len(long_read),
+                                                             rand_seq(200), "5" * 200)
+    # highly repetitive reads: few distinct k-mers with large counts
+    motif = rand_seq(45)
+    out["repeats"] = "".join("@m%d\n%s\n+\n%s\n" % (i, motif * 3, "I" * 135) for i in range(400))
+    # palindromic k-mers (equal to their reverse complement) at even k
+    pal = "ACGTACGTAATTACGTACGT"  # revcomp == itself
+    pal_reads = [pal + rand_seq(40) for _ in range(10)] * 2
+    out["palin"] = "".join("@p%d\n%s\n+\n%s\n" % (i, r, "I" * len(r)) for i, r in enumerate(pal_reads))
+    # empty sequence lines and a record missing its final newline; one
+    # record repeats so that high.bin is not empty (dump_tool refuses an
+    # empty high.bin, dump_tool.cpp collect_high_files)
+    dup = rand_seq(50)
+    out["edges"] = "@e0\n\n+\n\n@d0\n%s\n+\n%s\n@d1\n%s\n+\n%s\n@e1\n%s\n+\n%s\n@e2\n%s\n+\n%s" % (
+        dup, "I" * 50, dup, "I" * 50, rand_seq(80), "I" * 80, rand_seq(60), "I" * 60)
+    return out
+
+
+TINY_CASES = [
+    # (input, k, count_max)
+    ("mixed", 31, 0xFFFFFFFF), ("mixed", 21, 0xFFFFFFFF), ("mixed", 63, 0xFFFFFFFF),
+    ("mixed", 32, 0xFFFFFFFF), ("mixed", 33, 0xFFFFFFFF), ("mixed", 15, 255),
+    ("long", 31, 0xFFFFFFFF), ("long", 63, 0xFFFFFFFF),
+    ("repeats", 31, 255), ("repeats", 31, 65535), ("repeats", 31, 0xFFFFFFFF), ("repeats", 41, 300),
+    ("palin", 20, 0xFFFFFFFF), ("palin", 31, 0xFFFFFFFF),
+    ("edges", 31, 0xFFFFFFFF), ("edges", 25, 0xFFFFFF),
+]
+
+
+def write_tiny():
+    cases = []
+    for name, text in tiny_inputs().items():
+        with open(os.path.join(HERE, "tiny_%s.fastq" % name), "w") as f:
+            f.write(text)
+    for name, k, cm in TINY_CASES:
+        path = os.path.join(HERE, "tiny_%s.fastq" % name)
+        dump, hist, _ = ref_views(path, k, cm, hist_max=500)
+        base = "tiny_%s.k%d.c%d" % (name, k, cm)
+        case = {"input": "tiny_%s.fastq" % name, "k": k, "count_max": cm, "hist": base + ".hist.txt",
+                "hist_max": 500, "dump_sha256": hashlib.sha256(dump).hexdigest(), "dump_bytes": len(dump)}
+        if len(dump) <= 200_000:  # larger dumps are kept as digests only
+            with open(os.path.join(HERE, base + ".dump.txt"), "wb") as f:
+                f.write(dump)
+            case["dump"] = base + ".dump.txt"
+        with open(os.path.join(HERE, base + ".hist.txt"), "wb") as f:
+            f.write(hist)
+        cases.append(case)
+    with open(os.path.join(HERE, "tiny_cases.json"), "w") as f:
+        json.dump(cases, f, indent=1)
+
+
+def write_count_5mbp():
+    from ktcountermemorygreedy_amd import workloads as W
+    fq = W.COUNT_5MBP.reads(fastq=True)
+    with tempfile.NamedTemporaryFile(suffix=".fastq", delete=False) as f:
+        f.write(fq.tobytes())
+        path = f.name
+    try:
+        dump, hist, log = ref_views(path, 31, 0xFFFFFFFF)
+    finally:
+        os.unlink(path)
+    kmers = [int(l.split(":")[1]) for l in log.splitlines() if l.startswith("Total read k-mer count")][0]
+    rec = {
+        "workload": "count_5mbp", "k": 31, "count_max": 4294967295,
+        "fastq_sha256": hashlib.sha256(fq.tobytes()).hexdigest(),
+        "kmers": kmers, "distinct": dump.count(b"\n"),
+        "dump_sha256": hashlib.sha256(dump).hexdigest(), "dump_bytes": len(dump),
+        "hist_sha256": hashlib.sha256(hist).hexdigest(),
+        "hist_head": hist.decode().splitlines()[:40],
+    }
+    with open(os.path.join(HERE, "count_5mbp.json"), "w") as f:
+        json.dump(rec, f, indent=1)
+
+
+def write_hash_kat():
+    out = subprocess.run([os.path.join(REF, "hash_kat_ref"), "7", "48"], stdout=subprocess.PIPE,
+                         text=True, check=True).stdout
+    lines = [l for l in out.splitlines() if l[:1].isdigit() and len(l.split()) > 5]
+    with open(os.path.join(HERE, "hash_kat.txt"), "w") as f:
+        f.write("# nw words... xxh3_64 xxh3_128_lo xxh3_128_hi rapidhash komihash "
+                "concurrent_map_hash hash_func bloom_block(cap=2^20) bloom_mask\n")
+        f.write("\n".join(lines) + "\n")
+
+
+def run_sort_ref(keys, counts, nw, k, cpb, bits):
+    n = keys.shape[0]
+    hdr = np.array([nw, k, cpb, bits], dtype=np.uint32).tobytes() + np.array([n], dtype=np.uint64).tobytes()
+    rec = np.zeros((n, nw * 8 + 4), dtype=np.uint8)
+    rec[:, :nw * 8] = keys.view(np.uint8).reshape(n, nw * 8)
+    rec[:, nw * 8:] = counts.view(np.uint8).reshape(n, 4)
+    r = subprocess.run([os.path.join(REF, "sort_kat_ref")], input=hdr + rec.tobytes(),
+                       stdout=subprocess.PIPE, check=True)
+    out = np.frombuffer(r.stdout, dtype=np.uint8).reshape(n, nw * 8 + 4)
+    return out[:, :nw * 8].copy().view(np.uint64).reshape(n, nw), out[:, nw * 8:].copy().view(np.uint32).reshape(n)
+
+
+def write_sort_kat():
+    rng = np.random.default_rng(4)
+    for nw, k, bits in [(1, 31, 8), (1, 31, 12), (1, 28, 8), (1, 21, 12), (2, 63, 8), (2, 63, 12), (2, 45, 8),
+                        (2, 41, 12)]:
+        n = 6000
+        total = 64 * nw
+        keys = rng.integers(0, 2**63, size=(n, nw), dtype=np.uint64) * np.uint64(2) + \
+            rng.integers(0, 2, size=(n, nw), dtype=np.uint64)
+        # clear padding below 2k bits (in the 64*nw-bit number)
+        pad = total - 2 * k
+        for w in range(nw - 1, -1, -1):
+            b = min(pad, 64)
+            if b > 0:
+                keys[:, w] &= np.uint64(~((1 << b) - 1) & 0xFFFFFFFFFFFFFFFF)
+            pad -= b
+            if pad <= 0:
+                break
+        # shared 10-base prefix (top 20 bits of word 0) on half the records,
+        # duplicates of sort keys to exercise stability
+        keys[: n // 2, 0] = (keys[: n // 2, 0] & np.uint64((1 << 44) - 1)) | np.uint64(0xABCDE << 44)
+        keys[n // 2: n // 2 + 500] = keys[:500]
+        counts = np.arange(n, dtype=np.uint32)
+        sk, sc = run_sort_ref(keys, counts, nw, k, 10, bits)
+        np.savez(os.path.join(HERE, "sort_kat_%d_%d_%d.npz" % (nw, k, bits)), keys=keys, counts=counts,
+                 sorted_keys=sk, sorted_counts=sc, nw=nw, k=k, bits=bits, cpb=10)
+
+
+if __name__ == "__main__":
+    write_hash_kat()
+    write_sort_kat()
+    write_tiny()
+    write_count_5mbp()
+    print("goldens written to", HERE)

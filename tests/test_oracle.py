@@ -1,0 +1,172 @@
+"""The oracle (CPU restatement) against the reference's own outputs:
+golden dumps/histograms from the reference binaries, the vendored-header
+hash KATs, the reference sorts, and the reference tests' vectors."""
+import hashlib
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.kc_testlib import GOLDEN, load_json, read_golden
+
+
+def _tiny_cases():
+    return load_json("tiny_cases.json")
+
+
+@pytest.mark.parametrize("case", _tiny_cases(), ids=lambda c: "%s-k%d-c%d" % (c["input"], c["k"], c["count_max"]))
+def test_tiny_goldens(case):
+    fq = read_golden(case["input"])
+    dump, hist, _ = O.count_fastq_views(fq, case["k"], case["count_max"], 1, case["hist_max"])
+    assert hashlib.sha256(dump).hexdigest() == case["dump_sha256"]
+    if "dump" in case:
+        assert dump == read_golden(case["dump"])
+    assert hist == read_golden(case["hist"])
+
+
+def test_count_5mbp_golden():
+    from ktcountermemorygreedy_amd import workloads as W
+    g = load_json("count_5mbp.json")
+    fq = W.COUNT_5MBP.reads(fastq=True).tobytes()
+    assert hashlib.sha256(fq).hexdigest() == g["fastq_sha256"]
+    dump, hist, nk = O.count_fastq_views(fq, 31, g["count_max"])
+    assert nk == g["kmers"]
+    assert hashlib.sha256(dump).hexdigest() == g["dump_sha256"]
+    assert hashlib.sha256(hist).hexdigest() == g["hist_sha256"]
+
+
+def _hash_kat():
+    rows = []
+    for line in read_golden("hash_kat.txt", "r").splitlines():
+        if line.startswith("#") or not line.strip():
+            continue
+        f = line.split()
+        nw = int(f[0])
+        words = np.array([int(x, 16) for x in f[1:1 + nw]], dtype=np.uint64)
+        rows.append((nw, words, [int(x, 16) for x in f[1 + nw:]]))
+    return rows
+
+
+def test_hash_kat():
+    rows = _hash_kat()
+    assert len(rows) >= 100
+    for nw, w, v in rows:
+        got = [O.xxh3_64(w), *O.xxh3_128(w), O.rapidhash(w), O.komihash(w), O.concurrent_map_hash(w),
+               O.hash_func(w), *O.bloom_probe(w, 1 << 20)]
+        assert got == v, (nw, w)
+
+
+def _sort_kats():
+    import glob
+    import os
+    return sorted(glob.glob(os.path.join(GOLDEN, "sort_kat_*.npz")))
+
+
+def masked_key_order(keys, nw, lo, hi):
+    """Stable order of records by bits [lo, hi) (LSB-numbered) of the
+    64*nw-bit number, computed with numpy, independently of the oracle."""
+    cols = []
+    for b in range(lo, hi):
+        w = nw - 1 - b // 64
+        cols.append((keys[:, w] >> np.uint64(b % 64)) & np.uint64(1))
+    # np.lexsort: last key is primary -> most significant bit last
+    return np.lexsort(cols) if cols else np.arange(keys.shape[0])
+
+
+@pytest.mark.parametrize("path", _sort_kats(), ids=lambda p: p.split("/")[-1])
+def test_sort_kat(path):
+    d = np.load(path)
+    nw, k, bits = int(d["nw"]), int(d["k"]), int(d["bits"])
+    if bits == 8:
+        sk, sc = O.sort_records8(d["keys"], d["counts"], k)
+        # ExportRecordRadixSort.h:225-254: bits [2k-20+..] -> the last digit
+        # reads 8 bits at its position, so the range is [64nw-2k, hi)
+        lo = 64 * nw - 2 * k
+        end_pos, diff = 2 * k - 1, 2 * k - 20
+        hi = lo
+        while diff > 0:
+            bits_here = min(end_pos + 1 - (end_pos // 64) * 64, 8, diff)
+            diff -= bits_here
+            word_lsb = (63 - end_pos % 64)  # lsb of the digit in its word
+            w = end_pos // 64
+            top = 64 * (nw - 1 - w) + min(word_lsb + 8, 64)
+            hi = max(hi, top)
+            end_pos -= bits_here
+    else:
+        sk, sc = O.sort_records12(d["keys"], d["counts"], k, 10)
+        lo, hi = 64 * nw - 2 * k, 64 * nw - 20
+    assert np.array_equal(sk, d["sorted_keys"]) and np.array_equal(sc, d["sorted_counts"])
+    order = masked_key_order(d["keys"], nw, lo, hi)
+    assert np.array_equal(d["counts"][order], d["sorted_counts"])
+
+
+def test_sort12_rejects_bad_args():
+    keys = np.zeros((4, 1), np.uint64)
+    with pytest.raises(ValueError):
+        O.sort_records12(keys, np.zeros(4, np.uint32), 9, 10)  # k < common prefix
+
+
+def test_pack_vectors_test_export_reader():
+    # test_export_reader.cpp:175-182 (k=33) and :241-247 (k=64)
+    k33 = np.array([[0x0123456789abcdef, 0xc0000000000000ff], [0xfedcba9876543210, 0x8000000000001234],
+                    [0x1111222233334444, 0x40000000ffffffff]], dtype=np.uint64)
+    packed = O.pack(k33, 33)
+    assert len(packed) == 3 * 9
+    back = O.unpack(packed, 33)
+    expect = k33.copy()
+    expect[:, 1] &= np.uint64(0xC000000000000000)
+    assert np.array_equal(back, expect)
+    k64 = np.array([[0x0123456789abcdef, 0xfedcba9876543210], [0xaaaaaaaa55555555, 0x123456789abcdef0]],
+                   dtype=np.uint64)
+    packed = O.pack(k64, 64)
+    assert len(packed) == 32
+    assert np.array_equal(O.unpack(packed, 64), k64)
+    # LE word then the top tail bytes
+    assert packed[:8] == (0x0123456789abcdef).to_bytes(8, "little")
+
+
+def test_merge_rule_kat_low_frequency_query():
+    # test_low_frequency_query_thread_pool.cpp:94-138: low pass deltas
+    two = np.array([0x0123456789abcdef, 0xc000000000000000], dtype=np.uint64)
+    four = np.array([0x1111222233334444, 0x8000000000000000], dtype=np.uint64)
+    missing = np.array([0x9999aaaabbbbcccc, 0x4000000000000000], dtype=np.uint64)
+    hk = np.stack([two, four])
+    hc = np.array([2, 4], np.uint32)
+    lk = np.stack([two, four, missing])
+    with_low = O.histogram(hk, hc, lk, 2, 0xFFFFFFFF, 1, 4)
+    high_only = O.histogram(hk, hc, lk[:0], 2, 0xFFFFFFFF, 1, 4)
+    assert list(with_low - high_only) == [1, -1, 1, -1]
+
+
+def test_rolling_matches_closed_form():
+    rng = np.random.default_rng(5)
+    alphabet = np.frombuffer(b"ACGTacgtUNN\n", dtype=np.uint8)
+    for k in (1, 5, 15, 16, 21, 31, 32, 33, 47, 63, 64):
+        s = alphabet[rng.integers(0, len(alphabet) - 3, 3000)]
+        s[rng.integers(0, 3000, 20)] = ord("N")
+        s = s.tobytes()
+        got = O.extract(s, k)
+        want = []
+        for e in range(k - 1, len(s)):
+            w = O.encode_window(s[e - k + 1:e + 1], k)
+            if w is not None:
+                want.append(w)
+        want = np.array(want, dtype=np.uint64).reshape(-1, O.words_for_k(k))
+        assert np.array_equal(got, want), k
+
+
+def test_canonical_is_lexicographic_min():
+    comp = {ord("A"): "T", ord("C"): "G", ord("G"): "C", ord("T"): "A"}
+    rng = np.random.default_rng(6)
+    for k in (7, 31, 45, 63):
+        for _ in range(200):
+            s = "".join("ACGT"[i] for i in rng.integers(0, 4, k))
+            rc = "".join(comp[ord(c)] for c in reversed(s))
+            key = O.encode_window(s.encode(), k)
+            text = O.dump_text(key.reshape(1, -1), np.array([1], np.uint32), k).split(b"\t")[0].decode()
+            assert text == min(s, rc)
+
+
+def test_fastq_framing():
+    fq = b"@a\nACGT\n+\nIIII\n@b\nNNAC\n+\nIIII\n@c\nGG"
+    assert O.fastq_frame(fq) == b"ACGT\nNNAC\nGG"
