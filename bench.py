@@ -36,7 +36,10 @@ def parse_args():
     ap.add_argument("--workload", default="repeat_100mbp", choices=["repeat_100mbp", "count_5mbp"])
     ap.add_argument("--reads", type=int, default=0, help="override the read count (smaller runs)")
     ap.add_argument("--no-bloom", action="store_true")
-    ap.add_argument("--cpu-sample-reads", type=int, default=1_000_000)
+    ap.add_argument("--cpu-sample-reads", type=int, default=4_000_000)
+    ap.add_argument("--copy-probe-bytes", type=int, default=1 << 30,
+                    help="one device-to-device copy of this size before the timed region, the "
+                         "reference dispatch for scaling rocprof FETCH_SIZE/WRITE_SIZE counters")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     return ap.parse_args()
 
@@ -84,7 +87,6 @@ def main():
               "supported (the multi-GPU design is described in DESIGN.md)" % args.gpus)
         sys.exit(2)
     import numpy as np
-    import torch
 
     from ktcountermemorygreedy_amd import kc
     from ktcountermemorygreedy_amd import workloads as W
@@ -100,12 +102,9 @@ def main():
     seq_host = ws.reads(genome=genome)
     gen_s = time.time() - t0
     S = seq_host.size
-    dev = torch.device("cuda", 0)
-    d_seq = torch.empty(S + 16, dtype=torch.uint8, device=dev)
-    d_seq[:S].copy_(torch.from_numpy(seq_host))
-    torch.cuda.synchronize()
-
     ctx = kc.Context(0)
+    d_seq = ctx.upload(seq_host)
+
     # table sized for the workload: keys reaching the table (Bloom on: k-mers
     # seen at least twice, plus false positives) and the FIRST list
     est_distinct = int(ws.genome_len * 1.1 + S * ws.sub_rate * k * 1.1)
@@ -113,48 +112,37 @@ def main():
     table = kc.Table(ctx, k, count_max=0xFFFFFFFF, capacity=cap, max_distinct=est_distinct, bloom=bloom,
                      bloom_blocks=(1 << max(12, int(np.ceil(np.log2(est_distinct / 4))))) if bloom else 0,
                      low_capacity=est_distinct if bloom else 0)
-    stream = torch.cuda.ExternalStream(ctx.stream_ptr, device=dev)
 
-    # export buffers sized once (the export pass writes into them each step)
-    def one_step(ev=None):
-        table.clear()
-        if ev:
-            ev[0].record(stream)
-        table.count_seq(d_seq, S)
-        if ev:
-            ev[1].record(stream)
-        hb, lb = table_export_sizes()
-        return hb, lb
-
-    import ctypes as C
     bufs = {}
 
-    def table_export_sizes():
-        hb, lb = C.c_uint64(), C.c_uint64()
+    def one_step(slot=None):
+        table.clear()
+        if slot is not None:
+            ctx.record(slot)
+        table.count_seq(d_seq, S)
+        if slot is not None:
+            ctx.record(slot + 1)
         if not bufs:
-            kc.check(kc.lib().kc_table_export_device(table.h, None, 0, C.byref(hb), None, 0, C.byref(lb)),
-                     "export sizes")
-            bufs["h"] = torch.empty(int(hb.value * 1.05) + 4096, dtype=torch.uint8, device=dev)
-            bufs["l"] = torch.empty(int(lb.value * 1.05) + 4096, dtype=torch.uint8, device=dev)
-        kc.check(kc.lib().kc_table_export_device(table.h, kc._ptr(bufs["h"]), bufs["h"].numel(), C.byref(hb),
-                                                 kc._ptr(bufs["l"]), bufs["l"].numel(), C.byref(lb)), "export")
-        return hb.value, lb.value
+            hs, ls = table.export_sizes()
+            bufs["h"] = ctx.alloc(int(hs * 1.05) + 4096)
+            bufs["l"] = ctx.alloc(int(ls * 1.05) + 4096)
+        return table.export_into(bufs["h"], bufs["l"])
 
+    if args.copy_probe_bytes:
+        a, b = ctx.alloc(args.copy_probe_bytes), ctx.alloc(args.copy_probe_bytes)
+        kc.check(kc.lib().kc_copy_d2d(ctx.h, kc._ptr(b), kc._ptr(a), args.copy_probe_bytes), "copy probe")
+        a.free()
+        b.free()
     for _ in range(args.warmup):
         one_step()
     ctx.sync()
-    torch.cuda.synchronize()
 
-    events = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-              for _ in range(args.steps)]
-    torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(args.steps):
-        hb, lb = one_step(events[i])
+        hb, lb = one_step(2 * (i % 32))
     ctx.sync()
-    torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    count_ms = [a.elapsed_time(b) for a, b in events]
+    count_ms = [ctx.elapsed_ms(2 * (i % 32), 2 * (i % 32) + 1) for i in range(max(0, args.steps - 32), args.steps)]
 
     st = table.stats()
     assert st["errors"] == 0, st

@@ -67,6 +67,21 @@ int kc_ctx_sync(kc_ctx *ctx);
 /* hipStream_t of the context, for callers that record events on it. */
 void *kc_ctx_stream(kc_ctx *ctx);
 
+/* ---- device memory and timing on the context's device (no reference
+ *      counterpart: the reference is host-only).  Copies are synchronous
+ *      with respect to the host.  Events: a pool of KC_EVENT_SLOTS timing
+ *      events recorded on the context stream. ---- */
+#define KC_EVENT_SLOTS 64
+int kc_dev_alloc(kc_ctx *ctx, uint64_t bytes, void **out);
+int kc_dev_free(kc_ctx *ctx, void *d);
+int kc_host_alloc_pinned(kc_ctx *ctx, uint64_t bytes, void **out);
+int kc_host_free_pinned(kc_ctx *ctx, void *h);
+int kc_copy_h2d(kc_ctx *ctx, void *d, const void *h, uint64_t bytes);
+int kc_copy_d2h(kc_ctx *ctx, void *h, const void *d, uint64_t bytes);
+int kc_copy_d2d(kc_ctx *ctx, void *dst, const void *src, uint64_t bytes);
+int kc_event_record(kc_ctx *ctx, int slot);
+int kc_event_elapsed_ms(kc_ctx *ctx, int slot_start, int slot_end, float *ms);
+
 /* ---- counting table: replaces ConcurrentCountingHashMap<N>
  *      (ConcurrentCountingHashMap.h:211-312) and, on the production path,
  *      ConcurrentMap<N> (ConcurrentMap.h:231-300) fed by KmerTree. ---- */

@@ -679,7 +679,84 @@ extern "C" int kc_ctx_create(int device, kc_ctx **out)
         delete c;
         return KC_EDEVICE;
     }
+    for (int i = 0; i < KC_EVENT_SLOTS; ++i) c->events[i] = nullptr;
     *out = c;
+    return KC_OK;
+}
+
+extern "C" int kc_dev_alloc(kc_ctx *ctx, uint64_t bytes, void **out)
+{
+    if (!ctx || !out) return KC_EINVAL;
+    *out = nullptr;
+    KC_HIP(hipSetDevice(ctx->device));
+    if (hipMalloc(out, bytes ? bytes : 1) != hipSuccess) return KC_ENOMEM;
+    return KC_OK;
+}
+
+extern "C" int kc_dev_free(kc_ctx *ctx, void *d)
+{
+    if (!ctx) return KC_EINVAL;
+    if (!d) return KC_OK;
+    KC_HIP(hipStreamSynchronize(ctx->stream));
+    KC_HIP(hipFree(d));
+    return KC_OK;
+}
+
+extern "C" int kc_host_alloc_pinned(kc_ctx *ctx, uint64_t bytes, void **out)
+{
+    if (!ctx || !out) return KC_EINVAL;
+    *out = nullptr;
+    if (hipHostMalloc(out, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) return KC_ENOMEM;
+    return KC_OK;
+}
+
+extern "C" int kc_host_free_pinned(kc_ctx *ctx, void *h)
+{
+    if (!ctx) return KC_EINVAL;
+    if (!h) return KC_OK;
+    KC_HIP(hipStreamSynchronize(ctx->stream));
+    KC_HIP(hipHostFree(h));
+    return KC_OK;
+}
+
+static int copy_sync(kc_ctx *ctx, void *dst, const void *src, uint64_t bytes, hipMemcpyKind kind)
+{
+    if (!ctx || (bytes && (!dst || !src))) return KC_EINVAL;
+    if (!bytes) return KC_OK;
+    KC_HIP(hipMemcpyAsync(dst, src, bytes, kind, ctx->stream));
+    KC_HIP(hipStreamSynchronize(ctx->stream));
+    return KC_OK;
+}
+
+extern "C" int kc_copy_h2d(kc_ctx *ctx, void *d, const void *h, uint64_t bytes)
+{
+    return copy_sync(ctx, d, h, bytes, hipMemcpyHostToDevice);
+}
+
+extern "C" int kc_copy_d2h(kc_ctx *ctx, void *h, const void *d, uint64_t bytes)
+{
+    return copy_sync(ctx, h, d, bytes, hipMemcpyDeviceToHost);
+}
+
+extern "C" int kc_copy_d2d(kc_ctx *ctx, void *dst, const void *src, uint64_t bytes)
+{
+    return copy_sync(ctx, dst, src, bytes, hipMemcpyDeviceToDevice);
+}
+
+extern "C" int kc_event_record(kc_ctx *ctx, int slot)
+{
+    if (!ctx || slot < 0 || slot >= KC_EVENT_SLOTS) return KC_EINVAL;
+    if (!ctx->events[slot]) KC_HIP(hipEventCreate(&ctx->events[slot]));
+    KC_HIP(hipEventRecord(ctx->events[slot], ctx->stream));
+    return KC_OK;
+}
+
+extern "C" int kc_event_elapsed_ms(kc_ctx *ctx, int a, int b, float *ms)
+{
+    if (!ctx || !ms || a < 0 || b < 0 || a >= KC_EVENT_SLOTS || b >= KC_EVENT_SLOTS) return KC_EINVAL;
+    if (!ctx->events[a] || !ctx->events[b]) return KC_EINVAL;
+    KC_HIP(hipEventSynchronize(ctx->events[b]));
+    KC_HIP(hipEventElapsedTime(ms, ctx->events[a], ctx->events[b]));
     return KC_OK;
 }
 
@@ -688,6 +765,8 @@ extern "C" void kc_ctx_destroy(kc_ctx *ctx)
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    for (int i = 0; i < KC_EVENT_SLOTS; ++i)
+        if (ctx->events[i]) (void)hipEventDestroy(ctx->events[i]);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }

@@ -20,6 +20,7 @@ struct kc_ctx {
     int device;
     hipStream_t stream;
     uint32_t cu_count;
+    hipEvent_t events[KC_EVENT_SLOTS];
 };
 
 // Slot layout in HBM.

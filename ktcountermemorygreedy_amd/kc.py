@@ -1,7 +1,8 @@
 """Python binding of libkc.so (ctypes over the C ABI in include/kc_api.h).
 
-Device buffers are torch CUDA tensors (HBM); the library receives their raw
-pointers.  There is no CPU fallback: importing this module on a machine
+Device buffers are DeviceBuffer objects: HBM allocated through the library's
+own HIP runtime (kc_dev_alloc), so that only one HIP runtime is ever loaded
+in the process.  There is no CPU fallback: using this module on a machine
 without the built library raises, and every call checks the status code.
 """
 from __future__ import annotations
@@ -61,6 +62,15 @@ EXPORTS = {
     "kc_ctx_destroy": (None, [C.c_void_p]),
     "kc_ctx_sync": (C.c_int, [C.c_void_p]),
     "kc_ctx_stream": (C.c_void_p, [C.c_void_p]),
+    "kc_dev_alloc": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "kc_dev_free": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "kc_host_alloc_pinned": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "kc_host_free_pinned": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "kc_copy_h2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
+    "kc_copy_d2h": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
+    "kc_copy_d2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
+    "kc_event_record": (C.c_int, [C.c_void_p, C.c_int]),
+    "kc_event_elapsed_ms": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_float)]),
     "kc_table_create": (C.c_int, [C.c_void_p, C.POINTER(TableConfig), C.POINTER(C.c_void_p)]),
     "kc_table_destroy": (None, [C.c_void_p]),
     "kc_table_clear": (C.c_int, [C.c_void_p]),
@@ -115,12 +125,69 @@ def _ptr(t):
     if t is None:
         return None
     if isinstance(t, np.ndarray):
+        if not t.flags["C_CONTIGUOUS"]:
+            raise ValueError("expected a contiguous array")
         return t.ctypes.data_as(C.c_void_p)
-    if not t.is_cuda:
-        raise ValueError("expected a device (cuda) tensor")
-    if not t.is_contiguous():
-        raise ValueError("expected a contiguous tensor")
-    return C.c_void_p(t.data_ptr())
+    if isinstance(t, DeviceBuffer):
+        return C.c_void_p(t.ptr)
+    raise TypeError("expected a numpy array (host) or a DeviceBuffer (device)")
+
+
+class DeviceBuffer:
+    """A block of HBM owned by a Context (kc_dev_alloc / kc_dev_free)."""
+
+    def __init__(self, ctx, nbytes, ptr=None, owner=None):
+        self.ctx, self.nbytes = ctx, int(nbytes)
+        self._owner = owner
+        if ptr is None:
+            p = C.c_void_p()
+            check(lib().kc_dev_alloc(ctx.h, max(1, self.nbytes), C.byref(p)), "kc_dev_alloc")
+            self.ptr = p.value
+            self._own = True
+        else:
+            self.ptr = ptr
+            self._own = False
+
+    @classmethod
+    def from_numpy(cls, ctx, arr, pad=16):
+        arr = np.ascontiguousarray(arr)
+        b = cls(ctx, arr.nbytes + pad)
+        if arr.nbytes:
+            check(lib().kc_copy_h2d(ctx.h, C.c_void_p(b.ptr), arr.ctypes.data_as(C.c_void_p), arr.nbytes),
+                  "kc_copy_h2d")
+        b.valid = arr.nbytes
+        return b
+
+    def upload(self, arr, offset=0):
+        arr = np.ascontiguousarray(arr)
+        if offset + arr.nbytes > self.nbytes:
+            raise ValueError("upload exceeds the buffer")
+        check(lib().kc_copy_h2d(self.ctx.h, C.c_void_p(self.ptr + offset), arr.ctypes.data_as(C.c_void_p),
+                                arr.nbytes), "kc_copy_h2d")
+
+    def numpy(self, dtype=np.uint8, count=None, offset=0):
+        dtype = np.dtype(dtype)
+        if count is None:
+            count = (self.nbytes - offset) // dtype.itemsize
+        out = np.empty(count, dtype=dtype)
+        if out.nbytes:
+            check(lib().kc_copy_d2h(self.ctx.h, out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr + offset),
+                                    out.nbytes), "kc_copy_d2h")
+        return out
+
+    def view(self, offset, nbytes):
+        return DeviceBuffer(self.ctx, nbytes, ptr=self.ptr + offset, owner=self)
+
+    def free(self):
+        if self._own and self.ptr:
+            lib().kc_dev_free(self.ctx.h, C.c_void_p(self.ptr))
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 class Context:
@@ -150,24 +217,37 @@ class Context:
         except Exception:
             pass
 
-    # -- stateless device ops --
-    def hash_keys(self, kind, keys, nw):
-        import torch
-        n = keys.numel() // nw
-        out = torch.empty(n, dtype=torch.int64, device=keys.device)
-        check(lib().kc_hash_keys(self.h, kind, _ptr(keys), n, nw, _ptr(out)), "kc_hash_keys")
-        self.sync()
-        return out
+    def alloc(self, nbytes):
+        return DeviceBuffer(self, nbytes)
 
-    def bloom_probe(self, keys, nw, capacity):
-        import torch
-        n = keys.numel() // nw
-        blk = torch.empty(n, dtype=torch.int64, device=keys.device)
-        msk = torch.empty(n, dtype=torch.int64, device=keys.device)
-        check(lib().kc_bloom_probe_keys(self.h, _ptr(keys), n, nw, capacity, _ptr(blk), _ptr(msk)),
+    def upload(self, arr, pad=16):
+        return DeviceBuffer.from_numpy(self, arr, pad)
+
+    def record(self, slot):
+        check(lib().kc_event_record(self.h, slot), "kc_event_record")
+
+    def elapsed_ms(self, a, b):
+        ms = C.c_float()
+        check(lib().kc_event_elapsed_ms(self.h, a, b, C.byref(ms)), "kc_event_elapsed_ms")
+        return ms.value
+
+    # -- stateless device ops (keys: uint64 numpy array, n x nw) --
+    def hash_keys(self, kind, keys: np.ndarray, nw):
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        n = keys.size // nw
+        dk = self.upload(keys)
+        out = self.alloc(8 * n)
+        check(lib().kc_hash_keys(self.h, kind, _ptr(dk), n, nw, _ptr(out)), "kc_hash_keys")
+        return out.numpy(np.uint64, n)
+
+    def bloom_probe(self, keys: np.ndarray, nw, capacity):
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        n = keys.size // nw
+        dk = self.upload(keys)
+        blk, msk = self.alloc(8 * n), self.alloc(8 * n)
+        check(lib().kc_bloom_probe_keys(self.h, _ptr(dk), n, nw, capacity, _ptr(blk), _ptr(msk)),
               "kc_bloom_probe_keys")
-        self.sync()
-        return blk, msk
+        return blk.numpy(np.uint64, n), msk.numpy(np.uint64, n)
 
     def sort_records(self, recs, scratch, n, nw, k, cpb, digit_bits, record_bytes):
         flag = C.c_int(0)
@@ -203,7 +283,7 @@ class Table:
         check(lib().kc_table_clear(self.h), "kc_table_clear")
 
     def count_seq(self, d_seq, nbytes=None):
-        n = d_seq.numel() if nbytes is None else nbytes
+        n = getattr(d_seq, "valid", d_seq.nbytes) if nbytes is None else nbytes
         check(lib().kc_table_count_seq(self.h, _ptr(d_seq), n), "kc_table_count_seq")
 
     def count_seq_host(self, seq: bytes):
@@ -211,17 +291,21 @@ class Table:
         check(lib().kc_table_count_seq_host(self.h, _ptr(buf), len(seq)), "kc_table_count_seq_host")
         self.ctx.sync()
 
-    def increment(self, d_keys, d_inc=None):
-        n = d_keys.numel() // self.nw
-        check(lib().kc_table_increment(self.h, _ptr(d_keys), n, _ptr(d_inc)), "kc_table_increment")
+    def increment(self, keys: np.ndarray, inc: np.ndarray = None):
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        n = keys.size // self.nw
+        dk = self.ctx.upload(keys)
+        di = self.ctx.upload(np.ascontiguousarray(inc, dtype=np.uint32)) if inc is not None else None
+        check(lib().kc_table_increment(self.h, _ptr(dk), n, _ptr(di)), "kc_table_increment")
+        self.ctx.sync()
 
-    def query(self, d_keys, d_out=None):
-        import torch
-        n = d_keys.numel() // self.nw
-        if d_out is None:
-            d_out = torch.empty(n, dtype=torch.int32, device=d_keys.device)
-        check(lib().kc_table_query(self.h, _ptr(d_keys), n, _ptr(d_out)), "kc_table_query")
-        return d_out
+    def query(self, keys: np.ndarray) -> np.ndarray:
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        n = keys.size // self.nw
+        dk = self.ctx.upload(keys)
+        out = self.ctx.alloc(4 * n)
+        check(lib().kc_table_query(self.h, _ptr(dk), n, _ptr(out)), "kc_table_query")
+        return out.numpy(np.uint32, n)
 
     def stats(self):
         s = Stats()
@@ -231,18 +315,24 @@ class Table:
     def export_dir(self, path):
         check(lib().kc_table_export_dir(self.h, path.encode()), "kc_table_export_dir")
 
-    def export_device(self):
-        import torch
+    def export_sizes(self):
         hb, lb = C.c_uint64(), C.c_uint64()
         check(lib().kc_table_export_device(self.h, None, 0, C.byref(hb), None, 0, C.byref(lb)),
               "kc_table_export_device(size)")
-        dev = torch.device("cuda", self.ctx.device)
-        dh = torch.empty(max(hb.value, 16), dtype=torch.uint8, device=dev)
-        dl = torch.empty(max(lb.value, 16), dtype=torch.uint8, device=dev)
-        check(lib().kc_table_export_device(self.h, _ptr(dh), dh.numel(), C.byref(hb), _ptr(dl),
-                                           dl.numel(), C.byref(lb)), "kc_table_export_device")
-        self.ctx.sync()
-        return dh[:hb.value], dl[:lb.value]
+        return hb.value, lb.value
+
+    def export_into(self, dh, dl):
+        hb, lb = C.c_uint64(), C.c_uint64()
+        check(lib().kc_table_export_device(self.h, _ptr(dh), dh.nbytes, C.byref(hb), _ptr(dl), dl.nbytes,
+                                           C.byref(lb)), "kc_table_export_device")
+        return hb.value, lb.value
+
+    def export_device(self):
+        """Packed (high, low) export records as host bytes."""
+        hs, ls = self.export_sizes()
+        dh, dl = self.ctx.alloc(max(hs, 16)), self.ctx.alloc(max(ls, 16))
+        hb, lb = self.export_into(dh, dl)
+        return dh.numpy(np.uint8, hb).tobytes(), dl.numpy(np.uint8, lb).tobytes()
 
     def histogram(self, min_freq=1, max_freq=10000):
         bins = np.zeros(max_freq - min_freq + 1, dtype=np.int64)

@@ -36,17 +36,16 @@ def views_from_export(high: bytes, low: bytes, k: int, count_max: int, hist_max=
 
 def gpu_count(ctx, seq: bytes, k: int, count_max: int, bloom=False, host_path=False, max_distinct=None,
               **kw):
-    import torch
     from ktcountermemorygreedy_amd import kc
     md = max_distinct or max(1024, len(seq))
     t = kc.Table(ctx, k, count_max=count_max, max_distinct=md, bloom=bloom, **kw)
     if host_path:
         t.count_seq_host(seq)
     else:
-        d = torch.frombuffer(bytearray(seq), dtype=torch.uint8).to("cuda") if seq else \
-            torch.zeros(16, dtype=torch.uint8, device="cuda")
+        d = ctx.upload(np.frombuffer(seq, dtype=np.uint8))
         t.count_seq(d, len(seq))
         ctx.sync()
+        d.free()
     return t
 
 
@@ -54,9 +53,7 @@ def gpu_views(ctx, seq: bytes, k: int, count_max: int, bloom=False, hist_max=100
     t = gpu_count(ctx, seq, k, count_max, bloom=bloom, host_path=host_path)
     st = t.stats()
     assert st["errors"] == 0, st
-    dh, dl = t.export_device()
-    high = dh.cpu().numpy().tobytes()
-    low = dl.cpu().numpy().tobytes()
+    high, low = t.export_device()
     dump, hist_from_files = views_from_export(high, low, k, count_max, hist_max)
     dev_hist = O.histogram_text(t.histogram(1, hist_max), 1, hist_max)
     t.close()

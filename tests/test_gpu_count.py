@@ -52,7 +52,6 @@ def test_count_5mbp_parity(ctx, mode):
 @pytest.mark.parametrize("k", [21, 31, 63])
 def test_uneven_reads_counts_match_oracle(ctx, k):
     """Long and short reads in one launch: exact multiset vs the oracle."""
-    import torch
     from ktcountermemorygreedy_amd import workloads as W
     g = W.ReadSet("g", 3, 3_000_000, 13, 0).genome()
     seq = W.UNEVEN_50MBP.scaled(1500).reads(genome=g).tobytes()
@@ -62,32 +61,26 @@ def test_uneven_reads_counts_match_oracle(ctx, k):
     t = gpu_count(ctx, seq, k, 0xFFFFFFFF, max_distinct=len(seq))
     st = t.stats()
     assert st["errors"] == 0 and st["kmers"] == keys.shape[0] and st["distinct"] == uniq.shape[0]
-    q = torch.from_numpy(uniq.view(np.int64).reshape(-1)).cuda()
-    got = t.query(q).cpu().numpy().astype(np.uint64)
-    ctx.sync()
+    got = t.query(uniq).astype(np.uint64)
     assert np.array_equal(got, cnt)
     # absent keys read 0
     absent = uniq.copy()
     absent[:, nw - 1] |= np.uint64(1)  # padding bit set: never a stored key
     if 64 * nw - 2 * k > 0:
-        q2 = torch.from_numpy(absent.view(np.int64).reshape(-1)).cuda()
-        assert int(t.query(q2).sum().item()) == 0
+        assert int(t.query(absent).sum()) == 0
     t.close()
 
 
 def test_increment_and_query_saturation(ctx):
-    import torch
     from ktcountermemorygreedy_amd import kc
     rng = np.random.default_rng(9)
     keys = (rng.integers(0, 2**62, 5000, dtype=np.uint64) << np.uint64(2))
     rep = np.repeat(keys, rng.integers(1, 400, keys.size))
     rng.shuffle(rep)
     t = kc.Table(ctx, 31, count_max=255, capacity=1 << 14)
-    d = torch.from_numpy(rep.view(np.int64)).cuda()
-    t.increment(d)
-    inc = torch.full((keys.size,), 7, dtype=torch.int32, device="cuda")
-    t.increment(torch.from_numpy(keys.view(np.int64)).cuda(), inc)
-    got = t.query(torch.from_numpy(keys.view(np.int64)).cuda()).cpu().numpy()
+    t.increment(rep)
+    t.increment(keys, np.full(keys.size, 7, np.uint32))
+    got = t.query(keys)
     uniq, cnt = O.count(rep.reshape(-1, 1), 1)
     order = np.argsort(uniq[:, 0])
     want = dict(zip(uniq[:, 0].tolist(), (cnt + 7).tolist()))
@@ -98,11 +91,10 @@ def test_increment_and_query_saturation(ctx):
 
 
 def test_table_full_reports_error(ctx):
-    import torch
     from ktcountermemorygreedy_amd import kc
     t = kc.Table(ctx, 31, capacity=1024)
     keys = (np.arange(5000, dtype=np.uint64) << np.uint64(2))
-    t.increment(torch.from_numpy(keys.view(np.int64)).cuda())
+    t.increment(keys)
     st = t.stats()
     assert st["errors"] > 0
     with pytest.raises(kc.KCError) as e:
@@ -118,8 +110,8 @@ def test_clear_resets(ctx):
     a = t.stats()
     t.clear()
     assert t.stats()["kmers"] == 0
-    import torch
-    t.count_seq(torch.frombuffer(bytearray(seq), dtype=torch.uint8).cuda(), len(seq))
+    d = ctx.upload(np.frombuffer(seq, dtype=np.uint8))
+    t.count_seq(d, len(seq))
     b = t.stats()
     assert a == b
     t.close()
