@@ -49,7 +49,8 @@ struct TableView {
     uint64_t *bloom;       // nullptr when the Bloom split is off
     uint64_t bloom_mask;
     uint64_t *low;
-    uint64_t low_cap;
+    uint64_t seg_cap;            // keys per FIRST-list segment
+    const uint64_t *seg_off;     // KC_LOW_SEGS + 1 exclusive offsets (after counting)
     unsigned long long *counters;
     uint32_t k;
     uint32_t count_max;
@@ -74,13 +75,27 @@ __device__ __forceinline__ void add_agent(uint64_t *p, uint64_t v)
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Wave sum of a 32-bit value, then one device atomic per wave.
-__device__ __forceinline__ void wave_flush(unsigned long long *ctr, uint32_t v)
+// Wave sum of a 32-bit statistic, then one atomic per wave into this
+// block's lane of the statistic (lanes live on separate 128-byte lines).
+__device__ __forceinline__ void stat_add(unsigned long long *counters, int stat, uint32_t v)
 {
     uint64_t s = v;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-    if (__lane_id() == 0 && s) atomicAdd(ctr, (unsigned long long)s);
+    if (__lane_id() == 0 && s)
+        atomicAdd(counters + ((uint64_t)stat * KC_LANES + (blockIdx.x % KC_LANES)) * KC_LINE_WORDS,
+                  (unsigned long long)s);
+}
+
+// i-th key of the FIRST list seen as the concatenation of its segments.
+__device__ __forceinline__ const uint64_t *low_key(const TableView &tv, uint64_t i, int nw)
+{
+    int lo = 0, hi = KC_LOW_SEGS;  // find the last segment with seg_off <= i
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (tv.seg_off[mid] <= i) lo = mid; else hi = mid;
+    }
+    return tv.low + ((uint64_t)lo * tv.seg_cap + (i - tv.seg_off[lo])) * nw;
 }
 
 // ---- table insert / add ------------------------------------------------
@@ -94,18 +109,23 @@ __device__ __forceinline__ bool table_add(uint64_t *slots, uint64_t mask, const 
     uint64_t idx = xxh3_64<NW>(key) & mask;
     for (uint64_t probe = 0; probe <= mask; ++probe) {
         uint64_t *s = slots + idx * slot_words<NW>();
-        const uint64_t prev = atomicCAS((unsigned long long *)s, (unsigned long long)KC_EMPTY,
-                                        (unsigned long long)key[0]);
+        // A plain read settles the common case (key present: one atomic add).
+        // A stale EMPTY is harmless: the CAS below returns the real word.
+        uint64_t w0 = s[0];
+        bool owner = false;
+        if (w0 == KC_EMPTY) {
+            w0 = atomicCAS((unsigned long long *)s, (unsigned long long)KC_EMPTY, (unsigned long long)key[0]);
+            owner = w0 == KC_EMPTY;
+        }
         if constexpr (NW == 1) {
-            if (prev == KC_EMPTY || prev == key[0]) {
-                n_new += prev == KC_EMPTY;
+            if (owner || w0 == key[0]) {
+                n_new += owner;
                 add_agent(s + 1, inc);
                 return true;
             }
         } else {
             // Two-word key: the CAS winner publishes word 1; a thread that
             // finds word 0 equal waits for word 1 (bounded) and compares.
-            const bool owner = prev == KC_EMPTY;
             if (owner) __hip_atomic_store(s + 1, key[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             __builtin_amdgcn_wave_barrier();
             if (owner) {
@@ -113,7 +133,7 @@ __device__ __forceinline__ bool table_add(uint64_t *slots, uint64_t mask, const 
                 add_agent(s + 2, inc);
                 return true;
             }
-            if (prev == key[0]) {
+            if (w0 == key[0]) {
                 uint64_t w1 = ld_agent(s + 1);
                 for (int spin = 0; w1 == KC_EMPTY && spin < (1 << 16); ++spin) {
                     __builtin_amdgcn_s_sleep(1);
@@ -303,14 +323,17 @@ __global__ __launch_bounds__(kBlock) void k_count_seq(const uint8_t *__restrict_
                         if (firsts) {
                             const int leader = __ffsll((unsigned long long)firsts) - 1;
                             unsigned long long b0 = 0;
+                            const uint32_t seg = blockIdx.x % KC_LOW_SEGS;
                             if (__lane_id() == leader)
-                                b0 = atomicAdd(tv.counters + KC_CNT_LOW, (unsigned long long)__popcll(firsts));
+                                b0 = atomicAdd(tv.counters + KC_CURSOR_BASE + (uint64_t)seg * KC_LINE_WORDS,
+                                               (unsigned long long)__popcll(firsts));
                             b0 = __shfl(b0, leader, 64);
                             if (first) {
                                 const uint64_t at = b0 + __popcll(firsts & lanemask_lt());
-                                if (at < tv.low_cap) {
+                                if (at < tv.seg_cap) {
+                                    uint64_t *dst = tv.low + ((uint64_t)seg * tv.seg_cap + at) * NW;
 #pragma unroll
-                                    for (int w = 0; w < NW; ++w) tv.low[at * NW + w] = key[w];
+                                    for (int w = 0; w < NW; ++w) dst[w] = key[w];
                                 } else {
                                     ++n_err;
                                 }
@@ -323,9 +346,9 @@ __global__ __launch_bounds__(kBlock) void k_count_seq(const uint8_t *__restrict_
             }
         }
     }
-    wave_flush(tv.counters + KC_CNT_KMERS, n_kmers);
-    wave_flush(tv.counters + KC_CNT_DISTINCT, n_new);
-    wave_flush(tv.counters + KC_CNT_ERRORS, n_err);
+    stat_add(tv.counters, KC_STAT_KMERS, n_kmers);
+    stat_add(tv.counters, KC_STAT_DISTINCT, n_new);
+    stat_add(tv.counters, KC_STAT_ERRORS, n_err);
 }
 
 // increment(key, inc) for a batch of keys.
@@ -340,8 +363,24 @@ __global__ __launch_bounds__(kBlock) void k_increment(const uint64_t *__restrict
         for (int w = 0; w < NW; ++w) key[w] = keys[i * NW + w];
         table_add<NW>(tv.slots, tv.mask, key, inc ? inc[i] : 1, n_new, n_err);
     }
-    wave_flush(tv.counters + KC_CNT_DISTINCT, n_new);
-    wave_flush(tv.counters + KC_CNT_ERRORS, n_err);
+    stat_add(tv.counters, KC_STAT_DISTINCT, n_new);
+    stat_add(tv.counters, KC_STAT_ERRORS, n_err);
+}
+
+// increment(key, 1) for every key of the FIRST list (folds it into the table)
+template <int NW>
+__global__ __launch_bounds__(kBlock) void k_increment_low(uint64_t n, TableView tv)
+{
+    uint32_t n_new = 0, n_err = 0;
+    for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) {
+        uint64_t key[NW];
+        const uint64_t *src = low_key(tv, i, NW);
+#pragma unroll
+        for (int w = 0; w < NW; ++w) key[w] = src[w];
+        table_add<NW>(tv.slots, tv.mask, key, 1, n_new, n_err);
+    }
+    stat_add(tv.counters, KC_STAT_DISTINCT, n_new);
+    stat_add(tv.counters, KC_STAT_ERRORS, n_err);
 }
 
 template <int NW>
@@ -401,69 +440,162 @@ __device__ __forceinline__ void pack_key(uint8_t *dst, const uint64_t *key, cons
     }
 }
 
-// Wave-aggregated append: returns this lane's record index.
-__device__ __forceinline__ uint64_t wave_append(unsigned long long *ctr, bool want)
+// Export compaction is deterministic: a counting pass writes per-tile
+// record counts, one block scans them, and the writing pass places each
+// tile's records at its scanned offset in slot order.
+constexpr int kExportPer = 16;                       // slots per thread
+constexpr uint64_t kExportTile = (uint64_t)kBlock * kExportPer;
+
+// Classifies slot i: 1 = high record (count in cc), 2 = low record
+// (singleton without an active FIRST list), 0 = nothing.
+template <int NW>
+__device__ __forceinline__ int classify_slot(const TableView &tv, uint64_t i, bool singles_to_low,
+                                             uint64_t *key, uint32_t &cc)
 {
-    const uint64_t m = __ballot(want);
-    if (!m) return 0;
-    const int leader = __ffsll((unsigned long long)m) - 1;
-    unsigned long long b0 = 0;
-    if (__lane_id() == leader) b0 = atomicAdd(ctr, (unsigned long long)__popcll(m));
-    b0 = __shfl(b0, leader, 64);
-    return b0 + __popcll(m & lanemask_lt());
+    const uint64_t *s = tv.slots + i * slot_words<NW>();
+    if (s[0] == KC_EMPTY) return 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) key[w] = s[w];
+    const uint64_t c = s[NW == 1 ? 1 : 2];
+    if (singles_to_low && c == 1) return 2;
+    if (c == 0) return 0;
+    cc = (uint32_t)(c < tv.count_max ? c : tv.count_max);
+    return ((uint64_t)cc + 1 < tv.filter_min || cc > tv.filter_max) ? 0 : 1;
 }
 
-// Table slots -> high records (and, without an active FIRST list, the
-// singletons -> low records, the simplest valid split: merged views are
-// min(c, count_max) either way).  Records dropped by the high.bin filter
-// follow FinalDrainWriter.h:101-104.  out_* may be null to only count.
+// Block-wide exclusive scan of two per-thread counts (high, low).
+__device__ __forceinline__ void block_scan2(uint32_t &h, uint32_t &l, uint32_t &th, uint32_t &tl)
+{
+    __shared__ uint32_t sh[kBlock], sl[kBlock];
+    sh[threadIdx.x] = h;
+    sl[threadIdx.x] = l;
+    __syncthreads();
+    for (int o = 1; o < kBlock; o <<= 1) {
+        const uint32_t ah = threadIdx.x >= (uint32_t)o ? sh[threadIdx.x - o] : 0;
+        const uint32_t al = threadIdx.x >= (uint32_t)o ? sl[threadIdx.x - o] : 0;
+        __syncthreads();
+        sh[threadIdx.x] += ah;
+        sl[threadIdx.x] += al;
+        __syncthreads();
+    }
+    th = sh[kBlock - 1];
+    tl = sl[kBlock - 1];
+    h = sh[threadIdx.x] - h;
+    l = sl[threadIdx.x] - l;
+    __syncthreads();
+}
+
+// Pass 1: per-tile record counts -> tile_counts[2t], [2t+1].
 template <int NW>
-__global__ __launch_bounds__(kBlock) void k_export_table(TableView tv, PackSpec ps, bool singles_to_low,
-                                                         uint8_t *out_high, uint64_t high_cap,
-                                                         uint8_t *out_low, uint64_t low_cap)
+__global__ __launch_bounds__(kBlock) void k_export_count(TableView tv, bool singles_to_low, uint64_t n_tiles,
+                                                         uint64_t *tile_counts)
 {
     const uint64_t cap = tv.mask + 1;
-    const uint64_t iters = (cap + (uint64_t)gridDim.x * kBlock - 1) / ((uint64_t)gridDim.x * kBlock);
-    for (uint64_t it = 0; it < iters; ++it) {
-        const uint64_t i = (it * gridDim.x + blockIdx.x) * (uint64_t)kBlock + threadIdx.x;
-        bool is_high = false, is_low = false;
-        uint64_t key[NW] = {};
-        uint32_t cc = 0;
-        if (i < cap) {
-            const uint64_t *s = tv.slots + i * slot_words<NW>();
-            if (s[0] != KC_EMPTY) {
-#pragma unroll
-                for (int w = 0; w < NW; ++w) key[w] = s[w];
-                const uint64_t c = s[NW == 1 ? 1 : 2];
-                if (singles_to_low && c == 1) {
-                    is_low = true;
-                } else if (c > 0) {
-                    cc = (uint32_t)(c < tv.count_max ? c : tv.count_max);
-                    is_high = !((uint64_t)cc + 1 < tv.filter_min || cc > tv.filter_max);
-                }
+    for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+        uint32_t h = 0, l = 0;
+        for (int j = 0; j < kExportPer; ++j) {
+            const uint64_t i = tile * kExportTile + (uint64_t)j * kBlock + threadIdx.x;
+            if (i >= cap) break;
+            uint64_t key[NW];
+            uint32_t cc;
+            const int kind = classify_slot<NW>(tv, i, singles_to_low, key, cc);
+            h += kind == 1;
+            l += kind == 2;
+        }
+        uint32_t th, tl;
+        block_scan2(h, l, th, tl);
+        if (threadIdx.x == 0) {
+            tile_counts[2 * tile] = th;
+            tile_counts[2 * tile + 1] = tl;
+        }
+    }
+}
+
+// Exclusive scan of the per-tile pairs in one block; totals land at the end.
+__global__ __launch_bounds__(1024) void k_scan_tiles(uint64_t *tile_counts, uint64_t n_tiles)
+{
+    __shared__ uint64_t part_h[1024], part_l[1024];
+    const uint64_t per = (n_tiles + 1023) / 1024;
+    const uint64_t b = threadIdx.x * per, e = b + per < n_tiles ? b + per : n_tiles;
+    uint64_t h = 0, l = 0;
+    for (uint64_t i = b; i < e; ++i) {
+        h += tile_counts[2 * i];
+        l += tile_counts[2 * i + 1];
+    }
+    part_h[threadIdx.x] = h;
+    part_l[threadIdx.x] = l;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t rh = 0, rl = 0;
+        for (int i = 0; i < 1024; ++i) {
+            const uint64_t xh = part_h[i], xl = part_l[i];
+            part_h[i] = rh;
+            part_l[i] = rl;
+            rh += xh;
+            rl += xl;
+        }
+        tile_counts[2 * n_tiles] = rh;
+        tile_counts[2 * n_tiles + 1] = rl;
+    }
+    __syncthreads();
+    h = part_h[threadIdx.x];
+    l = part_l[threadIdx.x];
+    for (uint64_t i = b; i < e; ++i) {
+        const uint64_t xh = tile_counts[2 * i], xl = tile_counts[2 * i + 1];
+        tile_counts[2 * i] = h;
+        tile_counts[2 * i + 1] = l;
+        h += xh;
+        l += xl;
+    }
+}
+
+// Pass 2: pack records at the scanned offsets.  Low records from the table
+// (singletons) follow the n_list FIRST-list records in the low buffer.
+template <int NW>
+__global__ __launch_bounds__(kBlock) void k_export_write(TableView tv, PackSpec ps, bool singles_to_low,
+                                                         uint64_t n_tiles, const uint64_t *tile_off,
+                                                         uint8_t *out_high, uint8_t *out_low, uint64_t n_list)
+{
+    const uint64_t cap = tv.mask + 1;
+    const uint32_t hrec = ps.kmer_bytes + ps.count_bytes;
+    for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+        uint32_t h = 0, l = 0;
+        for (int j = 0; j < kExportPer; ++j) {
+            const uint64_t i = tile * kExportTile + (uint64_t)threadIdx.x * kExportPer + j;
+            if (i >= cap) break;
+            uint64_t key[NW];
+            uint32_t cc;
+            const int kind = classify_slot<NW>(tv, i, singles_to_low, key, cc);
+            h += kind == 1;
+            l += kind == 2;
+        }
+        uint32_t th, tl;
+        block_scan2(h, l, th, tl);
+        uint64_t oh = tile_off[2 * tile] + h, ol = n_list + tile_off[2 * tile + 1] + l;
+        for (int j = 0; j < kExportPer; ++j) {
+            const uint64_t i = tile * kExportTile + (uint64_t)threadIdx.x * kExportPer + j;
+            if (i >= cap) break;
+            uint64_t key[NW];
+            uint32_t cc = 0;
+            const int kind = classify_slot<NW>(tv, i, singles_to_low, key, cc);
+            if (kind == 1) {
+                uint8_t *d = out_high + oh * hrec;
+                pack_key<NW>(d, key, ps);
+                put_bytes(d + ps.kmer_bytes, cc, ps.count_bytes, 0);
+                ++oh;
+            } else if (kind == 2) {
+                pack_key<NW>(out_low + ol * ps.kmer_bytes, key, ps);
+                ++ol;
             }
         }
-        const uint64_t hi = wave_append(tv.counters + KC_CNT_OUT_HIGH, is_high);
-        const uint64_t lo = wave_append(tv.counters + KC_CNT_OUT_LOW, is_low);
-        if (is_high && out_high && hi < high_cap) {
-            uint8_t *d = out_high + hi * (ps.kmer_bytes + ps.count_bytes);
-            pack_key<NW>(d, key, ps);
-            put_bytes(d + ps.kmer_bytes, cc, ps.count_bytes, 0);
-        }
-        if (is_low && out_low && lo < low_cap) pack_key<NW>(out_low + lo * ps.kmer_bytes, key, ps);
     }
 }
 
 template <int NW>
-__global__ __launch_bounds__(kBlock) void k_export_low_list(TableView tv, PackSpec ps, uint64_t n,
-                                                            uint8_t *out_low, uint64_t low_base)
+__global__ __launch_bounds__(kBlock) void k_export_low_list(TableView tv, PackSpec ps, uint64_t n, uint8_t *out_low)
 {
-    for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) {
-        uint64_t key[NW];
-#pragma unroll
-        for (int w = 0; w < NW; ++w) key[w] = tv.low[i * NW + w];
-        pack_key<NW>(out_low + (low_base + i) * ps.kmer_bytes, key, ps);
-    }
+    for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock)
+        pack_key<NW>(out_low + i * ps.kmer_bytes, low_key(tv, i, NW), ps);
 }
 
 // ---- histogram (histogram_tool precise over the exported view) ----------
@@ -537,9 +669,7 @@ __global__ __launch_bounds__(kBlock) void k_hist_low_list(TableView tv, HistSpec
     for (uint32_t b = threadIdx.x; b < kLdsBins; b += kBlock) lds_bins[b] = 0;
     __syncthreads();
     for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) {
-        uint64_t key[NW];
-#pragma unroll
-        for (int w = 0; w < NW; ++w) key[w] = tv.low[i * NW + w];
+        const uint64_t *key = low_key(tv, i, NW);
         const uint64_t c = table_find<NW>(tv.slots, tv.mask, key);
         const int64_t cc = high_view<NW>(tv, c, false);
         if (cc >= 0 && (uint64_t)cc + 1 >= h.min_freq) {
@@ -620,7 +750,8 @@ TableView view_of(kc_table *t)
     v.bloom = t->bloom;
     v.bloom_mask = t->bloom_blocks ? t->bloom_blocks - 1 : 0;
     v.low = t->low;
-    v.low_cap = t->low_cap;
+    v.seg_cap = t->seg_cap;
+    v.seg_off = t->seg_off;
     v.counters = reinterpret_cast<unsigned long long *>(t->counters);
     v.k = t->cfg.k;
     v.count_max = t->cfg.count_max;
@@ -645,10 +776,39 @@ PackSpec pack_spec(const kc_table *t)
 
 bool low_list_active(const kc_table *t) { return t->bloom && !t->sealed; }
 
-int read_counters(kc_table *t, uint64_t *h)
+struct HostCounters {
+    uint64_t stat[KC_STAT_N];
+    uint64_t seg_fill[KC_LOW_SEGS];  // FIRST keys stored per segment
+    uint64_t low;                    // sum of seg_fill
+    bool low_overflow;
+};
+
+// Reads the device counters, sums the statistic lanes and, when the FIRST
+// list is active, uploads its segment offsets for the kernels that read it.
+int read_counters(kc_table *t, HostCounters &hc)
 {
-    KC_HIP(hipMemcpyAsync(h, t->counters, KC_CNT_N * 8, hipMemcpyDeviceToHost, t->ctx->stream));
+    std::vector<uint64_t> h(KC_COUNTER_WORDS);
+    KC_HIP(hipMemcpyAsync(h.data(), t->counters, KC_COUNTER_WORDS * 8, hipMemcpyDeviceToHost, t->ctx->stream));
     KC_HIP(hipStreamSynchronize(t->ctx->stream));
+    for (int st = 0; st < KC_STAT_N; ++st) {
+        hc.stat[st] = 0;
+        for (int l = 0; l < KC_LANES; ++l) hc.stat[st] += h[((uint64_t)st * KC_LANES + l) * KC_LINE_WORDS];
+    }
+    hc.low = 0;
+    hc.low_overflow = false;
+    std::vector<uint64_t> off(KC_LOW_SEGS + 1);
+    for (int sgi = 0; sgi < KC_LOW_SEGS; ++sgi) {
+        const uint64_t c = h[KC_CURSOR_BASE + (uint64_t)sgi * KC_LINE_WORDS];
+        hc.low_overflow |= c > t->seg_cap;
+        hc.seg_fill[sgi] = std::min(c, t->seg_cap);
+        off[sgi] = hc.low;
+        hc.low += hc.seg_fill[sgi];
+    }
+    off[KC_LOW_SEGS] = hc.low;
+    if (t->seg_off) {
+        KC_HIP(hipMemcpyAsync(t->seg_off, off.data(), off.size() * 8, hipMemcpyHostToDevice, t->ctx->stream));
+        KC_HIP(hipStreamSynchronize(t->ctx->stream));
+    }
     return KC_OK;
 }
 
@@ -790,7 +950,7 @@ static int table_init_device(kc_table *t)
     KC_HIP(hipGetLastError());
     int rc;
     if (t->bloom && (rc = kc_launch_fill(c, t->bloom, 2 * t->bloom_blocks, 0)) != KC_OK) return rc;
-    if ((rc = kc_launch_fill(c, t->counters, KC_CNT_N, 0)) != KC_OK) return rc;
+    if ((rc = kc_launch_fill(c, t->counters, KC_COUNTER_WORDS, 0)) != KC_OK) return rc;
     t->sealed = 0;
     return KC_OK;
 }
@@ -799,8 +959,8 @@ extern "C" int kc_table_create(kc_ctx *ctx, const kc_table_config *cfg, kc_table
 {
     if (!ctx || !cfg || !out) return KC_EINVAL;
     *out = nullptr;
-    if (cfg->k < 1 || cfg->k > 63 || cfg->count_max == 0 || cfg->filter_max < cfg->filter_min)
-        return cfg->k > 63 && cfg->k <= 128 ? KC_ENOTSUP : KC_EINVAL;
+    if (cfg->k > 63 && cfg->k <= 128) return KC_ENOTSUP;
+    if (cfg->k < 1 || cfg->k > 63 || cfg->count_max == 0 || cfg->filter_max < cfg->filter_min) return KC_EINVAL;
     if (cfg->capacity && !is_pow2(cfg->capacity)) return KC_EINVAL;
     if (cfg->bloom_blocks && !is_pow2(cfg->bloom_blocks)) return KC_EINVAL;
     if (!cfg->capacity && !cfg->max_distinct) return KC_EINVAL;
@@ -814,10 +974,15 @@ extern "C" int kc_table_create(kc_ctx *ctx, const kc_table_config *cfg, kc_table
     const uint64_t hint = cfg->max_distinct ? cfg->max_distinct : t->cap / 2;
     if (cfg->bloom) {
         t->bloom_blocks = cfg->bloom_blocks ? cfg->bloom_blocks : std::max<uint64_t>(4096, next_pow2(hint / 4));
-        t->low_cap = cfg->low_capacity ? cfg->low_capacity : hint;
+        // segments fill evenly (block b -> segment b % KC_LOW_SEGS); leave slack
+        const uint64_t want = cfg->low_capacity ? cfg->low_capacity : hint;
+        t->seg_cap = (want + want / 16 + KC_LOW_SEGS - 1) / KC_LOW_SEGS + 64;
+        t->low_cap = t->seg_cap * KC_LOW_SEGS;
     }
+    t->n_tiles = (t->cap + kExportTile - 1) / kExportTile;
     const uint64_t slot_bytes = t->nw == 1 ? 16 : 32;
-    const uint64_t need = t->cap * slot_bytes + 16 * t->bloom_blocks + 8ull * t->nw * t->low_cap + 8 * KC_CNT_N;
+    const uint64_t need = t->cap * slot_bytes + 16 * t->bloom_blocks + 8ull * t->nw * t->low_cap +
+                          8 * KC_COUNTER_WORDS + 16 * (t->n_tiles + 1);
     size_t free_b = 0, total_b = 0;
     KC_HIP(hipMemGetInfo(&free_b, &total_b));
     if (need > free_b) {
@@ -825,7 +990,9 @@ extern "C" int kc_table_create(kc_ctx *ctx, const kc_table_config *cfg, kc_table
         return KC_ENOMEM;
     }
     bool ok = hipMalloc((void **)&t->slots, t->cap * slot_bytes) == hipSuccess &&
-              hipMalloc((void **)&t->counters, 8 * KC_CNT_N) == hipSuccess;
+              hipMalloc((void **)&t->counters, 8 * KC_COUNTER_WORDS) == hipSuccess &&
+              hipMalloc((void **)&t->tile_counts, 16 * (t->n_tiles + 1)) == hipSuccess &&
+              hipMalloc((void **)&t->seg_off, 8 * (KC_LOW_SEGS + 1)) == hipSuccess;
     if (ok && t->bloom_blocks) ok = hipMalloc((void **)&t->bloom, 16 * t->bloom_blocks) == hipSuccess;
     if (ok && t->low_cap) ok = hipMalloc((void **)&t->low, 8ull * t->nw * t->low_cap) == hipSuccess;
     if (!ok) {
@@ -846,10 +1013,9 @@ extern "C" void kc_table_destroy(kc_table *t)
     if (!t) return;
     (void)hipSetDevice(t->ctx->device);
     (void)hipStreamSynchronize(t->ctx->stream);
-    if (t->slots) (void)hipFree(t->slots);
-    if (t->bloom) (void)hipFree(t->bloom);
-    if (t->low) (void)hipFree(t->low);
-    if (t->counters) (void)hipFree(t->counters);
+    for (void *p : {(void *)t->slots, (void *)t->bloom, (void *)t->low, (void *)t->counters, (void *)t->seg_off,
+                    (void *)t->tile_counts})
+        if (p) (void)hipFree(p);
     for (int i = 0; i < 2; ++i) {
         if (t->pinned[i]) (void)hipHostFree(t->pinned[i]);
         if (t->dstage[i]) (void)hipFree(t->dstage[i]);
@@ -942,12 +1108,17 @@ extern "C" int kc_table_increment(kc_table *t, const uint64_t *d_keys, uint64_t 
 static int seal_table(kc_table *t)
 {
     if (!low_list_active(t)) return KC_OK;
-    uint64_t h[KC_CNT_N];
-    int rc = read_counters(t, h);
+    HostCounters hc;
+    int rc = read_counters(t, hc);
     if (rc != KC_OK) return rc;
-    const uint64_t n = std::min<uint64_t>(h[KC_CNT_LOW], t->low_cap);
-    rc = kc_table_increment(t, t->low, n, nullptr);
-    if (rc != KC_OK) return rc;
+    if (hc.low_overflow) return KC_EFULL;
+    if (hc.low) {
+        const TableView v = view_of(t);
+        const uint32_t g = grid_for(hc.low, t->ctx->cu_count);
+        if (t->nw == 1) hipLaunchKernelGGL(k_increment_low<1>, dim3(g), dim3(kBlock), 0, t->ctx->stream, hc.low, v);
+        else hipLaunchKernelGGL(k_increment_low<2>, dim3(g), dim3(kBlock), 0, t->ctx->stream, hc.low, v);
+        KC_HIP(hipGetLastError());
+    }
     t->sealed = 1;
     return KC_OK;
 }
@@ -969,14 +1140,14 @@ extern "C" int kc_table_query(kc_table *t, const uint64_t *d_keys, uint64_t n, u
 extern "C" int kc_table_stats(kc_table *t, kc_stats *out)
 {
     if (!t || !out) return KC_EINVAL;
-    uint64_t h[KC_CNT_N];
-    int rc = read_counters(t, h);
+    HostCounters hc;
+    int rc = read_counters(t, hc);
     if (rc != KC_OK) return rc;
-    out->kmers = h[KC_CNT_KMERS];
-    out->distinct = h[KC_CNT_DISTINCT];
-    out->low = t->sealed ? 0 : h[KC_CNT_LOW];
+    out->kmers = hc.stat[KC_STAT_KMERS];
+    out->distinct = hc.stat[KC_STAT_DISTINCT];
+    out->low = t->sealed ? 0 : hc.low;
     out->capacity = t->cap;
-    out->errors = h[KC_CNT_ERRORS] + (h[KC_CNT_LOW] > t->low_cap ? 1 : 0);
+    out->errors = hc.stat[KC_STAT_ERRORS] + (hc.low_overflow ? 1 : 0);
     return KC_OK;
 }
 
@@ -985,38 +1156,36 @@ extern "C" int kc_table_export_device(kc_table *t, uint8_t *d_high, uint64_t hig
 {
     if (!t || !high_bytes || !low_bytes) return KC_EINVAL;
     kc_ctx *c = t->ctx;
-    uint64_t h[KC_CNT_N];
-    int rc = read_counters(t, h);
+    HostCounters hc;
+    int rc = read_counters(t, hc);
     if (rc != KC_OK) return rc;
-    if (h[KC_CNT_ERRORS] || h[KC_CNT_LOW] > t->low_cap) return KC_EFULL;
+    if (hc.stat[KC_STAT_ERRORS] || hc.low_overflow) return KC_EFULL;
     const PackSpec ps = pack_spec(t);
     const uint64_t hrec = ps.kmer_bytes + ps.count_bytes;
     const bool list = low_list_active(t);
-    const uint64_t n_list = list ? h[KC_CNT_LOW] : 0;
-    // reset the output cursors, then a counting (or writing) pass
-    if ((rc = kc_launch_fill(c, t->counters + KC_CNT_OUT_HIGH, 2, 0)) != KC_OK) return rc;
+    const uint64_t n_list = list ? hc.low : 0;
     const TableView v = view_of(t);
-    const uint32_t g = grid_for(t->cap, c->cu_count);
-    const bool write = d_high && d_low;
-    uint8_t *oh = write ? d_high : nullptr;
-    uint8_t *ol = write ? d_low + n_list * ps.kmer_bytes : nullptr;
-    const uint64_t hcap = write ? high_cap / hrec : 0;
-    const uint64_t lcap = write ? (low_cap / ps.kmer_bytes > n_list ? low_cap / ps.kmer_bytes - n_list : 0) : 0;
-    if (t->nw == 1) hipLaunchKernelGGL(k_export_table<1>, dim3(g), dim3(kBlock), 0, c->stream, v, ps, !list, oh, hcap, ol, lcap);
-    else hipLaunchKernelGGL(k_export_table<2>, dim3(g), dim3(kBlock), 0, c->stream, v, ps, !list, oh, hcap, ol, lcap);
+    const uint32_t g = (uint32_t)std::min<uint64_t>(t->n_tiles, (uint64_t)c->cu_count * 8);
+    if (t->nw == 1) hipLaunchKernelGGL(k_export_count<1>, dim3(g), dim3(kBlock), 0, c->stream, v, !list, t->n_tiles, t->tile_counts);
+    else hipLaunchKernelGGL(k_export_count<2>, dim3(g), dim3(kBlock), 0, c->stream, v, !list, t->n_tiles, t->tile_counts);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, c->stream, t->tile_counts, t->n_tiles);
     KC_HIP(hipGetLastError());
-    if (write && n_list) {
-        if (low_cap < n_list * ps.kmer_bytes) return KC_EINVAL;
+    uint64_t tot[2];
+    KC_HIP(hipMemcpyAsync(tot, t->tile_counts + 2 * t->n_tiles, 16, hipMemcpyDeviceToHost, c->stream));
+    KC_HIP(hipStreamSynchronize(c->stream));
+    *high_bytes = tot[0] * hrec;
+    *low_bytes = (tot[1] + n_list) * ps.kmer_bytes;
+    if (!d_high || !d_low) return KC_OK;
+    if (*high_bytes > high_cap || *low_bytes > low_cap) return KC_EINVAL;
+    if (t->nw == 1) hipLaunchKernelGGL(k_export_write<1>, dim3(g), dim3(kBlock), 0, c->stream, v, ps, !list, t->n_tiles, t->tile_counts, d_high, d_low, n_list);
+    else hipLaunchKernelGGL(k_export_write<2>, dim3(g), dim3(kBlock), 0, c->stream, v, ps, !list, t->n_tiles, t->tile_counts, d_high, d_low, n_list);
+    KC_HIP(hipGetLastError());
+    if (n_list) {
         const uint32_t g2 = grid_for(n_list, c->cu_count);
-        if (t->nw == 1) hipLaunchKernelGGL(k_export_low_list<1>, dim3(g2), dim3(kBlock), 0, c->stream, v, ps, n_list, d_low, 0ull);
-        else hipLaunchKernelGGL(k_export_low_list<2>, dim3(g2), dim3(kBlock), 0, c->stream, v, ps, n_list, d_low, 0ull);
+        if (t->nw == 1) hipLaunchKernelGGL(k_export_low_list<1>, dim3(g2), dim3(kBlock), 0, c->stream, v, ps, n_list, d_low);
+        else hipLaunchKernelGGL(k_export_low_list<2>, dim3(g2), dim3(kBlock), 0, c->stream, v, ps, n_list, d_low);
         KC_HIP(hipGetLastError());
     }
-    rc = read_counters(t, h);
-    if (rc != KC_OK) return rc;
-    *high_bytes = h[KC_CNT_OUT_HIGH] * hrec;
-    *low_bytes = (h[KC_CNT_OUT_LOW] + n_list) * ps.kmer_bytes;
-    if (write && (*high_bytes > high_cap || *low_bytes > low_cap)) return KC_EINVAL;
     return KC_OK;
 }
 
@@ -1065,10 +1234,10 @@ extern "C" int kc_table_histogram(kc_table *t, uint32_t min_freq, uint32_t max_f
 {
     if (!t || !h_bins || max_freq < min_freq) return KC_EINVAL;
     kc_ctx *c = t->ctx;
-    uint64_t h[KC_CNT_N];
-    int rc = read_counters(t, h);
+    HostCounters hc;
+    int rc = read_counters(t, hc);
     if (rc != KC_OK) return rc;
-    if (h[KC_CNT_ERRORS] || h[KC_CNT_LOW] > t->low_cap) return KC_EFULL;
+    if (hc.stat[KC_STAT_ERRORS] || hc.low_overflow) return KC_EFULL;
     const uint64_t nb = (uint64_t)max_freq - min_freq + 1;
     unsigned long long *gb = nullptr;
     KC_HIP(hipMalloc((void **)&gb, nb * 8));
@@ -1080,11 +1249,10 @@ extern "C" int kc_table_histogram(kc_table *t, uint32_t min_freq, uint32_t max_f
         const uint32_t g = grid_for(t->cap, c->cu_count);
         if (t->nw == 1) hipLaunchKernelGGL(k_hist_table<1>, dim3(g), dim3(kBlock), 0, c->stream, v, hs, gb);
         else hipLaunchKernelGGL(k_hist_table<2>, dim3(g), dim3(kBlock), 0, c->stream, v, hs, gb);
-        if (list && h[KC_CNT_LOW]) {
-            const uint64_t n = h[KC_CNT_LOW];
-            const uint32_t g2 = grid_for(n, c->cu_count);
-            if (t->nw == 1) hipLaunchKernelGGL(k_hist_low_list<1>, dim3(g2), dim3(kBlock), 0, c->stream, v, hs, n, gb);
-            else hipLaunchKernelGGL(k_hist_low_list<2>, dim3(g2), dim3(kBlock), 0, c->stream, v, hs, n, gb);
+        if (list && hc.low) {
+            const uint32_t g2 = grid_for(hc.low, c->cu_count);
+            if (t->nw == 1) hipLaunchKernelGGL(k_hist_low_list<1>, dim3(g2), dim3(kBlock), 0, c->stream, v, hs, hc.low, gb);
+            else hipLaunchKernelGGL(k_hist_low_list<2>, dim3(g2), dim3(kBlock), 0, c->stream, v, hs, hc.low, gb);
         }
         if (hipGetLastError() != hipSuccess ||
             hipMemcpyAsync(h_bins, gb, nb * 8, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||

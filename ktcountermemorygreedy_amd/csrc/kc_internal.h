@@ -32,15 +32,23 @@ constexpr uint64_t KC_EMPTY = ~0ULL;
 // Bit 63 of a count word marks nothing in the stored data; counts stay
 // below 2^63 by construction (at most 2^64 bytes of input).
 
-enum kc_counter_idx : int {
-    KC_CNT_KMERS = 0,
-    KC_CNT_DISTINCT = 1,
-    KC_CNT_LOW = 2,
-    KC_CNT_ERRORS = 3,
-    KC_CNT_OUT_HIGH = 4,
-    KC_CNT_OUT_LOW = 5,
-    KC_CNT_N = 8,
+// Device counters.  Statistics are spread over KC_LANES lanes, one 128-byte
+// line each, so that blocks update different lines (block b uses lane
+// b % KC_LANES); readers sum the lanes.  The FIRST-occurrence list is cut
+// into KC_LOW_SEGS segments with one cursor line each (block b appends to
+// segment b % KC_LOW_SEGS), for the same reason.
+enum kc_stat_idx : int {
+    KC_STAT_KMERS = 0,
+    KC_STAT_DISTINCT = 1,
+    KC_STAT_ERRORS = 2,
+    KC_STAT_N = 3,
 };
+constexpr int KC_LANES = 64;
+constexpr int KC_LINE_WORDS = 16;
+constexpr int KC_LOW_SEGS = 256;
+constexpr uint64_t KC_STAT_WORDS = (uint64_t)KC_STAT_N * KC_LANES * KC_LINE_WORDS;
+constexpr uint64_t KC_CURSOR_BASE = KC_STAT_WORDS;
+constexpr uint64_t KC_COUNTER_WORDS = KC_CURSOR_BASE + (uint64_t)KC_LOW_SEGS * KC_LINE_WORDS;
 
 struct kc_table {
     kc_ctx *ctx;
@@ -51,9 +59,13 @@ struct kc_table {
     uint64_t *slots;        // cap * (nw == 1 ? 2 : 4) words
     uint64_t bloom_blocks;  // 0 when Bloom is off
     uint64_t *bloom;        // 2 * bloom_blocks words {BF1, BF2}
-    uint64_t low_cap;
-    uint64_t *low;          // low_cap * nw words
-    uint64_t *counters;     // KC_CNT_N device u64
+    uint64_t low_cap;       // KC_LOW_SEGS * seg_cap
+    uint64_t seg_cap;       // keys per FIRST-list segment
+    uint64_t *low;          // low_cap * nw words, segment s at s * seg_cap
+    uint64_t *seg_off;      // KC_LOW_SEGS + 1 exclusive offsets (device)
+    uint64_t *counters;     // KC_COUNTER_WORDS device u64
+    uint64_t *tile_counts;  // export scan scratch: 2 per tile (+2)
+    uint64_t n_tiles;
     // host staging for kc_table_count_seq_host
     uint8_t *pinned[2];
     uint8_t *dstage[2];
