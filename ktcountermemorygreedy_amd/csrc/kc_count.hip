@@ -974,9 +974,12 @@ extern "C" int kc_table_create(kc_ctx *ctx, const kc_table_config *cfg, kc_table
     const uint64_t hint = cfg->max_distinct ? cfg->max_distinct : t->cap / 2;
     if (cfg->bloom) {
         t->bloom_blocks = cfg->bloom_blocks ? cfg->bloom_blocks : std::max<uint64_t>(4096, next_pow2(hint / 4));
-        // segments fill evenly (block b -> segment b % KC_LOW_SEGS); leave slack
+        // Block b appends to segment b % KC_LOW_SEGS: large grids fill the
+        // segments evenly (1/16 slack); a grid of fewer blocks than segments
+        // puts up to one tile of windows (kTile FIRSTs) into one segment.
         const uint64_t want = cfg->low_capacity ? cfg->low_capacity : hint;
-        t->seg_cap = (want + want / 16 + KC_LOW_SEGS - 1) / KC_LOW_SEGS + 64;
+        t->seg_cap = std::max<uint64_t>((want + want / 16 + KC_LOW_SEGS - 1) / KC_LOW_SEGS + 64,
+                                        std::min<uint64_t>(want, kTile));
         t->low_cap = t->seg_cap * KC_LOW_SEGS;
     }
     t->n_tiles = (t->cap + kExportTile - 1) / kExportTile;
