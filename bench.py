@@ -80,6 +80,19 @@ def cpu_baseline(ws, sample_reads, k, genome):
             "sample": "first %d reads, oracle extract+count" % sample_reads}
 
 
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of the kernel from the committed rocprof --pmc
+    summary (profiles/traffic.json), when it covers this kernel and workload."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            rec = json.load(f).get(kernel)
+        if rec and rec["workload"] == workload:
+            return rec["bytes_per_launch"]
+    except (OSError, ValueError, KeyError):
+        pass
+    return None
+
+
 def main():
     args = parse_args()
     if args.gpus != 1:
@@ -173,7 +186,7 @@ def main():
                    "export_high_bytes": hb, "export_low_bytes": lb,
                    "step": "clear + fused parse/canonical/Bloom/count + export packing (device)"},
         "roofline": {"bound": "hbm", "kernel": "k_count_seq", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("k_count_seq", ws.name),
                      "alg_bytes_per_launch": alg_bytes, "avg_launch_ms": avg_count_s * 1e3},
         "cpu_baseline": cpu,
         "checks": {"windows_match": windows == K, "hist_mass_equals_kmers": mass == K},
