@@ -133,6 +133,17 @@ int kc_sort_records(kc_ctx *ctx, void *d_recs, void *d_scratch, uint64_t n,
                     uint32_t digit_bits, uint32_t record_bytes,
                     int *result_in_scratch);
 
+/* Abundance histogram of a record array's counts: bins[f-min] for f in
+ * [min, max], host int64 (histogram_tool's bins over exact counts). */
+int kc_records_histogram(kc_ctx *ctx, const void *d_recs, uint64_t n, uint32_t nw,
+                         uint32_t record_bytes, uint32_t min_freq, uint32_t max_freq,
+                         int64_t *h_bins);
+/* Seeded record workload (no reference counterpart): key i is a 62-bit
+ * bijection of i in the k=31 layout, counts Zipf-like with exponent s,
+ * P(c >= x) = x^-(s-1), clipped to [1, 2^32-1].  record_bytes 12 or 16. */
+int kc_synth_records(kc_ctx *ctx, void *d_recs, uint64_t n, uint32_t record_bytes,
+                     uint64_t seed, double zipf_s);
+
 /* ---- hashes of the raw key bytes, bit-exact with the vendored headers ---- */
 #define KC_HASH_XXH3_64 0      /* XXH3_64bits (xxhash.h:6428) */
 #define KC_HASH_XXH3_128_LO 1  /* XXH3_128bits low64 (xxhash.h:7249) */

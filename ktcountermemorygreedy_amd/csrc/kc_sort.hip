@@ -1,6 +1,463 @@
-// placeholder replaced below
+// Stable LSD radix sort of export records on device, the replacement of
+// ExportRecordRadixSort (ExportRecordRadixSort.h:225-269, 8-bit digits) and
+// radix_sort_export_records_12bit (ExportRecordRadixSort12.h:444-646).
+//
+// Both reference sorts are stable sorts on a contiguous bit range of the
+// 64N-bit key (word 0 most significant).  A stable sort is unique, so any
+// stable sort on the same range reproduces them record for record; this one
+// uses 8-bit digits from the low end of the range:
+//   upsweep    per-tile digit counts (LDS histogram) -> counts[digit][tile]
+//   scan       exclusive scan of counts in digit-major order (global offsets)
+//   downsweep  per-tile stable ranking (wave ballots, waves in order), the
+//              tile is placed digit-sorted in LDS and written out in runs.
+// Records are 32-bit-word arrays: {key words (2N u32), count u32, [pad]} at
+// 12, 16, 20 or 24 bytes.  Integer work only, HBM-bound: each pass reads and
+// writes every record once (plus the key words once in the upsweep).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
 #include "kc_internal.h"
-extern "C" int kc_sort_records(kc_ctx *, void *, void *, uint64_t, uint32_t, uint32_t, uint32_t, uint32_t, uint32_t, int *)
+
+namespace kcs {
+
+constexpr int kThreads = 256;
+constexpr int kDigitBits = 8;
+constexpr int kBins = 1 << kDigitBits;
+
+template <int RW> struct Layout {
+    static constexpr int NW = RW >= 5 ? 2 : 1;   // key words (u64)
+    static constexpr int ITEMS = RW >= 5 ? 8 : 16; // records per thread per tile
+    static constexpr int TILE = kThreads * ITEMS;
+};
+
+// Digit of bits [bit, bit+width) of the key (LSB-numbered over the whole
+// 64*NW-bit number); the range may straddle two words.
+template <int NW>
+__device__ __forceinline__ uint32_t digit_of(const uint32_t *rec, uint32_t bit, uint32_t width)
 {
-    return KC_ENOTSUP;
+    // u32 words 2w, 2w+1 of the record are key word w (little-endian halves);
+    // the word pair is picked with selects so that register arrays are never
+    // indexed dynamically.
+    const uint64_t k0 = (uint64_t)rec[0] | ((uint64_t)rec[1] << 32);
+    uint64_t v;
+    if constexpr (NW == 1) {
+        v = k0 >> bit;
+    } else {
+        const uint64_t k1 = (uint64_t)rec[2] | ((uint64_t)rec[3] << 32);
+        const uint32_t sh = bit & 63;
+        if (bit >= 64) {
+            v = k0 >> sh;
+        } else {
+            v = k1 >> sh;
+            if (sh + width > 64) v |= k0 << (64 - sh);
+        }
+    }
+    return (uint32_t)(v & ((1u << width) - 1));
+}
+
+template <int RW>
+__device__ __forceinline__ void load_rec(const uint32_t *src, uint64_t i, uint32_t *r)
+{
+#pragma unroll
+    for (int j = 0; j < RW; ++j) r[j] = src[i * RW + j];
+}
+
+template <int RW>
+__global__ __launch_bounds__(kThreads) void k_upsweep(const uint32_t *__restrict__ src, uint64_t n, uint32_t bit,
+                                                      uint32_t width, uint64_t n_tiles, uint32_t *__restrict__ counts)
+{
+    using L = Layout<RW>;
+    __shared__ uint32_t hist[kBins];
+    for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+        hist[threadIdx.x] = 0;
+        __syncthreads();
+        for (int j = 0; j < L::ITEMS; ++j) {
+            const uint64_t i = tile * L::TILE + (uint64_t)j * kThreads + threadIdx.x;
+            if (i < n) {
+                uint32_t r[RW];
+                load_rec<RW>(src, i, r);
+                atomicAdd(&hist[digit_of<L::NW>(r, bit, width)], 1u);
+            }
+        }
+        __syncthreads();
+        counts[(uint64_t)threadIdx.x * n_tiles + tile] = hist[threadIdx.x];
+        __syncthreads();
+    }
+}
+
+// Exclusive scan of a u32 array into u64 offsets: per-block sums, a scan of
+// the block sums, then the per-block scan with its base.
+constexpr int kScanItems = 16;
+constexpr uint64_t kScanChunk = (uint64_t)kThreads * kScanItems;
+
+__device__ __forceinline__ uint64_t block_excl_scan(uint64_t v, uint64_t &total)
+{
+    __shared__ uint64_t s[kThreads];
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 1; o < kThreads; o <<= 1) {
+        const uint64_t a = threadIdx.x >= (uint32_t)o ? s[threadIdx.x - o] : 0;
+        __syncthreads();
+        s[threadIdx.x] += a;
+        __syncthreads();
+    }
+    total = s[kThreads - 1];
+    const uint64_t r = s[threadIdx.x] - v;
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(kThreads) void k_scan_reduce(const uint32_t *in, uint64_t n, uint64_t *block_sums)
+{
+    const uint64_t base = (uint64_t)blockIdx.x * kScanChunk;
+    uint64_t acc = 0;
+    for (int j = 0; j < kScanItems; ++j) {
+        const uint64_t i = base + (uint64_t)threadIdx.x * kScanItems + j;
+        if (i < n) acc += in[i];
+    }
+    uint64_t total;
+    block_excl_scan(acc, total);
+    if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kThreads) void k_scan_sums(uint64_t *sums, uint64_t nb)
+{
+    // single block: sequential chunks of kThreads
+    uint64_t carry = 0;
+    for (uint64_t b = 0; b < nb; b += kThreads) {
+        const uint64_t i = b + threadIdx.x;
+        const uint64_t v = i < nb ? sums[i] : 0;
+        uint64_t total;
+        const uint64_t ex = block_excl_scan(v, total);
+        if (i < nb) sums[i] = carry + ex;
+        carry += total;
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_scan_apply(const uint32_t *in, uint64_t n, const uint64_t *block_base,
+                                                         uint64_t *out)
+{
+    const uint64_t base = (uint64_t)blockIdx.x * kScanChunk;
+    uint64_t acc = 0;
+    uint32_t v[kScanItems];
+    for (int j = 0; j < kScanItems; ++j) {
+        const uint64_t i = base + (uint64_t)threadIdx.x * kScanItems + j;
+        v[j] = i < n ? in[i] : 0;
+        acc += v[j];
+    }
+    uint64_t total;
+    uint64_t run = block_excl_scan(acc, total) + block_base[blockIdx.x];
+    for (int j = 0; j < kScanItems; ++j) {
+        const uint64_t i = base + (uint64_t)threadIdx.x * kScanItems + j;
+        if (i < n) out[i] = run;
+        run += v[j];
+    }
+}
+
+// Stable scatter of one tile.  Chunks of kThreads consecutive records are
+// ranked in order; inside a chunk, wave w's records precede wave w+1's and
+// lanes are ordered by lane id, so ranks follow input order exactly.
+template <int RW>
+__global__ __launch_bounds__(kThreads) void k_downsweep(const uint32_t *__restrict__ src, uint32_t *__restrict__ dst,
+                                                        uint64_t n, uint32_t bit, uint32_t width, uint64_t n_tiles,
+                                                        const uint64_t *__restrict__ offsets)
+{
+    using L = Layout<RW>;
+    __shared__ uint32_t stage[L::TILE * RW];
+    __shared__ uint32_t running[kBins];   // per-digit count so far in this tile
+    __shared__ uint32_t wcnt[4][kBins];   // per-wave digit counts of the chunk
+    __shared__ uint32_t local_start[kBins];
+    __shared__ uint64_t gbase[kBins];
+    const int wave = threadIdx.x / 64;
+    const uint64_t lane_lt = (1ULL << __lane_id()) - 1ULL;
+
+    for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+        const uint64_t t0 = tile * L::TILE;
+        const uint32_t tn = (uint32_t)(n - t0 < (uint64_t)L::TILE ? n - t0 : L::TILE);
+        running[threadIdx.x] = 0;
+        gbase[threadIdx.x] = offsets[(uint64_t)threadIdx.x * n_tiles + tile];
+        uint32_t r[L::ITEMS][RW];
+        uint32_t rank[L::ITEMS], dig[L::ITEMS];
+#pragma unroll
+        for (int j = 0; j < L::ITEMS; ++j) {
+            const uint32_t li = (uint32_t)j * kThreads + threadIdx.x;
+            const bool valid = li < tn;
+            if (valid) load_rec<RW>(src, t0 + li, r[j]);
+            const uint32_t d = valid ? digit_of<L::NW>(r[j], bit, width) : 0;
+            dig[j] = d;
+            // peers: valid lanes of this wave holding the same digit
+            uint64_t peers = __ballot(valid);
+#pragma unroll
+            for (uint32_t b = 0; b < kDigitBits; ++b) {
+                const uint64_t m = __ballot((d >> b) & 1);
+                peers &= ((d >> b) & 1) ? m : ~m;
+            }
+            const uint32_t before = __popcll(peers & lane_lt);
+            const bool leader = valid && before == 0;
+            for (int w = 0; w < 4; ++w) wcnt[w][threadIdx.x] = 0;
+            __syncthreads();
+            if (leader) wcnt[wave][d] = __popcll(peers);
+            __syncthreads();
+            {
+                // digit threadIdx.x: prefix over waves, then advance running
+                uint32_t acc = running[threadIdx.x];
+                for (int w = 0; w < 4; ++w) {
+                    const uint32_t c = wcnt[w][threadIdx.x];
+                    wcnt[w][threadIdx.x] = acc;
+                    acc += c;
+                }
+                running[threadIdx.x] = acc;
+            }
+            __syncthreads();
+            rank[j] = valid ? wcnt[wave][d] + before : 0;
+            __syncthreads();
+        }
+        // tile-local start of each digit
+        {
+            const uint32_t c = running[threadIdx.x];
+            __shared__ uint32_t sc[kBins];
+            sc[threadIdx.x] = c;
+            __syncthreads();
+            for (int o = 1; o < kBins; o <<= 1) {
+                const uint32_t a = threadIdx.x >= (uint32_t)o ? sc[threadIdx.x - o] : 0;
+                __syncthreads();
+                sc[threadIdx.x] += a;
+                __syncthreads();
+            }
+            local_start[threadIdx.x] = sc[threadIdx.x] - c;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < L::ITEMS; ++j) {
+            const uint32_t li = (uint32_t)j * kThreads + threadIdx.x;
+            if (li < tn) {
+                const uint32_t p = local_start[dig[j]] + rank[j];
+#pragma unroll
+                for (int q = 0; q < RW; ++q) stage[p * RW + q] = r[j][q];
+            }
+        }
+        __syncthreads();
+        // write out in local order: consecutive positions of one digit land
+        // on consecutive global addresses
+        for (uint32_t p = threadIdx.x; p < tn; p += kThreads) {
+            const uint32_t *rec = stage + p * RW;
+            const uint32_t d = digit_of<L::NW>(rec, bit, width);
+            const uint64_t g = gbase[d] + (p - local_start[d]);
+#pragma unroll
+            for (int q = 0; q < RW; ++q) dst[g * RW + q] = rec[q];
+        }
+        __syncthreads();
+    }
+}
+
+// Abundance histogram of record counts (bins f-min for f in [min, max]).
+template <int RW>
+__global__ __launch_bounds__(kThreads) void k_count_hist(const uint32_t *__restrict__ recs, uint64_t n, uint32_t min_f,
+                                                         uint32_t max_f, unsigned long long *bins)
+{
+    constexpr int NW = Layout<RW>::NW;
+    __shared__ uint32_t lb[4096];
+    for (int b = threadIdx.x; b < 4096; b += kThreads) lb[b] = 0;
+    __syncthreads();
+    for (uint64_t i = blockIdx.x * (uint64_t)kThreads + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kThreads) {
+        const uint32_t c = recs[i * RW + 2 * NW];
+        if (c < min_f || c > max_f) continue;
+        const uint32_t b = c - min_f;
+        if (b < 4096) atomicAdd(&lb[b], 1u);
+        else atomicAdd(bins + b, 1ull);
+    }
+    __syncthreads();
+    const uint64_t nb = (uint64_t)max_f - min_f + 1;
+    for (uint32_t b = threadIdx.x; b < 4096 && b < nb; b += kThreads)
+        if (lb[b]) atomicAdd(bins + b, (unsigned long long)lb[b]);
+}
+
+// Seeded export-record workload: key i = a 62-bit bijection of i (k=31
+// layout: left-aligned in word 0, low 2 bits zero), count drawn from a
+// Zipf-like law P(c >= x) = x^-(s-1) by inverse CDF, clipped to [1, 2^32-1].
+__device__ __forceinline__ uint64_t mix62(uint64_t x, uint64_t seed)
+{
+    constexpr uint64_t M = (1ULL << 62) - 1;
+    x = (x ^ seed) & M;
+    x = (x * 0x9E3779B97F4A7C15ULL) & M;
+    x ^= x >> 31;
+    x = (x * 0xBF58476D1CE4E5B9ULL) & M;
+    x ^= x >> 29;
+    x = (x * 0x94D049BB133111EBULL) & M;
+    x ^= x >> 32;
+    return x;
+}
+
+template <int RW>
+__global__ __launch_bounds__(kThreads) void k_synth_records(uint32_t *recs, uint64_t n, uint64_t seed, double inv_exp)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)kThreads + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kThreads) {
+        const uint64_t key = mix62(i, seed) << 2;
+        uint64_t h = (i + 1) * 0xD1B54A32D192ED03ULL ^ seed;
+        h ^= h >> 33;
+        h *= 0xff51afd7ed558ccdULL;
+        h ^= h >> 33;
+        const double u = ((h >> 11) + 1) * (1.0 / 9007199254740992.0);  // (0, 1]
+        const double c = floor(pow(u, -inv_exp));
+        const uint32_t cnt = c >= 4294967295.0 ? 0xFFFFFFFFu : (c < 1.0 ? 1u : (uint32_t)c);
+        uint32_t *r = recs + i * RW;
+        r[0] = (uint32_t)key;
+        r[1] = (uint32_t)(key >> 32);
+#pragma unroll
+        for (int q = 2; q < RW; ++q) r[q] = 0;
+        r[2 * Layout<RW>::NW] = cnt;
+    }
+}
+
+}  // namespace kcs
+
+using namespace kcs;
+
+namespace {
+
+struct SortScratch {
+    uint32_t *counts = nullptr;
+    uint64_t *offsets = nullptr;
+    uint64_t *sums = nullptr;
+};
+
+template <int RW>
+int sort_impl(kc_ctx *ctx, uint32_t *a, uint32_t *b, uint64_t n, uint32_t lo, uint32_t hi, int *in_scratch)
+{
+    using L = Layout<RW>;
+    *in_scratch = 0;
+    if (n <= 1 || hi <= lo) return KC_OK;
+    const uint64_t n_tiles = (n + L::TILE - 1) / L::TILE;
+    const uint64_t m = (uint64_t)kBins * n_tiles;
+    const uint64_t nb = (m + kScanChunk - 1) / kScanChunk;
+    SortScratch s;
+    if (hipMalloc((void **)&s.counts, m * 4) != hipSuccess || hipMalloc((void **)&s.offsets, m * 8) != hipSuccess ||
+        hipMalloc((void **)&s.sums, nb * 8) != hipSuccess) {
+        (void)hipFree(s.counts);
+        (void)hipFree(s.offsets);
+        (void)hipFree(s.sums);
+        return KC_ENOMEM;
+    }
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(n_tiles, (uint64_t)ctx->cu_count * 8);
+    uint32_t *src = a, *dst = b;
+    int passes = 0;
+    hipError_t e = hipSuccess;
+    for (uint32_t bit = lo; bit < hi; bit += kDigitBits) {
+        const uint32_t width = std::min<uint32_t>(kDigitBits, hi - bit);
+        hipLaunchKernelGGL(k_upsweep<RW>, dim3(grid), dim3(kThreads), 0, ctx->stream, src, n, bit, width, n_tiles, s.counts);
+        hipLaunchKernelGGL(k_scan_reduce, dim3((uint32_t)nb), dim3(kThreads), 0, ctx->stream, s.counts, m, s.sums);
+        hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kThreads), 0, ctx->stream, s.sums, nb);
+        hipLaunchKernelGGL(k_scan_apply, dim3((uint32_t)nb), dim3(kThreads), 0, ctx->stream, s.counts, m, s.sums, s.offsets);
+        hipLaunchKernelGGL(k_downsweep<RW>, dim3(grid), dim3(kThreads), 0, ctx->stream, src, dst, n, bit, width, n_tiles, s.offsets);
+        if ((e = hipGetLastError()) != hipSuccess) break;
+        std::swap(src, dst);
+        ++passes;
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(s.counts);
+    (void)hipFree(s.offsets);
+    (void)hipFree(s.sums);
+    if (e != hipSuccess) {
+        kc_set_error("radix sort", e);
+        return KC_EDEVICE;
+    }
+    *in_scratch = passes & 1;
+    return KC_OK;
+}
+
+// Key range of ExportRecordRadixSort (ExportRecordRadixSort.h:225-254):
+// MSB-numbered positions from 2k-1 down in steps of at most 8 bits (never
+// crossing a word), 2k-20 bits in total, each step reading 8 bits.
+void range8(uint32_t nw, uint32_t k, uint32_t &lo, uint32_t &hi)
+{
+    lo = 64 * nw - 2 * k;
+    hi = lo;
+    uint32_t end_pos = 2 * k - 1, diff = 2 * k - 20;
+    while (diff > 0) {
+        uint32_t bits = end_pos + 1 - (end_pos / 64) * 64;
+        bits = std::min(bits, 8u);
+        bits = std::min(bits, diff);
+        diff -= bits;
+        const uint32_t word = end_pos / 64;
+        const uint32_t lsb = 63 - end_pos % 64;
+        hi = std::max(hi, 64 * (nw - 1 - word) + std::min(lsb + 8, 64u));
+        end_pos -= bits;
+    }
+}
+
+}  // namespace
+
+extern "C" int kc_sort_records(kc_ctx *ctx, void *d_recs, void *d_scratch, uint64_t n, uint32_t nw, uint32_t k,
+                               uint32_t cpb, uint32_t digit_bits, uint32_t record_bytes, int *result_in_scratch)
+{
+    if (!ctx || !result_in_scratch || (n && (!d_recs || !d_scratch)) || d_recs == d_scratch) return KC_EINVAL;
+    if (nw != 1 && nw != 2) return KC_EINVAL;
+    if (record_bytes != 8 * nw + 4 && record_bytes != 8 * nw + 8) return KC_EINVAL;
+    if ((reinterpret_cast<uintptr_t>(d_recs) | reinterpret_cast<uintptr_t>(d_scratch)) & 3) return KC_EINVAL;
+    uint32_t lo, hi;
+    if (digit_bits == 12) {
+        // ExportRecordRadixSort12.h:481-503 argument checks
+        if (k < cpb || k > 32 * nw) return KC_EINVAL;
+        if (n > 0xFFFFFFFFULL) return KC_EINVAL;
+        lo = 64 * nw - 2 * k;
+        hi = 64 * nw - 2 * cpb;
+    } else if (digit_bits == 8) {
+        if (k < 10 || k > 32 * nw) return KC_EINVAL;
+        range8(nw, k, lo, hi);
+    } else {
+        return KC_EINVAL;
+    }
+    KC_HIP(hipSetDevice(ctx->device));
+    uint32_t *a = static_cast<uint32_t *>(d_recs), *b = static_cast<uint32_t *>(d_scratch);
+    switch (record_bytes / 4) {
+    case 3: return sort_impl<3>(ctx, a, b, n, lo, hi, result_in_scratch);
+    case 4: return sort_impl<4>(ctx, a, b, n, lo, hi, result_in_scratch);
+    case 5: return sort_impl<5>(ctx, a, b, n, lo, hi, result_in_scratch);
+    case 6: return sort_impl<6>(ctx, a, b, n, lo, hi, result_in_scratch);
+    default: return KC_EINVAL;
+    }
+}
+
+extern "C" int kc_records_histogram(kc_ctx *ctx, const void *d_recs, uint64_t n, uint32_t nw, uint32_t record_bytes,
+                                    uint32_t min_freq, uint32_t max_freq, int64_t *h_bins)
+{
+    if (!ctx || !h_bins || (n && !d_recs) || max_freq < min_freq || (nw != 1 && nw != 2)) return KC_EINVAL;
+    if (record_bytes != 8 * nw + 4 && record_bytes != 8 * nw + 8) return KC_EINVAL;
+    const uint64_t nb = (uint64_t)max_freq - min_freq + 1;
+    unsigned long long *gb = nullptr;
+    KC_HIP(hipMalloc((void **)&gb, nb * 8));
+    int rc = kc_launch_fill(ctx, reinterpret_cast<uint64_t *>(gb), nb, 0);
+    if (rc == KC_OK && n) {
+        const uint32_t g = (uint32_t)std::min<uint64_t>((n + kThreads - 1) / kThreads, (uint64_t)ctx->cu_count * 8);
+        const uint32_t *r = static_cast<const uint32_t *>(d_recs);
+        switch (record_bytes / 4) {
+        case 3: hipLaunchKernelGGL(k_count_hist<3>, dim3(g), dim3(kThreads), 0, ctx->stream, r, n, min_freq, max_freq, gb); break;
+        case 4: hipLaunchKernelGGL(k_count_hist<4>, dim3(g), dim3(kThreads), 0, ctx->stream, r, n, min_freq, max_freq, gb); break;
+        case 5: hipLaunchKernelGGL(k_count_hist<5>, dim3(g), dim3(kThreads), 0, ctx->stream, r, n, min_freq, max_freq, gb); break;
+        default: hipLaunchKernelGGL(k_count_hist<6>, dim3(g), dim3(kThreads), 0, ctx->stream, r, n, min_freq, max_freq, gb); break;
+        }
+        if (hipGetLastError() != hipSuccess) rc = KC_EDEVICE;
+    }
+    if (rc == KC_OK && (hipMemcpyAsync(h_bins, gb, nb * 8, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+                        hipStreamSynchronize(ctx->stream) != hipSuccess))
+        rc = KC_EDEVICE;
+    (void)hipFree(gb);
+    return rc;
+}
+
+extern "C" int kc_synth_records(kc_ctx *ctx, void *d_recs, uint64_t n, uint32_t record_bytes, uint64_t seed,
+                                double zipf_s)
+{
+    if (!ctx || (n && !d_recs) || zipf_s <= 1.0 || (record_bytes != 12 && record_bytes != 16)) return KC_EINVAL;
+    if (!n) return KC_OK;
+    const uint32_t g = (uint32_t)std::min<uint64_t>((n + kThreads - 1) / kThreads, (uint64_t)ctx->cu_count * 16);
+    uint32_t *r = static_cast<uint32_t *>(d_recs);
+    if (record_bytes == 12)
+        hipLaunchKernelGGL(k_synth_records<3>, dim3(g), dim3(kThreads), 0, ctx->stream, r, n, seed, 1.0 / (zipf_s - 1.0));
+    else
+        hipLaunchKernelGGL(k_synth_records<4>, dim3(g), dim3(kThreads), 0, ctx->stream, r, n, seed, 1.0 / (zipf_s - 1.0));
+    KC_HIP(hipGetLastError());
+    return KC_OK;
 }

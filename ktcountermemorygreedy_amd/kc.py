@@ -85,6 +85,9 @@ EXPORTS = {
     "kc_table_histogram": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]),
     "kc_sort_records": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32,
                                   C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(C.c_int)]),
+    "kc_records_histogram": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32,
+                                       C.c_uint32, C.c_uint32, C.c_void_p]),
+    "kc_synth_records": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint64, C.c_double]),
     "kc_hash_keys": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_uint64, C.c_uint32, C.c_void_p]),
     "kc_bloom_probe_keys": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint64,
                                       C.c_void_p, C.c_void_p]),
@@ -250,10 +253,20 @@ class Context:
         return blk.numpy(np.uint64, n), msk.numpy(np.uint64, n)
 
     def sort_records(self, recs, scratch, n, nw, k, cpb, digit_bits, record_bytes):
+        """Stable sort of device records; returns True when the result is in scratch."""
         flag = C.c_int(0)
         check(lib().kc_sort_records(self.h, _ptr(recs), _ptr(scratch), n, nw, k, cpb, digit_bits,
                                     record_bytes, C.byref(flag)), "kc_sort_records")
         return bool(flag.value)
+
+    def records_histogram(self, recs, n, nw, record_bytes, min_freq=1, max_freq=10000):
+        bins = np.zeros(max_freq - min_freq + 1, dtype=np.int64)
+        check(lib().kc_records_histogram(self.h, _ptr(recs), n, nw, record_bytes, min_freq, max_freq,
+                                         _ptr(bins)), "kc_records_histogram")
+        return bins
+
+    def synth_records(self, recs, n, record_bytes=12, seed=4, zipf_s=1.2):
+        check(lib().kc_synth_records(self.h, _ptr(recs), n, record_bytes, seed, zipf_s), "kc_synth_records")
 
 
 class Table:
