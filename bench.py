@@ -33,7 +33,9 @@ def parse_args():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="repeat_100mbp", choices=["repeat_100mbp", "count_5mbp"])
+    ap.add_argument("--workload", default="repeat_100mbp",
+                    choices=["repeat_100mbp", "count_5mbp", "export_sort"])
+    ap.add_argument("--records-log2", type=int, default=30, help="export_sort: 2^N records")
     ap.add_argument("--reads", type=int, default=0, help="override the read count (smaller runs)")
     ap.add_argument("--no-bloom", action="store_true")
     ap.add_argument("--cpu-sample-reads", type=int, default=4_000_000)
@@ -93,12 +95,97 @@ def measured_traffic(kernel, workload):
     return None
 
 
+def sort_cpu_baseline(n_sample):
+    """Reference radix_sort_export_records_12bit (single thread, harness built
+    from the reference header) on n_sample records of the same law."""
+    import numpy as np
+    ref = os.path.join(ROOT, "oracle", "_ref", "sort_kat_ref")
+    rng = np.random.default_rng(4)
+    keys = (rng.integers(0, 2**62, n_sample, dtype=np.uint64) << np.uint64(2)).reshape(-1, 1)
+    counts = np.minimum(np.floor(rng.random(n_sample) ** -5.0), 2**32 - 1).astype(np.uint32)
+    if os.path.exists(ref):
+        hdr = np.array([1, 31, 10, 12], np.uint32).tobytes() + np.array([n_sample], np.uint64).tobytes()
+        rec = np.zeros((n_sample, 12), np.uint8)
+        rec[:, :8] = keys.view(np.uint8).reshape(-1, 8)
+        rec[:, 8:] = counts.view(np.uint8).reshape(-1, 4)
+        t0 = time.time()
+        subprocess.run([ref], input=hdr + rec.tobytes(), stdout=subprocess.DEVNULL, check=True)
+        dt = time.time() - t0
+        return {"value": n_sample / dt, "unit": "records/s", "cores": 1, "kind": "reference",
+                "sample": "%d records through radix_sort_export_records_12bit<1,10> (reference header, "
+                          "one thread), wall time incl. pipe I/O %.2f s" % (n_sample, dt)}
+    from oracle import oracle as O
+    t0 = time.time()
+    O.sort_records12(keys, counts, 31, 10)
+    dt = time.time() - t0
+    return {"value": n_sample / dt, "unit": "records/s", "cores": 1, "kind": "port",
+            "sample": "%d records, oracle 12-bit LSD" % n_sample}
+
+
+def bench_sort(args):
+    """Export/sort workload: 2^N distinct k=31 keys with Zipf(1.2) counts as
+    12-byte records; one step = stable radix sort on bits [2,44) (the
+    12-bit sort's range) + abundance histogram 1..10000."""
+    import numpy as np
+
+    from ktcountermemorygreedy_amd import kc
+    n = 1 << args.records_log2
+    ctx = kc.Context(0)
+    d, s = ctx.alloc(12 * n), ctx.alloc(12 * n)
+    src = ctx.alloc(12 * n)
+    ctx.synth_records(src, n, 12, seed=4, zipf_s=1.2)
+    ctx.sync()
+
+    def step(slot=None):
+        kc.check(kc.lib().kc_copy_d2d(ctx.h, kc._ptr(d), kc._ptr(src), 12 * n), "reset records")
+        if slot is not None:
+            ctx.record(slot)
+        in_s = ctx.sort_records(d, s, n, 1, 31, 10, 12, 12)
+        bins = ctx.records_histogram(s if in_s else d, n, 1, 12, 1, 10000)
+        if slot is not None:
+            ctx.record(slot + 1)
+        return in_s, bins
+
+    for _ in range(args.warmup):
+        step()
+    ms = []
+    for i in range(args.steps):
+        in_s, bins = step(2 * (i % 32))
+        ms.append(ctx.elapsed_ms(2 * (i % 32), 2 * (i % 32) + 1))
+    out = (s if in_s else d).numpy(np.uint32, 3 * min(n, 1 << 24)).reshape(-1, 3)
+    k = out[:, 0].astype(np.uint64) | (out[:, 1].astype(np.uint64) << np.uint64(32))
+    sk = (k >> np.uint64(2)) & np.uint64((1 << 42) - 1)
+    step_s = float(np.mean(ms)) * 1e-3
+    passes = (42 + 7) // 8
+    alg = n * (2 * 12 * 4 + 4)
+    line = {
+        "metric": "export records sorted/s (stable, 12-bit sort key range) + abundance histogram",
+        "value": n / step_s, "unit": "records/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": step_s * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u64", "data": "synthetic",
+        "config": {"workload": "export_sort_2^%d" % args.records_log2, "records": n, "record_bytes": 12,
+                   "key_bits": [2, 44], "digit_bits": 8, "passes": passes, "hist_bins": [1, 10000]},
+        "roofline": {"bound": "hbm", "kernel": "sort pipeline", "achieved": alg / step_s / 1e9,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": alg / step_s / 1e9 / HBM_PEAK_GBS,
+                     "traffic": None, "alg_bytes_per_launch": alg,
+                     "alg_bytes_rule": "n*(2*12*4+4): 4 passes of 12-bit digits as in the reference; this "
+                                       "build runs %d 8-bit passes" % passes},
+        "cpu_baseline": None if args.no_cpu_baseline else sort_cpu_baseline(1 << 24),
+        "checks": {"prefix_sorted": bool(np.all(sk[1:] >= sk[:-1])),
+                   "hist_mass": int(bins.sum())},
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
 def main():
     args = parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has one MI355X; --gpus %d is not "
               "supported (the multi-GPU design is described in DESIGN.md)" % args.gpus)
         sys.exit(2)
+    if args.workload == "export_sort":
+        return bench_sort(args)
     import numpy as np
 
     from ktcountermemorygreedy_amd import kc
