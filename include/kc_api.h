@@ -37,6 +37,13 @@ typedef struct kc_table kc_table;
 
 /* Table configuration.  Mirrors the Tree CLI knobs (main.cpp:484-554):
  * k, filter_min, filter_max, count_max; capacity replaces map_capacity. */
+/* Counting paths: partitioned (hash-partition scatter, then each table
+ * partition counted in LDS) or direct (global atomics per k-mer).  Both fill
+ * the same table layout; AUTO picks partitioned for large batches. */
+#define KC_PATH_AUTO 0
+#define KC_PATH_DIRECT 1
+#define KC_PATH_PARTITIONED 2
+
 typedef struct kc_table_config {
     uint32_t k;            /* 1..63 */
     uint32_t count_max;    /* saturation value (main.cpp:599-615), >= 1 */
@@ -45,7 +52,7 @@ typedef struct kc_table_config {
     uint64_t capacity;     /* table slots, power of two; 0 = derive from max_distinct */
     uint64_t max_distinct; /* sizing hint when capacity == 0 */
     uint32_t bloom;        /* 1 = Bloom singleton split (BloomFilter.h:149-298) */
-    uint32_t reserved;
+    uint32_t count_path;   /* KC_PATH_AUTO / _DIRECT / _PARTITIONED */
     uint64_t bloom_blocks; /* {BF1,BF2} blocks, power of two; 0 = derive */
     uint64_t low_capacity; /* FIRST-occurrence list capacity; 0 = derive */
 } kc_table_config;

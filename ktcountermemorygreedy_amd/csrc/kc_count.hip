@@ -11,15 +11,12 @@
 #include <string>
 #include <vector>
 
-#include "kc_hash_dev.h"
-#include "kc_internal.h"
+#include "kc_dev_common.h"
 
 namespace {
 thread_local char g_last_error[256];
 constexpr int kBlock = 256;
-constexpr int kSeg = 64;                       // bytes of window ends per thread
-constexpr int kTile = kBlock * kSeg;           // bytes per block
-constexpr uint32_t kValidMask = (1u << 0) | (1u << 2) | (1u << 6) | (1u << 19) | (1u << 20);
+constexpr int kTile = kc::kParseTile;           // bytes per block
 }  // namespace
 
 void kc_set_error(const char *what, hipError_t e)
@@ -43,203 +40,6 @@ extern "C" const char *kc_strerror(int s)
 
 namespace kc {
 
-struct TableView {
-    uint64_t *slots;
-    uint64_t mask;
-    uint64_t *bloom;       // nullptr when the Bloom split is off
-    uint64_t bloom_mask;
-    uint64_t *low;
-    uint64_t seg_cap;            // keys per FIRST-list segment
-    const uint64_t *seg_off;     // KC_LOW_SEGS + 1 exclusive offsets (after counting)
-    unsigned long long *counters;
-    uint32_t k;
-    uint32_t count_max;
-    uint32_t filter_min;
-    uint32_t filter_max;
-};
-
-template <int NW> constexpr int slot_words() { return NW == 1 ? 2 : 4; }
-
-__device__ __forceinline__ uint64_t lanemask_lt()
-{
-    return (1ULL << __lane_id()) - 1ULL;
-}
-
-__device__ __forceinline__ uint64_t ld_agent(const uint64_t *p)
-{
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ __forceinline__ void add_agent(uint64_t *p, uint64_t v)
-{
-    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Wave sum of a 32-bit statistic, then one atomic per wave into this
-// block's lane of the statistic (lanes live on separate 128-byte lines).
-__device__ __forceinline__ void stat_add(unsigned long long *counters, int stat, uint32_t v)
-{
-    uint64_t s = v;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-    if (__lane_id() == 0 && s)
-        atomicAdd(counters + ((uint64_t)stat * KC_LANES + (blockIdx.x % KC_LANES)) * KC_LINE_WORDS,
-                  (unsigned long long)s);
-}
-
-// i-th key of the FIRST list seen as the concatenation of its segments.
-__device__ __forceinline__ const uint64_t *low_key(const TableView &tv, uint64_t i, int nw)
-{
-    int lo = 0, hi = KC_LOW_SEGS;  // find the last segment with seg_off <= i
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (tv.seg_off[mid] <= i) lo = mid; else hi = mid;
-    }
-    return tv.low + ((uint64_t)lo * tv.seg_cap + (i - tv.seg_off[lo])) * nw;
-}
-
-// ---- table insert / add ------------------------------------------------
-// Open addressing, linear probing.  A slot is claimed with a 64-bit CAS on
-// its first key word; counts are 64-bit atomic adds (saturation is applied
-// when results are read, so min(true count, count_max) is exact).
-template <int NW>
-__device__ __forceinline__ bool table_add(uint64_t *slots, uint64_t mask, const uint64_t *key,
-                                          uint64_t inc, uint32_t &n_new, uint32_t &n_err)
-{
-    uint64_t idx = xxh3_64<NW>(key) & mask;
-    for (uint64_t probe = 0; probe <= mask; ++probe) {
-        uint64_t *s = slots + idx * slot_words<NW>();
-        // A plain read settles the common case (key present: one atomic add).
-        // A stale EMPTY is harmless: the CAS below returns the real word.
-        uint64_t w0 = s[0];
-        bool owner = false;
-        if (w0 == KC_EMPTY) {
-            w0 = atomicCAS((unsigned long long *)s, (unsigned long long)KC_EMPTY, (unsigned long long)key[0]);
-            owner = w0 == KC_EMPTY;
-        }
-        if constexpr (NW == 1) {
-            if (owner || w0 == key[0]) {
-                n_new += owner;
-                add_agent(s + 1, inc);
-                return true;
-            }
-        } else {
-            // Two-word key: the CAS winner publishes word 1; a thread that
-            // finds word 0 equal waits for word 1 (bounded) and compares.
-            if (owner) __hip_atomic_store(s + 1, key[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __builtin_amdgcn_wave_barrier();
-            if (owner) {
-                ++n_new;
-                add_agent(s + 2, inc);
-                return true;
-            }
-            if (w0 == key[0]) {
-                uint64_t w1 = ld_agent(s + 1);
-                for (int spin = 0; w1 == KC_EMPTY && spin < (1 << 16); ++spin) {
-                    __builtin_amdgcn_s_sleep(1);
-                    w1 = ld_agent(s + 1);
-                }
-                if (w1 == KC_EMPTY) {
-                    ++n_err;
-                    return false;
-                }
-                if (w1 == key[1]) {
-                    add_agent(s + 2, inc);
-                    return true;
-                }
-            }
-        }
-        idx = (idx + 1) & mask;
-    }
-    ++n_err;
-    return false;
-}
-
-// Lookup: returns the raw stored count (0 when absent).
-template <int NW>
-__device__ __forceinline__ uint64_t table_find(const uint64_t *slots, uint64_t mask, const uint64_t *key)
-{
-    uint64_t idx = xxh3_64<NW>(key) & mask;
-    for (uint64_t probe = 0; probe <= mask; ++probe) {
-        const uint64_t *s = slots + idx * slot_words<NW>();
-        const uint64_t w0 = s[0];
-        if (w0 == KC_EMPTY) return 0;
-        if (w0 == key[0]) {
-            if constexpr (NW == 1) return s[1];
-            else if (s[1] == key[1]) return s[2];
-        }
-        idx = (idx + 1) & mask;
-    }
-    return 0;
-}
-
-// Bloom singleton split state machine (BloomFilter.h:233-273): returns
-// 0 FIRST, 1 SECOND, 2 THIRD_PLUS.  Only atomic results decide a FIRST, so
-// each distinct key gets at most one FIRST.
-template <int NW>
-__device__ __forceinline__ int bloom_classify(uint64_t *bins, uint64_t bmask, const uint64_t *key)
-{
-    uint64_t blk, m;
-    bloom_probe<NW>(key, bmask, blk, m);
-    uint64_t *cell = bins + 2 * blk;
-    if ((ld_agent(cell) & m) == m) return 2;
-    if ((ld_agent(cell + 1) & m) == m) {
-        const uint64_t b1 = __hip_atomic_fetch_or(cell, m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        return ((b1 & m) == m) ? 2 : 1;
-    }
-    const uint64_t b2 = __hip_atomic_fetch_or(cell + 1, m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if ((b2 & m) == m) {
-        const uint64_t b1 = __hip_atomic_fetch_or(cell, m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        return ((b1 & m) == m) ? 2 : 1;
-    }
-    return 0;
-}
-
-// ---- rolling canonical state (GetKmer.h:58-82) -------------------------
-template <int NW> struct Roll;
-
-template <> struct Roll<1> {
-    uint64_t f = 0, r = 0;
-    uint64_t back_mask;
-    uint32_t rev_shift;
-    __device__ Roll(uint32_t k)
-    {
-        back_mask = (k % 32) ? (((1ULL << (2 * (k % 32))) - 1ULL) << (64 - 2 * (k % 32))) : ~0ULL;
-        rev_shift = (k % 32) ? 2 * (32 - k % 32) : 0;
-    }
-    __device__ __forceinline__ void push(uint64_t code)
-    {
-        f = ((f >> 2) | (code << 62)) & back_mask;
-        r = (r << 2) | ((code ^ 3) << rev_shift);
-    }
-    __device__ __forceinline__ void canon(uint64_t *out) const { out[0] = f < r ? f : r; }
-};
-
-template <> struct Roll<2> {
-    uint64_t f0 = 0, f1 = 0, r0 = 0, r1 = 0;
-    uint64_t back_mask;
-    uint32_t rev_shift;
-    __device__ Roll(uint32_t k)
-    {
-        back_mask = (k % 32) ? (((1ULL << (2 * (k % 32))) - 1ULL) << (64 - 2 * (k % 32))) : ~0ULL;
-        rev_shift = (k % 32) ? 2 * (32 - k % 32) : 0;
-    }
-    __device__ __forceinline__ void push(uint64_t code)
-    {
-        // data[0] most significant; the tail word (data[1]) is masked.
-        f1 = ((f1 >> 2) | (f0 << 62)) & back_mask;
-        f0 = (f0 >> 2) | (code << 62);
-        r0 = (r0 << 2) | (r1 >> 62);
-        r1 = (r1 << 2) | ((code ^ 3) << rev_shift);
-    }
-    __device__ __forceinline__ void canon(uint64_t *out) const
-    {
-        const bool fwd = (f0 < r0) || (f0 == r0 && f1 < r1);
-        out[0] = fwd ? f0 : r0;
-        out[1] = fwd ? f1 : r1;
-    }
-};
-
 // ---- fused count kernel ------------------------------------------------
 // One block stages kTile input bytes (+ the previous segment as halo) in
 // LDS, transposed so that lane t reading its own dword j hits consecutive
@@ -250,102 +50,19 @@ template <int NW, bool BLOOM>
 __global__ __launch_bounds__(kBlock) void k_count_seq(const uint8_t *__restrict__ seq, uint64_t n,
                                                       uint64_t emit_from, TableView tv)
 {
-    constexpr int kHaloWords = NW == 1 ? 8 : 16;   // >= k-1 bytes of history
-    __shared__ uint32_t lds[16][kBlock + 2];
-    const uint32_t t = threadIdx.x;
-    const uint64_t base = (uint64_t)blockIdx.x * kTile;
-
-    // Stage: 1024 uint4 of this tile plus the previous block's last segment.
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const uint32_t q = t + kBlock * r;      // uint4 index inside the tile
-        const uint64_t off = base + 16ull * q;
-        uint4 v = make_uint4(0, 0, 0, 0);
-        if (off + 16 <= n) {
-            v = *reinterpret_cast<const uint4 *>(seq + off);
-        } else if (off < n) {
-            uint32_t w[4] = {0, 0, 0, 0};
-            for (uint64_t b = off; b < n; ++b) w[(b - off) >> 2] |= (uint32_t)seq[b] << (8 * ((b - off) & 3));
-            v = make_uint4(w[0], w[1], w[2], w[3]);
-        }
-        const uint32_t seg = q >> 2, j0 = (q & 3) * 4;
-        lds[j0 + 0][seg + 1] = v.x;
-        lds[j0 + 1][seg + 1] = v.y;
-        lds[j0 + 2][seg + 1] = v.z;
-        lds[j0 + 3][seg + 1] = v.w;
-    }
-    if (t < 16) {
-        uint32_t v = 0;
-        if (base >= kSeg) {
-            const uint64_t off = base - kSeg + 4ull * t;
-            if (off + 4 <= n) v = *reinterpret_cast<const uint32_t *>(seq + off);
-        }
-        lds[t][0] = v;
-    }
-    __syncthreads();
-
-    const uint64_t start = base + (uint64_t)t * kSeg;
+    __shared__ ParseLds lds;
+    stage_tile(seq, n, blockIdx.x, lds);
     uint32_t n_kmers = 0, n_new = 0, n_err = 0;
-    if (start < n) {
-        Roll<NW> roll(tv.k);
-        uint32_t have = 0;
-        // halo: last kHaloWords dwords of the previous segment (column t)
-        for (int j = 16 - kHaloWords; j < 16; ++j) {
-            const uint32_t v = lds[j][t];
-#pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                const uint32_t c = (v >> (8 * b)) & 0xDF;
-                const uint32_t d = c - 0x41;
-                const bool valid = d < 32 && ((kValidMask >> d) & 1);
-                roll.push(((c >> 1) ^ (c >> 2)) & 3);
-                have = valid ? have + 1 : 0;
-            }
+    roll_segment<NW>(lds, blockIdx.x, n, emit_from, tv.k, [&](const uint64_t *key) {
+        ++n_kmers;
+        bool to_table = true;
+        if constexpr (BLOOM) {
+            const bool first = bloom_classify<NW>(tv, key) == 0;
+            first_append<NW>(tv, first, key, n_err);
+            to_table = !first;
         }
-        for (int j = 0; j < 16; ++j) {
-            const uint32_t v = lds[j][t + 1];
-#pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                const uint32_t c = (v >> (8 * b)) & 0xDF;
-                const uint32_t d = c - 0x41;
-                const bool valid = d < 32 && ((kValidMask >> d) & 1);
-                roll.push(((c >> 1) ^ (c >> 2)) & 3);
-                have = valid ? have + 1 : 0;
-                const uint64_t pos = start + 4 * j + b;
-                const bool emit = have >= tv.k && pos >= emit_from;
-                if (emit) {
-                    uint64_t key[NW];
-                    roll.canon(key);
-                    ++n_kmers;
-                    bool to_table = true;
-                    if constexpr (BLOOM) {
-                        const bool first = bloom_classify<NW>(tv.bloom, tv.bloom_mask, key) == 0;
-                        const uint64_t firsts = __ballot(first);
-                        if (firsts) {
-                            const int leader = __ffsll((unsigned long long)firsts) - 1;
-                            unsigned long long b0 = 0;
-                            const uint32_t seg = blockIdx.x % KC_LOW_SEGS;
-                            if (__lane_id() == leader)
-                                b0 = atomicAdd(tv.counters + KC_CURSOR_BASE + (uint64_t)seg * KC_LINE_WORDS,
-                                               (unsigned long long)__popcll(firsts));
-                            b0 = __shfl(b0, leader, 64);
-                            if (first) {
-                                const uint64_t at = b0 + __popcll(firsts & lanemask_lt());
-                                if (at < tv.seg_cap) {
-                                    uint64_t *dst = tv.low + ((uint64_t)seg * tv.seg_cap + at) * NW;
-#pragma unroll
-                                    for (int w = 0; w < NW; ++w) dst[w] = key[w];
-                                } else {
-                                    ++n_err;
-                                }
-                            }
-                        }
-                        to_table = !first;
-                    }
-                    if (to_table) table_add<NW>(tv.slots, tv.mask, key, 1, n_new, n_err);
-                }
-            }
-        }
-    }
+        if (to_table) table_add<NW>(tv, key, 1, n_new, n_err);
+    });
     stat_add(tv.counters, KC_STAT_KMERS, n_kmers);
     stat_add(tv.counters, KC_STAT_DISTINCT, n_new);
     stat_add(tv.counters, KC_STAT_ERRORS, n_err);
@@ -361,7 +78,7 @@ __global__ __launch_bounds__(kBlock) void k_increment(const uint64_t *__restrict
         uint64_t key[NW];
 #pragma unroll
         for (int w = 0; w < NW; ++w) key[w] = keys[i * NW + w];
-        table_add<NW>(tv.slots, tv.mask, key, inc ? inc[i] : 1, n_new, n_err);
+        table_add<NW>(tv, key, inc ? inc[i] : 1, n_new, n_err);
     }
     stat_add(tv.counters, KC_STAT_DISTINCT, n_new);
     stat_add(tv.counters, KC_STAT_ERRORS, n_err);
@@ -377,7 +94,7 @@ __global__ __launch_bounds__(kBlock) void k_increment_low(uint64_t n, TableView 
         const uint64_t *src = low_key(tv, i, NW);
 #pragma unroll
         for (int w = 0; w < NW; ++w) key[w] = src[w];
-        table_add<NW>(tv.slots, tv.mask, key, 1, n_new, n_err);
+        table_add<NW>(tv, key, 1, n_new, n_err);
     }
     stat_add(tv.counters, KC_STAT_DISTINCT, n_new);
     stat_add(tv.counters, KC_STAT_ERRORS, n_err);
@@ -391,7 +108,7 @@ __global__ __launch_bounds__(kBlock) void k_query(const uint64_t *__restrict__ k
         uint64_t key[NW];
 #pragma unroll
         for (int w = 0; w < NW; ++w) key[w] = keys[i * NW + w];
-        const uint64_t c = table_find<NW>(tv.slots, tv.mask, key);
+        const uint64_t c = table_find<NW>(tv, key);
         out[i] = (uint32_t)(c < tv.count_max ? c : tv.count_max);
     }
 }
@@ -670,7 +387,7 @@ __global__ __launch_bounds__(kBlock) void k_hist_low_list(TableView tv, HistSpec
     __syncthreads();
     for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) {
         const uint64_t *key = low_key(tv, i, NW);
-        const uint64_t c = table_find<NW>(tv.slots, tv.mask, key);
+        const uint64_t c = table_find<NW>(tv, key);
         const int64_t cc = high_view<NW>(tv, c, false);
         if (cc >= 0 && (uint64_t)cc + 1 >= h.min_freq) {
             hist_add(lds_bins, gbins, h, (uint64_t)cc, -1);
@@ -735,6 +452,13 @@ uint64_t next_pow2(uint64_t v)
 
 bool is_pow2(uint64_t v) { return v && !(v & (v - 1)); }
 
+uint32_t ilog2(uint64_t v)
+{
+    uint32_t r = 0;
+    while ((1ULL << r) < v) ++r;
+    return r;
+}
+
 uint32_t grid_for(uint64_t n, uint32_t cu)
 {
     const uint64_t want = (n + kBlock - 1) / kBlock;
@@ -747,8 +471,11 @@ TableView view_of(kc_table *t)
     TableView v;
     v.slots = t->slots;
     v.mask = t->cap - 1;
+    v.part_bits = t->part_bits;
+    v.part_log2 = t->part_log2;
     v.bloom = t->bloom;
     v.bloom_mask = t->bloom_blocks ? t->bloom_blocks - 1 : 0;
+    v.bloom_part_log2 = t->bloom_part_log2;
     v.low = t->low;
     v.seg_cap = t->seg_cap;
     v.seg_off = t->seg_off;
@@ -961,6 +688,7 @@ extern "C" int kc_table_create(kc_ctx *ctx, const kc_table_config *cfg, kc_table
     *out = nullptr;
     if (cfg->k > 63 && cfg->k <= 128) return KC_ENOTSUP;
     if (cfg->k < 1 || cfg->k > 63 || cfg->count_max == 0 || cfg->filter_max < cfg->filter_min) return KC_EINVAL;
+    if (cfg->count_path > KC_PATH_PARTITIONED) return KC_EINVAL;
     if (cfg->capacity && !is_pow2(cfg->capacity)) return KC_EINVAL;
     if (cfg->bloom_blocks && !is_pow2(cfg->bloom_blocks)) return KC_EINVAL;
     if (!cfg->capacity && !cfg->max_distinct) return KC_EINVAL;
@@ -972,8 +700,15 @@ extern "C" int kc_table_create(kc_ctx *ctx, const kc_table_config *cfg, kc_table
     t->nw = cfg->k <= 32 ? 1 : 2;
     t->cap = cfg->capacity ? cfg->capacity : std::max<uint64_t>(1024, next_pow2(cfg->max_distinct + cfg->max_distinct / 2));
     const uint64_t hint = cfg->max_distinct ? cfg->max_distinct : t->cap / 2;
+    // partitions of the size the LDS counting pass holds
+    const uint64_t lds_slots = kc_lds_part_slots(t->nw, cfg->bloom != 0);
+    const uint64_t n_parts = t->cap > lds_slots ? t->cap / lds_slots : 1;
+    t->part_bits = ilog2(n_parts);
+    t->part_log2 = ilog2(t->cap) - t->part_bits;
     if (cfg->bloom) {
-        t->bloom_blocks = cfg->bloom_blocks ? cfg->bloom_blocks : std::max<uint64_t>(4096, next_pow2(hint / 4));
+        t->bloom_blocks = cfg->bloom_blocks ? cfg->bloom_blocks : n_parts * kc_lds_part_bloom(t->nw);
+        t->bloom_blocks = std::max<uint64_t>(t->bloom_blocks, n_parts);
+        t->bloom_part_log2 = ilog2(t->bloom_blocks) - t->part_bits;
         // Block b appends to segment b % KC_LOW_SEGS: large grids fill the
         // segments evenly (1/16 slack); a grid of fewer blocks than segments
         // puts up to one tile of windows (kTile FIRSTs) into one segment.
@@ -1019,6 +754,7 @@ extern "C" void kc_table_destroy(kc_table *t)
     for (void *p : {(void *)t->slots, (void *)t->bloom, (void *)t->low, (void *)t->counters, (void *)t->seg_off,
                     (void *)t->tile_counts})
         if (p) (void)hipFree(p);
+    kc_part_free(t);
     for (int i = 0; i < 2; ++i) {
         if (t->pinned[i]) (void)hipHostFree(t->pinned[i]);
         if (t->dstage[i]) (void)hipFree(t->dstage[i]);
@@ -1034,11 +770,23 @@ extern "C" int kc_table_clear(kc_table *t)
     return table_init_device(t);
 }
 
+// Batches of at least this many bytes / keys take the partitioned path
+// under KC_PATH_AUTO.
+constexpr uint64_t kPartMinBytes = 32ull << 20;
+constexpr uint64_t kPartMinKeys = 4ull << 20;
+
+static bool use_partitioned(const kc_table *t, uint64_t size, uint64_t min_size)
+{
+    if (t->cfg.count_path == KC_PATH_DIRECT || !kc_part_eligible(t)) return false;
+    return t->cfg.count_path == KC_PATH_PARTITIONED || size >= min_size;
+}
+
 static int launch_count(kc_table *t, const uint8_t *d_seq, uint64_t nbytes, uint64_t emit_from)
 {
     if (!nbytes) return KC_OK;
-    const uint64_t blocks = (nbytes + kTile - 1) / kTile;
     const TableView v = view_of(t);
+    if (use_partitioned(t, nbytes, kPartMinBytes)) return kc_part_count(t, d_seq, nbytes, emit_from, nullptr, 0, v);
+    const uint64_t blocks = (nbytes + kTile - 1) / kTile;
     hipStream_t s = t->ctx->stream;
     const bool bloom = low_list_active(t);
     if (t->nw == 1) {
@@ -1099,6 +847,8 @@ extern "C" int kc_table_increment(kc_table *t, const uint64_t *d_keys, uint64_t 
     if (!t || (!d_keys && n)) return KC_EINVAL;
     if (!n) return KC_OK;
     const TableView v = view_of(t);
+    if (!d_inc && !(t->bloom && !t->sealed) && use_partitioned(t, n, kPartMinKeys))
+        return kc_part_count(t, nullptr, 0, 0, d_keys, n, v);
     const uint32_t g = grid_for(n, t->ctx->cu_count);
     if (t->nw == 1) hipLaunchKernelGGL(k_increment<1>, dim3(g), dim3(kBlock), 0, t->ctx->stream, d_keys, n, d_inc, v);
     else hipLaunchKernelGGL(k_increment<2>, dim3(g), dim3(kBlock), 0, t->ctx->stream, d_keys, n, d_inc, v);

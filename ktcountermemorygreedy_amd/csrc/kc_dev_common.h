@@ -1,0 +1,359 @@
+// Device-side pieces shared by the counting kernels: the table view and its
+// partitioned slot indexing, the global-atomic insert/lookup, the Bloom
+// state machine, the rolling canonical encoder and counter helpers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "kc_hash_dev.h"
+#include "kc_internal.h"
+
+namespace kc {
+
+constexpr uint32_t kValidMask = (1u << 0) | (1u << 2) | (1u << 6) | (1u << 19) | (1u << 20);
+
+// The table is cut into 2^part_bits partitions of part_slots slots (and
+// the Bloom filter into partitions of bloom_part blocks).  A key's partition
+// is the top part_bits of its XXH3_64 hash; inside the partition the slot
+// probe starts at the hash's low bits and wraps within the partition, and
+// the Bloom block is the reference's h1 & (blocks - 1) within the partition.
+struct TableView {
+    uint64_t *slots;
+    uint64_t mask;               // total slots - 1
+    uint32_t part_bits;          // log2(partitions)
+    uint32_t part_log2;          // log2(part_slots)
+    uint64_t *bloom;             // nullptr when the Bloom split is off
+    uint64_t bloom_mask;         // total blocks - 1
+    uint32_t bloom_part_log2;    // log2(blocks per partition)
+    uint64_t *low;
+    uint64_t seg_cap;            // keys per FIRST-list segment
+    const uint64_t *seg_off;     // KC_LOW_SEGS + 1 exclusive offsets (after counting)
+    unsigned long long *counters;
+    uint32_t k;
+    uint32_t count_max;
+    uint32_t filter_min;
+    uint32_t filter_max;
+};
+
+template <int NW> constexpr int slot_words() { return NW == 1 ? 2 : 4; }
+
+__device__ __forceinline__ uint64_t part_of(uint64_t h, uint32_t part_bits)
+{
+    return part_bits ? h >> (64 - part_bits) : 0;
+}
+
+// Slot index of probe step `probe` for a key hashing to h.
+__device__ __forceinline__ uint64_t slot_index(const TableView &tv, uint64_t h, uint64_t probe)
+{
+    const uint64_t pmask = (1ULL << tv.part_log2) - 1;
+    return (part_of(h, tv.part_bits) << tv.part_log2) | ((h + probe) & pmask);
+}
+
+__device__ __forceinline__ uint64_t lanemask_lt()
+{
+    return (1ULL << __lane_id()) - 1ULL;
+}
+
+__device__ __forceinline__ uint64_t ld_agent(const uint64_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void add_agent(uint64_t *p, uint64_t v)
+{
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Wave sum of a 32-bit statistic, then one atomic per wave into this
+// block's lane of the statistic (lanes live on separate 128-byte lines).
+__device__ __forceinline__ void stat_add(unsigned long long *counters, int stat, uint32_t v)
+{
+    uint64_t s = v;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (__lane_id() == 0 && s)
+        atomicAdd(counters + ((uint64_t)stat * KC_LANES + (blockIdx.x % KC_LANES)) * KC_LINE_WORDS,
+                  (unsigned long long)s);
+}
+
+// i-th key of the FIRST list seen as the concatenation of its segments.
+__device__ __forceinline__ const uint64_t *low_key(const TableView &tv, uint64_t i, int nw)
+{
+    int lo = 0, hi = KC_LOW_SEGS;  // find the last segment with seg_off <= i
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (tv.seg_off[mid] <= i) lo = mid; else hi = mid;
+    }
+    return tv.low + ((uint64_t)lo * tv.seg_cap + (i - tv.seg_off[lo])) * nw;
+}
+
+// ---- table insert / add ------------------------------------------------
+// Open addressing, linear probing.  A slot is claimed with a 64-bit CAS on
+// its first key word; counts are 64-bit atomic adds (saturation is applied
+// when results are read, so min(true count, count_max) is exact).
+template <int NW>
+__device__ __forceinline__ bool table_add(const TableView &tv, const uint64_t *key, uint64_t inc, uint32_t &n_new,
+                                          uint32_t &n_err)
+{
+    const uint64_t h = xxh3_64<NW>(key);
+    const uint64_t psize = 1ULL << tv.part_log2;
+    for (uint64_t probe = 0; probe < psize; ++probe) {
+        uint64_t *s = tv.slots + slot_index(tv, h, probe) * slot_words<NW>();
+        // A plain read settles the common case (key present: one atomic add).
+        // A stale EMPTY is harmless: the CAS below returns the real word.
+        uint64_t w0 = s[0];
+        bool owner = false;
+        if (w0 == KC_EMPTY) {
+            w0 = atomicCAS((unsigned long long *)s, (unsigned long long)KC_EMPTY, (unsigned long long)key[0]);
+            owner = w0 == KC_EMPTY;
+        }
+        if constexpr (NW == 1) {
+            if (owner || w0 == key[0]) {
+                n_new += owner;
+                add_agent(s + 1, inc);
+                return true;
+            }
+        } else {
+            // Two-word key: the CAS winner publishes word 1; a thread that
+            // finds word 0 equal waits for word 1 (bounded) and compares.
+            if (owner) __hip_atomic_store(s + 1, key[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __builtin_amdgcn_wave_barrier();
+            if (owner) {
+                ++n_new;
+                add_agent(s + 2, inc);
+                return true;
+            }
+            if (w0 == key[0]) {
+                uint64_t w1 = ld_agent(s + 1);
+                for (int spin = 0; w1 == KC_EMPTY && spin < (1 << 16); ++spin) {
+                    __builtin_amdgcn_s_sleep(1);
+                    w1 = ld_agent(s + 1);
+                }
+                if (w1 == KC_EMPTY) {
+                    ++n_err;
+                    return false;
+                }
+                if (w1 == key[1]) {
+                    add_agent(s + 2, inc);
+                    return true;
+                }
+            }
+        }
+    }
+    ++n_err;
+    return false;
+}
+
+// Lookup: returns the raw stored count (0 when absent).
+template <int NW>
+__device__ __forceinline__ uint64_t table_find(const TableView &tv, const uint64_t *key)
+{
+    const uint64_t h = xxh3_64<NW>(key);
+    const uint64_t psize = 1ULL << tv.part_log2;
+    for (uint64_t probe = 0; probe < psize; ++probe) {
+        const uint64_t *s = tv.slots + slot_index(tv, h, probe) * slot_words<NW>();
+        const uint64_t w0 = s[0];
+        if (w0 == KC_EMPTY) return 0;
+        if (w0 == key[0]) {
+            if constexpr (NW == 1) return s[1];
+            else if (s[1] == key[1]) return s[2];
+        }
+    }
+    return 0;
+}
+
+// Bloom singleton split state machine (BloomFilter.h:233-273): returns
+// 0 FIRST, 1 SECOND, 2 THIRD_PLUS.  Only atomic results decide a FIRST, so
+// each distinct key gets at most one FIRST.
+template <int NW>
+__device__ __forceinline__ int bloom_classify(const TableView &tv, const uint64_t *key)
+{
+    uint64_t blk, m;
+    bloom_probe<NW>(key, (1ULL << tv.bloom_part_log2) - 1, blk, m);
+    blk |= part_of(xxh3_64<NW>(key), tv.part_bits) << tv.bloom_part_log2;
+    uint64_t *cell = tv.bloom + 2 * blk;
+    if ((ld_agent(cell) & m) == m) return 2;
+    if ((ld_agent(cell + 1) & m) == m) {
+        const uint64_t b1 = __hip_atomic_fetch_or(cell, m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return ((b1 & m) == m) ? 2 : 1;
+    }
+    const uint64_t b2 = __hip_atomic_fetch_or(cell + 1, m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if ((b2 & m) == m) {
+        const uint64_t b1 = __hip_atomic_fetch_or(cell, m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return ((b1 & m) == m) ? 2 : 1;
+    }
+    return 0;
+}
+
+// ---- rolling canonical state (GetKmer.h:58-82) -------------------------
+template <int NW> struct Roll;
+
+template <> struct Roll<1> {
+    uint64_t f = 0, r = 0;
+    uint64_t back_mask;
+    uint32_t rev_shift;
+    __device__ Roll(uint32_t k)
+    {
+        back_mask = (k % 32) ? (((1ULL << (2 * (k % 32))) - 1ULL) << (64 - 2 * (k % 32))) : ~0ULL;
+        rev_shift = (k % 32) ? 2 * (32 - k % 32) : 0;
+    }
+    __device__ __forceinline__ void push(uint64_t code)
+    {
+        f = ((f >> 2) | (code << 62)) & back_mask;
+        r = (r << 2) | ((code ^ 3) << rev_shift);
+    }
+    __device__ __forceinline__ void canon(uint64_t *out) const { out[0] = f < r ? f : r; }
+};
+
+template <> struct Roll<2> {
+    uint64_t f0 = 0, f1 = 0, r0 = 0, r1 = 0;
+    uint64_t back_mask;
+    uint32_t rev_shift;
+    __device__ Roll(uint32_t k)
+    {
+        back_mask = (k % 32) ? (((1ULL << (2 * (k % 32))) - 1ULL) << (64 - 2 * (k % 32))) : ~0ULL;
+        rev_shift = (k % 32) ? 2 * (32 - k % 32) : 0;
+    }
+    __device__ __forceinline__ void push(uint64_t code)
+    {
+        // data[0] most significant; the tail word (data[1]) is masked.
+        f1 = ((f1 >> 2) | (f0 << 62)) & back_mask;
+        f0 = (f0 >> 2) | (code << 62);
+        r0 = (r0 << 2) | (r1 >> 62);
+        r1 = (r1 << 2) | ((code ^ 3) << rev_shift);
+    }
+    __device__ __forceinline__ void canon(uint64_t *out) const
+    {
+        const bool fwd = (f0 < r0) || (f0 == r0 && f1 < r1);
+        out[0] = fwd ? f0 : r0;
+        out[1] = fwd ? f1 : r1;
+    }
+};
+
+
+// Classifies one input byte (FastqParser.h:280-305 semantics, GetKmer.h
+// table): A/a=0 C/c=1 G/g=2 T/t/U/u=3, anything else invalid.
+__device__ __forceinline__ uint32_t base_code(uint32_t byte, bool &valid)
+{
+    const uint32_t c = byte & 0xDF;
+    const uint32_t d = c - 0x41;
+    valid = d < 32 && ((kValidMask >> d) & 1);
+    return ((c >> 1) ^ (c >> 2)) & 3;
+}
+
+// Appends the FIRST occurrences of this wave step to the block's FIRST-list
+// segment (one cursor atomic per wave step).  Every lane of the wave must
+// call it; `first` selects the lanes that append.
+template <int NW>
+__device__ __forceinline__ void first_append(const TableView &tv, bool first, const uint64_t *key, uint32_t &n_err)
+{
+    const uint64_t firsts = __ballot(first);
+    if (!firsts) return;
+    const int leader = __ffsll((unsigned long long)firsts) - 1;
+    const uint32_t seg = blockIdx.x % KC_LOW_SEGS;
+    unsigned long long b0 = 0;
+    if (__lane_id() == leader)
+        b0 = atomicAdd(tv.counters + KC_CURSOR_BASE + (uint64_t)seg * KC_LINE_WORDS, (unsigned long long)__popcll(firsts));
+    b0 = __shfl(b0, leader, 64);
+    if (first) {
+        const uint64_t at = b0 + __popcll(firsts & lanemask_lt());
+        if (at < tv.seg_cap) {
+            uint64_t *dst = tv.low + ((uint64_t)seg * tv.seg_cap + at) * NW;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) dst[w] = key[w];
+        } else {
+            ++n_err;
+        }
+    }
+}
+
+// ---- parse tile: 256 threads x 64 bytes -------------------------------
+constexpr int kParseBlock = 256;
+constexpr int kParseSeg = 64;
+constexpr int kParseTile = kParseBlock * kParseSeg;
+
+// LDS image of one tile (+ the previous segment as halo), transposed so
+// that lane t reading its own dword j touches consecutive banks.
+struct ParseLds {
+    uint32_t w[16][kParseBlock + 2];
+};
+
+// Stages tile `tile` of seq[0, n) into L (all threads; ends with a barrier).
+__device__ __forceinline__ void stage_tile(const uint8_t *__restrict__ seq, uint64_t n, uint64_t tile, ParseLds &L)
+{
+    const uint32_t t = threadIdx.x;
+    const uint64_t base = tile * kParseTile;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const uint32_t q = t + kParseBlock * r;  // uint4 index inside the tile
+        const uint64_t off = base + 16ull * q;
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (off + 16 <= n) {
+            v = *reinterpret_cast<const uint4 *>(seq + off);
+        } else if (off < n) {
+            uint32_t w[4] = {0, 0, 0, 0};
+            for (uint64_t b = off; b < n; ++b) w[(b - off) >> 2] |= (uint32_t)seq[b] << (8 * ((b - off) & 3));
+            v = make_uint4(w[0], w[1], w[2], w[3]);
+        }
+        const uint32_t seg = q >> 2, j0 = (q & 3) * 4;
+        L.w[j0 + 0][seg + 1] = v.x;
+        L.w[j0 + 1][seg + 1] = v.y;
+        L.w[j0 + 2][seg + 1] = v.z;
+        L.w[j0 + 3][seg + 1] = v.w;
+    }
+    if (t < 16) {
+        uint32_t v = 0;
+        if (base >= kParseSeg) {
+            const uint64_t off = base - kParseSeg + 4ull * t;
+            if (off + 4 <= n) v = *reinterpret_cast<const uint32_t *>(seq + off);
+        }
+        L.w[t][0] = v;
+    }
+    __syncthreads();
+}
+
+// Rolls thread t's segment of a staged tile and calls emit(key) for every
+// window whose last byte lies in the segment, at or after emit_from
+// (GetKmer.h:58-82 semantics; any invalid byte resets the window).
+template <int NW, typename F>
+__device__ __forceinline__ void roll_segment(const ParseLds &L, uint64_t tile, uint64_t n, uint64_t emit_from,
+                                             uint32_t k, F &&emit)
+{
+    constexpr int kHaloWords = NW == 1 ? 8 : 16;  // >= k-1 bytes of history
+    const uint32_t t = threadIdx.x;
+    const uint64_t start = tile * kParseTile + (uint64_t)t * kParseSeg;
+    if (start >= n) return;
+    Roll<NW> roll(k);
+    uint32_t have = 0;
+    for (int j = 16 - kHaloWords; j < 16; ++j) {
+        const uint32_t v = L.w[j][t];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            bool valid;
+            roll.push(base_code(v >> (8 * b), valid));
+            have = valid ? have + 1 : 0;
+        }
+    }
+    for (int j = 0; j < 16; ++j) {
+        const uint32_t v = L.w[j][t + 1];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            bool valid;
+            roll.push(base_code(v >> (8 * b), valid));
+            have = valid ? have + 1 : 0;
+            if (have >= k && start + 4 * j + b >= emit_from) {
+                uint64_t key[NW];
+                roll.canon(key);
+                emit(key);
+            }
+        }
+    }
+}
+
+}  // namespace kc
+
+// Partitioned counting pass (kc_part.hip).
+bool kc_part_eligible(const kc_table *t);
+void kc_part_free(kc_table *t);
+int kc_part_count(kc_table *t, const uint8_t *d_seq, uint64_t nbytes, uint64_t emit_from, const uint64_t *d_keys,
+                  uint64_t n_keys, const kc::TableView &tv);

@@ -50,12 +50,27 @@ constexpr uint64_t KC_STAT_WORDS = (uint64_t)KC_STAT_N * KC_LANES * KC_LINE_WORD
 constexpr uint64_t KC_CURSOR_BASE = KC_STAT_WORDS;
 constexpr uint64_t KC_COUNTER_WORDS = KC_CURSOR_BASE + (uint64_t)KC_LOW_SEGS * KC_LINE_WORDS;
 
+// Device scratch of the partitioned counting pass, grown on demand.
+struct kc_part_scratch {
+    uint64_t *keys_a = nullptr, *keys_b = nullptr;
+    uint64_t keys_cap_a = 0, keys_cap_b = 0;
+    uint32_t *m1 = nullptr, *m2 = nullptr;
+    uint64_t m1_cap = 0, m2_cap = 0;
+    uint64_t *o1 = nullptr, *o2 = nullptr;
+    uint64_t o1_cap = 0, o2_cap = 0;
+    uint64_t *sums = nullptr, *desc = nullptr, *coarse = nullptr, *part_off = nullptr, *starts = nullptr;
+    uint64_t sums_cap = 0, desc_cap = 0, coarse_cap = 0, part_off_cap = 0, starts_cap = 0;
+};
+
 struct kc_table {
     kc_ctx *ctx;
     kc_table_config cfg;
     uint32_t nw;
     uint32_t sealed;        // Bloom FIRST list folded into the table
     uint64_t cap;           // slots
+    uint32_t part_bits;     // log2(partitions)
+    uint32_t part_log2;     // log2(slots per partition)
+    uint32_t bloom_part_log2; // log2(Bloom blocks per partition)
     uint64_t *slots;        // cap * (nw == 1 ? 2 : 4) words
     uint64_t bloom_blocks;  // 0 when Bloom is off
     uint64_t *bloom;        // 2 * bloom_blocks words {BF1, BF2}
@@ -71,7 +86,14 @@ struct kc_table {
     uint8_t *dstage[2];
     uint64_t stage_bytes;
     hipEvent_t stage_done[2];
+    kc_part_scratch ps;
 };
 
-// Launch helpers implemented in kc_count.hip / kc_sort.hip
+// Slots / Bloom blocks of one partition that the LDS counting pass holds.
+constexpr uint32_t kc_lds_part_slots(uint32_t nw, bool bloom) { return (nw == 1 ? 4096u : 2048u) << (bloom ? 0 : 1); }
+constexpr uint32_t kc_lds_part_bloom(uint32_t nw) { return nw == 1 ? 2048u : 1024u; }
+
+// Launch helpers implemented in kc_count.hip / kc_sort.hip / kc_part.hip
 int kc_launch_fill(kc_ctx *ctx, uint64_t *p, uint64_t n_words, uint64_t value);
+uint64_t kc_scan_tmp_words(uint64_t n);
+int kc_scan_u32(kc_ctx *ctx, const uint32_t *in, uint64_t n, uint64_t *out, uint64_t *sums);

@@ -1,0 +1,488 @@
+// Partitioned counting: the MI355X form of KmerTree's prefix partitioning
+// (NewKmerTree.h:329-407, 600-614, 983-1137) feeding per-node maps
+// (NewKmerTree.h:526-570).  Where the reference moves 4 KiB blocks down a
+// 256 x 16 x 16 x 16 tree by the top bits of the key, this pass sends every
+// k-mer to one of 2^L table partitions by the top L bits of its XXH3_64 hash
+// (uniform by construction), in two exact scatter levels, and then counts
+// each partition in an LDS-resident copy of its sub-table (and Bloom part):
+//
+//   front count    parse tile -> LDS histogram of the coarse digit
+//                  (top L1 <= 8 bits) -> per-block counts matrix
+//   scan           exclusive scan: exact coarse offsets per block
+//   front scatter  parse again, write each key to its coarse region
+//   fine count / scan / scatter   the same over the remaining L2 bits inside
+//                  each coarse region (tiles never straddle a region)
+//   part count     one 1024-thread block per partition: load the sub-table
+//                  (+ Bloom part) into LDS, run the Bloom split and the
+//                  table inserts with LDS atomics, write back what changed.
+//
+// HBM traffic per k-mer: 8N bytes written and read twice by the scatter
+// levels, read once by the part pass; the random accesses all hit LDS.  The
+// FIRST list and the statistics use the same helpers as the direct path.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "kc_dev_common.h"
+
+namespace kcp {
+using namespace kc;
+
+constexpr int kKeyBlock = 256;
+constexpr int kKeysPerThread = 16;
+constexpr uint64_t kKeyTile = (uint64_t)kKeyBlock * kKeysPerThread;  // keys per scatter tile
+constexpr int kPartThreads = 1024;
+
+struct Digits {
+    uint32_t part_bits;  // L
+    uint32_t l1;         // coarse bits = min(L, 8)
+    uint32_t l2;         // fine bits = L - l1
+};
+
+__device__ __forceinline__ uint32_t coarse_of(uint64_t h, const Digits &d)
+{
+    return d.l1 ? (uint32_t)(h >> (64 - d.l1)) : 0;
+}
+
+__device__ __forceinline__ uint32_t fine_of(uint64_t h, const Digits &d)
+{
+    return d.l2 ? (uint32_t)(h >> (64 - d.part_bits)) & ((1u << d.l2) - 1) : 0;
+}
+
+// ---- front end over framed bytes ----------------------------------------
+template <int NW>
+__global__ __launch_bounds__(kParseBlock) void k_front_count_seq(const uint8_t *__restrict__ seq, uint64_t n,
+                                                                 uint64_t emit_from, uint32_t k, Digits d,
+                                                                 uint32_t *__restrict__ m1, uint64_t nblk)
+{
+    __shared__ ParseLds lds;
+    __shared__ uint32_t hist[256];
+    hist[threadIdx.x] = 0;
+    stage_tile(seq, n, blockIdx.x, lds);
+    roll_segment<NW>(lds, blockIdx.x, n, emit_from, k, [&](const uint64_t *key) {
+        atomicAdd(&hist[coarse_of(xxh3_64<NW>(key), d)], 1u);
+    });
+    __syncthreads();
+    if (threadIdx.x < (1u << d.l1)) m1[(uint64_t)threadIdx.x * nblk + blockIdx.x] = hist[threadIdx.x];
+}
+
+template <int NW>
+__global__ __launch_bounds__(kParseBlock) void k_front_scatter_seq(const uint8_t *__restrict__ seq, uint64_t n,
+                                                                   uint64_t emit_from, uint32_t k, Digits d,
+                                                                   const uint64_t *__restrict__ o1, uint64_t nblk,
+                                                                   uint64_t *__restrict__ out)
+{
+    __shared__ ParseLds lds;
+    __shared__ unsigned long long cur[256];
+    if (threadIdx.x < (1u << d.l1)) cur[threadIdx.x] = o1[(uint64_t)threadIdx.x * nblk + blockIdx.x];
+    stage_tile(seq, n, blockIdx.x, lds);
+    roll_segment<NW>(lds, blockIdx.x, n, emit_from, k, [&](const uint64_t *key) {
+        const uint64_t pos = atomicAdd(&cur[coarse_of(xxh3_64<NW>(key), d)], 1ull);
+#pragma unroll
+        for (int w = 0; w < NW; ++w) out[pos * NW + w] = key[w];
+    });
+}
+
+// ---- front end over keys (increment of key batches) ---------------------
+template <int NW>
+__global__ __launch_bounds__(kKeyBlock) void k_front_count_keys(const uint64_t *__restrict__ keys, uint64_t n,
+                                                                Digits d, uint32_t *__restrict__ m1, uint64_t nblk)
+{
+    __shared__ uint32_t hist[256];
+    hist[threadIdx.x] = 0;
+    __syncthreads();
+    for (int j = 0; j < kKeysPerThread; ++j) {
+        const uint64_t i = blockIdx.x * kKeyTile + (uint64_t)j * kKeyBlock + threadIdx.x;
+        if (i < n) atomicAdd(&hist[coarse_of(xxh3_64<NW>(keys + i * NW), d)], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < (1u << d.l1)) m1[(uint64_t)threadIdx.x * nblk + blockIdx.x] = hist[threadIdx.x];
+}
+
+template <int NW>
+__global__ __launch_bounds__(kKeyBlock) void k_front_scatter_keys(const uint64_t *__restrict__ keys, uint64_t n,
+                                                                  Digits d, const uint64_t *__restrict__ o1,
+                                                                  uint64_t nblk, uint64_t *__restrict__ out)
+{
+    __shared__ unsigned long long cur[256];
+    if (threadIdx.x < (1u << d.l1)) cur[threadIdx.x] = o1[(uint64_t)threadIdx.x * nblk + blockIdx.x];
+    __syncthreads();
+    for (int j = 0; j < kKeysPerThread; ++j) {
+        const uint64_t i = blockIdx.x * kKeyTile + (uint64_t)j * kKeyBlock + threadIdx.x;
+        if (i < n) {
+            const uint64_t pos = atomicAdd(&cur[coarse_of(xxh3_64<NW>(keys + i * NW), d)], 1ull);
+#pragma unroll
+            for (int w = 0; w < NW; ++w) out[pos * NW + w] = keys[i * NW + w];
+        }
+    }
+}
+
+// Coarse region starts: o1[c * nblk] for every coarse digit, plus the total.
+__global__ void k_coarse_starts(const uint64_t *o1, const uint32_t *m1, uint64_t nblk, uint32_t n_coarse,
+                                uint64_t *starts)
+{
+    const uint32_t c = threadIdx.x;
+    if (c < n_coarse) starts[c] = o1[(uint64_t)c * nblk];
+    if (c == 0) {
+        const uint64_t last = (uint64_t)n_coarse * nblk - 1;
+        starts[n_coarse] = o1[last] + m1[last];
+    }
+}
+
+// ---- second level: fine digit inside each coarse region -----------------
+// Tile descriptor: {key start, key count, coarse digit, tile index inside
+// its region}; per coarse region: {first matrix entry, tiles in region}.
+struct TileDesc {
+    uint64_t start;
+    uint32_t count;
+    uint32_t coarse;
+    uint32_t t_local;
+    uint32_t pad;
+};
+
+struct CoarseInfo {
+    uint64_t mbase;
+    uint64_t tiles;
+    uint64_t start;
+};
+
+template <int NW>
+__global__ __launch_bounds__(kKeyBlock) void k_fine_count(const uint64_t *__restrict__ keys, const TileDesc *tiles,
+                                                          const CoarseInfo *ci, Digits d, uint32_t *__restrict__ m2)
+{
+    __shared__ uint32_t hist[1024];
+    const TileDesc td = tiles[blockIdx.x];
+    const uint32_t nf = 1u << d.l2;
+    for (uint32_t f = threadIdx.x; f < nf; f += kKeyBlock) hist[f] = 0;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < td.count; i += kKeyBlock)
+        atomicAdd(&hist[fine_of(xxh3_64<NW>(keys + (td.start + i) * NW), d)], 1u);
+    __syncthreads();
+    const CoarseInfo c = ci[td.coarse];
+    for (uint32_t f = threadIdx.x; f < nf; f += kKeyBlock) m2[c.mbase + (uint64_t)f * c.tiles + td.t_local] = hist[f];
+}
+
+template <int NW>
+__global__ __launch_bounds__(kKeyBlock) void k_fine_scatter(const uint64_t *__restrict__ keys, const TileDesc *tiles,
+                                                            const CoarseInfo *ci, Digits d,
+                                                            const uint64_t *__restrict__ o2, uint64_t *__restrict__ out)
+{
+    __shared__ unsigned long long cur[1024];
+    const TileDesc td = tiles[blockIdx.x];
+    const uint32_t nf = 1u << d.l2;
+    const CoarseInfo c = ci[td.coarse];
+    for (uint32_t f = threadIdx.x; f < nf; f += kKeyBlock) cur[f] = o2[c.mbase + (uint64_t)f * c.tiles + td.t_local];
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < td.count; i += kKeyBlock) {
+        const uint64_t *src = keys + (td.start + i) * NW;
+        const uint64_t pos = atomicAdd(&cur[fine_of(xxh3_64<NW>(src), d)], 1ull);
+#pragma unroll
+        for (int w = 0; w < NW; ++w) out[pos * NW + w] = src[w];
+    }
+}
+
+// part p = (coarse c, fine f) starts at o2[mbase_c + f * tiles_c]
+__global__ void k_part_offsets(const uint64_t *o2, const CoarseInfo *ci, Digits d, uint64_t total,
+                               uint64_t *part_off)
+{
+    const uint64_t n_parts = 1ull << d.part_bits;
+    for (uint64_t p = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; p <= n_parts; p += (uint64_t)gridDim.x * blockDim.x) {
+        if (p == n_parts) {
+            part_off[p] = total;
+            continue;
+        }
+        const uint32_t c = (uint32_t)(p >> d.l2), f = (uint32_t)(p & ((1u << d.l2) - 1));
+        const CoarseInfo x = ci[c];
+        part_off[p] = x.tiles ? o2[x.mbase + (uint64_t)f * x.tiles] : x.start;
+    }
+}
+
+// ---- per-partition counting in LDS ---------------------------------------
+template <int NW, bool BLOOM>
+struct PartLds {
+    static constexpr uint32_t PS = kc_lds_part_slots(NW, BLOOM);
+    static constexpr uint32_t BS = BLOOM ? kc_lds_part_bloom(NW) : 1;
+    unsigned long long key[PS * NW];  // word 0 at [s], word 1 at [PS + s]
+    uint32_t cnt[PS];                 // occurrences added in this pass
+    unsigned long long bloom[2 * BS]; // {BF1, BF2} per block
+};
+
+template <int NW, bool BLOOM>
+__device__ __forceinline__ int lds_bloom(PartLds<NW, BLOOM> &S, const uint64_t *key, uint32_t bmask)
+{
+    uint64_t blk, m;
+    bloom_probe<NW>(key, bmask, blk, m);
+    unsigned long long *cell = S.bloom + 2 * blk;
+    if ((cell[0] & m) == m) return 2;
+    if ((cell[1] & m) == m) {
+        const uint64_t b1 = atomicOr(cell, (unsigned long long)m);
+        return ((b1 & m) == m) ? 2 : 1;
+    }
+    const uint64_t b2 = atomicOr(cell + 1, (unsigned long long)m);
+    if ((b2 & m) == m) {
+        const uint64_t b1 = atomicOr(cell, (unsigned long long)m);
+        return ((b1 & m) == m) ? 2 : 1;
+    }
+    return 0;
+}
+
+template <int NW, bool BLOOM>
+__device__ __forceinline__ bool lds_insert(PartLds<NW, BLOOM> &S, const uint64_t *key, uint32_t pmask,
+                                           uint32_t &n_new, uint32_t &n_err)
+{
+    constexpr uint32_t PS = PartLds<NW, BLOOM>::PS;
+    const uint64_t h = xxh3_64<NW>(key);
+    uint32_t s = (uint32_t)h & pmask;
+    for (uint32_t probe = 0; probe <= pmask; ++probe, s = (s + 1) & pmask) {
+        uint64_t w0 = __hip_atomic_load(&S.key[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        bool owner = false;
+        if (w0 == KC_EMPTY) {
+            w0 = atomicCAS(&S.key[s], (unsigned long long)KC_EMPTY, (unsigned long long)key[0]);
+            owner = w0 == KC_EMPTY;
+        }
+        if constexpr (NW == 1) {
+            if (owner || w0 == key[0]) {
+                n_new += owner;
+                atomicAdd(&S.cnt[s], 1u);
+                return true;
+            }
+        } else {
+            if (owner)
+                __hip_atomic_store(&S.key[PS + s], (unsigned long long)key[1], __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+            __builtin_amdgcn_wave_barrier();
+            if (owner) {
+                ++n_new;
+                atomicAdd(&S.cnt[s], 1u);
+                return true;
+            }
+            if (w0 == key[0]) {
+                uint64_t w1 = __hip_atomic_load(&S.key[PS + s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                for (int spin = 0; w1 == KC_EMPTY && spin < (1 << 16); ++spin) {
+                    __builtin_amdgcn_s_sleep(1);
+                    w1 = __hip_atomic_load(&S.key[PS + s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
+                if (w1 == KC_EMPTY) {
+                    ++n_err;
+                    return false;
+                }
+                if (w1 == key[1]) {
+                    atomicAdd(&S.cnt[s], 1u);
+                    return true;
+                }
+            }
+        }
+    }
+    ++n_err;
+    return false;
+}
+
+template <int NW, bool BLOOM>
+__global__ __launch_bounds__(kPartThreads) void k_count_parts(const uint64_t *__restrict__ keys,
+                                                              const uint64_t *__restrict__ part_off, TableView tv)
+{
+    constexpr uint32_t PS = PartLds<NW, BLOOM>::PS;
+    constexpr int SW = slot_words<NW>();
+    __shared__ PartLds<NW, BLOOM> S;
+    const uint64_t part = blockIdx.x;
+    const uint32_t psz = 1u << tv.part_log2;
+    const uint64_t sbase = part << tv.part_log2;
+    for (uint32_t s = threadIdx.x; s < psz; s += kPartThreads) {
+        const uint64_t *g = tv.slots + (sbase + s) * SW;
+        S.key[s] = g[0];
+        if constexpr (NW == 2) S.key[PS + s] = g[1];
+        S.cnt[s] = 0;
+    }
+    uint32_t bsz = 0;
+    uint64_t bbase = 0;
+    if constexpr (BLOOM) {
+        bsz = 1u << tv.bloom_part_log2;
+        bbase = part << tv.bloom_part_log2;
+        for (uint32_t b = threadIdx.x; b < 2 * bsz; b += kPartThreads) S.bloom[b] = tv.bloom[2 * bbase + b];
+    }
+    __syncthreads();
+
+    const uint64_t beg = part_off[part], end = part_off[part + 1];
+    uint32_t n_k = 0, n_new = 0, n_err = 0;
+    for (uint64_t base = beg; base < end; base += kPartThreads) {
+        const uint64_t i = base + threadIdx.x;
+        const bool valid = i < end;
+        uint64_t key[NW];
+#pragma unroll
+        for (int w = 0; w < NW; ++w) key[w] = valid ? keys[i * NW + w] : 0;
+        n_k += valid;
+        bool to_table = valid;
+        if constexpr (BLOOM) {
+            const bool first = valid && lds_bloom<NW, BLOOM>(S, key, bsz - 1) == 0;
+            first_append<NW>(tv, first, key, n_err);
+            to_table = valid && !first;
+        }
+        if (to_table) lds_insert<NW, BLOOM>(S, key, psz - 1, n_new, n_err);
+    }
+    __syncthreads();
+    for (uint32_t s = threadIdx.x; s < psz; s += kPartThreads) {
+        const uint32_t c = S.cnt[s];
+        if (!c) continue;
+        uint64_t *g = tv.slots + (sbase + s) * SW;
+        g[0] = S.key[s];
+        if constexpr (NW == 2) g[1] = S.key[PS + s];
+        g[NW == 1 ? 1 : 2] += c;
+    }
+    if constexpr (BLOOM) {
+        for (uint32_t b = threadIdx.x; b < 2 * bsz; b += kPartThreads) tv.bloom[2 * bbase + b] = S.bloom[b];
+    }
+    stat_add(tv.counters, KC_STAT_KMERS, n_k);
+    stat_add(tv.counters, KC_STAT_DISTINCT, n_new);
+    stat_add(tv.counters, KC_STAT_ERRORS, n_err);
+}
+
+}  // namespace kcp
+
+using namespace kcp;
+
+namespace {
+
+template <typename T>
+int ensure(T *&p, uint64_t &cap, uint64_t need)
+{
+    if (need <= cap && p) return KC_OK;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    const uint64_t n = std::max<uint64_t>(need, 1) + need / 8;
+    if (hipMalloc((void **)&p, n * sizeof(T)) != hipSuccess) return KC_ENOMEM;
+    cap = n;
+    return KC_OK;
+}
+
+}  // namespace
+
+// Whether the LDS partition pass can hold this table's partitions.
+bool kc_part_eligible(const kc_table *t)
+{
+    const bool bloom = t->bloom && !t->sealed;
+    if ((1ull << t->part_log2) > kc_lds_part_slots(t->nw, bloom)) return false;
+    if (bloom && (1ull << t->bloom_part_log2) > kc_lds_part_bloom(t->nw)) return false;
+    return t->part_bits <= 18;
+}
+
+void kc_part_free(kc_table *t)
+{
+    kc_part_scratch &s = t->ps;
+    for (void *p : {(void *)s.keys_a, (void *)s.keys_b, (void *)s.m1, (void *)s.o1, (void *)s.m2, (void *)s.o2,
+                    (void *)s.sums, (void *)s.desc, (void *)s.coarse, (void *)s.part_off, (void *)s.starts})
+        if (p) (void)hipFree(p);
+    s = kc_part_scratch();
+}
+
+// Counts the windows of d_seq (when d_keys is null) or the n_keys keys of
+// d_keys into the table through the partitioned pipeline.
+int kc_part_count(kc_table *t, const uint8_t *d_seq, uint64_t nbytes, uint64_t emit_from, const uint64_t *d_keys,
+                  uint64_t n_keys, const TableView &tv)
+{
+    kc_ctx *c = t->ctx;
+    hipStream_t st = c->stream;
+    kc_part_scratch &s = t->ps;
+    const int nw = (int)t->nw;
+    Digits d;
+    d.part_bits = t->part_bits;
+    d.l1 = std::min<uint32_t>(d.part_bits, 8);
+    d.l2 = d.part_bits - d.l1;
+    const uint32_t n_coarse = 1u << d.l1;
+    const bool seq = d_keys == nullptr;
+    const uint64_t nblk = seq ? (nbytes + kParseTile - 1) / kParseTile : (n_keys + kKeyTile - 1) / kKeyTile;
+    if (!nblk) return KC_OK;
+
+    // level 1: count, scan, coarse starts
+    const uint64_t m1n = (uint64_t)n_coarse * nblk;
+    int rc;
+    if ((rc = ensure(s.m1, s.m1_cap, m1n)) || (rc = ensure(s.o1, s.o1_cap, m1n)) ||
+        (rc = ensure(s.sums, s.sums_cap, kc_scan_tmp_words(m1n))) || (rc = ensure(s.starts, s.starts_cap, 257)))
+        return rc;
+    if (seq) {
+        if (nw == 1) hipLaunchKernelGGL(k_front_count_seq<1>, dim3(nblk), dim3(kParseBlock), 0, st, d_seq, nbytes, emit_from, t->cfg.k, d, s.m1, nblk);
+        else hipLaunchKernelGGL(k_front_count_seq<2>, dim3(nblk), dim3(kParseBlock), 0, st, d_seq, nbytes, emit_from, t->cfg.k, d, s.m1, nblk);
+    } else {
+        if (nw == 1) hipLaunchKernelGGL(k_front_count_keys<1>, dim3(nblk), dim3(kKeyBlock), 0, st, d_keys, n_keys, d, s.m1, nblk);
+        else hipLaunchKernelGGL(k_front_count_keys<2>, dim3(nblk), dim3(kKeyBlock), 0, st, d_keys, n_keys, d, s.m1, nblk);
+    }
+    KC_HIP(hipGetLastError());
+    if ((rc = kc_scan_u32(c, s.m1, m1n, s.o1, s.sums))) return rc;
+    hipLaunchKernelGGL(k_coarse_starts, dim3(1), dim3(256), 0, st, s.o1, s.m1, nblk, n_coarse, s.starts);
+    KC_HIP(hipGetLastError());
+    std::vector<uint64_t> starts(n_coarse + 1);
+    KC_HIP(hipMemcpyAsync(starts.data(), s.starts, (n_coarse + 1) * 8, hipMemcpyDeviceToHost, st));
+    KC_HIP(hipStreamSynchronize(st));
+    const uint64_t K = starts[n_coarse];
+    if (!K) return KC_OK;
+
+    // level 1 scatter
+    if ((rc = ensure(s.keys_a, s.keys_cap_a, K * nw))) return rc;
+    if (seq) {
+        if (nw == 1) hipLaunchKernelGGL(k_front_scatter_seq<1>, dim3(nblk), dim3(kParseBlock), 0, st, d_seq, nbytes, emit_from, t->cfg.k, d, s.o1, nblk, s.keys_a);
+        else hipLaunchKernelGGL(k_front_scatter_seq<2>, dim3(nblk), dim3(kParseBlock), 0, st, d_seq, nbytes, emit_from, t->cfg.k, d, s.o1, nblk, s.keys_a);
+    } else {
+        if (nw == 1) hipLaunchKernelGGL(k_front_scatter_keys<1>, dim3(nblk), dim3(kKeyBlock), 0, st, d_keys, n_keys, d, s.o1, nblk, s.keys_a);
+        else hipLaunchKernelGGL(k_front_scatter_keys<2>, dim3(nblk), dim3(kKeyBlock), 0, st, d_keys, n_keys, d, s.o1, nblk, s.keys_a);
+    }
+    KC_HIP(hipGetLastError());
+
+    const uint64_t n_parts = 1ull << d.part_bits;
+    if ((rc = ensure(s.part_off, s.part_off_cap, n_parts + 1))) return rc;
+    const uint64_t *part_keys = s.keys_a;
+    if (d.l2 == 0) {
+        // coarse regions are the partitions
+        KC_HIP(hipMemcpyAsync(s.part_off, s.starts, (n_parts + 1) * 8, hipMemcpyDeviceToDevice, st));
+    } else {
+        // level 2: tiles inside each coarse region
+        std::vector<CoarseInfo> ci(n_coarse);
+        std::vector<TileDesc> td;
+        uint64_t mbase = 0;
+        for (uint32_t cc = 0; cc < n_coarse; ++cc) {
+            const uint64_t beg = starts[cc], len = starts[cc + 1] - starts[cc];
+            const uint64_t tiles = (len + kKeyTile - 1) / kKeyTile;
+            ci[cc] = {mbase, tiles, beg};
+            for (uint64_t tl = 0; tl < tiles; ++tl) {
+                const uint64_t off = tl * kKeyTile;
+                td.push_back({beg + off, (uint32_t)std::min<uint64_t>(kKeyTile, len - off), cc, (uint32_t)tl, 0});
+            }
+            mbase += tiles << d.l2;
+        }
+        const uint64_t m2n = mbase;
+        const uint64_t desc_words = (td.size() * sizeof(TileDesc) + 7) / 8;
+        if ((rc = ensure(s.keys_b, s.keys_cap_b, K * nw)) || (rc = ensure(s.m2, s.m2_cap, m2n)) ||
+            (rc = ensure(s.o2, s.o2_cap, m2n)) || (rc = ensure(s.sums, s.sums_cap, kc_scan_tmp_words(std::max(m1n, m2n)))) ||
+            (rc = ensure(s.desc, s.desc_cap, desc_words)) ||
+            (rc = ensure(s.coarse, s.coarse_cap, (n_coarse * sizeof(CoarseInfo) + 7) / 8)))
+            return rc;
+        KC_HIP(hipMemcpyAsync(s.desc, td.data(), td.size() * sizeof(TileDesc), hipMemcpyHostToDevice, st));
+        KC_HIP(hipMemcpyAsync(s.coarse, ci.data(), ci.size() * sizeof(CoarseInfo), hipMemcpyHostToDevice, st));
+        const TileDesc *dtd = reinterpret_cast<const TileDesc *>(s.desc);
+        const CoarseInfo *dci = reinterpret_cast<const CoarseInfo *>(s.coarse);
+        const uint32_t ntiles = (uint32_t)td.size();
+        if (nw == 1) hipLaunchKernelGGL(k_fine_count<1>, dim3(ntiles), dim3(kKeyBlock), 0, st, s.keys_a, dtd, dci, d, s.m2);
+        else hipLaunchKernelGGL(k_fine_count<2>, dim3(ntiles), dim3(kKeyBlock), 0, st, s.keys_a, dtd, dci, d, s.m2);
+        KC_HIP(hipGetLastError());
+        if ((rc = kc_scan_u32(c, s.m2, m2n, s.o2, s.sums))) return rc;
+        if (nw == 1) hipLaunchKernelGGL(k_fine_scatter<1>, dim3(ntiles), dim3(kKeyBlock), 0, st, s.keys_a, dtd, dci, d, s.o2, s.keys_b);
+        else hipLaunchKernelGGL(k_fine_scatter<2>, dim3(ntiles), dim3(kKeyBlock), 0, st, s.keys_a, dtd, dci, d, s.o2, s.keys_b);
+        hipLaunchKernelGGL(k_part_offsets, dim3((uint32_t)std::min<uint64_t>((n_parts + 256) / 256, 4096)), dim3(256), 0,
+                           st, s.o2, dci, d, K, s.part_off);
+        KC_HIP(hipGetLastError());
+        part_keys = s.keys_b;
+    }
+
+    // per-partition LDS counting
+    const bool bloom = t->bloom && !t->sealed;
+    const dim3 g((uint32_t)n_parts);
+    if (nw == 1) {
+        if (bloom) hipLaunchKernelGGL((k_count_parts<1, true>), g, dim3(kPartThreads), 0, st, part_keys, s.part_off, tv);
+        else hipLaunchKernelGGL((k_count_parts<1, false>), g, dim3(kPartThreads), 0, st, part_keys, s.part_off, tv);
+    } else {
+        if (bloom) hipLaunchKernelGGL((k_count_parts<2, true>), g, dim3(kPartThreads), 0, st, part_keys, s.part_off, tv);
+        else hipLaunchKernelGGL((k_count_parts<2, false>), g, dim3(kPartThreads), 0, st, part_keys, s.part_off, tv);
+    }
+    KC_HIP(hipGetLastError());
+    return KC_OK;
+}

@@ -30,7 +30,7 @@ class TableConfig(C.Structure):
     _fields_ = [
         ("k", C.c_uint32), ("count_max", C.c_uint32), ("filter_min", C.c_uint32),
         ("filter_max", C.c_uint32), ("capacity", C.c_uint64), ("max_distinct", C.c_uint64),
-        ("bloom", C.c_uint32), ("reserved", C.c_uint32), ("bloom_blocks", C.c_uint64),
+        ("bloom", C.c_uint32), ("count_path", C.c_uint32), ("bloom_blocks", C.c_uint64),
         ("low_capacity", C.c_uint64),
     ]
 
@@ -273,10 +273,12 @@ class Table:
     """kc_table: a device-resident counting table (ConcurrentCountingHashMap<N>)."""
 
     def __init__(self, ctx: Context, k, count_max=0xFFFFFFFF, capacity=0, max_distinct=0,
-                 bloom=False, bloom_blocks=0, low_capacity=0, filter_min=0, filter_max=0xFFFFFFFF):
+                 bloom=False, bloom_blocks=0, low_capacity=0, filter_min=0, filter_max=0xFFFFFFFF,
+                 path="auto"):
+        paths = {"auto": 0, "direct": 1, "partitioned": 2}
         cfg = TableConfig(k=k, count_max=count_max, filter_min=filter_min, filter_max=filter_max,
                           capacity=capacity, max_distinct=max_distinct, bloom=1 if bloom else 0,
-                          reserved=0, bloom_blocks=bloom_blocks, low_capacity=low_capacity)
+                          count_path=paths[path], bloom_blocks=bloom_blocks, low_capacity=low_capacity)
         h = C.c_void_p()
         check(lib().kc_table_create(ctx.h, C.byref(cfg), C.byref(h)), "kc_table_create")
         self.h, self.ctx, self.k, self.nw, self.count_max = h, ctx, k, words_for_k(k), count_max

@@ -49,8 +49,8 @@ def gpu_count(ctx, seq: bytes, k: int, count_max: int, bloom=False, host_path=Fa
     return t
 
 
-def gpu_views(ctx, seq: bytes, k: int, count_max: int, bloom=False, hist_max=10000, host_path=False):
-    t = gpu_count(ctx, seq, k, count_max, bloom=bloom, host_path=host_path)
+def gpu_views(ctx, seq: bytes, k: int, count_max: int, bloom=False, hist_max=10000, host_path=False, **kw):
+    t = gpu_count(ctx, seq, k, count_max, bloom=bloom, host_path=host_path, **kw)
     st = t.stats()
     assert st["errors"] == 0, st
     high, low = t.export_device()

@@ -20,13 +20,14 @@ def _tiny_cases():
     return load_json("tiny_cases.json")
 
 
+@pytest.mark.parametrize("path", ["direct", "partitioned"])
 @pytest.mark.parametrize("bloom", [False, True], ids=["plain", "bloom"])
 @pytest.mark.parametrize("case", _tiny_cases(), ids=lambda c: "%s-k%d-c%d" % (c["input"], c["k"], c["count_max"]))
-def test_tiny_parity(ctx, case, bloom):
+def test_tiny_parity(ctx, case, bloom, path):
     from ktcountermemorygreedy_amd import kc
     seq = kc.fastq_frame(read_golden(case["input"]))
     dump, dev_hist, file_hist, st = gpu_views(ctx, seq, case["k"], case["count_max"], bloom=bloom,
-                                              hist_max=case["hist_max"])
+                                              hist_max=case["hist_max"], path=path)
     assert st["kmers"] == O.extract(seq, case["k"]).shape[0]
     assert hashlib.sha256(dump).hexdigest() == case["dump_sha256"]
     want_hist = read_golden(case["hist"])
@@ -34,13 +35,14 @@ def test_tiny_parity(ctx, case, bloom):
     assert dev_hist == want_hist
 
 
+@pytest.mark.parametrize("path", ["direct", "partitioned"])
 @pytest.mark.parametrize("mode", ["plain", "bloom", "host_staged"])
-def test_count_5mbp_parity(ctx, mode):
+def test_count_5mbp_parity(ctx, mode, path):
     from ktcountermemorygreedy_amd import workloads as W
     g = load_json("count_5mbp.json")
     seq = W.COUNT_5MBP.reads(fastq=False).tobytes()
     dump, dev_hist, file_hist, st = gpu_views(ctx, seq, 31, g["count_max"], bloom=(mode == "bloom"),
-                                              host_path=(mode == "host_staged"))
+                                              host_path=(mode == "host_staged"), path=path)
     assert st["kmers"] == g["kmers"]
     assert hashlib.sha256(dump).hexdigest() == g["dump_sha256"]
     assert hashlib.sha256(dev_hist).hexdigest() == g["hist_sha256"]
@@ -68,6 +70,43 @@ def test_uneven_reads_counts_match_oracle(ctx, k):
     absent[:, nw - 1] |= np.uint64(1)  # padding bit set: never a stored key
     if 64 * nw - 2 * k > 0:
         assert int(t.query(absent).sum()) == 0
+    t.close()
+
+
+@pytest.mark.parametrize("capacity", [1 << 12, 1 << 16, 1 << 20, 1 << 23])
+@pytest.mark.parametrize("bloom", [False, True], ids=["plain", "bloom"])
+@pytest.mark.parametrize("k", [31, 63])
+def test_partition_levels_match_oracle(ctx, capacity, bloom, k):
+    """Partitioned path with 1, <=256 (one scatter level) and >256 (two
+    levels) partitions: exact counts vs the oracle."""
+    from ktcountermemorygreedy_amd import workloads as W
+    g = W.ReadSet("g", 3, 400_000, 13, 0).genome()
+    seq = W.COUNT_5MBP.scaled(6000).reads(genome=g).tobytes()
+    nw = O.words_for_k(k)
+    keys = O.extract(seq, k)
+    uniq, cnt = O.count(keys, nw)
+    t = gpu_count(ctx, seq, k, 0xFFFFFFFF, bloom=bloom, capacity=capacity, path="partitioned")
+    st = t.stats()
+    assert st["errors"] == 0 and st["kmers"] == keys.shape[0]
+    got = t.query(uniq).astype(np.uint64)
+    assert np.array_equal(got, cnt)
+    bins = t.histogram(1, 1000)
+    want = np.bincount(np.minimum(cnt, 1000).astype(np.int64), minlength=1001)[1:]
+    assert np.array_equal(bins, want)
+    t.close()
+
+
+def test_partitioned_increment_keys(ctx):
+    from ktcountermemorygreedy_amd import kc
+    rng = np.random.default_rng(19)
+    keys = (rng.integers(0, 2**62, 200_000, dtype=np.uint64) << np.uint64(2))
+    rep = np.repeat(keys, rng.integers(1, 30, keys.size))
+    rng.shuffle(rep)
+    t = kc.Table(ctx, 31, capacity=1 << 22, path="partitioned")
+    t.increment(rep)
+    uniq, cnt = O.count(rep.reshape(-1, 1), 1)
+    assert np.array_equal(t.query(uniq).astype(np.uint64), cnt)
+    assert t.stats()["distinct"] == keys.size
     t.close()
 
 
