@@ -31,7 +31,8 @@ using namespace kc;
 
 constexpr int kKeyBlock = 256;
 constexpr int kKeysPerThread = 16;
-constexpr uint64_t kKeyTile = (uint64_t)kKeyBlock * kKeysPerThread;  // keys per scatter tile
+constexpr uint64_t kKeyTile = (uint64_t)kKeyBlock * kKeysPerThread;  // keys per front-end block
+constexpr uint64_t kFineTile = 16384;                                  // keys per fine-scatter tile
 constexpr int kPartThreads = 1024;
 
 struct Digits {
@@ -441,11 +442,11 @@ int kc_part_count(kc_table *t, const uint8_t *d_seq, uint64_t nbytes, uint64_t e
         uint64_t mbase = 0;
         for (uint32_t cc = 0; cc < n_coarse; ++cc) {
             const uint64_t beg = starts[cc], len = starts[cc + 1] - starts[cc];
-            const uint64_t tiles = (len + kKeyTile - 1) / kKeyTile;
+            const uint64_t tiles = (len + kFineTile - 1) / kFineTile;
             ci[cc] = {mbase, tiles, beg};
             for (uint64_t tl = 0; tl < tiles; ++tl) {
-                const uint64_t off = tl * kKeyTile;
-                td.push_back({beg + off, (uint32_t)std::min<uint64_t>(kKeyTile, len - off), cc, (uint32_t)tl, 0});
+                const uint64_t off = tl * kFineTile;
+                td.push_back({beg + off, (uint32_t)std::min<uint64_t>(kFineTile, len - off), cc, (uint32_t)tl, 0});
             }
             mbase += tiles << d.l2;
         }
@@ -458,6 +459,7 @@ int kc_part_count(kc_table *t, const uint8_t *d_seq, uint64_t nbytes, uint64_t e
             return rc;
         KC_HIP(hipMemcpyAsync(s.desc, td.data(), td.size() * sizeof(TileDesc), hipMemcpyHostToDevice, st));
         KC_HIP(hipMemcpyAsync(s.coarse, ci.data(), ci.size() * sizeof(CoarseInfo), hipMemcpyHostToDevice, st));
+        KC_HIP(hipStreamSynchronize(st));  // td / ci are host vectors of this scope
         const TileDesc *dtd = reinterpret_cast<const TileDesc *>(s.desc);
         const CoarseInfo *dci = reinterpret_cast<const CoarseInfo *>(s.coarse);
         const uint32_t ntiles = (uint32_t)td.size();
