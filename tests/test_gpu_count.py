@@ -73,7 +73,7 @@ def test_uneven_reads_counts_match_oracle(ctx, k):
     t.close()
 
 
-@pytest.mark.parametrize("capacity", [1 << 12, 1 << 16, 1 << 20, 1 << 23])
+@pytest.mark.parametrize("capacity", [1 << 20, 1 << 21, 1 << 23])
 @pytest.mark.parametrize("bloom", [False, True], ids=["plain", "bloom"])
 @pytest.mark.parametrize("k", [31, 63])
 def test_partition_levels_match_oracle(ctx, capacity, bloom, k):
