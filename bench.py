@@ -208,7 +208,10 @@ def main():
     # table sized for the workload: keys reaching the table (Bloom on: k-mers
     # seen at least twice, plus false positives) and the FIRST list
     est_distinct = int(ws.genome_len * 1.1 + S * ws.sub_rate * k * 1.1)
-    cap = 1 << max(20, int(np.ceil(np.log2(est_distinct * (0.5 if bloom else 1.0) * 1.6))))
+    # Bloom on: the table holds k-mers seen twice or more, about the genome's
+    # distinct k-mers; off: every distinct k-mer.  Target load <= ~50%.
+    est_table = ws.genome_len * 1.1 if bloom else est_distinct
+    cap = 1 << max(20, int(np.ceil(np.log2(est_table * 1.8))))
     table = kc.Table(ctx, k, count_max=0xFFFFFFFF, capacity=cap, max_distinct=est_distinct, bloom=bloom,
                      bloom_blocks=(1 << max(12, int(np.ceil(np.log2(est_distinct / 4))))) if bloom else 0,
                      low_capacity=est_distinct if bloom else 0)

@@ -183,6 +183,172 @@ __global__ __launch_bounds__(kKeyBlock) void k_fine_scatter(const uint64_t *__re
     }
 }
 
+
+// Exclusive scan of one value per thread of a 256-thread block.
+__device__ __forceinline__ uint32_t scan256(uint32_t v, uint32_t *tmp, uint32_t &total)
+{
+    tmp[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {
+        const uint32_t a = threadIdx.x >= (uint32_t)o ? tmp[threadIdx.x - o] : 0;
+        __syncthreads();
+        tmp[threadIdx.x] += a;
+        __syncthreads();
+    }
+    total = tmp[255];
+    const uint32_t r = tmp[threadIdx.x] - v;
+    __syncthreads();
+    return r;
+}
+
+// Front scatter with write combining (one-word keys): each thread rolls its
+// 64-byte segment in four 16-byte chunks; the block's keys of one chunk are
+// placed in LDS grouped by coarse digit and written out as contiguous runs.
+__global__ __launch_bounds__(kParseBlock) void k_front_scatter_seq_staged(const uint8_t *__restrict__ seq, uint64_t n,
+                                                                          uint64_t emit_from, uint32_t k, Digits d,
+                                                                          const uint64_t *__restrict__ o1,
+                                                                          uint64_t nblk, uint64_t *__restrict__ out)
+{
+    constexpr int kChunkBytes = 16;
+    constexpr int kStage = kParseBlock * kChunkBytes;
+    __shared__ ParseLds lds;
+    __shared__ uint64_t stage[kStage];
+    __shared__ uint8_t sbin[kStage];
+    __shared__ uint32_t hist[256], loff[256], tmp[256];
+    __shared__ uint64_t base[256];
+    const uint32_t t = threadIdx.x;
+    const uint32_t nbins = 1u << d.l1;
+    base[t] = t < nbins ? o1[(uint64_t)t * nblk + blockIdx.x] : 0;
+    hist[t] = 0;
+    stage_tile(seq, n, blockIdx.x, lds);
+
+    const uint64_t start = (uint64_t)blockIdx.x * kParseTile + (uint64_t)t * kParseSeg;
+    Roll<1> roll(k);
+    uint32_t have = 0;
+    for (int j = 8; j < 16; ++j) {  // 32-byte halo from the previous segment
+        const uint32_t v = lds.w[j][t];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            bool valid;
+            roll.push(base_code(v >> (8 * b), valid));
+            have = valid ? have + 1 : 0;
+        }
+    }
+    for (int q = 0; q < 4; ++q) {
+        uint64_t key[kChunkBytes];
+        uint32_t rank[kChunkBytes];
+        uint32_t bin[kChunkBytes];
+        uint32_t emitted = 0;
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) {
+            const uint32_t v = lds.w[q * 4 + jj][t + 1];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int e = jj * 4 + b;
+                bool valid;
+                roll.push(base_code(v >> (8 * b), valid));
+                have = valid ? have + 1 : 0;
+                const uint64_t pos = start + 4 * (q * 4 + jj) + b;
+                const bool emit = start < n && have >= k && pos >= emit_from;
+                if (emit) {
+                    roll.canon(&key[e]);
+                    bin[e] = coarse_of(xxh3_64<1>(&key[e]), d);
+                    rank[e] = atomicAdd(&hist[bin[e]], 1u);
+                    emitted |= 1u << e;
+                }
+            }
+        }
+        __syncthreads();
+        uint32_t total;
+        const uint32_t h = hist[t];
+        loff[t] = scan256(h, tmp, total);
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < kChunkBytes; ++e) {
+            if (emitted & (1u << e)) {
+                const uint32_t p = loff[bin[e]] + rank[e];
+                stage[p] = key[e];
+                sbin[p] = (uint8_t)bin[e];
+            }
+        }
+        __syncthreads();
+        for (uint32_t i = t; i < total; i += kParseBlock) {
+            const uint32_t c = sbin[i];
+            out[base[c] + (i - loff[c])] = stage[i];
+        }
+        __syncthreads();
+        base[t] += h;
+        hist[t] = 0;
+        __syncthreads();
+    }
+}
+
+// Fine scatter with write combining: a tile is handled in chunks; each
+// chunk's keys are ranked per fine digit (ranks kept in LDS), placed in LDS
+// grouped by digit and written out as contiguous runs.
+template <int NW>
+__global__ __launch_bounds__(kKeyBlock) void k_fine_scatter_staged(const uint64_t *__restrict__ keys,
+                                                                   const TileDesc *tiles, const CoarseInfo *ci,
+                                                                   Digits d, const uint64_t *__restrict__ o2,
+                                                                   uint64_t *__restrict__ out)
+{
+    constexpr uint32_t CH = NW == 1 ? 4096 : 2048;  // keys per chunk (32 KB)
+    __shared__ uint64_t stage[CH * NW];
+    __shared__ uint16_t sbin[CH];
+    __shared__ uint16_t rnk[CH];
+    __shared__ uint32_t hist[1024], loff[1024], tmp[256];
+    __shared__ uint64_t base[1024];
+    const TileDesc td = tiles[blockIdx.x];
+    const CoarseInfo c = ci[td.coarse];
+    const uint32_t nf = 1u << d.l2;
+    const uint32_t t = threadIdx.x;
+    for (uint32_t f = t; f < nf; f += kKeyBlock) {
+        base[f] = o2[c.mbase + (uint64_t)f * c.tiles + td.t_local];
+        hist[f] = 0;
+    }
+    __syncthreads();
+    for (uint32_t cs = 0; cs < td.count; cs += CH) {
+        const uint32_t cn = td.count - cs < CH ? td.count - cs : CH;
+        for (uint32_t i = t; i < cn; i += kKeyBlock) {
+            const uint32_t f = fine_of(xxh3_64<NW>(keys + (td.start + cs + i) * NW), d);
+            rnk[i] = (uint16_t)atomicAdd(&hist[f], 1u);
+            sbin[i] = (uint16_t)f;
+        }
+        __syncthreads();
+        // exclusive scan of hist[0..nf): each thread owns nf/256 consecutive digits
+        const uint32_t per = (nf + 255) / 256;
+        uint32_t acc = 0;
+        for (uint32_t f = t * per; f < (t + 1) * per && f < nf; ++f) acc += hist[f];
+        uint32_t total;
+        uint32_t run = scan256(acc, tmp, total);
+        for (uint32_t f = t * per; f < (t + 1) * per && f < nf; ++f) {
+            loff[f] = run;
+            run += hist[f];
+        }
+        __syncthreads();
+        for (uint32_t i = t; i < cn; i += kKeyBlock) {
+            const uint32_t p = loff[sbin[i]] + rnk[i];
+            const uint64_t *src = keys + (td.start + cs + i) * NW;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) stage[p * NW + w] = src[w];
+        }
+        __syncthreads();
+        // digit of staged position p: recomputed from the key
+        for (uint32_t p = t; p < cn; p += kKeyBlock) {
+            const uint32_t f = fine_of(xxh3_64<NW>(stage + p * NW), d);
+            const uint64_t g = base[f] + (p - loff[f]);
+#pragma unroll
+            for (int w = 0; w < NW; ++w) out[g * NW + w] = stage[p * NW + w];
+        }
+        __syncthreads();
+        for (uint32_t f = t; f < nf; f += kKeyBlock) {
+            base[f] += hist[f];
+            hist[f] = 0;
+        }
+        __syncthreads();
+    }
+}
+
 // part p = (coarse c, fine f) starts at o2[mbase_c + f * tiles_c]
 __global__ void k_part_offsets(const uint64_t *o2, const CoarseInfo *ci, Digits d, uint64_t total,
                                uint64_t *part_off)
@@ -421,7 +587,7 @@ int kc_part_count(kc_table *t, const uint8_t *d_seq, uint64_t nbytes, uint64_t e
     // level 1 scatter
     if ((rc = ensure(s.keys_a, s.keys_cap_a, K * nw))) return rc;
     if (seq) {
-        if (nw == 1) hipLaunchKernelGGL(k_front_scatter_seq<1>, dim3(nblk), dim3(kParseBlock), 0, st, d_seq, nbytes, emit_from, t->cfg.k, d, s.o1, nblk, s.keys_a);
+        if (nw == 1) hipLaunchKernelGGL(k_front_scatter_seq_staged, dim3(nblk), dim3(kParseBlock), 0, st, d_seq, nbytes, emit_from, t->cfg.k, d, s.o1, nblk, s.keys_a);
         else hipLaunchKernelGGL(k_front_scatter_seq<2>, dim3(nblk), dim3(kParseBlock), 0, st, d_seq, nbytes, emit_from, t->cfg.k, d, s.o1, nblk, s.keys_a);
     } else {
         if (nw == 1) hipLaunchKernelGGL(k_front_scatter_keys<1>, dim3(nblk), dim3(kKeyBlock), 0, st, d_keys, n_keys, d, s.o1, nblk, s.keys_a);
@@ -467,8 +633,8 @@ int kc_part_count(kc_table *t, const uint8_t *d_seq, uint64_t nbytes, uint64_t e
         else hipLaunchKernelGGL(k_fine_count<2>, dim3(ntiles), dim3(kKeyBlock), 0, st, s.keys_a, dtd, dci, d, s.m2);
         KC_HIP(hipGetLastError());
         if ((rc = kc_scan_u32(c, s.m2, m2n, s.o2, s.sums))) return rc;
-        if (nw == 1) hipLaunchKernelGGL(k_fine_scatter<1>, dim3(ntiles), dim3(kKeyBlock), 0, st, s.keys_a, dtd, dci, d, s.o2, s.keys_b);
-        else hipLaunchKernelGGL(k_fine_scatter<2>, dim3(ntiles), dim3(kKeyBlock), 0, st, s.keys_a, dtd, dci, d, s.o2, s.keys_b);
+        if (nw == 1) hipLaunchKernelGGL(k_fine_scatter_staged<1>, dim3(ntiles), dim3(kKeyBlock), 0, st, s.keys_a, dtd, dci, d, s.o2, s.keys_b);
+        else hipLaunchKernelGGL(k_fine_scatter_staged<2>, dim3(ntiles), dim3(kKeyBlock), 0, st, s.keys_a, dtd, dci, d, s.o2, s.keys_b);
         hipLaunchKernelGGL(k_part_offsets, dim3((uint32_t)std::min<uint64_t>((n_parts + 256) / 256, 4096)), dim3(256), 0,
                            st, s.o2, dci, d, K, s.part_off);
         KC_HIP(hipGetLastError());
