@@ -212,8 +212,8 @@ def main():
     # distinct k-mers; off: every distinct k-mer.  Target load <= ~50%.
     est_table = ws.genome_len * 1.1 if bloom else est_distinct
     cap = 1 << max(20, int(np.ceil(np.log2(est_table * 1.8))))
+    # Bloom blocks: the default, one LDS-sized Bloom part per table partition
     table = kc.Table(ctx, k, count_max=0xFFFFFFFF, capacity=cap, max_distinct=est_distinct, bloom=bloom,
-                     bloom_blocks=(1 << max(12, int(np.ceil(np.log2(est_distinct / 4))))) if bloom else 0,
                      low_capacity=est_distinct if bloom else 0)
 
     bufs = {}
