@@ -278,7 +278,7 @@ __global__ __launch_bounds__(kBlock) void k_export_write(TableView tv, PackSpec 
     for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
         uint32_t h = 0, l = 0;
         for (int j = 0; j < kExportPer; ++j) {
-            const uint64_t i = tile * kExportTile + (uint64_t)threadIdx.x * kExportPer + j;
+            const uint64_t i = tile * kExportTile + (uint64_t)j * kBlock + threadIdx.x;
             if (i >= cap) break;
             uint64_t key[NW];
             uint32_t cc;
@@ -290,7 +290,7 @@ __global__ __launch_bounds__(kBlock) void k_export_write(TableView tv, PackSpec 
         block_scan2(h, l, th, tl);
         uint64_t oh = tile_off[2 * tile] + h, ol = n_list + tile_off[2 * tile + 1] + l;
         for (int j = 0; j < kExportPer; ++j) {
-            const uint64_t i = tile * kExportTile + (uint64_t)threadIdx.x * kExportPer + j;
+            const uint64_t i = tile * kExportTile + (uint64_t)j * kBlock + threadIdx.x;
             if (i >= cap) break;
             uint64_t key[NW];
             uint32_t cc = 0;

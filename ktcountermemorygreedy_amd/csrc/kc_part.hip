@@ -283,19 +283,19 @@ __global__ __launch_bounds__(kParseBlock) void k_front_scatter_seq_staged(const 
     }
 }
 
-// Fine scatter with write combining: a tile is handled in chunks; each
-// chunk's keys are ranked per fine digit (ranks kept in LDS), placed in LDS
-// grouped by digit and written out as contiguous runs.
+// Fine scatter with write combining: a tile is handled in chunks of
+// kThreads x FPT keys held in registers; each chunk's keys are ranked per
+// fine digit, placed in LDS grouped by digit and written out as runs.
 template <int NW>
 __global__ __launch_bounds__(kKeyBlock) void k_fine_scatter_staged(const uint64_t *__restrict__ keys,
                                                                    const TileDesc *tiles, const CoarseInfo *ci,
                                                                    Digits d, const uint64_t *__restrict__ o2,
                                                                    uint64_t *__restrict__ out)
 {
-    constexpr uint32_t CH = NW == 1 ? 4096 : 2048;  // keys per chunk (32 KB)
+    constexpr uint32_t FPT = NW == 1 ? 16 : 8;          // keys per thread per chunk
+    constexpr uint32_t CH = kKeyBlock * FPT;            // 4096 / 2048 keys (32 KB)
     __shared__ uint64_t stage[CH * NW];
     __shared__ uint16_t sbin[CH];
-    __shared__ uint16_t rnk[CH];
     __shared__ uint32_t hist[1024], loff[1024], tmp[256];
     __shared__ uint64_t base[1024];
     const TileDesc td = tiles[blockIdx.x];
@@ -309,10 +309,18 @@ __global__ __launch_bounds__(kKeyBlock) void k_fine_scatter_staged(const uint64_
     __syncthreads();
     for (uint32_t cs = 0; cs < td.count; cs += CH) {
         const uint32_t cn = td.count - cs < CH ? td.count - cs : CH;
-        for (uint32_t i = t; i < cn; i += kKeyBlock) {
-            const uint32_t f = fine_of(xxh3_64<NW>(keys + (td.start + cs + i) * NW), d);
-            rnk[i] = (uint16_t)atomicAdd(&hist[f], 1u);
-            sbin[i] = (uint16_t)f;
+        uint64_t key[FPT][NW];
+        uint32_t bin[FPT], rank[FPT];
+#pragma unroll
+        for (uint32_t j = 0; j < FPT; ++j) {
+            const uint32_t i = j * kKeyBlock + t;
+            if (i < cn) {
+                const uint64_t *src = keys + (td.start + cs + i) * NW;
+#pragma unroll
+                for (int w = 0; w < NW; ++w) key[j][w] = src[w];
+                bin[j] = fine_of(xxh3_64<NW>(key[j]), d);
+                rank[j] = atomicAdd(&hist[bin[j]], 1u);
+            }
         }
         __syncthreads();
         // exclusive scan of hist[0..nf): each thread owns nf/256 consecutive digits
@@ -326,16 +334,19 @@ __global__ __launch_bounds__(kKeyBlock) void k_fine_scatter_staged(const uint64_
             run += hist[f];
         }
         __syncthreads();
-        for (uint32_t i = t; i < cn; i += kKeyBlock) {
-            const uint32_t p = loff[sbin[i]] + rnk[i];
-            const uint64_t *src = keys + (td.start + cs + i) * NW;
 #pragma unroll
-            for (int w = 0; w < NW; ++w) stage[p * NW + w] = src[w];
+        for (uint32_t j = 0; j < FPT; ++j) {
+            const uint32_t i = j * kKeyBlock + t;
+            if (i < cn) {
+                const uint32_t p = loff[bin[j]] + rank[j];
+#pragma unroll
+                for (int w = 0; w < NW; ++w) stage[p * NW + w] = key[j][w];
+                sbin[p] = (uint16_t)bin[j];
+            }
         }
         __syncthreads();
-        // digit of staged position p: recomputed from the key
         for (uint32_t p = t; p < cn; p += kKeyBlock) {
-            const uint32_t f = fine_of(xxh3_64<NW>(stage + p * NW), d);
+            const uint32_t f = sbin[p];
             const uint64_t g = base[f] + (p - loff[f]);
 #pragma unroll
             for (int w = 0; w < NW; ++w) out[g * NW + w] = stage[p * NW + w];
