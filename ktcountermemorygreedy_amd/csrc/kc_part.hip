@@ -184,21 +184,29 @@ __global__ __launch_bounds__(kKeyBlock) void k_fine_scatter(const uint64_t *__re
 }
 
 
-// Exclusive scan of one value per thread of a 256-thread block.
-__device__ __forceinline__ uint32_t scan256(uint32_t v, uint32_t *tmp, uint32_t &total)
+// Exclusive scan of one value per thread of a 256-thread block: a shuffle
+// scan inside each wave, then the four wave totals (two barriers).
+__device__ __forceinline__ uint32_t scan256(uint32_t v, uint32_t *wsum, uint32_t &total)
 {
-    tmp[threadIdx.x] = v;
-    __syncthreads();
-    for (int o = 1; o < 256; o <<= 1) {
-        const uint32_t a = threadIdx.x >= (uint32_t)o ? tmp[threadIdx.x - o] : 0;
-        __syncthreads();
-        tmp[threadIdx.x] += a;
-        __syncthreads();
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= (uint32_t)o) x += y;
     }
-    total = tmp[255];
-    const uint32_t r = tmp[threadIdx.x] - v;
+    if (lane == 63) wsum[wave] = x;
     __syncthreads();
-    return r;
+    uint32_t wbase = 0;
+    total = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < 4; ++w) {
+        const uint32_t sw = wsum[w];
+        wbase += w < wave ? sw : 0;
+        total += sw;
+    }
+    __syncthreads();
+    return wbase + x - v;
 }
 
 // Front scatter with write combining (one-word keys): each thread rolls its
@@ -214,7 +222,7 @@ __global__ __launch_bounds__(kParseBlock) void k_front_scatter_seq_staged(const 
     __shared__ ParseLds lds;
     __shared__ uint64_t stage[kStage];
     __shared__ uint8_t sbin[kStage];
-    __shared__ uint32_t hist[256], loff[256], tmp[256];
+    __shared__ uint32_t hist[256], loff[256], tmp[4];
     __shared__ uint64_t base[256];
     const uint32_t t = threadIdx.x;
     const uint32_t nbins = 1u << d.l1;
@@ -296,7 +304,7 @@ __global__ __launch_bounds__(kKeyBlock) void k_fine_scatter_staged(const uint64_
     constexpr uint32_t CH = kKeyBlock * FPT;            // 4096 / 2048 keys (32 KB)
     __shared__ uint64_t stage[CH * NW];
     __shared__ uint16_t sbin[CH];
-    __shared__ uint32_t hist[1024], loff[1024], tmp[256];
+    __shared__ uint32_t hist[1024], loff[1024], tmp[4];
     __shared__ uint64_t base[1024];
     const TileDesc td = tiles[blockIdx.x];
     const CoarseInfo c = ci[td.coarse];
