@@ -1,0 +1,66 @@
+// Exercises the C++ adaptors of include/kc/kc.hpp on the GPU: counting via
+// increment, point queries, both export sorts against a std::stable_sort on
+// the key bit range, the 12-bit sort's argument checks.  Prints "ok".
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <random>
+#include <stdexcept>
+#include <vector>
+
+#include "kc/kc.hpp"
+
+static uint64_t bits(const kc::kmer<1> &k, uint32_t lo, uint32_t hi)
+{
+    return (k.data[0] >> lo) & ((hi - lo == 64) ? ~0ULL : ((1ULL << (hi - lo)) - 1));
+}
+
+int main()
+{
+    kc::Context ctx(0);
+    std::mt19937_64 rng(12345);
+
+    // counting
+    kc::ConcurrentCountingHashMap<1> map(ctx, 31, 1 << 16);
+    std::vector<kc::kmer<1>> keys(1000);
+    for (auto &k : keys) k.data[0] = rng() & ~3ULL;
+    std::vector<kc::kmer<1>> stream;
+    for (size_t i = 0; i < keys.size(); ++i)
+        for (size_t r = 0; r <= i % 7; ++r) stream.push_back(keys[i]);
+    map.increment(stream.data(), stream.size());
+    map.increment(keys[0], 5);
+    auto got = map.find(keys.data(), keys.size());
+    for (size_t i = 0; i < keys.size(); ++i) {
+        const uint32_t want = (uint32_t)(i % 7 + 1) + (i == 0 ? 5 : 0);
+        if (got[i] != want) { std::printf("count mismatch at %zu: %u vs %u\n", i, got[i], want); return 1; }
+    }
+
+    // 12-bit sort: stable on bits [2, 44)
+    std::vector<kc::ExportRecord<1>> recs(50000), scratch;
+    for (size_t i = 0; i < recs.size(); ++i) {
+        recs[i].key.data[0] = (rng() & ~3ULL);
+        if (i % 5 == 0 && i) recs[i].key = recs[i - 1].key;
+        recs[i].count = (uint32_t)i;
+    }
+    auto want = recs;
+    std::stable_sort(want.begin(), want.end(), [](const auto &a, const auto &b) { return bits(a.key, 2, 44) < bits(b.key, 2, 44); });
+    kc::radix_sort_export_records_12bit<1, 10>(ctx, recs, scratch, 31);
+    for (size_t i = 0; i < recs.size(); ++i)
+        if (recs[i].count != want[i].count) { std::printf("12-bit sort mismatch at %zu\n", i); return 1; }
+
+    // 8-bit sort of k=31 reads bits [2, 50)
+    auto r8 = want;
+    std::shuffle(r8.begin(), r8.end(), rng);
+    auto want8 = r8;
+    std::stable_sort(want8.begin(), want8.end(), [](const auto &a, const auto &b) { return bits(a.key, 2, 50) < bits(b.key, 2, 50); });
+    std::vector<kc::ExportRecord<1>> tmp(r8.size());
+    auto *res = kc::export_record_radix_sort<1>(ctx, r8.data(), tmp.data(), r8.size(), 31);
+    for (size_t i = 0; i < r8.size(); ++i)
+        if (res[i].count != want8[i].count) { std::printf("8-bit sort mismatch at %zu\n", i); return 1; }
+
+    bool threw = false;
+    try { kc::radix_sort_export_records_12bit<1, 10>(ctx, recs, scratch, 9); } catch (const std::invalid_argument &) { threw = true; }
+    if (!threw) { std::printf("missing invalid_argument\n"); return 1; }
+    std::printf("ok\n");
+    return 0;
+}
