@@ -128,6 +128,15 @@ int kc_table_export_device(kc_table *t, uint8_t *d_high, uint64_t high_cap,
 int kc_table_histogram(kc_table *t, uint32_t min_freq, uint32_t max_freq,
                        int64_t *h_bins);
 
+/* dump_tool precise over the exported view (dump_tool.cpp:1202-1270):
+ * "<bases>\t<count>\n" lines of merged counts in [min, max], sorted
+ * lexicographically when sorted != 0 (the --sort order), built on the
+ * device.  d_out may be NULL to size; the file variant writes path. */
+int kc_table_dump_device(kc_table *t, uint32_t min_freq, uint32_t max_freq, int sorted,
+                         uint8_t *d_out, uint64_t out_cap, uint64_t *out_bytes);
+int kc_table_dump_file(kc_table *t, uint32_t min_freq, uint32_t max_freq, int sorted,
+                       const char *path);
+
 /* ---- sorts: ExportRecordRadixSort (ExportRecordRadixSort.h:256-269) and
  *      radix_sort_export_records_12bit (ExportRecordRadixSort12.h:639-646).
  * Records are {nw u64 key words, u32 count} at record_bytes stride

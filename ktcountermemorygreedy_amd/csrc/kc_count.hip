@@ -1046,3 +1046,18 @@ extern "C" int kc_bloom_probe_keys(kc_ctx *ctx, const uint64_t *d_keys, uint64_t
     KC_HIP(hipGetLastError());
     return KC_OK;
 }
+
+kc::TableView kc_table_view(kc_table *t) { return view_of(t); }
+
+// FIRST-list size (uploads the segment offsets) and whether the list is
+// active; KC_EFULL when counting overflowed.
+int kc_table_low_count(kc_table *t, uint64_t *n_low, bool *list_active)
+{
+    HostCounters hc;
+    int rc = read_counters(t, hc);
+    if (rc != KC_OK) return rc;
+    if (hc.stat[KC_STAT_ERRORS] || hc.low_overflow) return KC_EFULL;
+    *list_active = low_list_active(t);
+    *n_low = *list_active ? hc.low : 0;
+    return KC_OK;
+}

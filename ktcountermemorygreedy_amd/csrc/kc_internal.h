@@ -97,3 +97,5 @@ constexpr uint32_t kc_lds_part_bloom(uint32_t nw) { return nw == 1 ? 2048u : 102
 int kc_launch_fill(kc_ctx *ctx, uint64_t *p, uint64_t n_words, uint64_t value);
 uint64_t kc_scan_tmp_words(uint64_t n);
 int kc_scan_u32(kc_ctx *ctx, const uint32_t *in, uint64_t n, uint64_t *out, uint64_t *sums);
+int kc_sort_range(kc_ctx *ctx, uint32_t *a, uint32_t *b, uint64_t n, uint32_t rw, uint32_t lo, uint32_t hi,
+                  int *in_scratch);

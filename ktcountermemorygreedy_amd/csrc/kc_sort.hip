@@ -476,3 +476,16 @@ int kc_scan_u32(kc_ctx *ctx, const uint32_t *in, uint64_t n, uint64_t *out, uint
     KC_HIP(hipGetLastError());
     return KC_OK;
 }
+
+// Sort of rw-word records on key bits [lo, hi) (internal entry for the dump).
+int kc_sort_range(kc_ctx *ctx, uint32_t *a, uint32_t *b, uint64_t n, uint32_t rw, uint32_t lo, uint32_t hi,
+                  int *in_scratch)
+{
+    switch (rw) {
+    case 3: return sort_impl<3>(ctx, a, b, n, lo, hi, in_scratch);
+    case 4: return sort_impl<4>(ctx, a, b, n, lo, hi, in_scratch);
+    case 5: return sort_impl<5>(ctx, a, b, n, lo, hi, in_scratch);
+    case 6: return sort_impl<6>(ctx, a, b, n, lo, hi, in_scratch);
+    default: return KC_EINVAL;
+    }
+}

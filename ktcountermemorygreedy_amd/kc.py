@@ -83,6 +83,9 @@ EXPORTS = {
     "kc_table_export_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64),
                                          C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
     "kc_table_histogram": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]),
+    "kc_table_dump_device": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_int, C.c_void_p, C.c_uint64,
+                                       C.POINTER(C.c_uint64)]),
+    "kc_table_dump_file": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_int, C.c_char_p]),
     "kc_sort_records": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32,
                                   C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(C.c_int)]),
     "kc_records_histogram": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32,
@@ -348,6 +351,20 @@ class Table:
         dh, dl = self.ctx.alloc(max(hs, 16)), self.ctx.alloc(max(ls, 16))
         hb, lb = self.export_into(dh, dl)
         return dh.numpy(np.uint8, hb).tobytes(), dl.numpy(np.uint8, lb).tobytes()
+
+    def dump(self, min_freq=1, max_freq=0xFFFFFFFF, sorted=True) -> bytes:
+        """dump_tool precise text built on the device."""
+        n = C.c_uint64()
+        check(lib().kc_table_dump_device(self.h, min_freq, max_freq, 1 if sorted else 0, None, 0, C.byref(n)),
+              "kc_table_dump_device(size)")
+        buf = self.ctx.alloc(max(n.value, 16))
+        check(lib().kc_table_dump_device(self.h, min_freq, max_freq, 1 if sorted else 0, _ptr(buf), buf.nbytes,
+                                         C.byref(n)), "kc_table_dump_device")
+        return buf.numpy(np.uint8, n.value).tobytes()
+
+    def dump_file(self, path, min_freq=1, max_freq=0xFFFFFFFF, sorted=True):
+        check(lib().kc_table_dump_file(self.h, min_freq, max_freq, 1 if sorted else 0, path.encode()),
+              "kc_table_dump_file")
 
     def histogram(self, min_freq=1, max_freq=10000):
         bins = np.zeros(max_freq - min_freq + 1, dtype=np.int64)
