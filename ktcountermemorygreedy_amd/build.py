@@ -20,7 +20,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 HIP_SOURCES = ["kc_count.hip", "kc_sort.hip", "kc_part.hip", "kc_dump.hip"]
-CXX_SOURCES = ["kc_host.cpp"]
+CXX_SOURCES = ["kc_host.cpp", "kc_server.cpp"]
 
 
 def _run(cmd):

@@ -817,7 +817,8 @@ extern "C" int kc_table_count_seq_host(kc_table *t, const uint8_t *h_seq, uint64
     constexpr uint64_t kHalo = 64;  // >= k-1, multiple of 16
     hipStream_t s = t->ctx->stream;
     if (!t->pinned[0]) {
-        t->stage_bytes = kStage + kHalo;
+        // sized for this call (small jobs keep small buffers), at most kStage
+        t->stage_bytes = std::min<uint64_t>(kStage, (nbytes + 15) / 16 * 16) + kHalo;
         for (int i = 0; i < 2; ++i) {
             KC_HIP(hipHostMalloc((void **)&t->pinned[i], t->stage_bytes, hipHostMallocDefault));
             KC_HIP(hipMalloc((void **)&t->dstage[i], t->stage_bytes));
@@ -826,8 +827,9 @@ extern "C" int kc_table_count_seq_host(kc_table *t, const uint8_t *h_seq, uint64
     }
     uint64_t pos = 0;
     int buf = 0;
+    const uint64_t stage_data = t->stage_bytes - kHalo;
     while (pos < nbytes) {
-        const uint64_t chunk = std::min(kStage, nbytes - pos);
+        const uint64_t chunk = std::min(stage_data, nbytes - pos);
         const uint64_t halo = std::min(kHalo, pos);
         // the pinned buffer is reused two chunks later: wait for its copy
         KC_HIP(hipEventSynchronize(t->stage_done[buf]));

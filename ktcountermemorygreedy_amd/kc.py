@@ -53,6 +53,11 @@ class ReadSpec(C.Structure):
                 ("long_sub_rate", C.c_double), ("min_len", C.c_uint32), ("threads", C.c_uint32)]
 
 
+class RequestResult(C.Structure):
+    _fields_ = [("status", C.c_int), ("reserved", C.c_int), ("latency_ms", C.c_double),
+                ("queue_ms", C.c_double), ("kmers", C.c_uint64), ("distinct", C.c_uint64)]
+
+
 _lib = None
 
 # every symbol include/kc_api.h and include/kc_host.h declare
@@ -94,6 +99,16 @@ EXPORTS = {
     "kc_hash_keys": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_uint64, C.c_uint32, C.c_void_p]),
     "kc_bloom_probe_keys": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint64,
                                       C.c_void_p, C.c_void_p]),
+    "kc_server_create": (C.c_int, [C.c_int, C.c_uint32, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "kc_server_destroy": (None, [C.c_void_p]),
+    "kc_server_submit_count": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.c_void_p, C.c_uint64, C.c_uint64,
+                                         C.c_int, C.POINTER(C.c_uint64)]),
+    "kc_server_submit_query": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64, C.c_void_p,
+                                         C.POINTER(C.c_uint64)]),
+    "kc_server_wait": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(RequestResult)]),
+    "kc_server_drop_table": (C.c_int, [C.c_void_p, C.c_uint64]),
+    "kc_server_table_histogram": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_void_p]),
+    "kc_server_resident": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "kc_fastq_frame": (C.c_uint64, [C.c_void_p, C.c_uint64, C.c_void_p]),
     "kc_fastq_frame_file": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p), C.POINTER(C.c_uint64)]),
     "kc_host_free": (None, [C.c_void_p]),
@@ -416,3 +431,66 @@ def synth_reads(genome: np.ndarray, seed, n_reads, read_len=150, both_strands=Tr
     if wrote < 0:
         raise KCError(int(wrote), "kc_synth_reads")
     return out[:wrote]
+
+
+class Server:
+    """kc_server: count jobs and query batches against resident tables.
+
+    Host arrays handed to submit_* are kept alive here until wait()."""
+
+    def __init__(self, device=0, workers=4, hbm_budget=64 << 30):
+        h = C.c_void_p()
+        check(lib().kc_server_create(device, workers, hbm_budget, C.byref(h)), "kc_server_create")
+        self.h = h
+        self._hold = {}
+
+    def close(self):
+        if self.h:
+            lib().kc_server_destroy(self.h)
+            self.h = None
+            self._hold.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit_count(self, table_id, seq: np.ndarray, k=31, max_distinct=0, bloom=False):
+        seq = np.ascontiguousarray(seq, dtype=np.uint8)
+        rid = C.c_uint64()
+        check(lib().kc_server_submit_count(self.h, table_id, k, _ptr(seq), seq.size, max_distinct,
+                                           1 if bloom else 0, C.byref(rid)), "kc_server_submit_count")
+        self._hold[rid.value] = (seq, None)
+        return rid.value
+
+    def submit_query(self, table_id, keys: np.ndarray, nw=1):
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        n = keys.size // nw
+        out = np.zeros(max(n, 1), dtype=np.uint32)
+        rid = C.c_uint64()
+        check(lib().kc_server_submit_query(self.h, table_id, _ptr(keys), n, _ptr(out), C.byref(rid)),
+              "kc_server_submit_query")
+        self._hold[rid.value] = (keys, out[:n])
+        return rid.value
+
+    def wait(self, rid):
+        """Returns (result dict, query counts or None)."""
+        r = RequestResult()
+        check(lib().kc_server_wait(self.h, rid, C.byref(r)), "kc_server_wait")
+        _, out = self._hold.pop(rid)
+        return {f: getattr(r, f) for f, _ in RequestResult._fields_}, out
+
+    def histogram(self, table_id, min_freq=1, max_freq=10000):
+        bins = np.zeros(max_freq - min_freq + 1, dtype=np.int64)
+        check(lib().kc_server_table_histogram(self.h, table_id, min_freq, max_freq, _ptr(bins)),
+              "kc_server_table_histogram")
+        return bins
+
+    def drop(self, table_id):
+        check(lib().kc_server_drop_table(self.h, table_id), "kc_server_drop_table")
+
+    def resident(self):
+        n, b = C.c_uint64(), C.c_uint64()
+        check(lib().kc_server_resident(self.h, C.byref(n), C.byref(b)), "kc_server_resident")
+        return n.value, b.value
