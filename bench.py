@@ -37,7 +37,8 @@ def parse_args():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="repeat_100mbp",
-                    choices=["repeat_100mbp", "count_5mbp", "export_sort"])
+                    choices=["repeat_100mbp", "count_5mbp", "export_sort", "serving"])
+    ap.add_argument("--serve-seconds", type=float, default=5.0, help="serving: length of the schedule")
     ap.add_argument("--records-log2", type=int, default=30, help="export_sort: 2^N records")
     ap.add_argument("--reads", type=int, default=0, help="override the read count (smaller runs)")
     ap.add_argument("--no-bloom", action="store_true")
@@ -181,14 +182,53 @@ def bench_sort(args):
     ctx.close()
 
 
+def bench_serving(args):
+    """Serving workload: Poisson count jobs (20/s, lognormal 1e4..1e7 reads
+    from ten 20 Mbp genomes) and query batches (200/s, 1..65536 keys, half
+    present) against device-resident tables through kc_server.  The schedule
+    replays in real time; value = p99 latency of the query batches."""
+    from ktcountermemorygreedy_amd import kc
+    from ktcountermemorygreedy_amd.serving import ServingRun, ServingSpec
+
+    spec = ServingSpec(duration_s=args.serve_seconds)
+    ctx = kc.Context(0)
+    t0 = time.time()
+    run = ServingRun(ctx, spec)
+    prep_s = time.time() - t0
+    print("serving: %d jobs, %d query batches, pools and samples ready in %.1f s" %
+          (run.job_t.size, run.q_t.size, prep_s), file=sys.stderr, flush=True)
+    srv = kc.Server(0, spec.workers, spec.hbm_budget)
+    summ = run.run(srv)
+    srv.close()
+    line = {
+        "metric": "p99 query-batch latency under a Poisson job + query stream (serving)",
+        "value": summ["query_p99_ms"], "unit": "ms", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": summ["wall_s"] * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u64", "data": "synthetic",
+        "config": {"workload": "serving", "job_rate_per_s": spec.job_rate, "query_rate_per_s": spec.query_rate,
+                   "seconds": spec.duration_s, "genomes": spec.n_genomes, "genome_len": spec.genome_len,
+                   "job_reads_lognormal": [spec.reads_mu, spec.reads_sigma, spec.min_reads, spec.max_reads],
+                   "query_batch": [1, spec.max_query], "k": spec.k, "workers": spec.workers,
+                   "hbm_budget_gb": spec.hbm_budget / 1e9},
+        "serving": summ,
+        "roofline": None,
+        "cpu_baseline": None,
+        "prep_seconds": prep_s,
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
 def main():
     args = parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: every box in this setup has one MI355X; --gpus %d is not "
-              "supported (the multi-GPU design is described in DESIGN.md)" % args.gpus)
+        print("bench.py measures one GPU; --gpus %d is not supported: sharding over several GPUs "
+              "is a design, not an implementation, in this build" % args.gpus)
         sys.exit(2)
     if args.workload == "export_sort":
         return bench_sort(args)
+    if args.workload == "serving":
+        return bench_serving(args)
     import numpy as np
 
     from ktcountermemorygreedy_amd import kc

@@ -96,6 +96,11 @@ int kc_table_create(kc_ctx *ctx, const kc_table_config *cfg, kc_table **out);
 void kc_table_destroy(kc_table *t);
 /* Empties the table, Bloom filter, FIRST list and counters (device fill). */
 int kc_table_clear(kc_table *t);
+/* Frees the staging and partition scratch that counting keeps for reuse;
+ * the table stays countable and queryable.  No counterpart in the
+ * reference, whose buffers live until the process ends; resident serving
+ * tables need it. */
+int kc_table_trim(kc_table *t);
 
 /* Fused parse + canonical extraction + (Bloom split) + count over a
  * framed byte stream in device memory: sequence lines each followed by
@@ -103,7 +108,9 @@ int kc_table_clear(kc_table *t);
  * window (GetKmer.h:58-82, FastqParser.h:245-518).  Asynchronous. */
 int kc_table_count_seq(kc_table *t, const uint8_t *d_seq, uint64_t nbytes);
 /* Same from host memory: staged through pinned buffers, H2D overlapped
- * with counting; chunks cut with a k-1 byte halo (FastqReader.h:542-557). */
+ * with counting; chunks cut with a k-1 byte halo (FastqReader.h:542-557).
+ * A page-locked h_seq (kc_host_alloc_pinned) is copied from directly.
+ * Returns once h_seq may be reused; counting itself is asynchronous. */
 int kc_table_count_seq_host(kc_table *t, const uint8_t *h_seq, uint64_t nbytes);
 /* increment(key, inc) for n keys (ConcurrentCountingHashMap.h:238);
  * d_inc may be NULL for +1 each.  Keys: n*nw device words. */

@@ -7,7 +7,10 @@
  *
  * Host buffers passed to submit calls must stay valid until kc_server_wait
  * returns for that request.  Table ids are chosen by the caller and must be
- * unique among resident tables.
+ * unique among resident tables.  A query waits for the count jobs of its
+ * table that are queued or running; a query of a table that is not resident
+ * (never counted, dropped, or evicted to admit a job) fails with KC_EINVAL.
+ * Wait for every request before kc_server_destroy.
  */
 #ifndef KC_SERVER_H
 #define KC_SERVER_H
@@ -30,7 +33,9 @@ typedef struct kc_request_result {
 
 int kc_server_create(int device, uint32_t workers, uint64_t hbm_budget_bytes, kc_server **out);
 void kc_server_destroy(kc_server *s);
-/* Count job: framed sequence bytes (host) into a new resident table. */
+/* Count job: framed sequence bytes (host; page-locked memory is copied from
+ * directly) into a new resident table sized for max_distinct keys (0: one
+ * per input byte).  KC_ENOMEM at once if the job cannot fit the budget. */
 int kc_server_submit_count(kc_server *s, uint64_t table_id, uint32_t k, const uint8_t *h_seq,
                            uint64_t nbytes, uint64_t max_distinct, int bloom, uint64_t *req_id);
 /* Query batch: merged counts of n keys (n*nw host words) into h_counts. */

@@ -760,7 +760,30 @@ extern "C" void kc_table_destroy(kc_table *t)
         if (t->dstage[i]) (void)hipFree(t->dstage[i]);
         if (t->stage_done[i]) (void)hipEventDestroy(t->stage_done[i]);
     }
+    if (t->copy_done) (void)hipEventDestroy(t->copy_done);
     delete t;
+}
+
+// Staging and partition scratch are kept between counting calls for reuse;
+// a table that is only queried from now on gives them back.
+extern "C" int kc_table_trim(kc_table *t)
+{
+    if (!t) return KC_EINVAL;
+    KC_HIP(hipSetDevice(t->ctx->device));
+    KC_HIP(hipStreamSynchronize(t->ctx->stream));
+    kc_part_free(t);
+    for (int i = 0; i < 2; ++i) {
+        if (t->pinned[i]) KC_HIP(hipHostFree(t->pinned[i]));
+        if (t->dstage[i]) KC_HIP(hipFree(t->dstage[i]));
+        if (t->stage_done[i]) KC_HIP(hipEventDestroy(t->stage_done[i]));
+        t->pinned[i] = nullptr;
+        t->dstage[i] = nullptr;
+        t->stage_done[i] = nullptr;
+    }
+    if (t->copy_done) KC_HIP(hipEventDestroy(t->copy_done));
+    t->copy_done = nullptr;
+    t->stage_bytes = 0;
+    return KC_OK;
 }
 
 extern "C" int kc_table_clear(kc_table *t)
@@ -816,14 +839,31 @@ extern "C" int kc_table_count_seq_host(kc_table *t, const uint8_t *h_seq, uint64
     constexpr uint64_t kStage = 256ull << 20;
     constexpr uint64_t kHalo = 64;  // >= k-1, multiple of 16
     hipStream_t s = t->ctx->stream;
-    if (!t->pinned[0]) {
+    KC_HIP(hipSetDevice(t->ctx->device));
+    // Page-locked sources are copied to the device directly; pageable ones
+    // go through two page-locked staging buffers.
+    hipPointerAttribute_t attr{};
+    const bool direct = hipPointerGetAttributes(&attr, h_seq) == hipSuccess && attr.type == hipMemoryTypeHost;
+    (void)hipGetLastError();  // a pageable pointer leaves hipErrorInvalidValue behind
+    if (!t->dstage[0] || (!direct && !t->pinned[0])) {
         // sized for this call (small jobs keep small buffers), at most kStage
-        t->stage_bytes = std::min<uint64_t>(kStage, (nbytes + 15) / 16 * 16) + kHalo;
-        for (int i = 0; i < 2; ++i) {
-            KC_HIP(hipHostMalloc((void **)&t->pinned[i], t->stage_bytes, hipHostMallocDefault));
-            KC_HIP(hipMalloc((void **)&t->dstage[i], t->stage_bytes));
-            KC_HIP(hipEventCreateWithFlags(&t->stage_done[i], hipEventDisableTiming));
+        const uint64_t want = std::min<uint64_t>(kStage, (nbytes + 15) / 16 * 16) + kHalo;
+        if (t->dstage[0] && want > t->stage_bytes) {
+            KC_HIP(hipStreamSynchronize(s));
+            for (int i = 0; i < 2; ++i) {
+                if (t->pinned[i]) KC_HIP(hipHostFree(t->pinned[i]));
+                KC_HIP(hipFree(t->dstage[i]));
+                t->pinned[i] = t->dstage[i] = nullptr;
+            }
         }
+        if (!t->dstage[0]) t->stage_bytes = want;
+        for (int i = 0; i < 2; ++i) {
+            if (!direct && !t->pinned[i])
+                KC_HIP(hipHostMalloc((void **)&t->pinned[i], t->stage_bytes, hipHostMallocDefault));
+            if (!t->dstage[i]) KC_HIP(hipMalloc((void **)&t->dstage[i], t->stage_bytes));
+            if (!t->stage_done[i]) KC_HIP(hipEventCreateWithFlags(&t->stage_done[i], hipEventDisableTiming));
+        }
+        if (!t->copy_done) KC_HIP(hipEventCreateWithFlags(&t->copy_done, hipEventDisableTiming));
     }
     uint64_t pos = 0;
     int buf = 0;
@@ -831,16 +871,23 @@ extern "C" int kc_table_count_seq_host(kc_table *t, const uint8_t *h_seq, uint64
     while (pos < nbytes) {
         const uint64_t chunk = std::min(stage_data, nbytes - pos);
         const uint64_t halo = std::min(kHalo, pos);
-        // the pinned buffer is reused two chunks later: wait for its copy
+        // the staging buffers are reused two chunks later: wait for their use
         KC_HIP(hipEventSynchronize(t->stage_done[buf]));
-        std::memcpy(t->pinned[buf], h_seq + pos - halo, chunk + halo);
-        KC_HIP(hipMemcpyAsync(t->dstage[buf], t->pinned[buf], chunk + halo, hipMemcpyHostToDevice, s));
+        if (direct) {
+            KC_HIP(hipMemcpyAsync(t->dstage[buf], h_seq + pos - halo, chunk + halo, hipMemcpyHostToDevice, s));
+            if (pos + chunk == nbytes) KC_HIP(hipEventRecord(t->copy_done, s));
+        } else {
+            std::memcpy(t->pinned[buf], h_seq + pos - halo, chunk + halo);
+            KC_HIP(hipMemcpyAsync(t->dstage[buf], t->pinned[buf], chunk + halo, hipMemcpyHostToDevice, s));
+        }
         int rc = launch_count(t, t->dstage[buf], chunk + halo, halo);
         if (rc != KC_OK) return rc;
         KC_HIP(hipEventRecord(t->stage_done[buf], s));
         pos += chunk;
         buf ^= 1;
     }
+    // The caller may reuse h_seq on return: wait for the copies out of it.
+    if (direct) KC_HIP(hipEventSynchronize(t->copy_done));
     return KC_OK;
 }
 

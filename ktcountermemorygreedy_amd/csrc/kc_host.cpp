@@ -222,11 +222,13 @@ void write_read(const uint8_t *genome, const kc_read_spec *sp, ReadPlan p, uint6
     }
 }
 
+// Splits [0, n) into one contiguous range per thread; `grain` items at least
+// per thread.
 template <typename F>
-void parallel_for(uint64_t n, uint32_t threads, F fn)
+void parallel_for(uint64_t n, uint32_t threads, F fn, uint64_t grain = 4096)
 {
     uint32_t th = threads ? threads : std::max(1u, std::thread::hardware_concurrency());
-    th = (uint32_t)std::min<uint64_t>(th, std::max<uint64_t>(1, n / 4096));
+    th = (uint32_t)std::min<uint64_t>(th, std::max<uint64_t>(1, n / grain));
     if (th <= 1) {
         fn(0, n);
         return;
@@ -267,7 +269,7 @@ extern "C" int64_t kc_synth_reads(const uint8_t *genome, uint64_t glen, const kc
                 s += record_bytes(plan_read(sp, glen, i), i, fastq != 0);
             boff[bi + 1] = s;
         }
-    });
+    }, 1);
     for (uint64_t bi = 0; bi < nblk; ++bi) boff[bi + 1] += boff[bi];
     if (boff[nblk] > cap) return KC_EINVAL;
     parallel_for(nblk, sp->threads, [&](uint64_t a, uint64_t b) {
@@ -279,6 +281,6 @@ extern "C" int64_t kc_synth_reads(const uint8_t *genome, uint64_t glen, const kc
                 o += record_bytes(p, i, fastq != 0);
             }
         }
-    });
+    }, 1);
     return (int64_t)boff[nblk];
 }

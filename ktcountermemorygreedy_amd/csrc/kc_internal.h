@@ -86,6 +86,7 @@ struct kc_table {
     uint8_t *dstage[2];
     uint64_t stage_bytes;
     hipEvent_t stage_done[2];
+    hipEvent_t copy_done;   // last copy out of a page-locked caller buffer
     kc_part_scratch ps;
 };
 

@@ -60,7 +60,8 @@ struct Request {
     // count
     uint32_t k = 31, bloom = 0;
     const uint8_t *seq = nullptr;
-    uint64_t nbytes = 0, max_distinct = 0, footprint = 0;
+    uint64_t max_distinct = 0, footprint = 0, resident = 0;
+    uint64_t nbytes = 0;
     // query
     const uint64_t *keys = nullptr;
     uint64_t n = 0;
@@ -88,24 +89,25 @@ struct kc_server {
     std::deque<Request *> queue;
     std::unordered_map<uint64_t, std::unique_ptr<Request>> reqs;
     std::unordered_map<uint64_t, std::shared_ptr<Table>> tables;
+    std::unordered_map<uint64_t, int> pending_counts;  // queued or running count jobs per table id
     uint64_t used = 0;      // bytes of resident tables + reservations
     uint64_t running = 0;   // requests being executed
     uint64_t next_id = 1;
     bool stop = false;
 
-    // Footprint of a count job: table (16/32 B slots at the sizing below),
-    // Bloom and FIRST list when enabled, staging, partition scratch.
-    static uint64_t footprint(const Request &r, uint64_t &cap)
+    // Device bytes of a count job while it runs (footprint) and of its
+    // table afterwards (resident): slots (16/32 B at the sizing below),
+    // Bloom filter and FIRST list when enabled; while counting also the
+    // device staging and the partitioned pass's two key arrays.
+    static uint64_t footprint(const Request &r, uint64_t &cap, uint64_t &resident)
     {
-        const uint64_t slot = r.k <= 32 ? 16 : 32;
+        const uint64_t nw = r.k <= 32 ? 1 : 2;
         const uint64_t want = std::max<uint64_t>(r.max_distinct, 1024);
         cap = std::max<uint64_t>(1024, next_pow2(want + want / 2));
-        uint64_t b = cap * slot;
-        if (r.bloom) b += 16 * (cap / 2) + 8 * want * (r.k <= 32 ? 1 : 2);
-        const uint64_t stage = std::min<uint64_t>(r.nbytes + 64, 256ull << 20);
-        // pinned + device staging, and the partitioned pass's two key arrays
-        b += 2 * stage + 20 * (r.k <= 32 ? 1 : 2) * std::min<uint64_t>(r.nbytes, 256ull << 20);
-        return b;
+        resident = cap * 16 * nw + (1 << 20);
+        if (r.bloom) resident += 16 * (cap / 2) + 8 * nw * (want + want / 16);
+        const uint64_t chunk = std::min<uint64_t>(r.nbytes + 64, 256ull << 20);
+        return resident + 2 * chunk + 20 * nw * chunk;
     }
 
     // Picks the first request that can run now (queries of tables that are
@@ -115,6 +117,9 @@ struct kc_server {
         for (auto it = queue.begin(); it != queue.end(); ++it) {
             Request *r = *it;
             if (r->kind == 1) {
+                // a query waits for the count jobs of its table submitted before it
+                auto pc = pending_counts.find(r->table_id);
+                if (pc != pending_counts.end() && pc->second > 0) continue;
                 auto t = tables.find(r->table_id);
                 if (t != tables.end() && t->second->busy.load()) continue;
                 queue.erase(it);
@@ -154,8 +159,8 @@ struct kc_server {
 
     void run_count(Request *r)
     {
-        uint64_t cap = 0;
-        footprint(*r, cap);
+        uint64_t cap = 0, resident = 0;
+        footprint(*r, cap, resident);
         auto tab = std::make_shared<Table>();
         tab->id = r->table_id;
         tab->nw = r->k <= 32 ? 1 : 2;
@@ -173,6 +178,7 @@ struct kc_server {
         }
         if (rc == KC_OK) rc = kc_table_count_seq_host(tab->t, r->seq, r->nbytes);
         if (rc == KC_OK) rc = kc_ctx_sync(tab->ctx);
+        if (rc == KC_OK) rc = kc_table_trim(tab->t);
         kc_stats st{};
         if (rc == KC_OK) rc = kc_table_stats(tab->t, &st);
         if (rc == KC_OK && st.errors) rc = KC_EFULL;
@@ -182,9 +188,10 @@ struct kc_server {
         std::lock_guard<std::mutex> g(mu);
         if (rc == KC_OK && tables.count(tab->id)) rc = r->res.status = KC_EINVAL;  // id taken meanwhile
         if (rc == KC_OK) {
-            tab->bytes = r->footprint;
+            tab->bytes = resident;
             tab->last_use = Clock::now();
             tables[tab->id] = tab;
+            used -= r->footprint - resident;
         } else {
             used -= r->footprint;
         }
@@ -254,6 +261,7 @@ struct kc_server {
             {
                 std::lock_guard<std::mutex> g(mu);
                 if (busy_tab) busy_tab->busy.fetch_sub(1);
+                if (r->kind == 0 && --pending_counts[r->table_id] == 0) pending_counts.erase(r->table_id);
                 --running;
                 r->finished = true;
             }
@@ -322,8 +330,12 @@ extern "C" int kc_server_submit_count(kc_server *s, uint64_t table_id, uint32_t 
     r->nbytes = nbytes;
     r->max_distinct = max_distinct ? max_distinct : std::max<uint64_t>(nbytes, 1024);
     uint64_t cap;
-    r->footprint = kc_server::footprint(*r, cap);
+    r->footprint = kc_server::footprint(*r, cap, r->resident);
     if (r->footprint > s->budget) return KC_ENOMEM;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        ++s->pending_counts[table_id];
+    }
     *req_id = enqueue(s, std::move(r));
     return KC_OK;
 }

@@ -79,6 +79,7 @@ EXPORTS = {
     "kc_table_create": (C.c_int, [C.c_void_p, C.POINTER(TableConfig), C.POINTER(C.c_void_p)]),
     "kc_table_destroy": (None, [C.c_void_p]),
     "kc_table_clear": (C.c_int, [C.c_void_p]),
+    "kc_table_trim": (C.c_int, [C.c_void_p]),
     "kc_table_count_seq": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
     "kc_table_count_seq_host": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
     "kc_table_increment": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
@@ -211,6 +212,29 @@ class DeviceBuffer:
             pass
 
 
+class PinnedBuffer:
+    """Page-locked host memory (kc_host_alloc_pinned); .array is a uint8 view."""
+
+    def __init__(self, ctx, nbytes):
+        self.ctx, self.nbytes = ctx, int(nbytes)
+        p = C.c_void_p()
+        check(lib().kc_host_alloc_pinned(ctx.h, max(1, self.nbytes), C.byref(p)), "kc_host_alloc_pinned")
+        self.ptr = p.value
+        self.array = np.ctypeslib.as_array((C.c_uint8 * max(1, self.nbytes)).from_address(self.ptr))
+
+    def free(self):
+        if self.ptr:
+            self.array = None
+            lib().kc_host_free_pinned(self.ctx.h, C.c_void_p(self.ptr))
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 class Context:
     """kc_ctx: one device and one HIP stream."""
 
@@ -243,6 +267,9 @@ class Context:
 
     def upload(self, arr, pad=16):
         return DeviceBuffer.from_numpy(self, arr, pad)
+
+    def pinned(self, nbytes):
+        return PinnedBuffer(self, nbytes)
 
     def record(self, slot):
         check(lib().kc_event_record(self.h, slot), "kc_event_record")
