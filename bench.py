@@ -37,12 +37,14 @@ def parse_args():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="repeat_100mbp",
-                    choices=["repeat_100mbp", "count_5mbp", "export_sort", "serving"])
+                    choices=["repeat_100mbp", "count_5mbp", "uneven_50mbp", "export_sort", "serving"])
+    ap.add_argument("--k", type=int, default=31, help="k-mer length (uneven_50mbp is quoted at 21, 31 and 63)")
     ap.add_argument("--serve-seconds", type=float, default=5.0, help="serving: length of the schedule")
     ap.add_argument("--records-log2", type=int, default=30, help="export_sort: 2^N records")
     ap.add_argument("--reads", type=int, default=0, help="override the read count (smaller runs)")
     ap.add_argument("--no-bloom", action="store_true")
-    ap.add_argument("--cpu-sample-reads", type=int, default=4_000_000)
+    ap.add_argument("--cpu-sample-reads", type=int, default=0,
+                    help="reads timed on the CPU (0: 4M, or 300k for uneven_50mbp, about 10-20 s of CPU work)")
     ap.add_argument("--copy-probe-bytes", type=int, default=1 << 30,
                     help="one device-to-device copy of this size before the timed region, the "
                          "reference dispatch for scaling rocprof FETCH_SIZE/WRITE_SIZE counters")
@@ -182,13 +184,33 @@ def bench_sort(args):
     ctx.close()
 
 
+def serving_cpu_baseline(n_keys, n_batches):
+    """The reference's point lookup (FlatConcurrentHashMap find_prepared with
+    its query pool's prefetch loop) on this host: oracle/_ref/query_bench_ref."""
+    ref = os.path.join(ROOT, "oracle", "_ref", "query_bench_ref")
+    if not os.path.exists(ref):
+        return None
+    threads = max(1, min(16, os.cpu_count() or 1))
+    out = subprocess.run([ref, str(n_keys), str(n_batches), str(threads), "5"], capture_output=True, text=True,
+                         timeout=600)
+    if out.returncode != 0:
+        return {"error": out.stderr[-500:]}
+    r = json.loads(out.stdout.strip().splitlines()[-1])
+    return {"value": r["batch_p99_ms"], "unit": "ms", "cores": threads, "kind": "reference",
+            "keys_per_s": r["keys_per_s"], "batch_p50_ms": r["batch_p50_ms"],
+            "sample": "FlatConcurrentHashMap<1> of %d keys, %d batches of log-uniform size in [1, 65536] "
+                      "(half present), one batch per thread, prefetch distance 8" % (n_keys, n_batches)}
+
+
 def bench_serving(args):
     """Serving workload: Poisson count jobs (20/s, lognormal 1e4..1e7 reads
     from ten 20 Mbp genomes) and query batches (200/s, 1..65536 keys, half
     present) against device-resident tables through kc_server.  The schedule
     replays in real time; value = p99 latency of the query batches."""
+    import numpy as np
+
     from ktcountermemorygreedy_amd import kc
-    from ktcountermemorygreedy_amd.serving import ServingRun, ServingSpec
+    from ktcountermemorygreedy_amd.serving import ServingRun, ServingSpec, job_distinct_estimate
 
     spec = ServingSpec(duration_s=args.serve_seconds)
     ctx = kc.Context(0)
@@ -212,7 +234,8 @@ def bench_serving(args):
                    "hbm_budget_gb": spec.hbm_budget / 1e9},
         "serving": summ,
         "roofline": None,
-        "cpu_baseline": None,
+        "cpu_baseline": None if args.no_cpu_baseline else serving_cpu_baseline(
+            job_distinct_estimate(spec, int(np.median(run.job_reads))), 20000),
         "prep_seconds": prep_s,
     }
     print(json.dumps(line), flush=True)
@@ -234,11 +257,13 @@ def main():
     from ktcountermemorygreedy_amd import kc
     from ktcountermemorygreedy_amd import workloads as W
 
-    ws = {"repeat_100mbp": W.REPEAT_100MBP, "count_5mbp": W.COUNT_5MBP}[args.workload]
+    ws = {"repeat_100mbp": W.REPEAT_100MBP, "count_5mbp": W.COUNT_5MBP,
+          "uneven_50mbp": W.UNEVEN_50MBP}[args.workload]
     if args.reads:
         ws = ws.scaled(args.reads, ws.name)
-    k = 31
-    bloom = args.workload == "repeat_100mbp" and not args.no_bloom
+    k = args.k
+    nw = 1 if k <= 32 else 2
+    bloom = args.workload in ("repeat_100mbp", "uneven_50mbp") and not args.no_bloom
 
     t0 = time.time()
     genome = ws.genome()
@@ -249,13 +274,26 @@ def main():
     d_seq = ctx.upload(seq_host)
 
     # table sized for the workload: keys reaching the table (Bloom on: k-mers
-    # seen at least twice, plus false positives) and the FIRST list
-    est_distinct = int(ws.genome_len * 1.1 + S * ws.sub_rate * k * 1.1)
+    # seen at least twice, plus false positives) and the FIRST list.
+    # Distinct estimate: the genome's k-mers plus every window that overlaps
+    # a substitution (rate by read class: long reads are the non-150 bp ones).
+    nl = np.flatnonzero(seq_host == 10)
+    lens = np.diff(np.concatenate([[-1], nl])) - 1
+    win = np.maximum(lens - (k - 1), 0).astype(np.float64)
+    rate = np.where(lens == ws.read_len, ws.sub_rate, ws.long_sub_rate if ws.long_fraction else ws.sub_rate)
+    est_distinct = int(ws.genome_len * 1.1 + 1.1 * float(np.sum(win * (1.0 - (1.0 - rate) ** k))))
     # Bloom on: the table holds k-mers seen twice or more, about the genome's
     # distinct k-mers; off: every distinct k-mer.  Target load <= ~50%.
     est_table = ws.genome_len * 1.1 if bloom else est_distinct
     cap = 1 << max(20, int(np.ceil(np.log2(est_table * 1.8))))
-    # Bloom blocks: the default, one LDS-sized Bloom part per table partition
+    if bloom:
+        # Bloom blocks are the default, one LDS-sized Bloom part per table
+        # partition (2048 blocks per 4096 slots at k <= 32, 1024 per 2048
+        # above); keep about 6 distinct keys per 64-bit block or fewer, which
+        # for error-rich inputs means more (emptier) partitions
+        cap = max(cap, 1 << int(np.ceil(np.log2(est_distinct / 6.0 * 2.0))))
+        # the LDS partition pass takes at most 2^18 partitions
+        cap = max(min(cap, (1 << 18) * (4096 if nw == 1 else 2048)), 1 << int(np.ceil(np.log2(est_table * 1.8))))
     table = kc.Table(ctx, k, count_max=0xFFFFFFFF, capacity=cap, max_distinct=est_distinct, bloom=bloom,
                      low_capacity=est_distinct if bloom else 0)
 
@@ -301,15 +339,16 @@ def main():
         [[True], seq_host == 10]))) - 1 - (k - 1), 0))) if ws.n_rate == 0 else K
 
     k_table = K - st["low"]
-    alg_bytes = S + 24 * k_table + (32 * K if bloom else 0)
+    alg_bytes = S + 2 * (8 * nw + 4) * k_table + (32 * K if bloom else 0)
     avg_count_s = float(np.mean(count_ms)) * 1e-3
     achieved = alg_bytes / avg_count_s / 1e9
     step_s = elapsed / args.steps
     value = K / step_s
 
-    cpu = None if args.no_cpu_baseline else cpu_baseline(ws, min(args.cpu_sample_reads, ws.n_reads), k, genome)
+    sample_reads = args.cpu_sample_reads or (300_000 if args.workload == "uneven_50mbp" else 4_000_000)
+    cpu = None if args.no_cpu_baseline else cpu_baseline(ws, min(sample_reads, ws.n_reads), k, genome)
     line = {
-        "metric": "canonical 31-mers counted/s on 1 MI355X vs host CPU",
+        "metric": "canonical %d-mers counted/s on 1 MI355X vs host CPU" % k,
         "value": value, "unit": "k-mers/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": step_s * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "u64", "data": "synthetic",
