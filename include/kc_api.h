@@ -65,6 +65,18 @@ typedef struct kc_stats {
     uint64_t errors;       /* nonzero: overflow or failed wait; results invalid */
 } kc_stats;
 
+/* Launch shape of the counting pipeline and of host batches, as chosen by
+ * kc_autotune.  No reference counterpart: the reference sizes its thread
+ * pipeline from the CLI and a pre-read sample (main.cpp:248-256, 336-347).
+ * Zero fields take the built-in defaults. */
+typedef struct kc_tune {
+    uint32_t front_blocks;  /* persistent blocks of the parse / scatter front end */
+    uint32_t fine_slices;   /* blocks per coarse region in the second scatter level */
+    uint64_t batch_bytes;   /* capacity of one host batch (one staging queue slot) */
+    uint32_t queue_depth;   /* batches in flight (staging slots), 2..8 */
+    uint32_t path;          /* KC_PATH_* of batch launches; AUTO picks by batch size */
+} kc_tune;
+
 const char *kc_strerror(int status);
 
 /* ---- context ---------------------------------------------------------- */
@@ -143,6 +155,58 @@ int kc_table_dump_device(kc_table *t, uint32_t min_freq, uint32_t max_freq, int 
                          uint8_t *d_out, uint64_t out_cap, uint64_t *out_bytes);
 int kc_table_dump_file(kc_table *t, uint32_t min_freq, uint32_t max_freq, int sorted,
                        const char *path);
+
+/* ---- launch shape and batched counting.  No reference counterpart: the
+ *      reference feeds 32 KiB ring blocks with a k-1 base overlap to parser
+ *      threads (FastqReader.h:542-557, definition.h:53-54). ---- */
+
+/* Shape used by later counting calls of this table (kc_autotune's result,
+ * or the caller's).  tune->path applies when the table's count_path is
+ * AUTO. */
+int kc_table_set_tune(kc_table *t, const kc_tune *tune);
+int kc_table_get_tune(kc_table *t, kc_tune *out);
+
+/* Times kc_table_count_seq over a device-resident sample (reps timed runs
+ * per candidate after one warm-up) for each launch shape: partitioned with
+ * {2,4,8,16} persistent front blocks per CU x {2,4,8,16} fine slices, and
+ * direct; stores the fastest in the table's tune and returns it.  Every
+ * trial starts from an empty table and the table is left empty. */
+int kc_autotune(kc_table *t, const uint8_t *d_sample, uint64_t nbytes, uint32_t reps, kc_tune *best,
+                float *best_ms);
+
+/* A batch packs framed read chunks ("seq\n" streams of any length) into
+ * page-locked staging slots of tune->batch_bytes; each full slot (or
+ * kc_batch_flush) is copied to HBM on a copy stream and counted into the
+ * table by ONE launch of a captured HIP graph of the persistent pipeline,
+ * on the table's stream.  tune->queue_depth slots rotate, so packing, copy
+ * and counting of consecutive batches overlap.  A chunk not ending in '\n'
+ * gets one; a chunk larger than a slot is counted in pieces with a 64-byte
+ * halo.  tune NULL: the table's tune.  Counting is asynchronous:
+ * kc_batch_sync waits for it.  Same results as kc_table_count_seq over the
+ * concatenated chunks. */
+typedef struct kc_batch kc_batch;
+typedef struct kc_batch_info {
+    uint32_t path;            /* KC_PATH_DIRECT or KC_PATH_PARTITIONED */
+    uint32_t queue_depth;
+    uint64_t capacity;        /* bytes per batch */
+    uint32_t front_blocks;    /* persistent blocks of the captured launch */
+    uint32_t fine_slices;
+    uint64_t launches;        /* graph launches */
+    uint64_t graphs_captured;
+    uint64_t chunks;          /* chunks added */
+    uint64_t pieces;          /* slot-sized pieces of oversized chunks */
+    uint64_t bytes;           /* bytes counted */
+} kc_batch_info;
+int kc_batch_create(kc_table *t, const kc_tune *tune, kc_batch **out);
+void kc_batch_destroy(kc_batch *b);
+/* Copies the chunk into the current slot; returns when h_chunk may be reused. */
+int kc_batch_add(kc_batch *b, const uint8_t *h_chunk, uint64_t len);
+int kc_batch_flush(kc_batch *b);
+int kc_batch_sync(kc_batch *b);
+/* One graph launch over a packed batch already in HBM (16-byte aligned);
+ * graphs are cached per input pointer. */
+int kc_batch_launch_device(kc_batch *b, const uint8_t *d_seq, uint64_t nbytes);
+int kc_batch_get_info(kc_batch *b, kc_batch_info *out);
 
 /* ---- sorts: ExportRecordRadixSort (ExportRecordRadixSort.h:256-269) and
  *      radix_sort_export_records_12bit (ExportRecordRadixSort12.h:639-646).

@@ -19,7 +19,7 @@ HIP_FLAGS = [
     # packed FP32 ops stay off for all device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
-HIP_SOURCES = ["kc_count.hip", "kc_sort.hip", "kc_part.hip", "kc_dump.hip"]
+HIP_SOURCES = ["kc_count.hip", "kc_sort.hip", "kc_part.hip", "kc_dump.hip", "kc_batch.hip"]
 CXX_SOURCES = ["kc_host.cpp", "kc_server.cpp"]
 
 

@@ -41,28 +41,35 @@ extern "C" const char *kc_strerror(int s)
 namespace kc {
 
 // ---- fused count kernel ------------------------------------------------
-// One block stages kTile input bytes (+ the previous segment as halo) in
-// LDS, transposed so that lane t reading its own dword j hits consecutive
-// banks.  Thread t then rolls over halo + its 64-byte segment and handles
-// every window whose last byte lies in the segment (and at or after
-// emit_from, which excludes windows owned by an earlier chunk).
+// A block stages kTile input bytes (+ the previous segment as halo) in LDS,
+// transposed so that lane t reading its own dword j hits consecutive banks.
+// Thread t then rolls over halo + its 64-byte segment and handles every
+// window whose last byte lies in the segment (and at or after emit_from,
+// which excludes windows owned by an earlier chunk).  Block b takes tiles
+// b, b + grid, ...: one block per tile, or a persistent grid over a batch
+// whose size is read on the device.
 template <int NW, bool BLOOM>
-__global__ __launch_bounds__(kBlock) void k_count_seq(const uint8_t *__restrict__ seq, uint64_t n,
-                                                      uint64_t emit_from, TableView tv)
+__global__ __launch_bounds__(kBlock) void k_count_seq(SeqSrc src, TableView tv)
 {
     __shared__ ParseLds lds;
-    stage_tile(seq, n, blockIdx.x, lds);
+    uint64_t n, emit_from;
+    seq_bounds(src, n, emit_from);
     uint32_t n_kmers = 0, n_new = 0, n_err = 0;
-    roll_segment<NW>(lds, blockIdx.x, n, emit_from, tv.k, [&](const uint64_t *key) {
-        ++n_kmers;
-        bool to_table = true;
-        if constexpr (BLOOM) {
-            const bool first = bloom_classify<NW>(tv, key) == 0;
-            first_append<NW>(tv, first, key, n_err);
-            to_table = !first;
-        }
-        if (to_table) table_add<NW>(tv, key, 1, n_new, n_err);
-    });
+    const uint64_t ntiles = (n + kTile - 1) / kTile;
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        stage_tile(src.seq, n, tile, lds);
+        roll_segment<NW>(lds, tile, n, emit_from, tv.k, [&](const uint64_t *key) {
+            ++n_kmers;
+            bool to_table = true;
+            if constexpr (BLOOM) {
+                const bool first = bloom_classify<NW>(tv, key) == 0;
+                first_append<NW>(tv, first, key, n_err);
+                to_table = !first;
+            }
+            if (to_table) table_add<NW>(tv, key, 1, n_new, n_err);
+        });
+        __syncthreads();  // the next tile overwrites lds
+    }
     stat_add(tv.counters, KC_STAT_KMERS, n_kmers);
     stat_add(tv.counters, KC_STAT_DISTINCT, n_new);
     stat_add(tv.counters, KC_STAT_ERRORS, n_err);
@@ -800,8 +807,26 @@ constexpr uint64_t kPartMinKeys = 4ull << 20;
 
 static bool use_partitioned(const kc_table *t, uint64_t size, uint64_t min_size)
 {
-    if (t->cfg.count_path == KC_PATH_DIRECT || !kc_part_eligible(t)) return false;
-    return t->cfg.count_path == KC_PATH_PARTITIONED || size >= min_size;
+    const uint32_t p = t->cfg.count_path != KC_PATH_AUTO ? t->cfg.count_path : t->tune.path;
+    if (p == KC_PATH_DIRECT || !kc_part_eligible(t)) return false;
+    return p == KC_PATH_PARTITIONED || size >= min_size;
+}
+
+// Direct path: the fused count kernel over `src` with `grid` blocks on st.
+int kc_direct_enqueue(const kc_table *t, const SeqSrc &src, uint32_t grid, hipStream_t st)
+{
+    const TableView v = view_of(const_cast<kc_table *>(t));
+    const bool bloom = low_list_active(t);
+    const dim3 g(grid);
+    if (t->nw == 1) {
+        if (bloom) hipLaunchKernelGGL((k_count_seq<1, true>), g, dim3(kBlock), 0, st, src, v);
+        else hipLaunchKernelGGL((k_count_seq<1, false>), g, dim3(kBlock), 0, st, src, v);
+    } else {
+        if (bloom) hipLaunchKernelGGL((k_count_seq<2, true>), g, dim3(kBlock), 0, st, src, v);
+        else hipLaunchKernelGGL((k_count_seq<2, false>), g, dim3(kBlock), 0, st, src, v);
+    }
+    KC_HIP(hipGetLastError());
+    return KC_OK;
 }
 
 static int launch_count(kc_table *t, const uint8_t *d_seq, uint64_t nbytes, uint64_t emit_from)
@@ -810,17 +835,8 @@ static int launch_count(kc_table *t, const uint8_t *d_seq, uint64_t nbytes, uint
     const TableView v = view_of(t);
     if (use_partitioned(t, nbytes, kPartMinBytes)) return kc_part_count(t, d_seq, nbytes, emit_from, nullptr, 0, v);
     const uint64_t blocks = (nbytes + kTile - 1) / kTile;
-    hipStream_t s = t->ctx->stream;
-    const bool bloom = low_list_active(t);
-    if (t->nw == 1) {
-        if (bloom) hipLaunchKernelGGL((k_count_seq<1, true>), dim3(blocks), dim3(kBlock), 0, s, d_seq, nbytes, emit_from, v);
-        else hipLaunchKernelGGL((k_count_seq<1, false>), dim3(blocks), dim3(kBlock), 0, s, d_seq, nbytes, emit_from, v);
-    } else {
-        if (bloom) hipLaunchKernelGGL((k_count_seq<2, true>), dim3(blocks), dim3(kBlock), 0, s, d_seq, nbytes, emit_from, v);
-        else hipLaunchKernelGGL((k_count_seq<2, false>), dim3(blocks), dim3(kBlock), 0, s, d_seq, nbytes, emit_from, v);
-    }
-    KC_HIP(hipGetLastError());
-    return KC_OK;
+    return kc_direct_enqueue(t, SeqSrc{d_seq, nbytes, emit_from, nullptr}, (uint32_t)std::min<uint64_t>(blocks, 1u << 30),
+                             t->ctx->stream);
 }
 
 extern "C" int kc_table_count_seq(kc_table *t, const uint8_t *d_seq, uint64_t nbytes)

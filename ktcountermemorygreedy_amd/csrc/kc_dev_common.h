@@ -250,7 +250,7 @@ __device__ __forceinline__ void first_append(const TableView &tv, bool first, co
     const uint64_t firsts = __ballot(first);
     if (!firsts) return;
     const int leader = __ffsll((unsigned long long)firsts) - 1;
-    const uint32_t seg = blockIdx.x % KC_LOW_SEGS;
+    const uint32_t seg = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) % KC_LOW_SEGS;
     unsigned long long b0 = 0;
     if (__lane_id() == leader)
         b0 = atomicAdd(tv.counters + KC_CURSOR_BASE + (uint64_t)seg * KC_LINE_WORDS, (unsigned long long)__popcll(firsts));
@@ -264,6 +264,30 @@ __device__ __forceinline__ void first_append(const TableView &tv, bool first, co
         } else {
             ++n_err;
         }
+    }
+}
+
+// Framed input of one counting call.  When hdr is set, the byte count and
+// the first emitted position are read from the device words {hdr[0],
+// hdr[1]} (the count clamped to n, the capacity the launch was sized for),
+// so that one captured graph serves batches of any size; otherwise they are
+// n and emit_from.
+struct SeqSrc {
+    const uint8_t *seq;
+    uint64_t n;
+    uint64_t emit_from;
+    const uint64_t *hdr;
+};
+
+__device__ __forceinline__ void seq_bounds(const SeqSrc &s, uint64_t &n, uint64_t &emit_from)
+{
+    if (s.hdr) {
+        const uint64_t h0 = s.hdr[0];
+        n = h0 < s.n ? h0 : s.n;
+        emit_from = s.hdr[1];
+    } else {
+        n = s.n;
+        emit_from = s.emit_from;
     }
 }
 
@@ -355,5 +379,15 @@ __device__ __forceinline__ void roll_segment(const ParseLds &L, uint64_t tile, u
 // Partitioned counting pass (kc_part.hip).
 bool kc_part_eligible(const kc_table *t);
 void kc_part_free(kc_table *t);
+void kc_part_scratch_free(kc_part_scratch &s);
+kc_pipe_shape kc_part_shape(const kc_table *t, const kc_tune &tune, uint64_t key_cap);
+int kc_part_reserve(const kc_table *t, kc_part_scratch &s, const kc_pipe_shape &sh);
+// Enqueues the whole pass on st without host synchronisation: over the
+// framed input `seq` (when d_keys is null) or the n_keys keys of d_keys.
+int kc_part_enqueue(const kc_table *t, kc_part_scratch &s, const kc_pipe_shape &sh, const kc::SeqSrc &seq,
+                    const uint64_t *d_keys, uint64_t n_keys, const kc::TableView &tv, hipStream_t st);
 int kc_part_count(kc_table *t, const uint8_t *d_seq, uint64_t nbytes, uint64_t emit_from, const uint64_t *d_keys,
                   uint64_t n_keys, const kc::TableView &tv);
+// Direct path (kc_count.hip): the fused count kernel with `grid` blocks.
+int kc_direct_enqueue(const kc_table *t, const kc::SeqSrc &src, uint32_t grid, hipStream_t st);
+kc::TableView kc_table_view(kc_table *t);

@@ -35,8 +35,9 @@ constexpr uint64_t KC_EMPTY = ~0ULL;
 // Device counters.  Statistics are spread over KC_LANES lanes, one 128-byte
 // line each, so that blocks update different lines (block b uses lane
 // b % KC_LANES); readers sum the lanes.  The FIRST-occurrence list is cut
-// into KC_LOW_SEGS segments with one cursor line each (block b appends to
-// segment b % KC_LOW_SEGS), for the same reason.
+// into KC_LOW_SEGS segments with one cursor line each (wave w of block b
+// appends to segment (b * waves_per_block + w) % KC_LOW_SEGS), for the same
+// reason.
 enum kc_stat_idx : int {
     KC_STAT_KMERS = 0,
     KC_STAT_DISTINCT = 1,
@@ -50,16 +51,24 @@ constexpr uint64_t KC_STAT_WORDS = (uint64_t)KC_STAT_N * KC_LANES * KC_LINE_WORD
 constexpr uint64_t KC_CURSOR_BASE = KC_STAT_WORDS;
 constexpr uint64_t KC_COUNTER_WORDS = KC_CURSOR_BASE + (uint64_t)KC_LOW_SEGS * KC_LINE_WORDS;
 
-// Device scratch of the partitioned counting pass, grown on demand.
+// Launch shape of one partitioned counting pipeline.  Every size is fixed
+// before the first kernel runs, so the pass is enqueued without host
+// synchronisation (and can be captured in a graph): the key arrays hold
+// key_cap keys, an upper bound on the windows of the input.
+struct kc_pipe_shape {
+    uint32_t front_blocks;  // persistent blocks of the parse / scatter front end
+    uint32_t fine_slices;   // blocks per coarse region in the second scatter level
+    uint64_t key_cap;
+};
+
+// Device scratch of the partitioned counting pass, sized for a shape.
 struct kc_part_scratch {
-    uint64_t *keys_a = nullptr, *keys_b = nullptr;
-    uint64_t keys_cap_a = 0, keys_cap_b = 0;
-    uint32_t *m1 = nullptr, *m2 = nullptr;
-    uint64_t m1_cap = 0, m2_cap = 0;
-    uint64_t *o1 = nullptr, *o2 = nullptr;
-    uint64_t o1_cap = 0, o2_cap = 0;
-    uint64_t *sums = nullptr, *desc = nullptr, *coarse = nullptr, *part_off = nullptr, *starts = nullptr;
-    uint64_t sums_cap = 0, desc_cap = 0, coarse_cap = 0, part_off_cap = 0, starts_cap = 0;
+    uint64_t *keys_a = nullptr, *keys_b = nullptr;  // key_cap * nw words each
+    uint32_t *m1 = nullptr, *m2 = nullptr;          // digit counts per block / slice
+    uint64_t *o1 = nullptr, *o2 = nullptr;          // their exclusive offsets
+    uint64_t *sums = nullptr, *starts = nullptr, *part_off = nullptr;
+    uint64_t key_cap = 0;
+    uint64_t m1_n = 0, m2_n = 0, sums_n = 0, part_n = 0;
 };
 
 struct kc_table {
@@ -88,6 +97,7 @@ struct kc_table {
     hipEvent_t stage_done[2];
     hipEvent_t copy_done;   // last copy out of a page-locked caller buffer
     kc_part_scratch ps;
+    kc_tune tune;           // launch shape of counting calls (zeros: defaults)
 };
 
 // Slots / Bloom blocks of one partition that the LDS counting pass holds.
@@ -98,5 +108,6 @@ constexpr uint32_t kc_lds_part_bloom(uint32_t nw) { return nw == 1 ? 2048u : 102
 int kc_launch_fill(kc_ctx *ctx, uint64_t *p, uint64_t n_words, uint64_t value);
 uint64_t kc_scan_tmp_words(uint64_t n);
 int kc_scan_u32(kc_ctx *ctx, const uint32_t *in, uint64_t n, uint64_t *out, uint64_t *sums);
+int kc_scan_u32_on(hipStream_t st, const uint32_t *in, uint64_t n, uint64_t *out, uint64_t *sums);
 int kc_sort_range(kc_ctx *ctx, uint32_t *a, uint32_t *b, uint64_t n, uint32_t rw, uint32_t lo, uint32_t hi,
                   int *in_scratch);

@@ -6,15 +6,21 @@
 // (uniform by construction), in two exact scatter levels, and then counts
 // each partition in an LDS-resident copy of its sub-table (and Bloom part):
 //
-//   front count    parse tile -> LDS histogram of the coarse digit
-//                  (top L1 <= 8 bits) -> per-block counts matrix
+//   front count    G persistent blocks; block b parses tiles b, b+G, ... and
+//                  builds an LDS histogram of the coarse digit (top L1 <= 8
+//                  bits) -> counts matrix [coarse][block]
 //   scan           exclusive scan: exact coarse offsets per block
-//   front scatter  parse again, write each key to its coarse region
-//   fine count / scan / scatter   the same over the remaining L2 bits inside
-//                  each coarse region (tiles never straddle a region)
+//   front scatter  the same blocks parse the same tiles again and write each
+//                  key to its coarse region (write-combined through LDS)
+//   fine count / scan / scatter   S blocks per coarse region, each over one
+//                  contiguous slice of the region, on the remaining L2 bits
 //   part count     one 1024-thread block per partition: load the sub-table
 //                  (+ Bloom part) into LDS, run the Bloom split and the
 //                  table inserts with LDS atomics, write back what changed.
+//
+// Every grid and buffer size is fixed by the launch shape before the first
+// kernel runs (the input size is read on the device), so the pass is
+// enqueued without a host round trip and can be captured in a graph.
 //
 // HBM traffic per k-mer: 8N bytes written and read twice by the scatter
 // levels, read once by the part pass; the random accesses all hit LDS.  The
@@ -31,8 +37,7 @@ using namespace kc;
 
 constexpr int kKeyBlock = 256;
 constexpr int kKeysPerThread = 16;
-constexpr uint64_t kKeyTile = (uint64_t)kKeyBlock * kKeysPerThread;  // keys per front-end block
-constexpr uint64_t kFineTile = 16384;                                  // keys per fine-scatter tile
+constexpr uint64_t kKeyTile = (uint64_t)kKeyBlock * kKeysPerThread;  // keys per front-end tile
 constexpr int kPartThreads = 1024;
 
 struct Digits {
@@ -50,139 +55,6 @@ __device__ __forceinline__ uint32_t fine_of(uint64_t h, const Digits &d)
 {
     return d.l2 ? (uint32_t)(h >> (64 - d.part_bits)) & ((1u << d.l2) - 1) : 0;
 }
-
-// ---- front end over framed bytes ----------------------------------------
-template <int NW>
-__global__ __launch_bounds__(kParseBlock) void k_front_count_seq(const uint8_t *__restrict__ seq, uint64_t n,
-                                                                 uint64_t emit_from, uint32_t k, Digits d,
-                                                                 uint32_t *__restrict__ m1, uint64_t nblk)
-{
-    __shared__ ParseLds lds;
-    __shared__ uint32_t hist[256];
-    hist[threadIdx.x] = 0;
-    stage_tile(seq, n, blockIdx.x, lds);
-    roll_segment<NW>(lds, blockIdx.x, n, emit_from, k, [&](const uint64_t *key) {
-        atomicAdd(&hist[coarse_of(xxh3_64<NW>(key), d)], 1u);
-    });
-    __syncthreads();
-    if (threadIdx.x < (1u << d.l1)) m1[(uint64_t)threadIdx.x * nblk + blockIdx.x] = hist[threadIdx.x];
-}
-
-template <int NW>
-__global__ __launch_bounds__(kParseBlock) void k_front_scatter_seq(const uint8_t *__restrict__ seq, uint64_t n,
-                                                                   uint64_t emit_from, uint32_t k, Digits d,
-                                                                   const uint64_t *__restrict__ o1, uint64_t nblk,
-                                                                   uint64_t *__restrict__ out)
-{
-    __shared__ ParseLds lds;
-    __shared__ unsigned long long cur[256];
-    if (threadIdx.x < (1u << d.l1)) cur[threadIdx.x] = o1[(uint64_t)threadIdx.x * nblk + blockIdx.x];
-    stage_tile(seq, n, blockIdx.x, lds);
-    roll_segment<NW>(lds, blockIdx.x, n, emit_from, k, [&](const uint64_t *key) {
-        const uint64_t pos = atomicAdd(&cur[coarse_of(xxh3_64<NW>(key), d)], 1ull);
-#pragma unroll
-        for (int w = 0; w < NW; ++w) out[pos * NW + w] = key[w];
-    });
-}
-
-// ---- front end over keys (increment of key batches) ---------------------
-template <int NW>
-__global__ __launch_bounds__(kKeyBlock) void k_front_count_keys(const uint64_t *__restrict__ keys, uint64_t n,
-                                                                Digits d, uint32_t *__restrict__ m1, uint64_t nblk)
-{
-    __shared__ uint32_t hist[256];
-    hist[threadIdx.x] = 0;
-    __syncthreads();
-    for (int j = 0; j < kKeysPerThread; ++j) {
-        const uint64_t i = blockIdx.x * kKeyTile + (uint64_t)j * kKeyBlock + threadIdx.x;
-        if (i < n) atomicAdd(&hist[coarse_of(xxh3_64<NW>(keys + i * NW), d)], 1u);
-    }
-    __syncthreads();
-    if (threadIdx.x < (1u << d.l1)) m1[(uint64_t)threadIdx.x * nblk + blockIdx.x] = hist[threadIdx.x];
-}
-
-template <int NW>
-__global__ __launch_bounds__(kKeyBlock) void k_front_scatter_keys(const uint64_t *__restrict__ keys, uint64_t n,
-                                                                  Digits d, const uint64_t *__restrict__ o1,
-                                                                  uint64_t nblk, uint64_t *__restrict__ out)
-{
-    __shared__ unsigned long long cur[256];
-    if (threadIdx.x < (1u << d.l1)) cur[threadIdx.x] = o1[(uint64_t)threadIdx.x * nblk + blockIdx.x];
-    __syncthreads();
-    for (int j = 0; j < kKeysPerThread; ++j) {
-        const uint64_t i = blockIdx.x * kKeyTile + (uint64_t)j * kKeyBlock + threadIdx.x;
-        if (i < n) {
-            const uint64_t pos = atomicAdd(&cur[coarse_of(xxh3_64<NW>(keys + i * NW), d)], 1ull);
-#pragma unroll
-            for (int w = 0; w < NW; ++w) out[pos * NW + w] = keys[i * NW + w];
-        }
-    }
-}
-
-// Coarse region starts: o1[c * nblk] for every coarse digit, plus the total.
-__global__ void k_coarse_starts(const uint64_t *o1, const uint32_t *m1, uint64_t nblk, uint32_t n_coarse,
-                                uint64_t *starts)
-{
-    const uint32_t c = threadIdx.x;
-    if (c < n_coarse) starts[c] = o1[(uint64_t)c * nblk];
-    if (c == 0) {
-        const uint64_t last = (uint64_t)n_coarse * nblk - 1;
-        starts[n_coarse] = o1[last] + m1[last];
-    }
-}
-
-// ---- second level: fine digit inside each coarse region -----------------
-// Tile descriptor: {key start, key count, coarse digit, tile index inside
-// its region}; per coarse region: {first matrix entry, tiles in region}.
-struct TileDesc {
-    uint64_t start;
-    uint32_t count;
-    uint32_t coarse;
-    uint32_t t_local;
-    uint32_t pad;
-};
-
-struct CoarseInfo {
-    uint64_t mbase;
-    uint64_t tiles;
-    uint64_t start;
-};
-
-template <int NW>
-__global__ __launch_bounds__(kKeyBlock) void k_fine_count(const uint64_t *__restrict__ keys, const TileDesc *tiles,
-                                                          const CoarseInfo *ci, Digits d, uint32_t *__restrict__ m2)
-{
-    __shared__ uint32_t hist[1024];
-    const TileDesc td = tiles[blockIdx.x];
-    const uint32_t nf = 1u << d.l2;
-    for (uint32_t f = threadIdx.x; f < nf; f += kKeyBlock) hist[f] = 0;
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < td.count; i += kKeyBlock)
-        atomicAdd(&hist[fine_of(xxh3_64<NW>(keys + (td.start + i) * NW), d)], 1u);
-    __syncthreads();
-    const CoarseInfo c = ci[td.coarse];
-    for (uint32_t f = threadIdx.x; f < nf; f += kKeyBlock) m2[c.mbase + (uint64_t)f * c.tiles + td.t_local] = hist[f];
-}
-
-template <int NW>
-__global__ __launch_bounds__(kKeyBlock) void k_fine_scatter(const uint64_t *__restrict__ keys, const TileDesc *tiles,
-                                                            const CoarseInfo *ci, Digits d,
-                                                            const uint64_t *__restrict__ o2, uint64_t *__restrict__ out)
-{
-    __shared__ unsigned long long cur[1024];
-    const TileDesc td = tiles[blockIdx.x];
-    const uint32_t nf = 1u << d.l2;
-    const CoarseInfo c = ci[td.coarse];
-    for (uint32_t f = threadIdx.x; f < nf; f += kKeyBlock) cur[f] = o2[c.mbase + (uint64_t)f * c.tiles + td.t_local];
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < td.count; i += kKeyBlock) {
-        const uint64_t *src = keys + (td.start + i) * NW;
-        const uint64_t pos = atomicAdd(&cur[fine_of(xxh3_64<NW>(src), d)], 1ull);
-#pragma unroll
-        for (int w = 0; w < NW; ++w) out[pos * NW + w] = src[w];
-    }
-}
-
 
 // Exclusive scan of one value per thread of a 256-thread block: a shuffle
 // scan inside each wave, then the four wave totals (two barriers).
@@ -209,13 +81,57 @@ __device__ __forceinline__ uint32_t scan256(uint32_t v, uint32_t *wsum, uint32_t
     return wbase + x - v;
 }
 
+// ---- front end over framed bytes (persistent: block b owns tiles b + iG) --
+template <int NW>
+__global__ __launch_bounds__(kParseBlock) void k_front_count_seq(SeqSrc src, uint32_t k, Digits d,
+                                                                 uint32_t *__restrict__ m1)
+{
+    __shared__ ParseLds lds;
+    __shared__ uint32_t hist[256];
+    uint64_t n, emit_from;
+    seq_bounds(src, n, emit_from);
+    hist[threadIdx.x] = 0;
+    const uint64_t ntiles = (n + kParseTile - 1) / kParseTile;
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        stage_tile(src.seq, n, tile, lds);
+        roll_segment<NW>(lds, tile, n, emit_from, k, [&](const uint64_t *key) {
+            atomicAdd(&hist[coarse_of(xxh3_64<NW>(key), d)], 1u);
+        });
+        __syncthreads();  // the next tile overwrites lds
+    }
+    __syncthreads();
+    if (threadIdx.x < (1u << d.l1)) m1[(uint64_t)threadIdx.x * gridDim.x + blockIdx.x] = hist[threadIdx.x];
+}
+
+// Front scatter, two-word keys: keys go straight to their coarse cursor.
+template <int NW>
+__global__ __launch_bounds__(kParseBlock) void k_front_scatter_seq(SeqSrc src, uint32_t k, Digits d,
+                                                                   const uint64_t *__restrict__ o1,
+                                                                   uint64_t *__restrict__ out)
+{
+    __shared__ ParseLds lds;
+    __shared__ unsigned long long cur[256];
+    uint64_t n, emit_from;
+    seq_bounds(src, n, emit_from);
+    if (threadIdx.x < (1u << d.l1)) cur[threadIdx.x] = o1[(uint64_t)threadIdx.x * gridDim.x + blockIdx.x];
+    const uint64_t ntiles = (n + kParseTile - 1) / kParseTile;
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        stage_tile(src.seq, n, tile, lds);
+        roll_segment<NW>(lds, tile, n, emit_from, k, [&](const uint64_t *key) {
+            const uint64_t pos = atomicAdd(&cur[coarse_of(xxh3_64<NW>(key), d)], 1ull);
+#pragma unroll
+            for (int w = 0; w < NW; ++w) out[pos * NW + w] = key[w];
+        });
+        __syncthreads();
+    }
+}
+
 // Front scatter with write combining (one-word keys): each thread rolls its
 // 64-byte segment in four 16-byte chunks; the block's keys of one chunk are
 // placed in LDS grouped by coarse digit and written out as contiguous runs.
-__global__ __launch_bounds__(kParseBlock) void k_front_scatter_seq_staged(const uint8_t *__restrict__ seq, uint64_t n,
-                                                                          uint64_t emit_from, uint32_t k, Digits d,
+__global__ __launch_bounds__(kParseBlock) void k_front_scatter_seq_staged(SeqSrc src, uint32_t k, Digits d,
                                                                           const uint64_t *__restrict__ o1,
-                                                                          uint64_t nblk, uint64_t *__restrict__ out)
+                                                                          uint64_t *__restrict__ out)
 {
     constexpr int kChunkBytes = 16;
     constexpr int kStage = kParseBlock * kChunkBytes;
@@ -226,78 +142,168 @@ __global__ __launch_bounds__(kParseBlock) void k_front_scatter_seq_staged(const 
     __shared__ uint64_t base[256];
     const uint32_t t = threadIdx.x;
     const uint32_t nbins = 1u << d.l1;
-    base[t] = t < nbins ? o1[(uint64_t)t * nblk + blockIdx.x] : 0;
+    uint64_t n, emit_from;
+    seq_bounds(src, n, emit_from);
+    base[t] = t < nbins ? o1[(uint64_t)t * gridDim.x + blockIdx.x] : 0;
     hist[t] = 0;
-    stage_tile(seq, n, blockIdx.x, lds);
-
-    const uint64_t start = (uint64_t)blockIdx.x * kParseTile + (uint64_t)t * kParseSeg;
-    Roll<1> roll(k);
-    uint32_t have = 0;
-    for (int j = 8; j < 16; ++j) {  // 32-byte halo from the previous segment
-        const uint32_t v = lds.w[j][t];
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            bool valid;
-            roll.push(base_code(v >> (8 * b), valid));
-            have = valid ? have + 1 : 0;
-        }
-    }
-    for (int q = 0; q < 4; ++q) {
-        uint64_t key[kChunkBytes];
-        uint32_t rank[kChunkBytes];
-        uint32_t bin[kChunkBytes];
-        uint32_t emitted = 0;
-#pragma unroll
-        for (int jj = 0; jj < 4; ++jj) {
-            const uint32_t v = lds.w[q * 4 + jj][t + 1];
+    const uint64_t ntiles = (n + kParseTile - 1) / kParseTile;
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        stage_tile(src.seq, n, tile, lds);
+        const uint64_t start = tile * kParseTile + (uint64_t)t * kParseSeg;
+        Roll<1> roll(k);
+        uint32_t have = 0;
+        for (int j = 8; j < 16; ++j) {  // 32-byte halo from the previous segment
+            const uint32_t v = lds.w[j][t];
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
-                const int e = jj * 4 + b;
                 bool valid;
                 roll.push(base_code(v >> (8 * b), valid));
                 have = valid ? have + 1 : 0;
-                const uint64_t pos = start + 4 * (q * 4 + jj) + b;
-                const bool emit = start < n && have >= k && pos >= emit_from;
-                if (emit) {
-                    roll.canon(&key[e]);
-                    bin[e] = coarse_of(xxh3_64<1>(&key[e]), d);
-                    rank[e] = atomicAdd(&hist[bin[e]], 1u);
-                    emitted |= 1u << e;
+            }
+        }
+        for (int q = 0; q < 4; ++q) {
+            uint64_t key[kChunkBytes];
+            uint32_t rank[kChunkBytes];
+            uint32_t bin[kChunkBytes];
+            uint32_t emitted = 0;
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj) {
+                const uint32_t v = lds.w[q * 4 + jj][t + 1];
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    const int e = jj * 4 + b;
+                    bool valid;
+                    roll.push(base_code(v >> (8 * b), valid));
+                    have = valid ? have + 1 : 0;
+                    const uint64_t pos = start + 4 * (q * 4 + jj) + b;
+                    const bool emit = start < n && have >= k && pos >= emit_from;
+                    if (emit) {
+                        roll.canon(&key[e]);
+                        bin[e] = coarse_of(xxh3_64<1>(&key[e]), d);
+                        rank[e] = atomicAdd(&hist[bin[e]], 1u);
+                        emitted |= 1u << e;
+                    }
                 }
             }
-        }
-        __syncthreads();
-        uint32_t total;
-        const uint32_t h = hist[t];
-        loff[t] = scan256(h, tmp, total);
-        __syncthreads();
+            __syncthreads();
+            uint32_t total;
+            const uint32_t h = hist[t];
+            loff[t] = scan256(h, tmp, total);
+            __syncthreads();
 #pragma unroll
-        for (int e = 0; e < kChunkBytes; ++e) {
-            if (emitted & (1u << e)) {
-                const uint32_t p = loff[bin[e]] + rank[e];
-                stage[p] = key[e];
-                sbin[p] = (uint8_t)bin[e];
+            for (int e = 0; e < kChunkBytes; ++e) {
+                if (emitted & (1u << e)) {
+                    const uint32_t p = loff[bin[e]] + rank[e];
+                    stage[p] = key[e];
+                    sbin[p] = (uint8_t)bin[e];
+                }
             }
+            __syncthreads();
+            for (uint32_t i = t; i < total; i += kParseBlock) {
+                const uint32_t c = sbin[i];
+                out[base[c] + (i - loff[c])] = stage[i];
+            }
+            __syncthreads();
+            base[t] += h;
+            hist[t] = 0;
+            __syncthreads();
         }
-        __syncthreads();
-        for (uint32_t i = t; i < total; i += kParseBlock) {
-            const uint32_t c = sbin[i];
-            out[base[c] + (i - loff[c])] = stage[i];
-        }
-        __syncthreads();
-        base[t] += h;
-        hist[t] = 0;
-        __syncthreads();
     }
 }
 
-// Fine scatter with write combining: a tile is handled in chunks of
-// kThreads x FPT keys held in registers; each chunk's keys are ranked per
+// ---- front end over keys (increment of key batches) ---------------------
+template <int NW>
+__global__ __launch_bounds__(kKeyBlock) void k_front_count_keys(const uint64_t *__restrict__ keys, uint64_t n,
+                                                                Digits d, uint32_t *__restrict__ m1)
+{
+    __shared__ uint32_t hist[256];
+    hist[threadIdx.x] = 0;
+    __syncthreads();
+    const uint64_t ntiles = (n + kKeyTile - 1) / kKeyTile;
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        for (int j = 0; j < kKeysPerThread; ++j) {
+            const uint64_t i = tile * kKeyTile + (uint64_t)j * kKeyBlock + threadIdx.x;
+            if (i < n) atomicAdd(&hist[coarse_of(xxh3_64<NW>(keys + i * NW), d)], 1u);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < (1u << d.l1)) m1[(uint64_t)threadIdx.x * gridDim.x + blockIdx.x] = hist[threadIdx.x];
+}
+
+template <int NW>
+__global__ __launch_bounds__(kKeyBlock) void k_front_scatter_keys(const uint64_t *__restrict__ keys, uint64_t n,
+                                                                  Digits d, const uint64_t *__restrict__ o1,
+                                                                  uint64_t *__restrict__ out)
+{
+    __shared__ unsigned long long cur[256];
+    if (threadIdx.x < (1u << d.l1)) cur[threadIdx.x] = o1[(uint64_t)threadIdx.x * gridDim.x + blockIdx.x];
+    __syncthreads();
+    const uint64_t ntiles = (n + kKeyTile - 1) / kKeyTile;
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        for (int j = 0; j < kKeysPerThread; ++j) {
+            const uint64_t i = tile * kKeyTile + (uint64_t)j * kKeyBlock + threadIdx.x;
+            if (i < n) {
+                const uint64_t pos = atomicAdd(&cur[coarse_of(xxh3_64<NW>(keys + i * NW), d)], 1ull);
+#pragma unroll
+                for (int w = 0; w < NW; ++w) out[pos * NW + w] = keys[i * NW + w];
+            }
+        }
+    }
+}
+
+// Coarse region starts: o1[c * G] for every coarse digit, plus the total.
+__global__ void k_coarse_starts(const uint64_t *o1, const uint32_t *m1, uint32_t G, uint32_t n_coarse,
+                                uint64_t *starts)
+{
+    const uint32_t c = threadIdx.x;
+    if (c < n_coarse) starts[c] = o1[(uint64_t)c * G];
+    if (c == 0) {
+        const uint64_t last = (uint64_t)n_coarse * G - 1;
+        starts[n_coarse] = o1[last] + m1[last];
+    }
+}
+
+// ---- second level: fine digit inside each coarse region -----------------
+// Block b works on slice s = b % S of coarse region c = b / S: the keys
+// [start_c + len_c*s/S, start_c + len_c*(s+1)/S).  Its counts go to
+// m2[((c << L2) + f) * S + s], so that the scan lays partition (c, f) out as
+// slices 0..S-1 in order.
+__device__ __forceinline__ void slice_of(const uint64_t *starts, uint32_t S, uint32_t &c, uint32_t &s, uint64_t &beg,
+                                         uint64_t &end)
+{
+    c = blockIdx.x / S;
+    s = blockIdx.x % S;
+    const uint64_t b0 = starts[c], len = starts[c + 1] - b0;
+    beg = b0 + len * s / S;
+    end = b0 + len * (s + 1) / S;
+}
+
+template <int NW>
+__global__ __launch_bounds__(kKeyBlock) void k_fine_count(const uint64_t *__restrict__ keys,
+                                                          const uint64_t *__restrict__ starts, Digits d, uint32_t S,
+                                                          uint32_t *__restrict__ m2)
+{
+    __shared__ uint32_t hist[1024];
+    uint32_t c, s;
+    uint64_t beg, end;
+    slice_of(starts, S, c, s, beg, end);
+    const uint32_t nf = 1u << d.l2;
+    for (uint32_t f = threadIdx.x; f < nf; f += kKeyBlock) hist[f] = 0;
+    __syncthreads();
+    for (uint64_t i = beg + threadIdx.x; i < end; i += kKeyBlock)
+        atomicAdd(&hist[fine_of(xxh3_64<NW>(keys + i * NW), d)], 1u);
+    __syncthreads();
+    const uint64_t row = (uint64_t)c << d.l2;
+    for (uint32_t f = threadIdx.x; f < nf; f += kKeyBlock) m2[(row + f) * S + s] = hist[f];
+}
+
+// Fine scatter with write combining: a slice is handled in chunks of
+// kKeyBlock x FPT keys held in registers; each chunk's keys are ranked per
 // fine digit, placed in LDS grouped by digit and written out as runs.
 template <int NW>
 __global__ __launch_bounds__(kKeyBlock) void k_fine_scatter_staged(const uint64_t *__restrict__ keys,
-                                                                   const TileDesc *tiles, const CoarseInfo *ci,
-                                                                   Digits d, const uint64_t *__restrict__ o2,
+                                                                   const uint64_t *__restrict__ starts, Digits d,
+                                                                   uint32_t S, const uint64_t *__restrict__ o2,
                                                                    uint64_t *__restrict__ out)
 {
     constexpr uint32_t FPT = NW == 1 ? 16 : 8;          // keys per thread per chunk
@@ -306,24 +312,26 @@ __global__ __launch_bounds__(kKeyBlock) void k_fine_scatter_staged(const uint64_
     __shared__ uint16_t sbin[CH];
     __shared__ uint32_t hist[1024], loff[1024], tmp[4];
     __shared__ uint64_t base[1024];
-    const TileDesc td = tiles[blockIdx.x];
-    const CoarseInfo c = ci[td.coarse];
+    uint32_t c, s;
+    uint64_t beg, end;
+    slice_of(starts, S, c, s, beg, end);
     const uint32_t nf = 1u << d.l2;
     const uint32_t t = threadIdx.x;
+    const uint64_t row = (uint64_t)c << d.l2;
     for (uint32_t f = t; f < nf; f += kKeyBlock) {
-        base[f] = o2[c.mbase + (uint64_t)f * c.tiles + td.t_local];
+        base[f] = o2[(row + f) * S + s];
         hist[f] = 0;
     }
     __syncthreads();
-    for (uint32_t cs = 0; cs < td.count; cs += CH) {
-        const uint32_t cn = td.count - cs < CH ? td.count - cs : CH;
+    for (uint64_t cs = beg; cs < end; cs += CH) {
+        const uint32_t cn = (uint32_t)(end - cs < CH ? end - cs : CH);
         uint64_t key[FPT][NW];
         uint32_t bin[FPT], rank[FPT];
 #pragma unroll
         for (uint32_t j = 0; j < FPT; ++j) {
             const uint32_t i = j * kKeyBlock + t;
             if (i < cn) {
-                const uint64_t *src = keys + (td.start + cs + i) * NW;
+                const uint64_t *src = keys + (cs + i) * NW;
 #pragma unroll
                 for (int w = 0; w < NW; ++w) key[j][w] = src[w];
                 bin[j] = fine_of(xxh3_64<NW>(key[j]), d);
@@ -368,19 +376,15 @@ __global__ __launch_bounds__(kKeyBlock) void k_fine_scatter_staged(const uint64_
     }
 }
 
-// part p = (coarse c, fine f) starts at o2[mbase_c + f * tiles_c]
-__global__ void k_part_offsets(const uint64_t *o2, const CoarseInfo *ci, Digits d, uint64_t total,
-                               uint64_t *part_off)
+// Partition p starts at o2[p * S] (two levels) or at the coarse start p (one
+// level); part_off[2^L] = total keys.
+__global__ void k_part_offsets(const uint64_t *o2, const uint64_t *starts, Digits d, uint32_t S, uint64_t *part_off)
 {
     const uint64_t n_parts = 1ull << d.part_bits;
-    for (uint64_t p = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; p <= n_parts; p += (uint64_t)gridDim.x * blockDim.x) {
-        if (p == n_parts) {
-            part_off[p] = total;
-            continue;
-        }
-        const uint32_t c = (uint32_t)(p >> d.l2), f = (uint32_t)(p & ((1u << d.l2) - 1));
-        const CoarseInfo x = ci[c];
-        part_off[p] = x.tiles ? o2[x.mbase + (uint64_t)f * x.tiles] : x.start;
+    for (uint64_t p = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; p <= n_parts;
+         p += (uint64_t)gridDim.x * blockDim.x) {
+        if (p == n_parts) part_off[p] = starts[1u << d.l1];
+        else part_off[p] = d.l2 ? o2[p * S] : starts[p];
     }
 }
 
@@ -472,6 +476,8 @@ __global__ __launch_bounds__(kPartThreads) void k_count_parts(const uint64_t *__
     constexpr int SW = slot_words<NW>();
     __shared__ PartLds<NW, BLOOM> S;
     const uint64_t part = blockIdx.x;
+    const uint64_t beg = part_off[part], end = part_off[part + 1];
+    if (beg == end) return;  // nothing to count: the partition stays as it is
     const uint32_t psz = 1u << tv.part_log2;
     const uint64_t sbase = part << tv.part_log2;
     for (uint32_t s = threadIdx.x; s < psz; s += kPartThreads) {
@@ -489,7 +495,6 @@ __global__ __launch_bounds__(kPartThreads) void k_count_parts(const uint64_t *__
     }
     __syncthreads();
 
-    const uint64_t beg = part_off[part], end = part_off[part + 1];
     uint32_t n_k = 0, n_new = 0, n_err = 0;
     for (uint64_t base = beg; base < end; base += kPartThreads) {
         const uint64_t i = base + threadIdx.x;
@@ -529,17 +534,19 @@ using namespace kcp;
 
 namespace {
 
-template <typename T>
-int ensure(T *&p, uint64_t &cap, uint64_t need)
+Digits digits_of(const kc_table *t)
 {
-    if (need <= cap && p) return KC_OK;
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    cap = 0;
-    const uint64_t n = std::max<uint64_t>(need, 1) + need / 8;
-    if (hipMalloc((void **)&p, n * sizeof(T)) != hipSuccess) return KC_ENOMEM;
-    cap = n;
-    return KC_OK;
+    Digits d;
+    d.part_bits = t->part_bits;
+    d.l1 = std::min<uint32_t>(d.part_bits, 8);
+    d.l2 = d.part_bits - d.l1;
+    return d;
+}
+
+template <typename T>
+bool dev_alloc(T *&p, uint64_t n)
+{
+    return hipMalloc((void **)&p, std::max<uint64_t>(n, 1) * sizeof(T)) == hipSuccess;
 }
 
 }  // namespace
@@ -553,112 +560,122 @@ bool kc_part_eligible(const kc_table *t)
     return t->part_bits <= 18;
 }
 
-void kc_part_free(kc_table *t)
+void kc_part_scratch_free(kc_part_scratch &s)
 {
-    kc_part_scratch &s = t->ps;
     for (void *p : {(void *)s.keys_a, (void *)s.keys_b, (void *)s.m1, (void *)s.o1, (void *)s.m2, (void *)s.o2,
-                    (void *)s.sums, (void *)s.desc, (void *)s.coarse, (void *)s.part_off, (void *)s.starts})
+                    (void *)s.sums, (void *)s.part_off, (void *)s.starts})
         if (p) (void)hipFree(p);
     s = kc_part_scratch();
 }
 
-// Counts the windows of d_seq (when d_keys is null) or the n_keys keys of
-// d_keys into the table through the partitioned pipeline.
-int kc_part_count(kc_table *t, const uint8_t *d_seq, uint64_t nbytes, uint64_t emit_from, const uint64_t *d_keys,
-                  uint64_t n_keys, const TableView &tv)
+void kc_part_free(kc_table *t) { kc_part_scratch_free(t->ps); }
+
+// Launch shape for up to key_cap keys: the tuned values, else
+// 8 front blocks per CU (rounded to a multiple of 64) and 8 slices per
+// coarse region.
+kc_pipe_shape kc_part_shape(const kc_table *t, const kc_tune &tune, uint64_t key_cap)
 {
-    kc_ctx *c = t->ctx;
-    hipStream_t st = c->stream;
-    kc_part_scratch &s = t->ps;
+    kc_pipe_shape sh;
+    uint32_t g = tune.front_blocks ? tune.front_blocks : t->ctx->cu_count * 8;
+    g = std::max<uint32_t>(64, std::min<uint32_t>(g, 1u << 16));
+    sh.front_blocks = (g + 63) / 64 * 64;
+    sh.fine_slices = std::max<uint32_t>(1, std::min<uint32_t>(tune.fine_slices ? tune.fine_slices : 8, 64));
+    sh.key_cap = key_cap;
+    return sh;
+}
+
+// Sizes the scratch for a shape (reallocating what is too small).
+int kc_part_reserve(const kc_table *t, kc_part_scratch &s, const kc_pipe_shape &sh)
+{
+    const Digits d = digits_of(t);
+    const uint64_t n_coarse = 1ull << d.l1, n_parts = 1ull << d.part_bits;
+    const uint64_t m1n = n_coarse * sh.front_blocks;
+    const uint64_t m2n = d.l2 ? n_parts * sh.fine_slices : 0;
+    const uint64_t sumsn = kc_scan_tmp_words(std::max(m1n, m2n));
+    const uint64_t nw = t->nw;
+    auto grow = [](auto *&p, uint64_t &have, uint64_t need) {
+        if (p && have >= need) return true;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        have = 0;
+        if (!dev_alloc(p, need)) return false;
+        have = need;
+        return true;
+    };
+    uint64_t kc_a = s.key_cap, kc_b = s.key_cap, m1c = s.m1_n, o1c = s.m1_n, m2c = s.m2_n, o2c = s.m2_n;
+    uint64_t sc = s.sums_n, pc = s.part_n, stc = s.starts ? 257 : 0;
+    const uint64_t keys_words = std::max<uint64_t>(sh.key_cap, 1) * nw;
+    kc_a *= nw;
+    kc_b *= nw;
+    const bool ok = grow(s.keys_a, kc_a, keys_words) && (d.l2 == 0 || grow(s.keys_b, kc_b, keys_words)) &&
+                    grow(s.m1, m1c, m1n) && grow(s.o1, o1c, m1n) && (d.l2 == 0 || grow(s.m2, m2c, m2n)) &&
+                    (d.l2 == 0 || grow(s.o2, o2c, m2n)) && grow(s.sums, sc, sumsn) &&
+                    grow(s.part_off, pc, n_parts + 1) && grow(s.starts, stc, 257);
+    if (!ok) {
+        kc_part_scratch_free(s);
+        return KC_ENOMEM;
+    }
+    s.key_cap = std::min(kc_a, d.l2 ? kc_b : kc_a) / nw;
+    s.m1_n = std::min(m1c, o1c);
+    s.m2_n = d.l2 ? std::min(m2c, o2c) : s.m2_n;
+    s.sums_n = sc;
+    s.part_n = pc;
+    return KC_OK;
+}
+
+int kc_part_enqueue(const kc_table *t, kc_part_scratch &s, const kc_pipe_shape &sh, const SeqSrc &seq,
+                    const uint64_t *d_keys, uint64_t n_keys, const TableView &tv, hipStream_t st)
+{
     const int nw = (int)t->nw;
-    Digits d;
-    d.part_bits = t->part_bits;
-    d.l1 = std::min<uint32_t>(d.part_bits, 8);
-    d.l2 = d.part_bits - d.l1;
+    const Digits d = digits_of(t);
     const uint32_t n_coarse = 1u << d.l1;
-    const bool seq = d_keys == nullptr;
-    const uint64_t nblk = seq ? (nbytes + kParseTile - 1) / kParseTile : (n_keys + kKeyTile - 1) / kKeyTile;
-    if (!nblk) return KC_OK;
-
-    // level 1: count, scan, coarse starts
-    const uint64_t m1n = (uint64_t)n_coarse * nblk;
-    int rc;
-    if ((rc = ensure(s.m1, s.m1_cap, m1n)) || (rc = ensure(s.o1, s.o1_cap, m1n)) ||
-        (rc = ensure(s.sums, s.sums_cap, kc_scan_tmp_words(m1n))) || (rc = ensure(s.starts, s.starts_cap, 257)))
-        return rc;
-    if (seq) {
-        if (nw == 1) hipLaunchKernelGGL(k_front_count_seq<1>, dim3(nblk), dim3(kParseBlock), 0, st, d_seq, nbytes, emit_from, t->cfg.k, d, s.m1, nblk);
-        else hipLaunchKernelGGL(k_front_count_seq<2>, dim3(nblk), dim3(kParseBlock), 0, st, d_seq, nbytes, emit_from, t->cfg.k, d, s.m1, nblk);
-    } else {
-        if (nw == 1) hipLaunchKernelGGL(k_front_count_keys<1>, dim3(nblk), dim3(kKeyBlock), 0, st, d_keys, n_keys, d, s.m1, nblk);
-        else hipLaunchKernelGGL(k_front_count_keys<2>, dim3(nblk), dim3(kKeyBlock), 0, st, d_keys, n_keys, d, s.m1, nblk);
-    }
-    KC_HIP(hipGetLastError());
-    if ((rc = kc_scan_u32(c, s.m1, m1n, s.o1, s.sums))) return rc;
-    hipLaunchKernelGGL(k_coarse_starts, dim3(1), dim3(256), 0, st, s.o1, s.m1, nblk, n_coarse, s.starts);
-    KC_HIP(hipGetLastError());
-    std::vector<uint64_t> starts(n_coarse + 1);
-    KC_HIP(hipMemcpyAsync(starts.data(), s.starts, (n_coarse + 1) * 8, hipMemcpyDeviceToHost, st));
-    KC_HIP(hipStreamSynchronize(st));
-    const uint64_t K = starts[n_coarse];
-    if (!K) return KC_OK;
-
-    // level 1 scatter
-    if ((rc = ensure(s.keys_a, s.keys_cap_a, K * nw))) return rc;
-    if (seq) {
-        if (nw == 1) hipLaunchKernelGGL(k_front_scatter_seq_staged, dim3(nblk), dim3(kParseBlock), 0, st, d_seq, nbytes, emit_from, t->cfg.k, d, s.o1, nblk, s.keys_a);
-        else hipLaunchKernelGGL(k_front_scatter_seq<2>, dim3(nblk), dim3(kParseBlock), 0, st, d_seq, nbytes, emit_from, t->cfg.k, d, s.o1, nblk, s.keys_a);
-    } else {
-        if (nw == 1) hipLaunchKernelGGL(k_front_scatter_keys<1>, dim3(nblk), dim3(kKeyBlock), 0, st, d_keys, n_keys, d, s.o1, nblk, s.keys_a);
-        else hipLaunchKernelGGL(k_front_scatter_keys<2>, dim3(nblk), dim3(kKeyBlock), 0, st, d_keys, n_keys, d, s.o1, nblk, s.keys_a);
-    }
-    KC_HIP(hipGetLastError());
-
     const uint64_t n_parts = 1ull << d.part_bits;
-    if ((rc = ensure(s.part_off, s.part_off_cap, n_parts + 1))) return rc;
-    const uint64_t *part_keys = s.keys_a;
-    if (d.l2 == 0) {
-        // coarse regions are the partitions
-        KC_HIP(hipMemcpyAsync(s.part_off, s.starts, (n_parts + 1) * 8, hipMemcpyDeviceToDevice, st));
+    const uint32_t G = sh.front_blocks, S = sh.fine_slices;
+    const bool from_seq = d_keys == nullptr;
+    // the shape the scratch was sized for must cover this launch
+    const uint64_t m1n = (uint64_t)n_coarse * G, m2n = d.l2 ? n_parts * S : 0;
+    const uint64_t need_keys = from_seq ? seq.n - std::min(seq.n, seq.hdr ? 0 : seq.emit_from) : n_keys;
+    if (s.key_cap < need_keys || s.key_cap < sh.key_cap || s.m1_n < m1n || s.m2_n < m2n ||
+        s.sums_n < kc_scan_tmp_words(std::max(m1n, m2n)) || s.part_n < n_parts + 1 || !s.starts)
+        return KC_EINVAL;
+
+    // level 1: count, scan, coarse starts, scatter
+    if (from_seq) {
+        if (nw == 1) hipLaunchKernelGGL(k_front_count_seq<1>, dim3(G), dim3(kParseBlock), 0, st, seq, t->cfg.k, d, s.m1);
+        else hipLaunchKernelGGL(k_front_count_seq<2>, dim3(G), dim3(kParseBlock), 0, st, seq, t->cfg.k, d, s.m1);
     } else {
-        // level 2: tiles inside each coarse region
-        std::vector<CoarseInfo> ci(n_coarse);
-        std::vector<TileDesc> td;
-        uint64_t mbase = 0;
-        for (uint32_t cc = 0; cc < n_coarse; ++cc) {
-            const uint64_t beg = starts[cc], len = starts[cc + 1] - starts[cc];
-            const uint64_t tiles = (len + kFineTile - 1) / kFineTile;
-            ci[cc] = {mbase, tiles, beg};
-            for (uint64_t tl = 0; tl < tiles; ++tl) {
-                const uint64_t off = tl * kFineTile;
-                td.push_back({beg + off, (uint32_t)std::min<uint64_t>(kFineTile, len - off), cc, (uint32_t)tl, 0});
-            }
-            mbase += tiles << d.l2;
-        }
-        const uint64_t m2n = mbase;
-        const uint64_t desc_words = (td.size() * sizeof(TileDesc) + 7) / 8;
-        if ((rc = ensure(s.keys_b, s.keys_cap_b, K * nw)) || (rc = ensure(s.m2, s.m2_cap, m2n)) ||
-            (rc = ensure(s.o2, s.o2_cap, m2n)) || (rc = ensure(s.sums, s.sums_cap, kc_scan_tmp_words(std::max(m1n, m2n)))) ||
-            (rc = ensure(s.desc, s.desc_cap, desc_words)) ||
-            (rc = ensure(s.coarse, s.coarse_cap, (n_coarse * sizeof(CoarseInfo) + 7) / 8)))
-            return rc;
-        KC_HIP(hipMemcpyAsync(s.desc, td.data(), td.size() * sizeof(TileDesc), hipMemcpyHostToDevice, st));
-        KC_HIP(hipMemcpyAsync(s.coarse, ci.data(), ci.size() * sizeof(CoarseInfo), hipMemcpyHostToDevice, st));
-        KC_HIP(hipStreamSynchronize(st));  // td / ci are host vectors of this scope
-        const TileDesc *dtd = reinterpret_cast<const TileDesc *>(s.desc);
-        const CoarseInfo *dci = reinterpret_cast<const CoarseInfo *>(s.coarse);
-        const uint32_t ntiles = (uint32_t)td.size();
-        if (nw == 1) hipLaunchKernelGGL(k_fine_count<1>, dim3(ntiles), dim3(kKeyBlock), 0, st, s.keys_a, dtd, dci, d, s.m2);
-        else hipLaunchKernelGGL(k_fine_count<2>, dim3(ntiles), dim3(kKeyBlock), 0, st, s.keys_a, dtd, dci, d, s.m2);
+        if (nw == 1) hipLaunchKernelGGL(k_front_count_keys<1>, dim3(G), dim3(kKeyBlock), 0, st, d_keys, n_keys, d, s.m1);
+        else hipLaunchKernelGGL(k_front_count_keys<2>, dim3(G), dim3(kKeyBlock), 0, st, d_keys, n_keys, d, s.m1);
+    }
+    KC_HIP(hipGetLastError());
+    int rc;
+    if ((rc = kc_scan_u32_on(st, s.m1, m1n, s.o1, s.sums))) return rc;
+    hipLaunchKernelGGL(k_coarse_starts, dim3(1), dim3(256), 0, st, s.o1, s.m1, G, n_coarse, s.starts);
+    if (from_seq) {
+        if (nw == 1) hipLaunchKernelGGL(k_front_scatter_seq_staged, dim3(G), dim3(kParseBlock), 0, st, seq, t->cfg.k, d, s.o1, s.keys_a);
+        else hipLaunchKernelGGL(k_front_scatter_seq<2>, dim3(G), dim3(kParseBlock), 0, st, seq, t->cfg.k, d, s.o1, s.keys_a);
+    } else {
+        if (nw == 1) hipLaunchKernelGGL(k_front_scatter_keys<1>, dim3(G), dim3(kKeyBlock), 0, st, d_keys, n_keys, d, s.o1, s.keys_a);
+        else hipLaunchKernelGGL(k_front_scatter_keys<2>, dim3(G), dim3(kKeyBlock), 0, st, d_keys, n_keys, d, s.o1, s.keys_a);
+    }
+    KC_HIP(hipGetLastError());
+
+    // level 2: S slices per coarse region
+    const uint64_t *part_keys = s.keys_a;
+    if (d.l2) {
+        const dim3 g2(n_coarse * S);
+        if (nw == 1) hipLaunchKernelGGL(k_fine_count<1>, g2, dim3(kKeyBlock), 0, st, s.keys_a, s.starts, d, S, s.m2);
+        else hipLaunchKernelGGL(k_fine_count<2>, g2, dim3(kKeyBlock), 0, st, s.keys_a, s.starts, d, S, s.m2);
         KC_HIP(hipGetLastError());
-        if ((rc = kc_scan_u32(c, s.m2, m2n, s.o2, s.sums))) return rc;
-        if (nw == 1) hipLaunchKernelGGL(k_fine_scatter_staged<1>, dim3(ntiles), dim3(kKeyBlock), 0, st, s.keys_a, dtd, dci, d, s.o2, s.keys_b);
-        else hipLaunchKernelGGL(k_fine_scatter_staged<2>, dim3(ntiles), dim3(kKeyBlock), 0, st, s.keys_a, dtd, dci, d, s.o2, s.keys_b);
-        hipLaunchKernelGGL(k_part_offsets, dim3((uint32_t)std::min<uint64_t>((n_parts + 256) / 256, 4096)), dim3(256), 0,
-                           st, s.o2, dci, d, K, s.part_off);
+        if ((rc = kc_scan_u32_on(st, s.m2, m2n, s.o2, s.sums))) return rc;
+        if (nw == 1) hipLaunchKernelGGL(k_fine_scatter_staged<1>, g2, dim3(kKeyBlock), 0, st, s.keys_a, s.starts, d, S, s.o2, s.keys_b);
+        else hipLaunchKernelGGL(k_fine_scatter_staged<2>, g2, dim3(kKeyBlock), 0, st, s.keys_a, s.starts, d, S, s.o2, s.keys_b);
         KC_HIP(hipGetLastError());
         part_keys = s.keys_b;
     }
+    hipLaunchKernelGGL(k_part_offsets, dim3((uint32_t)std::min<uint64_t>((n_parts + 256) / 256, 4096)), dim3(256), 0,
+                       st, s.o2, s.starts, d, S, s.part_off);
+    KC_HIP(hipGetLastError());
 
     // per-partition LDS counting
     const bool bloom = t->bloom && !t->sealed;
@@ -672,4 +689,20 @@ int kc_part_count(kc_table *t, const uint8_t *d_seq, uint64_t nbytes, uint64_t e
     }
     KC_HIP(hipGetLastError());
     return KC_OK;
+}
+
+// Counts the windows of d_seq (when d_keys is null) or the n_keys keys of
+// d_keys into the table through the partitioned pipeline, on the table's
+// stream, with the table's own scratch (grown to the largest call).
+int kc_part_count(kc_table *t, const uint8_t *d_seq, uint64_t nbytes, uint64_t emit_from, const uint64_t *d_keys,
+                  uint64_t n_keys, const TableView &tv)
+{
+    const bool from_seq = d_keys == nullptr;
+    const uint64_t key_cap = from_seq ? nbytes - std::min(nbytes, emit_from) : n_keys;
+    if (!key_cap) return KC_OK;
+    const kc_pipe_shape sh = kc_part_shape(t, t->tune, key_cap);
+    int rc = kc_part_reserve(t, t->ps, sh);
+    if (rc != KC_OK) return rc;
+    SeqSrc src{d_seq, nbytes, emit_from, nullptr};
+    return kc_part_enqueue(t, t->ps, sh, src, d_keys, n_keys, tv, t->ctx->stream);
 }

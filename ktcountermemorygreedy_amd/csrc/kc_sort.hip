@@ -466,15 +466,20 @@ extern "C" int kc_synth_records(kc_ctx *ctx, void *d_recs, uint64_t n, uint32_t 
 // sums must hold kc_scan_tmp_words(n) words.
 uint64_t kc_scan_tmp_words(uint64_t n) { return (n + kScanChunk - 1) / kScanChunk + 1; }
 
-int kc_scan_u32(kc_ctx *ctx, const uint32_t *in, uint64_t n, uint64_t *out, uint64_t *sums)
+int kc_scan_u32_on(hipStream_t st, const uint32_t *in, uint64_t n, uint64_t *out, uint64_t *sums)
 {
     if (!n) return KC_OK;
     const uint64_t nb = (n + kScanChunk - 1) / kScanChunk;
-    hipLaunchKernelGGL(k_scan_reduce, dim3((uint32_t)nb), dim3(kThreads), 0, ctx->stream, in, n, sums);
-    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kThreads), 0, ctx->stream, sums, nb);
-    hipLaunchKernelGGL(k_scan_apply, dim3((uint32_t)nb), dim3(kThreads), 0, ctx->stream, in, n, sums, out);
+    hipLaunchKernelGGL(k_scan_reduce, dim3((uint32_t)nb), dim3(kThreads), 0, st, in, n, sums);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kThreads), 0, st, sums, nb);
+    hipLaunchKernelGGL(k_scan_apply, dim3((uint32_t)nb), dim3(kThreads), 0, st, in, n, sums, out);
     KC_HIP(hipGetLastError());
     return KC_OK;
+}
+
+int kc_scan_u32(kc_ctx *ctx, const uint32_t *in, uint64_t n, uint64_t *out, uint64_t *sums)
+{
+    return kc_scan_u32_on(ctx->stream, in, n, out, sums);
 }
 
 // Sort of rw-word records on key bits [lo, hi) (internal entry for the dump).

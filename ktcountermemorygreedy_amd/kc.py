@@ -53,6 +53,18 @@ class ReadSpec(C.Structure):
                 ("long_sub_rate", C.c_double), ("min_len", C.c_uint32), ("threads", C.c_uint32)]
 
 
+class Tune(C.Structure):
+    _fields_ = [("front_blocks", C.c_uint32), ("fine_slices", C.c_uint32), ("batch_bytes", C.c_uint64),
+                ("queue_depth", C.c_uint32), ("path", C.c_uint32)]
+
+
+class BatchInfo(C.Structure):
+    _fields_ = [("path", C.c_uint32), ("queue_depth", C.c_uint32), ("capacity", C.c_uint64),
+                ("front_blocks", C.c_uint32), ("fine_slices", C.c_uint32), ("launches", C.c_uint64),
+                ("graphs_captured", C.c_uint64), ("chunks", C.c_uint64), ("pieces", C.c_uint64),
+                ("bytes", C.c_uint64)]
+
+
 class RequestResult(C.Structure):
     _fields_ = [("status", C.c_int), ("reserved", C.c_int), ("latency_ms", C.c_double),
                 ("queue_ms", C.c_double), ("kmers", C.c_uint64), ("distinct", C.c_uint64)]
@@ -92,6 +104,17 @@ EXPORTS = {
     "kc_table_dump_device": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_int, C.c_void_p, C.c_uint64,
                                        C.POINTER(C.c_uint64)]),
     "kc_table_dump_file": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_int, C.c_char_p]),
+    "kc_table_set_tune": (C.c_int, [C.c_void_p, C.POINTER(Tune)]),
+    "kc_table_get_tune": (C.c_int, [C.c_void_p, C.POINTER(Tune)]),
+    "kc_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32, C.POINTER(Tune),
+                              C.POINTER(C.c_float)]),
+    "kc_batch_create": (C.c_int, [C.c_void_p, C.POINTER(Tune), C.POINTER(C.c_void_p)]),
+    "kc_batch_destroy": (None, [C.c_void_p]),
+    "kc_batch_add": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
+    "kc_batch_flush": (C.c_int, [C.c_void_p]),
+    "kc_batch_sync": (C.c_int, [C.c_void_p]),
+    "kc_batch_launch_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
+    "kc_batch_get_info": (C.c_int, [C.c_void_p, C.POINTER(BatchInfo)]),
     "kc_sort_records": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32,
                                   C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(C.c_int)]),
     "kc_records_histogram": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32,
@@ -412,6 +435,72 @@ class Table:
         bins = np.zeros(max_freq - min_freq + 1, dtype=np.int64)
         check(lib().kc_table_histogram(self.h, min_freq, max_freq, _ptr(bins)), "kc_table_histogram")
         return bins
+
+    def set_tune(self, **kw):
+        t = Tune(**{k: v for k, v in kw.items() if v is not None})
+        check(lib().kc_table_set_tune(self.h, C.byref(t)), "kc_table_set_tune")
+
+    def tune(self):
+        t = Tune()
+        check(lib().kc_table_get_tune(self.h, C.byref(t)), "kc_table_get_tune")
+        return {f: getattr(t, f) for f, _ in Tune._fields_}
+
+    def autotune(self, d_sample, nbytes=None, reps=2):
+        """Launch-shape search over a device-resident sample (leaves the
+        table empty); returns (tune dict, best ms)."""
+        n = getattr(d_sample, "valid", d_sample.nbytes) if nbytes is None else nbytes
+        t, ms = Tune(), C.c_float()
+        check(lib().kc_autotune(self.h, _ptr(d_sample), n, reps, C.byref(t), C.byref(ms)), "kc_autotune")
+        return {f: getattr(t, f) for f, _ in Tune._fields_}, ms.value
+
+
+class Batch:
+    """kc_batch: framed read chunks packed into staging slots, each slot
+    counted into `table` by one captured-graph launch."""
+
+    def __init__(self, table: Table, batch_bytes=0, queue_depth=0, path=None, front_blocks=0, fine_slices=0):
+        paths = {None: None, "auto": 0, "direct": 1, "partitioned": 2}
+        h = C.c_void_p()
+        if batch_bytes or queue_depth or path is not None or front_blocks or fine_slices:
+            base = table.tune()
+            t = Tune(front_blocks=front_blocks or base["front_blocks"], fine_slices=fine_slices or base["fine_slices"],
+                     batch_bytes=batch_bytes or base["batch_bytes"], queue_depth=queue_depth or base["queue_depth"],
+                     path=base["path"] if path is None else paths[path])
+            check(lib().kc_batch_create(table.h, C.byref(t), C.byref(h)), "kc_batch_create")
+        else:
+            check(lib().kc_batch_create(table.h, None, C.byref(h)), "kc_batch_create")
+        self.h, self.table = h, table
+
+    def add(self, chunk):
+        buf = np.frombuffer(chunk, dtype=np.uint8) if isinstance(chunk, (bytes, bytearray)) else \
+            np.ascontiguousarray(chunk, dtype=np.uint8)
+        check(lib().kc_batch_add(self.h, _ptr(buf), buf.size), "kc_batch_add")
+
+    def flush(self):
+        check(lib().kc_batch_flush(self.h), "kc_batch_flush")
+
+    def sync(self):
+        check(lib().kc_batch_sync(self.h), "kc_batch_sync")
+
+    def launch_device(self, d_seq, nbytes=None):
+        n = getattr(d_seq, "valid", d_seq.nbytes) if nbytes is None else nbytes
+        check(lib().kc_batch_launch_device(self.h, _ptr(d_seq), n), "kc_batch_launch_device")
+
+    def info(self):
+        i = BatchInfo()
+        check(lib().kc_batch_get_info(self.h, C.byref(i)), "kc_batch_get_info")
+        return {f: getattr(i, f) for f, _ in BatchInfo._fields_}
+
+    def close(self):
+        if self.h:
+            lib().kc_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ---- host helpers ----------------------------------------------------------
