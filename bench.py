@@ -49,6 +49,12 @@ def parse_args():
                     help="one device-to-device copy of this size before the timed region, the "
                          "reference dispatch for scaling rocprof FETCH_SIZE/WRITE_SIZE counters")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--launch", default="auto", choices=["auto", "eager", "graph"],
+                    help="counting call: eager stream launches or one captured-graph launch (kc_batch); "
+                         "auto = graph for uneven_50mbp")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
+    ap.add_argument("--no-host-batch", action="store_true",
+                    help="skip the host-batched (PCIe-inclusive) line of uneven_50mbp")
     return ap.parse_args()
 
 
@@ -297,13 +303,30 @@ def main():
     table = kc.Table(ctx, k, count_max=0xFFFFFFFF, capacity=cap, max_distinct=est_distinct, bloom=bloom,
                      low_capacity=est_distinct if bloom else 0)
 
+    # launch shape: autotuned on the workload itself (outside the timed region)
+    from ktcountermemorygreedy_amd import autotune as AT
+    tune = None
+    if not args.no_autotune:
+        t0 = time.time()
+        tune = AT.tune_shape(table, d_seq, S, reps=1, use_cache=False)
+        tune["seconds"] = time.time() - t0
+        print("autotune: %s" % tune, file=sys.stderr, flush=True)
+    launch = args.launch if args.launch != "auto" else ("graph" if args.workload == "uneven_50mbp" else "eager")
+    batch = None
+    if launch == "graph":
+        table.trim()  # the graph owns its pipeline scratch
+        batch = kc.Batch(table)
+
     bufs = {}
 
     def one_step(slot=None):
         table.clear()
         if slot is not None:
             ctx.record(slot)
-        table.count_seq(d_seq, S)
+        if batch is not None:
+            batch.launch_device(d_seq, S)
+        else:
+            table.count_seq(d_seq, S)
         if slot is not None:
             ctx.record(slot + 1)
         if not bufs:
@@ -345,6 +368,27 @@ def main():
     step_s = elapsed / args.steps
     value = K / step_s
 
+    launch_info = {"launch": launch, "tune": tune}
+    if batch is not None:
+        launch_info["batch"] = batch.info()
+
+    # host-batched line (PCIe-inclusive, never the headline): the reads as
+    # uneven chunks of 1..64 reads packed by kc_batch from host memory, with
+    # the staging queue (slot size x slots in flight) tuned on this input
+    host_batched = None
+    if args.workload == "uneven_50mbp" and not args.no_host_batch:
+        bounds = AT.chunk_bounds(seq_host, np.random.default_rng(7))
+        q = AT.tune_queue(table, seq_host, bounds, use_cache=False)
+        dt, info = AT.run_batches(table, seq_host, bounds, q["batch_bytes"], q["queue_depth"])
+        st_h = table.stats()
+        bins_h = table.histogram(1, 100000)
+        host_batched = {"kmers_per_s": st_h["kmers"] / dt, "seconds": dt, "chunks": int(bounds.size),
+                        "batch_bytes": q["batch_bytes"], "queue_depth": q["queue_depth"],
+                        "launches": info["launches"], "queue_trials": q["trials"],
+                        "same_as_device": bool(st_h["kmers"] == K and st_h["errors"] == 0 and
+                                               np.array_equal(bins_h, bins))}
+        table.clear()
+
     sample_reads = args.cpu_sample_reads or (300_000 if args.workload == "uneven_50mbp" else 4_000_000)
     cpu = None if args.no_cpu_baseline else cpu_baseline(ws, min(sample_reads, ws.n_reads), k, genome)
     line = {
@@ -363,9 +407,13 @@ def main():
                      "alg_bytes_per_launch": alg_bytes, "avg_launch_ms": avg_count_s * 1e3},
         "cpu_baseline": cpu,
         "checks": {"windows_match": windows == K, "hist_mass_equals_kmers": mass == K},
+        "launch": launch_info,
+        "host_batched": host_batched,
         "gen_seconds": gen_s,
     }
     print(json.dumps(line), flush=True)
+    if batch is not None:
+        batch.close()
     table.close()
     ctx.close()
 

@@ -169,10 +169,29 @@ int launch_slot(kc_batch *b, uint32_t i, uint64_t emit_from)
     return KC_OK;
 }
 
-// Makes the current slot's page-locked buffer writable (its last copy done).
+// Makes the current slot's page-locked buffer writable (its last copy
+// done); slots are allocated on first use, so a batch that only launches
+// over device-resident input holds no staging memory.
 int slot_ready(kc_batch *b)
 {
     Slot &s = b->slot[b->cur];
+    if (!s.pinned) {
+        Slot n;
+        const bool ok = hipHostMalloc((void **)&n.pinned, b->cap, hipHostMallocDefault) == hipSuccess &&
+                        hipMalloc((void **)&n.dbuf, b->cap + 16) == hipSuccess &&
+                        hipMalloc((void **)&n.hdr, 16) == hipSuccess &&
+                        hipEventCreateWithFlags(&n.copied, hipEventDisableTiming) == hipSuccess &&
+                        hipEventCreateWithFlags(&n.done, hipEventDisableTiming) == hipSuccess;
+        if (!ok) {
+            if (n.pinned) (void)hipHostFree(n.pinned);
+            if (n.dbuf) (void)hipFree(n.dbuf);
+            if (n.hdr) (void)hipFree(n.hdr);
+            if (n.copied) (void)hipEventDestroy(n.copied);
+            if (n.done) (void)hipEventDestroy(n.done);
+            return KC_ENOMEM;
+        }
+        s = n;
+    }
     if (s.used && s.fill == 0) KC_HIP(hipEventSynchronize(s.copied));
     return KC_OK;
 }
@@ -222,24 +241,17 @@ extern "C" int kc_batch_create(kc_table *t, const kc_tune *tune, kc_batch **out)
     b->info.path = b->path;
     b->info.capacity = b->cap;
     b->info.queue_depth = b->depth;
-    bool ok = hipStreamCreateWithFlags(&b->copy_st, hipStreamNonBlocking) == hipSuccess &&
-              hipStreamCreateWithFlags(&b->cap_st, hipStreamNonBlocking) == hipSuccess;
-    for (uint32_t i = 0; ok && i < b->depth; ++i) {
-        Slot &s = b->slot[i];
-        ok = hipHostMalloc((void **)&s.pinned, b->cap, hipHostMallocDefault) == hipSuccess &&
-             hipMalloc((void **)&s.dbuf, b->cap + 16) == hipSuccess &&
-             hipMalloc((void **)&s.hdr, 16) == hipSuccess &&
-             hipEventCreateWithFlags(&s.copied, hipEventDisableTiming) == hipSuccess &&
-             hipEventCreateWithFlags(&s.done, hipEventDisableTiming) == hipSuccess;
-    }
-    int rc = ok ? ensure_scratch(b, b->cap) : KC_ENOMEM;
-    if (rc != KC_OK) {
+    const bool ok = hipStreamCreateWithFlags(&b->copy_st, hipStreamNonBlocking) == hipSuccess &&
+                    hipStreamCreateWithFlags(&b->cap_st, hipStreamNonBlocking) == hipSuccess;
+    if (!ok) {
         kc_batch_destroy(b);
-        return rc;
+        return KC_EDEVICE;
     }
+    // staging slots and pipeline scratch are allocated on first use
     if (b->path == KC_PATH_PARTITIONED) {
-        b->info.front_blocks = b->sh.front_blocks;
-        b->info.fine_slices = b->sh.fine_slices;
+        const kc_pipe_shape sh = kc_part_shape(t, b->tune, b->cap);
+        b->info.front_blocks = sh.front_blocks;
+        b->info.fine_slices = sh.fine_slices;
     } else {
         b->info.front_blocks = b->direct_grid;
     }

@@ -56,14 +56,15 @@ __global__ __launch_bounds__(kBlock) void k_count_seq(SeqSrc src, TableView tv)
     seq_bounds(src, n, emit_from);
     uint32_t n_kmers = 0, n_new = 0, n_err = 0;
     const uint64_t ntiles = (n + kTile - 1) / kTile;
-    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    uint32_t step = 0;
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x, ++step) {
         stage_tile(src.seq, n, tile, lds);
         roll_segment<NW>(lds, tile, n, emit_from, tv.k, [&](const uint64_t *key) {
             ++n_kmers;
             bool to_table = true;
             if constexpr (BLOOM) {
                 const bool first = bloom_classify<NW>(tv, key) == 0;
-                first_append<NW>(tv, first, key, n_err);
+                first_append<NW>(tv, first, key, n_err, step);
                 to_table = !first;
             }
             if (to_table) table_add<NW>(tv, key, 1, n_new, n_err);
@@ -716,9 +717,10 @@ extern "C" int kc_table_create(kc_ctx *ctx, const kc_table_config *cfg, kc_table
         t->bloom_blocks = cfg->bloom_blocks ? cfg->bloom_blocks : n_parts * kc_lds_part_bloom(t->nw);
         t->bloom_blocks = std::max<uint64_t>(t->bloom_blocks, n_parts);
         t->bloom_part_log2 = ilog2(t->bloom_blocks) - t->part_bits;
-        // Block b appends to segment b % KC_LOW_SEGS: large grids fill the
-        // segments evenly (1/16 slack); a grid of fewer blocks than segments
-        // puts up to one tile of windows (kTile FIRSTs) into one segment.
+        // Blocks, their waves and their loop steps rotate over the segments
+        // (first_append): the segments fill evenly (1/16 slack); a grid too
+        // small to reach every segment puts at most one tile of windows
+        // (kTile FIRSTs) into one segment.
         const uint64_t want = cfg->low_capacity ? cfg->low_capacity : hint;
         t->seg_cap = std::max<uint64_t>((want + want / 16 + KC_LOW_SEGS - 1) / KC_LOW_SEGS + 64,
                                         std::min<uint64_t>(want, kTile));

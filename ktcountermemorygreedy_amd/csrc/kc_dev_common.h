@@ -241,16 +241,21 @@ __device__ __forceinline__ uint32_t base_code(uint32_t byte, bool &valid)
     return ((c >> 1) ^ (c >> 2)) & 3;
 }
 
-// Appends the FIRST occurrences of this wave step to the block's FIRST-list
-// segment (one cursor atomic per wave step).  Every lane of the wave must
-// call it; `first` selects the lanes that append.
+// Appends the FIRST occurrences of this wave step to a FIRST-list segment
+// (one cursor atomic per wave step).  Every lane of the wave must call it;
+// `first` selects the lanes that append.
 template <int NW>
-__device__ __forceinline__ void first_append(const TableView &tv, bool first, const uint64_t *key, uint32_t &n_err)
+__device__ __forceinline__ void first_append(const TableView &tv, bool first, const uint64_t *key, uint32_t &n_err,
+                                             uint32_t rot)
 {
     const uint64_t firsts = __ballot(first);
     if (!firsts) return;
     const int leader = __ffsll((unsigned long long)firsts) - 1;
-    const uint32_t seg = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) % KC_LOW_SEGS;
+    // consecutive blocks take consecutive segments (blocks that run only a
+    // few waves still spread over all of them), the waves of one block are
+    // 16 segments apart, and `rot` (the block's loop step) moves a block on
+    // to the next segment, so that a small grid spreads over all of them too
+    const uint32_t seg = (blockIdx.x + 16u * (threadIdx.x >> 6) + rot) % KC_LOW_SEGS;
     unsigned long long b0 = 0;
     if (__lane_id() == leader)
         b0 = atomicAdd(tv.counters + KC_CURSOR_BASE + (uint64_t)seg * KC_LINE_WORDS, (unsigned long long)__popcll(firsts));

@@ -36,8 +36,7 @@ constexpr uint64_t KC_EMPTY = ~0ULL;
 // line each, so that blocks update different lines (block b uses lane
 // b % KC_LANES); readers sum the lanes.  The FIRST-occurrence list is cut
 // into KC_LOW_SEGS segments with one cursor line each (wave w of block b
-// appends to segment (b * waves_per_block + w) % KC_LOW_SEGS), for the same
-// reason.
+// appends to segment (b + 16 w) % KC_LOW_SEGS), for the same reason.
 enum kc_stat_idx : int {
     KC_STAT_KMERS = 0,
     KC_STAT_DISTINCT = 1,

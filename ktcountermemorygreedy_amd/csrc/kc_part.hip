@@ -495,8 +495,8 @@ __global__ __launch_bounds__(kPartThreads) void k_count_parts(const uint64_t *__
     }
     __syncthreads();
 
-    uint32_t n_k = 0, n_new = 0, n_err = 0;
-    for (uint64_t base = beg; base < end; base += kPartThreads) {
+    uint32_t n_k = 0, n_new = 0, n_err = 0, step = 0;
+    for (uint64_t base = beg; base < end; base += kPartThreads, ++step) {
         const uint64_t i = base + threadIdx.x;
         const bool valid = i < end;
         uint64_t key[NW];
@@ -506,7 +506,7 @@ __global__ __launch_bounds__(kPartThreads) void k_count_parts(const uint64_t *__
         bool to_table = valid;
         if constexpr (BLOOM) {
             const bool first = valid && lds_bloom<NW, BLOOM>(S, key, bsz - 1) == 0;
-            first_append<NW>(tv, first, key, n_err);
+            first_append<NW>(tv, first, key, n_err, step);
             to_table = valid && !first;
         }
         if (to_table) lds_insert<NW, BLOOM>(S, key, psz - 1, n_new, n_err);

@@ -350,6 +350,7 @@ class Table:
         h = C.c_void_p()
         check(lib().kc_table_create(ctx.h, C.byref(cfg), C.byref(h)), "kc_table_create")
         self.h, self.ctx, self.k, self.nw, self.count_max = h, ctx, k, words_for_k(k), count_max
+        self.bloom = bool(bloom)
 
     def close(self):
         if self.h:
@@ -364,6 +365,10 @@ class Table:
 
     def clear(self):
         check(lib().kc_table_clear(self.h), "kc_table_clear")
+
+    def trim(self):
+        """Frees the counting scratch kept for reuse (kc_table_trim)."""
+        check(lib().kc_table_trim(self.h), "kc_table_trim")
 
     def count_seq(self, d_seq, nbytes=None):
         n = getattr(d_seq, "valid", d_seq.nbytes) if nbytes is None else nbytes

@@ -194,3 +194,39 @@ def test_partitioned_shapes_exact(ctx, front_blocks, fine_slices):
     ctx.sync()
     _check_exact(t, seq, 31)
     t.close()
+
+
+def test_batch_small_launches_many_partitions(ctx):
+    """Many small launches into a table of many partitions: each partition
+    sees a few hundred keys per launch, so only a few waves of its counting
+    block are active.  The FIRST list must still fill its segments evenly
+    (a tight list capacity would overflow otherwise); results equal the
+    eager count."""
+    from ktcountermemorygreedy_amd import kc
+    short, long_ = _reads(31, 60000, 200, genome_len=4_000_000)
+    seq = short + long_
+    d = ctx.upload(np.frombuffer(seq, dtype=np.uint8))
+    te = kc.Table(ctx, 31, capacity=1 << 24, bloom=True, max_distinct=len(seq))
+    te.count_seq(d, len(seq))
+    ctx.sync()
+    se = te.stats()
+    assert se["errors"] == 0
+    tb = kc.Table(ctx, 31, capacity=1 << 24, bloom=True, max_distinct=len(seq),
+                  low_capacity=int(se["low"] * 1.05))
+    b = kc.Batch(tb, batch_bytes=1 << 20, queue_depth=3, path="partitioned")
+    bounds = np.flatnonzero(np.frombuffer(seq, dtype=np.uint8) == 10)[::50] + 1
+    start = 0
+    for end in bounds.tolist() + [len(seq)]:
+        if end > start:
+            b.add(seq[start:end])
+        start = end
+    b.flush()
+    b.sync()
+    assert b.info()["launches"] >= 10
+    b.close()
+    sb = tb.stats()
+    print("eager", se, "batched", sb, flush=True)
+    assert sb["errors"] == 0 and sb["kmers"] == se["kmers"]
+    assert np.array_equal(tb.histogram(1, 5000), te.histogram(1, 5000))
+    te.close()
+    tb.close()
