@@ -308,36 +308,46 @@ struct ParseLds {
 };
 
 // Stages tile `tile` of seq[0, n) into L (all threads; ends with a barrier).
+// A whole tile issues its four 16-byte loads per thread back to back (no
+// wait in between); only the last, partial tile takes the byte-wise path.
 __device__ __forceinline__ void stage_tile(const uint8_t *__restrict__ seq, uint64_t n, uint64_t tile, ParseLds &L)
 {
     const uint32_t t = threadIdx.x;
     const uint64_t base = tile * kParseTile;
+    uint4 v[4];
+    if (base + kParseTile <= n) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            v[r] = *reinterpret_cast<const uint4 *>(seq + base + 16ull * (t + kParseBlock * r));
+    } else {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const uint64_t off = base + 16ull * (t + kParseBlock * r);
+            v[r] = make_uint4(0, 0, 0, 0);
+            if (off + 16 <= n) {
+                v[r] = *reinterpret_cast<const uint4 *>(seq + off);
+            } else if (off < n) {
+                uint32_t w[4] = {0, 0, 0, 0};
+                for (uint64_t b = off; b < n; ++b) w[(b - off) >> 2] |= (uint32_t)seq[b] << (8 * ((b - off) & 3));
+                v[r] = make_uint4(w[0], w[1], w[2], w[3]);
+            }
+        }
+    }
+    uint32_t hv = 0;
+    if (t < 16 && base >= kParseSeg) {
+        const uint64_t off = base - kParseSeg + 4ull * t;
+        if (off + 4 <= n) hv = *reinterpret_cast<const uint32_t *>(seq + off);
+    }
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const uint32_t q = t + kParseBlock * r;  // uint4 index inside the tile
-        const uint64_t off = base + 16ull * q;
-        uint4 v = make_uint4(0, 0, 0, 0);
-        if (off + 16 <= n) {
-            v = *reinterpret_cast<const uint4 *>(seq + off);
-        } else if (off < n) {
-            uint32_t w[4] = {0, 0, 0, 0};
-            for (uint64_t b = off; b < n; ++b) w[(b - off) >> 2] |= (uint32_t)seq[b] << (8 * ((b - off) & 3));
-            v = make_uint4(w[0], w[1], w[2], w[3]);
-        }
         const uint32_t seg = q >> 2, j0 = (q & 3) * 4;
-        L.w[j0 + 0][seg + 1] = v.x;
-        L.w[j0 + 1][seg + 1] = v.y;
-        L.w[j0 + 2][seg + 1] = v.z;
-        L.w[j0 + 3][seg + 1] = v.w;
+        L.w[j0 + 0][seg + 1] = v[r].x;
+        L.w[j0 + 1][seg + 1] = v[r].y;
+        L.w[j0 + 2][seg + 1] = v[r].z;
+        L.w[j0 + 3][seg + 1] = v[r].w;
     }
-    if (t < 16) {
-        uint32_t v = 0;
-        if (base >= kParseSeg) {
-            const uint64_t off = base - kParseSeg + 4ull * t;
-            if (off + 4 <= n) v = *reinterpret_cast<const uint32_t *>(seq + off);
-        }
-        L.w[t][0] = v;
-    }
+    if (t < 16) L.w[t][0] = hv;
     __syncthreads();
 }
 

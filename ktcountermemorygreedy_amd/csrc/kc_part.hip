@@ -290,8 +290,21 @@ __global__ __launch_bounds__(kKeyBlock) void k_fine_count(const uint64_t *__rest
     const uint32_t nf = 1u << d.l2;
     for (uint32_t f = threadIdx.x; f < nf; f += kKeyBlock) hist[f] = 0;
     __syncthreads();
-    for (uint64_t i = beg + threadIdx.x; i < end; i += kKeyBlock)
-        atomicAdd(&hist[fine_of(xxh3_64<NW>(keys + i * NW), d)], 1u);
+    // U keys per thread per step, all loads issued before the first use
+    constexpr uint32_t U = 16;
+    for (uint64_t i0 = beg; i0 < end; i0 += (uint64_t)U * kKeyBlock) {
+        uint64_t key[U][NW];
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) {
+            const uint64_t i = i0 + u * kKeyBlock + threadIdx.x;
+            const uint64_t *src = keys + (i < end ? i : beg) * NW;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) key[u][w] = src[w];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u)
+            if (i0 + u * kKeyBlock + threadIdx.x < end) atomicAdd(&hist[fine_of(xxh3_64<NW>(key[u]), d)], 1u);
+    }
     __syncthreads();
     const uint64_t row = (uint64_t)c << d.l2;
     for (uint32_t f = threadIdx.x; f < nf; f += kKeyBlock) m2[(row + f) * S + s] = hist[f];
@@ -327,13 +340,18 @@ __global__ __launch_bounds__(kKeyBlock) void k_fine_scatter_staged(const uint64_
         const uint32_t cn = (uint32_t)(end - cs < CH ? end - cs : CH);
         uint64_t key[FPT][NW];
         uint32_t bin[FPT], rank[FPT];
+        // all FPT loads first (index clamped into the chunk), then the ranks
+#pragma unroll
+        for (uint32_t j = 0; j < FPT; ++j) {
+            const uint32_t i = j * kKeyBlock + t;
+            const uint64_t *src = keys + (cs + (i < cn ? i : cn - 1)) * NW;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) key[j][w] = src[w];
+        }
 #pragma unroll
         for (uint32_t j = 0; j < FPT; ++j) {
             const uint32_t i = j * kKeyBlock + t;
             if (i < cn) {
-                const uint64_t *src = keys + (cs + i) * NW;
-#pragma unroll
-                for (int w = 0; w < NW; ++w) key[j][w] = src[w];
                 bin[j] = fine_of(xxh3_64<NW>(key[j]), d);
                 rank[j] = atomicAdd(&hist[bin[j]], 1u);
             }
@@ -398,23 +416,22 @@ struct PartLds {
     unsigned long long bloom[2 * BS]; // {BF1, BF2} per block
 };
 
+// Bloom state machine of BloomFilter.h:233-273 on the LDS copy: 0 FIRST,
+// 1 SECOND, 2 THIRD_PLUS.  BF1 only ever receives bits that BF2 holds, so
+// the reference's plain read of BF1 before its RMWs can be folded into the
+// RMWs: fetch_or(BF2) decides FIRST (the one caller that completes the
+// mask), fetch_or(BF1) decides SECOND vs THIRD_PLUS -- the same outcomes for
+// every interleaving, with one branch instead of three.
 template <int NW, bool BLOOM>
 __device__ __forceinline__ int lds_bloom(PartLds<NW, BLOOM> &S, const uint64_t *key, uint32_t bmask)
 {
     uint64_t blk, m;
     bloom_probe<NW>(key, bmask, blk, m);
     unsigned long long *cell = S.bloom + 2 * blk;
-    if ((cell[0] & m) == m) return 2;
-    if ((cell[1] & m) == m) {
-        const uint64_t b1 = atomicOr(cell, (unsigned long long)m);
-        return ((b1 & m) == m) ? 2 : 1;
-    }
     const uint64_t b2 = atomicOr(cell + 1, (unsigned long long)m);
-    if ((b2 & m) == m) {
-        const uint64_t b1 = atomicOr(cell, (unsigned long long)m);
-        return ((b1 & m) == m) ? 2 : 1;
-    }
-    return 0;
+    if ((b2 & m) != m) return 0;
+    const uint64_t b1 = atomicOr(cell, (unsigned long long)m);
+    return ((b1 & m) == m) ? 2 : 1;
 }
 
 template <int NW, bool BLOOM>
@@ -425,12 +442,10 @@ __device__ __forceinline__ bool lds_insert(PartLds<NW, BLOOM> &S, const uint64_t
     const uint64_t h = xxh3_64<NW>(key);
     uint32_t s = (uint32_t)h & pmask;
     for (uint32_t probe = 0; probe <= pmask; ++probe, s = (s + 1) & pmask) {
-        uint64_t w0 = __hip_atomic_load(&S.key[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        bool owner = false;
-        if (w0 == KC_EMPTY) {
-            w0 = atomicCAS(&S.key[s], (unsigned long long)KC_EMPTY, (unsigned long long)key[0]);
-            owner = w0 == KC_EMPTY;
-        }
+        // one LDS CAS per probe: it claims an empty slot or returns the
+        // slot's key (a present key is the common case and is not written)
+        const uint64_t w0 = atomicCAS(&S.key[s], (unsigned long long)KC_EMPTY, (unsigned long long)key[0]);
+        const bool owner = w0 == KC_EMPTY;
         if constexpr (NW == 1) {
             if (owner || w0 == key[0]) {
                 n_new += owner;
@@ -496,12 +511,24 @@ __global__ __launch_bounds__(kPartThreads) void k_count_parts(const uint64_t *__
     __syncthreads();
 
     uint32_t n_k = 0, n_new = 0, n_err = 0, step = 0;
+    // the next step's key is loaded while this step's key is counted
+    uint64_t nxt[NW];
+    {
+        const uint64_t i = beg + threadIdx.x;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) nxt[w] = keys[(i < end ? i : beg) * NW + w];
+    }
     for (uint64_t base = beg; base < end; base += kPartThreads, ++step) {
         const uint64_t i = base + threadIdx.x;
         const bool valid = i < end;
         uint64_t key[NW];
 #pragma unroll
-        for (int w = 0; w < NW; ++w) key[w] = valid ? keys[i * NW + w] : 0;
+        for (int w = 0; w < NW; ++w) key[w] = nxt[w];
+        if (base + kPartThreads < end) {
+            const uint64_t j = i + kPartThreads;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) nxt[w] = keys[(j < end ? j : beg) * NW + w];
+        }
         n_k += valid;
         bool to_table = valid;
         if constexpr (BLOOM) {
