@@ -373,6 +373,8 @@ __device__ __forceinline__ void roll_segment(const ParseLds &L, uint64_t tile, u
             have = valid ? have + 1 : 0;
         }
     }
+    // bytes of this segment before emit_from emit nothing
+    const uint32_t skip = emit_from > start ? (uint32_t)(emit_from - start < 64 ? emit_from - start : 64) : 0;
     for (int j = 0; j < 16; ++j) {
         const uint32_t v = L.w[j][t + 1];
 #pragma unroll
@@ -380,7 +382,7 @@ __device__ __forceinline__ void roll_segment(const ParseLds &L, uint64_t tile, u
             bool valid;
             roll.push(base_code(v >> (8 * b), valid));
             have = valid ? have + 1 : 0;
-            if (have >= k && start + 4 * j + b >= emit_from) {
+            if (have >= k && (uint32_t)(4 * j + b) >= skip) {
                 uint64_t key[NW];
                 roll.canon(key);
                 emit(key);

@@ -28,6 +28,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "kc_dev_common.h"
@@ -150,6 +151,9 @@ __global__ __launch_bounds__(kParseBlock) void k_front_scatter_seq_staged(SeqSrc
     for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         stage_tile(src.seq, n, tile, lds);
         const uint64_t start = tile * kParseTile + (uint64_t)t * kParseSeg;
+        // segment bytes that emit nothing: before emit_from, or past the end
+        const uint32_t skip = start >= n ? 64u
+                            : emit_from > start ? (uint32_t)(emit_from - start < 64 ? emit_from - start : 64) : 0u;
         Roll<1> roll(k);
         uint32_t have = 0;
         for (int j = 8; j < 16; ++j) {  // 32-byte halo from the previous segment
@@ -175,8 +179,7 @@ __global__ __launch_bounds__(kParseBlock) void k_front_scatter_seq_staged(SeqSrc
                     bool valid;
                     roll.push(base_code(v >> (8 * b), valid));
                     have = valid ? have + 1 : 0;
-                    const uint64_t pos = start + 4 * (q * 4 + jj) + b;
-                    const bool emit = start < n && have >= k && pos >= emit_from;
+                    const bool emit = have >= k && (uint32_t)(4 * (q * 4 + jj) + b) >= skip;
                     if (emit) {
                         roll.canon(&key[e]);
                         bin[e] = coarse_of(xxh3_64<1>(&key[e]), d);
@@ -313,18 +316,24 @@ __global__ __launch_bounds__(kKeyBlock) void k_fine_count(const uint64_t *__rest
 // Fine scatter with write combining: a slice is handled in chunks of
 // kKeyBlock x FPT keys held in registers; each chunk's keys are ranked per
 // fine digit, placed in LDS grouped by digit and written out as runs.
-template <int NW>
+// NF = the largest fine digit count the instance serves (256, or 1024 for
+// partition counts above 2^16).  Chunks of 8192 keys (64 KB) give runs of
+// about 32 keys per digit: longer runs leave fewer partly written lines,
+// and two blocks per CU keep the open runs within L2.
+template <int NW, int NF>
 __global__ __launch_bounds__(kKeyBlock) void k_fine_scatter_staged(const uint64_t *__restrict__ keys,
                                                                    const uint64_t *__restrict__ starts, Digits d,
                                                                    uint32_t S, const uint64_t *__restrict__ o2,
                                                                    uint64_t *__restrict__ out)
 {
-    constexpr uint32_t FPT = NW == 1 ? 16 : 8;          // keys per thread per chunk
-    constexpr uint32_t CH = kKeyBlock * FPT;            // 4096 / 2048 keys (32 KB)
+    using BinT = typename std::conditional<NF <= 256, uint8_t, uint16_t>::type;
+    constexpr uint32_t FPT = NW == 1 ? 32 : 16;         // keys per thread per chunk
+    constexpr uint32_t CH = kKeyBlock * FPT;            // 8192 / 4096 keys (64 KB)
     __shared__ uint64_t stage[CH * NW];
-    __shared__ uint16_t sbin[CH];
-    __shared__ uint32_t hist[1024], loff[1024], tmp[4];
-    __shared__ uint64_t base[1024];
+    __shared__ BinT sbin[CH];
+    __shared__ uint32_t hist[NF], loff[NF];
+    __shared__ uint64_t base[NF];
+    uint32_t *tmp = loff + NF - 4;  // scan256's wave sums: loff is rewritten only after the scan
     uint32_t c, s;
     uint64_t beg, end;
     slice_of(starts, S, c, s, beg, end);
@@ -375,7 +384,7 @@ __global__ __launch_bounds__(kKeyBlock) void k_fine_scatter_staged(const uint64_
                 const uint32_t p = loff[bin[j]] + rank[j];
 #pragma unroll
                 for (int w = 0; w < NW; ++w) stage[p * NW + w] = key[j][w];
-                sbin[p] = (uint16_t)bin[j];
+                sbin[p] = (BinT)bin[j];
             }
         }
         __syncthreads();
@@ -695,8 +704,13 @@ int kc_part_enqueue(const kc_table *t, kc_part_scratch &s, const kc_pipe_shape &
         else hipLaunchKernelGGL(k_fine_count<2>, g2, dim3(kKeyBlock), 0, st, s.keys_a, s.starts, d, S, s.m2);
         KC_HIP(hipGetLastError());
         if ((rc = kc_scan_u32_on(st, s.m2, m2n, s.o2, s.sums))) return rc;
-        if (nw == 1) hipLaunchKernelGGL(k_fine_scatter_staged<1>, g2, dim3(kKeyBlock), 0, st, s.keys_a, s.starts, d, S, s.o2, s.keys_b);
-        else hipLaunchKernelGGL(k_fine_scatter_staged<2>, g2, dim3(kKeyBlock), 0, st, s.keys_a, s.starts, d, S, s.o2, s.keys_b);
+        if (d.l2 <= 8) {
+            if (nw == 1) hipLaunchKernelGGL((k_fine_scatter_staged<1, 256>), g2, dim3(kKeyBlock), 0, st, s.keys_a, s.starts, d, S, s.o2, s.keys_b);
+            else hipLaunchKernelGGL((k_fine_scatter_staged<2, 256>), g2, dim3(kKeyBlock), 0, st, s.keys_a, s.starts, d, S, s.o2, s.keys_b);
+        } else {
+            if (nw == 1) hipLaunchKernelGGL((k_fine_scatter_staged<1, 1024>), g2, dim3(kKeyBlock), 0, st, s.keys_a, s.starts, d, S, s.o2, s.keys_b);
+            else hipLaunchKernelGGL((k_fine_scatter_staged<2, 1024>), g2, dim3(kKeyBlock), 0, st, s.keys_a, s.starts, d, S, s.o2, s.keys_b);
+        }
         KC_HIP(hipGetLastError());
         part_keys = s.keys_b;
     }
