@@ -61,13 +61,14 @@ __global__ __launch_bounds__(kBlock) void k_count_seq(SeqSrc src, TableView tv)
         stage_tile(src.seq, n, tile, lds);
         roll_segment<NW>(lds, tile, n, emit_from, tv.k, [&](const uint64_t *key) {
             ++n_kmers;
+            const KeyHash kh = key_hash<NW>(tv, key);
             bool to_table = true;
             if constexpr (BLOOM) {
-                const bool first = bloom_classify<NW>(tv, key) == 0;
+                const bool first = bloom_classify(tv, kh) == 0;
                 first_append<NW>(tv, first, key, n_err, step);
                 to_table = !first;
             }
-            if (to_table) table_add<NW>(tv, key, 1, n_new, n_err);
+            if (to_table) table_add<NW>(tv, key, kh, 1, n_new, n_err);
         });
         __syncthreads();  // the next tile overwrites lds
     }
@@ -86,7 +87,7 @@ __global__ __launch_bounds__(kBlock) void k_increment(const uint64_t *__restrict
         uint64_t key[NW];
 #pragma unroll
         for (int w = 0; w < NW; ++w) key[w] = keys[i * NW + w];
-        table_add<NW>(tv, key, inc ? inc[i] : 1, n_new, n_err);
+        table_add<NW>(tv, key, key_hash<NW>(tv, key), inc ? inc[i] : 1, n_new, n_err);
     }
     stat_add(tv.counters, KC_STAT_DISTINCT, n_new);
     stat_add(tv.counters, KC_STAT_ERRORS, n_err);
@@ -102,7 +103,7 @@ __global__ __launch_bounds__(kBlock) void k_increment_low(uint64_t n, TableView 
         const uint64_t *src = low_key(tv, i, NW);
 #pragma unroll
         for (int w = 0; w < NW; ++w) key[w] = src[w];
-        table_add<NW>(tv, key, 1, n_new, n_err);
+        table_add<NW>(tv, key, key_hash<NW>(tv, key), 1, n_new, n_err);
     }
     stat_add(tv.counters, KC_STAT_DISTINCT, n_new);
     stat_add(tv.counters, KC_STAT_ERRORS, n_err);

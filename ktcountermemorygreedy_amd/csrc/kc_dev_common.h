@@ -42,11 +42,32 @@ __device__ __forceinline__ uint64_t part_of(uint64_t h, uint32_t part_bits)
     return part_bits ? h >> (64 - part_bits) : 0;
 }
 
-// Slot index of probe step `probe` for a key hashing to h.
-__device__ __forceinline__ uint64_t slot_index(const TableView &tv, uint64_t h, uint64_t probe)
+// Where a key lives.  Its partition is the top part_bits of XXH3_64 (the
+// digit the scatter levels sort by).  Inside the partition, one XXH3_128
+// gives both the Bloom probe (block lo & (blocks - 1), mask bits from lo and
+// hi | 1: BloomFilter.h:215-222, 287-297, on the low bits) and the linear
+// probe start (hi >> 40, bits the Bloom probe does not use).
+struct KeyHash {
+    uint64_t part;
+    uint64_t lo, hi;
+};
+
+template <int NW>
+__device__ __forceinline__ KeyHash key_hash(const TableView &tv, const uint64_t *key)
+{
+    KeyHash kh;
+    kh.part = part_of(xxh3_64<NW>(key), tv.part_bits);
+    xxh3_128<NW>(key, kh.lo, kh.hi);
+    return kh;
+}
+
+__device__ __forceinline__ uint32_t probe_start(uint64_t hi) { return (uint32_t)(hi >> 40); }
+
+// Slot index of probe step `probe` of a key.
+__device__ __forceinline__ uint64_t slot_index(const TableView &tv, const KeyHash &kh, uint64_t probe)
 {
     const uint64_t pmask = (1ULL << tv.part_log2) - 1;
-    return (part_of(h, tv.part_bits) << tv.part_log2) | ((h + probe) & pmask);
+    return (kh.part << tv.part_log2) | ((probe_start(kh.hi) + probe) & pmask);
 }
 
 __device__ __forceinline__ uint64_t lanemask_lt()
@@ -92,13 +113,12 @@ __device__ __forceinline__ const uint64_t *low_key(const TableView &tv, uint64_t
 // its first key word; counts are 64-bit atomic adds (saturation is applied
 // when results are read, so min(true count, count_max) is exact).
 template <int NW>
-__device__ __forceinline__ bool table_add(const TableView &tv, const uint64_t *key, uint64_t inc, uint32_t &n_new,
-                                          uint32_t &n_err)
+__device__ __forceinline__ bool table_add(const TableView &tv, const uint64_t *key, const KeyHash &kh, uint64_t inc,
+                                          uint32_t &n_new, uint32_t &n_err)
 {
-    const uint64_t h = xxh3_64<NW>(key);
     const uint64_t psize = 1ULL << tv.part_log2;
     for (uint64_t probe = 0; probe < psize; ++probe) {
-        uint64_t *s = tv.slots + slot_index(tv, h, probe) * slot_words<NW>();
+        uint64_t *s = tv.slots + slot_index(tv, kh, probe) * slot_words<NW>();
         // A plain read settles the common case (key present: one atomic add).
         // A stale EMPTY is harmless: the CAS below returns the real word.
         uint64_t w0 = s[0];
@@ -144,33 +164,37 @@ __device__ __forceinline__ bool table_add(const TableView &tv, const uint64_t *k
     return false;
 }
 
+// The slot holding key, or nullptr when absent.
+template <int NW>
+__device__ __forceinline__ uint64_t *table_locate(const TableView &tv, const uint64_t *key)
+{
+    const KeyHash kh = key_hash<NW>(tv, key);
+    const uint64_t psize = 1ULL << tv.part_log2;
+    for (uint64_t probe = 0; probe < psize; ++probe) {
+        uint64_t *s = tv.slots + slot_index(tv, kh, probe) * slot_words<NW>();
+        const uint64_t w0 = s[0];
+        if (w0 == KC_EMPTY) return nullptr;
+        if (w0 == key[0] && (NW == 1 || s[1] == key[NW - 1])) return s;
+    }
+    return nullptr;
+}
+
 // Lookup: returns the raw stored count (0 when absent).
 template <int NW>
 __device__ __forceinline__ uint64_t table_find(const TableView &tv, const uint64_t *key)
 {
-    const uint64_t h = xxh3_64<NW>(key);
-    const uint64_t psize = 1ULL << tv.part_log2;
-    for (uint64_t probe = 0; probe < psize; ++probe) {
-        const uint64_t *s = tv.slots + slot_index(tv, h, probe) * slot_words<NW>();
-        const uint64_t w0 = s[0];
-        if (w0 == KC_EMPTY) return 0;
-        if (w0 == key[0]) {
-            if constexpr (NW == 1) return s[1];
-            else if (s[1] == key[1]) return s[2];
-        }
-    }
-    return 0;
+    const uint64_t *s = table_locate<NW>(tv, key);
+    return s ? s[NW == 1 ? 1 : 2] : 0;
 }
 
 // Bloom singleton split state machine (BloomFilter.h:233-273): returns
 // 0 FIRST, 1 SECOND, 2 THIRD_PLUS.  Only atomic results decide a FIRST, so
 // each distinct key gets at most one FIRST.
-template <int NW>
-__device__ __forceinline__ int bloom_classify(const TableView &tv, const uint64_t *key)
+__device__ __forceinline__ int bloom_classify(const TableView &tv, const KeyHash &kh)
 {
     uint64_t blk, m;
-    bloom_probe<NW>(key, (1ULL << tv.bloom_part_log2) - 1, blk, m);
-    blk |= part_of(xxh3_64<NW>(key), tv.part_bits) << tv.bloom_part_log2;
+    bloom_from(kh.lo, kh.hi, (1ULL << tv.bloom_part_log2) - 1, blk, m);
+    blk |= kh.part << tv.bloom_part_log2;
     uint64_t *cell = tv.bloom + 2 * blk;
     if ((ld_agent(cell) & m) == m) return 2;
     if ((ld_agent(cell + 1) & m) == m) {

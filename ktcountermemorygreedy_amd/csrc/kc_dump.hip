@@ -124,18 +124,7 @@ __global__ __launch_bounds__(kThreads) void k_dump_low_list(TableView tv, DumpSp
             const uint64_t *lk = low_key(tv, i, NW);
 #pragma unroll
             for (int w = 0; w < NW; ++w) key[w] = lk[w];
-            // find the slot (same probing as table_find)
-            const uint64_t h = xxh3_64<NW>(key);
-            const uint64_t psize = 1ULL << tv.part_log2;
-            uint64_t *slot = nullptr;
-            for (uint64_t probe = 0; probe < psize; ++probe) {
-                uint64_t *s = tv.slots + slot_index(tv, h, probe) * SW;
-                if (s[0] == KC_EMPTY) break;
-                if (s[0] == key[0] && (NW == 1 || s[1] == key[NW - 1])) {
-                    slot = s;
-                    break;
-                }
-            }
+            uint64_t *slot = table_locate<NW>(tv, key);
             const int64_t cc = slot ? slot_high<NW>(tv, slot[NW == 1 ? 1 : 2] & ~kFlag, false) : -1;
             if (cc >= 0 && in_range((uint64_t)cc, ds)) {
                 uint64_t merged = (uint64_t)cc + 1;

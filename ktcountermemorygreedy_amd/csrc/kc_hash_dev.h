@@ -210,15 +210,21 @@ __device__ __forceinline__ uint64_t hash_func(const uint64_t *w)
 
 // ConcurrentBloomFilter::prepare_insert + calculate_insert_num
 // (BloomFilter.h:215-222, :287-297).
+__device__ __forceinline__ void bloom_from(uint64_t lo, uint64_t hi, uint64_t cap_mask, uint64_t &block,
+                                           uint64_t &mask)
+{
+    const uint64_t h2 = hi | 1ULL;
+    mask = (1ULL << (lo & 63)) | (1ULL << ((lo + h2) & 63)) | (1ULL << ((lo + 2 * h2) & 63));
+    block = lo & cap_mask;
+}
+
 template <int NW>
 __device__ __forceinline__ void bloom_probe(const uint64_t *w, uint64_t cap_mask,
                                             uint64_t &block, uint64_t &mask)
 {
     uint64_t lo, hi;
     xxh3_128<NW>(w, lo, hi);
-    const uint64_t h2 = hi | 1ULL;
-    mask = (1ULL << (lo & 63)) | (1ULL << ((lo + h2) & 63)) | (1ULL << ((lo + 2 * h2) & 63));
-    block = lo & cap_mask;
+    bloom_from(lo, hi, cap_mask, block, mask);
 }
 
 } // namespace kc

@@ -432,10 +432,10 @@ struct PartLds {
 // mask), fetch_or(BF1) decides SECOND vs THIRD_PLUS -- the same outcomes for
 // every interleaving, with one branch instead of three.
 template <int NW, bool BLOOM>
-__device__ __forceinline__ int lds_bloom(PartLds<NW, BLOOM> &S, const uint64_t *key, uint32_t bmask)
+__device__ __forceinline__ int lds_bloom(PartLds<NW, BLOOM> &S, uint64_t lo, uint64_t hi, uint32_t bmask)
 {
     uint64_t blk, m;
-    bloom_probe<NW>(key, bmask, blk, m);
+    bloom_from(lo, hi, bmask, blk, m);
     unsigned long long *cell = S.bloom + 2 * blk;
     const uint64_t b2 = atomicOr(cell + 1, (unsigned long long)m);
     if ((b2 & m) != m) return 0;
@@ -444,12 +444,11 @@ __device__ __forceinline__ int lds_bloom(PartLds<NW, BLOOM> &S, const uint64_t *
 }
 
 template <int NW, bool BLOOM>
-__device__ __forceinline__ bool lds_insert(PartLds<NW, BLOOM> &S, const uint64_t *key, uint32_t pmask,
+__device__ __forceinline__ bool lds_insert(PartLds<NW, BLOOM> &S, const uint64_t *key, uint64_t hi, uint32_t pmask,
                                            uint32_t &n_new, uint32_t &n_err)
 {
     constexpr uint32_t PS = PartLds<NW, BLOOM>::PS;
-    const uint64_t h = xxh3_64<NW>(key);
-    uint32_t s = (uint32_t)h & pmask;
+    uint32_t s = probe_start(hi) & pmask;
     for (uint32_t probe = 0; probe <= pmask; ++probe, s = (s + 1) & pmask) {
         // one LDS CAS per probe: it claims an empty slot or returns the
         // slot's key (a present key is the common case and is not written)
@@ -539,13 +538,15 @@ __global__ __launch_bounds__(kPartThreads) void k_count_parts(const uint64_t *__
             for (int w = 0; w < NW; ++w) nxt[w] = keys[(j < end ? j : beg) * NW + w];
         }
         n_k += valid;
+        uint64_t lo, hi;  // one XXH3_128: Bloom probe and slot probe start
+        xxh3_128<NW>(key, lo, hi);
         bool to_table = valid;
         if constexpr (BLOOM) {
-            const bool first = valid && lds_bloom<NW, BLOOM>(S, key, bsz - 1) == 0;
+            const bool first = valid && lds_bloom<NW, BLOOM>(S, lo, hi, bsz - 1) == 0;
             first_append<NW>(tv, first, key, n_err, step);
             to_table = valid && !first;
         }
-        if (to_table) lds_insert<NW, BLOOM>(S, key, psz - 1, n_new, n_err);
+        if (to_table) lds_insert<NW, BLOOM>(S, key, hi, psz - 1, n_new, n_err);
     }
     __syncthreads();
     for (uint32_t s = threadIdx.x; s < psz; s += kPartThreads) {
