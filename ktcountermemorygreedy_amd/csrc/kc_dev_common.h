@@ -37,13 +37,44 @@ struct TableView {
 
 template <int NW> constexpr int slot_words() { return NW == 1 ? 2 : 4; }
 
-__device__ __forceinline__ uint64_t part_of(uint64_t h, uint32_t part_bits)
+// A table of 2^L partitions splits L into a coarse digit (the top
+// min(L, 8) bits) and a fine digit (the other L - min(L, 8) bits); the two
+// scatter levels sort by them in turn.  The coarse digit comes from a cheap
+// multiply-shift mix (the two parse passes compute it for every window),
+// the fine digit from XXH3_64 bits below its top byte.
+constexpr uint32_t kCoarseMaxBits = 8;
+
+template <int NW>
+__device__ __forceinline__ uint32_t coarse_mix(const uint64_t *key)
 {
-    return part_bits ? h >> (64 - part_bits) : 0;
+    uint64_t x = key[0];
+    if constexpr (NW == 2) x ^= ((key[1] >> 32) | (key[1] << 32)) * 0xC2B2AE3D27D4EB4FULL;
+    x ^= x >> 29;
+    return (uint32_t)((x * 0x9E3779B97F4A7C15ULL) >> 32);
 }
 
-// Where a key lives.  Its partition is the top part_bits of XXH3_64 (the
-// digit the scatter levels sort by).  Inside the partition, one XXH3_128
+__device__ __forceinline__ uint32_t coarse_bits(uint32_t part_bits)
+{
+    return part_bits < kCoarseMaxBits ? part_bits : kCoarseMaxBits;
+}
+
+__device__ __forceinline__ uint32_t coarse_digit(uint32_t mix, uint32_t l1) { return l1 ? mix >> (32 - l1) : 0; }
+
+__device__ __forceinline__ uint32_t fine_digit(uint64_t h64, uint32_t part_bits, uint32_t l2)
+{
+    return l2 ? (uint32_t)(h64 >> (64 - part_bits)) & ((1u << l2) - 1) : 0;
+}
+
+template <int NW>
+__device__ __forceinline__ uint64_t part_of_key(const uint64_t *key, uint32_t part_bits)
+{
+    const uint32_t l1 = coarse_bits(part_bits), l2 = part_bits - l1;
+    const uint64_t c = coarse_digit(coarse_mix<NW>(key), l1);
+    return l2 ? (c << l2) | fine_digit(xxh3_64<NW>(key), part_bits, l2) : c;
+}
+
+// Where a key lives.  Its partition is (coarse digit, fine digit) as
+// above.  Inside the partition, one XXH3_128
 // gives both the Bloom probe (block lo & (blocks - 1), mask bits from lo and
 // hi | 1: BloomFilter.h:215-222, 287-297, on the low bits) and the linear
 // probe start (hi >> 40, bits the Bloom probe does not use).
@@ -56,7 +87,7 @@ template <int NW>
 __device__ __forceinline__ KeyHash key_hash(const TableView &tv, const uint64_t *key)
 {
     KeyHash kh;
-    kh.part = part_of(xxh3_64<NW>(key), tv.part_bits);
+    kh.part = part_of_key<NW>(key, tv.part_bits);
     xxh3_128<NW>(key, kh.lo, kh.hi);
     return kh;
 }

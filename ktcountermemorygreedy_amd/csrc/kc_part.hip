@@ -2,8 +2,9 @@
 // (NewKmerTree.h:329-407, 600-614, 983-1137) feeding per-node maps
 // (NewKmerTree.h:526-570).  Where the reference moves 4 KiB blocks down a
 // 256 x 16 x 16 x 16 tree by the top bits of the key, this pass sends every
-// k-mer to one of 2^L table partitions by the top L bits of its XXH3_64 hash
-// (uniform by construction), in two exact scatter levels, and then counts
+// k-mer to one of 2^L table partitions by hash digits (a coarse digit of up
+// to 8 bits from a multiply-shift mix, a fine digit from XXH3_64), in two
+// exact scatter levels, and then counts
 // each partition in an LDS-resident copy of its sub-table (and Bloom part):
 //
 //   front count    G persistent blocks; block b parses tiles b, b+G, ... and
@@ -47,15 +48,13 @@ struct Digits {
     uint32_t l2;         // fine bits = L - l1
 };
 
-__device__ __forceinline__ uint32_t coarse_of(uint64_t h, const Digits &d)
+template <int NW>
+__device__ __forceinline__ uint32_t coarse_of(const uint64_t *key, const Digits &d)
 {
-    return d.l1 ? (uint32_t)(h >> (64 - d.l1)) : 0;
+    return coarse_digit(coarse_mix<NW>(key), d.l1);
 }
 
-__device__ __forceinline__ uint32_t fine_of(uint64_t h, const Digits &d)
-{
-    return d.l2 ? (uint32_t)(h >> (64 - d.part_bits)) & ((1u << d.l2) - 1) : 0;
-}
+__device__ __forceinline__ uint32_t fine_of(uint64_t h, const Digits &d) { return fine_digit(h, d.part_bits, d.l2); }
 
 // Exclusive scan of one value per thread of a 256-thread block: a shuffle
 // scan inside each wave, then the four wave totals (two barriers).
@@ -96,7 +95,7 @@ __global__ __launch_bounds__(kParseBlock) void k_front_count_seq(SeqSrc src, uin
     for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         stage_tile(src.seq, n, tile, lds);
         roll_segment<NW>(lds, tile, n, emit_from, k, [&](const uint64_t *key) {
-            atomicAdd(&hist[coarse_of(xxh3_64<NW>(key), d)], 1u);
+            atomicAdd(&hist[coarse_of<NW>(key, d)], 1u);
         });
         __syncthreads();  // the next tile overwrites lds
     }
@@ -119,7 +118,7 @@ __global__ __launch_bounds__(kParseBlock) void k_front_scatter_seq(SeqSrc src, u
     for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         stage_tile(src.seq, n, tile, lds);
         roll_segment<NW>(lds, tile, n, emit_from, k, [&](const uint64_t *key) {
-            const uint64_t pos = atomicAdd(&cur[coarse_of(xxh3_64<NW>(key), d)], 1ull);
+            const uint64_t pos = atomicAdd(&cur[coarse_of<NW>(key, d)], 1ull);
 #pragma unroll
             for (int w = 0; w < NW; ++w) out[pos * NW + w] = key[w];
         });
@@ -182,7 +181,7 @@ __global__ __launch_bounds__(kParseBlock) void k_front_scatter_seq_staged(SeqSrc
                     const bool emit = have >= k && (uint32_t)(4 * (q * 4 + jj) + b) >= skip;
                     if (emit) {
                         roll.canon(&key[e]);
-                        bin[e] = coarse_of(xxh3_64<1>(&key[e]), d);
+                        bin[e] = coarse_of<1>(&key[e], d);
                         rank[e] = atomicAdd(&hist[bin[e]], 1u);
                         emitted |= 1u << e;
                     }
@@ -226,7 +225,7 @@ __global__ __launch_bounds__(kKeyBlock) void k_front_count_keys(const uint64_t *
     for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         for (int j = 0; j < kKeysPerThread; ++j) {
             const uint64_t i = tile * kKeyTile + (uint64_t)j * kKeyBlock + threadIdx.x;
-            if (i < n) atomicAdd(&hist[coarse_of(xxh3_64<NW>(keys + i * NW), d)], 1u);
+            if (i < n) atomicAdd(&hist[coarse_of<NW>(keys + i * NW, d)], 1u);
         }
     }
     __syncthreads();
@@ -246,7 +245,7 @@ __global__ __launch_bounds__(kKeyBlock) void k_front_scatter_keys(const uint64_t
         for (int j = 0; j < kKeysPerThread; ++j) {
             const uint64_t i = tile * kKeyTile + (uint64_t)j * kKeyBlock + threadIdx.x;
             if (i < n) {
-                const uint64_t pos = atomicAdd(&cur[coarse_of(xxh3_64<NW>(keys + i * NW), d)], 1ull);
+                const uint64_t pos = atomicAdd(&cur[coarse_of<NW>(keys + i * NW, d)], 1ull);
 #pragma unroll
                 for (int w = 0; w < NW; ++w) out[pos * NW + w] = keys[i * NW + w];
             }
@@ -575,7 +574,7 @@ Digits digits_of(const kc_table *t)
 {
     Digits d;
     d.part_bits = t->part_bits;
-    d.l1 = std::min<uint32_t>(d.part_bits, 8);
+    d.l1 = std::min<uint32_t>(d.part_bits, kCoarseMaxBits);
     d.l2 = d.part_bits - d.l1;
     return d;
 }
