@@ -406,6 +406,45 @@ __device__ __forceinline__ void stage_tile(const uint8_t *__restrict__ seq, uint
     __syncthreads();
 }
 
+// Loads thread t's 64-byte segment of tile `tile` straight into registers:
+// w[0..7] = the 32 bytes before it (history for the first windows),
+// w[8..23] = the segment; bytes outside [0, n) read as 0 (invalid).  The
+// halo words are the previous lane's segment tail, so they hit in L1.
+__device__ __forceinline__ void load_segment(const uint8_t *__restrict__ seq, uint64_t n, uint64_t start,
+                                             uint32_t (&w)[24])
+{
+    if (start >= 32 && start + kParseSeg <= n) {
+        const uint4 *p = reinterpret_cast<const uint4 *>(seq + start - 32);
+#pragma unroll
+        for (int g = 0; g < 6; ++g) {
+            const uint4 v = p[g];
+            w[4 * g] = v.x;
+            w[4 * g + 1] = v.y;
+            w[4 * g + 2] = v.z;
+            w[4 * g + 3] = v.w;
+        }
+        return;
+    }
+#pragma unroll
+    for (int g = 0; g < 6; ++g) {
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (start + 16 * g >= 32) {
+            const uint64_t off = start + 16 * g - 32;
+            if (off + 16 <= n) {
+                v = *reinterpret_cast<const uint4 *>(seq + off);
+            } else if (off < n) {
+                uint32_t x[4] = {0, 0, 0, 0};
+                for (uint64_t b = off; b < n; ++b) x[(b - off) >> 2] |= (uint32_t)seq[b] << (8 * ((b - off) & 3));
+                v = make_uint4(x[0], x[1], x[2], x[3]);
+            }
+        }
+        w[4 * g] = v.x;
+        w[4 * g + 1] = v.y;
+        w[4 * g + 2] = v.z;
+        w[4 * g + 3] = v.w;
+    }
+}
+
 // Rolls thread t's segment of a staged tile and calls emit(key) for every
 // window whose last byte lies in the segment, at or after emit_from
 // (GetKmer.h:58-82 semantics; any invalid byte resets the window).

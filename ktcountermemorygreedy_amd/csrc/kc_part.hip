@@ -126,59 +126,66 @@ __global__ __launch_bounds__(kParseBlock) void k_front_scatter_seq(SeqSrc src, u
     }
 }
 
-// Front scatter with write combining (one-word keys): each thread rolls its
-// 64-byte segment in four 16-byte chunks; the block's keys of one chunk are
-// placed in LDS grouped by coarse digit and written out as contiguous runs.
+// Front scatter with write combining (one-word keys): each thread loads its
+// 64-byte segment (and 32 bytes of history) into registers and rolls it in
+// two 32-byte chunks; the block's keys of one chunk (up to 8192) are placed
+// in LDS grouped by coarse digit and written out as contiguous runs (about
+// 32 keys per digit).  No LDS copy of the input.
 __global__ __launch_bounds__(kParseBlock) void k_front_scatter_seq_staged(SeqSrc src, uint32_t k, Digits d,
                                                                           const uint64_t *__restrict__ o1,
                                                                           uint64_t *__restrict__ out)
 {
-    constexpr int kChunkBytes = 16;
+    constexpr int kChunkBytes = 32;
+    constexpr int kChunks = kParseSeg / kChunkBytes;
+    constexpr int kWords = kChunkBytes / 4;
     constexpr int kStage = kParseBlock * kChunkBytes;
-    __shared__ ParseLds lds;
     __shared__ uint64_t stage[kStage];
     __shared__ uint8_t sbin[kStage];
-    __shared__ uint32_t hist[256], loff[256], tmp[4];
+    __shared__ uint32_t hist[256], loff[256];
     __shared__ uint64_t base[256];
+    uint32_t *tmp = loff + 252;  // scan256's wave sums: loff is rewritten only after the scan
     const uint32_t t = threadIdx.x;
     const uint32_t nbins = 1u << d.l1;
     uint64_t n, emit_from;
     seq_bounds(src, n, emit_from);
     base[t] = t < nbins ? o1[(uint64_t)t * gridDim.x + blockIdx.x] : 0;
     hist[t] = 0;
+    __syncthreads();
     const uint64_t ntiles = (n + kParseTile - 1) / kParseTile;
     for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        stage_tile(src.seq, n, tile, lds);
         const uint64_t start = tile * kParseTile + (uint64_t)t * kParseSeg;
+        uint32_t w[24];
+        load_segment(src.seq, n, start, w);
         // segment bytes that emit nothing: before emit_from, or past the end
         const uint32_t skip = start >= n ? 64u
                             : emit_from > start ? (uint32_t)(emit_from - start < 64 ? emit_from - start : 64) : 0u;
         Roll<1> roll(k);
         uint32_t have = 0;
-        for (int j = 8; j < 16; ++j) {  // 32-byte halo from the previous segment
-            const uint32_t v = lds.w[j][t];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {  // 32-byte history from the previous segment
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
                 bool valid;
-                roll.push(base_code(v >> (8 * b), valid));
+                roll.push(base_code(w[j] >> (8 * b), valid));
                 have = valid ? have + 1 : 0;
             }
         }
-        for (int q = 0; q < 4; ++q) {
+#pragma unroll
+        for (int q = 0; q < kChunks; ++q) {
             uint64_t key[kChunkBytes];
             uint32_t rank[kChunkBytes];
             uint32_t bin[kChunkBytes];
-            uint32_t emitted = 0;
+            uint32_t emitted = 0;  // bit e: byte e of the chunk emitted a key
 #pragma unroll
-            for (int jj = 0; jj < 4; ++jj) {
-                const uint32_t v = lds.w[q * 4 + jj][t + 1];
+            for (int jj = 0; jj < kWords; ++jj) {
+                const uint32_t v = w[8 + q * kWords + jj];
 #pragma unroll
                 for (int b = 0; b < 4; ++b) {
                     const int e = jj * 4 + b;
                     bool valid;
                     roll.push(base_code(v >> (8 * b), valid));
                     have = valid ? have + 1 : 0;
-                    const bool emit = have >= k && (uint32_t)(4 * (q * 4 + jj) + b) >= skip;
+                    const bool emit = have >= k && (uint32_t)(4 * (q * kWords + jj) + b) >= skip;
                     if (emit) {
                         roll.canon(&key[e]);
                         bin[e] = coarse_of<1>(&key[e], d);
@@ -190,7 +197,8 @@ __global__ __launch_bounds__(kParseBlock) void k_front_scatter_seq_staged(SeqSrc
             __syncthreads();
             uint32_t total;
             const uint32_t h = hist[t];
-            loff[t] = scan256(h, tmp, total);
+            const uint32_t off = scan256(h, tmp, total);
+            loff[t] = off;
             __syncthreads();
 #pragma unroll
             for (int e = 0; e < kChunkBytes; ++e) {
