@@ -320,34 +320,59 @@ __global__ __launch_bounds__(kKeyBlock) void k_fine_count(const uint64_t *__rest
     for (uint32_t f = threadIdx.x; f < nf; f += kKeyBlock) m2[(row + f) * S + s] = hist[f];
 }
 
+// Exclusive scan of one value per thread of a TB-thread block: a shuffle
+// scan inside each wave, then the wave totals (two barriers).
+template <int TB>
+__device__ __forceinline__ uint32_t scan_block(uint32_t v, uint32_t *wsum, uint32_t &total)
+{
+    constexpr uint32_t W = TB / 64;
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= (uint32_t)o) x += y;
+    }
+    if (lane == 63) wsum[wave] = x;
+    __syncthreads();
+    uint32_t wbase = 0;
+    total = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < W; ++w) {
+        const uint32_t sw = wsum[w];
+        wbase += w < wave ? sw : 0;
+        total += sw;
+    }
+    __syncthreads();
+    return wbase + x - v;
+}
+
 // Fine scatter with write combining: a slice is handled in chunks of
-// kKeyBlock x FPT keys held in registers; each chunk's keys are ranked per
-// fine digit, placed in LDS grouped by digit and written out as runs.
-// NF = the largest fine digit count the instance serves (256, or 1024 for
-// partition counts above 2^16).  Chunks of 8192 keys (64 KB) give runs of
-// about 32 keys per digit: longer runs leave fewer partly written lines,
-// and two blocks per CU keep the open runs within L2.
-template <int NW, int NF>
-__global__ __launch_bounds__(kKeyBlock) void k_fine_scatter_staged(const uint64_t *__restrict__ keys,
-                                                                   const uint64_t *__restrict__ starts, Digits d,
-                                                                   uint32_t S, const uint64_t *__restrict__ o2,
-                                                                   uint64_t *__restrict__ out)
+// TB x FPT keys held in registers; each chunk's keys are ranked per fine
+// digit, placed in LDS grouped by digit and written out as runs.  Longer
+// runs leave fewer partly written lines: with 256 fine digits a chunk is
+// 16384 keys (one 1024-thread block per CU, ~64-key runs); NF = 1024 (more
+// than 2^16 partitions) uses 8192-key chunks in 256-thread blocks.
+template <int NW, int NF, int TB, int FPT>
+__global__ __launch_bounds__(TB) void k_fine_scatter_staged(const uint64_t *__restrict__ keys,
+                                                            const uint64_t *__restrict__ starts, Digits d,
+                                                            uint32_t S, const uint64_t *__restrict__ o2,
+                                                            uint64_t *__restrict__ out)
 {
     using BinT = typename std::conditional<NF <= 256, uint8_t, uint16_t>::type;
-    constexpr uint32_t FPT = NW == 1 ? 32 : 16;         // keys per thread per chunk
-    constexpr uint32_t CH = kKeyBlock * FPT;            // 8192 / 4096 keys (64 KB)
+    constexpr uint32_t CH = (uint32_t)TB * FPT;
     __shared__ uint64_t stage[CH * NW];
     __shared__ BinT sbin[CH];
     __shared__ uint32_t hist[NF], loff[NF];
     __shared__ uint64_t base[NF];
-    uint32_t *tmp = loff + NF - 4;  // scan256's wave sums: loff is rewritten only after the scan
+    uint32_t *tmp = loff + NF - TB / 64;  // the scan's wave sums: loff is rewritten only after the scan
     uint32_t c, s;
     uint64_t beg, end;
     slice_of(starts, S, c, s, beg, end);
     const uint32_t nf = 1u << d.l2;
     const uint32_t t = threadIdx.x;
     const uint64_t row = (uint64_t)c << d.l2;
-    for (uint32_t f = t; f < nf; f += kKeyBlock) {
+    for (uint32_t f = t; f < nf; f += TB) {
         base[f] = o2[(row + f) * S + s];
         hist[f] = 0;
     }
@@ -358,35 +383,35 @@ __global__ __launch_bounds__(kKeyBlock) void k_fine_scatter_staged(const uint64_
         uint32_t bin[FPT], rank[FPT];
         // all FPT loads first (index clamped into the chunk), then the ranks
 #pragma unroll
-        for (uint32_t j = 0; j < FPT; ++j) {
-            const uint32_t i = j * kKeyBlock + t;
+        for (uint32_t j = 0; j < (uint32_t)FPT; ++j) {
+            const uint32_t i = j * TB + t;
             const uint64_t *src = keys + (cs + (i < cn ? i : cn - 1)) * NW;
 #pragma unroll
             for (int w = 0; w < NW; ++w) key[j][w] = src[w];
         }
 #pragma unroll
-        for (uint32_t j = 0; j < FPT; ++j) {
-            const uint32_t i = j * kKeyBlock + t;
+        for (uint32_t j = 0; j < (uint32_t)FPT; ++j) {
+            const uint32_t i = j * TB + t;
             if (i < cn) {
                 bin[j] = fine_of(xxh3_64<NW>(key[j]), d);
                 rank[j] = atomicAdd(&hist[bin[j]], 1u);
             }
         }
         __syncthreads();
-        // exclusive scan of hist[0..nf): each thread owns nf/256 consecutive digits
-        const uint32_t per = (nf + 255) / 256;
+        // exclusive scan of hist[0..nf): each thread owns nf/TB consecutive digits
+        const uint32_t per = (nf + TB - 1) / TB;
         uint32_t acc = 0;
         for (uint32_t f = t * per; f < (t + 1) * per && f < nf; ++f) acc += hist[f];
         uint32_t total;
-        uint32_t run = scan256(acc, tmp, total);
+        uint32_t run = scan_block<TB>(acc, tmp, total);
         for (uint32_t f = t * per; f < (t + 1) * per && f < nf; ++f) {
             loff[f] = run;
             run += hist[f];
         }
         __syncthreads();
 #pragma unroll
-        for (uint32_t j = 0; j < FPT; ++j) {
-            const uint32_t i = j * kKeyBlock + t;
+        for (uint32_t j = 0; j < (uint32_t)FPT; ++j) {
+            const uint32_t i = j * TB + t;
             if (i < cn) {
                 const uint32_t p = loff[bin[j]] + rank[j];
 #pragma unroll
@@ -395,14 +420,14 @@ __global__ __launch_bounds__(kKeyBlock) void k_fine_scatter_staged(const uint64_
             }
         }
         __syncthreads();
-        for (uint32_t p = t; p < cn; p += kKeyBlock) {
+        for (uint32_t p = t; p < cn; p += TB) {
             const uint32_t f = sbin[p];
             const uint64_t g = base[f] + (p - loff[f]);
 #pragma unroll
             for (int w = 0; w < NW; ++w) out[g * NW + w] = stage[p * NW + w];
         }
         __syncthreads();
-        for (uint32_t f = t; f < nf; f += kKeyBlock) {
+        for (uint32_t f = t; f < nf; f += TB) {
             base[f] += hist[f];
             hist[f] = 0;
         }
@@ -713,11 +738,11 @@ int kc_part_enqueue(const kc_table *t, kc_part_scratch &s, const kc_pipe_shape &
         KC_HIP(hipGetLastError());
         if ((rc = kc_scan_u32_on(st, s.m2, m2n, s.o2, s.sums))) return rc;
         if (d.l2 <= 8) {
-            if (nw == 1) hipLaunchKernelGGL((k_fine_scatter_staged<1, 256>), g2, dim3(kKeyBlock), 0, st, s.keys_a, s.starts, d, S, s.o2, s.keys_b);
-            else hipLaunchKernelGGL((k_fine_scatter_staged<2, 256>), g2, dim3(kKeyBlock), 0, st, s.keys_a, s.starts, d, S, s.o2, s.keys_b);
+            if (nw == 1) hipLaunchKernelGGL((k_fine_scatter_staged<1, 256, 1024, 16>), g2, dim3(1024), 0, st, s.keys_a, s.starts, d, S, s.o2, s.keys_b);
+            else hipLaunchKernelGGL((k_fine_scatter_staged<2, 256, 1024, 8>), g2, dim3(1024), 0, st, s.keys_a, s.starts, d, S, s.o2, s.keys_b);
         } else {
-            if (nw == 1) hipLaunchKernelGGL((k_fine_scatter_staged<1, 1024>), g2, dim3(kKeyBlock), 0, st, s.keys_a, s.starts, d, S, s.o2, s.keys_b);
-            else hipLaunchKernelGGL((k_fine_scatter_staged<2, 1024>), g2, dim3(kKeyBlock), 0, st, s.keys_a, s.starts, d, S, s.o2, s.keys_b);
+            if (nw == 1) hipLaunchKernelGGL((k_fine_scatter_staged<1, 1024, 256, 32>), g2, dim3(256), 0, st, s.keys_a, s.starts, d, S, s.o2, s.keys_b);
+            else hipLaunchKernelGGL((k_fine_scatter_staged<2, 1024, 256, 16>), g2, dim3(256), 0, st, s.keys_a, s.starts, d, S, s.o2, s.keys_b);
         }
         KC_HIP(hipGetLastError());
         part_keys = s.keys_b;
