@@ -41,6 +41,12 @@ constexpr int kKeyBlock = 256;
 constexpr int kKeysPerThread = 16;
 constexpr uint64_t kKeyTile = (uint64_t)kKeyBlock * kKeysPerThread;  // keys per front-end tile
 constexpr int kPartThreads = 1024;
+// Front scatter blocks of one-word keys: 256 threads (the count pass's
+// tiles), 32-byte chunks (8192-key chunks, ~32-key runs, two blocks per CU).
+// 1024-thread blocks with 16-byte chunks (longer runs, one block per CU)
+// measured slower on this ALU-heavy pass.
+constexpr int kFrontBlock1 = kParseBlock;
+constexpr int kFrontChunk1 = 32;
 
 struct Digits {
     uint32_t part_bits;  // L
@@ -73,6 +79,33 @@ __device__ __forceinline__ uint32_t scan256(uint32_t v, uint32_t *wsum, uint32_t
     total = 0;
 #pragma unroll
     for (uint32_t w = 0; w < 4; ++w) {
+        const uint32_t sw = wsum[w];
+        wbase += w < wave ? sw : 0;
+        total += sw;
+    }
+    __syncthreads();
+    return wbase + x - v;
+}
+
+// Exclusive scan of one value per thread of a TB-thread block: a shuffle
+// scan inside each wave, then the wave totals (two barriers).
+template <int TB>
+__device__ __forceinline__ uint32_t scan_block(uint32_t v, uint32_t *wsum, uint32_t &total)
+{
+    constexpr uint32_t W = TB / 64;
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= (uint32_t)o) x += y;
+    }
+    if (lane == 63) wsum[wave] = x;
+    __syncthreads();
+    uint32_t wbase = 0;
+    total = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < W; ++w) {
         const uint32_t sw = wsum[w];
         wbase += w < wave ? sw : 0;
         total += sw;
@@ -126,34 +159,39 @@ __global__ __launch_bounds__(kParseBlock) void k_front_scatter_seq(SeqSrc src, u
     }
 }
 
-// Front scatter with write combining (one-word keys): each thread loads its
-// 64-byte segment (and 32 bytes of history) into registers and rolls it in
-// two 32-byte chunks; the block's keys of one chunk (up to 8192) are placed
-// in LDS grouped by coarse digit and written out as contiguous runs (about
-// 32 keys per digit).  No LDS copy of the input.
-__global__ __launch_bounds__(kParseBlock) void k_front_scatter_seq_staged(SeqSrc src, uint32_t k, Digits d,
-                                                                          const uint64_t *__restrict__ o1,
-                                                                          uint64_t *__restrict__ out)
+// Front scatter for one-word keys in TB-thread blocks: a tile is TB x 64
+// bytes (16 KB at TB = 256, the tiles of k_front_count_seq); each thread
+// loads its 64-byte segment (and 32 bytes of history) straight into
+// registers.
+// Front scatter with write combining (one-word keys): the segment is rolled
+// in chunks of CB bytes; the block's keys of one chunk (up to TB x CB) are
+// placed in LDS grouped by coarse digit and written out as contiguous runs
+// (TB x CB / 256 keys per digit per chunk on average).
+template <int TB, int CB>
+__global__ __launch_bounds__(TB) void k_front_scatter_seq_staged(SeqSrc src, uint32_t k, Digits d,
+                                                                 const uint64_t *__restrict__ o1,
+                                                                 uint64_t *__restrict__ out)
 {
-    constexpr int kChunkBytes = 32;
-    constexpr int kChunks = kParseSeg / kChunkBytes;
-    constexpr int kWords = kChunkBytes / 4;
-    constexpr int kStage = kParseBlock * kChunkBytes;
+    constexpr int kChunks = kParseSeg / CB;
+    constexpr int kWords = CB / 4;
+    constexpr int kStage = TB * CB;
     __shared__ uint64_t stage[kStage];
     __shared__ uint8_t sbin[kStage];
-    __shared__ uint32_t hist[256], loff[256];
+    __shared__ uint32_t hist[256], loff[256], wsum[TB / 64];
     __shared__ uint64_t base[256];
-    uint32_t *tmp = loff + 252;  // scan256's wave sums: loff is rewritten only after the scan
     const uint32_t t = threadIdx.x;
     const uint32_t nbins = 1u << d.l1;
     uint64_t n, emit_from;
     seq_bounds(src, n, emit_from);
-    base[t] = t < nbins ? o1[(uint64_t)t * gridDim.x + blockIdx.x] : 0;
-    hist[t] = 0;
+    for (uint32_t f = t; f < 256; f += TB) {
+        base[f] = f < nbins ? o1[(uint64_t)f * gridDim.x + blockIdx.x] : 0;
+        hist[f] = 0;
+    }
     __syncthreads();
-    const uint64_t ntiles = (n + kParseTile - 1) / kParseTile;
+    constexpr uint64_t kTileBytes = (uint64_t)TB * kParseSeg;
+    const uint64_t ntiles = (n + kTileBytes - 1) / kTileBytes;
     for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        const uint64_t start = tile * kParseTile + (uint64_t)t * kParseSeg;
+        const uint64_t start = tile * kTileBytes + (uint64_t)t * kParseSeg;
         uint32_t w[24];
         load_segment(src.seq, n, start, w);
         // segment bytes that emit nothing: before emit_from, or past the end
@@ -172,9 +210,8 @@ __global__ __launch_bounds__(kParseBlock) void k_front_scatter_seq_staged(SeqSrc
         }
 #pragma unroll
         for (int q = 0; q < kChunks; ++q) {
-            uint64_t key[kChunkBytes];
-            uint32_t rank[kChunkBytes];
-            uint32_t bin[kChunkBytes];
+            uint64_t key[CB];
+            uint32_t rb[CB];       // rank << 8 | coarse digit
             uint32_t emitted = 0;  // bit e: byte e of the chunk emitted a key
 #pragma unroll
             for (int jj = 0; jj < kWords; ++jj) {
@@ -188,34 +225,36 @@ __global__ __launch_bounds__(kParseBlock) void k_front_scatter_seq_staged(SeqSrc
                     const bool emit = have >= k && (uint32_t)(4 * (q * kWords + jj) + b) >= skip;
                     if (emit) {
                         roll.canon(&key[e]);
-                        bin[e] = coarse_of<1>(&key[e], d);
-                        rank[e] = atomicAdd(&hist[bin[e]], 1u);
+                        const uint32_t c = coarse_of<1>(&key[e], d);
+                        rb[e] = (atomicAdd(&hist[c], 1u) << 8) | c;
                         emitted |= 1u << e;
                     }
                 }
             }
             __syncthreads();
             uint32_t total;
-            const uint32_t h = hist[t];
-            const uint32_t off = scan256(h, tmp, total);
-            loff[t] = off;
+            const uint32_t h = t < 256 ? hist[t] : 0;
+            const uint32_t off = scan_block<TB>(h, wsum, total);
+            if (t < 256) loff[t] = off;
             __syncthreads();
 #pragma unroll
-            for (int e = 0; e < kChunkBytes; ++e) {
+            for (int e = 0; e < CB; ++e) {
                 if (emitted & (1u << e)) {
-                    const uint32_t p = loff[bin[e]] + rank[e];
+                    const uint32_t p = loff[rb[e] & 0xFF] + (rb[e] >> 8);
                     stage[p] = key[e];
-                    sbin[p] = (uint8_t)bin[e];
+                    sbin[p] = (uint8_t)rb[e];
                 }
             }
             __syncthreads();
-            for (uint32_t i = t; i < total; i += kParseBlock) {
+            for (uint32_t i = t; i < total; i += TB) {
                 const uint32_t c = sbin[i];
                 out[base[c] + (i - loff[c])] = stage[i];
             }
             __syncthreads();
-            base[t] += h;
-            hist[t] = 0;
+            if (t < 256) {
+                base[t] += h;
+                hist[t] = 0;
+            }
             __syncthreads();
         }
     }
@@ -318,33 +357,6 @@ __global__ __launch_bounds__(kKeyBlock) void k_fine_count(const uint64_t *__rest
     __syncthreads();
     const uint64_t row = (uint64_t)c << d.l2;
     for (uint32_t f = threadIdx.x; f < nf; f += kKeyBlock) m2[(row + f) * S + s] = hist[f];
-}
-
-// Exclusive scan of one value per thread of a TB-thread block: a shuffle
-// scan inside each wave, then the wave totals (two barriers).
-template <int TB>
-__device__ __forceinline__ uint32_t scan_block(uint32_t v, uint32_t *wsum, uint32_t &total)
-{
-    constexpr uint32_t W = TB / 64;
-    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    uint32_t x = v;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t y = __shfl_up(x, o, 64);
-        if (lane >= (uint32_t)o) x += y;
-    }
-    if (lane == 63) wsum[wave] = x;
-    __syncthreads();
-    uint32_t wbase = 0;
-    total = 0;
-#pragma unroll
-    for (uint32_t w = 0; w < W; ++w) {
-        const uint32_t sw = wsum[w];
-        wbase += w < wave ? sw : 0;
-        total += sw;
-    }
-    __syncthreads();
-    return wbase + x - v;
 }
 
 // Fine scatter with write combining: a slice is handled in chunks of
@@ -721,7 +733,7 @@ int kc_part_enqueue(const kc_table *t, kc_part_scratch &s, const kc_pipe_shape &
     if ((rc = kc_scan_u32_on(st, s.m1, m1n, s.o1, s.sums))) return rc;
     hipLaunchKernelGGL(k_coarse_starts, dim3(1), dim3(256), 0, st, s.o1, s.m1, G, n_coarse, s.starts);
     if (from_seq) {
-        if (nw == 1) hipLaunchKernelGGL(k_front_scatter_seq_staged, dim3(G), dim3(kParseBlock), 0, st, seq, t->cfg.k, d, s.o1, s.keys_a);
+        if (nw == 1) hipLaunchKernelGGL((k_front_scatter_seq_staged<kFrontBlock1, kFrontChunk1>), dim3(G), dim3(kFrontBlock1), 0, st, seq, t->cfg.k, d, s.o1, s.keys_a);
         else hipLaunchKernelGGL(k_front_scatter_seq<2>, dim3(G), dim3(kParseBlock), 0, st, seq, t->cfg.k, d, s.o1, s.keys_a);
     } else {
         if (nw == 1) hipLaunchKernelGGL(k_front_scatter_keys<1>, dim3(G), dim3(kKeyBlock), 0, st, d_keys, n_keys, d, s.o1, s.keys_a);
