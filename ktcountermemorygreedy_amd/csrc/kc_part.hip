@@ -535,9 +535,24 @@ __device__ __forceinline__ bool lds_insert(PartLds<NW, BLOOM> &S, const uint64_t
     return false;
 }
 
+// Position of the n-th key counted by wave `wave` of a partition block (the
+// block takes 1024 consecutive keys per step, wave w the w-th 64 of them).
+__device__ __forceinline__ uint64_t wave_pos(uint64_t beg, uint32_t wave, uint32_t n)
+{
+    return beg + (uint64_t)(n >> 6) * kPartThreads + wave * 64u + (n & 63u);
+}
+
+// FIRST occurrences found by a wave are staged in `first_out` (the pipeline's
+// other key array, idle during this pass) at the positions of the wave's own
+// keys: the wave's n-th FIRST goes where its n-th key was, so runs of
+// different waves and partitions never overlap and no atomic is needed while
+// counting.  After the loop each wave moves its run to the FIRST list in
+// pieces of up to 1024 keys, one cursor atomic per piece, on segments that
+// rotate with the partition, the wave and the piece.
 template <int NW, bool BLOOM>
 __global__ __launch_bounds__(kPartThreads) void k_count_parts(const uint64_t *__restrict__ keys,
-                                                              const uint64_t *__restrict__ part_off, TableView tv)
+                                                              const uint64_t *__restrict__ part_off, TableView tv,
+                                                              uint64_t *__restrict__ first_out)
 {
     constexpr uint32_t PS = PartLds<NW, BLOOM>::PS;
     constexpr int SW = slot_words<NW>();
@@ -562,7 +577,9 @@ __global__ __launch_bounds__(kPartThreads) void k_count_parts(const uint64_t *__
     }
     __syncthreads();
 
-    uint32_t n_k = 0, n_new = 0, n_err = 0, step = 0;
+    uint32_t n_k = 0, n_new = 0, n_err = 0;
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    uint32_t fcur = 0;  // FIRSTs staged by this wave (wave-uniform)
     // the next step's key is loaded while this step's key is counted
     uint64_t nxt[NW];
     {
@@ -570,7 +587,7 @@ __global__ __launch_bounds__(kPartThreads) void k_count_parts(const uint64_t *__
 #pragma unroll
         for (int w = 0; w < NW; ++w) nxt[w] = keys[(i < end ? i : beg) * NW + w];
     }
-    for (uint64_t base = beg; base < end; base += kPartThreads, ++step) {
+    for (uint64_t base = beg; base < end; base += kPartThreads) {
         const uint64_t i = base + threadIdx.x;
         const bool valid = i < end;
         uint64_t key[NW];
@@ -587,10 +604,37 @@ __global__ __launch_bounds__(kPartThreads) void k_count_parts(const uint64_t *__
         bool to_table = valid;
         if constexpr (BLOOM) {
             const bool first = valid && lds_bloom<NW, BLOOM>(S, lo, hi, bsz - 1) == 0;
-            first_append<NW>(tv, first, key, n_err, step);
+            const uint64_t fb = __ballot(first);
+            if (first) {
+                uint64_t *dst = first_out + wave_pos(beg, wave, fcur + (uint32_t)__popcll(fb & lanemask_lt())) * NW;
+#pragma unroll
+                for (int w = 0; w < NW; ++w) dst[w] = key[w];
+            }
+            fcur += (uint32_t)__popcll(fb);
             to_table = valid && !first;
         }
         if (to_table) lds_insert<NW, BLOOM>(S, key, hi, psz - 1, n_new, n_err);
+    }
+    if constexpr (BLOOM) {
+        __threadfence_block();  // the wave's staged FIRSTs are read back by other lanes
+        for (uint32_t p0 = 0; p0 < fcur; p0 += 1024) {
+            const uint32_t cnt = fcur - p0 < 1024 ? fcur - p0 : 1024;
+            const uint32_t seg = (uint32_t)(part + 16u * wave + p0 / 1024) % KC_LOW_SEGS;
+            unsigned long long at = 0;
+            if (lane == 0)
+                at = atomicAdd(tv.counters + KC_CURSOR_BASE + (uint64_t)seg * KC_LINE_WORDS, (unsigned long long)cnt);
+            at = __shfl(at, 0, 64);
+            for (uint32_t n = lane; n < cnt; n += 64) {
+                if (at + n < tv.seg_cap) {
+                    const uint64_t *src = first_out + wave_pos(beg, wave, p0 + n) * NW;
+                    uint64_t *dst = tv.low + ((uint64_t)seg * tv.seg_cap + at + n) * NW;
+#pragma unroll
+                    for (int w = 0; w < NW; ++w) dst[w] = src[w];
+                } else {
+                    ++n_err;
+                }
+            }
+        }
     }
     __syncthreads();
     for (uint32_t s = threadIdx.x; s < psz; s += kPartThreads) {
@@ -688,7 +732,7 @@ int kc_part_reserve(const kc_table *t, kc_part_scratch &s, const kc_pipe_shape &
     const uint64_t keys_words = std::max<uint64_t>(sh.key_cap, 1) * nw;
     kc_a *= nw;
     kc_b *= nw;
-    const bool ok = grow(s.keys_a, kc_a, keys_words) && (d.l2 == 0 || grow(s.keys_b, kc_b, keys_words)) &&
+    const bool ok = grow(s.keys_a, kc_a, keys_words) && grow(s.keys_b, kc_b, keys_words) &&
                     grow(s.m1, m1c, m1n) && grow(s.o1, o1c, m1n) && (d.l2 == 0 || grow(s.m2, m2c, m2n)) &&
                     (d.l2 == 0 || grow(s.o2, o2c, m2n)) && grow(s.sums, sc, sumsn) &&
                     grow(s.part_off, pc, n_parts + 1) && grow(s.starts, stc, 257);
@@ -696,7 +740,7 @@ int kc_part_reserve(const kc_table *t, kc_part_scratch &s, const kc_pipe_shape &
         kc_part_scratch_free(s);
         return KC_ENOMEM;
     }
-    s.key_cap = std::min(kc_a, d.l2 ? kc_b : kc_a) / nw;
+    s.key_cap = std::min(kc_a, kc_b) / nw;
     s.m1_n = std::min(m1c, o1c);
     s.m2_n = d.l2 ? std::min(m2c, o2c) : s.m2_n;
     s.sums_n = sc;
@@ -766,12 +810,13 @@ int kc_part_enqueue(const kc_table *t, kc_part_scratch &s, const kc_pipe_shape &
     // per-partition LDS counting
     const bool bloom = t->bloom && !t->sealed;
     const dim3 g((uint32_t)n_parts);
+    uint64_t *first_out = part_keys == s.keys_b ? s.keys_a : s.keys_b;  // the idle key array
     if (nw == 1) {
-        if (bloom) hipLaunchKernelGGL((k_count_parts<1, true>), g, dim3(kPartThreads), 0, st, part_keys, s.part_off, tv);
-        else hipLaunchKernelGGL((k_count_parts<1, false>), g, dim3(kPartThreads), 0, st, part_keys, s.part_off, tv);
+        if (bloom) hipLaunchKernelGGL((k_count_parts<1, true>), g, dim3(kPartThreads), 0, st, part_keys, s.part_off, tv, first_out);
+        else hipLaunchKernelGGL((k_count_parts<1, false>), g, dim3(kPartThreads), 0, st, part_keys, s.part_off, tv, first_out);
     } else {
-        if (bloom) hipLaunchKernelGGL((k_count_parts<2, true>), g, dim3(kPartThreads), 0, st, part_keys, s.part_off, tv);
-        else hipLaunchKernelGGL((k_count_parts<2, false>), g, dim3(kPartThreads), 0, st, part_keys, s.part_off, tv);
+        if (bloom) hipLaunchKernelGGL((k_count_parts<2, true>), g, dim3(kPartThreads), 0, st, part_keys, s.part_off, tv, first_out);
+        else hipLaunchKernelGGL((k_count_parts<2, false>), g, dim3(kPartThreads), 0, st, part_keys, s.part_off, tv, first_out);
     }
     KC_HIP(hipGetLastError());
     return KC_OK;
