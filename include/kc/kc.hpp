@@ -8,6 +8,10 @@
 //   kc::export_record_radix_sort<N>      ExportRecordRadixSort.h:256-269
 //   kc::radix_sort_export_records_12bit  ExportRecordRadixSort12.h:639-646
 //   kc::ExportWriter<N>                  ExportWriter.h:70-313 (low.bin)
+//   kc::ReadBatch                        batched counting of uneven read
+//                                        chunks (kc_batch; no reference
+//                                        counterpart: FastqReader.h:542-557
+//                                        feeds fixed 32 KiB blocks)
 //
 // Errors surface as kc::Error (an std::runtime_error carrying the status).
 #pragma once
@@ -159,12 +163,59 @@ public:
         return bins;
     }
 
+    // Framed bytes already in HBM (asynchronous on the context's stream).
+    void count_device(const DeviceBuffer &seq, uint64_t bytes)
+    {
+        check(kc_table_count_seq(t_, static_cast<const uint8_t *>(seq.get()), bytes), "kc_table_count_seq");
+    }
+
+    // Launch-shape search on a device-resident sample; leaves the table empty.
+    kc_tune autotune(const DeviceBuffer &sample, uint64_t bytes, uint32_t reps = 1)
+    {
+        kc_tune t{};
+        float ms = 0;
+        check(kc_autotune(t_, static_cast<const uint8_t *>(sample.get()), bytes, reps, &t, &ms), "kc_autotune");
+        return t;
+    }
+
+    void set_tune(const kc_tune &t) { check(kc_table_set_tune(t_, &t), "kc_table_set_tune"); }
+    void clear() { check(kc_table_clear(t_), "kc_table_clear"); }
+
     kc_table *get() const { return t_; }
 
 private:
     Context &c_;
     uint32_t k_;
     kc_table *t_ = nullptr;
+};
+
+// Many read chunks of uneven length counted into a table, one captured-graph
+// launch per staging slot (kc_batch).  Destroy before the table.
+class ReadBatch {
+public:
+    template <uint32_t N>
+    explicit ReadBatch(ConcurrentCountingHashMap<N> &table, const kc_tune *tune = nullptr)
+    {
+        check(kc_batch_create(table.get(), tune, &b_), "kc_batch_create");
+    }
+    ~ReadBatch() { kc_batch_destroy(b_); }
+    ReadBatch(const ReadBatch &) = delete;
+    ReadBatch &operator=(const ReadBatch &) = delete;
+    void add(const char *chunk, uint64_t bytes)
+    {
+        check(kc_batch_add(b_, reinterpret_cast<const uint8_t *>(chunk), bytes), "kc_batch_add");
+    }
+    void flush() { check(kc_batch_flush(b_), "kc_batch_flush"); }
+    void sync() { check(kc_batch_sync(b_), "kc_batch_sync"); }
+    kc_batch_info info() const
+    {
+        kc_batch_info i;
+        check(kc_batch_get_info(b_, &i), "kc_batch_get_info");
+        return i;
+    }
+
+private:
+    kc_batch *b_ = nullptr;
 };
 
 namespace detail {

@@ -1,11 +1,13 @@
 // Exercises the C++ adaptors of include/kc/kc.hpp on the GPU: counting via
 // increment, point queries, both export sorts against a std::stable_sort on
-// the key bit range, the 12-bit sort's argument checks.  Prints "ok".
+// the key bit range, batched counting of uneven read chunks against counting
+// the whole stream, the 12-bit sort's argument checks.  Prints "ok".
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <random>
 #include <stdexcept>
+#include <string>
 #include <vector>
 
 #include "kc/kc.hpp"
@@ -57,6 +59,38 @@ int main()
     auto *res = kc::export_record_radix_sort<1>(ctx, r8.data(), tmp.data(), r8.size(), 31);
     for (size_t i = 0; i < r8.size(); ++i)
         if (res[i].count != want8[i].count) { std::printf("8-bit sort mismatch at %zu\n", i); return 1; }
+
+    // batched counting of uneven read chunks equals counting the whole stream
+    {
+        std::string genome(200000, 'A');
+        for (auto &c : genome) c = "ACGT"[rng() & 3];
+        std::string all;
+        std::vector<std::string> chunks;
+        for (int r = 0; r < 4000; ++r) {
+            const size_t len = 100 + rng() % 900, pos = rng() % (genome.size() - len);
+            std::string read = genome.substr(pos, len) + "\n";
+            if (chunks.empty() || rng() % 4 == 0) chunks.emplace_back();
+            chunks.back() += read;
+            all += read;
+        }
+        kc::ConcurrentCountingHashMap<1> whole(ctx, 31, 1 << 22, 0xFFFFFFFFu, true);
+        whole.count_reads(all.data(), all.size());
+        kc::ConcurrentCountingHashMap<1> batched(ctx, 31, 1 << 22, 0xFFFFFFFFu, true);
+        kc_tune tune{};
+        tune.batch_bytes = 1 << 20;
+        tune.queue_depth = 3;
+        {
+            kc::ReadBatch b(batched, &tune);
+            for (const auto &c : chunks) b.add(c.data(), c.size());
+            b.flush();
+            b.sync();
+            if (b.info().launches < 2) { std::printf("expected several batch launches\n"); return 1; }
+        }
+        if (whole.histogram(1, 1000) != batched.histogram(1, 1000) || whole.stats().kmers != batched.stats().kmers) {
+            std::printf("batched counting differs\n");
+            return 1;
+        }
+    }
 
     bool threw = false;
     try { kc::radix_sort_export_records_12bit<1, 10>(ctx, recs, scratch, 9); } catch (const std::invalid_argument &) { threw = true; }
