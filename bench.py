@@ -37,7 +37,7 @@ def parse_args():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="repeat_100mbp",
-                    choices=["repeat_100mbp", "count_5mbp", "uneven_50mbp", "export_sort", "serving"])
+                    choices=["repeat_100mbp", "count_5mbp", "uneven_50mbp", "export_sort", "serving", "ingest_gz"])
     ap.add_argument("--k", type=int, default=31, help="k-mer length (uneven_50mbp is quoted at 21, 31 and 63)")
     ap.add_argument("--serve-seconds", type=float, default=5.0, help="serving: length of the schedule")
     ap.add_argument("--records-log2", type=int, default=30, help="export_sort: 2^N records")
@@ -190,6 +190,94 @@ def bench_sort(args):
     ctx.close()
 
 
+def bench_ingest(args):
+    """End-to-end from gzip FASTQ: the first reads of repeat_100mbp (default
+    4M) as 16 .fastq.gz files (level 6), counted by kc_count_fastq_files
+    (16 host threads inflate and frame in parallel into a kc_batch) -- host
+    inflate included, so this is the PCIe- and zlib-inclusive rate, never the
+    headline.  The reference Tree binary counts the same 16 files on the same
+    host beside it."""
+    import gzip
+    import shutil
+    from concurrent.futures import ThreadPoolExecutor
+
+    import numpy as np
+
+    from ktcountermemorygreedy_amd import kc
+    from ktcountermemorygreedy_amd import workloads as W
+
+    n_reads = args.reads or 4_000_000
+    n_files = 16
+    ws = W.REPEAT_100MBP.scaled(n_reads, "repeat_100mbp_gz_%d" % n_reads)
+    genome = ws.genome()
+    t0 = time.time()
+    fq = ws.reads(fastq=True, genome=genome)
+    tmp = tempfile.mkdtemp(prefix="kcingest_")
+    # cut at record boundaries (4 lines per record)
+    nl = np.flatnonzero(fq == 10)
+    rec_end = nl[3::4] + 1
+    bounds = [0] + [int(rec_end[min(len(rec_end) - 1, (i + 1) * len(rec_end) // n_files - 1)])
+                    for i in range(n_files)]
+    paths = [os.path.join(tmp, "part%02d.fastq.gz" % i) for i in range(n_files)]
+
+    def write(i):
+        with open(paths[i], "wb") as f:
+            f.write(gzip.compress(fq[bounds[i]:bounds[i + 1]].tobytes(), compresslevel=6))
+
+    with ThreadPoolExecutor(n_files) as ex:
+        list(ex.map(write, range(n_files)))
+    gz_bytes = sum(os.path.getsize(p) for p in paths)
+    prep_s = time.time() - t0
+    framed_len = int(kc.fastq_frame(fq[:bounds[-1]].tobytes()).__len__())
+    del fq
+
+    ctx = kc.Context(0)
+    est_distinct = int(ws.genome_len * 1.1 + n_reads * 120 * 0.15)
+    cap = 1 << max(20, int(np.ceil(np.log2(max(ws.genome_len * 1.1, est_distinct / 3.0) * 1.8))))
+    table = kc.Table(ctx, 31, count_max=0xFFFFFFFF, capacity=cap, max_distinct=est_distinct, bloom=True,
+                     low_capacity=est_distinct)
+    runs = []
+    for _ in range(max(1, args.steps)):
+        table.clear()
+        t1 = time.perf_counter()
+        st = kc.count_fastq_files(table, paths, threads=n_files, batch_bytes=256 << 20, queue_depth=3)
+        runs.append((time.perf_counter() - t1, st))
+    wall, st = min(runs, key=lambda r: r[0])
+    stats = table.stats()
+    assert stats["errors"] == 0, stats
+    cpu = None
+    ref = os.path.join(ROOT, "oracle", "_ref", "kmc_ref")
+    if not args.no_cpu_baseline and os.path.exists(ref):
+        out_dir = os.path.join(tmp, "out")
+        os.makedirs(out_dir)
+        t1 = time.time()
+        out = subprocess.run([ref, ",".join(paths), "31", str(n_files), "16", out_dir + "/", "1024", "2",
+                              "4294967295", "4294967295"], stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                             text=True, timeout=900).stdout
+        ref_wall = time.time() - t1
+        kmers = int(re.search(r"Total read k-mer count: (\d+)", out).group(1))
+        us = int(re.search(r"Total elapsed us: (\d+)", out).group(1))
+        cpu = {"value": kmers / (us * 1e-6), "unit": "k-mers/s", "cores": n_files, "kind": "reference",
+               "sample": "the same %d .fastq.gz files, Tree <f1,...,f16> 31 %d 16 <tmp> 1024 2 4294967295 "
+                         "4294967295; elapsed %.2f s (wall %.2f s); k-mers %s GPU" %
+                         (n_files, n_files, us * 1e-6, ref_wall, "==" if kmers == stats["kmers"] else "!=")}
+    shutil.rmtree(tmp, ignore_errors=True)
+    line = {
+        "metric": "canonical 31-mers counted/s end to end from gzip FASTQ (host inflate + PCIe + count)",
+        "value": stats["kmers"] / wall, "unit": "k-mers/s", "n_gpus": 1, "steps": len(runs), "warmup": 0,
+        "ms_per_step": wall * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u64", "data": "synthetic",
+        "config": {"workload": ws.name, "reads": n_reads, "files": n_files, "gzip_level": 6, "gz_bytes": gz_bytes,
+                   "framed_bytes": framed_len, "kmers": stats["kmers"], "host_threads": n_files,
+                   "batch_bytes": 256 << 20, "queue_depth": 3},
+        "ingest": st, "roofline": None, "cpu_baseline": cpu, "prep_seconds": prep_s,
+        "checks": {"framed_bytes_match": st["framed_bytes"] == framed_len},
+    }
+    print(json.dumps(line), flush=True)
+    table.close()
+    ctx.close()
+
+
 def serving_cpu_baseline(n_keys, n_batches):
     """The reference's point lookup (FlatConcurrentHashMap find_prepared with
     its query pool's prefetch loop) on this host: oracle/_ref/query_bench_ref."""
@@ -258,6 +346,8 @@ def main():
         return bench_sort(args)
     if args.workload == "serving":
         return bench_serving(args)
+    if args.workload == "ingest_gz":
+        return bench_ingest(args)
     import numpy as np
 
     from ktcountermemorygreedy_amd import kc

@@ -24,6 +24,28 @@ uint64_t kc_fastq_frame(const uint8_t *fq, uint64_t len, uint8_t *out);
 int kc_fastq_frame_file(const char *path, uint8_t **out, uint64_t *out_len);
 void kc_host_free(void *p);
 
+/* Streaming ingest: counts the reads of n_paths FASTQ files (plain or .gz)
+ * into table t.  `threads` host threads take files in turn, inflate each with
+ * zlib and frame it (FastqReader.h:478-487, 575-616), and hand whole reads in
+ * chunks of about 4 MB to a kc_batch built with `tune` (NULL: the table's
+ * tune): the chunks go straight into the batch's page-locked staging slots
+ * and each full slot is counted by one graph launch while inflating goes on.
+ * Several files inflate in parallel (the reference reads a directory or a
+ * comma-separated file list, main.cpp:501-530).  Returns when every read is
+ * counted.  stats may be NULL. */
+typedef struct kc_ingest_stats {
+    uint64_t files;
+    uint64_t threads;
+    uint64_t input_bytes;     /* decompressed FASTQ bytes */
+    uint64_t framed_bytes;    /* "seq\n" bytes counted */
+    uint64_t chunks;
+    uint64_t launches;        /* graph launches of the batch */
+    double inflate_seconds;   /* summed over threads */
+    double wall_seconds;
+} kc_ingest_stats;
+int kc_count_fastq_files(kc_table *t, const char *const *paths, uint32_t n_paths, uint32_t threads,
+                         const kc_tune *tune, kc_ingest_stats *stats);
+
 typedef struct kc_genome_spec {
     uint64_t seed;
     uint64_t length;            /* bases */

@@ -1,11 +1,14 @@
-// Host ingest (FASTQ framing, zlib inflate) and the seeded synthetic
-// workload generator.
+// Host ingest (FASTQ framing, zlib inflate, streaming multi-file ingest into
+// batches) and the seeded synthetic workload generator.
 #include <zlib.h>
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -283,4 +286,122 @@ extern "C" int64_t kc_synth_reads(const uint8_t *genome, uint64_t glen, const kc
         }
     }, 1);
     return (int64_t)boff[nblk];
+}
+
+// ---- streaming ingest: FASTQ(.gz) files -> framed chunks -> kc_batch -------
+// Host threads take files from a shared cursor; each inflates its file with
+// zlib (gzread transparently reads plain files too, as FastqReader.h:575-616
+// does for ".gz"), frames it (line 2 of every record + '\n') into a local
+// buffer, and hands whole reads to the batch in chunks of about kChunk bytes.
+// The batch copies each chunk into its page-locked staging slot, and every
+// full slot is counted by one graph launch while the threads keep inflating.
+namespace {
+
+constexpr uint64_t kIngestChunk = 4ull << 20;
+constexpr unsigned kGzRead = 1u << 20;
+
+}  // namespace
+
+extern "C" int kc_count_fastq_files(kc_table *t, const char *const *paths, uint32_t n_paths, uint32_t threads,
+                                    const kc_tune *tune, kc_ingest_stats *stats)
+{
+    if (!t || (n_paths && !paths)) return KC_EINVAL;
+    for (uint32_t i = 0; i < n_paths; ++i)
+        if (!paths[i]) return KC_EINVAL;
+    kc_batch *b = nullptr;
+    int rc = kc_batch_create(t, tune, &b);
+    if (rc != KC_OK) return rc;
+    const auto t0 = std::chrono::steady_clock::now();
+    std::mutex mu;  // guards the batch and the shared results
+    std::atomic<uint32_t> next{0};
+    int first_error = KC_OK;
+    uint64_t in_bytes = 0, framed = 0, chunks = 0;
+    double inflate_s = 0;
+    auto worker = [&]() {
+        std::vector<uint8_t> in(kGzRead), buf(kIngestChunk + kGzRead);
+        for (;;) {
+            const uint32_t f = next.fetch_add(1);
+            if (f >= n_paths) return;
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                if (first_error != KC_OK) return;
+            }
+            gzFile g = gzopen(paths[f], "rb");
+            if (!g) {
+                std::lock_guard<std::mutex> lk(mu);
+                if (first_error == KC_OK) first_error = KC_EIO;
+                return;
+            }
+            gzbuffer(g, kGzRead);
+            Framer fr;
+            uint64_t used = 0, my_in = 0, my_framed = 0, my_chunks = 0;
+            double my_inflate = 0;
+            int err = KC_OK;
+            for (;;) {
+                const auto a = std::chrono::steady_clock::now();
+                const int r = gzread(g, in.data(), kGzRead);
+                my_inflate += std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
+                if (r < 0) {
+                    err = KC_EIO;
+                    break;
+                }
+                if (r == 0) break;
+                my_in += (uint64_t)r;
+                used += fr.feed(in.data(), (uint64_t)r, buf.data() + used);
+                if (used >= kIngestChunk) {
+                    // hand over whole reads; a partial sequence line waits for the next block
+                    const uint8_t *nl = static_cast<const uint8_t *>(memrchr(buf.data(), '\n', used));
+                    const uint64_t cut = nl ? (uint64_t)(nl - buf.data()) + 1 : 0;
+                    if (cut) {
+                        std::lock_guard<std::mutex> lk(mu);
+                        err = kc_batch_add(b, buf.data(), cut);
+                    }
+                    if (err != KC_OK) break;
+                    std::memmove(buf.data(), buf.data() + cut, used - cut);
+                    my_framed += cut;
+                    my_chunks += cut ? 1 : 0;
+                    used -= cut;
+                    if (used > kIngestChunk) {  // one sequence line longer than a chunk: grow
+                        buf.resize(used + kIngestChunk + kGzRead);
+                    }
+                }
+                if (buf.size() < used + kGzRead) buf.resize(used + kGzRead);
+            }
+            gzclose(g);
+            if (err == KC_OK && used) {
+                std::lock_guard<std::mutex> lk(mu);
+                err = kc_batch_add(b, buf.data(), used);  // the file's last read (newline added if missing)
+                my_framed += used;
+                ++my_chunks;
+            }
+            std::lock_guard<std::mutex> lk(mu);
+            in_bytes += my_in;
+            framed += my_framed;
+            chunks += my_chunks;
+            inflate_s += my_inflate;
+            if (err != KC_OK && first_error == KC_OK) first_error = err;
+            if (first_error != KC_OK) return;
+        }
+    };
+    const uint32_t nt = std::max<uint32_t>(1, std::min<uint32_t>(threads ? threads : 1, std::max<uint32_t>(n_paths, 1)));
+    std::vector<std::thread> pool;
+    for (uint32_t i = 0; i < nt; ++i) pool.emplace_back(worker);
+    for (auto &th : pool) th.join();
+    if (first_error == KC_OK) first_error = kc_batch_flush(b);
+    const int rs = kc_batch_sync(b);
+    if (first_error == KC_OK) first_error = rs;
+    kc_batch_info bi{};
+    kc_batch_get_info(b, &bi);
+    kc_batch_destroy(b);
+    if (stats) {
+        stats->files = n_paths;
+        stats->threads = nt;
+        stats->input_bytes = in_bytes;
+        stats->framed_bytes = framed;
+        stats->chunks = chunks;
+        stats->launches = bi.launches;
+        stats->inflate_seconds = inflate_s;
+        stats->wall_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return first_error;
 }

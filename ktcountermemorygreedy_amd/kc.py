@@ -65,6 +65,12 @@ class BatchInfo(C.Structure):
                 ("bytes", C.c_uint64)]
 
 
+class IngestStats(C.Structure):
+    _fields_ = [("files", C.c_uint64), ("threads", C.c_uint64), ("input_bytes", C.c_uint64),
+                ("framed_bytes", C.c_uint64), ("chunks", C.c_uint64), ("launches", C.c_uint64),
+                ("inflate_seconds", C.c_double), ("wall_seconds", C.c_double)]
+
+
 class RequestResult(C.Structure):
     _fields_ = [("status", C.c_int), ("reserved", C.c_int), ("latency_ms", C.c_double),
                 ("queue_ms", C.c_double), ("kmers", C.c_uint64), ("distinct", C.c_uint64)]
@@ -136,6 +142,8 @@ EXPORTS = {
     "kc_fastq_frame": (C.c_uint64, [C.c_void_p, C.c_uint64, C.c_void_p]),
     "kc_fastq_frame_file": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p), C.POINTER(C.c_uint64)]),
     "kc_host_free": (None, [C.c_void_p]),
+    "kc_count_fastq_files": (C.c_int, [C.c_void_p, C.POINTER(C.c_char_p), C.c_uint32, C.c_uint32,
+                                       C.POINTER(Tune), C.POINTER(IngestStats)]),
     "kc_synth_genome": (C.c_int, [C.POINTER(GenomeSpec), C.c_void_p]),
     "kc_synth_reads_size": (C.c_int64, [C.c_void_p, C.c_uint64, C.POINTER(ReadSpec), C.c_int]),
     "kc_synth_reads": (C.c_int64, [C.c_void_p, C.c_uint64, C.POINTER(ReadSpec), C.c_int,
@@ -525,6 +533,22 @@ def fastq_frame_file(path) -> np.ndarray:
             if n.value else np.zeros(0, np.uint8)
     finally:
         lib().kc_host_free(p)
+
+
+def count_fastq_files(table: "Table", paths, threads=4, batch_bytes=0, queue_depth=0):
+    """Streaming ingest of FASTQ(.gz) files into table (host threads inflate
+    and frame, a kc_batch counts); returns the ingest statistics."""
+    arr = (C.c_char_p * max(1, len(paths)))(*[os.fsencode(p) for p in paths])
+    st = IngestStats()
+    tune = None
+    if batch_bytes or queue_depth:
+        base = table.tune()
+        tune = Tune(front_blocks=base["front_blocks"], fine_slices=base["fine_slices"],
+                    batch_bytes=batch_bytes or base["batch_bytes"], queue_depth=queue_depth or base["queue_depth"],
+                    path=base["path"])
+    check(lib().kc_count_fastq_files(table.h, arr, len(paths), threads, C.byref(tune) if tune else None,
+                                     C.byref(st)), "kc_count_fastq_files")
+    return {f: getattr(st, f) for f, _ in IngestStats._fields_}
 
 
 def synth_genome(seed, length, n_segments=0, segment_len=0, repeat_fraction=0.0,
