@@ -23,8 +23,9 @@
 namespace kcs {
 
 constexpr int kThreads = 256;
-constexpr int kDigitBits = 8;
-constexpr int kBins = 1 << kDigitBits;
+// Digits of up to DB bits (2^DB bins); the host splits the key range into
+// the fewest passes of at most kSortDigitBits bits.
+constexpr int kSortDigitBits = 11;
 
 template <int RW> struct Layout {
     static constexpr int NW = RW >= 5 ? 2 : 1;   // key words (u64)
@@ -64,27 +65,42 @@ __device__ __forceinline__ void load_rec(const uint32_t *src, uint64_t i, uint32
     for (int j = 0; j < RW; ++j) r[j] = src[i * RW + j];
 }
 
-template <int RW>
+// Block b owns the contiguous tiles [b*T, (b+1)*T): block order is input
+// order, so per-block digit counts scanned digit-major give every block the
+// start of its records of each digit (and the sort stays stable).
+__device__ __forceinline__ void block_tiles(uint64_t n_tiles, uint64_t &first, uint64_t &last)
+{
+    const uint64_t per = (n_tiles + gridDim.x - 1) / gridDim.x;
+    first = blockIdx.x * per;
+    last = first + per < n_tiles ? first + per : n_tiles;
+}
+
+template <int RW, int DB>
 __global__ __launch_bounds__(kThreads) void k_upsweep(const uint32_t *__restrict__ src, uint64_t n, uint32_t bit,
                                                       uint32_t width, uint64_t n_tiles, uint32_t *__restrict__ counts)
 {
     using L = Layout<RW>;
+    constexpr uint32_t kBins = 1u << DB;
     __shared__ uint32_t hist[kBins];
-    for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
-        hist[threadIdx.x] = 0;
-        __syncthreads();
+    for (uint32_t b = threadIdx.x; b < kBins; b += kThreads) hist[b] = 0;
+    __syncthreads();
+    uint64_t first, last;
+    block_tiles(n_tiles, first, last);
+    for (uint64_t tile = first; tile < last; ++tile) {
+        const uint64_t t0 = tile * L::TILE;
+        uint32_t r[L::ITEMS][RW];
+        // all loads first (index clamped into the array), then the counts
+#pragma unroll
         for (int j = 0; j < L::ITEMS; ++j) {
-            const uint64_t i = tile * L::TILE + (uint64_t)j * kThreads + threadIdx.x;
-            if (i < n) {
-                uint32_t r[RW];
-                load_rec<RW>(src, i, r);
-                atomicAdd(&hist[digit_of<L::NW>(r, bit, width)], 1u);
-            }
+            const uint64_t i = t0 + (uint64_t)j * kThreads + threadIdx.x;
+            load_rec<RW>(src, i < n ? i : n - 1, r[j]);
         }
-        __syncthreads();
-        counts[(uint64_t)threadIdx.x * n_tiles + tile] = hist[threadIdx.x];
-        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < L::ITEMS; ++j)
+            if (t0 + (uint64_t)j * kThreads + threadIdx.x < n) atomicAdd(&hist[digit_of<L::NW>(r[j], bit, width)], 1u);
     }
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b < kBins; b += kThreads) counts[(uint64_t)b * gridDim.x + blockIdx.x] = hist[b];
 }
 
 // Exclusive scan of a u32 array into u64 offsets: per-block sums, a scan of
@@ -156,84 +172,109 @@ __global__ __launch_bounds__(kThreads) void k_scan_apply(const uint32_t *in, uin
     }
 }
 
-// Stable scatter of one tile.  Chunks of kThreads consecutive records are
-// ranked in order; inside a chunk, wave w's records precede wave w+1's and
-// lanes are ordered by lane id, so ranks follow input order exactly.
-template <int RW>
+// Stable scatter.  Block b walks its tiles in order, carrying a running
+// global offset per digit.  Inside a tile, wave w ranks its own contiguous
+// quarter (records w*TILE/4 .. (w+1)*TILE/4, 64 consecutive records per
+// step) with wave-private digit counters: same-digit peers from 8 ballots,
+// rank = counter + peers before the lane, then the group's first lane bumps
+// the counter.  One cross-wave prefix per digit and one scan over digits
+// place every record; the tile is staged digit-sorted in LDS and written out
+// in runs.  Order inside a digit = input order, so the sort is stable.
+template <int RW, int DB>
 __global__ __launch_bounds__(kThreads) void k_downsweep(const uint32_t *__restrict__ src, uint32_t *__restrict__ dst,
                                                         uint64_t n, uint32_t bit, uint32_t width, uint64_t n_tiles,
                                                         const uint64_t *__restrict__ offsets)
 {
     using L = Layout<RW>;
+    constexpr uint32_t kBins = 1u << DB;
+    constexpr uint32_t kPer = kBins / kThreads;  // digits owned by one thread (contiguous)
+    constexpr int kWaves = kThreads / 64;
+    constexpr uint32_t kQuarter = L::TILE / kWaves;  // records per wave per tile
+    static_assert(kBins % kThreads == 0, "at least one digit per thread");
     __shared__ uint32_t stage[L::TILE * RW];
-    __shared__ uint32_t running[kBins];   // per-digit count so far in this tile
-    __shared__ uint32_t wcnt[4][kBins];   // per-wave digit counts of the chunk
+    __shared__ uint16_t wrun[kWaves][kBins];  // per wave: digit counts, then the wave's start in the digit
     __shared__ uint32_t local_start[kBins];
+    __shared__ uint32_t wsum[kWaves];
     __shared__ uint64_t gbase[kBins];
-    const int wave = threadIdx.x / 64;
-    const uint64_t lane_lt = (1ULL << __lane_id()) - 1ULL;
-
-    for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+    const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63;
+    const uint64_t lane_lt = (1ULL << lane) - 1ULL;
+    for (uint32_t d = t; d < kBins; d += kThreads) gbase[d] = offsets[(uint64_t)d * gridDim.x + blockIdx.x];
+    uint64_t first, last;
+    block_tiles(n_tiles, first, last);
+    for (uint64_t tile = first; tile < last; ++tile) {
         const uint64_t t0 = tile * L::TILE;
         const uint32_t tn = (uint32_t)(n - t0 < (uint64_t)L::TILE ? n - t0 : L::TILE);
-        running[threadIdx.x] = 0;
-        gbase[threadIdx.x] = offsets[(uint64_t)threadIdx.x * n_tiles + tile];
+        for (uint32_t d = t; d < kBins; d += kThreads)
+            for (int w = 0; w < kWaves; ++w) wrun[w][d] = 0;
         uint32_t r[L::ITEMS][RW];
+#pragma unroll
+        for (int j = 0; j < L::ITEMS; ++j) {
+            const uint32_t li = wave * kQuarter + (uint32_t)j * 64 + lane;
+            const uint64_t i = t0 + (li < tn ? li : tn - 1);
+            load_rec<RW>(src, i, r[j]);
+        }
+        __syncthreads();  // counters cleared
         uint32_t rank[L::ITEMS], dig[L::ITEMS];
 #pragma unroll
         for (int j = 0; j < L::ITEMS; ++j) {
-            const uint32_t li = (uint32_t)j * kThreads + threadIdx.x;
+            const uint32_t li = wave * kQuarter + (uint32_t)j * 64 + lane;
             const bool valid = li < tn;
-            if (valid) load_rec<RW>(src, t0 + li, r[j]);
             const uint32_t d = valid ? digit_of<L::NW>(r[j], bit, width) : 0;
             dig[j] = d;
-            // peers: valid lanes of this wave holding the same digit
             uint64_t peers = __ballot(valid);
-#pragma unroll
-            for (uint32_t b = 0; b < kDigitBits; ++b) {
+            for (uint32_t b = 0; b < width; ++b) {
                 const uint64_t m = __ballot((d >> b) & 1);
                 peers &= ((d >> b) & 1) ? m : ~m;
             }
             const uint32_t before = __popcll(peers & lane_lt);
-            const bool leader = valid && before == 0;
-            for (int w = 0; w < 4; ++w) wcnt[w][threadIdx.x] = 0;
-            __syncthreads();
-            if (leader) wcnt[wave][d] = __popcll(peers);
-            __syncthreads();
-            {
-                // digit threadIdx.x: prefix over waves, then advance running
-                uint32_t acc = running[threadIdx.x];
-                for (int w = 0; w < 4; ++w) {
-                    const uint32_t c = wcnt[w][threadIdx.x];
-                    wcnt[w][threadIdx.x] = acc;
-                    acc += c;
-                }
-                running[threadIdx.x] = acc;
-            }
-            __syncthreads();
-            rank[j] = valid ? wcnt[wave][d] + before : 0;
-            __syncthreads();
+            rank[j] = wrun[wave][d] + before;  // read by every lane before the leader's update
+            __builtin_amdgcn_wave_barrier();
+            if (valid && before == 0) wrun[wave][d] = (uint16_t)(wrun[wave][d] + __popcll(peers));
+            __builtin_amdgcn_wave_barrier();
         }
-        // tile-local start of each digit
-        {
-            const uint32_t c = running[threadIdx.x];
-            __shared__ uint32_t sc[kBins];
-            sc[threadIdx.x] = c;
-            __syncthreads();
-            for (int o = 1; o < kBins; o <<= 1) {
-                const uint32_t a = threadIdx.x >= (uint32_t)o ? sc[threadIdx.x - o] : 0;
-                __syncthreads();
-                sc[threadIdx.x] += a;
-                __syncthreads();
+        __syncthreads();
+        // digits [t*kPer, (t+1)*kPer): start of each wave inside the digit,
+        // then the digit's start in the tile (block scan over the threads)
+        uint32_t tot[kPer];
+        uint32_t mine = 0;
+#pragma unroll
+        for (uint32_t q = 0; q < kPer; ++q) {
+            const uint32_t d = t * kPer + q;
+            uint32_t acc = 0;
+#pragma unroll
+            for (int w = 0; w < kWaves; ++w) {
+                const uint32_t c = wrun[w][d];
+                wrun[w][d] = (uint16_t)acc;
+                acc += c;
             }
-            local_start[threadIdx.x] = sc[threadIdx.x] - c;
+            tot[q] = acc;
+            mine += acc;
+        }
+        {
+            uint32_t x = mine;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t y = __shfl_up(x, o, 64);
+                if (lane >= (uint32_t)o) x += y;
+            }
+            if (lane == 63) wsum[wave] = x;
+            __syncthreads();
+            uint32_t run = 0;
+#pragma unroll
+            for (int w = 0; w < kWaves; ++w) run += w < (int)wave ? wsum[w] : 0;
+            run += x - mine;
+#pragma unroll
+            for (uint32_t q = 0; q < kPer; ++q) {
+                local_start[t * kPer + q] = run;
+                run += tot[q];
+            }
         }
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < L::ITEMS; ++j) {
-            const uint32_t li = (uint32_t)j * kThreads + threadIdx.x;
+            const uint32_t li = wave * kQuarter + (uint32_t)j * 64 + lane;
             if (li < tn) {
-                const uint32_t p = local_start[dig[j]] + rank[j];
+                const uint32_t p = local_start[dig[j]] + wrun[wave][dig[j]] + rank[j];
 #pragma unroll
                 for (int q = 0; q < RW; ++q) stage[p * RW + q] = r[j][q];
             }
@@ -241,7 +282,7 @@ __global__ __launch_bounds__(kThreads) void k_downsweep(const uint32_t *__restri
         __syncthreads();
         // write out in local order: consecutive positions of one digit land
         // on consecutive global addresses
-        for (uint32_t p = threadIdx.x; p < tn; p += kThreads) {
+        for (uint32_t p = t; p < tn; p += kThreads) {
             const uint32_t *rec = stage + p * RW;
             const uint32_t d = digit_of<L::NW>(rec, bit, width);
             const uint64_t g = gbase[d] + (p - local_start[d]);
@@ -249,6 +290,8 @@ __global__ __launch_bounds__(kThreads) void k_downsweep(const uint32_t *__restri
             for (int q = 0; q < RW; ++q) dst[g * RW + q] = rec[q];
         }
         __syncthreads();
+#pragma unroll
+        for (uint32_t q = 0; q < kPer; ++q) gbase[t * kPer + q] += tot[q];
     }
 }
 
@@ -323,14 +366,19 @@ struct SortScratch {
     uint64_t *sums = nullptr;
 };
 
-template <int RW>
-int sort_impl(kc_ctx *ctx, uint32_t *a, uint32_t *b, uint64_t n, uint32_t lo, uint32_t hi, int *in_scratch)
+// One LSD pass per digit of at most DB bits over [lo, hi), digits balanced
+// (42 bits with DB = 11: 11, 11, 10, 10).
+template <int RW, int DB>
+int sort_passes(kc_ctx *ctx, uint32_t *a, uint32_t *b, uint64_t n, uint32_t lo, uint32_t hi, int *in_scratch)
 {
     using L = Layout<RW>;
+    constexpr uint32_t kBins = 1u << DB;
     *in_scratch = 0;
     if (n <= 1 || hi <= lo) return KC_OK;
     const uint64_t n_tiles = (n + L::TILE - 1) / L::TILE;
-    const uint64_t m = (uint64_t)kBins * n_tiles;
+    // blocks own contiguous tile ranges: counts are [digit][block]
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(n_tiles, (uint64_t)ctx->cu_count * 8);
+    const uint64_t m = (uint64_t)kBins * grid;
     const uint64_t nb = (m + kScanChunk - 1) / kScanChunk;
     SortScratch s;
     if (hipMalloc((void **)&s.counts, m * 4) != hipSuccess || hipMalloc((void **)&s.offsets, m * 8) != hipSuccess ||
@@ -340,20 +388,24 @@ int sort_impl(kc_ctx *ctx, uint32_t *a, uint32_t *b, uint64_t n, uint32_t lo, ui
         (void)hipFree(s.sums);
         return KC_ENOMEM;
     }
-    const uint32_t grid = (uint32_t)std::min<uint64_t>(n_tiles, (uint64_t)ctx->cu_count * 8);
+    const uint32_t range = hi - lo;
+    const uint32_t npass = (range + DB - 1) / DB;
     uint32_t *src = a, *dst = b;
     int passes = 0;
     hipError_t e = hipSuccess;
-    for (uint32_t bit = lo; bit < hi; bit += kDigitBits) {
-        const uint32_t width = std::min<uint32_t>(kDigitBits, hi - bit);
-        hipLaunchKernelGGL(k_upsweep<RW>, dim3(grid), dim3(kThreads), 0, ctx->stream, src, n, bit, width, n_tiles, s.counts);
+    uint32_t bit = lo;
+    for (uint32_t p = 0; p < npass; ++p) {
+        // the first (range % npass) passes take one more bit
+        const uint32_t width = range / npass + (p < range % npass ? 1 : 0);
+        hipLaunchKernelGGL((k_upsweep<RW, DB>), dim3(grid), dim3(kThreads), 0, ctx->stream, src, n, bit, width, n_tiles, s.counts);
         hipLaunchKernelGGL(k_scan_reduce, dim3((uint32_t)nb), dim3(kThreads), 0, ctx->stream, s.counts, m, s.sums);
         hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kThreads), 0, ctx->stream, s.sums, nb);
         hipLaunchKernelGGL(k_scan_apply, dim3((uint32_t)nb), dim3(kThreads), 0, ctx->stream, s.counts, m, s.sums, s.offsets);
-        hipLaunchKernelGGL(k_downsweep<RW>, dim3(grid), dim3(kThreads), 0, ctx->stream, src, dst, n, bit, width, n_tiles, s.offsets);
+        hipLaunchKernelGGL((k_downsweep<RW, DB>), dim3(grid), dim3(kThreads), 0, ctx->stream, src, dst, n, bit, width, n_tiles, s.offsets);
         if ((e = hipGetLastError()) != hipSuccess) break;
         std::swap(src, dst);
         ++passes;
+        bit += width;
     }
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     (void)hipFree(s.counts);
@@ -365,6 +417,12 @@ int sort_impl(kc_ctx *ctx, uint32_t *a, uint32_t *b, uint64_t n, uint32_t lo, ui
     }
     *in_scratch = passes & 1;
     return KC_OK;
+}
+
+template <int RW>
+int sort_impl(kc_ctx *ctx, uint32_t *a, uint32_t *b, uint64_t n, uint32_t lo, uint32_t hi, int *in_scratch)
+{
+    return sort_passes<RW, kSortDigitBits>(ctx, a, b, n, lo, hi, in_scratch);
 }
 
 // Key range of ExportRecordRadixSort (ExportRecordRadixSort.h:225-254):
