@@ -168,7 +168,7 @@ def bench_sort(args):
     k = out[:, 0].astype(np.uint64) | (out[:, 1].astype(np.uint64) << np.uint64(32))
     sk = (k >> np.uint64(2)) & np.uint64((1 << 42) - 1)
     step_s = float(np.mean(ms)) * 1e-3
-    digit_bits = 11  # the device sort's widest digit (kSortDigitBits)
+    digit_bits = 9  # the device sort's widest digit (kSortDigitBits)
     passes = (42 + digit_bits - 1) // digit_bits
     alg = n * (2 * 12 * 4 + 4)
     line = {
