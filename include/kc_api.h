@@ -75,6 +75,9 @@ typedef struct kc_tune {
     uint64_t batch_bytes;   /* capacity of one host batch (one staging queue slot) */
     uint32_t queue_depth;   /* batches in flight (staging slots), 2..8 */
     uint32_t path;          /* KC_PATH_* of batch launches; AUTO picks by batch size */
+    uint64_t piece_bytes;   /* largest input one partitioned pass counts (0 = 4 GB); larger
+                               inputs go in pieces, bounding the pass's two key arrays to
+                               2 x 8N x piece_bytes bytes */
 } kc_tune;
 
 const char *kc_strerror(int status);

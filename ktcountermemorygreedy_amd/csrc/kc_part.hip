@@ -824,16 +824,40 @@ int kc_part_enqueue(const kc_table *t, kc_part_scratch &s, const kc_pipe_shape &
 
 // Counts the windows of d_seq (when d_keys is null) or the n_keys keys of
 // d_keys into the table through the partitioned pipeline, on the table's
-// stream, with the table's own scratch (grown to the largest call).
+// stream, with the table's own scratch (grown to the largest call).  Inputs
+// larger than the tune's piece_bytes (default 4 GB; keys) are counted in
+// pieces, so the key arrays stay within 2 x 8N x piece_bytes bytes; byte
+// pieces after the first start 64 bytes early and emit nothing there (the
+// k-1 overlap rule of FastqReader.h:542-557).
+constexpr uint64_t kPartPiece = 4ull << 30;
+constexpr uint64_t kPartHalo = 64;
+
 int kc_part_count(kc_table *t, const uint8_t *d_seq, uint64_t nbytes, uint64_t emit_from, const uint64_t *d_keys,
                   uint64_t n_keys, const TableView &tv)
 {
     const bool from_seq = d_keys == nullptr;
-    const uint64_t key_cap = from_seq ? nbytes - std::min(nbytes, emit_from) : n_keys;
-    if (!key_cap) return KC_OK;
-    const kc_pipe_shape sh = kc_part_shape(t, t->tune, key_cap);
-    int rc = kc_part_reserve(t, t->ps, sh);
-    if (rc != KC_OK) return rc;
-    SeqSrc src{d_seq, nbytes, emit_from, nullptr};
-    return kc_part_enqueue(t, t->ps, sh, src, d_keys, n_keys, tv, t->ctx->stream);
+    const uint64_t total = from_seq ? nbytes : n_keys;
+    uint64_t pos = from_seq ? std::min(nbytes, emit_from) : 0;
+    // pieces are whole 16-byte groups, so every piece starts 16-byte aligned
+    const uint64_t piece = std::max<uint64_t>(1ull << 16, (t->tune.piece_bytes ? t->tune.piece_bytes : kPartPiece) & ~15ull);
+    while (pos < total) {
+        const uint64_t take = std::min(piece, total - pos);
+        int rc;
+        if (from_seq) {
+            // piece [pos - halo, pos + take), windows ending before pos excluded
+            const uint64_t halo = std::min(kPartHalo, pos);
+            const kc_pipe_shape sh = kc_part_shape(t, t->tune, take);
+            if ((rc = kc_part_reserve(t, t->ps, sh)) != KC_OK) return rc;
+            SeqSrc src{d_seq + pos - halo, take + halo, halo, nullptr};
+            rc = kc_part_enqueue(t, t->ps, sh, src, nullptr, 0, tv, t->ctx->stream);
+        } else {
+            const kc_pipe_shape sh = kc_part_shape(t, t->tune, take);
+            if ((rc = kc_part_reserve(t, t->ps, sh)) != KC_OK) return rc;
+            SeqSrc none{nullptr, 0, 0, nullptr};
+            rc = kc_part_enqueue(t, t->ps, sh, none, d_keys + pos * t->nw, take, tv, t->ctx->stream);
+        }
+        if (rc != KC_OK) return rc;
+        pos += take;
+    }
+    return KC_OK;
 }

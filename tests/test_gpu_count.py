@@ -154,3 +154,25 @@ def test_clear_resets(ctx):
     b = t.stats()
     assert a == b
     t.close()
+
+
+@pytest.mark.parametrize("bloom", [False, True], ids=["plain", "bloom"])
+def test_partitioned_pieces_exact(ctx, bloom):
+    """An input larger than the tune's piece_bytes is counted in pieces that
+    overlap by 64 bytes; every window is counted exactly once."""
+    from ktcountermemorygreedy_amd import kc
+    from ktcountermemorygreedy_amd import workloads as W
+    g = W.ReadSet("g", 3, 1_000_000, 13, 0).genome()
+    seq = W.UNEVEN_50MBP.scaled(2000).reads(genome=g).tobytes()
+    keys = O.extract(seq, 31)
+    uniq, cnt = O.count(keys, 1)
+    t = kc.Table(ctx, 31, capacity=1 << 23, bloom=bloom, max_distinct=len(seq), path="partitioned")
+    t.set_tune(piece_bytes=(1 << 18) + 48)  # rounded down to whole 16-byte groups
+    d = ctx.upload(np.frombuffer(seq, dtype=np.uint8))
+    t.count_seq(d, len(seq))
+    ctx.sync()
+    st = t.stats()
+    assert len(seq) > 4 * (1 << 18)
+    assert st["errors"] == 0 and st["kmers"] == keys.shape[0]
+    assert np.array_equal(t.query(uniq).astype(np.uint64), cnt)
+    t.close()
