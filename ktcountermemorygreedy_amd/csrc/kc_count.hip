@@ -168,9 +168,30 @@ __device__ __forceinline__ void pack_key(uint8_t *dst, const uint64_t *key, cons
 
 // Export compaction is deterministic: a counting pass writes per-tile
 // record counts, one block scans them, and the writing pass places each
-// tile's records at its scanned offset in slot order.
-constexpr int kExportPer = 16;                       // slots per thread
+// tile's records at its scanned offset.  Records are packed into an LDS
+// staging buffer first and the tile's contiguous output bytes are then
+// written with coalesced stores.
+constexpr int kExportPer = 8;                        // slots per thread
 constexpr uint64_t kExportTile = (uint64_t)kBlock * kExportPer;
+constexpr uint32_t kExportStage = (uint32_t)kExportTile * 20;  // bytes: k=63 high record
+
+// Copies nbytes of LDS staging to dst (any alignment): byte stores up to a
+// 4-byte boundary, then 4-byte stores, then the tail; consecutive threads
+// take consecutive words, so every wave store is one contiguous segment.
+__device__ __forceinline__ void stage_copy(uint8_t *dst, const uint8_t *lds, uint32_t nbytes)
+{
+    const uint32_t head = (uint32_t)((4 - (reinterpret_cast<uintptr_t>(dst) & 3)) & 3);
+    const uint32_t h = head < nbytes ? head : nbytes;
+    if (threadIdx.x < h) dst[threadIdx.x] = lds[threadIdx.x];
+    const uint32_t words = (nbytes - h) / 4;
+    uint32_t *dw = reinterpret_cast<uint32_t *>(dst + h);
+    for (uint32_t w = threadIdx.x; w < words; w += kBlock) {
+        const uint8_t *b = lds + h + 4 * w;
+        dw[w] = (uint32_t)b[0] | ((uint32_t)b[1] << 8) | ((uint32_t)b[2] << 16) | ((uint32_t)b[3] << 24);
+    }
+    const uint32_t done = h + 4 * words;
+    if (threadIdx.x < nbytes - done) dst[done + threadIdx.x] = lds[done + threadIdx.x];
+}
 
 // Classifies slot i: 1 = high record (count in cc), 2 = low record
 // (singleton without an active FIRST list), 0 = nothing.
@@ -219,14 +240,18 @@ __global__ __launch_bounds__(kBlock) void k_export_count(TableView tv, bool sing
     const uint64_t cap = tv.mask + 1;
     for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
         uint32_t h = 0, l = 0;
-        for (int j = 0; j < kExportPer; ++j) {
+        int kind[kExportPer];
+#pragma unroll
+        for (int j = 0; j < kExportPer; ++j) {  // every slot load issued before the first use
             const uint64_t i = tile * kExportTile + (uint64_t)j * kBlock + threadIdx.x;
-            if (i >= cap) break;
             uint64_t key[NW];
             uint32_t cc;
-            const int kind = classify_slot<NW>(tv, i, singles_to_low, key, cc);
-            h += kind == 1;
-            l += kind == 2;
+            kind[j] = i < cap ? classify_slot<NW>(tv, i, singles_to_low, key, cc) : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < kExportPer; ++j) {
+            h += kind[j] == 1;
+            l += kind[j] == 2;
         }
         uint32_t th, tl;
         block_scan2(h, l, th, tl);
@@ -276,52 +301,82 @@ __global__ __launch_bounds__(1024) void k_scan_tiles(uint64_t *tile_counts, uint
 }
 
 // Pass 2: pack records at the scanned offsets.  Low records from the table
-// (singletons) follow the n_list FIRST-list records in the low buffer.
+// (singletons) follow the n_list FIRST-list records in the low buffer.  Each
+// tile's high records, then its low records, are packed into LDS and copied
+// out as one contiguous byte range.
 template <int NW>
 __global__ __launch_bounds__(kBlock) void k_export_write(TableView tv, PackSpec ps, bool singles_to_low,
                                                          uint64_t n_tiles, const uint64_t *tile_off,
                                                          uint8_t *out_high, uint8_t *out_low, uint64_t n_list)
 {
+    __shared__ uint8_t stage[kExportStage];
     const uint64_t cap = tv.mask + 1;
     const uint32_t hrec = ps.kmer_bytes + ps.count_bytes;
     for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+        uint64_t key[kExportPer][NW];
+        uint32_t cc[kExportPer];
+        int kind[kExportPer];
         uint32_t h = 0, l = 0;
+#pragma unroll
         for (int j = 0; j < kExportPer; ++j) {
             const uint64_t i = tile * kExportTile + (uint64_t)j * kBlock + threadIdx.x;
-            if (i >= cap) break;
-            uint64_t key[NW];
-            uint32_t cc;
-            const int kind = classify_slot<NW>(tv, i, singles_to_low, key, cc);
-            h += kind == 1;
-            l += kind == 2;
+            kind[j] = i < cap ? classify_slot<NW>(tv, i, singles_to_low, key[j], cc[j]) : 0;
+            h += kind[j] == 1;
+            l += kind[j] == 2;
         }
         uint32_t th, tl;
         block_scan2(h, l, th, tl);
-        uint64_t oh = tile_off[2 * tile] + h, ol = n_list + tile_off[2 * tile + 1] + l;
+        // high records of the tile
+        uint32_t at = h;
+#pragma unroll
         for (int j = 0; j < kExportPer; ++j) {
-            const uint64_t i = tile * kExportTile + (uint64_t)j * kBlock + threadIdx.x;
-            if (i >= cap) break;
-            uint64_t key[NW];
-            uint32_t cc = 0;
-            const int kind = classify_slot<NW>(tv, i, singles_to_low, key, cc);
-            if (kind == 1) {
-                uint8_t *d = out_high + oh * hrec;
-                pack_key<NW>(d, key, ps);
-                put_bytes(d + ps.kmer_bytes, cc, ps.count_bytes, 0);
-                ++oh;
-            } else if (kind == 2) {
-                pack_key<NW>(out_low + ol * ps.kmer_bytes, key, ps);
-                ++ol;
+            if (kind[j] == 1) {
+                uint8_t *d = stage + at * hrec;
+                pack_key<NW>(d, key[j], ps);
+                put_bytes(d + ps.kmer_bytes, cc[j], ps.count_bytes, 0);
+                ++at;
             }
+        }
+        __syncthreads();
+        stage_copy(out_high + tile_off[2 * tile] * hrec, stage, th * hrec);
+        __syncthreads();
+        if (tl) {
+            at = l;
+#pragma unroll
+            for (int j = 0; j < kExportPer; ++j) {
+                if (kind[j] == 2) {
+                    pack_key<NW>(stage + at * ps.kmer_bytes, key[j], ps);
+                    ++at;
+                }
+            }
+            __syncthreads();
+            stage_copy(out_low + (n_list + tile_off[2 * tile + 1]) * ps.kmer_bytes, stage, tl * ps.kmer_bytes);
+            __syncthreads();
         }
     }
 }
 
+// FIRST-list records: block (chunk, segment) packs rows [chunk * tile,
+// (chunk + 1) * tile) of one segment (they are consecutive in the
+// concatenated list) into LDS and copies them out as one byte range.
 template <int NW>
-__global__ __launch_bounds__(kBlock) void k_export_low_list(TableView tv, PackSpec ps, uint64_t n, uint8_t *out_low)
+__global__ __launch_bounds__(kBlock) void k_export_low_list(TableView tv, PackSpec ps, uint8_t *out_low)
 {
-    for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock)
-        pack_key<NW>(out_low + i * ps.kmer_bytes, low_key(tv, i, NW), ps);
+    __shared__ uint8_t stage[kExportStage];
+    const uint32_t seg = blockIdx.y;
+    const uint64_t s0 = tv.seg_off[seg], cnt = tv.seg_off[seg + 1] - s0;
+    for (uint64_t r0 = blockIdx.x * kExportTile; r0 < cnt; r0 += (uint64_t)gridDim.x * kExportTile) {
+        const uint32_t rn = (uint32_t)(cnt - r0 < kExportTile ? cnt - r0 : kExportTile);
+#pragma unroll
+        for (int j = 0; j < kExportPer; ++j) {
+            const uint32_t r = (uint32_t)j * kBlock + threadIdx.x;
+            if (r < rn)
+                pack_key<NW>(stage + r * ps.kmer_bytes, tv.low + ((uint64_t)seg * tv.seg_cap + r0 + r) * NW, ps);
+        }
+        __syncthreads();
+        stage_copy(out_low + (s0 + r0) * ps.kmer_bytes, stage, rn * ps.kmer_bytes);
+        __syncthreads();
+    }
 }
 
 // ---- histogram (histogram_tool precise over the exported view) ----------
@@ -1002,9 +1057,13 @@ extern "C" int kc_table_export_device(kc_table *t, uint8_t *d_high, uint64_t hig
     else hipLaunchKernelGGL(k_export_write<2>, dim3(g), dim3(kBlock), 0, c->stream, v, ps, !list, t->n_tiles, t->tile_counts, d_high, d_low, n_list);
     KC_HIP(hipGetLastError());
     if (n_list) {
-        const uint32_t g2 = grid_for(n_list, c->cu_count);
-        if (t->nw == 1) hipLaunchKernelGGL(k_export_low_list<1>, dim3(g2), dim3(kBlock), 0, c->stream, v, ps, n_list, d_low);
-        else hipLaunchKernelGGL(k_export_low_list<2>, dim3(g2), dim3(kBlock), 0, c->stream, v, ps, n_list, d_low);
+        // chunks per segment: enough blocks for the fullest segment, capped
+        uint64_t most = 0;
+        for (int sgi = 0; sgi < KC_LOW_SEGS; ++sgi) most = std::max(most, hc.seg_fill[sgi]);
+        const dim3 g2((uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((most + kExportTile - 1) / kExportTile, 64)),
+                      KC_LOW_SEGS);
+        if (t->nw == 1) hipLaunchKernelGGL(k_export_low_list<1>, g2, dim3(kBlock), 0, c->stream, v, ps, d_low);
+        else hipLaunchKernelGGL(k_export_low_list<2>, g2, dim3(kBlock), 0, c->stream, v, ps, d_low);
         KC_HIP(hipGetLastError());
     }
     return KC_OK;
