@@ -406,17 +406,22 @@ __device__ __forceinline__ void stage_tile(const uint8_t *__restrict__ seq, uint
     __syncthreads();
 }
 
-// Loads thread t's 64-byte segment of tile `tile` straight into registers:
-// w[0..7] = the 32 bytes before it (history for the first windows),
-// w[8..23] = the segment; bytes outside [0, n) read as 0 (invalid).  The
-// halo words are the previous lane's segment tail, so they hit in L1.
+// Loads the 64-byte segment at `start` straight into registers with HW
+// words of history before it: w[0..HW-1] = the 4*HW bytes before the
+// segment (32 bytes cover k <= 33, 64 bytes k <= 63), w[HW..HW+15] = the
+// segment; bytes outside [0, n) read as 0 (invalid).  The history words are
+// the previous lane's segment tail, so they hit in L1.
+template <int HW>
 __device__ __forceinline__ void load_segment(const uint8_t *__restrict__ seq, uint64_t n, uint64_t start,
-                                             uint32_t (&w)[24])
+                                             uint32_t (&w)[HW + 16])
 {
-    if (start >= 32 && start + kParseSeg <= n) {
-        const uint4 *p = reinterpret_cast<const uint4 *>(seq + start - 32);
+    static_assert(HW % 4 == 0, "history is whole 16-byte groups");
+    constexpr int G = HW / 4 + 4;
+    constexpr uint64_t H = 4 * HW;
+    if (start >= H && start + kParseSeg <= n) {
+        const uint4 *p = reinterpret_cast<const uint4 *>(seq + start - H);
 #pragma unroll
-        for (int g = 0; g < 6; ++g) {
+        for (int g = 0; g < G; ++g) {
             const uint4 v = p[g];
             w[4 * g] = v.x;
             w[4 * g + 1] = v.y;
@@ -426,10 +431,10 @@ __device__ __forceinline__ void load_segment(const uint8_t *__restrict__ seq, ui
         return;
     }
 #pragma unroll
-    for (int g = 0; g < 6; ++g) {
+    for (int g = 0; g < G; ++g) {
         uint4 v = make_uint4(0, 0, 0, 0);
-        if (start + 16 * g >= 32) {
-            const uint64_t off = start + 16 * g - 32;
+        if (start + 16 * g >= H) {
+            const uint64_t off = start + 16 * g - H;
             if (off + 16 <= n) {
                 v = *reinterpret_cast<const uint4 *>(seq + off);
             } else if (off < n) {

@@ -41,12 +41,21 @@ constexpr int kKeyBlock = 256;
 constexpr int kKeysPerThread = 16;
 constexpr uint64_t kKeyTile = (uint64_t)kKeyBlock * kKeysPerThread;  // keys per front-end tile
 constexpr int kPartThreads = 1024;
+// How the partition pass hands FIRST occurrences to the FIRST list: staged
+// per wave for one-word keys (fewer cursor atomics; measured faster on
+// repeat-rich input), appended per wave step for two-word keys (moves 16-byte
+// FIRSTs once; 57 vs 70 ms per count call on 1.9e9 63-mers, 86% FIRSTs).
+constexpr bool kStageFirst1 = true;
+constexpr bool kStageFirst2 = false;
 // Front scatter blocks of one-word keys: 256 threads (the count pass's
 // tiles), 32-byte chunks (8192-key chunks, ~32-key runs, two blocks per CU).
 // 1024-thread blocks with 16-byte chunks (longer runs, one block per CU)
 // measured slower on this ALU-heavy pass.
 constexpr int kFrontBlock1 = kParseBlock;
 constexpr int kFrontChunk1 = 32;
+// Two-word keys: 16-byte chunks (4096 keys of 16 bytes = 64 KB of stage).
+constexpr int kFrontBlock2 = kParseBlock;
+constexpr int kFrontChunk2 = 16;
 
 struct Digits {
     uint32_t part_bits;  // L
@@ -136,46 +145,23 @@ __global__ __launch_bounds__(kParseBlock) void k_front_count_seq(SeqSrc src, uin
     if (threadIdx.x < (1u << d.l1)) m1[(uint64_t)threadIdx.x * gridDim.x + blockIdx.x] = hist[threadIdx.x];
 }
 
-// Front scatter, two-word keys: keys go straight to their coarse cursor.
-template <int NW>
-__global__ __launch_bounds__(kParseBlock) void k_front_scatter_seq(SeqSrc src, uint32_t k, Digits d,
-                                                                   const uint64_t *__restrict__ o1,
-                                                                   uint64_t *__restrict__ out)
-{
-    __shared__ ParseLds lds;
-    __shared__ unsigned long long cur[256];
-    uint64_t n, emit_from;
-    seq_bounds(src, n, emit_from);
-    if (threadIdx.x < (1u << d.l1)) cur[threadIdx.x] = o1[(uint64_t)threadIdx.x * gridDim.x + blockIdx.x];
-    const uint64_t ntiles = (n + kParseTile - 1) / kParseTile;
-    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        stage_tile(src.seq, n, tile, lds);
-        roll_segment<NW>(lds, tile, n, emit_from, k, [&](const uint64_t *key) {
-            const uint64_t pos = atomicAdd(&cur[coarse_of<NW>(key, d)], 1ull);
-#pragma unroll
-            for (int w = 0; w < NW; ++w) out[pos * NW + w] = key[w];
-        });
-        __syncthreads();
-    }
-}
-
-// Front scatter for one-word keys in TB-thread blocks: a tile is TB x 64
-// bytes (16 KB at TB = 256, the tiles of k_front_count_seq); each thread
-// loads its 64-byte segment (and 32 bytes of history) straight into
-// registers.
-// Front scatter with write combining (one-word keys): the segment is rolled
-// in chunks of CB bytes; the block's keys of one chunk (up to TB x CB) are
-// placed in LDS grouped by coarse digit and written out as contiguous runs
-// (TB x CB / 256 keys per digit per chunk on average).
-template <int TB, int CB>
+// Front scatter with write combining: each thread loads its 64-byte
+// segment (and k-1 <= 4*HW bytes of history) straight into registers and
+// rolls it in chunks of CB bytes; the block's keys of one chunk (up to
+// TB x CB) are placed in LDS grouped by coarse digit and written out as
+// contiguous runs (TB x CB / 256 keys per digit per chunk on average).  A
+// tile is TB x 64 bytes, the tiles of k_front_count_seq at TB = 256, so the
+// per-block offsets of the count pass apply.
+template <int NW, int TB, int CB>
 __global__ __launch_bounds__(TB) void k_front_scatter_seq_staged(SeqSrc src, uint32_t k, Digits d,
                                                                  const uint64_t *__restrict__ o1,
                                                                  uint64_t *__restrict__ out)
 {
+    constexpr int HW = NW == 1 ? 8 : 16;
     constexpr int kChunks = kParseSeg / CB;
     constexpr int kWords = CB / 4;
     constexpr int kStage = TB * CB;
-    __shared__ uint64_t stage[kStage];
+    __shared__ __align__(16) uint64_t stage[kStage * NW];
     __shared__ uint8_t sbin[kStage];
     __shared__ uint32_t hist[256], loff[256], wsum[TB / 64];
     __shared__ uint64_t base[256];
@@ -192,15 +178,15 @@ __global__ __launch_bounds__(TB) void k_front_scatter_seq_staged(SeqSrc src, uin
     const uint64_t ntiles = (n + kTileBytes - 1) / kTileBytes;
     for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const uint64_t start = tile * kTileBytes + (uint64_t)t * kParseSeg;
-        uint32_t w[24];
-        load_segment(src.seq, n, start, w);
+        uint32_t w[HW + 16];
+        load_segment<HW>(src.seq, n, start, w);
         // segment bytes that emit nothing: before emit_from, or past the end
         const uint32_t skip = start >= n ? 64u
                             : emit_from > start ? (uint32_t)(emit_from - start < 64 ? emit_from - start : 64) : 0u;
-        Roll<1> roll(k);
+        Roll<NW> roll(k);
         uint32_t have = 0;
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {  // 32-byte history from the previous segment
+        for (int j = 0; j < HW; ++j) {  // history from the previous segment
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
                 bool valid;
@@ -210,12 +196,12 @@ __global__ __launch_bounds__(TB) void k_front_scatter_seq_staged(SeqSrc src, uin
         }
 #pragma unroll
         for (int q = 0; q < kChunks; ++q) {
-            uint64_t key[CB];
+            uint64_t key[CB * NW];
             uint32_t rb[CB];       // rank << 8 | coarse digit
             uint32_t emitted = 0;  // bit e: byte e of the chunk emitted a key
 #pragma unroll
             for (int jj = 0; jj < kWords; ++jj) {
-                const uint32_t v = w[8 + q * kWords + jj];
+                const uint32_t v = w[HW + q * kWords + jj];
 #pragma unroll
                 for (int b = 0; b < 4; ++b) {
                     const int e = jj * 4 + b;
@@ -224,8 +210,8 @@ __global__ __launch_bounds__(TB) void k_front_scatter_seq_staged(SeqSrc src, uin
                     have = valid ? have + 1 : 0;
                     const bool emit = have >= k && (uint32_t)(4 * (q * kWords + jj) + b) >= skip;
                     if (emit) {
-                        roll.canon(&key[e]);
-                        const uint32_t c = coarse_of<1>(&key[e], d);
+                        roll.canon(&key[e * NW]);
+                        const uint32_t c = coarse_of<NW>(&key[e * NW], d);
                         rb[e] = (atomicAdd(&hist[c], 1u) << 8) | c;
                         emitted |= 1u << e;
                     }
@@ -241,14 +227,20 @@ __global__ __launch_bounds__(TB) void k_front_scatter_seq_staged(SeqSrc src, uin
             for (int e = 0; e < CB; ++e) {
                 if (emitted & (1u << e)) {
                     const uint32_t p = loff[rb[e] & 0xFF] + (rb[e] >> 8);
-                    stage[p] = key[e];
+#pragma unroll
+                    for (int x = 0; x < NW; ++x) stage[p * NW + x] = key[e * NW + x];
                     sbin[p] = (uint8_t)rb[e];
                 }
             }
             __syncthreads();
             for (uint32_t i = t; i < total; i += TB) {
                 const uint32_t c = sbin[i];
-                out[base[c] + (i - loff[c])] = stage[i];
+                const uint64_t at = base[c] + (i - loff[c]);
+                if constexpr (NW == 1) {
+                    out[at] = stage[i];
+                } else {
+                    *reinterpret_cast<ulonglong2 *>(out + at * 2) = *reinterpret_cast<const ulonglong2 *>(stage + i * 2);
+                }
             }
             __syncthreads();
             if (t < 256) {
@@ -260,7 +252,6 @@ __global__ __launch_bounds__(TB) void k_front_scatter_seq_staged(SeqSrc src, uin
     }
 }
 
-// ---- front end over keys (increment of key batches) ---------------------
 template <int NW>
 __global__ __launch_bounds__(kKeyBlock) void k_front_count_keys(const uint64_t *__restrict__ keys, uint64_t n,
                                                                 Digits d, uint32_t *__restrict__ m1)
@@ -365,7 +356,10 @@ __global__ __launch_bounds__(kKeyBlock) void k_fine_count(const uint64_t *__rest
 // runs leave fewer partly written lines: with 256 fine digits a chunk is
 // 16384 keys (one 1024-thread block per CU, ~64-key runs); NF = 1024 (more
 // than 2^16 partitions) uses 8192-key chunks in 256-thread blocks.
-template <int NW, int NF, int TB, int FPT>
+// SB: keep each staged key's digit in LDS (sbin); without it the digit is
+// recomputed from the staged key at write-out, which leaves the LDS to
+// larger chunks (two-word keys at 1024 digits: 8192-key chunks, ~8-key runs).
+template <int NW, int NF, int TB, int FPT, bool SB = true>
 __global__ __launch_bounds__(TB) void k_fine_scatter_staged(const uint64_t *__restrict__ keys,
                                                             const uint64_t *__restrict__ starts, Digits d,
                                                             uint32_t S, const uint64_t *__restrict__ o2,
@@ -373,8 +367,8 @@ __global__ __launch_bounds__(TB) void k_fine_scatter_staged(const uint64_t *__re
 {
     using BinT = typename std::conditional<NF <= 256, uint8_t, uint16_t>::type;
     constexpr uint32_t CH = (uint32_t)TB * FPT;
-    __shared__ uint64_t stage[CH * NW];
-    __shared__ BinT sbin[CH];
+    __shared__ __align__(16) uint64_t stage[CH * NW];
+    __shared__ BinT sbin[SB ? CH : 1];
     __shared__ uint32_t hist[NF], loff[NF];
     __shared__ uint64_t base[NF];
     uint32_t *tmp = loff + NF - TB / 64;  // the scan's wave sums: loff is rewritten only after the scan
@@ -398,8 +392,13 @@ __global__ __launch_bounds__(TB) void k_fine_scatter_staged(const uint64_t *__re
         for (uint32_t j = 0; j < (uint32_t)FPT; ++j) {
             const uint32_t i = j * TB + t;
             const uint64_t *src = keys + (cs + (i < cn ? i : cn - 1)) * NW;
-#pragma unroll
-            for (int w = 0; w < NW; ++w) key[j][w] = src[w];
+            if constexpr (NW == 2) {
+                const ulonglong2 v = *reinterpret_cast<const ulonglong2 *>(src);
+                key[j][0] = v.x;
+                key[j][1] = v.y;
+            } else {
+                key[j][0] = src[0];
+            }
         }
 #pragma unroll
         for (uint32_t j = 0; j < (uint32_t)FPT; ++j) {
@@ -428,15 +427,20 @@ __global__ __launch_bounds__(TB) void k_fine_scatter_staged(const uint64_t *__re
                 const uint32_t p = loff[bin[j]] + rank[j];
 #pragma unroll
                 for (int w = 0; w < NW; ++w) stage[p * NW + w] = key[j][w];
-                sbin[p] = (BinT)bin[j];
+                if constexpr (SB) sbin[p] = (BinT)bin[j];
             }
         }
         __syncthreads();
         for (uint32_t p = t; p < cn; p += TB) {
-            const uint32_t f = sbin[p];
+            uint32_t f;
+            if constexpr (SB) f = sbin[p];
+            else f = fine_of(xxh3_64<NW>(stage + p * NW), d);
             const uint64_t g = base[f] + (p - loff[f]);
-#pragma unroll
-            for (int w = 0; w < NW; ++w) out[g * NW + w] = stage[p * NW + w];
+            if constexpr (NW == 2) {
+                *reinterpret_cast<ulonglong2 *>(out + g * 2) = *reinterpret_cast<const ulonglong2 *>(stage + p * 2);
+            } else {
+                out[g] = stage[p];
+            }
         }
         __syncthreads();
         for (uint32_t f = t; f < nf; f += TB) {
@@ -542,14 +546,16 @@ __device__ __forceinline__ uint64_t wave_pos(uint64_t beg, uint32_t wave, uint32
     return beg + (uint64_t)(n >> 6) * kPartThreads + wave * 64u + (n & 63u);
 }
 
-// FIRST occurrences found by a wave are staged in `first_out` (the pipeline's
+// FIRST occurrences, STAGE = true: staged in `first_out` (the pipeline's
 // other key array, idle during this pass) at the positions of the wave's own
 // keys: the wave's n-th FIRST goes where its n-th key was, so runs of
 // different waves and partitions never overlap and no atomic is needed while
 // counting.  After the loop each wave moves its run to the FIRST list in
 // pieces of up to 1024 keys, one cursor atomic per piece, on segments that
-// rotate with the partition, the wave and the piece.
-template <int NW, bool BLOOM>
+// rotate with the partition, the wave and the piece.  STAGE = false: each
+// wave step appends its FIRSTs to the FIRST list at once (one cursor atomic
+// per wave step), which moves the FIRST bytes once instead of three times.
+template <int NW, bool BLOOM, bool STAGE>
 __global__ __launch_bounds__(kPartThreads) void k_count_parts(const uint64_t *__restrict__ keys,
                                                               const uint64_t *__restrict__ part_off, TableView tv,
                                                               uint64_t *__restrict__ first_out)
@@ -604,18 +610,22 @@ __global__ __launch_bounds__(kPartThreads) void k_count_parts(const uint64_t *__
         bool to_table = valid;
         if constexpr (BLOOM) {
             const bool first = valid && lds_bloom<NW, BLOOM>(S, lo, hi, bsz - 1) == 0;
-            const uint64_t fb = __ballot(first);
-            if (first) {
-                uint64_t *dst = first_out + wave_pos(beg, wave, fcur + (uint32_t)__popcll(fb & lanemask_lt())) * NW;
+            if constexpr (STAGE) {
+                const uint64_t fb = __ballot(first);
+                if (first) {
+                    uint64_t *dst = first_out + wave_pos(beg, wave, fcur + (uint32_t)__popcll(fb & lanemask_lt())) * NW;
 #pragma unroll
-                for (int w = 0; w < NW; ++w) dst[w] = key[w];
+                    for (int w = 0; w < NW; ++w) dst[w] = key[w];
+                }
+                fcur += (uint32_t)__popcll(fb);
+            } else {
+                first_append<NW>(tv, first, key, n_err, (uint32_t)((base - beg) >> 10));
             }
-            fcur += (uint32_t)__popcll(fb);
             to_table = valid && !first;
         }
         if (to_table) lds_insert<NW, BLOOM>(S, key, hi, psz - 1, n_new, n_err);
     }
-    if constexpr (BLOOM) {
+    if constexpr (BLOOM && STAGE) {
         __threadfence_block();  // the wave's staged FIRSTs are read back by other lanes
         for (uint32_t p0 = 0; p0 < fcur; p0 += 1024) {
             const uint32_t cnt = fcur - p0 < 1024 ? fcur - p0 : 1024;
@@ -777,8 +787,12 @@ int kc_part_enqueue(const kc_table *t, kc_part_scratch &s, const kc_pipe_shape &
     if ((rc = kc_scan_u32_on(st, s.m1, m1n, s.o1, s.sums))) return rc;
     hipLaunchKernelGGL(k_coarse_starts, dim3(1), dim3(256), 0, st, s.o1, s.m1, G, n_coarse, s.starts);
     if (from_seq) {
-        if (nw == 1) hipLaunchKernelGGL((k_front_scatter_seq_staged<kFrontBlock1, kFrontChunk1>), dim3(G), dim3(kFrontBlock1), 0, st, seq, t->cfg.k, d, s.o1, s.keys_a);
-        else hipLaunchKernelGGL(k_front_scatter_seq<2>, dim3(G), dim3(kParseBlock), 0, st, seq, t->cfg.k, d, s.o1, s.keys_a);
+        if (nw == 1)
+            hipLaunchKernelGGL((k_front_scatter_seq_staged<1, kFrontBlock1, kFrontChunk1>), dim3(G), dim3(kFrontBlock1), 0,
+                               st, seq, t->cfg.k, d, s.o1, s.keys_a);
+        else
+            hipLaunchKernelGGL((k_front_scatter_seq_staged<2, kFrontBlock2, kFrontChunk2>), dim3(G), dim3(kFrontBlock2), 0,
+                               st, seq, t->cfg.k, d, s.o1, s.keys_a);
     } else {
         if (nw == 1) hipLaunchKernelGGL(k_front_scatter_keys<1>, dim3(G), dim3(kKeyBlock), 0, st, d_keys, n_keys, d, s.o1, s.keys_a);
         else hipLaunchKernelGGL(k_front_scatter_keys<2>, dim3(G), dim3(kKeyBlock), 0, st, d_keys, n_keys, d, s.o1, s.keys_a);
@@ -798,7 +812,7 @@ int kc_part_enqueue(const kc_table *t, kc_part_scratch &s, const kc_pipe_shape &
             else hipLaunchKernelGGL((k_fine_scatter_staged<2, 256, 1024, 8>), g2, dim3(1024), 0, st, s.keys_a, s.starts, d, S, s.o2, s.keys_b);
         } else {
             if (nw == 1) hipLaunchKernelGGL((k_fine_scatter_staged<1, 1024, 256, 32>), g2, dim3(256), 0, st, s.keys_a, s.starts, d, S, s.o2, s.keys_b);
-            else hipLaunchKernelGGL((k_fine_scatter_staged<2, 1024, 256, 16>), g2, dim3(256), 0, st, s.keys_a, s.starts, d, S, s.o2, s.keys_b);
+            else hipLaunchKernelGGL((k_fine_scatter_staged<2, 1024, 1024, 8, false>), g2, dim3(1024), 0, st, s.keys_a, s.starts, d, S, s.o2, s.keys_b);
         }
         KC_HIP(hipGetLastError());
         part_keys = s.keys_b;
@@ -812,11 +826,11 @@ int kc_part_enqueue(const kc_table *t, kc_part_scratch &s, const kc_pipe_shape &
     const dim3 g((uint32_t)n_parts);
     uint64_t *first_out = part_keys == s.keys_b ? s.keys_a : s.keys_b;  // the idle key array
     if (nw == 1) {
-        if (bloom) hipLaunchKernelGGL((k_count_parts<1, true>), g, dim3(kPartThreads), 0, st, part_keys, s.part_off, tv, first_out);
-        else hipLaunchKernelGGL((k_count_parts<1, false>), g, dim3(kPartThreads), 0, st, part_keys, s.part_off, tv, first_out);
+        if (bloom) hipLaunchKernelGGL((k_count_parts<1, true, kStageFirst1>), g, dim3(kPartThreads), 0, st, part_keys, s.part_off, tv, first_out);
+        else hipLaunchKernelGGL((k_count_parts<1, false, true>), g, dim3(kPartThreads), 0, st, part_keys, s.part_off, tv, first_out);
     } else {
-        if (bloom) hipLaunchKernelGGL((k_count_parts<2, true>), g, dim3(kPartThreads), 0, st, part_keys, s.part_off, tv, first_out);
-        else hipLaunchKernelGGL((k_count_parts<2, false>), g, dim3(kPartThreads), 0, st, part_keys, s.part_off, tv, first_out);
+        if (bloom) hipLaunchKernelGGL((k_count_parts<2, true, kStageFirst2>), g, dim3(kPartThreads), 0, st, part_keys, s.part_off, tv, first_out);
+        else hipLaunchKernelGGL((k_count_parts<2, false, true>), g, dim3(kPartThreads), 0, st, part_keys, s.part_off, tv, first_out);
     }
     KC_HIP(hipGetLastError());
     return KC_OK;
