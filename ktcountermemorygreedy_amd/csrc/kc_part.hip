@@ -47,6 +47,8 @@ constexpr int kPartThreads = 1024;
 // FIRSTs once; 57 vs 70 ms per count call on 1.9e9 63-mers, 86% FIRSTs).
 constexpr bool kStageFirst1 = true;
 constexpr bool kStageFirst2 = false;
+// Steps of keys the partition pass keeps in flight ahead of the one counted.
+constexpr int kPartPrefetch = 1;
 // Front scatter blocks of one-word keys: 256 threads (the count pass's
 // tiles), 32-byte chunks (8192-key chunks, ~32-key runs, two blocks per CU).
 // 1024-thread blocks with 16-byte chunks (longer runs, one block per CU)
@@ -163,8 +165,9 @@ __global__ __launch_bounds__(TB) void k_front_scatter_seq_staged(SeqSrc src, uin
     constexpr int kStage = TB * CB;
     __shared__ __align__(16) uint64_t stage[kStage * NW];
     __shared__ uint8_t sbin[kStage];
-    __shared__ uint32_t hist[256], loff[256], wsum[TB / 64];
+    __shared__ uint32_t hist[256], loff[256];
     __shared__ uint64_t base[256];
+    uint32_t *wsum = loff + 256 - TB / 64;  // the scan's wave sums: loff is written only after the scan
     const uint32_t t = threadIdx.x;
     const uint32_t nbins = 1u << d.l1;
     uint64_t n, emit_from;
@@ -358,7 +361,7 @@ __global__ __launch_bounds__(kKeyBlock) void k_fine_count(const uint64_t *__rest
 // than 2^16 partitions) uses 8192-key chunks in 256-thread blocks.
 // SB: keep each staged key's digit in LDS (sbin); without it the digit is
 // recomputed from the staged key at write-out, which leaves the LDS to
-// larger chunks (two-word keys at 1024 digits: 8192-key chunks, ~8-key runs).
+// larger chunks (1024 digits: 16384 one-word or 8192 two-word keys per chunk).
 template <int NW, int NF, int TB, int FPT, bool SB = true>
 __global__ __launch_bounds__(TB) void k_fine_scatter_staged(const uint64_t *__restrict__ keys,
                                                             const uint64_t *__restrict__ starts, Digits d,
@@ -586,23 +589,30 @@ __global__ __launch_bounds__(kPartThreads) void k_count_parts(const uint64_t *__
     uint32_t n_k = 0, n_new = 0, n_err = 0;
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     uint32_t fcur = 0;  // FIRSTs staged by this wave (wave-uniform)
-    // the next step's key is loaded while this step's key is counted
-    uint64_t nxt[NW];
-    {
-        const uint64_t i = beg + threadIdx.x;
+    // the keys of the next PF steps are in flight while this step's key is
+    // counted (indexes past the end read the partition's first key)
+    constexpr int PF = kPartPrefetch;
+    uint64_t nxt[PF][NW];
 #pragma unroll
-        for (int w = 0; w < NW; ++w) nxt[w] = keys[(i < end ? i : beg) * NW + w];
+    for (int p = 0; p < PF; ++p) {
+        const uint64_t i = beg + (uint64_t)p * kPartThreads + threadIdx.x;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) nxt[p][w] = keys[(i < end ? i : beg) * NW + w];
     }
     for (uint64_t base = beg; base < end; base += kPartThreads) {
         const uint64_t i = base + threadIdx.x;
         const bool valid = i < end;
         uint64_t key[NW];
 #pragma unroll
-        for (int w = 0; w < NW; ++w) key[w] = nxt[w];
-        if (base + kPartThreads < end) {
-            const uint64_t j = i + kPartThreads;
+        for (int w = 0; w < NW; ++w) key[w] = nxt[0][w];
 #pragma unroll
-            for (int w = 0; w < NW; ++w) nxt[w] = keys[(j < end ? j : beg) * NW + w];
+        for (int p = 0; p + 1 < PF; ++p)
+#pragma unroll
+            for (int w = 0; w < NW; ++w) nxt[p][w] = nxt[p + 1][w];
+        if (base + (uint64_t)PF * kPartThreads < end) {
+            const uint64_t j = i + (uint64_t)PF * kPartThreads;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) nxt[PF - 1][w] = keys[(j < end ? j : beg) * NW + w];
         }
         n_k += valid;
         uint64_t lo, hi;  // one XXH3_128: Bloom probe and slot probe start
@@ -811,7 +821,7 @@ int kc_part_enqueue(const kc_table *t, kc_part_scratch &s, const kc_pipe_shape &
             if (nw == 1) hipLaunchKernelGGL((k_fine_scatter_staged<1, 256, 1024, 16>), g2, dim3(1024), 0, st, s.keys_a, s.starts, d, S, s.o2, s.keys_b);
             else hipLaunchKernelGGL((k_fine_scatter_staged<2, 256, 1024, 8>), g2, dim3(1024), 0, st, s.keys_a, s.starts, d, S, s.o2, s.keys_b);
         } else {
-            if (nw == 1) hipLaunchKernelGGL((k_fine_scatter_staged<1, 1024, 256, 32>), g2, dim3(256), 0, st, s.keys_a, s.starts, d, S, s.o2, s.keys_b);
+            if (nw == 1) hipLaunchKernelGGL((k_fine_scatter_staged<1, 1024, 1024, 16, false>), g2, dim3(1024), 0, st, s.keys_a, s.starts, d, S, s.o2, s.keys_b);
             else hipLaunchKernelGGL((k_fine_scatter_staged<2, 1024, 1024, 8, false>), g2, dim3(1024), 0, st, s.keys_a, s.starts, d, S, s.o2, s.keys_b);
         }
         KC_HIP(hipGetLastError());

@@ -73,12 +73,14 @@ def test_uneven_reads_counts_match_oracle(ctx, k):
     t.close()
 
 
-@pytest.mark.parametrize("capacity", [1 << 20, 1 << 21, 1 << 23])
+@pytest.mark.parametrize("capacity", [1 << 20, 1 << 21, 1 << 23, 1 << 29])
 @pytest.mark.parametrize("bloom", [False, True], ids=["plain", "bloom"])
 @pytest.mark.parametrize("k", [31, 63])
 def test_partition_levels_match_oracle(ctx, capacity, bloom, k):
     """Partitioned path with 1, <=256 (one scatter level) and >256 (two
-    levels) partitions: exact counts vs the oracle."""
+    levels) partitions, up to 2^18 partitions (2^29 slots: 256 to 1024 fine
+    digits, the scatter forms that recompute digits): exact counts vs the
+    oracle."""
     from ktcountermemorygreedy_amd import workloads as W
     g = W.ReadSet("g", 3, 400_000, 13, 0).genome()
     seq = W.COUNT_5MBP.scaled(6000).reads(genome=g).tobytes()
