@@ -53,6 +53,9 @@ def parse_args():
                     help="counting call: eager stream launches or one captured-graph launch (kc_batch); "
                          "auto = graph for uneven_50mbp")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
+    ap.add_argument("--passes", type=int, default=1,
+                    help="also time multi-pass counting (a power of two): the same input counted in P hash-slice "
+                         "passes into a table of 1/P the slots (the out-of-HBM fallback); a 'multipass' object")
     ap.add_argument("--no-host-batch", action="store_true",
                     help="skip the host-batched (PCIe-inclusive) line of uneven_50mbp")
     return ap.parse_args()
@@ -480,6 +483,10 @@ def main():
                                                np.array_equal(bins_h, bins))}
         table.clear()
 
+    multipass = None
+    if args.passes > 1:
+        multipass = bench_passes(ctx, args, table, d_seq, S, k, bloom, est_distinct, K, bins, tune)
+
     sample_reads = args.cpu_sample_reads or (300_000 if args.workload == "uneven_50mbp" else 4_000_000)
     cpu = None if args.no_cpu_baseline else cpu_baseline(ws, min(sample_reads, ws.n_reads), k, genome)
     line = {
@@ -500,6 +507,7 @@ def main():
         "checks": {"windows_match": windows == K, "hist_mass_equals_kmers": mass == K},
         "launch": launch_info,
         "host_batched": host_batched,
+        "multipass": multipass,
         "gen_seconds": gen_s,
     }
     print(json.dumps(line), flush=True)
@@ -507,6 +515,58 @@ def main():
         batch.close()
     table.close()
     ctx.close()
+
+
+def bench_passes(ctx, args, table, d_seq, S, k, bloom, est_distinct, K, bins, tune):
+    """The out-of-HBM fallback on the same input: P passes (clear + count of
+    hash slice p + export packing) into a table of 1/P the slots; checks
+    that the slices' k-mers and summed histograms equal the one-pass run."""
+    import numpy as np
+
+    from ktcountermemorygreedy_amd import kc
+    P = args.passes
+    if P & (P - 1) or P > 256:
+        raise SystemExit("--passes must be a power of two <= 256")
+    one = table.stats()
+    cap = max(1 << 20, one["capacity"] // P)
+    md = est_distinct // P + (1 << 20)
+    t = kc.Table(ctx, k, count_max=0xFFFFFFFF, capacity=cap, max_distinct=md, bloom=bloom,
+                 low_capacity=md if bloom else 0)
+    if tune:
+        t.set_tune(**{f: tune[f] for f in ("front_blocks", "fine_slices", "path", "piece_bytes") if f in tune})
+    bufs = {}
+
+    def passes(check=False):
+        kmers, acc = 0, np.zeros_like(bins)
+        for p in range(P):
+            t.clear()
+            t.set_pass(p, P)
+            t.count_seq(d_seq, S)
+            if not bufs:
+                hs, ls = t.export_sizes()
+                bufs["h"], bufs["l"] = ctx.alloc(int(hs * 1.25) + 4096), ctx.alloc(int(ls * 1.25) + 4096)
+            t.export_into(bufs["h"], bufs["l"])
+            if check:
+                st = t.stats()
+                assert st["errors"] == 0, st
+                kmers += st["kmers"]
+                acc += t.histogram(1, bins.size)
+        return kmers, acc
+
+    kmers, acc = passes(check=True)  # warm-up and checks
+    ctx.sync()
+    t0 = time.perf_counter()
+    steps = max(1, args.steps)
+    for _ in range(steps):
+        passes()
+    ctx.sync()
+    step_s = (time.perf_counter() - t0) / steps
+    for b in bufs.values():
+        b.free()
+    t.close()
+    return {"passes": P, "table_slots": cap, "ms_per_step": step_s * 1e3, "kmers_per_s": K / step_s,
+            "step": "P x (clear + count of one hash slice + export packing)",
+            "checks": {"slice_kmers_sum": kmers == K, "summed_hist_equals_one_pass": bool(np.array_equal(acc, bins))}}
 
 
 if __name__ == "__main__":

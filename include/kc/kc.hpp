@@ -155,6 +155,14 @@ public:
 
     // infos.bin / high.bin / low.bin into dir (the reference's temp_dir contract)
     void export_dir(const std::string &dir) { check(kc_table_export_dir(t_, dir.c_str()), "kc_table_export_dir"); }
+    // Appends this table's records to an existing export (one pass of a
+    // multi-pass count; the partition files of Partition.h:135-346).
+    void export_dir_append(const std::string &dir)
+    {
+        check(kc_table_export_dir_append(t_, dir.c_str()), "kc_table_export_dir_append");
+    }
+    // Later counting calls count only hash slice `pass` of `n_passes`.
+    void set_pass(uint32_t pass, uint32_t n_passes) { check(kc_table_set_pass(t_, pass, n_passes), "kc_table_set_pass"); }
 
     std::vector<int64_t> histogram(uint32_t min_freq = 1, uint32_t max_freq = 10000)
     {

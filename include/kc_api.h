@@ -137,9 +137,25 @@ int kc_table_query(kc_table *t, const uint64_t *d_keys, uint64_t n,
                    uint32_t *d_counts);
 int kc_table_stats(kc_table *t, kc_stats *out);
 
+/* Multi-pass counting for tables larger than HBM: the k-mers are split into
+ * n_passes hash slices (n_passes a power of two <= 256; slice = top
+ * log2(n_passes) bits of XXH3_64 of the key) and later counting calls
+ * (count_seq, count_seq_host, increment, batches created afterwards) count
+ * only the k-mers of slice `pass`; kc_stats.kmers counts those.  A table
+ * then needs room for one slice's distinct k-mers.  Counting every slice,
+ * each into a cleared table, and concatenating the exports gives the
+ * one-pass result.  The out-of-memory fallback of the reference partitions
+ * the exported files by key prefix instead (dump_tool.cpp:1043-1134,
+ * Partition.h:135-346).  (1, 1) restores one pass. */
+int kc_table_set_pass(kc_table *t, uint32_t pass, uint32_t n_passes);
+
 /* Export in the reference's on-disk format: infos.bin (main.cpp:590-597),
  * high.bin (FinalDrainWriter.h:71-146), low.bin (ExportWriter.h:296-313). */
 int kc_table_export_dir(kc_table *t, const char *dir);
+/* Same, appending the records to existing high.bin / low.bin (created when
+ * absent): the exports of the passes of a multi-pass count concatenate into
+ * one export, as the reference's partition files do (Partition.h:135-346). */
+int kc_table_export_dir_append(kc_table *t, const char *dir);
 /* Device-side export: packed high/low records into device buffers.
  * Either buffer may be NULL to query the required sizes. */
 int kc_table_export_device(kc_table *t, uint8_t *d_high, uint64_t high_cap,

@@ -46,6 +46,34 @@ typedef struct kc_ingest_stats {
 int kc_count_fastq_files(kc_table *t, const char *const *paths, uint32_t n_paths, uint32_t threads,
                          const kc_tune *tune, kc_ingest_stats *stats);
 
+/* Out-of-HBM counting: the files are counted in n_passes passes, pass p
+ * keeping the k-mers of hash slice p (kc_table_set_pass) in the cleared
+ * table t, which then needs room for one slice's distinct k-mers.  After
+ * each pass its export is appended to export_dir (pass 0 starts it), its
+ * dump (dump_tool precise over [dump_min, dump_max], sorted when
+ * dump_sorted) written to a part file, and its histogram over
+ * [hist_min, hist_max] added to hist_bins; the parts are merged at the end
+ * (sorted parts by key, kc_merge_sorted_dumps; otherwise concatenated).
+ * The reference's fallback partitions its exported files by key prefix and
+ * merges the partition outputs the same way (dump_tool.cpp:1043-1134,
+ * Partition.h:135-346).  Any of export_dir / dump_path / hist_bins may be
+ * NULL.  The table keeps the last pass; its pass setting is reset to one
+ * pass.  stats sum over the passes (every pass reads the files again). */
+typedef struct kc_pass_plan {
+    uint32_t n_passes;       /* power of two, 1..256 */
+    const char *export_dir;  /* infos.bin, high.bin, low.bin */
+    const char *dump_path;
+    uint32_t dump_min, dump_max;
+    int dump_sorted;
+    uint32_t hist_min, hist_max;
+    int64_t *hist_bins;      /* hist_max - hist_min + 1 host bins */
+} kc_pass_plan;
+int kc_count_fastq_files_passes(kc_table *t, const char *const *paths, uint32_t n_paths, uint32_t threads,
+                                const kc_tune *tune, const kc_pass_plan *plan, kc_ingest_stats *stats);
+/* k-way merge of dump texts sorted by k-mer (keys unique across the inputs)
+ * into out_path (merge_sorted_outputs, dump_tool.cpp:1043-1134).  Host only. */
+int kc_merge_sorted_dumps(const char *const *paths, uint32_t n_paths, const char *out_path);
+
 typedef struct kc_genome_spec {
     uint64_t seed;
     uint64_t length;            /* bases */

@@ -60,6 +60,7 @@ __global__ __launch_bounds__(kBlock) void k_count_seq(SeqSrc src, TableView tv)
     for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x, ++step) {
         stage_tile(src.seq, n, tile, lds);
         roll_segment<NW>(lds, tile, n, emit_from, tv.k, [&](const uint64_t *key) {
+            if (!in_pass<NW>(key, tv.pass_bits, tv.pass)) return;
             ++n_kmers;
             const KeyHash kh = key_hash<NW>(tv, key);
             bool to_table = true;
@@ -87,6 +88,7 @@ __global__ __launch_bounds__(kBlock) void k_increment(const uint64_t *__restrict
         uint64_t key[NW];
 #pragma unroll
         for (int w = 0; w < NW; ++w) key[w] = keys[i * NW + w];
+        if (!in_pass<NW>(key, tv.pass_bits, tv.pass)) continue;
         table_add<NW>(tv, key, key_hash<NW>(tv, key), inc ? inc[i] : 1, n_new, n_err);
     }
     stat_add(tv.counters, KC_STAT_DISTINCT, n_new);
@@ -548,6 +550,8 @@ TableView view_of(kc_table *t)
     v.count_max = t->cfg.count_max;
     v.filter_min = t->cfg.filter_min;
     v.filter_max = t->cfg.filter_max;
+    v.pass_bits = t->pass_bits;
+    v.pass = t->pass;
     return v;
 }
 
@@ -1069,15 +1073,15 @@ extern "C" int kc_table_export_device(kc_table *t, uint8_t *d_high, uint64_t hig
     return KC_OK;
 }
 
-static int write_file(const std::string &path, const void *data, uint64_t n)
+static int write_file(const std::string &path, const void *data, uint64_t n, bool append = false)
 {
-    FILE *f = std::fopen(path.c_str(), "wb");
+    FILE *f = std::fopen(path.c_str(), append ? "ab" : "wb");
     if (!f) return KC_EIO;
     const bool ok = n == 0 || std::fwrite(data, 1, n, f) == n;
     return (std::fclose(f) == 0 && ok) ? KC_OK : KC_EIO;
 }
 
-extern "C" int kc_table_export_dir(kc_table *t, const char *dir)
+static int export_dir(kc_table *t, const char *dir, bool append)
 {
     if (!t || !dir) return KC_EINVAL;
     std::string d(dir);
@@ -1106,8 +1110,22 @@ extern "C" int kc_table_export_dir(kc_table *t, const char *dir)
     if (rc != KC_OK) return rc;
     const uint32_t infos[2] = {t->cfg.k, t->cfg.count_max};
     if ((rc = write_file(d + "infos.bin", infos, sizeof infos)) != KC_OK) return rc;
-    if ((rc = write_file(d + "high.bin", hh.data(), hh.size())) != KC_OK) return rc;
-    return write_file(d + "low.bin", hl.data(), hl.size());
+    if ((rc = write_file(d + "high.bin", hh.data(), hh.size(), append)) != KC_OK) return rc;
+    return write_file(d + "low.bin", hl.data(), hl.size(), append);
+}
+
+extern "C" int kc_table_export_dir(kc_table *t, const char *dir) { return export_dir(t, dir, false); }
+
+// high.bin and low.bin are record streams, so the exports of several passes
+// (or tables) concatenate into one valid export.
+extern "C" int kc_table_export_dir_append(kc_table *t, const char *dir) { return export_dir(t, dir, true); }
+
+extern "C" int kc_table_set_pass(kc_table *t, uint32_t pass, uint32_t n_passes)
+{
+    if (!t || n_passes == 0 || n_passes > 256 || !is_pow2(n_passes) || pass >= n_passes) return KC_EINVAL;
+    t->pass_bits = ilog2(n_passes);
+    t->pass = pass;
+    return KC_OK;
 }
 
 extern "C" int kc_table_histogram(kc_table *t, uint32_t min_freq, uint32_t max_freq, int64_t *h_bins)

@@ -33,6 +33,8 @@ struct TableView {
     uint32_t count_max;
     uint32_t filter_min;
     uint32_t filter_max;
+    uint32_t pass_bits;          // multi-pass counting: log2(passes), 0 = one pass
+    uint32_t pass;               // the pass whose k-mers are counted
 };
 
 template <int NW> constexpr int slot_words() { return NW == 1 ? 2 : 4; }
@@ -63,6 +65,17 @@ __device__ __forceinline__ uint32_t coarse_digit(uint32_t mix, uint32_t l1) { re
 __device__ __forceinline__ uint32_t fine_digit(uint64_t h64, uint32_t part_bits, uint32_t l2)
 {
     return l2 ? (uint32_t)(h64 >> (64 - part_bits)) & ((1u << l2) - 1) : 0;
+}
+
+// Multi-pass counting (tables larger than HBM are counted in 2^pass_bits
+// passes over the input, each holding one hash slice of the k-mers): a key
+// belongs to pass (XXH3_64 >> (64 - pass_bits)).  The fine digit uses
+// XXH3_64 bits below the top byte, so the slices and the partitions are
+// independent.
+template <int NW>
+__device__ __forceinline__ bool in_pass(const uint64_t *key, uint32_t pass_bits, uint32_t pass)
+{
+    return pass_bits == 0 || (uint32_t)(xxh3_64<NW>(key) >> (64 - pass_bits)) == pass;
 }
 
 template <int NW>

@@ -10,6 +10,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <string_view>
 #include <thread>
 #include <vector>
 
@@ -404,4 +405,151 @@ extern "C" int kc_count_fastq_files(kc_table *t, const char *const *paths, uint3
         stats->wall_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
     return first_error;
+}
+
+// ---- out-of-HBM counting: several passes over the files ------------------
+// Merge of sorted dump texts ("<bases>\t<count>\n", one key per line, keys
+// unique across the inputs) by the bases field, the order dump_tool --sort
+// writes (dump_tool.cpp:345-354); the reference merges its partition outputs
+// the same way (merge_sorted_outputs, dump_tool.cpp:1043-1134).
+namespace {
+
+struct DumpReader {
+    FILE *f = nullptr;
+    std::string line;  // current line without its '\n'; empty at the end
+    std::vector<char> buf;
+    bool next()
+    {
+        line.clear();
+        int c;
+        while ((c = std::getc(f)) != EOF && c != '\n') line.push_back((char)c);
+        return !line.empty() || c == '\n';
+    }
+    std::string_view key() const
+    {
+        const size_t tab = line.find('\t');
+        return std::string_view(line).substr(0, tab == std::string::npos ? line.size() : tab);
+    }
+};
+
+}  // namespace
+
+extern "C" int kc_merge_sorted_dumps(const char *const *paths, uint32_t n_paths, const char *out_path)
+{
+    if (!out_path || (n_paths && !paths)) return KC_EINVAL;
+    for (uint32_t i = 0; i < n_paths; ++i)
+        if (!paths[i]) return KC_EINVAL;
+    std::vector<DumpReader> in(n_paths);
+    int rc = KC_OK;
+    for (uint32_t i = 0; i < n_paths && rc == KC_OK; ++i) {
+        in[i].f = std::fopen(paths[i], "rb");
+        if (!in[i].f) rc = KC_EIO;
+        else {
+            in[i].buf.resize(1 << 20);
+            std::setvbuf(in[i].f, in[i].buf.data(), _IOFBF, in[i].buf.size());
+        }
+    }
+    FILE *out = rc == KC_OK ? std::fopen(out_path, "wb") : nullptr;
+    if (rc == KC_OK && !out) rc = KC_EIO;
+    if (rc == KC_OK) {
+        std::vector<char> obuf(1 << 20);
+        std::setvbuf(out, obuf.data(), _IOFBF, obuf.size());
+        // min-heap of the readers by their current key (the heap holds the
+        // readers that still have a line)
+        std::vector<uint32_t> heap;
+        auto greater = [&](uint32_t a, uint32_t b) { return in[a].key() > in[b].key(); };
+        for (uint32_t i = 0; i < n_paths; ++i)
+            if (in[i].next()) heap.push_back(i);
+        std::make_heap(heap.begin(), heap.end(), greater);
+        while (!heap.empty()) {
+            std::pop_heap(heap.begin(), heap.end(), greater);
+            const uint32_t i = heap.back();
+            const std::string &l = in[i].line;
+            if (std::fwrite(l.data(), 1, l.size(), out) != l.size() || std::fputc('\n', out) == EOF) {
+                rc = KC_EIO;
+                break;
+            }
+            if (in[i].next()) std::push_heap(heap.begin(), heap.end(), greater);
+            else heap.pop_back();
+        }
+        if (std::fclose(out) != 0 && rc == KC_OK) rc = KC_EIO;
+    }
+    for (auto &r : in)
+        if (r.f) std::fclose(r.f);
+    return rc;
+}
+
+static int append_file(const std::string &from, FILE *to)
+{
+    FILE *f = std::fopen(from.c_str(), "rb");
+    if (!f) return KC_EIO;
+    std::vector<char> b(1 << 20);
+    size_t r;
+    int rc = KC_OK;
+    while ((r = std::fread(b.data(), 1, b.size(), f)) > 0)
+        if (std::fwrite(b.data(), 1, r, to) != r) {
+            rc = KC_EIO;
+            break;
+        }
+    std::fclose(f);
+    return rc;
+}
+
+extern "C" int kc_count_fastq_files_passes(kc_table *t, const char *const *paths, uint32_t n_paths, uint32_t threads,
+                                           const kc_tune *tune, const kc_pass_plan *plan, kc_ingest_stats *stats)
+{
+    if (!t || !plan) return KC_EINVAL;
+    const uint32_t P = plan->n_passes;
+    if (P == 0 || P > 256 || (P & (P - 1))) return KC_EINVAL;
+    if (plan->hist_bins && plan->hist_max < plan->hist_min) return KC_EINVAL;
+    if (plan->dump_path && plan->dump_max < plan->dump_min) return KC_EINVAL;
+    const uint64_t nb = plan->hist_bins ? (uint64_t)plan->hist_max - plan->hist_min + 1 : 0;
+    std::vector<int64_t> acc(nb, 0), bins(nb, 0);
+    std::vector<std::string> parts;
+    kc_ingest_stats sum{};
+    int rc = KC_OK;
+    for (uint32_t p = 0; p < P && rc == KC_OK; ++p) {
+        if ((rc = kc_table_clear(t)) != KC_OK || (rc = kc_table_set_pass(t, p, P)) != KC_OK) break;
+        kc_ingest_stats st{};
+        if ((rc = kc_count_fastq_files(t, paths, n_paths, threads, tune, &st)) != KC_OK) break;
+        sum.files = st.files;
+        sum.threads = st.threads;
+        sum.input_bytes += st.input_bytes;
+        sum.framed_bytes += st.framed_bytes;
+        sum.chunks += st.chunks;
+        sum.launches += st.launches;
+        sum.inflate_seconds += st.inflate_seconds;
+        sum.wall_seconds += st.wall_seconds;
+        if (plan->export_dir)
+            rc = p == 0 ? kc_table_export_dir(t, plan->export_dir) : kc_table_export_dir_append(t, plan->export_dir);
+        if (rc == KC_OK && plan->dump_path) {
+            const std::string part = P == 1 ? std::string(plan->dump_path)
+                                            : std::string(plan->dump_path) + ".pass" + std::to_string(p);
+            rc = kc_table_dump_file(t, plan->dump_min, plan->dump_max, plan->dump_sorted, part.c_str());
+            parts.push_back(part);
+        }
+        if (rc == KC_OK && nb) {
+            rc = kc_table_histogram(t, plan->hist_min, plan->hist_max, bins.data());
+            for (uint64_t i = 0; i < nb && rc == KC_OK; ++i) acc[i] += bins[i];
+        }
+    }
+    kc_table_set_pass(t, 0, 1);
+    if (rc == KC_OK && plan->dump_path && P > 1) {
+        // slices are hash-disjoint: sorted parts merge, unsorted parts concatenate
+        if (plan->dump_sorted) {
+            std::vector<const char *> pp;
+            for (auto &s : parts) pp.push_back(s.c_str());
+            rc = kc_merge_sorted_dumps(pp.data(), (uint32_t)pp.size(), plan->dump_path);
+        } else {
+            FILE *out = std::fopen(plan->dump_path, "wb");
+            if (!out) rc = KC_EIO;
+            for (size_t i = 0; i < parts.size() && rc == KC_OK; ++i) rc = append_file(parts[i], out);
+            if (out && std::fclose(out) != 0 && rc == KC_OK) rc = KC_EIO;
+        }
+    }
+    if (P > 1)
+        for (auto &s : parts) std::remove(s.c_str());
+    if (rc == KC_OK && nb) std::memcpy(plan->hist_bins, acc.data(), nb * sizeof(int64_t));
+    if (stats) *stats = sum;
+    return rc;
 }

@@ -97,6 +97,8 @@ struct kc_table {
     hipEvent_t copy_done;   // last copy out of a page-locked caller buffer
     kc_part_scratch ps;
     kc_tune tune;           // launch shape of counting calls (zeros: defaults)
+    uint32_t pass_bits;     // multi-pass counting: log2(passes) (kc_table_set_pass)
+    uint32_t pass;
 };
 
 // Slots / Bloom blocks of one partition that the LDS counting pass holds.

@@ -63,7 +63,15 @@ struct Digits {
     uint32_t part_bits;  // L
     uint32_t l1;         // coarse bits = min(L, 8)
     uint32_t l2;         // fine bits = L - l1
+    uint32_t pass_bits;  // multi-pass counting: only keys of `pass` enter
+    uint32_t pass;
 };
+
+template <int NW>
+__device__ __forceinline__ bool in_pass_of(const uint64_t *key, const Digits &d)
+{
+    return in_pass<NW>(key, d.pass_bits, d.pass);
+}
 
 template <int NW>
 __device__ __forceinline__ uint32_t coarse_of(const uint64_t *key, const Digits &d)
@@ -139,7 +147,7 @@ __global__ __launch_bounds__(kParseBlock) void k_front_count_seq(SeqSrc src, uin
     for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         stage_tile(src.seq, n, tile, lds);
         roll_segment<NW>(lds, tile, n, emit_from, k, [&](const uint64_t *key) {
-            atomicAdd(&hist[coarse_of<NW>(key, d)], 1u);
+            if (in_pass_of<NW>(key, d)) atomicAdd(&hist[coarse_of<NW>(key, d)], 1u);
         });
         __syncthreads();  // the next tile overwrites lds
     }
@@ -214,9 +222,11 @@ __global__ __launch_bounds__(TB) void k_front_scatter_seq_staged(SeqSrc src, uin
                     const bool emit = have >= k && (uint32_t)(4 * (q * kWords + jj) + b) >= skip;
                     if (emit) {
                         roll.canon(&key[e * NW]);
-                        const uint32_t c = coarse_of<NW>(&key[e * NW], d);
-                        rb[e] = (atomicAdd(&hist[c], 1u) << 8) | c;
-                        emitted |= 1u << e;
+                        if (in_pass_of<NW>(&key[e * NW], d)) {
+                            const uint32_t c = coarse_of<NW>(&key[e * NW], d);
+                            rb[e] = (atomicAdd(&hist[c], 1u) << 8) | c;
+                            emitted |= 1u << e;
+                        }
                     }
                 }
             }
@@ -266,7 +276,7 @@ __global__ __launch_bounds__(kKeyBlock) void k_front_count_keys(const uint64_t *
     for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         for (int j = 0; j < kKeysPerThread; ++j) {
             const uint64_t i = tile * kKeyTile + (uint64_t)j * kKeyBlock + threadIdx.x;
-            if (i < n) atomicAdd(&hist[coarse_of<NW>(keys + i * NW, d)], 1u);
+            if (i < n && in_pass_of<NW>(keys + i * NW, d)) atomicAdd(&hist[coarse_of<NW>(keys + i * NW, d)], 1u);
         }
     }
     __syncthreads();
@@ -285,7 +295,7 @@ __global__ __launch_bounds__(kKeyBlock) void k_front_scatter_keys(const uint64_t
     for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         for (int j = 0; j < kKeysPerThread; ++j) {
             const uint64_t i = tile * kKeyTile + (uint64_t)j * kKeyBlock + threadIdx.x;
-            if (i < n) {
+            if (i < n && in_pass_of<NW>(keys + i * NW, d)) {
                 const uint64_t pos = atomicAdd(&cur[coarse_of<NW>(keys + i * NW, d)], 1ull);
 #pragma unroll
                 for (int w = 0; w < NW; ++w) out[pos * NW + w] = keys[i * NW + w];
@@ -685,6 +695,8 @@ Digits digits_of(const kc_table *t)
     d.part_bits = t->part_bits;
     d.l1 = std::min<uint32_t>(d.part_bits, kCoarseMaxBits);
     d.l2 = d.part_bits - d.l1;
+    d.pass_bits = t->pass_bits;
+    d.pass = t->pass;
     return d;
 }
 

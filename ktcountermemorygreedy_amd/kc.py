@@ -71,6 +71,12 @@ class IngestStats(C.Structure):
                 ("inflate_seconds", C.c_double), ("wall_seconds", C.c_double)]
 
 
+class PassPlan(C.Structure):
+    _fields_ = [("n_passes", C.c_uint32), ("export_dir", C.c_char_p), ("dump_path", C.c_char_p),
+                ("dump_min", C.c_uint32), ("dump_max", C.c_uint32), ("dump_sorted", C.c_int),
+                ("hist_min", C.c_uint32), ("hist_max", C.c_uint32), ("hist_bins", C.c_void_p)]
+
+
 class RequestResult(C.Structure):
     _fields_ = [("status", C.c_int), ("reserved", C.c_int), ("latency_ms", C.c_double),
                 ("queue_ms", C.c_double), ("kmers", C.c_uint64), ("distinct", C.c_uint64)]
@@ -103,7 +109,9 @@ EXPORTS = {
     "kc_table_increment": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
     "kc_table_query": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
     "kc_table_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
+    "kc_table_set_pass": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32]),
     "kc_table_export_dir": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "kc_table_export_dir_append": (C.c_int, [C.c_void_p, C.c_char_p]),
     "kc_table_export_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64),
                                          C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
     "kc_table_histogram": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]),
@@ -144,6 +152,9 @@ EXPORTS = {
     "kc_host_free": (None, [C.c_void_p]),
     "kc_count_fastq_files": (C.c_int, [C.c_void_p, C.POINTER(C.c_char_p), C.c_uint32, C.c_uint32,
                                        C.POINTER(Tune), C.POINTER(IngestStats)]),
+    "kc_count_fastq_files_passes": (C.c_int, [C.c_void_p, C.POINTER(C.c_char_p), C.c_uint32, C.c_uint32,
+                                              C.POINTER(Tune), C.POINTER(PassPlan), C.POINTER(IngestStats)]),
+    "kc_merge_sorted_dumps": (C.c_int, [C.POINTER(C.c_char_p), C.c_uint32, C.c_char_p]),
     "kc_synth_genome": (C.c_int, [C.POINTER(GenomeSpec), C.c_void_p]),
     "kc_synth_reads_size": (C.c_int64, [C.c_void_p, C.c_uint64, C.POINTER(ReadSpec), C.c_int]),
     "kc_synth_reads": (C.c_int64, [C.c_void_p, C.c_uint64, C.POINTER(ReadSpec), C.c_int,
@@ -408,8 +419,16 @@ class Table:
         check(lib().kc_table_stats(self.h, C.byref(s)), "kc_table_stats")
         return {f: getattr(s, f) for f, _ in Stats._fields_}
 
-    def export_dir(self, path):
-        check(lib().kc_table_export_dir(self.h, path.encode()), "kc_table_export_dir")
+    def export_dir(self, path, append=False):
+        if append:
+            check(lib().kc_table_export_dir_append(self.h, path.encode()), "kc_table_export_dir_append")
+        else:
+            check(lib().kc_table_export_dir(self.h, path.encode()), "kc_table_export_dir")
+
+    def set_pass(self, pass_id, n_passes):
+        """Later counting calls count only the k-mers of hash slice pass_id of
+        n_passes (a power of two <= 256); set_pass(0, 1) restores one pass."""
+        check(lib().kc_table_set_pass(self.h, pass_id, n_passes), "kc_table_set_pass")
 
     def export_sizes(self):
         hb, lb = C.c_uint64(), C.c_uint64()
@@ -549,6 +568,31 @@ def count_fastq_files(table: "Table", paths, threads=4, batch_bytes=0, queue_dep
     check(lib().kc_count_fastq_files(table.h, arr, len(paths), threads, C.byref(tune) if tune else None,
                                      C.byref(st)), "kc_count_fastq_files")
     return {f: getattr(st, f) for f, _ in IngestStats._fields_}
+
+
+def count_fastq_files_passes(table: "Table", paths, n_passes, threads=4, export_dir=None, dump_path=None,
+                             dump_range=(1, 0xFFFFFFFF), dump_sorted=True, hist_range=None):
+    """Out-of-HBM counting: n_passes passes over the files, pass p counting
+    hash slice p in the cleared table, exports appended to export_dir, dumps
+    merged into dump_path, histograms summed.  Returns (bins or None, stats)."""
+    arr = (C.c_char_p * max(1, len(paths)))(*[os.fsencode(p) for p in paths])
+    bins = None
+    plan = PassPlan(n_passes=n_passes, export_dir=os.fsencode(export_dir) if export_dir else None,
+                    dump_path=os.fsencode(dump_path) if dump_path else None,
+                    dump_min=dump_range[0], dump_max=dump_range[1], dump_sorted=int(bool(dump_sorted)))
+    if hist_range is not None:
+        bins = np.zeros(hist_range[1] - hist_range[0] + 1, dtype=np.int64)
+        plan.hist_min, plan.hist_max, plan.hist_bins = hist_range[0], hist_range[1], bins.ctypes.data
+    st = IngestStats()
+    check(lib().kc_count_fastq_files_passes(table.h, arr, len(paths), threads, None, C.byref(plan), C.byref(st)),
+          "kc_count_fastq_files_passes")
+    return bins, {f: getattr(st, f) for f, _ in IngestStats._fields_}
+
+
+def merge_sorted_dumps(paths, out_path):
+    """k-way merge of key-sorted dump texts (host)."""
+    arr = (C.c_char_p * max(1, len(paths)))(*[os.fsencode(p) for p in paths])
+    check(lib().kc_merge_sorted_dumps(arr, len(paths), os.fsencode(out_path)), "kc_merge_sorted_dumps")
 
 
 def synth_genome(seed, length, n_segments=0, segment_len=0, repeat_fraction=0.0,

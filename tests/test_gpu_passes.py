@@ -1,0 +1,137 @@
+"""Multi-pass counting, the out-of-HBM fallback: the k-mers are counted in
+2^b hash slices, one pass each (kc_table_set_pass), so a table needs room for
+one slice only.  Each slice holds exactly the keys whose XXH3_64 top b bits
+name it (oracle hash) with their one-pass counts, on both counting paths and
+through increments and batches; the multi-pass file driver's merged dump and
+summed histogram of the 5 Mbp workload, and its concatenated export read by
+the reference's own tools, equal the reference's goldens byte for byte."""
+import gzip
+import hashlib
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.kc_testlib import load_json
+from tests.test_gpu_tools import REF, run_tools
+
+pytestmark = pytest.mark.gpu
+
+
+def _slices(uniq, bits):
+    return np.array([O.xxh3_64(u) >> (64 - bits) for u in uniq], dtype=np.uint64)
+
+
+def _small_input():
+    from ktcountermemorygreedy_amd import workloads as W
+    g = W.ReadSet("g", 3, 400_000, 13, 0).genome()
+    return W.COUNT_5MBP.scaled(6000).reads(genome=g).tobytes()
+
+
+@pytest.mark.parametrize("bloom", [False, True], ids=["plain", "bloom"])
+@pytest.mark.parametrize("path", ["direct", "partitioned"])
+@pytest.mark.parametrize("k", [31, 63])
+def test_pass_slices_exact(ctx, k, path, bloom):
+    from ktcountermemorygreedy_amd import kc
+    seq = _small_input()
+    nw = O.words_for_k(k)
+    keys = O.extract(seq, k)
+    uniq, cnt = O.count(keys, nw)
+    P = 4
+    sl = _slices(uniq, 2)
+    t = kc.Table(ctx, k, capacity=1 << 20, bloom=bloom, max_distinct=len(seq), path=path)
+    d = ctx.upload(np.frombuffer(seq, dtype=np.uint8))
+    total = 0
+    for p in range(P):
+        t.clear()
+        t.set_pass(p, P)
+        t.count_seq(d, len(seq))
+        ctx.sync()
+        st = t.stats()
+        mine = sl == p
+        assert st["errors"] == 0
+        assert st["kmers"] == int(cnt[mine].sum()), (p, st)
+        bins = t.histogram(1, 1000)
+        want = np.bincount(np.minimum(cnt[mine], 1000).astype(np.int64), minlength=1001)[1:]
+        assert np.array_equal(bins, want), p
+        got = t.query(uniq).astype(np.uint64)
+        assert np.array_equal(got, np.where(mine, cnt, 0)), p
+        total += st["kmers"]
+    assert total == keys.shape[0]
+    # one pass again: everything
+    t.clear()
+    t.set_pass(0, 1)
+    t.count_seq(d, len(seq))
+    ctx.sync()
+    assert t.stats()["kmers"] == keys.shape[0]
+    d.free()
+    t.close()
+
+
+def test_pass_increment_and_batch(ctx):
+    from ktcountermemorygreedy_amd import kc
+    seq = _small_input()
+    keys = O.extract(seq, 31)
+    uniq, cnt = O.count(keys, 1)
+    sl = _slices(uniq, 3)
+    # increments: only the slice's keys are counted
+    t = kc.Table(ctx, 31, capacity=1 << 21, max_distinct=len(seq))
+    t.set_pass(5, 8)
+    t.increment(np.ascontiguousarray(keys))
+    got = t.query(uniq).astype(np.uint64)
+    assert np.array_equal(got, np.where(sl == 5, cnt, 0))
+    t.close()
+    # a batch created after set_pass counts the slice
+    t = kc.Table(ctx, 31, capacity=1 << 21, bloom=True, max_distinct=len(seq))
+    t.set_pass(2, 8)
+    b = kc.Batch(t, batch_bytes=1 << 20, queue_depth=2)
+    lines = seq.split(b"\n")[:-1]
+    for i in range(0, len(lines), 700):
+        b.add(np.frombuffer(b"".join(x + b"\n" for x in lines[i:i + 700]), dtype=np.uint8))
+    b.flush()
+    b.sync()
+    b.close()
+    st = t.stats()
+    assert st["errors"] == 0 and st["kmers"] == int(cnt[sl == 2].sum())
+    got = t.query(uniq).astype(np.uint64)
+    assert np.array_equal(got, np.where(sl == 2, cnt, 0))
+    t.close()
+
+
+def test_set_pass_rejects_bad_arguments(ctx):
+    from ktcountermemorygreedy_amd import kc
+    t = kc.Table(ctx, 31, capacity=1 << 16)
+    for pass_id, n in [(0, 0), (0, 3), (4, 4), (0, 512)]:
+        with pytest.raises(kc.KCError):
+            t.set_pass(pass_id, n)
+    t.close()
+
+
+@pytest.mark.parametrize("n_passes", [1, 4])
+def test_passes_c1_goldens(ctx, tmp_path, n_passes):
+    """count_5mbp (reference goldens): the multi-pass file driver's sorted
+    dump and histogram, and the reference tools over its appended export."""
+    from ktcountermemorygreedy_amd import kc
+    from ktcountermemorygreedy_amd import workloads as W
+    g = load_json("count_5mbp.json")
+    fq = W.COUNT_5MBP.reads(fastq=True).tobytes()
+    assert hashlib.sha256(fq).hexdigest() == g["fastq_sha256"]
+    path = tmp_path / "c1.fastq.gz"
+    path.write_bytes(gzip.compress(fq, compresslevel=1))
+    exp = tmp_path / "exp"
+    exp.mkdir()
+    dump = tmp_path / "dump.txt"
+    t = kc.Table(ctx, 31, count_max=g["count_max"], max_distinct=g["distinct"] // n_passes + (1 << 18), bloom=True)
+    bins, st = kc.count_fastq_files_passes(t, [str(path)], n_passes, threads=2, export_dir=str(exp),
+                                           dump_path=str(dump), hist_range=(1, 10000))
+    print("passes", n_passes, st, flush=True)
+    assert st["framed_bytes"] > 0
+    with open(dump, "rb") as f:
+        assert hashlib.sha256(f.read()).hexdigest() == g["dump_sha256"]
+    assert hashlib.sha256(O.histogram_text(bins, 1, 10000)).hexdigest() == g["hist_sha256"]
+    t.close()
+    if os.path.exists(os.path.join(REF, "dump_tool_ref")):
+        rd, rh = run_tools(str(exp), 10000)
+        assert hashlib.sha256(rd).hexdigest() == g["dump_sha256"]
+        assert hashlib.sha256(rh).hexdigest() == g["hist_sha256"]

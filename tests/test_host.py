@@ -2,6 +2,7 @@
 import hashlib
 
 import numpy as np
+import pytest
 
 from oracle import oracle as O
 from tests.kc_testlib import load_json, read_golden
@@ -48,3 +49,28 @@ def test_uneven_reads_lengths():
     short = lens == 150
     assert 0.75 < short.mean() < 0.85
     assert 6000 < lens[~short].mean() < 10000
+
+
+def test_merge_sorted_dumps(tmp_path):
+    """k-way merge of key-sorted dump texts (the multi-pass dump merge):
+    equals the sorted union; empty inputs and a missing final newline."""
+    from ktcountermemorygreedy_amd import kc
+    rng = np.random.default_rng(7)
+    keys = sorted({"".join(rng.choice(list("ACGT"), 21)) for _ in range(3000)})
+    parts = [[], [], [], []]
+    for i, key in enumerate(keys):
+        parts[int(rng.integers(0, 3))].append("%s\t%d\n" % (key, i % 97 + 1))
+    paths = []
+    for j, lines in enumerate(parts):  # part 3 stays empty
+        p = tmp_path / ("p%d.txt" % j)
+        text = "".join(lines)
+        if j == 1 and text:
+            text = text[:-1]
+        p.write_text(text)
+        paths.append(str(p))
+    out = tmp_path / "merged.txt"
+    kc.merge_sorted_dumps(paths, str(out))
+    want = "".join("%s\t%d\n" % (key, i % 97 + 1) for i, key in enumerate(keys))
+    assert out.read_text() == want
+    with pytest.raises(kc.KCError):
+        kc.merge_sorted_dumps([str(tmp_path / "missing.txt")], str(out))
