@@ -27,10 +27,14 @@ constexpr int kThreads = 256;
 // the fewest passes of at most kSortDigitBits bits.
 constexpr int kSortDigitBits = 9;  // 5 passes over 42 bits; 11 (4 passes) measured slower: one block per CU
 
+// Downsweep tiles: TB threads x ITEMS records (8192 records of up to 16
+// bytes, 4096 of 20-24 bytes), staged in LDS (96-128 KB) so that a digit's
+// records of a tile leave as one run (16 records per digit at 9 bits).
 template <int RW> struct Layout {
     static constexpr int NW = RW >= 5 ? 2 : 1;   // key words (u64)
-    static constexpr int ITEMS = RW >= 5 ? 8 : 16; // records per thread per tile
-    static constexpr int TILE = kThreads * ITEMS;
+    static constexpr int TB = 1024;
+    static constexpr int ITEMS = RW >= 5 ? 4 : 8; // records per thread per tile
+    static constexpr int TILE = TB * ITEMS;
 };
 
 // Digit of bits [bit, bit+width) of the key (LSB-numbered over the whole
@@ -75,6 +79,10 @@ __device__ __forceinline__ void block_tiles(uint64_t n_tiles, uint64_t &first, u
     last = first + per < n_tiles ? first + per : n_tiles;
 }
 
+// Digit counts of block b's records: the records of its downsweep tiles,
+// read in chunks of kThreads x kUpItems.
+constexpr int kUpItems = 16;
+
 template <int RW, int DB>
 __global__ __launch_bounds__(kThreads) void k_upsweep(const uint32_t *__restrict__ src, uint64_t n, uint32_t bit,
                                                       uint32_t width, uint64_t n_tiles, uint32_t *__restrict__ counts)
@@ -86,18 +94,19 @@ __global__ __launch_bounds__(kThreads) void k_upsweep(const uint32_t *__restrict
     __syncthreads();
     uint64_t first, last;
     block_tiles(n_tiles, first, last);
-    for (uint64_t tile = first; tile < last; ++tile) {
-        const uint64_t t0 = tile * L::TILE;
-        uint32_t r[L::ITEMS][RW];
-        // all loads first (index clamped into the array), then the counts
+    const uint64_t r0 = first * L::TILE, r1 = last * L::TILE < n ? last * L::TILE : n;
+    for (uint64_t c0 = r0; c0 < r1; c0 += (uint64_t)kThreads * kUpItems) {
+        uint32_t r[kUpItems][RW];
+        // all loads first (index clamped into the range), then the counts
 #pragma unroll
-        for (int j = 0; j < L::ITEMS; ++j) {
-            const uint64_t i = t0 + (uint64_t)j * kThreads + threadIdx.x;
-            load_rec<RW>(src, i < n ? i : n - 1, r[j]);
+        for (int j = 0; j < kUpItems; ++j) {
+            const uint64_t i = c0 + (uint64_t)j * kThreads + threadIdx.x;
+            load_rec<RW>(src, i < r1 ? i : r1 - 1, r[j]);
         }
 #pragma unroll
-        for (int j = 0; j < L::ITEMS; ++j)
-            if (t0 + (uint64_t)j * kThreads + threadIdx.x < n) atomicAdd(&hist[digit_of<L::NW>(r[j], bit, width)], 1u);
+        for (int j = 0; j < kUpItems; ++j)
+            if (c0 + (uint64_t)j * kThreads + threadIdx.x < r1)
+                atomicAdd(&hist[digit_of<L::NW>(r[j], bit, width)], 1u);
     }
     __syncthreads();
     for (uint32_t b = threadIdx.x; b < kBins; b += kThreads) counts[(uint64_t)b * gridDim.x + blockIdx.x] = hist[b];
@@ -174,23 +183,26 @@ __global__ __launch_bounds__(kThreads) void k_scan_apply(const uint32_t *in, uin
 
 // Stable scatter.  Block b walks its tiles in order, carrying a running
 // global offset per digit.  Inside a tile, wave w ranks its own contiguous
-// quarter (records w*TILE/4 .. (w+1)*TILE/4, 64 consecutive records per
-// step) with wave-private digit counters: same-digit peers from 8 ballots,
-// rank = counter + peers before the lane, then the group's first lane bumps
-// the counter.  One cross-wave prefix per digit and one scan over digits
-// place every record; the tile is staged digit-sorted in LDS and written out
-// in runs.  Order inside a digit = input order, so the sort is stable.
+// share (records w*TILE/W .. (w+1)*TILE/W of W waves, 64 consecutive
+// records per step) with wave-private digit counters: same-digit peers from
+// one ballot per digit bit, rank = counter + peers before the lane, then the
+// group's first lane bumps the counter.  One cross-wave prefix per digit and
+// one scan over digits place every record; the tile is staged digit-sorted
+// in LDS and written out in runs.  Order inside a digit = input order, so
+// the sort is stable.
 template <int RW, int DB>
-__global__ __launch_bounds__(kThreads) void k_downsweep(const uint32_t *__restrict__ src, uint32_t *__restrict__ dst,
-                                                        uint64_t n, uint32_t bit, uint32_t width, uint64_t n_tiles,
-                                                        const uint64_t *__restrict__ offsets)
+__global__ __launch_bounds__(Layout<RW>::TB) void k_downsweep(const uint32_t *__restrict__ src,
+                                                              uint32_t *__restrict__ dst, uint64_t n, uint32_t bit,
+                                                              uint32_t width, uint64_t n_tiles,
+                                                              const uint64_t *__restrict__ offsets)
 {
     using L = Layout<RW>;
+    constexpr int TB = L::TB;
     constexpr uint32_t kBins = 1u << DB;
-    constexpr uint32_t kPer = kBins / kThreads;  // digits owned by one thread (contiguous)
-    constexpr int kWaves = kThreads / 64;
-    constexpr uint32_t kQuarter = L::TILE / kWaves;  // records per wave per tile
-    static_assert(kBins % kThreads == 0, "at least one digit per thread");
+    constexpr uint32_t kPer = (kBins + TB - 1) / TB;  // digits owned by one thread (contiguous)
+    constexpr int kWaves = TB / 64;
+    constexpr uint32_t kPerWave = L::TILE / kWaves;  // records per wave per tile
+    static_assert(kPerWave <= 0xFFFF && L::TILE <= 0xFFFF, "16-bit wave counters");
     __shared__ uint32_t stage[L::TILE * RW];
     __shared__ uint16_t wrun[kWaves][kBins];  // per wave: digit counts, then the wave's start in the digit
     __shared__ uint32_t local_start[kBins];
@@ -198,18 +210,18 @@ __global__ __launch_bounds__(kThreads) void k_downsweep(const uint32_t *__restri
     __shared__ uint64_t gbase[kBins];
     const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63;
     const uint64_t lane_lt = (1ULL << lane) - 1ULL;
-    for (uint32_t d = t; d < kBins; d += kThreads) gbase[d] = offsets[(uint64_t)d * gridDim.x + blockIdx.x];
+    for (uint32_t d = t; d < kBins; d += TB) gbase[d] = offsets[(uint64_t)d * gridDim.x + blockIdx.x];
     uint64_t first, last;
     block_tiles(n_tiles, first, last);
     for (uint64_t tile = first; tile < last; ++tile) {
         const uint64_t t0 = tile * L::TILE;
         const uint32_t tn = (uint32_t)(n - t0 < (uint64_t)L::TILE ? n - t0 : L::TILE);
-        for (uint32_t d = t; d < kBins; d += kThreads)
+        for (uint32_t d = t; d < kBins; d += TB)
             for (int w = 0; w < kWaves; ++w) wrun[w][d] = 0;
         uint32_t r[L::ITEMS][RW];
 #pragma unroll
         for (int j = 0; j < L::ITEMS; ++j) {
-            const uint32_t li = wave * kQuarter + (uint32_t)j * 64 + lane;
+            const uint32_t li = wave * kPerWave + (uint32_t)j * 64 + lane;
             const uint64_t i = t0 + (li < tn ? li : tn - 1);
             load_rec<RW>(src, i, r[j]);
         }
@@ -217,7 +229,7 @@ __global__ __launch_bounds__(kThreads) void k_downsweep(const uint32_t *__restri
         uint32_t rank[L::ITEMS], dig[L::ITEMS];
 #pragma unroll
         for (int j = 0; j < L::ITEMS; ++j) {
-            const uint32_t li = wave * kQuarter + (uint32_t)j * 64 + lane;
+            const uint32_t li = wave * kPerWave + (uint32_t)j * 64 + lane;
             const bool valid = li < tn;
             const uint32_t d = valid ? digit_of<L::NW>(r[j], bit, width) : 0;
             dig[j] = d;
@@ -241,11 +253,13 @@ __global__ __launch_bounds__(kThreads) void k_downsweep(const uint32_t *__restri
         for (uint32_t q = 0; q < kPer; ++q) {
             const uint32_t d = t * kPer + q;
             uint32_t acc = 0;
+            if (d < kBins) {
 #pragma unroll
-            for (int w = 0; w < kWaves; ++w) {
-                const uint32_t c = wrun[w][d];
-                wrun[w][d] = (uint16_t)acc;
-                acc += c;
+                for (int w = 0; w < kWaves; ++w) {
+                    const uint32_t c = wrun[w][d];
+                    wrun[w][d] = (uint16_t)acc;
+                    acc += c;
+                }
             }
             tot[q] = acc;
             mine += acc;
@@ -265,14 +279,15 @@ __global__ __launch_bounds__(kThreads) void k_downsweep(const uint32_t *__restri
             run += x - mine;
 #pragma unroll
             for (uint32_t q = 0; q < kPer; ++q) {
-                local_start[t * kPer + q] = run;
+                const uint32_t d = t * kPer + q;
+                if (d < kBins) local_start[d] = run;
                 run += tot[q];
             }
         }
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < L::ITEMS; ++j) {
-            const uint32_t li = wave * kQuarter + (uint32_t)j * 64 + lane;
+            const uint32_t li = wave * kPerWave + (uint32_t)j * 64 + lane;
             if (li < tn) {
                 const uint32_t p = local_start[dig[j]] + wrun[wave][dig[j]] + rank[j];
 #pragma unroll
@@ -282,7 +297,7 @@ __global__ __launch_bounds__(kThreads) void k_downsweep(const uint32_t *__restri
         __syncthreads();
         // write out in local order: consecutive positions of one digit land
         // on consecutive global addresses
-        for (uint32_t p = t; p < tn; p += kThreads) {
+        for (uint32_t p = t; p < tn; p += TB) {
             const uint32_t *rec = stage + p * RW;
             const uint32_t d = digit_of<L::NW>(rec, bit, width);
             const uint64_t g = gbase[d] + (p - local_start[d]);
@@ -291,7 +306,10 @@ __global__ __launch_bounds__(kThreads) void k_downsweep(const uint32_t *__restri
         }
         __syncthreads();
 #pragma unroll
-        for (uint32_t q = 0; q < kPer; ++q) gbase[t * kPer + q] += tot[q];
+        for (uint32_t q = 0; q < kPer; ++q) {
+            const uint32_t d = t * kPer + q;
+            if (d < kBins) gbase[d] += tot[q];
+        }
     }
 }
 
@@ -401,7 +419,7 @@ int sort_passes(kc_ctx *ctx, uint32_t *a, uint32_t *b, uint64_t n, uint32_t lo, 
         hipLaunchKernelGGL(k_scan_reduce, dim3((uint32_t)nb), dim3(kThreads), 0, ctx->stream, s.counts, m, s.sums);
         hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kThreads), 0, ctx->stream, s.sums, nb);
         hipLaunchKernelGGL(k_scan_apply, dim3((uint32_t)nb), dim3(kThreads), 0, ctx->stream, s.counts, m, s.sums, s.offsets);
-        hipLaunchKernelGGL((k_downsweep<RW, DB>), dim3(grid), dim3(kThreads), 0, ctx->stream, src, dst, n, bit, width, n_tiles, s.offsets);
+        hipLaunchKernelGGL((k_downsweep<RW, DB>), dim3(grid), dim3(L::TB), 0, ctx->stream, src, dst, n, bit, width, n_tiles, s.offsets);
         if ((e = hipGetLastError()) != hipSuccess) break;
         std::swap(src, dst);
         ++passes;
