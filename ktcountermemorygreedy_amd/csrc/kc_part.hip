@@ -134,7 +134,7 @@ __device__ __forceinline__ uint32_t scan_block(uint32_t v, uint32_t *wsum, uint3
 }
 
 // ---- front end over framed bytes (persistent: block b owns tiles b + iG) --
-template <int NW>
+template <int NW, bool PASS>
 __global__ __launch_bounds__(kParseBlock) void k_front_count_seq(SeqSrc src, uint32_t k, Digits d,
                                                                  uint32_t *__restrict__ m1)
 {
@@ -147,7 +147,7 @@ __global__ __launch_bounds__(kParseBlock) void k_front_count_seq(SeqSrc src, uin
     for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         stage_tile(src.seq, n, tile, lds);
         roll_segment<NW>(lds, tile, n, emit_from, k, [&](const uint64_t *key) {
-            if (in_pass_of<NW>(key, d)) atomicAdd(&hist[coarse_of<NW>(key, d)], 1u);
+            if (!PASS || in_pass_of<NW>(key, d)) atomicAdd(&hist[coarse_of<NW>(key, d)], 1u);
         });
         __syncthreads();  // the next tile overwrites lds
     }
@@ -162,7 +162,7 @@ __global__ __launch_bounds__(kParseBlock) void k_front_count_seq(SeqSrc src, uin
 // contiguous runs (TB x CB / 256 keys per digit per chunk on average).  A
 // tile is TB x 64 bytes, the tiles of k_front_count_seq at TB = 256, so the
 // per-block offsets of the count pass apply.
-template <int NW, int TB, int CB>
+template <int NW, int TB, int CB, bool PASS>
 __global__ __launch_bounds__(TB) void k_front_scatter_seq_staged(SeqSrc src, uint32_t k, Digits d,
                                                                  const uint64_t *__restrict__ o1,
                                                                  uint64_t *__restrict__ out)
@@ -222,7 +222,7 @@ __global__ __launch_bounds__(TB) void k_front_scatter_seq_staged(SeqSrc src, uin
                     const bool emit = have >= k && (uint32_t)(4 * (q * kWords + jj) + b) >= skip;
                     if (emit) {
                         roll.canon(&key[e * NW]);
-                        if (in_pass_of<NW>(&key[e * NW], d)) {
+                        if (!PASS || in_pass_of<NW>(&key[e * NW], d)) {
                             const uint32_t c = coarse_of<NW>(&key[e * NW], d);
                             rb[e] = (atomicAdd(&hist[c], 1u) << 8) | c;
                             emitted |= 1u << e;
@@ -798,8 +798,11 @@ int kc_part_enqueue(const kc_table *t, kc_part_scratch &s, const kc_pipe_shape &
 
     // level 1: count, scan, coarse starts, scatter
     if (from_seq) {
-        if (nw == 1) hipLaunchKernelGGL(k_front_count_seq<1>, dim3(G), dim3(kParseBlock), 0, st, seq, t->cfg.k, d, s.m1);
-        else hipLaunchKernelGGL(k_front_count_seq<2>, dim3(G), dim3(kParseBlock), 0, st, seq, t->cfg.k, d, s.m1);
+        // the pass filter (multi-pass counting) is compiled in only where it is used
+        const bool pass = d.pass_bits != 0;
+        auto fc = nw == 1 ? (pass ? &k_front_count_seq<1, true> : &k_front_count_seq<1, false>)
+                          : (pass ? &k_front_count_seq<2, true> : &k_front_count_seq<2, false>);
+        hipLaunchKernelGGL(fc, dim3(G), dim3(kParseBlock), 0, st, seq, t->cfg.k, d, s.m1);
     } else {
         if (nw == 1) hipLaunchKernelGGL(k_front_count_keys<1>, dim3(G), dim3(kKeyBlock), 0, st, d_keys, n_keys, d, s.m1);
         else hipLaunchKernelGGL(k_front_count_keys<2>, dim3(G), dim3(kKeyBlock), 0, st, d_keys, n_keys, d, s.m1);
@@ -809,12 +812,13 @@ int kc_part_enqueue(const kc_table *t, kc_part_scratch &s, const kc_pipe_shape &
     if ((rc = kc_scan_u32_on(st, s.m1, m1n, s.o1, s.sums))) return rc;
     hipLaunchKernelGGL(k_coarse_starts, dim3(1), dim3(256), 0, st, s.o1, s.m1, G, n_coarse, s.starts);
     if (from_seq) {
-        if (nw == 1)
-            hipLaunchKernelGGL((k_front_scatter_seq_staged<1, kFrontBlock1, kFrontChunk1>), dim3(G), dim3(kFrontBlock1), 0,
-                               st, seq, t->cfg.k, d, s.o1, s.keys_a);
-        else
-            hipLaunchKernelGGL((k_front_scatter_seq_staged<2, kFrontBlock2, kFrontChunk2>), dim3(G), dim3(kFrontBlock2), 0,
-                               st, seq, t->cfg.k, d, s.o1, s.keys_a);
+        const bool pass = d.pass_bits != 0;
+        auto f1 = pass ? &k_front_scatter_seq_staged<1, kFrontBlock1, kFrontChunk1, true>
+                       : &k_front_scatter_seq_staged<1, kFrontBlock1, kFrontChunk1, false>;
+        auto f2 = pass ? &k_front_scatter_seq_staged<2, kFrontBlock2, kFrontChunk2, true>
+                       : &k_front_scatter_seq_staged<2, kFrontBlock2, kFrontChunk2, false>;
+        if (nw == 1) hipLaunchKernelGGL(f1, dim3(G), dim3(kFrontBlock1), 0, st, seq, t->cfg.k, d, s.o1, s.keys_a);
+        else hipLaunchKernelGGL(f2, dim3(G), dim3(kFrontBlock2), 0, st, seq, t->cfg.k, d, s.o1, s.keys_a);
     } else {
         if (nw == 1) hipLaunchKernelGGL(k_front_scatter_keys<1>, dim3(G), dim3(kKeyBlock), 0, st, d_keys, n_keys, d, s.o1, s.keys_a);
         else hipLaunchKernelGGL(k_front_scatter_keys<2>, dim3(G), dim3(kKeyBlock), 0, st, d_keys, n_keys, d, s.o1, s.keys_a);
