@@ -140,8 +140,9 @@ int kc_table_stats(kc_table *t, kc_stats *out);
 /* Multi-pass counting for tables larger than HBM: the k-mers are split into
  * n_passes hash slices (n_passes a power of two <= 256; slice = top
  * log2(n_passes) bits of XXH3_64 of the key) and later counting calls
- * (count_seq, count_seq_host, increment, batches created afterwards) count
- * only the k-mers of slice `pass`; kc_stats.kmers counts those.  A table
+ * (count_seq, count_seq_host, increment, batch launches: a batch captures its
+ * graphs again after a change, so flush and sync it before changing the
+ * pass) count only the k-mers of slice `pass`; kc_stats.kmers counts those.  A table
  * then needs room for one slice's distinct k-mers.  Counting every slice,
  * each into a cleared table, and concatenating the exports gives the
  * one-pass result.  The out-of-memory fallback of the reference partitions

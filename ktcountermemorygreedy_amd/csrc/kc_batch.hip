@@ -75,6 +75,7 @@ struct kc_batch {
     std::vector<DevGraph> dev;
     uint64_t tick = 0;
     kc_batch_info info{};
+    uint32_t pass_key = 0;        // table pass (bits << 8 | pass) the graphs were captured with
 };
 
 namespace {
@@ -137,9 +138,22 @@ int capture(kc_batch *b, const uint8_t *seq, uint64_t cap, const uint64_t *hdr, 
     return KC_OK;
 }
 
-int check_table(const kc_batch *b)
+// The captured graphs carry the table's multi-pass filter as a kernel
+// argument: after kc_table_set_pass they are dropped and captured again.
+int sync_pass(kc_batch *b)
 {
-    return (b->t->bloom && b->t->sealed) ? KC_EINVAL : KC_OK;  // FIRST list already folded
+    const uint32_t key = (b->t->pass_bits << 8) | b->t->pass;
+    if (key == b->pass_key) return KC_OK;
+    KC_HIP(hipStreamSynchronize(b->t->ctx->stream));
+    destroy_graphs(b);
+    b->pass_key = key;
+    return KC_OK;
+}
+
+int check_table(kc_batch *b)
+{
+    if (b->t->bloom && b->t->sealed) return KC_EINVAL;  // FIRST list already folded
+    return sync_pass(b);
 }
 
 // Launches slot i over its packed bytes (emit_from = halo of a piece).

@@ -82,20 +82,26 @@ def test_pass_increment_and_batch(ctx):
     got = t.query(uniq).astype(np.uint64)
     assert np.array_equal(got, np.where(sl == 5, cnt, 0))
     t.close()
-    # a batch created after set_pass counts the slice
+    # a batch counts the slice in effect at each launch: its graphs are
+    # captured again after set_pass (one batch used for two slices in turn)
     t = kc.Table(ctx, 31, capacity=1 << 21, bloom=True, max_distinct=len(seq))
-    t.set_pass(2, 8)
     b = kc.Batch(t, batch_bytes=1 << 20, queue_depth=2)
     lines = seq.split(b"\n")[:-1]
-    for i in range(0, len(lines), 700):
-        b.add(np.frombuffer(b"".join(x + b"\n" for x in lines[i:i + 700]), dtype=np.uint8))
-    b.flush()
-    b.sync()
+    for slice_id in (2, 6):
+        t.clear()
+        t.set_pass(slice_id, 8)
+        for i in range(0, len(lines), 700):
+            b.add(np.frombuffer(b"".join(x + b"\n" for x in lines[i:i + 700]), dtype=np.uint8))
+        b.flush()
+        b.sync()
+        st = t.stats()
+        assert st["errors"] == 0 and st["kmers"] == int(cnt[sl == slice_id].sum()), slice_id
+        bins = t.histogram(1, 1000)
+        want = np.bincount(np.minimum(cnt[sl == slice_id], 1000).astype(np.int64), minlength=1001)[1:]
+        assert np.array_equal(bins, want), slice_id
     b.close()
-    st = t.stats()
-    assert st["errors"] == 0 and st["kmers"] == int(cnt[sl == 2].sum())
     got = t.query(uniq).astype(np.uint64)
-    assert np.array_equal(got, np.where(sl == 2, cnt, 0))
+    assert np.array_equal(got, np.where(sl == 6, cnt, 0))
     t.close()
 
 
