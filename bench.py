@@ -373,29 +373,13 @@ def main():
     ctx = kc.Context(0)
     d_seq = ctx.upload(seq_host)
 
-    # table sized for the workload: keys reaching the table (Bloom on: k-mers
-    # seen at least twice, plus false positives) and the FIRST list.
-    # Distinct estimate: the genome's k-mers plus every window that overlaps
-    # a substitution (rate by read class: long reads are the non-150 bp ones).
-    nl = np.flatnonzero(seq_host == 10)
-    lens = np.diff(np.concatenate([[-1], nl])) - 1
-    win = np.maximum(lens - (k - 1), 0).astype(np.float64)
-    rate = np.where(lens == ws.read_len, ws.sub_rate, ws.long_sub_rate if ws.long_fraction else ws.sub_rate)
-    est_distinct = int(ws.genome_len * 1.1 + 1.1 * float(np.sum(win * (1.0 - (1.0 - rate) ** k))))
-    # Bloom on: the table holds k-mers seen twice or more, about the genome's
-    # distinct k-mers; off: every distinct k-mer.  Target load <= ~50%.
-    est_table = ws.genome_len * 1.1 if bloom else est_distinct
-    cap = 1 << max(20, int(np.ceil(np.log2(est_table * 1.8))))
-    if bloom:
-        # Bloom blocks are the default, one LDS-sized Bloom part per table
-        # partition (2048 blocks per 4096 slots at k <= 32, 1024 per 2048
-        # above); keep about 6 distinct keys per 64-bit block or fewer, which
-        # for error-rich inputs means more (emptier) partitions
-        cap = max(cap, 1 << int(np.ceil(np.log2(est_distinct / 6.0 * 2.0))))
-        # the LDS partition pass takes at most 2^18 partitions
-        cap = max(min(cap, (1 << 18) * (4096 if nw == 1 else 2048)), 1 << int(np.ceil(np.log2(est_table * 1.8))))
-    table = kc.Table(ctx, k, count_max=0xFFFFFFFF, capacity=cap, max_distinct=est_distinct, bloom=bloom,
-                     low_capacity=est_distinct if bloom else 0)
+    # table sized from the input itself: one hash slice of its k-mers counted
+    # exactly on the device and scaled (kc_plan_table), outside the timed region
+    t0 = time.time()
+    plan_kw, plan = kc.plan_table(ctx, d_seq, k, S, bloom=bloom, sample_slices=16)
+    plan["seconds"] = time.time() - t0
+    est_distinct = plan["distinct"]
+    table = kc.Table(ctx, k, **plan_kw)
 
     # launch shape: autotuned on the workload itself (outside the timed region)
     from ktcountermemorygreedy_amd import autotune as AT
@@ -497,7 +481,7 @@ def main():
         "config": {"workload": ws.name, "reads": ws.n_reads, "read_len": ws.read_len, "k": k,
                    "bloom": bloom, "seq_bytes": S, "kmers": K, "table_keys": st["distinct"],
                    "first_occurrences": st["low"], "table_slots": st["capacity"],
-                   "export_high_bytes": hb, "export_low_bytes": lb,
+                   "export_high_bytes": hb, "export_low_bytes": lb, "plan": plan,
                    "step": "clear + fused parse/canonical/Bloom/count + export packing (device)"},
         "roofline": {"bound": "hbm", "kernel": COUNT_KERNELS, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,

@@ -27,7 +27,7 @@ extern "C" {
 #define KC_OK 0
 #define KC_EINVAL (-1)   /* bad argument */
 #define KC_ENOMEM (-2)   /* device or host allocation failed */
-#define KC_EFULL (-3)    /* table or FIRST-occurrence list overflowed */
+#define KC_EFULL (-3)    /* table overflowed (a full FIRST-list segment spills into the table) */
 #define KC_EDEVICE (-4)  /* HIP runtime error */
 #define KC_EIO (-5)      /* file error */
 #define KC_ENOTSUP (-6)  /* configuration not supported by this build */
@@ -132,7 +132,10 @@ int kc_table_count_seq_host(kc_table *t, const uint8_t *h_seq, uint64_t nbytes);
 int kc_table_increment(kc_table *t, const uint64_t *d_keys, uint64_t n,
                        const uint32_t *d_inc);
 /* Point lookup: merged count min(c, count_max) of each key, 0 if absent
- * (FlatConcurrentHashMap::find_prepared, FlatConcurrentHashMap.h:142-205). */
+ * (FlatConcurrentHashMap::find_prepared, FlatConcurrentHashMap.h:142-205).
+ * With the Bloom split on, the first lookup folds the FIRST list into the
+ * table, which then needs a slot for every distinct k-mer (KC_EFULL if it
+ * has not; size it with max_distinct, not kc_plan_table's Bloom sizing). */
 int kc_table_query(kc_table *t, const uint64_t *d_keys, uint64_t n,
                    uint32_t *d_counts);
 int kc_table_stats(kc_table *t, kc_stats *out);
@@ -175,6 +178,29 @@ int kc_table_dump_device(kc_table *t, uint32_t min_freq, uint32_t max_freq, int 
                          uint8_t *d_out, uint64_t out_cap, uint64_t *out_bytes);
 int kc_table_dump_file(kc_table *t, uint32_t min_freq, uint32_t max_freq, int sorted,
                        const char *path);
+
+/* ---- table sizing from a device sample.  The reference plans its Bloom
+ *      filters and maps from the file size, a pre-read quality average and a
+ *      prefix sample (main.cpp:41-217, FastqPrefixCounter.h:28-133); here one
+ *      hash slice of the input's own k-mers is counted exactly and scaled. ---- */
+typedef struct kc_plan {
+    uint64_t kmers;         /* estimated windows of the input */
+    uint64_t distinct;      /* estimated distinct k-mers (the FIRST occurrences when the Bloom split is on) */
+    uint64_t repeated;      /* estimated distinct k-mers seen at least twice */
+    uint64_t device_bytes;  /* table + Bloom filter + FIRST list of the planned table (one pass) */
+    uint32_t passes;        /* hash-slice passes (kc_table_set_pass) that keep device_bytes <= hbm_budget */
+    uint32_t sample_slices; /* the estimate is slice 0 of this many, scaled up */
+} kc_plan;
+/* Counts hash slice 0 of sample_slices (a power of two <= 256; raised for
+ * large inputs so the probe table stays below 2^26 keys) of the framed input
+ * d_seq exactly in a scratch table, scales its distinct and repeated keys,
+ * and fills cfg (k, count_max, filters, bloom, count_path as given) with
+ * capacity (table keys at <= ~62% load), bloom_blocks (3-6 distinct keys per
+ * block), low_capacity and max_distinct for ONE of plan->passes passes
+ * (hbm_budget = 0: no limit, one pass).  Synchronous; the scratch table is
+ * freed before returning. */
+int kc_plan_table(kc_ctx *ctx, const uint8_t *d_seq, uint64_t nbytes, uint32_t sample_slices, uint64_t hbm_budget,
+                  kc_table_config *cfg, kc_plan *plan);
 
 /* ---- launch shape and batched counting.  No reference counterpart: the
  *      reference feeds 32 KiB ring blocks with a k-1 base overlap to parser

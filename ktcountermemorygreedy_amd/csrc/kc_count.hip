@@ -66,8 +66,8 @@ __global__ __launch_bounds__(kBlock) void k_count_seq(SeqSrc src, TableView tv)
             bool to_table = true;
             if constexpr (BLOOM) {
                 const bool first = bloom_classify(tv, kh) == 0;
-                first_append<NW>(tv, first, key, n_err, step);
-                to_table = !first;
+                const bool kept = first_append<NW>(tv, first, key, step);
+                to_table = !first || !kept;  // a FIRST the list has no room for is counted here
             }
             if (to_table) table_add<NW>(tv, key, kh, 1, n_new, n_err);
         });
@@ -575,7 +575,6 @@ struct HostCounters {
     uint64_t stat[KC_STAT_N];
     uint64_t seg_fill[KC_LOW_SEGS];  // FIRST keys stored per segment
     uint64_t low;                    // sum of seg_fill
-    bool low_overflow;
 };
 
 // Reads the device counters, sums the statistic lanes and, when the FIRST
@@ -590,11 +589,11 @@ int read_counters(kc_table *t, HostCounters &hc)
         for (int l = 0; l < KC_LANES; ++l) hc.stat[st] += h[((uint64_t)st * KC_LANES + l) * KC_LINE_WORDS];
     }
     hc.low = 0;
-    hc.low_overflow = false;
     std::vector<uint64_t> off(KC_LOW_SEGS + 1);
     for (int sgi = 0; sgi < KC_LOW_SEGS; ++sgi) {
         const uint64_t c = h[KC_CURSOR_BASE + (uint64_t)sgi * KC_LINE_WORDS];
-        hc.low_overflow |= c > t->seg_cap;
+        // a segment's cursor runs past its capacity when it overflowed; the
+        // FIRSTs it had no room for were counted in the table instead
         hc.seg_fill[sgi] = std::min(c, t->seg_cap);
         off[sgi] = hc.low;
         hc.low += hc.seg_fill[sgi];
@@ -750,6 +749,9 @@ static int table_init_device(kc_table *t)
     return KC_OK;
 }
 
+// Smallest table partition the Bloom-driven split above makes.
+constexpr uint64_t kMinPartSlots = 256;
+
 extern "C" int kc_table_create(kc_ctx *ctx, const kc_table_config *cfg, kc_table **out)
 {
     if (!ctx || !cfg || !out) return KC_EINVAL;
@@ -768,9 +770,17 @@ extern "C" int kc_table_create(kc_ctx *ctx, const kc_table_config *cfg, kc_table
     t->nw = cfg->k <= 32 ? 1 : 2;
     t->cap = cfg->capacity ? cfg->capacity : std::max<uint64_t>(1024, next_pow2(cfg->max_distinct + cfg->max_distinct / 2));
     const uint64_t hint = cfg->max_distinct ? cfg->max_distinct : t->cap / 2;
-    // partitions of the size the LDS counting pass holds
+    // partitions of the size the LDS counting pass holds; a Bloom filter
+    // larger than the table's partitions carry (error-rich input: many more
+    // distinct keys than keys seen twice) splits the table into more, smaller
+    // partitions instead, so that each Bloom part still fits the LDS copy
     const uint64_t lds_slots = kc_lds_part_slots(t->nw, cfg->bloom != 0);
-    const uint64_t n_parts = t->cap > lds_slots ? t->cap / lds_slots : 1;
+    uint64_t n_parts = t->cap > lds_slots ? t->cap / lds_slots : 1;
+    if (cfg->bloom && cfg->bloom_blocks) {
+        const uint64_t want = cfg->bloom_blocks / kc_lds_part_bloom(t->nw);
+        n_parts = std::max(n_parts, std::min(want, std::min<uint64_t>(t->cap / kMinPartSlots, 1ull << 18)));
+        n_parts = std::max<uint64_t>(n_parts, 1);
+    }
     t->part_bits = ilog2(n_parts);
     t->part_log2 = ilog2(t->cap) - t->part_bits;
     if (cfg->bloom) {
@@ -778,11 +788,13 @@ extern "C" int kc_table_create(kc_ctx *ctx, const kc_table_config *cfg, kc_table
         t->bloom_blocks = std::max<uint64_t>(t->bloom_blocks, n_parts);
         t->bloom_part_log2 = ilog2(t->bloom_blocks) - t->part_bits;
         // Blocks, their waves and their loop steps rotate over the segments
-        // (first_append): the segments fill evenly (1/16 slack); a grid too
-        // small to reach every segment puts at most one tile of windows
-        // (kTile FIRSTs) into one segment.
+        // (first_append), so the segments fill evenly up to a few wave steps
+        // (a wave step adds at most 64 x 64 windows): 1/8 slack plus four
+        // such steps; a grid too small to reach every segment puts at most
+        // one tile of windows (kTile FIRSTs) into one segment.  A FIRST that
+        // still finds its segment full is counted in the table instead.
         const uint64_t want = cfg->low_capacity ? cfg->low_capacity : hint;
-        t->seg_cap = std::max<uint64_t>((want + want / 16 + KC_LOW_SEGS - 1) / KC_LOW_SEGS + 64,
+        t->seg_cap = std::max<uint64_t>((want + want / 8 + KC_LOW_SEGS - 1) / KC_LOW_SEGS + 4 * 4096,
                                         std::min<uint64_t>(want, kTile));
         t->low_cap = t->seg_cap * KC_LOW_SEGS;
     }
@@ -991,13 +1003,18 @@ static int seal_table(kc_table *t)
     HostCounters hc;
     int rc = read_counters(t, hc);
     if (rc != KC_OK) return rc;
-    if (hc.low_overflow) return KC_EFULL;
+    if (hc.stat[KC_STAT_ERRORS]) return KC_EFULL;
     if (hc.low) {
         const TableView v = view_of(t);
         const uint32_t g = grid_for(hc.low, t->ctx->cu_count);
         if (t->nw == 1) hipLaunchKernelGGL(k_increment_low<1>, dim3(g), dim3(kBlock), 0, t->ctx->stream, hc.low, v);
         else hipLaunchKernelGGL(k_increment_low<2>, dim3(g), dim3(kBlock), 0, t->ctx->stream, hc.low, v);
         KC_HIP(hipGetLastError());
+        // the folded keys need table slots: a table sized for the keys seen
+        // twice only (Bloom split) may not hold them all
+        if ((rc = read_counters(t, hc)) != KC_OK) return rc;
+        t->sealed = 1;
+        if (hc.stat[KC_STAT_ERRORS]) return KC_EFULL;
     }
     t->sealed = 1;
     return KC_OK;
@@ -1027,7 +1044,7 @@ extern "C" int kc_table_stats(kc_table *t, kc_stats *out)
     out->distinct = hc.stat[KC_STAT_DISTINCT];
     out->low = t->sealed ? 0 : hc.low;
     out->capacity = t->cap;
-    out->errors = hc.stat[KC_STAT_ERRORS] + (hc.low_overflow ? 1 : 0);
+    out->errors = hc.stat[KC_STAT_ERRORS];
     return KC_OK;
 }
 
@@ -1039,7 +1056,7 @@ extern "C" int kc_table_export_device(kc_table *t, uint8_t *d_high, uint64_t hig
     HostCounters hc;
     int rc = read_counters(t, hc);
     if (rc != KC_OK) return rc;
-    if (hc.stat[KC_STAT_ERRORS] || hc.low_overflow) return KC_EFULL;
+    if (hc.stat[KC_STAT_ERRORS]) return KC_EFULL;
     const PackSpec ps = pack_spec(t);
     const uint64_t hrec = ps.kmer_bytes + ps.count_bytes;
     const bool list = low_list_active(t);
@@ -1135,7 +1152,7 @@ extern "C" int kc_table_histogram(kc_table *t, uint32_t min_freq, uint32_t max_f
     HostCounters hc;
     int rc = read_counters(t, hc);
     if (rc != KC_OK) return rc;
-    if (hc.stat[KC_STAT_ERRORS] || hc.low_overflow) return KC_EFULL;
+    if (hc.stat[KC_STAT_ERRORS]) return KC_EFULL;
     const uint64_t nb = (uint64_t)max_freq - min_freq + 1;
     unsigned long long *gb = nullptr;
     KC_HIP(hipMalloc((void **)&gb, nb * 8));
@@ -1201,7 +1218,7 @@ int kc_table_low_count(kc_table *t, uint64_t *n_low, bool *list_active)
     HostCounters hc;
     int rc = read_counters(t, hc);
     if (rc != KC_OK) return rc;
-    if (hc.stat[KC_STAT_ERRORS] || hc.low_overflow) return KC_EFULL;
+    if (hc.stat[KC_STAT_ERRORS]) return KC_EFULL;
     *list_active = low_list_active(t);
     *n_low = *list_active ? hc.low : 0;
     return KC_OK;

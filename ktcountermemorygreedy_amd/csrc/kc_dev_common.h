@@ -311,13 +311,15 @@ __device__ __forceinline__ uint32_t base_code(uint32_t byte, bool &valid)
 
 // Appends the FIRST occurrences of this wave step to a FIRST-list segment
 // (one cursor atomic per wave step).  Every lane of the wave must call it;
-// `first` selects the lanes that append.
+// `first` selects the lanes that append.  Returns false for a FIRST the
+// segment has no room for: the caller counts that occurrence in the table
+// instead (+1), which gives the same merged count (table count + 1 if the key
+// is on the FIRST list), so a full segment never loses a k-mer.
 template <int NW>
-__device__ __forceinline__ void first_append(const TableView &tv, bool first, const uint64_t *key, uint32_t &n_err,
-                                             uint32_t rot)
+__device__ __forceinline__ bool first_append(const TableView &tv, bool first, const uint64_t *key, uint32_t rot)
 {
     const uint64_t firsts = __ballot(first);
-    if (!firsts) return;
+    if (!firsts) return true;
     const int leader = __ffsll((unsigned long long)firsts) - 1;
     // consecutive blocks take consecutive segments (blocks that run only a
     // few waves still spread over all of them), the waves of one block are
@@ -328,16 +330,13 @@ __device__ __forceinline__ void first_append(const TableView &tv, bool first, co
     if (__lane_id() == leader)
         b0 = atomicAdd(tv.counters + KC_CURSOR_BASE + (uint64_t)seg * KC_LINE_WORDS, (unsigned long long)__popcll(firsts));
     b0 = __shfl(b0, leader, 64);
-    if (first) {
-        const uint64_t at = b0 + __popcll(firsts & lanemask_lt());
-        if (at < tv.seg_cap) {
-            uint64_t *dst = tv.low + ((uint64_t)seg * tv.seg_cap + at) * NW;
+    if (!first) return true;
+    const uint64_t at = b0 + __popcll(firsts & lanemask_lt());
+    if (at >= tv.seg_cap) return false;
+    uint64_t *dst = tv.low + ((uint64_t)seg * tv.seg_cap + at) * NW;
 #pragma unroll
-            for (int w = 0; w < NW; ++w) dst[w] = key[w];
-        } else {
-            ++n_err;
-        }
-    }
+    for (int w = 0; w < NW; ++w) dst[w] = key[w];
+    return true;
 }
 
 // Framed input of one counting call.  When hdr is set, the byte count and

@@ -553,3 +553,102 @@ extern "C" int kc_count_fastq_files_passes(kc_table *t, const char *const *paths
     if (stats) *stats = sum;
     return rc;
 }
+
+// ---- table sizing from a device sample -----------------------------------
+namespace {
+
+// Slots of one table partition of the LDS counting pass (Bloom split on).
+uint64_t kc_lds_slots(uint32_t nw) { return nw == 1 ? 4096 : 2048; }
+
+uint64_t pow2_at_most(uint64_t v)
+{
+    uint64_t p = 1;
+    while (p * 2 <= v) p <<= 1;
+    return p;
+}
+
+uint64_t pow2_at_least(uint64_t v)
+{
+    uint64_t p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+}  // namespace
+
+extern "C" int kc_plan_table(kc_ctx *ctx, const uint8_t *d_seq, uint64_t nbytes, uint32_t sample_slices,
+                             uint64_t hbm_budget, kc_table_config *cfg, kc_plan *plan)
+{
+    if (!ctx || !cfg || !plan || (nbytes && !d_seq)) return KC_EINVAL;
+    if (cfg->k < 1 || cfg->k > 63) return cfg->k > 63 ? KC_ENOTSUP : KC_EINVAL;
+    uint32_t S = sample_slices ? sample_slices : 16;
+    if (S > 256 || (S & (S - 1))) return KC_EINVAL;
+    // the probe holds every distinct key of its slice: at most nbytes / S
+    while (S < 256 && nbytes / S > (1ull << 26)) S <<= 1;
+    const uint32_t nw = cfg->k <= 32 ? 1 : 2;
+    kc_table_config pc{};
+    pc.k = cfg->k;
+    pc.count_max = 0xFFFFFFFFu;
+    pc.filter_max = 0xFFFFFFFFu;
+    pc.capacity = pow2_at_least(std::max<uint64_t>(nbytes / S + nbytes / S / 2, 1 << 16));
+    pc.count_path = KC_PATH_AUTO;
+    kc_table *probe = nullptr;
+    int rc = kc_table_create(ctx, &pc, &probe);
+    if (rc != KC_OK) return rc;
+    kc_stats st{};
+    int64_t singles = 0;
+    if ((rc = kc_table_set_pass(probe, 0, S)) == KC_OK && (rc = kc_table_count_seq(probe, d_seq, nbytes)) == KC_OK &&
+        (rc = kc_table_stats(probe, &st)) == KC_OK && (rc = kc_table_histogram(probe, 1, 1, &singles)) == KC_OK &&
+        st.errors)
+        rc = KC_EFULL;
+    kc_table_destroy(probe);
+    if (rc != KC_OK) return rc;
+    plan->sample_slices = S;
+    plan->kmers = st.kmers * S;
+    plan->distinct = st.distinct * S;
+    plan->repeated = (st.distinct - (uint64_t)singles) * S;
+    // one pass, then halve per pass until the device bytes fit the budget
+    const uint64_t slot_bytes = nw == 1 ? 16 : 32;
+    const uint64_t lds_bloom = nw == 1 ? 2048 : 1024;
+    const uint64_t max_bloom = lds_bloom << 18;  // the partition pass takes 2^18 Bloom parts
+    uint32_t P = 1;
+    for (;;) {
+        const uint64_t distinct = plan->distinct / P + 1024, repeated = plan->repeated / P + 1024;
+        uint64_t cap, bloom = 0, low = 0;
+        if (cfg->bloom) {
+            // 2-4 distinct keys per {BF1, BF2} block, up to what the
+            // partition pass takes (then more keys per block, at most 8)
+            bloom = std::min(max_bloom, std::max<uint64_t>(pow2_at_most(distinct / 2), 1 << 12));
+            // Bloom false positives send singletons to the table as well.  The
+            // block index takes the low bits of XXH3_128.low, which also fix
+            // the first of the three mask bits (BloomFilter.h:215-222,
+            // 287-297), so a key's mask in its block depends on 6 bits of the
+            // high word only: with n keys per block a new key repeats an
+            // existing mask with probability ~n/64, and finds two free bits
+            // set with ~(n/32)^2.
+            const double n = (double)distinct / (double)bloom;
+            const double fp = std::min(1.0, n / 64.0 + (n / 32.0) * (n / 32.0));
+            const uint64_t table_keys = repeated + (uint64_t)(fp * (double)(distinct - std::min(distinct, repeated)));
+            cap = pow2_at_least(std::max<uint64_t>(table_keys + table_keys * 3 / 5, 1 << 16));
+            // and at least the LDS-sized partitions the filter's parts need
+            // (the table is split into more, smaller partitions otherwise)
+            bloom = std::max(bloom, cap / kc_lds_slots(nw) * lds_bloom);
+            cap = std::max(cap, bloom * 256 / lds_bloom);  // partitions of >= 256 slots
+            low = distinct + distinct / 20 + (1 << 16);
+        } else {
+            cap = pow2_at_least(std::max<uint64_t>(distinct + distinct * 3 / 5, 1 << 16));
+        }
+        const uint64_t bytes = cap * slot_bytes + 16 * bloom + 8ull * nw * low;
+        const bool fits = (!hbm_budget || bytes <= hbm_budget) && (!cfg->bloom || distinct <= 8 * bloom);
+        if (fits || P == 256) {
+            cfg->capacity = cap;
+            cfg->bloom_blocks = bloom;
+            cfg->low_capacity = low;
+            cfg->max_distinct = distinct;
+            plan->device_bytes = bytes;
+            plan->passes = P;
+            return fits ? KC_OK : KC_ENOMEM;
+        }
+        P <<= 1;
+    }
+}

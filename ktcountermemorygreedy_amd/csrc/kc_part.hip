@@ -638,10 +638,11 @@ __global__ __launch_bounds__(kPartThreads) void k_count_parts(const uint64_t *__
                     for (int w = 0; w < NW; ++w) dst[w] = key[w];
                 }
                 fcur += (uint32_t)__popcll(fb);
+                to_table = valid && !first;
             } else {
-                first_append<NW>(tv, first, key, n_err, (uint32_t)((base - beg) >> 10));
+                const bool kept = first_append<NW>(tv, first, key, (uint32_t)((base - beg) >> 10));
+                to_table = valid && (!first || !kept);  // no room on the list: counted in the table
             }
-            to_table = valid && !first;
         }
         if (to_table) lds_insert<NW, BLOOM>(S, key, hi, psz - 1, n_new, n_err);
     }
@@ -655,13 +656,19 @@ __global__ __launch_bounds__(kPartThreads) void k_count_parts(const uint64_t *__
                 at = atomicAdd(tv.counters + KC_CURSOR_BASE + (uint64_t)seg * KC_LINE_WORDS, (unsigned long long)cnt);
             at = __shfl(at, 0, 64);
             for (uint32_t n = lane; n < cnt; n += 64) {
+                const uint64_t *src = first_out + wave_pos(beg, wave, p0 + n) * NW;
                 if (at + n < tv.seg_cap) {
-                    const uint64_t *src = first_out + wave_pos(beg, wave, p0 + n) * NW;
                     uint64_t *dst = tv.low + ((uint64_t)seg * tv.seg_cap + at + n) * NW;
 #pragma unroll
                     for (int w = 0; w < NW; ++w) dst[w] = src[w];
                 } else {
-                    ++n_err;
+                    // no room on the list: the occurrence is counted in the
+                    // partition's table (still in LDS) instead
+                    uint64_t key[NW], lo, hi;
+#pragma unroll
+                    for (int w = 0; w < NW; ++w) key[w] = src[w];
+                    xxh3_128<NW>(key, lo, hi);
+                    lds_insert<NW, BLOOM>(S, key, hi, psz - 1, n_new, n_err);
                 }
             }
         }

@@ -9,6 +9,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -77,6 +78,11 @@ class PassPlan(C.Structure):
                 ("hist_min", C.c_uint32), ("hist_max", C.c_uint32), ("hist_bins", C.c_void_p)]
 
 
+class Plan(C.Structure):
+    _fields_ = [("kmers", C.c_uint64), ("distinct", C.c_uint64), ("repeated", C.c_uint64),
+                ("device_bytes", C.c_uint64), ("passes", C.c_uint32), ("sample_slices", C.c_uint32)]
+
+
 class RequestResult(C.Structure):
     _fields_ = [("status", C.c_int), ("reserved", C.c_int), ("latency_ms", C.c_double),
                 ("queue_ms", C.c_double), ("kmers", C.c_uint64), ("distinct", C.c_uint64)]
@@ -118,6 +124,8 @@ EXPORTS = {
     "kc_table_dump_device": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_int, C.c_void_p, C.c_uint64,
                                        C.POINTER(C.c_uint64)]),
     "kc_table_dump_file": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_int, C.c_char_p]),
+    "kc_plan_table": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint64, C.POINTER(TableConfig),
+                                C.POINTER(Plan)]),
     "kc_table_set_tune": (C.c_int, [C.c_void_p, C.POINTER(Tune)]),
     "kc_table_get_tune": (C.c_int, [C.c_void_p, C.POINTER(Tune)]),
     "kc_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32, C.POINTER(Tune),
@@ -244,7 +252,8 @@ class DeviceBuffer:
 
     def free(self):
         if self._own and self.ptr:
-            lib().kc_dev_free(self.ctx.h, C.c_void_p(self.ptr))
+            if self.ctx.h:  # a closed context has released its device
+                lib().kc_dev_free(self.ctx.h, C.c_void_p(self.ptr))
             self.ptr = 0
 
     def __del__(self):
@@ -285,6 +294,7 @@ class Context:
         check(lib().kc_ctx_create(device, C.byref(h)), "kc_ctx_create")
         self.h = h
         self.device = device
+        self._tables = weakref.WeakSet()  # closed before the context: they use its stream
 
     def sync(self):
         check(lib().kc_ctx_sync(self.h), "kc_ctx_sync")
@@ -295,6 +305,8 @@ class Context:
 
     def close(self):
         if self.h:
+            for t in list(self._tables):
+                t.close()
             lib().kc_ctx_destroy(self.h)
             self.h = None
 
@@ -370,9 +382,13 @@ class Table:
         check(lib().kc_table_create(ctx.h, C.byref(cfg), C.byref(h)), "kc_table_create")
         self.h, self.ctx, self.k, self.nw, self.count_max = h, ctx, k, words_for_k(k), count_max
         self.bloom = bool(bloom)
+        self._batches = weakref.WeakSet()  # destroyed before the table: they count into it
+        ctx._tables.add(self)
 
     def close(self):
         if self.h:
+            for b in list(self._batches):
+                b.close()
             lib().kc_table_destroy(self.h)
             self.h = None
 
@@ -502,6 +518,7 @@ class Batch:
         else:
             check(lib().kc_batch_create(table.h, None, C.byref(h)), "kc_batch_create")
         self.h, self.table = h, table
+        table._batches.add(self)
 
     def add(self, chunk):
         buf = np.frombuffer(chunk, dtype=np.uint8) if isinstance(chunk, (bytes, bytearray)) else \
@@ -587,6 +604,21 @@ def count_fastq_files_passes(table: "Table", paths, n_passes, threads=4, export_
     check(lib().kc_count_fastq_files_passes(table.h, arr, len(paths), threads, None, C.byref(plan), C.byref(st)),
           "kc_count_fastq_files_passes")
     return bins, {f: getattr(st, f) for f, _ in IngestStats._fields_}
+
+
+def plan_table(ctx: "Context", d_seq, k, nbytes=None, bloom=False, count_max=0xFFFFFFFF, sample_slices=16,
+               hbm_budget=0):
+    """Table sizing from one exactly counted hash slice of the input on the
+    device (kc_plan_table).  Returns (table kwargs for Table(ctx, k, **kw),
+    plan dict with the scaled estimates and the pass count)."""
+    n = getattr(d_seq, "valid", d_seq.nbytes) if nbytes is None else nbytes
+    cfg = TableConfig(k=k, count_max=count_max, filter_min=0, filter_max=0xFFFFFFFF, bloom=1 if bloom else 0)
+    p = Plan()
+    check(lib().kc_plan_table(ctx.h, _ptr(d_seq), n, sample_slices, hbm_budget, C.byref(cfg), C.byref(p)),
+          "kc_plan_table")
+    kw = {"count_max": count_max, "capacity": cfg.capacity, "max_distinct": cfg.max_distinct, "bloom": bool(bloom),
+          "bloom_blocks": cfg.bloom_blocks, "low_capacity": cfg.low_capacity}
+    return kw, {f: getattr(p, f) for f, _ in Plan._fields_}
 
 
 def merge_sorted_dumps(paths, out_path):

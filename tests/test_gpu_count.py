@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
-from tests.kc_testlib import gpu_count, gpu_views, load_json, read_golden
+from tests.kc_testlib import gpu_count, gpu_views, load_json, read_golden, views_from_export
 
 pytestmark = pytest.mark.gpu
 
@@ -178,3 +178,39 @@ def test_partitioned_pieces_exact(ctx, bloom):
     assert st["errors"] == 0 and st["kmers"] == keys.shape[0]
     assert np.array_equal(t.query(uniq).astype(np.uint64), cnt)
     t.close()
+
+
+@pytest.mark.parametrize("path", ["direct", "partitioned"])
+@pytest.mark.parametrize("k", [31, 63])
+def test_first_list_spill_exact(ctx, k, path):
+    """A FIRST list far too small (about 1/10 of the FIRST occurrences): full
+    segments spill their FIRSTs into the table, the merged counts and the
+    exported views stay exact and no error is reported."""
+    from ktcountermemorygreedy_amd import kc
+    from ktcountermemorygreedy_amd import workloads as W
+    g = W.ReadSet("g", 3, 4_000_000, 13, 0).genome()
+    seq = W.UNEVEN_50MBP.scaled(6000).reads(genome=g).tobytes()
+    nw = O.words_for_k(k)
+    keys = O.extract(seq, k)
+    uniq, cnt = O.count(keys, nw)
+    # the list keeps at least 4 wave steps of slack per segment (~4.2M keys)
+    # whatever its nominal size: this input has about twice as many FIRSTs
+    t = kc.Table(ctx, k, capacity=1 << 24, bloom=True, max_distinct=uniq.shape[0],
+                 low_capacity=uniq.shape[0] // 10, path=path)
+    d = ctx.upload(np.frombuffer(seq, dtype=np.uint8))
+    t.count_seq(d, len(seq))
+    ctx.sync()
+    st = t.stats()
+    assert st["errors"] == 0 and st["kmers"] == keys.shape[0]
+    assert st["low"] < 0.75 * uniq.shape[0]  # a large part of the FIRSTs was spilled
+    bins = t.histogram(1, 1000)
+    want = np.bincount(np.minimum(cnt, 1000).astype(np.int64), minlength=1001)[1:]
+    assert np.array_equal(bins, want)
+    high, low = t.export_device()
+    dump, hist = views_from_export(high, low, k, 0xFFFFFFFF, 1000)
+    assert dump == O.dump_text(uniq, cnt, k)
+    assert hist == O.histogram_text(want, 1, 1000)
+    got = t.query(uniq).astype(np.uint64)
+    assert np.array_equal(got, cnt)
+    t.close()
+    d.free()

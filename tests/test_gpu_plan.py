@@ -1,0 +1,100 @@
+"""Table sizing from a device sample (kc_plan_table): one hash slice of the
+input's k-mers counted exactly and scaled.  The estimates of distinct and
+repeated k-mers match the oracle's full count, a table built from the plan
+counts the input exactly, the HBM budget turns into passes; and a Bloom
+filter larger than the table's partitions carry splits the table into
+smaller partitions (error-rich input), counted exactly on both paths."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.kc_testlib import views_from_export
+
+pytestmark = pytest.mark.gpu
+
+
+def _input():
+    from ktcountermemorygreedy_amd import workloads as W
+    g = W.ReadSet("g", 3, 2_000_000, 13, 0).genome()
+    return W.UNEVEN_50MBP.scaled(2500).reads(genome=g).tobytes()
+
+
+@pytest.mark.parametrize("bloom", [False, True], ids=["plain", "bloom"])
+@pytest.mark.parametrize("k", [31, 63])
+def test_plan_estimates_and_table(ctx, k, bloom):
+    from ktcountermemorygreedy_amd import kc
+    seq = _input()
+    nw = O.words_for_k(k)
+    keys = O.extract(seq, k)
+    uniq, cnt = O.count(keys, nw)
+    distinct, repeated = uniq.shape[0], int((cnt >= 2).sum())
+    d = ctx.upload(np.frombuffer(seq, dtype=np.uint8))
+    kw, plan = kc.plan_table(ctx, d, k, len(seq), bloom=bloom, sample_slices=8)
+    print("plan", k, bloom, plan, kw, "truth", distinct, repeated, keys.shape[0], flush=True)
+    assert plan["sample_slices"] == 8 and plan["passes"] == 1
+    assert abs(plan["distinct"] - distinct) <= 0.05 * distinct
+    assert abs(plan["repeated"] - repeated) <= 0.10 * repeated + 2048
+    assert kw["capacity"] >= 1.5 * (repeated if bloom else distinct)
+    if bloom:
+        assert kw["bloom_blocks"] * 6 >= distinct and kw["low_capacity"] >= distinct
+    t = kc.Table(ctx, k, **kw)
+    t.count_seq(d, len(seq))
+    ctx.sync()
+    st = t.stats()
+    assert st["errors"] == 0 and st["kmers"] == keys.shape[0]
+    bins = t.histogram(1, 1000)
+    want = np.bincount(np.minimum(cnt, 1000).astype(np.int64), minlength=1001)[1:]
+    assert np.array_equal(bins, want)
+    high, low = t.export_device()
+    dump, _ = views_from_export(high, low, k, 0xFFFFFFFF, 1000)
+    assert dump == O.dump_text(uniq, cnt, k)
+    if bloom:
+        # the planned table holds the keys seen twice; point lookups fold
+        # every FIRST into it and need room for all distinct keys
+        with pytest.raises(kc.KCError):
+            t.query(uniq[:16])
+    else:
+        got = t.query(uniq).astype(np.uint64)
+        assert np.array_equal(got, cnt)
+    t.close()
+    d.free()
+
+
+def test_plan_budget_passes(ctx):
+    from ktcountermemorygreedy_amd import kc
+    seq = _input()
+    d = ctx.upload(np.frombuffer(seq, dtype=np.uint8))
+    kw1, p1 = kc.plan_table(ctx, d, 31, len(seq), bloom=True, sample_slices=4)
+    budget = p1["device_bytes"] // 3
+    kw, p = kc.plan_table(ctx, d, 31, len(seq), bloom=True, sample_slices=4, hbm_budget=budget)
+    assert p["passes"] >= 4 and p["device_bytes"] <= budget
+    assert kw["capacity"] <= kw1["capacity"] and kw["max_distinct"] < kw1["max_distinct"]
+    d.free()
+
+
+@pytest.mark.parametrize("path", ["direct", "partitioned"])
+@pytest.mark.parametrize("k", [31, 63])
+def test_bloom_splits_partitions(ctx, k, path):
+    """bloom_blocks 8x what the table's LDS-sized partitions carry: the table
+    is split into 8x more partitions of 1/8 the slots; exact counts."""
+    from ktcountermemorygreedy_amd import kc
+    seq = _input()
+    nw = O.words_for_k(k)
+    keys = O.extract(seq, k)
+    uniq, cnt = O.count(keys, nw)
+    cap = 1 << 23  # room for every distinct key: point lookups fold the FIRST list in
+    lds_slots, lds_bloom = (4096, 2048) if nw == 1 else (2048, 1024)
+    bloom_blocks = 8 * (cap // lds_slots) * lds_bloom
+    t = kc.Table(ctx, k, capacity=cap, bloom=True, bloom_blocks=bloom_blocks, max_distinct=len(seq), path=path)
+    d = ctx.upload(np.frombuffer(seq, dtype=np.uint8))
+    t.count_seq(d, len(seq))
+    ctx.sync()
+    st = t.stats()
+    assert st["errors"] == 0 and st["kmers"] == keys.shape[0]
+    bins = t.histogram(1, 1000)
+    want = np.bincount(np.minimum(cnt, 1000).astype(np.int64), minlength=1001)[1:]
+    assert np.array_equal(bins, want)
+    got = t.query(uniq).astype(np.uint64)
+    assert np.array_equal(got, cnt)
+    t.close()
+    d.free()
