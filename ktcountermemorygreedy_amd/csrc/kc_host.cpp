@@ -609,7 +609,7 @@ extern "C" int kc_plan_table(kc_ctx *ctx, const uint8_t *d_seq, uint64_t nbytes,
     plan->repeated = (st.distinct - (uint64_t)singles) * S;
     // one pass, then halve per pass until the device bytes fit the budget
     const uint64_t slot_bytes = nw == 1 ? 16 : 32;
-    const uint64_t lds_bloom = nw == 1 ? 2048 : 1024;
+    const uint64_t lds_bloom = 2048;  // Bloom blocks of one partition in the LDS pass
     const uint64_t max_bloom = lds_bloom << 18;  // the partition pass takes 2^18 Bloom parts
     uint32_t P = 1;
     for (;;) {

@@ -103,7 +103,7 @@ struct kc_table {
 
 // Slots / Bloom blocks of one partition that the LDS counting pass holds.
 constexpr uint32_t kc_lds_part_slots(uint32_t nw, bool bloom) { return (nw == 1 ? 4096u : 2048u) << (bloom ? 0 : 1); }
-constexpr uint32_t kc_lds_part_bloom(uint32_t nw) { return nw == 1 ? 2048u : 1024u; }
+constexpr uint32_t kc_lds_part_bloom(uint32_t) { return 2048u; }
 
 // Launch helpers implemented in kc_count.hip / kc_sort.hip / kc_part.hip
 int kc_launch_fill(kc_ctx *ctx, uint64_t *p, uint64_t n_words, uint64_t value);
