@@ -181,15 +181,149 @@ __global__ __launch_bounds__(kThreads) void k_scan_apply(const uint32_t *in, uin
     }
 }
 
-// Stable scatter.  Block b walks its tiles in order, carrying a running
-// global offset per digit.  Inside a tile, wave w ranks its own contiguous
-// share (records w*TILE/W .. (w+1)*TILE/W of W waves, 64 consecutive
-// records per step) with wave-private digit counters: same-digit peers from
-// one ballot per digit bit, rank = counter + peers before the lane, then the
-// group's first lane bumps the counter.  One cross-wave prefix per digit and
-// one scan over digits place every record; the tile is staged digit-sorted
-// in LDS and written out in runs.  Order inside a digit = input order, so
-// the sort is stable.
+// ---- tile ranking and scatter (shared by both scatter kernels) ----------
+// Inside a tile, wave w ranks its own contiguous share (records
+// w*TILE/W .. (w+1)*TILE/W of W waves, 64 consecutive records per step) with
+// wave-private digit counters: same-digit peers from one ballot per digit
+// bit, rank = counter + peers before the lane, then the group's first lane
+// bumps the counter.  One cross-wave prefix per digit and one scan over
+// digits place every record; the tile is staged digit-sorted in LDS and
+// written out in runs.  Order inside a digit = input order: the sort is
+// stable.
+template <int RW, int DB> struct TileLds {
+    using L = Layout<RW>;
+    static constexpr int kWaves = L::TB / 64;
+    static constexpr uint32_t kBins = 1u << DB;
+    uint32_t stage[L::TILE * RW];
+    uint16_t wrun[kWaves][kBins];  // per wave: digit counts, then the wave's start in the digit
+    uint32_t local_start[kBins];   // digit's start in the tile
+    uint32_t wsum[kWaves];
+    uint64_t gbase[kBins];         // digit's global start for this tile
+};
+
+template <int RW, int DB> struct TileRegs {
+    using L = Layout<RW>;
+    static constexpr uint32_t kPer = ((1u << DB) + L::TB - 1) / L::TB;  // digits owned by one thread
+    uint32_t r[L::ITEMS][RW];
+    uint32_t rank[L::ITEMS], dig[L::ITEMS];
+    uint32_t tot[kPer];  // tile counts of the thread's digits [t*kPer, (t+1)*kPer)
+};
+
+// Loads and ranks the tile's tn records from t0; leaves local_start and the
+// per-wave starts in S, the thread's digit totals in R.tot.  Ends with a
+// barrier.
+template <int RW, int DB>
+__device__ __forceinline__ void tile_rank(TileLds<RW, DB> &S, TileRegs<RW, DB> &R, const uint32_t *__restrict__ src,
+                                          uint64_t t0, uint32_t tn, uint32_t bit, uint32_t width)
+{
+    using L = Layout<RW>;
+    constexpr int TB = L::TB;
+    constexpr uint32_t kBins = 1u << DB;
+    constexpr uint32_t kPer = TileRegs<RW, DB>::kPer;
+    constexpr int kWaves = TB / 64;
+    constexpr uint32_t kPerWave = L::TILE / kWaves;  // records per wave per tile
+    static_assert(kPerWave <= 0xFFFF && L::TILE <= 0xFFFF, "16-bit wave counters");
+    const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63;
+    const uint64_t lane_lt = (1ULL << lane) - 1ULL;
+    for (uint32_t d = t; d < kBins; d += TB)
+        for (int w = 0; w < kWaves; ++w) S.wrun[w][d] = 0;
+#pragma unroll
+    for (int j = 0; j < L::ITEMS; ++j) {
+        const uint32_t li = wave * kPerWave + (uint32_t)j * 64 + lane;
+        const uint64_t i = t0 + (li < tn ? li : tn - 1);
+        load_rec<RW>(src, i, R.r[j]);
+    }
+    __syncthreads();  // counters cleared
+#pragma unroll
+    for (int j = 0; j < L::ITEMS; ++j) {
+        const uint32_t li = wave * kPerWave + (uint32_t)j * 64 + lane;
+        const bool valid = li < tn;
+        const uint32_t d = valid ? digit_of<L::NW>(R.r[j], bit, width) : 0;
+        R.dig[j] = d;
+        uint64_t peers = __ballot(valid);
+        for (uint32_t b = 0; b < width; ++b) {
+            const uint64_t m = __ballot((d >> b) & 1);
+            peers &= ((d >> b) & 1) ? m : ~m;
+        }
+        const uint32_t before = __popcll(peers & lane_lt);
+        R.rank[j] = S.wrun[wave][d] + before;  // read by every lane before the leader's update
+        __builtin_amdgcn_wave_barrier();
+        if (valid && before == 0) S.wrun[wave][d] = (uint16_t)(S.wrun[wave][d] + __popcll(peers));
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    // digits [t*kPer, (t+1)*kPer): start of each wave inside the digit,
+    // then the digit's start in the tile (block scan over the threads)
+    uint32_t mine = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < kPer; ++q) {
+        const uint32_t d = t * kPer + q;
+        uint32_t acc = 0;
+        if (d < kBins) {
+#pragma unroll
+            for (int w = 0; w < kWaves; ++w) {
+                const uint32_t c = S.wrun[w][d];
+                S.wrun[w][d] = (uint16_t)acc;
+                acc += c;
+            }
+        }
+        R.tot[q] = acc;
+        mine += acc;
+    }
+    uint32_t x = mine;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= (uint32_t)o) x += y;
+    }
+    if (lane == 63) S.wsum[wave] = x;
+    __syncthreads();
+    uint32_t run = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) run += w < (int)wave ? S.wsum[w] : 0;
+    run += x - mine;
+#pragma unroll
+    for (uint32_t q = 0; q < kPer; ++q) {
+        const uint32_t d = t * kPer + q;
+        if (d < kBins) S.local_start[d] = run;
+        run += R.tot[q];
+    }
+    __syncthreads();
+}
+
+// Stages the ranked tile digit-sorted in LDS and writes it to dst at
+// S.gbase[d] + (position - S.local_start[d]).  Ends with a barrier.
+template <int RW, int DB>
+__device__ __forceinline__ void tile_scatter(TileLds<RW, DB> &S, const TileRegs<RW, DB> &R,
+                                             uint32_t *__restrict__ dst, uint32_t tn, uint32_t bit, uint32_t width)
+{
+    using L = Layout<RW>;
+    constexpr int TB = L::TB;
+    constexpr uint32_t kPerWave = L::TILE / (TB / 64);
+    const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63;
+#pragma unroll
+    for (int j = 0; j < L::ITEMS; ++j) {
+        const uint32_t li = wave * kPerWave + (uint32_t)j * 64 + lane;
+        if (li < tn) {
+            const uint32_t p = S.local_start[R.dig[j]] + S.wrun[wave][R.dig[j]] + R.rank[j];
+#pragma unroll
+            for (int q = 0; q < RW; ++q) S.stage[p * RW + q] = R.r[j][q];
+        }
+    }
+    __syncthreads();
+    // consecutive positions of one digit land on consecutive global addresses
+    for (uint32_t p = t; p < tn; p += TB) {
+        const uint32_t *rec = S.stage + p * RW;
+        const uint32_t d = digit_of<Layout<RW>::NW>(rec, bit, width);
+        const uint64_t g = S.gbase[d] + (p - S.local_start[d]);
+#pragma unroll
+        for (int q = 0; q < RW; ++q) dst[g * RW + q] = rec[q];
+    }
+    __syncthreads();
+}
+
+// Scatter with per-block offsets from an upsweep: block b walks its
+// contiguous tiles in order, carrying a running global offset per digit.
 template <int RW, int DB>
 __global__ __launch_bounds__(Layout<RW>::TB) void k_downsweep(const uint32_t *__restrict__ src,
                                                               uint32_t *__restrict__ dst, uint64_t n, uint32_t bit,
@@ -197,118 +331,23 @@ __global__ __launch_bounds__(Layout<RW>::TB) void k_downsweep(const uint32_t *__
                                                               const uint64_t *__restrict__ offsets)
 {
     using L = Layout<RW>;
-    constexpr int TB = L::TB;
     constexpr uint32_t kBins = 1u << DB;
-    constexpr uint32_t kPer = (kBins + TB - 1) / TB;  // digits owned by one thread (contiguous)
-    constexpr int kWaves = TB / 64;
-    constexpr uint32_t kPerWave = L::TILE / kWaves;  // records per wave per tile
-    static_assert(kPerWave <= 0xFFFF && L::TILE <= 0xFFFF, "16-bit wave counters");
-    __shared__ uint32_t stage[L::TILE * RW];
-    __shared__ uint16_t wrun[kWaves][kBins];  // per wave: digit counts, then the wave's start in the digit
-    __shared__ uint32_t local_start[kBins];
-    __shared__ uint32_t wsum[kWaves];
-    __shared__ uint64_t gbase[kBins];
-    const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63;
-    const uint64_t lane_lt = (1ULL << lane) - 1ULL;
-    for (uint32_t d = t; d < kBins; d += TB) gbase[d] = offsets[(uint64_t)d * gridDim.x + blockIdx.x];
+    constexpr uint32_t kPer = TileRegs<RW, DB>::kPer;
+    __shared__ TileLds<RW, DB> S;
+    TileRegs<RW, DB> R;
+    const uint32_t t = threadIdx.x;
+    for (uint32_t d = t; d < kBins; d += L::TB) S.gbase[d] = offsets[(uint64_t)d * gridDim.x + blockIdx.x];
     uint64_t first, last;
     block_tiles(n_tiles, first, last);
     for (uint64_t tile = first; tile < last; ++tile) {
         const uint64_t t0 = tile * L::TILE;
         const uint32_t tn = (uint32_t)(n - t0 < (uint64_t)L::TILE ? n - t0 : L::TILE);
-        for (uint32_t d = t; d < kBins; d += TB)
-            for (int w = 0; w < kWaves; ++w) wrun[w][d] = 0;
-        uint32_t r[L::ITEMS][RW];
-#pragma unroll
-        for (int j = 0; j < L::ITEMS; ++j) {
-            const uint32_t li = wave * kPerWave + (uint32_t)j * 64 + lane;
-            const uint64_t i = t0 + (li < tn ? li : tn - 1);
-            load_rec<RW>(src, i, r[j]);
-        }
-        __syncthreads();  // counters cleared
-        uint32_t rank[L::ITEMS], dig[L::ITEMS];
-#pragma unroll
-        for (int j = 0; j < L::ITEMS; ++j) {
-            const uint32_t li = wave * kPerWave + (uint32_t)j * 64 + lane;
-            const bool valid = li < tn;
-            const uint32_t d = valid ? digit_of<L::NW>(r[j], bit, width) : 0;
-            dig[j] = d;
-            uint64_t peers = __ballot(valid);
-            for (uint32_t b = 0; b < width; ++b) {
-                const uint64_t m = __ballot((d >> b) & 1);
-                peers &= ((d >> b) & 1) ? m : ~m;
-            }
-            const uint32_t before = __popcll(peers & lane_lt);
-            rank[j] = wrun[wave][d] + before;  // read by every lane before the leader's update
-            __builtin_amdgcn_wave_barrier();
-            if (valid && before == 0) wrun[wave][d] = (uint16_t)(wrun[wave][d] + __popcll(peers));
-            __builtin_amdgcn_wave_barrier();
-        }
-        __syncthreads();
-        // digits [t*kPer, (t+1)*kPer): start of each wave inside the digit,
-        // then the digit's start in the tile (block scan over the threads)
-        uint32_t tot[kPer];
-        uint32_t mine = 0;
+        tile_rank<RW, DB>(S, R, src, t0, tn, bit, width);
+        tile_scatter<RW, DB>(S, R, dst, tn, bit, width);
 #pragma unroll
         for (uint32_t q = 0; q < kPer; ++q) {
             const uint32_t d = t * kPer + q;
-            uint32_t acc = 0;
-            if (d < kBins) {
-#pragma unroll
-                for (int w = 0; w < kWaves; ++w) {
-                    const uint32_t c = wrun[w][d];
-                    wrun[w][d] = (uint16_t)acc;
-                    acc += c;
-                }
-            }
-            tot[q] = acc;
-            mine += acc;
-        }
-        {
-            uint32_t x = mine;
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                const uint32_t y = __shfl_up(x, o, 64);
-                if (lane >= (uint32_t)o) x += y;
-            }
-            if (lane == 63) wsum[wave] = x;
-            __syncthreads();
-            uint32_t run = 0;
-#pragma unroll
-            for (int w = 0; w < kWaves; ++w) run += w < (int)wave ? wsum[w] : 0;
-            run += x - mine;
-#pragma unroll
-            for (uint32_t q = 0; q < kPer; ++q) {
-                const uint32_t d = t * kPer + q;
-                if (d < kBins) local_start[d] = run;
-                run += tot[q];
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < L::ITEMS; ++j) {
-            const uint32_t li = wave * kPerWave + (uint32_t)j * 64 + lane;
-            if (li < tn) {
-                const uint32_t p = local_start[dig[j]] + wrun[wave][dig[j]] + rank[j];
-#pragma unroll
-                for (int q = 0; q < RW; ++q) stage[p * RW + q] = r[j][q];
-            }
-        }
-        __syncthreads();
-        // write out in local order: consecutive positions of one digit land
-        // on consecutive global addresses
-        for (uint32_t p = t; p < tn; p += TB) {
-            const uint32_t *rec = stage + p * RW;
-            const uint32_t d = digit_of<L::NW>(rec, bit, width);
-            const uint64_t g = gbase[d] + (p - local_start[d]);
-#pragma unroll
-            for (int q = 0; q < RW; ++q) dst[g * RW + q] = rec[q];
-        }
-        __syncthreads();
-#pragma unroll
-        for (uint32_t q = 0; q < kPer; ++q) {
-            const uint32_t d = t * kPer + q;
-            if (d < kBins) gbase[d] += tot[q];
+            if (d < kBins) S.gbase[d] += R.tot[q];
         }
     }
 }
@@ -382,59 +421,67 @@ struct SortScratch {
     uint32_t *counts = nullptr;
     uint64_t *offsets = nullptr;
     uint64_t *sums = nullptr;
+    void release()
+    {
+        for (void *q : {(void *)counts, (void *)offsets, (void *)sums})
+            if (q) (void)hipFree(q);
+        *this = SortScratch();
+    }
 };
 
-// One LSD pass per digit of at most DB bits over [lo, hi), digits balanced
-// (42 bits with DB = 9: 9, 9, 8, 8, 8).
+// One pass: per-block digit counts (upsweep), their scan, the scatter.
 template <int RW, int DB>
-int sort_passes(kc_ctx *ctx, uint32_t *a, uint32_t *b, uint64_t n, uint32_t lo, uint32_t hi, int *in_scratch)
+int sort_pass(kc_ctx *ctx, SortScratch &s, const uint32_t *src, uint32_t *dst, uint64_t n, uint32_t bit,
+              uint32_t width)
 {
     using L = Layout<RW>;
     constexpr uint32_t kBins = 1u << DB;
-    *in_scratch = 0;
-    if (n <= 1 || hi <= lo) return KC_OK;
     const uint64_t n_tiles = (n + L::TILE - 1) / L::TILE;
     // blocks own contiguous tile ranges: counts are [digit][block]
     const uint32_t grid = (uint32_t)std::min<uint64_t>(n_tiles, (uint64_t)ctx->cu_count * 8);
     const uint64_t m = (uint64_t)kBins * grid;
     const uint64_t nb = (m + kScanChunk - 1) / kScanChunk;
-    SortScratch s;
-    if (hipMalloc((void **)&s.counts, m * 4) != hipSuccess || hipMalloc((void **)&s.offsets, m * 8) != hipSuccess ||
-        hipMalloc((void **)&s.sums, nb * 8) != hipSuccess) {
-        (void)hipFree(s.counts);
-        (void)hipFree(s.offsets);
-        (void)hipFree(s.sums);
+    if (!s.counts && (hipMalloc((void **)&s.counts, m * 4) != hipSuccess ||
+                      hipMalloc((void **)&s.offsets, m * 8) != hipSuccess ||
+                      hipMalloc((void **)&s.sums, nb * 8) != hipSuccess))
         return KC_ENOMEM;
-    }
+    hipLaunchKernelGGL((k_upsweep<RW, DB>), dim3(grid), dim3(kThreads), 0, ctx->stream, src, n, bit, width, n_tiles, s.counts);
+    hipLaunchKernelGGL(k_scan_reduce, dim3((uint32_t)nb), dim3(kThreads), 0, ctx->stream, s.counts, m, s.sums);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kThreads), 0, ctx->stream, s.sums, nb);
+    hipLaunchKernelGGL(k_scan_apply, dim3((uint32_t)nb), dim3(kThreads), 0, ctx->stream, s.counts, m, s.sums, s.offsets);
+    hipLaunchKernelGGL((k_downsweep<RW, DB>), dim3(grid), dim3(L::TB), 0, ctx->stream, src, dst, n, bit, width, n_tiles, s.offsets);
+    return hipGetLastError() == hipSuccess ? KC_OK : KC_EDEVICE;
+}
+
+// One LSD pass per digit of at most DB bits over [lo, hi), digits balanced
+// (42 bits with DB = 9: 9, 9, 8, 8, 8).  A one-sweep form (decoupled
+// look-back instead of the upsweep, one block per tile) measured slower on
+// MI355X: 10.3-10.9 ms per pass against 9.6 for upsweep + scatter.
+template <int RW, int DB>
+int sort_passes(kc_ctx *ctx, uint32_t *a, uint32_t *b, uint64_t n, uint32_t lo, uint32_t hi, int *in_scratch)
+{
+    *in_scratch = 0;
+    if (n <= 1 || hi <= lo) return KC_OK;
     const uint32_t range = hi - lo;
     const uint32_t npass = (range + DB - 1) / DB;
+    SortScratch s;
     uint32_t *src = a, *dst = b;
-    int passes = 0;
-    hipError_t e = hipSuccess;
-    uint32_t bit = lo;
-    for (uint32_t p = 0; p < npass; ++p) {
+    int rc = KC_OK;
+    for (uint32_t p = 0, bit = lo; p < npass && rc == KC_OK; ++p) {
         // the first (range % npass) passes take one more bit
         const uint32_t width = range / npass + (p < range % npass ? 1 : 0);
-        hipLaunchKernelGGL((k_upsweep<RW, DB>), dim3(grid), dim3(kThreads), 0, ctx->stream, src, n, bit, width, n_tiles, s.counts);
-        hipLaunchKernelGGL(k_scan_reduce, dim3((uint32_t)nb), dim3(kThreads), 0, ctx->stream, s.counts, m, s.sums);
-        hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kThreads), 0, ctx->stream, s.sums, nb);
-        hipLaunchKernelGGL(k_scan_apply, dim3((uint32_t)nb), dim3(kThreads), 0, ctx->stream, s.counts, m, s.sums, s.offsets);
-        hipLaunchKernelGGL((k_downsweep<RW, DB>), dim3(grid), dim3(L::TB), 0, ctx->stream, src, dst, n, bit, width, n_tiles, s.offsets);
-        if ((e = hipGetLastError()) != hipSuccess) break;
+        rc = sort_pass<RW, DB>(ctx, s, src, dst, n, bit, width);
         std::swap(src, dst);
-        ++passes;
         bit += width;
     }
-    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    (void)hipFree(s.counts);
-    (void)hipFree(s.offsets);
-    (void)hipFree(s.sums);
-    if (e != hipSuccess) {
+    const hipError_t e = hipStreamSynchronize(ctx->stream);
+    s.release();
+    if (rc == KC_OK && e != hipSuccess) {
         kc_set_error("radix sort", e);
-        return KC_EDEVICE;
+        rc = KC_EDEVICE;
     }
-    *in_scratch = passes & 1;
-    return KC_OK;
+    if (rc == KC_OK) *in_scratch = npass & 1;
+    return rc;
 }
 
 template <int RW>
