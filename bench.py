@@ -37,7 +37,8 @@ def parse_args():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="repeat_100mbp",
-                    choices=["repeat_100mbp", "count_5mbp", "uneven_50mbp", "export_sort", "serving", "ingest_gz"])
+                    choices=["repeat_100mbp", "count_5mbp", "uneven_50mbp", "export_sort", "serving", "ingest_gz",
+                             "tools_5mbp"])
     ap.add_argument("--k", type=int, default=31, help="k-mer length (uneven_50mbp is quoted at 21, 31 and 63)")
     ap.add_argument("--serve-seconds", type=float, default=5.0, help="serving: length of the schedule")
     ap.add_argument("--records-log2", type=int, default=30, help="export_sort: 2^N records")
@@ -282,6 +283,101 @@ def bench_ingest(args):
     ctx.close()
 
 
+def bench_tools(args):
+    """Device dump_tool / histogram_tool equivalents on count_5mbp (k=31,
+    Bloom split, count_max 2^32-1): one step = kc_table_histogram (bins
+    1..10000) + kc_table_dump_device (sorted "<bases>\t<count>\n" lines of
+    every k-mer, into HBM).  Both outputs are checked against the reference
+    goldens (sha256); the reference's dump_tool precise --sort and
+    histogram_tool precise (built from source, 16 threads) run beside it on
+    the same export."""
+    import ctypes as C
+    import hashlib
+    import shutil
+
+    import numpy as np
+
+    from ktcountermemorygreedy_amd import kc
+    from ktcountermemorygreedy_amd import workloads as W
+    from oracle import oracle as O
+
+    with open(os.path.join(ROOT, "tests", "golden", "count_5mbp.json")) as f:
+        golden = json.load(f)
+    ws = W.COUNT_5MBP
+    seq = ws.reads()
+    ctx = kc.Context(0)
+    d_seq = ctx.upload(seq)
+    kw, plan = kc.plan_table(ctx, d_seq, 31, seq.size, bloom=True)
+    kw["max_distinct"] = max(kw["max_distinct"], golden["distinct"])
+    table = kc.Table(ctx, 31, **dict(kw, capacity=max(kw["capacity"], 1 << 24)))
+    table.count_seq(d_seq, seq.size)
+    ctx.sync()
+    n = C.c_uint64()
+    kc.check(kc.lib().kc_table_dump_device(table.h, 1, 0xFFFFFFFF, 1, None, 0, C.byref(n)), "dump size")
+    out = ctx.alloc(n.value + 4096)
+
+    def step():
+        bins = table.histogram(1, 10000)
+        m = C.c_uint64()
+        kc.check(kc.lib().kc_table_dump_device(table.h, 1, 0xFFFFFFFF, 1, kc._ptr(out), out.nbytes, C.byref(m)),
+                 "kc_table_dump_device")
+        ctx.sync()
+        return bins, m.value
+
+    for _ in range(args.warmup):
+        step()
+    times = []
+    for _ in range(max(1, args.steps)):
+        t0 = time.perf_counter()
+        bins, nbytes = step()
+        times.append(time.perf_counter() - t0)
+    step_s = float(np.median(times))
+    dump = out.numpy(np.uint8, nbytes).tobytes()
+    hist = O.histogram_text(bins, 1, 10000)
+    lines = dump.count(b"\n")
+    checks = {"dump_sha256_golden": hashlib.sha256(dump).hexdigest() == golden["dump_sha256"],
+              "hist_sha256_golden": hashlib.sha256(hist).hexdigest() == golden["hist_sha256"],
+              "lines": lines}
+    cpu = None
+    d_ref = os.path.join(ROOT, "oracle", "_ref", "dump_tool_ref")
+    h_ref = os.path.join(ROOT, "oracle", "_ref", "histogram_tool_ref")
+    if not args.no_cpu_baseline and os.path.exists(d_ref) and os.path.exists(h_ref):
+        tmp = tempfile.mkdtemp(prefix="kctools_")
+        table.export_dir(tmp)
+        threads = "16"
+        t0 = time.time()
+        subprocess.run([d_ref, "precise", tmp + "/", threads, "16", tmp + "/dump.txt", "1", "4294967295", "--sort"],
+                       stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, check=True, timeout=900)
+        t1 = time.time()
+        subprocess.run([h_ref, "precise", tmp + "/", threads, "16", tmp + "/hist.txt", "1", "10000"],
+                       stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, check=True, timeout=900)
+        t2 = time.time()
+        with open(tmp + "/dump.txt", "rb") as f:
+            same_dump = f.read() == dump
+        with open(tmp + "/hist.txt", "rb") as f:
+            same_hist = f.read() == hist
+        shutil.rmtree(tmp, ignore_errors=True)
+        cpu = {"value": lines / (t2 - t0), "unit": "dump lines/s (dump + histogram)", "cores": 16,
+               "kind": "reference",
+               "sample": "dump_tool precise <export> 16 16 dump.txt 1 4294967295 --sort (%.2f s) + histogram_tool "
+                         "precise <export> 16 16 hist.txt 1 10000 (%.2f s) on the same export; outputs %s" %
+                         (t1 - t0, t2 - t1, "equal" if same_dump and same_hist else "DIFFERENT")}
+    line = {
+        "metric": "sorted dump lines/s (dump_tool precise --sort + histogram_tool precise, on the device)",
+        "value": lines / step_s, "unit": "lines/s", "n_gpus": 1, "steps": len(times), "warmup": args.warmup,
+        "ms_per_step": step_s * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u64", "data": "synthetic",
+        "config": {"workload": "tools_" + ws.name, "k": 31, "bloom": True, "distinct": lines,
+                   "dump_bytes": nbytes, "step": "kc_table_histogram(1..10000) + kc_table_dump_device(sorted)"},
+        "roofline": None, "cpu_baseline": cpu, "checks": checks,
+    }
+    print(json.dumps(line), flush=True)
+    out.free()
+    d_seq.free()
+    table.close()
+    ctx.close()
+
+
 def serving_cpu_baseline(n_keys, n_batches):
     """The reference's point lookup (FlatConcurrentHashMap find_prepared with
     its query pool's prefetch loop) on this host: oracle/_ref/query_bench_ref."""
@@ -352,6 +448,8 @@ def main():
         return bench_serving(args)
     if args.workload == "ingest_gz":
         return bench_ingest(args)
+    if args.workload == "tools_5mbp":
+        return bench_tools(args)
     import numpy as np
 
     from ktcountermemorygreedy_amd import kc
