@@ -78,6 +78,9 @@ typedef struct kc_tune {
     uint64_t piece_bytes;   /* largest input one partitioned pass counts (0 = 4 GB); larger
                                inputs go in pieces, bounding the pass's two key arrays to
                                2 x 8N x piece_bytes bytes */
+    uint32_t first_mode;    /* FIRST occurrences in the partition pass: 0 = by key width (staged
+                               for k <= 32, appended above), 1 = staged per wave and moved to the
+                               list at the end, 2 = appended per wave step */
 } kc_tune;
 
 const char *kc_strerror(int status);

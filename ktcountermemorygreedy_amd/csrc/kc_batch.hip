@@ -453,6 +453,28 @@ extern "C" int kc_autotune(kc_table *t, const uint8_t *d_sample, uint64_t nbytes
             win = t->tune;
         }
     }
+    // the other FIRST hand-over of the partition pass on the winning shape
+    if (rc == KC_OK && best_t > 0 && win.path == KC_PATH_PARTITIONED && t->bloom) {
+        const bool staged = kc_part_shape(t, win, 0).stage_first;
+        t->tune = win;
+        t->tune.first_mode = staged ? 2 : 1;
+        float tmin = 0;
+        for (uint32_t r = 0; r <= reps && rc == KC_OK; ++r) {
+            if ((rc = kc_table_clear(t)) != KC_OK) break;
+            if (hipEventRecord(e0, t->ctx->stream) != hipSuccess) rc = KC_EDEVICE;
+            if (rc == KC_OK) rc = kc_table_count_seq(t, d_sample, nbytes);
+            if (rc == KC_OK && hipEventRecord(e1, t->ctx->stream) != hipSuccess) rc = KC_EDEVICE;
+            float ms = 0;
+            if (rc == KC_OK && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+                rc = KC_EDEVICE;
+            if (rc == KC_OK && r > 0 && (tmin == 0 || ms < tmin)) tmin = ms;
+        }
+        kc_stats st{};
+        if (rc == KC_OK && (rc = kc_table_stats(t, &st)) == KC_OK && !st.errors && tmin > 0 && tmin < best_t) {
+            best_t = tmin;
+            win = t->tune;
+        }
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     t->tune = rc == KC_OK ? win : saved;

@@ -59,14 +59,16 @@ __global__ __launch_bounds__(kBlock) void k_count_seq(SeqSrc src, TableView tv)
     uint32_t step = 0;
     for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x, ++step) {
         stage_tile(src.seq, n, tile, lds);
-        roll_segment<NW>(lds, tile, n, emit_from, tv.k, [&](const uint64_t *key) {
+        roll_segment<NW>(lds, tile, n, emit_from, tv.k, [&](const uint64_t *key, uint32_t e) {
             if (!in_pass<NW>(key, tv.pass_bits, tv.pass)) return;
             ++n_kmers;
             const KeyHash kh = key_hash<NW>(tv, key);
             bool to_table = true;
             if constexpr (BLOOM) {
                 const bool first = bloom_classify(tv, kh) == 0;
-                const bool kept = first_append<NW>(tv, first, key, step);
+                // each byte position of the tile appends to its own segment,
+                // so that even a grid of a few blocks fills every segment
+                const bool kept = first_append<NW>(tv, first, key, step * kParseSeg + e);
                 to_table = !first || !kept;  // a FIRST the list has no room for is counted here
             }
             if (to_table) table_add<NW>(tv, key, kh, 1, n_new, n_err);
@@ -788,14 +790,13 @@ extern "C" int kc_table_create(kc_ctx *ctx, const kc_table_config *cfg, kc_table
         t->bloom_blocks = std::max<uint64_t>(t->bloom_blocks, n_parts);
         t->bloom_part_log2 = ilog2(t->bloom_blocks) - t->part_bits;
         // Blocks, their waves and their loop steps rotate over the segments
-        // (first_append), so the segments fill evenly up to a few wave steps
-        // (a wave step adds at most 64 x 64 windows): 1/8 slack plus four
-        // such steps; a grid too small to reach every segment puts at most
-        // one tile of windows (kTile FIRSTs) into one segment.  A FIRST that
-        // still finds its segment full is counted in the table instead.
+        // (first_append), so the segments fill evenly up to a few appends: a
+        // wave's share of a tile (64 x 64 windows) goes to one segment, the
+        // partition pass moves at most 1024 FIRSTs at a time.  1/8 slack
+        // plus one wave's share; a FIRST that still finds its segment full is
+        // counted in the table instead.
         const uint64_t want = cfg->low_capacity ? cfg->low_capacity : hint;
-        t->seg_cap = std::max<uint64_t>((want + want / 8 + KC_LOW_SEGS - 1) / KC_LOW_SEGS + 4 * 4096,
-                                        std::min<uint64_t>(want, kTile));
+        t->seg_cap = (want + want / 8 + KC_LOW_SEGS - 1) / KC_LOW_SEGS + 4096;
         t->low_cap = t->seg_cap * KC_LOW_SEGS;
     }
     t->n_tiles = (t->cap + kExportTile - 1) / kExportTile;

@@ -309,6 +309,18 @@ __device__ __forceinline__ uint32_t base_code(uint32_t byte, bool &valid)
     return ((c >> 1) ^ (c >> 2)) & 3;
 }
 
+// FIRST-list segment of an append by (block, wave, step): a mix of the three,
+// so that every segment fills alike whatever the grid (a few blocks of one
+// tile each, or 2^18 partitions) and appends of one wave do not collide.
+__device__ __forceinline__ uint32_t first_segment(uint32_t block, uint32_t wave, uint32_t step)
+{
+    uint32_t h = block * 0x9E3779B1u ^ (wave + 1) * 0x85EBCA77u ^ step * 0xC2B2AE3Du;
+    h ^= h >> 16;
+    h *= 0x7FEB352Du;
+    h ^= h >> 15;
+    return h % KC_LOW_SEGS;
+}
+
 // Appends the FIRST occurrences of this wave step to a FIRST-list segment
 // (one cursor atomic per wave step).  Every lane of the wave must call it;
 // `first` selects the lanes that append.  Returns false for a FIRST the
@@ -321,11 +333,7 @@ __device__ __forceinline__ bool first_append(const TableView &tv, bool first, co
     const uint64_t firsts = __ballot(first);
     if (!firsts) return true;
     const int leader = __ffsll((unsigned long long)firsts) - 1;
-    // consecutive blocks take consecutive segments (blocks that run only a
-    // few waves still spread over all of them), the waves of one block are
-    // 16 segments apart, and `rot` (the block's loop step) moves a block on
-    // to the next segment, so that a small grid spreads over all of them too
-    const uint32_t seg = (blockIdx.x + 16u * (threadIdx.x >> 6) + rot) % KC_LOW_SEGS;
+    const uint32_t seg = first_segment(blockIdx.x, threadIdx.x >> 6, rot);
     unsigned long long b0 = 0;
     if (__lane_id() == leader)
         b0 = atomicAdd(tv.counters + KC_CURSOR_BASE + (uint64_t)seg * KC_LINE_WORDS, (unsigned long long)__popcll(firsts));
@@ -462,8 +470,9 @@ __device__ __forceinline__ void load_segment(const uint8_t *__restrict__ seq, ui
     }
 }
 
-// Rolls thread t's segment of a staged tile and calls emit(key) for every
-// window whose last byte lies in the segment, at or after emit_from
+// Rolls thread t's segment of a staged tile and calls emit(key, e) for every
+// window whose last byte lies in the segment, at or after emit_from, e being
+// that byte's index in the segment (the same for every lane of one call)
 // (GetKmer.h:58-82 semantics; any invalid byte resets the window).
 template <int NW, typename F>
 __device__ __forceinline__ void roll_segment(const ParseLds &L, uint64_t tile, uint64_t n, uint64_t emit_from,
@@ -496,7 +505,7 @@ __device__ __forceinline__ void roll_segment(const ParseLds &L, uint64_t tile, u
             if (have >= k && (uint32_t)(4 * j + b) >= skip) {
                 uint64_t key[NW];
                 roll.canon(key);
-                emit(key);
+                emit(key, (uint32_t)(4 * j + b));
             }
         }
     }

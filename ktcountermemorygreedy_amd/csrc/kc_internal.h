@@ -35,8 +35,8 @@ constexpr uint64_t KC_EMPTY = ~0ULL;
 // Device counters.  Statistics are spread over KC_LANES lanes, one 128-byte
 // line each, so that blocks update different lines (block b uses lane
 // b % KC_LANES); readers sum the lanes.  The FIRST-occurrence list is cut
-// into KC_LOW_SEGS segments with one cursor line each (wave w of block b
-// appends to segment (b + 16 w) % KC_LOW_SEGS), for the same reason.
+// into KC_LOW_SEGS segments with one cursor line each (an append goes to a
+// segment mixed from its block, wave and step), for the same reason.
 enum kc_stat_idx : int {
     KC_STAT_KMERS = 0,
     KC_STAT_DISTINCT = 1,
@@ -45,7 +45,7 @@ enum kc_stat_idx : int {
 };
 constexpr int KC_LANES = 64;
 constexpr int KC_LINE_WORDS = 16;
-constexpr int KC_LOW_SEGS = 256;
+constexpr int KC_LOW_SEGS = 1024;
 constexpr uint64_t KC_STAT_WORDS = (uint64_t)KC_STAT_N * KC_LANES * KC_LINE_WORDS;
 constexpr uint64_t KC_CURSOR_BASE = KC_STAT_WORDS;
 constexpr uint64_t KC_COUNTER_WORDS = KC_CURSOR_BASE + (uint64_t)KC_LOW_SEGS * KC_LINE_WORDS;
@@ -58,6 +58,7 @@ struct kc_pipe_shape {
     uint32_t front_blocks;  // persistent blocks of the parse / scatter front end
     uint32_t fine_slices;   // blocks per coarse region in the second scatter level
     uint64_t key_cap;
+    bool stage_first;       // partition pass: FIRSTs staged per wave (else appended per wave step)
 };
 
 // Device scratch of the partitioned counting pass, sized for a shape.

@@ -41,10 +41,12 @@ constexpr int kKeyBlock = 256;
 constexpr int kKeysPerThread = 16;
 constexpr uint64_t kKeyTile = (uint64_t)kKeyBlock * kKeysPerThread;  // keys per front-end tile
 constexpr int kPartThreads = 1024;
-// How the partition pass hands FIRST occurrences to the FIRST list: staged
-// per wave for one-word keys (fewer cursor atomics; measured faster on
-// repeat-rich input), appended per wave step for two-word keys (moves 16-byte
-// FIRSTs once; 57 vs 70 ms per count call on 1.9e9 63-mers, 86% FIRSTs).
+// How the partition pass hands FIRST occurrences to the FIRST list by
+// default (kc_tune.first_mode 0): staged per wave for one-word keys (fewer
+// cursor atomics; faster on repeat-rich input, 37.0 vs 38.3 ms per count
+// call), appended per wave step for two-word keys (moves 16-byte FIRSTs once;
+// 57 vs 70 ms on 1.9e9 63-mers, 86% FIRSTs).  FIRST-rich one-word input is
+// faster appended (36.4 vs 38.2 ms at 73% FIRSTs): the autotuner tries both.
 constexpr bool kStageFirst1 = true;
 constexpr bool kStageFirst2 = false;
 // Steps of keys the partition pass keeps in flight ahead of the one counted.
@@ -146,7 +148,7 @@ __global__ __launch_bounds__(kParseBlock) void k_front_count_seq(SeqSrc src, uin
     const uint64_t ntiles = (n + kParseTile - 1) / kParseTile;
     for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         stage_tile(src.seq, n, tile, lds);
-        roll_segment<NW>(lds, tile, n, emit_from, k, [&](const uint64_t *key) {
+        roll_segment<NW>(lds, tile, n, emit_from, k, [&](const uint64_t *key, uint32_t) {
             if (!PASS || in_pass_of<NW>(key, d)) atomicAdd(&hist[coarse_of<NW>(key, d)], 1u);
         });
         __syncthreads();  // the next tile overwrites lds
@@ -650,7 +652,7 @@ __global__ __launch_bounds__(kPartThreads) void k_count_parts(const uint64_t *__
         __threadfence_block();  // the wave's staged FIRSTs are read back by other lanes
         for (uint32_t p0 = 0; p0 < fcur; p0 += 1024) {
             const uint32_t cnt = fcur - p0 < 1024 ? fcur - p0 : 1024;
-            const uint32_t seg = (uint32_t)(part + 16u * wave + p0 / 1024) % KC_LOW_SEGS;
+            const uint32_t seg = first_segment((uint32_t)part, wave, p0 / 1024);
             unsigned long long at = 0;
             if (lane == 0)
                 at = atomicAdd(tv.counters + KC_CURSOR_BASE + (uint64_t)seg * KC_LINE_WORDS, (unsigned long long)cnt);
@@ -745,6 +747,7 @@ kc_pipe_shape kc_part_shape(const kc_table *t, const kc_tune &tune, uint64_t key
     sh.front_blocks = (g + 63) / 64 * 64;
     sh.fine_slices = std::max<uint32_t>(1, std::min<uint32_t>(tune.fine_slices ? tune.fine_slices : 8, 64));
     sh.key_cap = key_cap;
+    sh.stage_first = tune.first_mode == 1 || (tune.first_mode != 2 && (t->nw == 1 ? kStageFirst1 : kStageFirst2));
     return sh;
 }
 
@@ -859,10 +862,12 @@ int kc_part_enqueue(const kc_table *t, kc_part_scratch &s, const kc_pipe_shape &
     const dim3 g((uint32_t)n_parts);
     uint64_t *first_out = part_keys == s.keys_b ? s.keys_a : s.keys_b;  // the idle key array
     if (nw == 1) {
-        if (bloom) hipLaunchKernelGGL((k_count_parts<1, true, kStageFirst1>), g, dim3(kPartThreads), 0, st, part_keys, s.part_off, tv, first_out);
+        auto f = sh.stage_first ? &k_count_parts<1, true, true> : &k_count_parts<1, true, false>;
+        if (bloom) hipLaunchKernelGGL(f, g, dim3(kPartThreads), 0, st, part_keys, s.part_off, tv, first_out);
         else hipLaunchKernelGGL((k_count_parts<1, false, true>), g, dim3(kPartThreads), 0, st, part_keys, s.part_off, tv, first_out);
     } else {
-        if (bloom) hipLaunchKernelGGL((k_count_parts<2, true, kStageFirst2>), g, dim3(kPartThreads), 0, st, part_keys, s.part_off, tv, first_out);
+        auto f = sh.stage_first ? &k_count_parts<2, true, true> : &k_count_parts<2, true, false>;
+        if (bloom) hipLaunchKernelGGL(f, g, dim3(kPartThreads), 0, st, part_keys, s.part_off, tv, first_out);
         else hipLaunchKernelGGL((k_count_parts<2, false, true>), g, dim3(kPartThreads), 0, st, part_keys, s.part_off, tv, first_out);
     }
     KC_HIP(hipGetLastError());

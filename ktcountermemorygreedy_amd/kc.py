@@ -56,7 +56,8 @@ class ReadSpec(C.Structure):
 
 class Tune(C.Structure):
     _fields_ = [("front_blocks", C.c_uint32), ("fine_slices", C.c_uint32), ("batch_bytes", C.c_uint64),
-                ("queue_depth", C.c_uint32), ("path", C.c_uint32), ("piece_bytes", C.c_uint64)]
+                ("queue_depth", C.c_uint32), ("path", C.c_uint32), ("piece_bytes", C.c_uint64),
+                ("first_mode", C.c_uint32)]
 
 
 class BatchInfo(C.Structure):
@@ -513,7 +514,8 @@ class Batch:
             base = table.tune()
             t = Tune(front_blocks=front_blocks or base["front_blocks"], fine_slices=fine_slices or base["fine_slices"],
                      batch_bytes=batch_bytes or base["batch_bytes"], queue_depth=queue_depth or base["queue_depth"],
-                     path=base["path"] if path is None else paths[path], piece_bytes=base["piece_bytes"])
+                     path=base["path"] if path is None else paths[path], piece_bytes=base["piece_bytes"],
+                     first_mode=base["first_mode"])
             check(lib().kc_batch_create(table.h, C.byref(t), C.byref(h)), "kc_batch_create")
         else:
             check(lib().kc_batch_create(table.h, None, C.byref(h)), "kc_batch_create")
@@ -581,7 +583,7 @@ def count_fastq_files(table: "Table", paths, threads=4, batch_bytes=0, queue_dep
         base = table.tune()
         tune = Tune(front_blocks=base["front_blocks"], fine_slices=base["fine_slices"],
                     batch_bytes=batch_bytes or base["batch_bytes"], queue_depth=queue_depth or base["queue_depth"],
-                    path=base["path"], piece_bytes=base["piece_bytes"])
+                    path=base["path"], piece_bytes=base["piece_bytes"], first_mode=base["first_mode"])
     check(lib().kc_count_fastq_files(table.h, arr, len(paths), threads, C.byref(tune) if tune else None,
                                      C.byref(st)), "kc_count_fastq_files")
     return {f: getattr(st, f) for f, _ in IngestStats._fields_}

@@ -193,8 +193,8 @@ def test_first_list_spill_exact(ctx, k, path):
     nw = O.words_for_k(k)
     keys = O.extract(seq, k)
     uniq, cnt = O.count(keys, nw)
-    # the list keeps at least 4 wave steps of slack per segment (~4.2M keys)
-    # whatever its nominal size: this input has about twice as many FIRSTs
+    # the list keeps 4096 keys of slack per segment (~4.2M keys) whatever
+    # its nominal size: this input has about twice as many FIRSTs
     t = kc.Table(ctx, k, capacity=1 << 24, bloom=True, max_distinct=uniq.shape[0],
                  low_capacity=uniq.shape[0] // 10, path=path)
     d = ctx.upload(np.frombuffer(seq, dtype=np.uint8))
