@@ -789,12 +789,11 @@ extern "C" int kc_table_create(kc_ctx *ctx, const kc_table_config *cfg, kc_table
         t->bloom_blocks = cfg->bloom_blocks ? cfg->bloom_blocks : n_parts * kc_lds_part_bloom(t->nw);
         t->bloom_blocks = std::max<uint64_t>(t->bloom_blocks, n_parts);
         t->bloom_part_log2 = ilog2(t->bloom_blocks) - t->part_bits;
-        // Blocks, their waves and their loop steps rotate over the segments
-        // (first_append), so the segments fill evenly up to a few appends: a
-        // wave's share of a tile (64 x 64 windows) goes to one segment, the
-        // partition pass moves at most 1024 FIRSTs at a time.  1/8 slack
-        // plus one wave's share; a FIRST that still finds its segment full is
-        // counted in the table instead.
+        // Appends go to segments hashed from block, wave and step
+        // (first_segment), so the segments fill evenly up to a few appends:
+        // one append moves at most 64 FIRSTs (1024 in the partition pass).
+        // 1/8 slack plus 4096 keys; a FIRST that still finds its segment full
+        // is counted in the table instead.
         const uint64_t want = cfg->low_capacity ? cfg->low_capacity : hint;
         t->seg_cap = (want + want / 8 + KC_LOW_SEGS - 1) / KC_LOW_SEGS + 4096;
         t->low_cap = t->seg_cap * KC_LOW_SEGS;
