@@ -99,12 +99,14 @@ __global__ __launch_bounds__(kBlock) void k_increment(const uint64_t *__restrict
 
 // increment(key, 1) for every key of the FIRST list (folds it into the table)
 template <int NW>
-__global__ __launch_bounds__(kBlock) void k_increment_low(uint64_t n, TableView tv)
+__global__ __launch_bounds__(kBlock) void k_increment_low(TableView tv)
 {
     uint32_t n_new = 0, n_err = 0;
+    const uint32_t seg = blockIdx.y;
+    const uint64_t n = low_seg_fill(tv, seg);
     for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) {
         uint64_t key[NW];
-        const uint64_t *src = low_key(tv, i, NW);
+        const uint64_t *src = low_seg_key(tv, seg, i, NW);
 #pragma unroll
         for (int w = 0; w < NW; ++w) key[w] = src[w];
         table_add<NW>(tv, key, key_hash<NW>(tv, key), 1, n_new, n_err);
@@ -447,14 +449,15 @@ __global__ __launch_bounds__(kBlock) void k_hist_table(TableView tv, HistSpec h,
 }
 
 template <int NW>
-__global__ __launch_bounds__(kBlock) void k_hist_low_list(TableView tv, HistSpec h, uint64_t n,
-                                                          unsigned long long *gbins)
+__global__ __launch_bounds__(kBlock) void k_hist_low_list(TableView tv, HistSpec h, unsigned long long *gbins)
 {
     __shared__ int lds_bins[kLdsBins];
     for (uint32_t b = threadIdx.x; b < kLdsBins; b += kBlock) lds_bins[b] = 0;
     __syncthreads();
+    const uint32_t seg = blockIdx.y;
+    const uint64_t n = low_seg_fill(tv, seg);
     for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) {
-        const uint64_t *key = low_key(tv, i, NW);
+        const uint64_t *key = low_seg_key(tv, seg, i, NW);
         const uint64_t c = table_find<NW>(tv, key);
         const int64_t cc = high_view<NW>(tv, c, false);
         if (cc >= 0 && (uint64_t)cc + 1 >= h.min_freq) {
@@ -1006,9 +1009,11 @@ static int seal_table(kc_table *t)
     if (hc.stat[KC_STAT_ERRORS]) return KC_EFULL;
     if (hc.low) {
         const TableView v = view_of(t);
-        const uint32_t g = grid_for(hc.low, t->ctx->cu_count);
-        if (t->nw == 1) hipLaunchKernelGGL(k_increment_low<1>, dim3(g), dim3(kBlock), 0, t->ctx->stream, hc.low, v);
-        else hipLaunchKernelGGL(k_increment_low<2>, dim3(g), dim3(kBlock), 0, t->ctx->stream, hc.low, v);
+        uint64_t most = 0;
+        for (int sgi = 0; sgi < KC_LOW_SEGS; ++sgi) most = std::max(most, hc.seg_fill[sgi]);
+        const dim3 g(kc_low_chunks(most, 4 * kBlock), KC_LOW_SEGS);
+        if (t->nw == 1) hipLaunchKernelGGL(k_increment_low<1>, g, dim3(kBlock), 0, t->ctx->stream, v);
+        else hipLaunchKernelGGL(k_increment_low<2>, g, dim3(kBlock), 0, t->ctx->stream, v);
         KC_HIP(hipGetLastError());
         // the folded keys need table slots: a table sized for the keys seen
         // twice only (Bloom split) may not hold them all
@@ -1165,9 +1170,11 @@ extern "C" int kc_table_histogram(kc_table *t, uint32_t min_freq, uint32_t max_f
         if (t->nw == 1) hipLaunchKernelGGL(k_hist_table<1>, dim3(g), dim3(kBlock), 0, c->stream, v, hs, gb);
         else hipLaunchKernelGGL(k_hist_table<2>, dim3(g), dim3(kBlock), 0, c->stream, v, hs, gb);
         if (list && hc.low) {
-            const uint32_t g2 = grid_for(hc.low, c->cu_count);
-            if (t->nw == 1) hipLaunchKernelGGL(k_hist_low_list<1>, dim3(g2), dim3(kBlock), 0, c->stream, v, hs, hc.low, gb);
-            else hipLaunchKernelGGL(k_hist_low_list<2>, dim3(g2), dim3(kBlock), 0, c->stream, v, hs, hc.low, gb);
+            uint64_t most = 0;
+            for (int sgi = 0; sgi < KC_LOW_SEGS; ++sgi) most = std::max(most, hc.seg_fill[sgi]);
+            const dim3 g2(kc_low_chunks(most, 4 * kBlock), KC_LOW_SEGS);
+            if (t->nw == 1) hipLaunchKernelGGL(k_hist_low_list<1>, g2, dim3(kBlock), 0, c->stream, v, hs, gb);
+            else hipLaunchKernelGGL(k_hist_low_list<2>, g2, dim3(kBlock), 0, c->stream, v, hs, gb);
         }
         if (hipGetLastError() != hipSuccess ||
             hipMemcpyAsync(h_bins, gb, nb * 8, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||

@@ -141,15 +141,17 @@ __device__ __forceinline__ void stat_add(unsigned long long *counters, int stat,
                   (unsigned long long)s);
 }
 
-// i-th key of the FIRST list seen as the concatenation of its segments.
-__device__ __forceinline__ const uint64_t *low_key(const TableView &tv, uint64_t i, int nw)
+// FIRST-list walks run on a grid (chunks, KC_LOW_SEGS): block (x, s) takes
+// keys x*B + t, (x + gridDim.x)*B + t, ... of segment s, so every key is
+// read from its segment directly (no search over the segment offsets).
+__device__ __forceinline__ uint64_t low_seg_fill(const TableView &tv, uint32_t seg)
 {
-    int lo = 0, hi = KC_LOW_SEGS;  // find the last segment with seg_off <= i
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (tv.seg_off[mid] <= i) lo = mid; else hi = mid;
-    }
-    return tv.low + ((uint64_t)lo * tv.seg_cap + (i - tv.seg_off[lo])) * nw;
+    return tv.seg_off[seg + 1] - tv.seg_off[seg];
+}
+
+__device__ __forceinline__ const uint64_t *low_seg_key(const TableView &tv, uint32_t seg, uint64_t r, int nw)
+{
+    return tv.low + ((uint64_t)seg * tv.seg_cap + r) * nw;
 }
 
 // ---- table insert / add ------------------------------------------------

@@ -109,11 +109,14 @@ __device__ __forceinline__ uint64_t append(unsigned long long *cursor, bool want
 
 // low pass over the FIRST list
 template <int NW>
-__global__ __launch_bounds__(kThreads) void k_dump_low_list(TableView tv, DumpSpec ds, uint64_t n, uint32_t *recs,
+__global__ __launch_bounds__(kThreads) void k_dump_low_list(TableView tv, DumpSpec ds, uint32_t *recs,
                                                             unsigned long long *cursor)
 {
     constexpr int SW = slot_words<NW>();
     constexpr int RW = 2 * NW + 1;
+    const uint32_t seg = blockIdx.y;
+    const uint64_t n = low_seg_fill(tv, seg);
+    // the same iteration count for every lane of the block (append's ballot)
     const uint64_t iters = (n + (uint64_t)gridDim.x * kThreads - 1) / ((uint64_t)gridDim.x * kThreads);
     for (uint64_t it = 0; it < iters; ++it) {
         const uint64_t i = (it * gridDim.x + blockIdx.x) * (uint64_t)kThreads + threadIdx.x;
@@ -121,7 +124,7 @@ __global__ __launch_bounds__(kThreads) void k_dump_low_list(TableView tv, DumpSp
         uint32_t cnt = 0;
         uint64_t key[NW] = {};
         if (i < n) {
-            const uint64_t *lk = low_key(tv, i, NW);
+            const uint64_t *lk = low_seg_key(tv, seg, i, NW);
 #pragma unroll
             for (int w = 0; w < NW; ++w) key[w] = lk[w];
             uint64_t *slot = table_locate<NW>(tv, key);
@@ -277,8 +280,10 @@ static int dump_device(kc_table *t, uint32_t min_freq, uint32_t max_freq, int so
     const uint32_t g = (uint32_t)std::min<uint64_t>((std::max<uint64_t>(t->cap, n_low) + kThreads - 1) / kThreads,
                                                     (uint64_t)c->cu_count * 16);
     if (list && n_low) {
-        if (nw == 1) hipLaunchKernelGGL(k_dump_low_list<1>, dim3(g), dim3(kThreads), 0, st, tv, ds, n_low, r, cur);
-        else hipLaunchKernelGGL(k_dump_low_list<2>, dim3(g), dim3(kThreads), 0, st, tv, ds, n_low, r, cur);
+        // segments fill evenly (hashed appends): twice the mean covers the fullest
+        const dim3 gl(kc_low_chunks(2 * n_low / KC_LOW_SEGS + 1, 4 * kThreads), KC_LOW_SEGS);
+        if (nw == 1) hipLaunchKernelGGL(k_dump_low_list<1>, gl, dim3(kThreads), 0, st, tv, ds, r, cur);
+        else hipLaunchKernelGGL(k_dump_low_list<2>, gl, dim3(kThreads), 0, st, tv, ds, r, cur);
     }
     if (nw == 1) hipLaunchKernelGGL(k_dump_table<1>, dim3(g), dim3(kThreads), 0, st, tv, ds, r, cur);
     else hipLaunchKernelGGL(k_dump_table<2>, dim3(g), dim3(kThreads), 0, st, tv, ds, r, cur);

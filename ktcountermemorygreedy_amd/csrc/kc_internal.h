@@ -106,6 +106,14 @@ struct kc_table {
 constexpr uint32_t kc_lds_part_slots(uint32_t nw, bool bloom) { return (nw == 1 ? 4096u : 2048u) << (bloom ? 0 : 1); }
 constexpr uint32_t kc_lds_part_bloom(uint32_t) { return 2048u; }
 
+// Grid of a FIRST-list walk (see low_seg_key): enough chunks of `per_block`
+// keys for the fullest segment, at most 64.
+inline unsigned kc_low_chunks(uint64_t most, uint64_t per_block)
+{
+    const uint64_t c = (most + per_block - 1) / per_block;
+    return (unsigned)(c < 1 ? 1 : (c > 64 ? 64 : c));
+}
+
 // Launch helpers implemented in kc_count.hip / kc_sort.hip / kc_part.hip
 int kc_launch_fill(kc_ctx *ctx, uint64_t *p, uint64_t n_words, uint64_t value);
 uint64_t kc_scan_tmp_words(uint64_t n);
