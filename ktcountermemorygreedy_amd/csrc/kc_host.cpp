@@ -416,14 +416,23 @@ namespace {
 
 struct DumpReader {
     FILE *f = nullptr;
-    std::string line;  // current line without its '\n'; empty at the end
+    std::string line;  // current line without its '\n'
     std::vector<char> buf;
+    char *raw = nullptr;
+    size_t raw_cap = 0;
+    DumpReader() = default;
+    DumpReader(const DumpReader &) = delete;
+    DumpReader &operator=(const DumpReader &) = delete;
+    ~DumpReader() { std::free(raw); }
     bool next()
     {
-        line.clear();
-        int c;
-        while ((c = std::getc(f)) != EOF && c != '\n') line.push_back((char)c);
-        return !line.empty() || c == '\n';
+        const ssize_t r = getline(&raw, &raw_cap, f);
+        if (r <= 0) {
+            line.clear();
+            return false;
+        }
+        line.assign(raw, (size_t)r - (raw[r - 1] == '\n' ? 1 : 0));
+        return true;
     }
     std::string_view key() const
     {
@@ -606,7 +615,7 @@ extern "C" int kc_plan_table(kc_ctx *ctx, const uint8_t *d_seq, uint64_t nbytes,
     plan->sample_slices = S;
     plan->kmers = st.kmers * S;
     plan->distinct = st.distinct * S;
-    plan->repeated = (st.distinct - (uint64_t)singles) * S;
+    plan->repeated = (st.distinct - std::min<uint64_t>(st.distinct, (uint64_t)singles)) * S;
     // one pass, then halve per pass until the device bytes fit the budget
     const uint64_t slot_bytes = nw == 1 ? 16 : 32;
     const uint64_t lds_bloom = 2048;  // Bloom blocks of one partition in the LDS pass
