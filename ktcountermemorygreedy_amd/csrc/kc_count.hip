@@ -30,7 +30,7 @@ extern "C" const char *kc_strerror(int s)
     case KC_OK: return "ok";
     case KC_EINVAL: return "invalid argument";
     case KC_ENOMEM: return "out of memory";
-    case KC_EFULL: return "table or FIRST list full";
+    case KC_EFULL: return "table full";
     case KC_EDEVICE: return g_last_error[0] ? g_last_error : "HIP error";
     case KC_EIO: return "I/O error";
     case KC_ENOTSUP: return "not supported";
