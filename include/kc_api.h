@@ -198,12 +198,17 @@ typedef struct kc_plan {
  * large inputs so the probe table stays below 2^26 keys) of the framed input
  * d_seq exactly in a scratch table, scales its distinct and repeated keys,
  * and fills cfg (k, count_max, filters, bloom, count_path as given) with
- * capacity (table keys at <= ~62% load), bloom_blocks (3-6 distinct keys per
+ * capacity (table keys at <= ~62% load), bloom_blocks (2-4 distinct keys per
  * block), low_capacity and max_distinct for ONE of plan->passes passes
  * (hbm_budget = 0: no limit, one pass).  Synchronous; the scratch table is
  * freed before returning. */
 int kc_plan_table(kc_ctx *ctx, const uint8_t *d_seq, uint64_t nbytes, uint32_t sample_slices, uint64_t hbm_budget,
                   kc_table_config *cfg, kc_plan *plan);
+/* The sizing rule of kc_plan_table from known totals (distinct k-mers and
+ * distinct k-mers seen at least twice), e.g. a caller's own estimate; host
+ * only.  Fills cfg's sizes and plan's device_bytes / passes (kmers and
+ * sample_slices are left 0).  KC_ENOMEM when even 256 passes do not fit. */
+int kc_plan_sizes(uint64_t distinct, uint64_t repeated, uint64_t hbm_budget, kc_table_config *cfg, kc_plan *plan);
 
 /* ---- launch shape and batched counting.  No reference counterpart: the
  *      reference feeds 32 KiB ring blocks with a k-1 base overlap to parser

@@ -585,37 +585,14 @@ uint64_t pow2_at_least(uint64_t v)
 
 }  // namespace
 
-extern "C" int kc_plan_table(kc_ctx *ctx, const uint8_t *d_seq, uint64_t nbytes, uint32_t sample_slices,
-                             uint64_t hbm_budget, kc_table_config *cfg, kc_plan *plan)
+extern "C" int kc_plan_sizes(uint64_t distinct_total, uint64_t repeated_total, uint64_t hbm_budget,
+                             kc_table_config *cfg, kc_plan *plan)
 {
-    if (!ctx || !cfg || !plan || (nbytes && !d_seq)) return KC_EINVAL;
+    if (!cfg || !plan) return KC_EINVAL;
     if (cfg->k < 1 || cfg->k > 63) return cfg->k > 63 ? KC_ENOTSUP : KC_EINVAL;
-    uint32_t S = sample_slices ? sample_slices : 16;
-    if (S > 256 || (S & (S - 1))) return KC_EINVAL;
-    // the probe holds every distinct key of its slice: at most nbytes / S
-    while (S < 256 && nbytes / S > (1ull << 26)) S <<= 1;
     const uint32_t nw = cfg->k <= 32 ? 1 : 2;
-    kc_table_config pc{};
-    pc.k = cfg->k;
-    pc.count_max = 0xFFFFFFFFu;
-    pc.filter_max = 0xFFFFFFFFu;
-    pc.capacity = pow2_at_least(std::max<uint64_t>(nbytes / S + nbytes / S / 2, 1 << 16));
-    pc.count_path = KC_PATH_AUTO;
-    kc_table *probe = nullptr;
-    int rc = kc_table_create(ctx, &pc, &probe);
-    if (rc != KC_OK) return rc;
-    kc_stats st{};
-    int64_t singles = 0;
-    if ((rc = kc_table_set_pass(probe, 0, S)) == KC_OK && (rc = kc_table_count_seq(probe, d_seq, nbytes)) == KC_OK &&
-        (rc = kc_table_stats(probe, &st)) == KC_OK && (rc = kc_table_histogram(probe, 1, 1, &singles)) == KC_OK &&
-        st.errors)
-        rc = KC_EFULL;
-    kc_table_destroy(probe);
-    if (rc != KC_OK) return rc;
-    plan->sample_slices = S;
-    plan->kmers = st.kmers * S;
-    plan->distinct = st.distinct * S;
-    plan->repeated = (st.distinct - std::min<uint64_t>(st.distinct, (uint64_t)singles)) * S;
+    plan->distinct = distinct_total;
+    plan->repeated = std::min(repeated_total, distinct_total);
     // one pass, then halve per pass until the device bytes fit the budget
     const uint64_t slot_bytes = nw == 1 ? 16 : 32;
     const uint64_t lds_bloom = 2048;  // Bloom blocks of one partition in the LDS pass
@@ -660,4 +637,37 @@ extern "C" int kc_plan_table(kc_ctx *ctx, const uint8_t *d_seq, uint64_t nbytes,
         }
         P <<= 1;
     }
+}
+
+extern "C" int kc_plan_table(kc_ctx *ctx, const uint8_t *d_seq, uint64_t nbytes, uint32_t sample_slices,
+                             uint64_t hbm_budget, kc_table_config *cfg, kc_plan *plan)
+{
+    if (!ctx || !cfg || !plan || (nbytes && !d_seq)) return KC_EINVAL;
+    if (cfg->k < 1 || cfg->k > 63) return cfg->k > 63 ? KC_ENOTSUP : KC_EINVAL;
+    uint32_t S = sample_slices ? sample_slices : 16;
+    if (S > 256 || (S & (S - 1))) return KC_EINVAL;
+    // the probe holds every distinct key of its slice: at most nbytes / S
+    while (S < 256 && nbytes / S > (1ull << 26)) S <<= 1;
+    kc_table_config pc{};
+    pc.k = cfg->k;
+    pc.count_max = 0xFFFFFFFFu;
+    pc.filter_max = 0xFFFFFFFFu;
+    pc.capacity = pow2_at_least(std::max<uint64_t>(nbytes / S + nbytes / S / 2, 1 << 16));
+    pc.count_path = KC_PATH_AUTO;
+    kc_table *probe = nullptr;
+    int rc = kc_table_create(ctx, &pc, &probe);
+    if (rc != KC_OK) return rc;
+    kc_stats st{};
+    int64_t singles = 0;
+    if ((rc = kc_table_set_pass(probe, 0, S)) == KC_OK && (rc = kc_table_count_seq(probe, d_seq, nbytes)) == KC_OK &&
+        (rc = kc_table_stats(probe, &st)) == KC_OK && (rc = kc_table_histogram(probe, 1, 1, &singles)) == KC_OK &&
+        st.errors)
+        rc = KC_EFULL;
+    kc_table_destroy(probe);
+    if (rc != KC_OK) return rc;
+    rc = kc_plan_sizes(st.distinct * S, (st.distinct - std::min<uint64_t>(st.distinct, (uint64_t)singles)) * S,
+                       hbm_budget, cfg, plan);
+    plan->sample_slices = S;
+    plan->kmers = st.kmers * S;
+    return rc;
 }

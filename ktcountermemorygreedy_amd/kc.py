@@ -127,6 +127,7 @@ EXPORTS = {
     "kc_table_dump_file": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_int, C.c_char_p]),
     "kc_plan_table": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint64, C.POINTER(TableConfig),
                                 C.POINTER(Plan)]),
+    "kc_plan_sizes": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint64, C.POINTER(TableConfig), C.POINTER(Plan)]),
     "kc_table_set_tune": (C.c_int, [C.c_void_p, C.POINTER(Tune)]),
     "kc_table_get_tune": (C.c_int, [C.c_void_p, C.POINTER(Tune)]),
     "kc_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32, C.POINTER(Tune),
@@ -618,6 +619,17 @@ def plan_table(ctx: "Context", d_seq, k, nbytes=None, bloom=False, count_max=0xF
     p = Plan()
     check(lib().kc_plan_table(ctx.h, _ptr(d_seq), n, sample_slices, hbm_budget, C.byref(cfg), C.byref(p)),
           "kc_plan_table")
+    kw = {"count_max": count_max, "capacity": cfg.capacity, "max_distinct": cfg.max_distinct, "bloom": bool(bloom),
+          "bloom_blocks": cfg.bloom_blocks, "low_capacity": cfg.low_capacity}
+    return kw, {f: getattr(p, f) for f, _ in Plan._fields_}
+
+
+def plan_sizes(k, distinct, repeated, bloom=False, count_max=0xFFFFFFFF, hbm_budget=0):
+    """kc_plan_table's sizing rule from known totals (host only): returns
+    (table kwargs for Table(ctx, k, **kw), plan dict)."""
+    cfg = TableConfig(k=k, count_max=count_max, filter_min=0, filter_max=0xFFFFFFFF, bloom=1 if bloom else 0)
+    p = Plan()
+    check(lib().kc_plan_sizes(distinct, repeated, hbm_budget, C.byref(cfg), C.byref(p)), "kc_plan_sizes")
     kw = {"count_max": count_max, "capacity": cfg.capacity, "max_distinct": cfg.max_distinct, "bloom": bool(bloom),
           "bloom_blocks": cfg.bloom_blocks, "low_capacity": cfg.low_capacity}
     return kw, {f: getattr(p, f) for f, _ in Plan._fields_}

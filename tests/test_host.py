@@ -74,3 +74,49 @@ def test_merge_sorted_dumps(tmp_path):
     assert out.read_text() == want
     with pytest.raises(kc.KCError):
         kc.merge_sorted_dumps([str(tmp_path / "missing.txt")], str(out))
+
+
+def test_plan_sizes_reproduces_device_plans():
+    """kc_plan_sizes is the sizing half of kc_plan_table: fed the scaled
+    totals the device probe reported on two MI355X workloads it returns the
+    same device bytes, and those sizes held what the full count then found
+    (table keys, FIRST occurrences).  Totals and outcomes are those runs'."""
+    from ktcountermemorygreedy_amd import kc
+    runs = [  # k, distinct, repeated, device_bytes, measured table keys, FIRST occurrences
+        (31, 391599168, 101114048, 9732416840, 116714803, 374634073),    # 20M x 150 bp over 100 Mbp
+        (63, 1836491392, 40757888, 48033990336, 134327905, 1741620449),  # 1.16M uneven reads over 50 Mbp
+    ]
+    for k, distinct, repeated, dev_bytes, table_keys, firsts in runs:
+        kw, plan = kc.plan_sizes(k, distinct, repeated, bloom=True)
+        assert plan["device_bytes"] == dev_bytes and plan["passes"] == 1
+        cap, bloom = kw["capacity"], kw["bloom_blocks"]
+        assert cap & (cap - 1) == 0 and bloom & (bloom - 1) == 0
+        assert table_keys <= 0.625 * cap
+        assert firsts <= kw["low_capacity"]
+        assert distinct <= 8 * bloom
+        slot = 16 if k <= 32 else 32
+        assert dev_bytes == cap * slot + 16 * bloom + 8 * (k > 32) * kw["low_capacity"] + 8 * kw["low_capacity"]
+
+
+def test_plan_sizes_passes_and_budget():
+    from ktcountermemorygreedy_amd import kc
+    # no Bloom: smallest power-of-two table at <= 62.5% load
+    kw, plan = kc.plan_sizes(31, 1 << 26, 1 << 26)
+    cap = kw["capacity"]
+    assert cap == 1 << 27 and plan["device_bytes"] == cap * 16 and plan["passes"] == 1
+    assert (1 << 26) <= 0.625 * cap and (1 << 26) > 0.3125 * cap
+    # a budget of half the bytes takes two passes, each of half the keys
+    kw2, plan2 = kc.plan_sizes(31, 1 << 26, 1 << 26, hbm_budget=plan["device_bytes"] // 2)
+    assert plan2["passes"] == 2 and kw2["capacity"] == cap // 2
+    assert plan2["device_bytes"] <= plan["device_bytes"] // 2
+    # passes never decrease as the budget shrinks, and every plan fits
+    last = 1
+    for budget in (1 << 36, 1 << 34, 1 << 32, 1 << 30, 1 << 28):
+        kw, p = kc.plan_sizes(63, 1 << 30, 1 << 26, bloom=True, hbm_budget=budget)
+        assert p["passes"] >= last and p["device_bytes"] <= budget
+        assert kw["max_distinct"] * p["passes"] >= 1 << 30
+        last = p["passes"]
+    with pytest.raises(kc.KCError):  # even 256 passes do not fit
+        kc.plan_sizes(31, 1 << 34, 1 << 30, bloom=True, hbm_budget=1 << 20)
+    with pytest.raises(kc.KCError):
+        kc.plan_sizes(64, 1000, 10)
