@@ -549,8 +549,9 @@ def main():
         launch_info["batch"] = batch.info()
 
     # host-batched line (PCIe-inclusive, never the headline): the reads as
-    # uneven chunks of 1..64 reads packed by kc_batch from host memory, with
-    # the staging queue (slot size x slots in flight) tuned on this input
+    # uneven chunks of 1..64 reads packed by kc_batch from host memory (one
+    # kc_batch_add_chunks call), with the staging queue (slot size x slots in
+    # flight) tuned on this input; the same with one kc_batch_add per chunk
     host_batched = None
     if args.workload == "uneven_50mbp" and not args.no_host_batch:
         bounds = AT.chunk_bounds(seq_host, np.random.default_rng(7))
@@ -558,12 +559,15 @@ def main():
         dt, info = AT.run_batches(table, seq_host, bounds, q["batch_bytes"], q["queue_depth"])
         st_h = table.stats()
         bins_h = table.histogram(1, 100000)
+        same = bool(st_h["kmers"] == K and st_h["errors"] == 0 and np.array_equal(bins_h, bins))
+        table.clear()
+        dt1, _ = AT.run_batches(table, seq_host, bounds, q["batch_bytes"], q["queue_depth"], one_call=False)
+        same = same and bool(np.array_equal(table.histogram(1, 100000), bins))
+        table.clear()
         host_batched = {"kmers_per_s": st_h["kmers"] / dt, "seconds": dt, "chunks": int(bounds.size),
                         "batch_bytes": q["batch_bytes"], "queue_depth": q["queue_depth"],
                         "launches": info["launches"], "queue_trials": q["trials"],
-                        "same_as_device": bool(st_h["kmers"] == K and st_h["errors"] == 0 and
-                                               np.array_equal(bins_h, bins))}
-        table.clear()
+                        "per_chunk_add_seconds": dt1, "same_as_device": same}
 
     multipass = None
     if args.passes > 1:

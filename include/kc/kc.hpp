@@ -213,6 +213,11 @@ public:
     {
         check(kc_batch_add(b_, reinterpret_cast<const uint8_t *>(chunk), bytes), "kc_batch_add");
     }
+    // chunk i = base[ends[i-1], ends[i]) (ends[-1] = 0)
+    void add_chunks(const char *base, const uint64_t *ends, uint64_t n)
+    {
+        check(kc_batch_add_chunks(b_, reinterpret_cast<const uint8_t *>(base), ends, n), "kc_batch_add_chunks");
+    }
     void flush() { check(kc_batch_flush(b_), "kc_batch_flush"); }
     void sync() { check(kc_batch_sync(b_), "kc_batch_sync"); }
     kc_batch_info info() const

@@ -255,6 +255,11 @@ int kc_batch_create(kc_table *t, const kc_tune *tune, kc_batch **out);
 void kc_batch_destroy(kc_batch *b);
 /* Copies the chunk into the current slot; returns when h_chunk may be reused. */
 int kc_batch_add(kc_batch *b, const uint8_t *h_chunk, uint64_t len);
+/* n_chunks consecutive chunks of one host buffer, chunk i being
+ * h_base[ends[i-1], ends[i]) (ends[-1] = 0, ends non-decreasing): the same
+ * batches as kc_batch_add per chunk, packed in multithreaded span copies
+ * with one call.  Returns when h_base may be reused. */
+int kc_batch_add_chunks(kc_batch *b, const uint8_t *h_base, const uint64_t *ends, uint64_t n_chunks);
 int kc_batch_flush(kc_batch *b);
 int kc_batch_sync(kc_batch *b);
 /* One graph launch over a packed batch already in HBM (16-byte aligned);

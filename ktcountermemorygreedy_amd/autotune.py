@@ -89,15 +89,20 @@ def chunk_bounds(seq: np.ndarray, rng, max_reads=64):
     return np.asarray(cuts, dtype=np.int64)
 
 
-def run_batches(table: kc.Table, seq: np.ndarray, bounds, batch_bytes, queue_depth):
-    """Counts every chunk through one kc_batch; returns (seconds, info)."""
+def run_batches(table: kc.Table, seq: np.ndarray, bounds, batch_bytes, queue_depth, one_call=True):
+    """Counts every chunk through one kc_batch, all in one
+    kc_batch_add_chunks call (or one kc_batch_add per chunk); returns
+    (seconds, info)."""
     b = kc.Batch(table, batch_bytes=batch_bytes, queue_depth=queue_depth)
     try:
         t0 = time.perf_counter()
-        start = 0
-        for end in bounds:
-            b.add(seq[start:end])
-            start = int(end)
+        if one_call:
+            b.add_chunks(seq, bounds)
+        else:
+            start = 0
+            for end in bounds:
+                b.add(seq[start:end])
+                start = int(end)
         b.flush()
         b.sync()
         dt = time.perf_counter() - t0

@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "kc_dev_common.h"
@@ -210,6 +211,34 @@ int slot_ready(kc_batch *b)
     return KC_OK;
 }
 
+// Host copy into a staging slot, split over up to 8 threads for large spans
+// (one thread does not reach the host's memory bandwidth).
+void copy_par(uint8_t *dst, const uint8_t *src, uint64_t n)
+{
+    constexpr uint64_t kPerThread = 8ull << 20;
+    const uint64_t hw = std::max(1u, std::thread::hardware_concurrency());
+    const uint64_t T = std::min<uint64_t>(std::min<uint64_t>(8, hw), n / kPerThread);
+    if (T <= 1) {
+        std::memcpy(dst, src, n);
+        return;
+    }
+    const uint64_t per = (n / T + 4095) & ~4095ull;
+    std::vector<std::thread> th;
+    uint64_t done = per;  // [0, per) is copied by this thread
+    for (uint64_t lo = per; lo < n; lo += per) {
+        const uint64_t len = std::min(per, n - lo);
+        try {
+            th.emplace_back([=] { std::memcpy(dst + lo, src + lo, len); });
+            done = lo + len;
+        } catch (...) {
+            break;  // no thread: this one copies the rest
+        }
+    }
+    std::memcpy(dst, src, std::min(per, n));
+    if (done < n) std::memcpy(dst + done, src + done, n - done);
+    for (std::thread &x : th) x.join();
+}
+
 void resolve_tune(const kc_table *t, const kc_tune *in, kc_tune &out, uint32_t &path)
 {
     out = in ? *in : t->tune;
@@ -308,7 +337,7 @@ extern "C" int kc_batch_add(kc_batch *b, const uint8_t *h_chunk, uint64_t len)
         if (b->slot[b->cur].fill + need > b->cap && (rc = launch_slot(b, b->cur, 0)) != KC_OK) return rc;
         if ((rc = slot_ready(b)) != KC_OK) return rc;
         Slot &s = b->slot[b->cur];
-        std::memcpy(s.pinned + s.fill, h_chunk, len);
+        copy_par(s.pinned + s.fill, h_chunk, len);
         if (!nl) s.pinned[s.fill + len] = '\n';
         s.fill += need;
         return KC_OK;
@@ -321,11 +350,66 @@ extern "C" int kc_batch_add(kc_batch *b, const uint8_t *h_chunk, uint64_t len)
         const uint64_t take = std::min(b->cap - halo, len - pos);
         if ((rc = slot_ready(b)) != KC_OK) return rc;
         Slot &s = b->slot[b->cur];
-        std::memcpy(s.pinned, h_chunk + pos - halo, halo + take);
+        copy_par(s.pinned, h_chunk + pos - halo, halo + take);
         s.fill = halo + take;
         ++b->info.pieces;
         if ((rc = launch_slot(b, b->cur, halo)) != KC_OK) return rc;
         pos += take;
+    }
+    return KC_OK;
+}
+
+extern "C" int kc_batch_add_chunks(kc_batch *b, const uint8_t *h_base, const uint64_t *ends, uint64_t n_chunks)
+{
+    if (!b || (n_chunks && !ends)) return KC_EINVAL;
+    uint64_t prev = 0;
+    for (uint64_t i = 0; i < n_chunks; ++i) {
+        if (ends[i] < prev) return KC_EINVAL;
+        prev = ends[i];
+    }
+    if (prev && !h_base) return KC_EINVAL;
+    if (!prev) return KC_OK;
+    int rc = check_table(b);
+    if (rc != KC_OK) return rc;
+    KC_HIP(hipSetDevice(b->t->ctx->device));
+    uint64_t i = 0, pos = 0;  // chunk i is h_base[pos, ends[i])
+    while (i < n_chunks) {
+        const uint64_t len0 = ends[i] - pos;
+        if (!len0) {
+            ++i;
+            continue;
+        }
+        const uint64_t need0 = len0 + (h_base[ends[i] - 1] == '\n' ? 0 : 1);
+        if (need0 > b->cap) {  // pieces with halos, as kc_batch_add sends them
+            if ((rc = kc_batch_add(b, h_base + pos, len0)) != KC_OK) return rc;
+            pos = ends[i++];
+            continue;
+        }
+        if (b->slot[b->cur].fill + need0 > b->cap && (rc = launch_slot(b, b->cur, 0)) != KC_OK) return rc;
+        if ((rc = slot_ready(b)) != KC_OK) return rc;
+        Slot &s = b->slot[b->cur];
+        // the chunks that fit in this slot; a span of chunks that end in
+        // '\n' is contiguous in h_base and goes in as one copy
+        uint64_t span = pos;  // first source byte not copied yet (lands at s.pinned + s.fill)
+        while (i < n_chunks) {
+            const uint64_t len = ends[i] - pos;
+            if (!len) {
+                ++i;
+                continue;
+            }
+            const bool nl = h_base[ends[i] - 1] == '\n';
+            if (s.fill + (pos - span) + len + (nl ? 0 : 1) > b->cap) break;
+            ++b->info.chunks;
+            pos = ends[i++];
+            if (!nl) {
+                copy_par(s.pinned + s.fill, h_base + span, pos - span);
+                s.fill += pos - span;
+                s.pinned[s.fill++] = '\n';
+                span = pos;
+            }
+        }
+        copy_par(s.pinned + s.fill, h_base + span, pos - span);
+        s.fill += pos - span;
     }
     return KC_OK;
 }

@@ -135,6 +135,7 @@ EXPORTS = {
     "kc_batch_create": (C.c_int, [C.c_void_p, C.POINTER(Tune), C.POINTER(C.c_void_p)]),
     "kc_batch_destroy": (None, [C.c_void_p]),
     "kc_batch_add": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
+    "kc_batch_add_chunks": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
     "kc_batch_flush": (C.c_int, [C.c_void_p]),
     "kc_batch_sync": (C.c_int, [C.c_void_p]),
     "kc_batch_launch_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
@@ -527,6 +528,16 @@ class Batch:
         buf = np.frombuffer(chunk, dtype=np.uint8) if isinstance(chunk, (bytes, bytearray)) else \
             np.ascontiguousarray(chunk, dtype=np.uint8)
         check(lib().kc_batch_add(self.h, _ptr(buf), buf.size), "kc_batch_add")
+
+    def add_chunks(self, buf, ends):
+        """Chunks buf[ends[i-1]:ends[i]] of one host buffer in one call
+        (the batches of add() per chunk)."""
+        buf = np.frombuffer(buf, dtype=np.uint8) if isinstance(buf, (bytes, bytearray)) else \
+            np.ascontiguousarray(buf, dtype=np.uint8)
+        e = np.ascontiguousarray(ends, dtype=np.uint64)
+        if e.size and int(e[-1]) > buf.size:
+            raise ValueError("chunk ends past the buffer")
+        check(lib().kc_batch_add_chunks(self.h, _ptr(buf), _ptr(e), e.size), "kc_batch_add_chunks")
 
     def flush(self):
         check(lib().kc_batch_flush(self.h), "kc_batch_flush")

@@ -91,6 +91,65 @@ def test_batch_host_chunks_exact(ctx, k, bloom, path):
 
 
 @pytest.mark.parametrize("path", ["direct", "partitioned"])
+def test_batch_add_chunks_matches_add(ctx, path):
+    """kc_batch_add_chunks over one buffer packs the same batches as
+    kc_batch_add per chunk (same launches, pieces and bytes) and counts the
+    same keys; spans of >= 16 MB take the multithreaded copy, and the input
+    holds empty chunks, chunks without a final newline and one chunk larger
+    than a slot."""
+    from ktcountermemorygreedy_amd import kc
+    g = kc.synth_genome(41, 2_000_000)
+    seq = kc.synth_reads(g, 42, 200_000, 150, True, 0.005, 1e-4).tobytes()
+    rng = np.random.default_rng(43)
+    lines = seq.split(b"\n")[:-1]
+    chunks, i = [], 0
+    while i < len(lines):
+        j = i + (130_000 if len(chunks) == 20 else int(rng.integers(1, 400)))
+        c = b"\n".join(lines[i:j]) + b"\n"
+        if i < len(lines) // 2 and rng.random() < 0.02:  # later spans run to a full slot
+            c = c[:-1]
+        chunks.append(c)
+        if rng.random() < 0.01:
+            chunks.append(b"")
+        i = j
+    buf = np.frombuffer(b"".join(chunks), dtype=np.uint8)
+    ends = np.cumsum([len(c) for c in chunks]).astype(np.uint64)
+    print("add_chunks: path=%s chunks=%d bytes=%d" % (path, len(chunks), buf.size), flush=True)
+    ta = kc.Table(ctx, 31, capacity=1 << 25, bloom=True, max_distinct=buf.size)
+    tc = kc.Table(ctx, 31, capacity=1 << 25, bloom=True, max_distinct=buf.size)
+    ba = kc.Batch(ta, batch_bytes=16 << 20, queue_depth=3, path=path)
+    bc = kc.Batch(tc, batch_bytes=16 << 20, queue_depth=3, path=path)
+    for c in chunks:
+        ba.add(c)
+    bc.add_chunks(buf, ends)
+    for b in (ba, bc):
+        b.flush()
+        b.sync()
+    ia, ic = ba.info(), bc.info()
+    for f in ("chunks", "pieces", "launches", "bytes"):
+        assert ia[f] == ic[f], (f, ia, ic)
+    assert ic["pieces"] >= 2 and ic["chunks"] == sum(1 for c in chunks if c)
+    ba.close()
+    bc.close()
+    sa, sc = ta.stats(), tc.stats()
+    # (table keys before the FIRST-list fold follow the Bloom decisions,
+    # which depend on the order concurrent waves reach the filter)
+    assert sc["errors"] == 0 and sa["kmers"] == sc["kmers"]
+    assert sc["kmers"] == O.extract(_joined(chunks), 31).shape[0]
+    assert np.array_equal(ta.histogram(1, 2000), tc.histogram(1, 2000))
+    keys = O.extract(seq[:200_000], 31)[:, 0].copy()
+    assert np.array_equal(ta.query(keys), tc.query(keys))
+    with pytest.raises(kc.KCError):  # ends must not decrease
+        b = kc.Batch(tc, batch_bytes=1 << 20, path=path)
+        try:
+            b.add_chunks(buf, np.array([100, 50], dtype=np.uint64))
+        finally:
+            b.close()
+    ta.close()
+    tc.close()
+
+
+@pytest.mark.parametrize("path", ["direct", "partitioned"])
 def test_batch_graph_replays_match_eager(ctx, path):
     """The same slot graphs replayed over different batches, with eager
     counting and synchronisations in between: every replay equals the eager
