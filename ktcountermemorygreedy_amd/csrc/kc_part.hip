@@ -189,10 +189,12 @@ __global__ __launch_bounds__(TB) void k_front_scatter_seq_staged(SeqSrc src, uin
     __syncthreads();
     constexpr uint64_t kTileBytes = (uint64_t)TB * kParseSeg;
     const uint64_t ntiles = (n + kTileBytes - 1) / kTileBytes;
+    // this thread's segment of the tile; the next tile's is loaded once the
+    // last chunk is parsed, so it is in flight during that chunk's write-out
+    uint32_t w[HW + 16];
+    if (blockIdx.x < ntiles) load_segment<HW>(src.seq, n, (uint64_t)blockIdx.x * kTileBytes + (uint64_t)t * kParseSeg, w);
     for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const uint64_t start = tile * kTileBytes + (uint64_t)t * kParseSeg;
-        uint32_t w[HW + 16];
-        load_segment<HW>(src.seq, n, start, w);
         // segment bytes that emit nothing: before emit_from, or past the end
         const uint32_t skip = start >= n ? 64u
                             : emit_from > start ? (uint32_t)(emit_from - start < 64 ? emit_from - start : 64) : 0u;
@@ -232,6 +234,8 @@ __global__ __launch_bounds__(TB) void k_front_scatter_seq_staged(SeqSrc src, uin
                     }
                 }
             }
+            if (q == kChunks - 1 && tile + gridDim.x < ntiles)
+                load_segment<HW>(src.seq, n, start + (uint64_t)gridDim.x * kTileBytes, w);
             __syncthreads();
             uint32_t total;
             const uint32_t h = t < 256 ? hist[t] : 0;
@@ -368,12 +372,13 @@ __global__ __launch_bounds__(kKeyBlock) void k_fine_count(const uint64_t *__rest
 // Fine scatter with write combining: a slice is handled in chunks of
 // TB x FPT keys held in registers; each chunk's keys are ranked per fine
 // digit, placed in LDS grouped by digit and written out as runs.  Longer
-// runs leave fewer partly written lines: with 256 fine digits a chunk is
-// 16384 keys (one 1024-thread block per CU, ~64-key runs); NF = 1024 (more
-// than 2^16 partitions) uses 8192-key chunks in 256-thread blocks.
+// runs leave fewer partly written lines: a chunk is 16384 one-word or 8192
+// two-word keys (one 1024-thread block per CU; ~64-key runs at 256 fine
+// digits, ~16 or ~8 at 1024).  The next chunk's keys are loaded while this
+// one is written out, so one block per CU still keeps reads in flight.
 // SB: keep each staged key's digit in LDS (sbin); without it the digit is
 // recomputed from the staged key at write-out, which leaves the LDS to
-// larger chunks (1024 digits: 16384 one-word or 8192 two-word keys per chunk).
+// larger chunks.
 template <int NW, int NF, int TB, int FPT, bool SB = true>
 __global__ __launch_bounds__(TB) void k_fine_scatter_staged(const uint64_t *__restrict__ keys,
                                                             const uint64_t *__restrict__ starts, Digits d,
@@ -398,11 +403,10 @@ __global__ __launch_bounds__(TB) void k_fine_scatter_staged(const uint64_t *__re
         hist[f] = 0;
     }
     __syncthreads();
-    for (uint64_t cs = beg; cs < end; cs += CH) {
+    // a chunk's FPT keys per thread (index clamped into the chunk)
+    uint64_t key[FPT][NW];
+    auto load_chunk = [&](uint64_t cs) {
         const uint32_t cn = (uint32_t)(end - cs < CH ? end - cs : CH);
-        uint64_t key[FPT][NW];
-        uint32_t bin[FPT], rank[FPT];
-        // all FPT loads first (index clamped into the chunk), then the ranks
 #pragma unroll
         for (uint32_t j = 0; j < (uint32_t)FPT; ++j) {
             const uint32_t i = j * TB + t;
@@ -415,6 +419,11 @@ __global__ __launch_bounds__(TB) void k_fine_scatter_staged(const uint64_t *__re
                 key[j][0] = src[0];
             }
         }
+    };
+    if (beg < end) load_chunk(beg);
+    for (uint64_t cs = beg; cs < end; cs += CH) {
+        const uint32_t cn = (uint32_t)(end - cs < CH ? end - cs : CH);
+        uint32_t bin[FPT], rank[FPT];
 #pragma unroll
         for (uint32_t j = 0; j < (uint32_t)FPT; ++j) {
             const uint32_t i = j * TB + t;
@@ -445,6 +454,9 @@ __global__ __launch_bounds__(TB) void k_fine_scatter_staged(const uint64_t *__re
                 if constexpr (SB) sbin[p] = (BinT)bin[j];
             }
         }
+        // the next chunk's loads stay in flight through the write-out below
+        // (the barriers wait for LDS operations only)
+        if (cs + CH < end) load_chunk(cs + CH);
         __syncthreads();
         for (uint32_t p = t; p < cn; p += TB) {
             uint32_t f;
