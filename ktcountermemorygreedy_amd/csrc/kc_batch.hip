@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <new>
 #include <thread>
 #include <vector>
 
@@ -31,6 +32,7 @@ namespace {
 
 constexpr uint64_t kHalo = 64;  // >= k-1 for k <= 63, multiple of 16
 constexpr uint32_t kMaxSlots = 8;
+constexpr size_t kMaxDevGraphs = 8;  // graphs cached by kc_batch_launch_device
 constexpr uint64_t kPartMinBatch = 32ull << 20;
 
 __global__ void k_set_hdr(uint64_t *hdr, uint64_t n, uint64_t emit_from)
@@ -275,7 +277,14 @@ extern "C" int kc_batch_create(kc_table *t, const kc_tune *tune, kc_batch **out)
     *out = nullptr;
     if (tune && tune->path > KC_PATH_PARTITIONED) return KC_EINVAL;
     KC_HIP(hipSetDevice(t->ctx->device));
-    kc_batch *b = new kc_batch();
+    kc_batch *b = new (std::nothrow) kc_batch();
+    if (!b) return KC_ENOMEM;
+    try {
+        b->dev.reserve(kMaxDevGraphs);  // kc_batch_launch_device then never reallocates
+    } catch (...) {
+        delete b;
+        return KC_ENOMEM;
+    }
     b->t = t;
     resolve_tune(t, tune, b->tune, b->path);
     b->cap = b->tune.batch_bytes;
@@ -443,7 +452,7 @@ extern "C" int kc_batch_launch_device(kc_batch *b, const uint8_t *d_seq, uint64_
     for (DevGraph &x : b->dev)
         if (x.ptr == d_seq && x.cap >= nbytes) g = &x;
     if (!g) {
-        if (b->dev.size() >= 8) {  // drop the least recently used graph
+        if (b->dev.size() >= kMaxDevGraphs) {  // drop the least recently used graph
             auto lru = std::min_element(b->dev.begin(), b->dev.end(),
                                         [](const DevGraph &a, const DevGraph &c) { return a.last_use < c.last_use; });
             KC_HIP(hipStreamSynchronize(b->t->ctx->stream));
@@ -500,17 +509,19 @@ extern "C" int kc_autotune(kc_table *t, const uint8_t *d_sample, uint64_t nbytes
     struct Cand {
         uint32_t path, blocks, slices;
     };
-    std::vector<Cand> cands;
+    Cand cands[17];
+    uint32_t n_cands = 0;
     const uint32_t cu = t->ctx->cu_count;
     const uint32_t forced = t->cfg.count_path;  // a path fixed by the table config is kept
     if (kc_part_eligible(t) && forced != KC_PATH_DIRECT)
         for (uint32_t bpc : {2u, 4u, 8u, 16u})
-            for (uint32_t sl : {2u, 4u, 8u, 16u}) cands.push_back({(uint32_t)KC_PATH_PARTITIONED, cu * bpc, sl});
-    if (forced != KC_PATH_PARTITIONED) cands.push_back({(uint32_t)KC_PATH_DIRECT, 0, 0});
+            for (uint32_t sl : {2u, 4u, 8u, 16u}) cands[n_cands++] = {(uint32_t)KC_PATH_PARTITIONED, cu * bpc, sl};
+    if (forced != KC_PATH_PARTITIONED) cands[n_cands++] = {(uint32_t)KC_PATH_DIRECT, 0, 0};
     float best_t = 0;
     int rc = KC_OK;
     kc_tune win = saved;
-    for (const Cand &c : cands) {
+    for (uint32_t ci = 0; ci < n_cands; ++ci) {
+        const Cand &c = cands[ci];
         t->tune = saved;
         t->tune.front_blocks = c.blocks;
         t->tune.fine_slices = c.slices;

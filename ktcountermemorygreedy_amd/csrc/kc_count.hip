@@ -8,10 +8,12 @@
 #include <cstdio>
 #include <cstring>
 #include <mutex>
+#include <new>
 #include <string>
 #include <vector>
 
 #include "kc_dev_common.h"
+#include "kc_guard.h"
 
 namespace {
 thread_local char g_last_error[256];
@@ -586,7 +588,13 @@ struct HostCounters {
 // list is active, uploads its segment offsets for the kernels that read it.
 int read_counters(kc_table *t, HostCounters &hc)
 {
-    std::vector<uint64_t> h(KC_COUNTER_WORDS);
+    std::vector<uint64_t> h, off;
+    try {
+        h.resize(KC_COUNTER_WORDS);
+        off.resize(KC_LOW_SEGS + 1);
+    } catch (...) {
+        return KC_ENOMEM;
+    }
     KC_HIP(hipMemcpyAsync(h.data(), t->counters, KC_COUNTER_WORDS * 8, hipMemcpyDeviceToHost, t->ctx->stream));
     KC_HIP(hipStreamSynchronize(t->ctx->stream));
     for (int st = 0; st < KC_STAT_N; ++st) {
@@ -594,7 +602,6 @@ int read_counters(kc_table *t, HostCounters &hc)
         for (int l = 0; l < KC_LANES; ++l) hc.stat[st] += h[((uint64_t)st * KC_LANES + l) * KC_LINE_WORDS];
     }
     hc.low = 0;
-    std::vector<uint64_t> off(KC_LOW_SEGS + 1);
     for (int sgi = 0; sgi < KC_LOW_SEGS; ++sgi) {
         const uint64_t c = h[KC_CURSOR_BASE + (uint64_t)sgi * KC_LINE_WORDS];
         // a segment's cursor runs past its capacity when it overflowed; the
@@ -631,7 +638,8 @@ extern "C" int kc_ctx_create(int device, kc_ctx **out)
     KC_HIP(hipSetDevice(device));
     hipDeviceProp_t prop;
     KC_HIP(hipGetDeviceProperties(&prop, device));
-    kc_ctx *c = new kc_ctx;
+    kc_ctx *c = new (std::nothrow) kc_ctx;
+    if (!c) return KC_ENOMEM;
     c->device = device;
     c->cu_count = (uint32_t)prop.multiProcessorCount;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -769,7 +777,8 @@ extern "C" int kc_table_create(kc_ctx *ctx, const kc_table_config *cfg, kc_table
     if (!cfg->capacity && !cfg->max_distinct) return KC_EINVAL;
     KC_HIP(hipSetDevice(ctx->device));
 
-    kc_table *t = new kc_table();
+    kc_table *t = new (std::nothrow) kc_table();
+    if (!t) return KC_ENOMEM;
     t->ctx = ctx;
     t->cfg = *cfg;
     t->nw = cfg->k <= 32 ? 1 : 2;
@@ -1121,12 +1130,16 @@ static int export_dir(kc_table *t, const char *dir, bool append)
     rc = kc_table_export_device(t, dh, hb, &hb2, dl, lb, &lb2);
     std::vector<uint8_t> hh, hl;
     if (rc == KC_OK) {
-        hh.resize(hb2);
-        hl.resize(lb2);
-        if (hipMemcpy(hh.data(), dh, hb2, hipMemcpyDeviceToHost) != hipSuccess ||
-            hipMemcpy(hl.data(), dl, lb2, hipMemcpyDeviceToHost) != hipSuccess)
-            rc = KC_EDEVICE;
+        try {
+            hh.resize(hb2);
+            hl.resize(lb2);
+        } catch (...) {
+            rc = KC_ENOMEM;
+        }
     }
+    if (rc == KC_OK && (hipMemcpy(hh.data(), dh, hb2, hipMemcpyDeviceToHost) != hipSuccess ||
+                        hipMemcpy(hl.data(), dl, lb2, hipMemcpyDeviceToHost) != hipSuccess))
+        rc = KC_EDEVICE;
     (void)hipFree(dh);
     (void)hipFree(dl);
     if (rc != KC_OK) return rc;
@@ -1136,11 +1149,17 @@ static int export_dir(kc_table *t, const char *dir, bool append)
     return write_file(d + "low.bin", hl.data(), hl.size(), append);
 }
 
-extern "C" int kc_table_export_dir(kc_table *t, const char *dir) { return export_dir(t, dir, false); }
+extern "C" int kc_table_export_dir(kc_table *t, const char *dir)
+{
+    return kc_guard([&] { return export_dir(t, dir, false); });
+}
 
 // high.bin and low.bin are record streams, so the exports of several passes
 // (or tables) concatenate into one valid export.
-extern "C" int kc_table_export_dir_append(kc_table *t, const char *dir) { return export_dir(t, dir, true); }
+extern "C" int kc_table_export_dir_append(kc_table *t, const char *dir)
+{
+    return kc_guard([&] { return export_dir(t, dir, true); });
+}
 
 extern "C" int kc_table_set_pass(kc_table *t, uint32_t pass, uint32_t n_passes)
 {

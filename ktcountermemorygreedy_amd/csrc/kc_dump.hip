@@ -353,10 +353,15 @@ extern "C" int kc_table_dump_file(kc_table *t, uint32_t min_freq, uint32_t max_f
     uint64_t bytes = 0;
     int rc = dump_device(t, min_freq, max_freq, sorted, text, bytes);
     if (rc != KC_OK) return rc;
+    constexpr uint64_t kChunk = 64ull << 20;
+    std::vector<uint8_t> buf;
+    try {
+        buf.resize(std::min<uint64_t>(kChunk, std::max<uint64_t>(bytes, 1)));
+    } catch (...) {
+        return KC_ENOMEM;
+    }
     FILE *f = std::fopen(path, "wb");
     if (!f) return KC_EIO;
-    constexpr uint64_t kChunk = 64ull << 20;
-    std::vector<uint8_t> buf(std::min<uint64_t>(kChunk, std::max<uint64_t>(bytes, 1)));
     for (uint64_t o = 0; o < bytes && rc == KC_OK; o += kChunk) {
         const uint64_t m = std::min(kChunk, bytes - o);
         if (hipMemcpy(buf.data(), static_cast<uint8_t *>(text.p) + o, m, hipMemcpyDeviceToHost) != hipSuccess)

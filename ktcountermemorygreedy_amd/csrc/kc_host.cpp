@@ -16,6 +16,7 @@
 
 #include "../../include/kc_api.h"
 #include "../../include/kc_host.h"
+#include "kc_guard.h"
 
 namespace {
 
@@ -75,66 +76,74 @@ extern "C" void kc_host_free(void *p) { std::free(p); }
 
 extern "C" int kc_fastq_frame_file(const char *path, uint8_t **out, uint64_t *out_len)
 {
-    if (!path || !out || !out_len) return KC_EINVAL;
-    const std::string f(path);
-    const bool gz = f.size() >= 3 && f.compare(f.size() - 3, 3, ".gz") == 0;
-    gzFile g = gzopen(path, "rb");  // reads plain files transparently too
-    if (!g) return KC_EIO;
-    gzbuffer(g, 1u << 20);
-    (void)gz;
-    uint64_t cap = 1ull << 24, used = 0;
-    uint8_t *buf = static_cast<uint8_t *>(std::malloc(cap));
-    std::vector<uint8_t> in(1u << 22);
-    Framer fr;
-    for (;;) {
-        const int r = gzread(g, in.data(), (unsigned)in.size());
-        if (r < 0) {
+    return kc_guard([&]() -> int {
+        if (!path || !out || !out_len) return KC_EINVAL;
+        const std::string f(path);
+        const bool gz = f.size() >= 3 && f.compare(f.size() - 3, 3, ".gz") == 0;
+        std::vector<uint8_t> in(1u << 22);
+        gzFile g = gzopen(path, "rb");  // reads plain files transparently too
+        if (!g) return KC_EIO;
+        gzbuffer(g, 1u << 20);
+        (void)gz;
+        uint64_t cap = 1ull << 24, used = 0;
+        uint8_t *buf = static_cast<uint8_t *>(std::malloc(cap));
+        if (!buf) {
             gzclose(g);
-            std::free(buf);
-            return KC_EIO;
+            return KC_ENOMEM;
         }
-        if (r == 0) break;
-        if (used + (uint64_t)r > cap) {
-            while (used + (uint64_t)r > cap) cap *= 2;
-            uint8_t *nb = static_cast<uint8_t *>(std::realloc(buf, cap));
-            if (!nb) {
+        Framer fr;
+        for (;;) {
+            const int r = gzread(g, in.data(), (unsigned)in.size());
+            if (r < 0) {
                 gzclose(g);
                 std::free(buf);
-                return KC_ENOMEM;
+                return KC_EIO;
             }
-            buf = nb;
+            if (r == 0) break;
+            if (used + (uint64_t)r > cap) {
+                while (used + (uint64_t)r > cap) cap *= 2;
+                uint8_t *nb = static_cast<uint8_t *>(std::realloc(buf, cap));
+                if (!nb) {
+                    gzclose(g);
+                    std::free(buf);
+                    return KC_ENOMEM;
+                }
+                buf = nb;
+            }
+            used += fr.feed(in.data(), (uint64_t)r, buf + used);
         }
-        used += fr.feed(in.data(), (uint64_t)r, buf + used);
-    }
-    gzclose(g);
-    *out = buf;
-    *out_len = used;
-    return KC_OK;
+        gzclose(g);
+        *out = buf;
+        *out_len = used;
+        return KC_OK;
+    });
 }
 
 extern "C" int kc_synth_genome(const kc_genome_spec *sp, uint8_t *out)
 {
-    if (!sp || !out || !sp->length) return KC_EINVAL;
-    SplitMix rng{sp->seed};
-    for (uint64_t i = 0; i < sp->length; ++i) out[i] = kBase[rng.next() & 3];
-    if (!sp->n_segments || sp->repeat_fraction <= 0) return KC_OK;
-    if (sp->segment_len == 0 || sp->segment_len > sp->length) return KC_EINVAL;
-    // repeat families, then diverged copies until repeat_fraction is reached
-    std::vector<uint8_t> seg((uint64_t)sp->n_segments * sp->segment_len);
-    for (auto &b : seg) b = (uint8_t)(rng.next() & 3);
-    const uint64_t target = (uint64_t)(sp->repeat_fraction * (double)sp->length);
-    const uint64_t div = threshold(sp->segment_divergence);
-    for (uint64_t copied = 0; copied < target; copied += sp->segment_len) {
-        const uint64_t s = rng.below(sp->n_segments);
-        const uint64_t pos = rng.below(sp->length - sp->segment_len + 1);
-        for (uint32_t j = 0; j < sp->segment_len; ++j) {
-            uint8_t b = seg[s * sp->segment_len + j];
-            const uint64_t r = rng.next();
-            if (r < div) b = (uint8_t)((b + 1 + (r & 0xFFFF) % 3) & 3);
-            out[pos + j] = kBase[b];
+    return kc_guard([&]() -> int {
+        if (!sp || !out || !sp->length) return KC_EINVAL;
+        SplitMix rng{sp->seed};
+        for (uint64_t i = 0; i < sp->length; ++i) out[i] = kBase[rng.next() & 3];
+        if (!sp->n_segments || sp->repeat_fraction <= 0) return KC_OK;
+        if (sp->segment_len == 0 || sp->segment_len > sp->length) return KC_EINVAL;
+        // repeat families, then diverged copies until repeat_fraction is reached
+        std::vector<uint8_t> seg((uint64_t)sp->n_segments * sp->segment_len);
+        for (auto &b : seg) b = (uint8_t)(rng.next() & 3);
+        const uint64_t target = (uint64_t)(sp->repeat_fraction * (double)sp->length);
+        const uint64_t div = threshold(sp->segment_divergence);
+        for (uint64_t copied = 0; copied < target; copied += sp->segment_len) {
+            const uint64_t s = rng.below(sp->n_segments);
+            const uint64_t pos = rng.below(sp->length - sp->segment_len + 1);
+            for (uint32_t j = 0; j < sp->segment_len; ++j) {
+                uint8_t b = seg[s * sp->segment_len + j];
+                const uint64_t r = rng.next();
+                if (r < div) b = (uint8_t)((b + 1 + (r & 0xFFFF) % 3) & 3);
+                out[pos + j] = kBase[b];
+            }
         }
-    }
-    return KC_OK;
+        return KC_OK;
+    });
 }
 
 namespace {
@@ -238,8 +247,17 @@ void parallel_for(uint64_t n, uint32_t threads, F fn, uint64_t grain = 4096)
         return;
     }
     std::vector<std::thread> ws;
-    for (uint32_t t = 0; t < th; ++t)
-        ws.emplace_back([=] { fn(n * t / th, n * (t + 1) / th); });
+    uint32_t started = 0;
+    try {
+        ws.reserve(th);
+        for (; started < th; ++started) {
+            const uint32_t t = started;
+            ws.emplace_back([=] { fn(n * t / th, n * (t + 1) / th); });
+        }
+    } catch (...) {
+        // a thread that could not start: this one takes the ranges left
+    }
+    for (uint32_t t = started; t < th; ++t) fn(n * t / th, n * (t + 1) / th);
     for (auto &w : ws) w.join();
 }
 
@@ -247,46 +265,50 @@ void parallel_for(uint64_t n, uint32_t threads, F fn, uint64_t grain = 4096)
 
 extern "C" int64_t kc_synth_reads_size(const uint8_t *genome, uint64_t glen, const kc_read_spec *sp, int fastq)
 {
-    if (!genome || !sp || !glen) return KC_EINVAL;
-    uint64_t total = 0;
-    parallel_for(sp->n_reads, sp->threads, [&](uint64_t a, uint64_t b) {
-        uint64_t s = 0;
-        for (uint64_t i = a; i < b; ++i) s += record_bytes(plan_read(sp, glen, i), i, fastq != 0);
-        __atomic_fetch_add(&total, s, __ATOMIC_RELAXED);
+    return kc_guard([&]() -> int64_t {
+        if (!genome || !sp || !glen) return KC_EINVAL;
+        uint64_t total = 0;
+        parallel_for(sp->n_reads, sp->threads, [&](uint64_t a, uint64_t b) {
+            uint64_t s = 0;
+            for (uint64_t i = a; i < b; ++i) s += record_bytes(plan_read(sp, glen, i), i, fastq != 0);
+            __atomic_fetch_add(&total, s, __ATOMIC_RELAXED);
+        });
+        return (int64_t)total;
     });
-    return (int64_t)total;
 }
 
 extern "C" int64_t kc_synth_reads(const uint8_t *genome, uint64_t glen, const kc_read_spec *sp, int fastq,
                                   uint8_t *out, uint64_t cap)
 {
-    if (!genome || !sp || !out || !glen) return KC_EINVAL;
-    const uint64_t n = sp->n_reads;
-    // offsets: per-block sums, then a second pass writes
-    const uint64_t kBlk = 1 << 14;
-    const uint64_t nblk = (n + kBlk - 1) / kBlk;
-    std::vector<uint64_t> boff(nblk + 1, 0);
-    parallel_for(nblk, sp->threads, [&](uint64_t a, uint64_t b) {
-        for (uint64_t bi = a; bi < b; ++bi) {
-            uint64_t s = 0;
-            for (uint64_t i = bi * kBlk; i < std::min(n, (bi + 1) * kBlk); ++i)
-                s += record_bytes(plan_read(sp, glen, i), i, fastq != 0);
-            boff[bi + 1] = s;
-        }
-    }, 1);
-    for (uint64_t bi = 0; bi < nblk; ++bi) boff[bi + 1] += boff[bi];
-    if (boff[nblk] > cap) return KC_EINVAL;
-    parallel_for(nblk, sp->threads, [&](uint64_t a, uint64_t b) {
-        for (uint64_t bi = a; bi < b; ++bi) {
-            uint64_t o = boff[bi];
-            for (uint64_t i = bi * kBlk; i < std::min(n, (bi + 1) * kBlk); ++i) {
-                const ReadPlan p = plan_read(sp, glen, i);
-                write_read(genome, sp, p, i, fastq != 0, out + o);
-                o += record_bytes(p, i, fastq != 0);
+    return kc_guard([&]() -> int64_t {
+        if (!genome || !sp || !out || !glen) return KC_EINVAL;
+        const uint64_t n = sp->n_reads;
+        // offsets: per-block sums, then a second pass writes
+        const uint64_t kBlk = 1 << 14;
+        const uint64_t nblk = (n + kBlk - 1) / kBlk;
+        std::vector<uint64_t> boff(nblk + 1, 0);
+        parallel_for(nblk, sp->threads, [&](uint64_t a, uint64_t b) {
+            for (uint64_t bi = a; bi < b; ++bi) {
+                uint64_t s = 0;
+                for (uint64_t i = bi * kBlk; i < std::min(n, (bi + 1) * kBlk); ++i)
+                    s += record_bytes(plan_read(sp, glen, i), i, fastq != 0);
+                boff[bi + 1] = s;
             }
-        }
-    }, 1);
-    return (int64_t)boff[nblk];
+        }, 1);
+        for (uint64_t bi = 0; bi < nblk; ++bi) boff[bi + 1] += boff[bi];
+        if (boff[nblk] > cap) return KC_EINVAL;
+        parallel_for(nblk, sp->threads, [&](uint64_t a, uint64_t b) {
+            for (uint64_t bi = a; bi < b; ++bi) {
+                uint64_t o = boff[bi];
+                for (uint64_t i = bi * kBlk; i < std::min(n, (bi + 1) * kBlk); ++i) {
+                    const ReadPlan p = plan_read(sp, glen, i);
+                    write_read(genome, sp, p, i, fastq != 0, out + o);
+                    o += record_bytes(p, i, fastq != 0);
+                }
+            }
+        }, 1);
+        return (int64_t)boff[nblk];
+    });
 }
 
 // ---- streaming ingest: FASTQ(.gz) files -> framed chunks -> kc_batch -------
@@ -306,105 +328,128 @@ constexpr unsigned kGzRead = 1u << 20;
 extern "C" int kc_count_fastq_files(kc_table *t, const char *const *paths, uint32_t n_paths, uint32_t threads,
                                     const kc_tune *tune, kc_ingest_stats *stats)
 {
-    if (!t || (n_paths && !paths)) return KC_EINVAL;
-    for (uint32_t i = 0; i < n_paths; ++i)
-        if (!paths[i]) return KC_EINVAL;
-    kc_batch *b = nullptr;
-    int rc = kc_batch_create(t, tune, &b);
-    if (rc != KC_OK) return rc;
-    const auto t0 = std::chrono::steady_clock::now();
-    std::mutex mu;  // guards the batch and the shared results
-    std::atomic<uint32_t> next{0};
-    int first_error = KC_OK;
-    uint64_t in_bytes = 0, framed = 0, chunks = 0;
-    double inflate_s = 0;
-    auto worker = [&]() {
-        std::vector<uint8_t> in(kGzRead), buf(kIngestChunk + kGzRead);
-        for (;;) {
-            const uint32_t f = next.fetch_add(1);
-            if (f >= n_paths) return;
-            {
+    return kc_guard([&]() -> int {
+        if (!t || (n_paths && !paths)) return KC_EINVAL;
+        for (uint32_t i = 0; i < n_paths; ++i)
+            if (!paths[i]) return KC_EINVAL;
+        kc_batch *b = nullptr;
+        int rc = kc_batch_create(t, tune, &b);
+        if (rc != KC_OK) return rc;
+        const auto t0 = std::chrono::steady_clock::now();
+        std::mutex mu;  // guards the batch and the shared results
+        std::atomic<uint32_t> next{0};
+        int first_error = KC_OK;
+        uint64_t in_bytes = 0, framed = 0, chunks = 0;
+        double inflate_s = 0;
+        auto work = [&]() {
+            std::vector<uint8_t> in(kGzRead), buf(kIngestChunk + kGzRead);
+            for (;;) {
+                const uint32_t f = next.fetch_add(1);
+                if (f >= n_paths) return;
+                {
+                    std::lock_guard<std::mutex> lk(mu);
+                    if (first_error != KC_OK) return;
+                }
+                gzFile g = gzopen(paths[f], "rb");
+                if (!g) {
+                    std::lock_guard<std::mutex> lk(mu);
+                    if (first_error == KC_OK) first_error = KC_EIO;
+                    return;
+                }
+                gzbuffer(g, kGzRead);
+                Framer fr;
+                uint64_t used = 0, my_in = 0, my_framed = 0, my_chunks = 0;
+                double my_inflate = 0;
+                int err = KC_OK;
+                for (;;) {
+                    const auto a = std::chrono::steady_clock::now();
+                    const int r = gzread(g, in.data(), kGzRead);
+                    my_inflate += std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
+                    if (r < 0) {
+                        err = KC_EIO;
+                        break;
+                    }
+                    if (r == 0) break;
+                    my_in += (uint64_t)r;
+                    used += fr.feed(in.data(), (uint64_t)r, buf.data() + used);
+                    if (used >= kIngestChunk) {
+                        // hand over whole reads; a partial sequence line waits for the next block
+                        const uint8_t *nl = static_cast<const uint8_t *>(memrchr(buf.data(), '\n', used));
+                        const uint64_t cut = nl ? (uint64_t)(nl - buf.data()) + 1 : 0;
+                        if (cut) {
+                            std::lock_guard<std::mutex> lk(mu);
+                            err = kc_batch_add(b, buf.data(), cut);
+                        }
+                        if (err != KC_OK) break;
+                        std::memmove(buf.data(), buf.data() + cut, used - cut);
+                        my_framed += cut;
+                        my_chunks += cut ? 1 : 0;
+                        used -= cut;
+                    }
+                    // room for the next block (more than a chunk when one sequence line is longer)
+                    if (buf.size() < used + kGzRead + (used > kIngestChunk ? kIngestChunk : 0)) {
+                        try {
+                            buf.resize(used + kGzRead + (used > kIngestChunk ? kIngestChunk : 0));
+                        } catch (...) {
+                            err = KC_ENOMEM;
+                            break;
+                        }
+                    }
+                }
+                gzclose(g);
+                if (err == KC_OK && used) {
+                    std::lock_guard<std::mutex> lk(mu);
+                    err = kc_batch_add(b, buf.data(), used);  // the file's last read (newline added if missing)
+                    my_framed += used;
+                    ++my_chunks;
+                }
                 std::lock_guard<std::mutex> lk(mu);
+                in_bytes += my_in;
+                framed += my_framed;
+                chunks += my_chunks;
+                inflate_s += my_inflate;
+                if (err != KC_OK && first_error == KC_OK) first_error = err;
                 if (first_error != KC_OK) return;
             }
-            gzFile g = gzopen(paths[f], "rb");
-            if (!g) {
+        };
+        // an allocation that fails in a worker ends the ingest with KC_ENOMEM
+        auto worker = [&]() {
+            try {
+                work();
+            } catch (...) {
                 std::lock_guard<std::mutex> lk(mu);
-                if (first_error == KC_OK) first_error = KC_EIO;
-                return;
+                if (first_error == KC_OK) first_error = KC_ENOMEM;
             }
-            gzbuffer(g, kGzRead);
-            Framer fr;
-            uint64_t used = 0, my_in = 0, my_framed = 0, my_chunks = 0;
-            double my_inflate = 0;
-            int err = KC_OK;
-            for (;;) {
-                const auto a = std::chrono::steady_clock::now();
-                const int r = gzread(g, in.data(), kGzRead);
-                my_inflate += std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
-                if (r < 0) {
-                    err = KC_EIO;
-                    break;
-                }
-                if (r == 0) break;
-                my_in += (uint64_t)r;
-                used += fr.feed(in.data(), (uint64_t)r, buf.data() + used);
-                if (used >= kIngestChunk) {
-                    // hand over whole reads; a partial sequence line waits for the next block
-                    const uint8_t *nl = static_cast<const uint8_t *>(memrchr(buf.data(), '\n', used));
-                    const uint64_t cut = nl ? (uint64_t)(nl - buf.data()) + 1 : 0;
-                    if (cut) {
-                        std::lock_guard<std::mutex> lk(mu);
-                        err = kc_batch_add(b, buf.data(), cut);
-                    }
-                    if (err != KC_OK) break;
-                    std::memmove(buf.data(), buf.data() + cut, used - cut);
-                    my_framed += cut;
-                    my_chunks += cut ? 1 : 0;
-                    used -= cut;
-                    if (used > kIngestChunk) {  // one sequence line longer than a chunk: grow
-                        buf.resize(used + kIngestChunk + kGzRead);
-                    }
-                }
-                if (buf.size() < used + kGzRead) buf.resize(used + kGzRead);
-            }
-            gzclose(g);
-            if (err == KC_OK && used) {
-                std::lock_guard<std::mutex> lk(mu);
-                err = kc_batch_add(b, buf.data(), used);  // the file's last read (newline added if missing)
-                my_framed += used;
-                ++my_chunks;
-            }
-            std::lock_guard<std::mutex> lk(mu);
-            in_bytes += my_in;
-            framed += my_framed;
-            chunks += my_chunks;
-            inflate_s += my_inflate;
-            if (err != KC_OK && first_error == KC_OK) first_error = err;
-            if (first_error != KC_OK) return;
+        };
+        const uint32_t nt =
+            std::max<uint32_t>(1, std::min<uint32_t>(threads ? threads : 1, std::max<uint32_t>(n_paths, 1)));
+        std::vector<std::thread> pool;
+        try {
+            pool.reserve(nt);
+            for (uint32_t i = 0; i < nt; ++i) pool.emplace_back(worker);
+        } catch (...) {
+            // fewer threads: the ones running take every file from the cursor
         }
-    };
-    const uint32_t nt = std::max<uint32_t>(1, std::min<uint32_t>(threads ? threads : 1, std::max<uint32_t>(n_paths, 1)));
-    std::vector<std::thread> pool;
-    for (uint32_t i = 0; i < nt; ++i) pool.emplace_back(worker);
-    for (auto &th : pool) th.join();
-    if (first_error == KC_OK) first_error = kc_batch_flush(b);
-    const int rs = kc_batch_sync(b);
-    if (first_error == KC_OK) first_error = rs;
-    kc_batch_info bi{};
-    kc_batch_get_info(b, &bi);
-    kc_batch_destroy(b);
-    if (stats) {
-        stats->files = n_paths;
-        stats->threads = nt;
-        stats->input_bytes = in_bytes;
-        stats->framed_bytes = framed;
-        stats->chunks = chunks;
-        stats->launches = bi.launches;
-        stats->inflate_seconds = inflate_s;
-        stats->wall_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    }
-    return first_error;
+        if (pool.empty()) worker();
+        for (auto &th : pool) th.join();
+        if (first_error == KC_OK) first_error = kc_batch_flush(b);
+        const int rs = kc_batch_sync(b);
+        if (first_error == KC_OK) first_error = rs;
+        kc_batch_info bi{};
+        kc_batch_get_info(b, &bi);
+        kc_batch_destroy(b);
+        if (stats) {
+            stats->files = n_paths;
+            stats->threads = nt;
+            stats->input_bytes = in_bytes;
+            stats->framed_bytes = framed;
+            stats->chunks = chunks;
+            stats->launches = bi.launches;
+            stats->inflate_seconds = inflate_s;
+            stats->wall_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        }
+        return first_error;
+    });
 }
 
 // ---- out-of-HBM counting: several passes over the files ------------------
@@ -423,7 +468,11 @@ struct DumpReader {
     DumpReader() = default;
     DumpReader(const DumpReader &) = delete;
     DumpReader &operator=(const DumpReader &) = delete;
-    ~DumpReader() { std::free(raw); }
+    ~DumpReader()
+    {
+        if (f) std::fclose(f);
+        std::free(raw);
+    }
     bool next()
     {
         const ssize_t r = getline(&raw, &raw_cap, f);
@@ -445,54 +494,59 @@ struct DumpReader {
 
 extern "C" int kc_merge_sorted_dumps(const char *const *paths, uint32_t n_paths, const char *out_path)
 {
-    if (!out_path || (n_paths && !paths)) return KC_EINVAL;
-    for (uint32_t i = 0; i < n_paths; ++i)
-        if (!paths[i]) return KC_EINVAL;
-    std::vector<DumpReader> in(n_paths);
-    int rc = KC_OK;
-    for (uint32_t i = 0; i < n_paths && rc == KC_OK; ++i) {
-        in[i].f = std::fopen(paths[i], "rb");
-        if (!in[i].f) rc = KC_EIO;
-        else {
-            in[i].buf.resize(1 << 20);
-            std::setvbuf(in[i].f, in[i].buf.data(), _IOFBF, in[i].buf.size());
-        }
-    }
-    FILE *out = rc == KC_OK ? std::fopen(out_path, "wb") : nullptr;
-    if (rc == KC_OK && !out) rc = KC_EIO;
-    if (rc == KC_OK) {
-        std::vector<char> obuf(1 << 20);
-        std::setvbuf(out, obuf.data(), _IOFBF, obuf.size());
-        // min-heap of the readers by their current key (the heap holds the
-        // readers that still have a line)
-        std::vector<uint32_t> heap;
-        auto greater = [&](uint32_t a, uint32_t b) { return in[a].key() > in[b].key(); };
+    return kc_guard([&]() -> int {
+        if (!out_path || (n_paths && !paths)) return KC_EINVAL;
         for (uint32_t i = 0; i < n_paths; ++i)
-            if (in[i].next()) heap.push_back(i);
-        std::make_heap(heap.begin(), heap.end(), greater);
-        while (!heap.empty()) {
-            std::pop_heap(heap.begin(), heap.end(), greater);
-            const uint32_t i = heap.back();
-            const std::string &l = in[i].line;
-            if (std::fwrite(l.data(), 1, l.size(), out) != l.size() || std::fputc('\n', out) == EOF) {
-                rc = KC_EIO;
-                break;
+            if (!paths[i]) return KC_EINVAL;
+        std::vector<DumpReader> in(n_paths);
+        int rc = KC_OK;
+        for (uint32_t i = 0; i < n_paths && rc == KC_OK; ++i) {
+            in[i].f = std::fopen(paths[i], "rb");
+            if (!in[i].f) rc = KC_EIO;
+            else {
+                in[i].buf.resize(1 << 20);
+                std::setvbuf(in[i].f, in[i].buf.data(), _IOFBF, in[i].buf.size());
             }
-            if (in[i].next()) std::push_heap(heap.begin(), heap.end(), greater);
-            else heap.pop_back();
         }
-        if (std::fclose(out) != 0 && rc == KC_OK) rc = KC_EIO;
-    }
-    for (auto &r : in)
-        if (r.f) std::fclose(r.f);
-    return rc;
+        FILE *out = rc == KC_OK ? std::fopen(out_path, "wb") : nullptr;
+        if (rc == KC_OK && !out) rc = KC_EIO;
+        if (rc == KC_OK) {
+            std::vector<char> obuf;  // outlives the stream that buffers into it
+            try {
+                obuf.resize(1 << 20);
+                std::setvbuf(out, obuf.data(), _IOFBF, obuf.size());
+                // min-heap of the readers by their current key (the heap holds
+                // the readers that still have a line)
+                std::vector<uint32_t> heap;
+                auto greater = [&](uint32_t a, uint32_t b) { return in[a].key() > in[b].key(); };
+                for (uint32_t i = 0; i < n_paths; ++i)
+                    if (in[i].next()) heap.push_back(i);
+                std::make_heap(heap.begin(), heap.end(), greater);
+                while (!heap.empty()) {
+                    std::pop_heap(heap.begin(), heap.end(), greater);
+                    const uint32_t i = heap.back();
+                    const std::string &l = in[i].line;
+                    if (std::fwrite(l.data(), 1, l.size(), out) != l.size() || std::fputc('\n', out) == EOF) {
+                        rc = KC_EIO;
+                        break;
+                    }
+                    if (in[i].next()) std::push_heap(heap.begin(), heap.end(), greater);
+                    else heap.pop_back();
+                }
+            } catch (...) {
+                rc = KC_ENOMEM;
+            }
+            if (std::fclose(out) != 0 && rc == KC_OK) rc = KC_EIO;
+        }
+        return rc;  // the readers close their files
+    });
 }
 
 static int append_file(const std::string &from, FILE *to)
 {
+    std::vector<char> b(1 << 20);
     FILE *f = std::fopen(from.c_str(), "rb");
     if (!f) return KC_EIO;
-    std::vector<char> b(1 << 20);
     size_t r;
     int rc = KC_OK;
     while ((r = std::fread(b.data(), 1, b.size(), f)) > 0)
@@ -507,60 +561,67 @@ static int append_file(const std::string &from, FILE *to)
 extern "C" int kc_count_fastq_files_passes(kc_table *t, const char *const *paths, uint32_t n_paths, uint32_t threads,
                                            const kc_tune *tune, const kc_pass_plan *plan, kc_ingest_stats *stats)
 {
-    if (!t || !plan) return KC_EINVAL;
-    const uint32_t P = plan->n_passes;
-    if (P == 0 || P > 256 || (P & (P - 1))) return KC_EINVAL;
-    if (plan->hist_bins && plan->hist_max < plan->hist_min) return KC_EINVAL;
-    if (plan->dump_path && plan->dump_max < plan->dump_min) return KC_EINVAL;
-    const uint64_t nb = plan->hist_bins ? (uint64_t)plan->hist_max - plan->hist_min + 1 : 0;
-    std::vector<int64_t> acc(nb, 0), bins(nb, 0);
-    std::vector<std::string> parts;
-    kc_ingest_stats sum{};
-    int rc = KC_OK;
-    for (uint32_t p = 0; p < P && rc == KC_OK; ++p) {
-        if ((rc = kc_table_clear(t)) != KC_OK || (rc = kc_table_set_pass(t, p, P)) != KC_OK) break;
-        kc_ingest_stats st{};
-        if ((rc = kc_count_fastq_files(t, paths, n_paths, threads, tune, &st)) != KC_OK) break;
-        sum.files = st.files;
-        sum.threads = st.threads;
-        sum.input_bytes += st.input_bytes;
-        sum.framed_bytes += st.framed_bytes;
-        sum.chunks += st.chunks;
-        sum.launches += st.launches;
-        sum.inflate_seconds += st.inflate_seconds;
-        sum.wall_seconds += st.wall_seconds;
-        if (plan->export_dir)
-            rc = p == 0 ? kc_table_export_dir(t, plan->export_dir) : kc_table_export_dir_append(t, plan->export_dir);
-        if (rc == KC_OK && plan->dump_path) {
-            const std::string part = P == 1 ? std::string(plan->dump_path)
-                                            : std::string(plan->dump_path) + ".pass" + std::to_string(p);
-            rc = kc_table_dump_file(t, plan->dump_min, plan->dump_max, plan->dump_sorted, part.c_str());
-            parts.push_back(part);
+    return kc_guard([&]() -> int {
+        if (!t || !plan) return KC_EINVAL;
+        const uint32_t P = plan->n_passes;
+        if (P == 0 || P > 256 || (P & (P - 1))) return KC_EINVAL;
+        if (plan->hist_bins && plan->hist_max < plan->hist_min) return KC_EINVAL;
+        if (plan->dump_path && plan->dump_max < plan->dump_min) return KC_EINVAL;
+        const uint64_t nb = plan->hist_bins ? (uint64_t)plan->hist_max - plan->hist_min + 1 : 0;
+        std::vector<int64_t> acc(nb, 0), bins(nb, 0);
+        std::vector<std::string> parts;
+        parts.reserve(P);
+        kc_ingest_stats sum{};
+        int rc = KC_OK;
+        try {
+            for (uint32_t p = 0; p < P && rc == KC_OK; ++p) {
+                if ((rc = kc_table_clear(t)) != KC_OK || (rc = kc_table_set_pass(t, p, P)) != KC_OK) break;
+                kc_ingest_stats st{};
+                if ((rc = kc_count_fastq_files(t, paths, n_paths, threads, tune, &st)) != KC_OK) break;
+                sum.files = st.files;
+                sum.threads = st.threads;
+                sum.input_bytes += st.input_bytes;
+                sum.framed_bytes += st.framed_bytes;
+                sum.chunks += st.chunks;
+                sum.launches += st.launches;
+                sum.inflate_seconds += st.inflate_seconds;
+                sum.wall_seconds += st.wall_seconds;
+                if (plan->export_dir)
+                    rc = p == 0 ? kc_table_export_dir(t, plan->export_dir)
+                                : kc_table_export_dir_append(t, plan->export_dir);
+                if (rc == KC_OK && plan->dump_path) {
+                    parts.push_back(P == 1 ? std::string(plan->dump_path)
+                                           : std::string(plan->dump_path) + ".pass" + std::to_string(p));
+                    rc = kc_table_dump_file(t, plan->dump_min, plan->dump_max, plan->dump_sorted, parts.back().c_str());
+                }
+                if (rc == KC_OK && nb) {
+                    rc = kc_table_histogram(t, plan->hist_min, plan->hist_max, bins.data());
+                    for (uint64_t i = 0; i < nb && rc == KC_OK; ++i) acc[i] += bins[i];
+                }
+            }
+            if (rc == KC_OK && plan->dump_path && P > 1) {
+                // slices are hash-disjoint: sorted parts merge, unsorted parts concatenate
+                if (plan->dump_sorted) {
+                    std::vector<const char *> pp;
+                    for (auto &s : parts) pp.push_back(s.c_str());
+                    rc = kc_merge_sorted_dumps(pp.data(), (uint32_t)pp.size(), plan->dump_path);
+                } else {
+                    FILE *out = std::fopen(plan->dump_path, "wb");
+                    if (!out) rc = KC_EIO;
+                    for (size_t i = 0; i < parts.size() && rc == KC_OK; ++i) rc = append_file(parts[i], out);
+                    if (out && std::fclose(out) != 0 && rc == KC_OK) rc = KC_EIO;
+                }
+            }
+        } catch (...) {
+            rc = KC_ENOMEM;
         }
-        if (rc == KC_OK && nb) {
-            rc = kc_table_histogram(t, plan->hist_min, plan->hist_max, bins.data());
-            for (uint64_t i = 0; i < nb && rc == KC_OK; ++i) acc[i] += bins[i];
-        }
-    }
-    kc_table_set_pass(t, 0, 1);
-    if (rc == KC_OK && plan->dump_path && P > 1) {
-        // slices are hash-disjoint: sorted parts merge, unsorted parts concatenate
-        if (plan->dump_sorted) {
-            std::vector<const char *> pp;
-            for (auto &s : parts) pp.push_back(s.c_str());
-            rc = kc_merge_sorted_dumps(pp.data(), (uint32_t)pp.size(), plan->dump_path);
-        } else {
-            FILE *out = std::fopen(plan->dump_path, "wb");
-            if (!out) rc = KC_EIO;
-            for (size_t i = 0; i < parts.size() && rc == KC_OK; ++i) rc = append_file(parts[i], out);
-            if (out && std::fclose(out) != 0 && rc == KC_OK) rc = KC_EIO;
-        }
-    }
-    if (P > 1)
-        for (auto &s : parts) std::remove(s.c_str());
-    if (rc == KC_OK && nb) std::memcpy(plan->hist_bins, acc.data(), nb * sizeof(int64_t));
-    if (stats) *stats = sum;
-    return rc;
+        kc_table_set_pass(t, 0, 1);
+        if (P > 1)
+            for (auto &s : parts) std::remove(s.c_str());
+        if (rc == KC_OK && nb) std::memcpy(plan->hist_bins, acc.data(), nb * sizeof(int64_t));
+        if (stats) *stats = sum;
+        return rc;
+    });
 }
 
 // ---- table sizing from a device sample -----------------------------------

@@ -22,12 +22,14 @@
 #include <list>
 #include <memory>
 #include <mutex>
+#include <new>
 #include <thread>
 #include <unordered_map>
 #include <vector>
 
 #include "../../include/kc_api.h"
 #include "../../include/kc_server.h"
+#include "kc_guard.h"
 
 namespace {
 
@@ -161,7 +163,15 @@ struct kc_server {
     {
         uint64_t cap = 0, resident = 0;
         footprint(*r, cap, resident);
-        auto tab = std::make_shared<Table>();
+        std::shared_ptr<Table> tab;
+        try {
+            tab = std::make_shared<Table>();
+        } catch (...) {
+            std::lock_guard<std::mutex> g(mu);
+            r->res.status = KC_ENOMEM;
+            used -= r->footprint;
+            return;
+        }
         tab->id = r->table_id;
         tab->nw = r->k <= 32 ? 1 : 2;
         int rc = kc_ctx_create(device, &tab->ctx);
@@ -186,15 +196,19 @@ struct kc_server {
         r->res.distinct = st.distinct;
         r->res.status = rc;
         std::lock_guard<std::mutex> g(mu);
-        if (rc == KC_OK && tables.count(tab->id)) rc = r->res.status = KC_EINVAL;  // id taken meanwhile
+        if (rc == KC_OK && tables.count(tab->id)) rc = KC_EINVAL;  // id taken meanwhile
         if (rc == KC_OK) {
             tab->bytes = resident;
             tab->last_use = Clock::now();
-            tables[tab->id] = tab;
-            used -= r->footprint - resident;
-        } else {
-            used -= r->footprint;
+            try {
+                tables.emplace(tab->id, tab);
+            } catch (...) {
+                rc = KC_ENOMEM;
+            }
         }
+        r->res.status = rc;
+        if (rc == KC_OK) used -= r->footprint - resident;
+        else used -= r->footprint;
     }
 
     void run_query(Request *r)
@@ -279,17 +293,26 @@ struct kc_server {
 
 extern "C" int kc_server_create(int device, uint32_t workers, uint64_t hbm_budget_bytes, kc_server **out)
 {
-    if (!out || workers == 0 || workers > 64) return KC_EINVAL;
-    kc_ctx *probe = nullptr;
-    int rc = kc_ctx_create(device, &probe);  // validates the device
-    if (rc != KC_OK) return rc;
-    kc_ctx_destroy(probe);
-    auto *s = new kc_server();
-    s->device = device;
-    s->budget = hbm_budget_bytes;
-    for (uint32_t i = 0; i < workers; ++i) s->workers.emplace_back([s] { s->worker(); });
-    *out = s;
-    return KC_OK;
+    return kc_guard([&]() -> int {
+        if (!out || workers == 0 || workers > 64) return KC_EINVAL;
+        kc_ctx *probe = nullptr;
+        int rc = kc_ctx_create(device, &probe);  // validates the device
+        if (rc != KC_OK) return rc;
+        kc_ctx_destroy(probe);
+        auto *s = new (std::nothrow) kc_server();
+        if (!s) return KC_ENOMEM;
+        s->device = device;
+        s->budget = hbm_budget_bytes;
+        try {
+            s->workers.reserve(workers);
+            for (uint32_t i = 0; i < workers; ++i) s->workers.emplace_back([s] { s->worker(); });
+        } catch (...) {  // a thread that could not start: stop the ones that did
+            kc_server_destroy(s);
+            return KC_ENOMEM;
+        }
+        *out = s;
+        return KC_OK;
+    });
 }
 
 extern "C" void kc_server_destroy(kc_server *s)
@@ -305,14 +328,30 @@ extern "C" void kc_server_destroy(kc_server *s)
     delete s;
 }
 
+// Queues r (a count job also counts as pending on its table).  Throws on a
+// failed allocation with the server left as it was.
 static uint64_t enqueue(kc_server *s, std::unique_ptr<Request> r)
 {
     std::lock_guard<std::mutex> g(s->mu);
-    r->id = s->next_id++;
-    r->submit = Clock::now();
-    const uint64_t id = r->id;
-    s->queue.push_back(r.get());
-    s->reqs[id] = std::move(r);
+    const uint64_t id = s->next_id++;
+    const uint64_t table_id = r->table_id;
+    const bool count = r->kind == 0;
+    Request *raw = r.get();
+    raw->id = id;
+    raw->submit = Clock::now();
+    s->reqs.emplace(id, std::move(r));
+    try {
+        s->queue.push_back(raw);
+        try {
+            if (count) ++s->pending_counts[table_id];
+        } catch (...) {
+            s->queue.pop_back();
+            throw;
+        }
+    } catch (...) {
+        s->reqs.erase(id);
+        throw;
+    }
     s->cv_work.notify_one();
     return id;
 }
@@ -320,38 +359,38 @@ static uint64_t enqueue(kc_server *s, std::unique_ptr<Request> r)
 extern "C" int kc_server_submit_count(kc_server *s, uint64_t table_id, uint32_t k, const uint8_t *h_seq,
                                       uint64_t nbytes, uint64_t max_distinct, int bloom, uint64_t *req_id)
 {
-    if (!s || !req_id || (!h_seq && nbytes) || k < 1 || k > 63) return KC_EINVAL;
-    auto r = std::make_unique<Request>();
-    r->kind = 0;
-    r->table_id = table_id;
-    r->k = k;
-    r->bloom = bloom ? 1 : 0;
-    r->seq = h_seq;
-    r->nbytes = nbytes;
-    r->max_distinct = max_distinct ? max_distinct : std::max<uint64_t>(nbytes, 1024);
-    uint64_t cap;
-    r->footprint = kc_server::footprint(*r, cap, r->resident);
-    if (r->footprint > s->budget) return KC_ENOMEM;
-    {
-        std::lock_guard<std::mutex> g(s->mu);
-        ++s->pending_counts[table_id];
-    }
-    *req_id = enqueue(s, std::move(r));
-    return KC_OK;
+    return kc_guard([&]() -> int {
+        if (!s || !req_id || (!h_seq && nbytes) || k < 1 || k > 63) return KC_EINVAL;
+        auto r = std::make_unique<Request>();
+        r->kind = 0;
+        r->table_id = table_id;
+        r->k = k;
+        r->bloom = bloom ? 1 : 0;
+        r->seq = h_seq;
+        r->nbytes = nbytes;
+        r->max_distinct = max_distinct ? max_distinct : std::max<uint64_t>(nbytes, 1024);
+        uint64_t cap;
+        r->footprint = kc_server::footprint(*r, cap, r->resident);
+        if (r->footprint > s->budget) return KC_ENOMEM;
+        *req_id = enqueue(s, std::move(r));
+        return KC_OK;
+    });
 }
 
 extern "C" int kc_server_submit_query(kc_server *s, uint64_t table_id, const uint64_t *h_keys, uint64_t n,
                                       uint32_t *h_counts, uint64_t *req_id)
 {
-    if (!s || !req_id || (n && (!h_keys || !h_counts))) return KC_EINVAL;
-    auto r = std::make_unique<Request>();
-    r->kind = 1;
-    r->table_id = table_id;
-    r->keys = h_keys;
-    r->n = n;
-    r->counts = h_counts;
-    *req_id = enqueue(s, std::move(r));
-    return KC_OK;
+    return kc_guard([&]() -> int {
+        if (!s || !req_id || (n && (!h_keys || !h_counts))) return KC_EINVAL;
+        auto r = std::make_unique<Request>();
+        r->kind = 1;
+        r->table_id = table_id;
+        r->keys = h_keys;
+        r->n = n;
+        r->counts = h_counts;
+        *req_id = enqueue(s, std::move(r));
+        return KC_OK;
+    });
 }
 
 extern "C" int kc_server_wait(kc_server *s, uint64_t req_id, kc_request_result *out)

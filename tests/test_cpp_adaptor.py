@@ -15,6 +15,17 @@ def test_adaptor_header_compiles():
     subprocess.check_call(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", "-I", os.path.join(ROOT, "include"), SRC])
 
 
+def test_guard_maps_exceptions_to_status(tmp_path):
+    """The C-ABI exception guard turns allocation and thread-start failures
+    into KC_ENOMEM and passes ordinary returns through (host only)."""
+    src = os.path.join(ROOT, "tests", "cpp", "guard_check.cpp")
+    exe = str(tmp_path / "guard_check")
+    subprocess.check_call(["g++", "-std=c++17", "-Wall", "-Werror", "-O1", "-I", os.path.join(PKG, "csrc"), src,
+                           "-o", exe])
+    out = subprocess.run([exe], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=60)
+    assert out.returncode == 0 and out.stdout.strip() == "ok", out.stdout
+
+
 @pytest.mark.gpu
 def test_adaptor_runs(tmp_path):
     exe = str(tmp_path / "adaptor_check")
