@@ -219,6 +219,7 @@ class DeviceBuffer:
             check(lib().kc_dev_alloc(ctx.h, max(1, self.nbytes), C.byref(p)), "kc_dev_alloc")
             self.ptr = p.value
             self._own = True
+            ctx._buffers.add(self)
         else:
             self.ptr = ptr
             self._own = False
@@ -255,7 +256,7 @@ class DeviceBuffer:
 
     def free(self):
         if self._own and self.ptr:
-            if self.ctx.h:  # a closed context has released its device
+            if self.ctx.h:  # a closed context has freed its buffers
                 lib().kc_dev_free(self.ctx.h, C.c_void_p(self.ptr))
             self.ptr = 0
 
@@ -275,12 +276,13 @@ class PinnedBuffer:
         check(lib().kc_host_alloc_pinned(ctx.h, max(1, self.nbytes), C.byref(p)), "kc_host_alloc_pinned")
         self.ptr = p.value
         self.array = np.ctypeslib.as_array((C.c_uint8 * max(1, self.nbytes)).from_address(self.ptr))
+        ctx._buffers.add(self)
 
     def free(self):
-        if self.ptr:
+        if self.ptr and self.ctx.h:  # a closed context has freed its buffers
             self.array = None
             lib().kc_host_free_pinned(self.ctx.h, C.c_void_p(self.ptr))
-            self.ptr = 0
+        self.ptr = 0
 
     def __del__(self):
         try:
@@ -298,6 +300,7 @@ class Context:
         self.h = h
         self.device = device
         self._tables = weakref.WeakSet()  # closed before the context: they use its stream
+        self._buffers = weakref.WeakSet()  # device and pinned buffers it allocated, freed on close
 
     def sync(self):
         check(lib().kc_ctx_sync(self.h), "kc_ctx_sync")
@@ -310,6 +313,8 @@ class Context:
         if self.h:
             for t in list(self._tables):
                 t.close()
+            for b in list(self._buffers):
+                b.free()
             lib().kc_ctx_destroy(self.h)
             self.h = None
 

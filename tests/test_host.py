@@ -120,3 +120,61 @@ def test_plan_sizes_passes_and_budget():
         kc.plan_sizes(31, 1 << 34, 1 << 30, bloom=True, hbm_budget=1 << 20)
     with pytest.raises(kc.KCError):
         kc.plan_sizes(64, 1000, 10)
+
+
+def test_context_close_frees_its_buffers(monkeypatch):
+    """Closing a Context frees the device and pinned buffers it allocated
+    (once, before the context itself), not the views into them; freeing a
+    buffer after its context closed is a no-op.  Runs against a stand-in
+    for libkc that records the calls."""
+    import ctypes as C
+    from ktcountermemorygreedy_amd import kc
+    calls, keep = [], []
+
+    class Fake:
+        def kc_ctx_create(self, dev, out):
+            out._obj.value = 0x10
+            return 0
+
+        def kc_dev_alloc(self, h, n, out):
+            out._obj.value = 0x1000 * (len(calls) + 1)
+            calls.append(("alloc", out._obj.value))
+            return 0
+
+        def kc_dev_free(self, h, p):
+            calls.append(("free", p.value))
+            return 0
+
+        def kc_host_alloc_pinned(self, h, n, out):
+            buf = (C.c_uint8 * n)()
+            keep.append(buf)
+            out._obj.value = C.addressof(buf)
+            return 0
+
+        def kc_host_free_pinned(self, h, p):
+            calls.append(("pinned_free", p.value))
+            return 0
+
+        def kc_ctx_destroy(self, h):
+            calls.append(("destroy",))
+
+    fake = Fake()
+    monkeypatch.setattr(kc, "lib", lambda: fake)
+    ctx = kc.Context(0)
+    a = kc.DeviceBuffer(ctx, 64)
+    b = kc.DeviceBuffer(ctx, 64)
+    v = a.view(0, 16)
+    p = kc.PinnedBuffer(ctx, 32)
+    a_addr, b_addr, pin_addr = a.ptr, b.ptr, p.ptr
+    b.free()  # freed by its owner first: not again at close
+    ctx.close()
+    frees = [c for c in calls if c[0] in ("free", "pinned_free")]
+    assert sorted(frees) == sorted([("free", b_addr), ("free", a_addr), ("pinned_free", pin_addr)])
+    assert calls[-1] == ("destroy",) and calls.count(("destroy",)) == 1
+    assert a.ptr == 0 and p.ptr == 0 and v.ptr == a_addr  # the view owns nothing
+    n = len(calls)
+    a.free()
+    p.free()
+    v.free()
+    ctx.close()
+    assert len(calls) == n
