@@ -439,8 +439,8 @@ def bench_serving(args):
 def main():
     args = parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU; --gpus %d is not supported: sharding over several GPUs "
-              "is a design, not an implementation, in this build" % args.gpus)
+        print("bench.py measures one GPU; --gpus %d is not supported: the library counts on one GPU "
+              "(sharding over several GPUs is designed, not implemented)" % args.gpus)
         sys.exit(2)
     if args.workload == "export_sort":
         return bench_sort(args)

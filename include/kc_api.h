@@ -30,7 +30,7 @@ extern "C" {
 #define KC_EFULL (-3)    /* table overflowed (a full FIRST-list segment spills into the table) */
 #define KC_EDEVICE (-4)  /* HIP runtime error */
 #define KC_EIO (-5)      /* file error */
-#define KC_ENOTSUP (-6)  /* configuration not supported by this build */
+#define KC_ENOTSUP (-6)  /* configuration the library does not support (k > 63) */
 
 typedef struct kc_ctx kc_ctx;
 typedef struct kc_table kc_table;
