@@ -337,7 +337,7 @@ __device__ __forceinline__ bool first_append(const TableView &tv, bool first, co
     const int leader = __ffsll((unsigned long long)firsts) - 1;
     const uint32_t seg = first_segment(blockIdx.x, threadIdx.x >> 6, rot);
     unsigned long long b0 = 0;
-    if (__lane_id() == leader)
+    if ((int)__lane_id() == leader)
         b0 = atomicAdd(tv.counters + KC_CURSOR_BASE + (uint64_t)seg * KC_LINE_WORDS, (unsigned long long)__popcll(firsts));
     b0 = __shfl(b0, leader, 64);
     if (!first) return true;

@@ -102,7 +102,7 @@ __device__ __forceinline__ uint64_t append(unsigned long long *cursor, bool want
     if (!m) return 0;
     const int leader = __ffsll((unsigned long long)m) - 1;
     unsigned long long b0 = 0;
-    if (__lane_id() == leader) b0 = atomicAdd(cursor, (unsigned long long)__popcll(m));
+    if ((int)__lane_id() == leader) b0 = atomicAdd(cursor, (unsigned long long)__popcll(m));
     b0 = __shfl(b0, leader, 64);
     return b0 + __popcll(m & ((1ULL << __lane_id()) - 1));
 }
@@ -112,7 +112,6 @@ template <int NW>
 __global__ __launch_bounds__(kThreads) void k_dump_low_list(TableView tv, DumpSpec ds, uint32_t *recs,
                                                             unsigned long long *cursor)
 {
-    constexpr int SW = slot_words<NW>();
     constexpr int RW = 2 * NW + 1;
     const uint32_t seg = blockIdx.y;
     const uint64_t n = low_seg_fill(tv, seg);
