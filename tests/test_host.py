@@ -178,3 +178,54 @@ def test_context_close_frees_its_buffers(monkeypatch):
     v.free()
     ctx.close()
     assert len(calls) == n
+
+
+def test_batch_add_chunks_binding(monkeypatch):
+    """Batch.add_chunks hands the buffer, the chunk ends as uint64 and their
+    count to kc_batch_add_chunks in one call, takes bytes or arrays, and
+    rejects ends past the buffer before calling.  Runs against a stand-in
+    for libkc that records the calls."""
+    import ctypes as C
+    from ktcountermemorygreedy_amd import kc
+    seen = []
+
+    class Fake:
+        def kc_ctx_create(self, dev, out):
+            out._obj.value = 0x10
+            return 0
+
+        def kc_table_create(self, ctx, cfg, out):
+            out._obj.value = 0x20
+            return 0
+
+        def kc_batch_create(self, t, tune, out):
+            out._obj.value = 0x30
+            return 0
+
+        def kc_batch_add_chunks(self, b, base, ends, n):
+            e = np.ctypeslib.as_array((C.c_uint64 * n).from_address(ends.value)) if n else np.zeros(0, np.uint64)
+            data = C.string_at(base.value, int(e[-1])) if n else b""
+            seen.append((b.value, data, e.tolist()))
+            return 0
+
+        def kc_batch_destroy(self, b):
+            pass
+
+        def kc_table_destroy(self, t):
+            pass
+
+        def kc_ctx_destroy(self, h):
+            pass
+
+    fake = Fake()
+    monkeypatch.setattr(kc, "lib", lambda: fake)
+    ctx = kc.Context(0)
+    t = kc.Table(ctx, 31, capacity=1 << 16)
+    b = kc.Batch(t)
+    b.add_chunks(b"ACGT\nAC\nGGT", [5, 8, 8, 11])
+    b.add_chunks(np.frombuffer(b"TTTT\n", dtype=np.uint8), np.array([2, 5], dtype=np.int64))
+    assert seen == [(0x30, b"ACGT\nAC\nGGT", [5, 8, 8, 11]), (0x30, b"TTTT\n", [2, 5])]
+    with pytest.raises(ValueError):
+        b.add_chunks(b"ACGT\n", [3, 9])
+    assert len(seen) == 2
+    ctx.close()
