@@ -228,7 +228,12 @@ int kc_table_get_tune(kc_table *t, kc_tune *out);
 int kc_autotune(kc_table *t, const uint8_t *d_sample, uint64_t nbytes, uint32_t reps, kc_tune *best,
                 float *best_ms);
 
-/* A batch packs framed read chunks ("seq\n" streams of any length) into
+/* Batches have no counterpart in the reference, whose reader threads cut
+ * the input into fixed 32 KiB ring blocks with a k-1 base overlap
+ * (FastqReader.h:542-557, definition.h:53-54) and hand them to parser
+ * threads one at a time.
+ *
+ * A batch packs framed read chunks ("seq\n" streams of any length) into
  * page-locked staging slots of tune->batch_bytes; each full slot (or
  * kc_batch_flush) is copied to HBM on a copy stream and counted into the
  * table by ONE launch of a captured HIP graph of the persistent pipeline,
