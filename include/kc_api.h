@@ -106,6 +106,9 @@ int kc_copy_d2h(kc_ctx *ctx, void *h, const void *d, uint64_t bytes);
 int kc_copy_d2d(kc_ctx *ctx, void *dst, const void *src, uint64_t bytes);
 int kc_event_record(kc_ctx *ctx, int slot);
 int kc_event_elapsed_ms(kc_ctx *ctx, int slot_start, int slot_end, float *ms);
+/* Free and total HBM of a device, as the HIP runtime reports them now
+ * (serving admits count jobs by it). */
+int kc_device_mem_info(int device, uint64_t *free_bytes, uint64_t *total_bytes);
 
 /* ---- counting table: replaces ConcurrentCountingHashMap<N>
  *      (ConcurrentCountingHashMap.h:211-312) and, on the production path,
@@ -135,10 +138,14 @@ int kc_table_count_seq_host(kc_table *t, const uint8_t *h_seq, uint64_t nbytes);
 int kc_table_increment(kc_table *t, const uint64_t *d_keys, uint64_t n,
                        const uint32_t *d_inc);
 /* Point lookup: merged count min(c, count_max) of each key, 0 if absent
- * (FlatConcurrentHashMap::find_prepared, FlatConcurrentHashMap.h:142-205).
- * With the Bloom split on, the first lookup folds the FIRST list into the
- * table, which then needs a slot for every distinct k-mer (KC_EFULL if it
- * has not; size it with max_distinct, not kc_plan_table's Bloom sizing). */
+ * (FlatConcurrentHashMap::find_prepared, FlatConcurrentHashMap.h:142-205,
+ * with the tools' merge of a low.bin key, LowFrequencyQueryThreadPool.h:
+ * 213-232).  Read-only: the table, its FIRST list and its counters are not
+ * changed, so counting may go on afterwards.  With the Bloom split on, a
+ * lookup after the counts changed first builds a hash index of the FIRST
+ * list (2 to 4 slots of 8N bytes per FIRST occurrence in device memory,
+ * kept with the table); n = 0 builds it without looking anything up.
+ * Asynchronous, except for that index build. */
 int kc_table_query(kc_table *t, const uint64_t *d_keys, uint64_t n,
                    uint32_t *d_counts);
 int kc_table_stats(kc_table *t, kc_stats *out);

@@ -153,9 +153,11 @@ int sync_pass(kc_batch *b)
     return KC_OK;
 }
 
+// Before a launch: the table's counts change (see kc_table_touch), and the
+// graphs must carry the table's current pass.
 int check_table(kc_batch *b)
 {
-    if (b->t->bloom && b->t->sealed) return KC_EINVAL;  // FIRST list already folded
+    kc_table_touch(b->t);
     return sync_pass(b);
 }
 
@@ -496,7 +498,6 @@ extern "C" int kc_autotune(kc_table *t, const uint8_t *d_sample, uint64_t nbytes
 {
     if (!t || !d_sample || !nbytes || !best) return KC_EINVAL;
     if (reinterpret_cast<uintptr_t>(d_sample) % 16) return KC_EINVAL;
-    if (t->bloom && t->sealed) return KC_EINVAL;
     KC_HIP(hipSetDevice(t->ctx->device));
     reps = std::max<uint32_t>(1, std::min<uint32_t>(reps, 8));
     const kc_tune saved = t->tune;

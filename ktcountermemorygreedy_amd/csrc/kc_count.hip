@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <new>
@@ -99,33 +100,83 @@ __global__ __launch_bounds__(kBlock) void k_increment(const uint64_t *__restrict
     stat_add(tv.counters, KC_STAT_ERRORS, n_err);
 }
 
-// increment(key, 1) for every key of the FIRST list (folds it into the table)
+// ---- read-only FIRST-list index ---------------------------------------
+// A point lookup of a Bloom-split table answers the table count plus one
+// when the key is on the FIRST list (the merge rule of
+// LowFrequencyQueryThreadPool.h:213-232 for one key).  The list is an array
+// of unique keys (one FIRST per distinct key), so an open-addressing set of
+// them at load <= 1/2 is built once per change of the counts and probed by
+// lookups: a lookup reads the table and the index and writes neither.
 template <int NW>
-__global__ __launch_bounds__(kBlock) void k_increment_low(TableView tv)
+__device__ __forceinline__ uint64_t idx_hash(const uint64_t *key)
 {
-    uint32_t n_new = 0, n_err = 0;
+    uint64_t x = key[0];
+    if constexpr (NW == 2) x ^= ((key[1] >> 29) | (key[1] << 35)) * 0x94D049BB133111EBULL;
+    x ^= x >> 31;
+    x *= 0xBF58476D1CE4E5B9ULL;
+    x ^= x >> 29;
+    x *= 0x94D049BB133111EBULL;
+    return x ^ (x >> 32);
+}
+
+// Inserts every FIRST-list key (block (chunk, segment) walks one segment).
+// idx[(imask + 1) * NW] counts keys that found no slot (never at load 1/2).
+template <int NW>
+__global__ __launch_bounds__(kBlock) void k_low_index_build(TableView tv, uint64_t *__restrict__ idx, uint64_t imask)
+{
+    uint32_t n_err = 0;
     const uint32_t seg = blockIdx.y;
     const uint64_t n = low_seg_fill(tv, seg);
     for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) {
-        uint64_t key[NW];
         const uint64_t *src = low_seg_key(tv, seg, i, NW);
+        uint64_t key[NW];
 #pragma unroll
         for (int w = 0; w < NW; ++w) key[w] = src[w];
-        table_add<NW>(tv, key, key_hash<NW>(tv, key), 1, n_new, n_err);
+        uint64_t s = idx_hash<NW>(key) & imask;
+        bool done = false;
+        for (uint64_t probe = 0; probe <= imask && !done; ++probe, s = (s + 1) & imask) {
+            uint64_t *slot = idx + s * NW;
+            // the list's keys are unique: an occupied slot holds another key
+            if (atomicCAS((unsigned long long *)slot, (unsigned long long)KC_EMPTY, (unsigned long long)key[0]) ==
+                KC_EMPTY) {
+                if constexpr (NW == 2) slot[1] = key[1];
+                done = true;
+            }
+        }
+        n_err += !done;
     }
-    stat_add(tv.counters, KC_STAT_DISTINCT, n_new);
-    stat_add(tv.counters, KC_STAT_ERRORS, n_err);
+    uint64_t e = n_err;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) e += __shfl_xor(e, o, 64);
+    if (__lane_id() == 0 && e) atomicAdd((unsigned long long *)(idx + (imask + 1) * NW), (unsigned long long)e);
 }
 
 template <int NW>
+__device__ __forceinline__ bool idx_contains(const uint64_t *__restrict__ idx, uint64_t imask, const uint64_t *key)
+{
+    uint64_t s = idx_hash<NW>(key) & imask;
+    for (uint64_t probe = 0; probe <= imask; ++probe, s = (s + 1) & imask) {
+        const uint64_t *slot = idx + s * NW;
+        const uint64_t w0 = slot[0];
+        if (w0 == KC_EMPTY) return false;
+        if (w0 == key[0] && (NW == 1 || slot[NW - 1] == key[NW - 1])) return true;
+    }
+    return false;
+}
+
+// Merged count of each key: table count (+1 when on the FIRST list, IDX),
+// clamped at count_max.  Reads only.
+template <int NW, bool IDX>
 __global__ __launch_bounds__(kBlock) void k_query(const uint64_t *__restrict__ keys, uint64_t n,
-                                                  uint32_t *__restrict__ out, TableView tv)
+                                                  uint32_t *__restrict__ out, TableView tv,
+                                                  const uint64_t *__restrict__ idx, uint64_t imask)
 {
     for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) {
         uint64_t key[NW];
 #pragma unroll
         for (int w = 0; w < NW; ++w) key[w] = keys[i * NW + w];
-        const uint64_t c = table_find<NW>(tv, key);
+        uint64_t c = table_find<NW>(tv, key);
+        if constexpr (IDX) c += idx_contains<NW>(idx, imask, key);
         out[i] = (uint32_t)(c < tv.count_max ? c : tv.count_max);
     }
 }
@@ -215,7 +266,7 @@ __device__ __forceinline__ int classify_slot(const TableView &tv, uint64_t i, bo
     if (singles_to_low && c == 1) return 2;
     if (c == 0) return 0;
     cc = (uint32_t)(c < tv.count_max ? c : tv.count_max);
-    return ((uint64_t)cc + 1 < tv.filter_min || cc > tv.filter_max) ? 0 : 1;
+    return export_filtered(cc, tv) ? 0 : 1;
 }
 
 // Block-wide exclusive scan of two per-thread counts (high, low).
@@ -421,8 +472,8 @@ __device__ __forceinline__ int64_t high_view(const TableView &tv, uint64_t c, bo
 {
     if (c == 0) return -1;
     if (singles_low && c == 1) return -1;
-    const uint64_t cc = c < tv.count_max ? c : tv.count_max;
-    if (cc + 1 < tv.filter_min || cc > tv.filter_max) return -1;
+    const uint32_t cc = (uint32_t)(c < tv.count_max ? c : tv.count_max);
+    if (export_filtered(cc, tv)) return -1;
     return (int64_t)cc;
 }
 
@@ -576,7 +627,7 @@ PackSpec pack_spec(const kc_table *t)
     return p;
 }
 
-bool low_list_active(const kc_table *t) { return t->bloom && !t->sealed; }
+bool low_list_active(const kc_table *t) { return t->bloom != nullptr; }
 
 struct HostCounters {
     uint64_t stat[KC_STAT_N];
@@ -586,15 +637,11 @@ struct HostCounters {
 
 // Reads the device counters, sums the statistic lanes and, when the FIRST
 // list is active, uploads its segment offsets for the kernels that read it.
+// The host buffers belong to the table (sized at creation): nothing is
+// allocated here.
 int read_counters(kc_table *t, HostCounters &hc)
 {
-    std::vector<uint64_t> h, off;
-    try {
-        h.resize(KC_COUNTER_WORDS);
-        off.resize(KC_LOW_SEGS + 1);
-    } catch (...) {
-        return KC_ENOMEM;
-    }
+    std::vector<uint64_t> &h = t->hcounters, &off = t->hseg_off;
     KC_HIP(hipMemcpyAsync(h.data(), t->counters, KC_COUNTER_WORDS * 8, hipMemcpyDeviceToHost, t->ctx->stream));
     KC_HIP(hipStreamSynchronize(t->ctx->stream));
     for (int st = 0; st < KC_STAT_N; ++st) {
@@ -727,6 +774,20 @@ extern "C" int kc_event_elapsed_ms(kc_ctx *ctx, int a, int b, float *ms)
     return KC_OK;
 }
 
+extern "C" int kc_device_mem_info(int device, uint64_t *free_bytes, uint64_t *total_bytes)
+{
+    if (!free_bytes || !total_bytes) return KC_EINVAL;
+    int ndev = 0;
+    KC_HIP(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return KC_EINVAL;
+    KC_HIP(hipSetDevice(device));
+    size_t f = 0, t = 0;
+    KC_HIP(hipMemGetInfo(&f, &t));
+    *free_bytes = f;
+    *total_bytes = t;
+    return KC_OK;
+}
+
 extern "C" void kc_ctx_destroy(kc_ctx *ctx)
 {
     if (!ctx) return;
@@ -758,7 +819,7 @@ static int table_init_device(kc_table *t)
     int rc;
     if (t->bloom && (rc = kc_launch_fill(c, t->bloom, 2 * t->bloom_blocks, 0)) != KC_OK) return rc;
     if ((rc = kc_launch_fill(c, t->counters, KC_COUNTER_WORDS, 0)) != KC_OK) return rc;
-    t->sealed = 0;
+    kc_table_touch(t);
     return KC_OK;
 }
 
@@ -779,6 +840,17 @@ extern "C" int kc_table_create(kc_ctx *ctx, const kc_table_config *cfg, kc_table
 
     kc_table *t = new (std::nothrow) kc_table();
     if (!t) return KC_ENOMEM;
+    try {
+        t->hcounters.resize(KC_COUNTER_WORDS);
+        t->hseg_off.resize(KC_LOW_SEGS + 1);
+    } catch (...) {
+        delete t;
+        return KC_ENOMEM;
+    }
+    t->epoch = 1;
+    t->low_idx_epoch = 0;
+    // test-only fault injection, see kc_internal.h
+    if (const char *f = std::getenv("KC_TEST_SKEW_COUNTS")) t->fault_skew = (uint32_t)std::atoi(f);
     t->ctx = ctx;
     t->cfg = *cfg;
     t->nw = cfg->k <= 32 ? 1 : 2;
@@ -845,7 +917,7 @@ extern "C" void kc_table_destroy(kc_table *t)
     (void)hipSetDevice(t->ctx->device);
     (void)hipStreamSynchronize(t->ctx->stream);
     for (void *p : {(void *)t->slots, (void *)t->bloom, (void *)t->low, (void *)t->counters, (void *)t->seg_off,
-                    (void *)t->tile_counts})
+                    (void *)t->tile_counts, (void *)t->low_idx})
         if (p) (void)hipFree(p);
     kc_part_free(t);
     for (int i = 0; i < 2; ++i) {
@@ -918,6 +990,7 @@ int kc_direct_enqueue(const kc_table *t, const SeqSrc &src, uint32_t grid, hipSt
 static int launch_count(kc_table *t, const uint8_t *d_seq, uint64_t nbytes, uint64_t emit_from)
 {
     if (!nbytes) return KC_OK;
+    kc_table_touch(t);
     const TableView v = view_of(t);
     if (use_partitioned(t, nbytes, kPartMinBytes)) return kc_part_count(t, d_seq, nbytes, emit_from, nullptr, 0, v);
     const uint64_t blocks = (nbytes + kTile - 1) / kTile;
@@ -929,14 +1002,12 @@ extern "C" int kc_table_count_seq(kc_table *t, const uint8_t *d_seq, uint64_t nb
 {
     if (!t || (!d_seq && nbytes)) return KC_EINVAL;
     if (reinterpret_cast<uintptr_t>(d_seq) % 16) return KC_EINVAL;
-    if (t->bloom && t->sealed) return KC_EINVAL;  // FIRST list already folded
     return launch_count(t, d_seq, nbytes, 0);
 }
 
 extern "C" int kc_table_count_seq_host(kc_table *t, const uint8_t *h_seq, uint64_t nbytes)
 {
     if (!t || (!h_seq && nbytes)) return KC_EINVAL;
-    if (t->bloom && t->sealed) return KC_EINVAL;
     if (!nbytes) return KC_OK;
     constexpr uint64_t kStage = 256ull << 20;
     constexpr uint64_t kHalo = 64;  // >= k-1, multiple of 16
@@ -947,9 +1018,10 @@ extern "C" int kc_table_count_seq_host(kc_table *t, const uint8_t *h_seq, uint64
     hipPointerAttribute_t attr{};
     const bool direct = hipPointerGetAttributes(&attr, h_seq) == hipSuccess && attr.type == hipMemoryTypeHost;
     (void)hipGetLastError();  // a pageable pointer leaves hipErrorInvalidValue behind
-    if (!t->dstage[0] || (!direct && !t->pinned[0])) {
-        // sized for this call (small jobs keep small buffers), at most kStage
-        const uint64_t want = std::min<uint64_t>(kStage, (nbytes + 15) / 16 * 16) + kHalo;
+    // sized for the largest call so far (small jobs keep small buffers), at
+    // most kStage; a larger call grows them
+    const uint64_t want = std::min<uint64_t>(kStage, (nbytes + 15) / 16 * 16) + kHalo;
+    if (!t->dstage[0] || (!direct && !t->pinned[0]) || want > t->stage_bytes) {
         if (t->dstage[0] && want > t->stage_bytes) {
             KC_HIP(hipStreamSynchronize(s));
             for (int i = 0; i < 2; ++i) {
@@ -997,8 +1069,11 @@ extern "C" int kc_table_increment(kc_table *t, const uint64_t *d_keys, uint64_t 
 {
     if (!t || (!d_keys && n)) return KC_EINVAL;
     if (!n) return KC_OK;
+    kc_table_touch(t);
     const TableView v = view_of(t);
-    if (!d_inc && !(t->bloom && !t->sealed) && use_partitioned(t, n, kPartMinKeys))
+    // increments add to the table directly (no Bloom split), as increment()
+    // of the reference's maps does
+    if (!d_inc && !t->bloom && use_partitioned(t, n, kPartMinKeys))
         return kc_part_count(t, nullptr, 0, 0, d_keys, n, v);
     const uint32_t g = grid_for(n, t->ctx->cu_count);
     if (t->nw == 1) hipLaunchKernelGGL(k_increment<1>, dim3(g), dim3(kBlock), 0, t->ctx->stream, d_keys, n, d_inc, v);
@@ -1007,43 +1082,77 @@ extern "C" int kc_table_increment(kc_table *t, const uint64_t *d_keys, uint64_t 
     return KC_OK;
 }
 
-// Folds the FIRST list into the table so that stored counts are the true
-// counts (needed before point queries of a Bloom-split table).
-static int seal_table(kc_table *t)
+// Brings the FIRST-list index up to date with the table's counts (nothing
+// to do when the counts have not changed since it was built).  *mask is the
+// index's slot mask, 0 when lookups need no index (no Bloom split, or an
+// empty list).  Writes only the index: the table, its list and its counters
+// are left as they are.
+static int ensure_low_index(kc_table *t, uint64_t *mask)
 {
+    *mask = 0;
     if (!low_list_active(t)) return KC_OK;
+    if (t->low_idx_epoch == t->epoch) {
+        *mask = t->low_idx_mask;
+        return KC_OK;
+    }
     HostCounters hc;
     int rc = read_counters(t, hc);
     if (rc != KC_OK) return rc;
     if (hc.stat[KC_STAT_ERRORS]) return KC_EFULL;
+    t->low_idx_mask = 0;
     if (hc.low) {
-        const TableView v = view_of(t);
+        const uint64_t cap = std::max<uint64_t>(1024, next_pow2(2 * hc.low));
+        const uint64_t words = cap * t->nw + 1;  // + the error word
+        if (cap > t->low_idx_cap) {
+            KC_HIP(hipStreamSynchronize(t->ctx->stream));
+            if (t->low_idx) KC_HIP(hipFree(t->low_idx));
+            t->low_idx = nullptr;
+            t->low_idx_cap = 0;
+            if (hipMalloc((void **)&t->low_idx, words * 8) != hipSuccess) {
+                t->low_idx = nullptr;
+                return KC_ENOMEM;
+            }
+            t->low_idx_cap = cap;
+        }
+        if ((rc = kc_launch_fill(t->ctx, t->low_idx, cap * t->nw, KC_EMPTY)) != KC_OK) return rc;
+        if ((rc = kc_launch_fill(t->ctx, t->low_idx + cap * t->nw, 1, 0)) != KC_OK) return rc;
         uint64_t most = 0;
         for (int sgi = 0; sgi < KC_LOW_SEGS; ++sgi) most = std::max(most, hc.seg_fill[sgi]);
+        const TableView v = view_of(t);
         const dim3 g(kc_low_chunks(most, 4 * kBlock), KC_LOW_SEGS);
-        if (t->nw == 1) hipLaunchKernelGGL(k_increment_low<1>, g, dim3(kBlock), 0, t->ctx->stream, v);
-        else hipLaunchKernelGGL(k_increment_low<2>, g, dim3(kBlock), 0, t->ctx->stream, v);
+        if (t->nw == 1) hipLaunchKernelGGL(k_low_index_build<1>, g, dim3(kBlock), 0, t->ctx->stream, v, t->low_idx, cap - 1);
+        else hipLaunchKernelGGL(k_low_index_build<2>, g, dim3(kBlock), 0, t->ctx->stream, v, t->low_idx, cap - 1);
         KC_HIP(hipGetLastError());
-        // the folded keys need table slots: a table sized for the keys seen
-        // twice only (Bloom split) may not hold them all
-        if ((rc = read_counters(t, hc)) != KC_OK) return rc;
-        t->sealed = 1;
-        if (hc.stat[KC_STAT_ERRORS]) return KC_EFULL;
+        uint64_t err = 0;
+        KC_HIP(hipMemcpyAsync(&err, t->low_idx + cap * t->nw, 8, hipMemcpyDeviceToHost, t->ctx->stream));
+        KC_HIP(hipStreamSynchronize(t->ctx->stream));
+        if (err) return KC_EDEVICE;  // a set at load 1/2 always has room: a broken list
+        t->low_idx_mask = cap - 1;
     }
-    t->sealed = 1;
+    t->low_idx_epoch = t->epoch;
+    *mask = t->low_idx_mask;
     return KC_OK;
 }
 
 extern "C" int kc_table_query(kc_table *t, const uint64_t *d_keys, uint64_t n, uint32_t *d_counts)
 {
     if (!t || (n && (!d_keys || !d_counts))) return KC_EINVAL;
-    int rc = seal_table(t);
+    KC_HIP(hipSetDevice(t->ctx->device));
+    uint64_t imask = 0;
+    int rc = ensure_low_index(t, &imask);
     if (rc != KC_OK) return rc;
     if (!n) return KC_OK;
     const TableView v = view_of(t);
     const uint32_t g = grid_for(n, t->ctx->cu_count);
-    if (t->nw == 1) hipLaunchKernelGGL(k_query<1>, dim3(g), dim3(kBlock), 0, t->ctx->stream, d_keys, n, d_counts, v);
-    else hipLaunchKernelGGL(k_query<2>, dim3(g), dim3(kBlock), 0, t->ctx->stream, d_keys, n, d_counts, v);
+    const uint64_t *idx = t->low_idx;
+    hipStream_t st = t->ctx->stream;
+    if (t->nw == 1) {
+        if (imask) hipLaunchKernelGGL((k_query<1, true>), dim3(g), dim3(kBlock), 0, st, d_keys, n, d_counts, v, idx, imask);
+        else hipLaunchKernelGGL((k_query<1, false>), dim3(g), dim3(kBlock), 0, st, d_keys, n, d_counts, v, idx, imask);
+    } else {
+        if (imask) hipLaunchKernelGGL((k_query<2, true>), dim3(g), dim3(kBlock), 0, st, d_keys, n, d_counts, v, idx, imask);
+        else hipLaunchKernelGGL((k_query<2, false>), dim3(g), dim3(kBlock), 0, st, d_keys, n, d_counts, v, idx, imask);
+    }
     KC_HIP(hipGetLastError());
     return KC_OK;
 }
@@ -1056,7 +1165,7 @@ extern "C" int kc_table_stats(kc_table *t, kc_stats *out)
     if (rc != KC_OK) return rc;
     out->kmers = hc.stat[KC_STAT_KMERS];
     out->distinct = hc.stat[KC_STAT_DISTINCT];
-    out->low = t->sealed ? 0 : hc.low;
+    out->low = hc.low;
     out->capacity = t->cap;
     out->errors = hc.stat[KC_STAT_ERRORS];
     return KC_OK;

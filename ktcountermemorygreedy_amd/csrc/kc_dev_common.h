@@ -39,6 +39,17 @@ struct TableView {
 
 template <int NW> constexpr int slot_words() { return NW == 1 ? 2 : 4; }
 
+// The counter's export filter (NewKmerTree.h:1303-1311, FinalDrainWriter.h:
+// 95-104): a record with count c is left out of high.bin when c + 1 <
+// filter_min or c > filter_max, with c + 1 computed in u32 as the reference
+// does, so a count saturated at 2^32 - 1 wraps to 0 and is left out whenever
+// filter_min >= 1.  (The tools' own range test is u64,
+// HighFrequencyInsertThreadPool.h:161.)
+__device__ __forceinline__ bool export_filtered(uint32_t c, const TableView &tv)
+{
+    return (uint32_t)(c + 1u) < tv.filter_min || c > tv.filter_max;
+}
+
 // A table of 2^L partitions splits L into a coarse digit (the top
 // min(L, 8) bits) and a fine digit (the other L - min(L, 8) bits); the two
 // scatter levels sort by them in turn.  The coarse digit comes from a cheap

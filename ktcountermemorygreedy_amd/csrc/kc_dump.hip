@@ -91,8 +91,8 @@ template <int NW>
 __device__ __forceinline__ int64_t slot_high(const TableView &tv, uint64_t c, bool singles_low)
 {
     if (c == 0 || (singles_low && c == 1)) return -1;
-    const uint64_t cc = c < tv.count_max ? c : tv.count_max;
-    if (cc + 1 < tv.filter_min || cc > tv.filter_max) return -1;
+    const uint32_t cc = (uint32_t)(c < tv.count_max ? c : tv.count_max);
+    if (export_filtered(cc, tv)) return -1;
     return (int64_t)cc;
 }
 

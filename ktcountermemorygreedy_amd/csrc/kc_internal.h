@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "../../include/kc_api.h"
 
 #define KC_HIP(expr)                                                      \
@@ -75,7 +77,7 @@ struct kc_table {
     kc_ctx *ctx;
     kc_table_config cfg;
     uint32_t nw;
-    uint32_t sealed;        // Bloom FIRST list folded into the table
+    uint64_t epoch;         // bumped by every call that changes the counts
     uint64_t cap;           // slots
     uint32_t part_bits;     // log2(partitions)
     uint32_t part_log2;     // log2(slots per partition)
@@ -100,7 +102,23 @@ struct kc_table {
     kc_tune tune;           // launch shape of counting calls (zeros: defaults)
     uint32_t pass_bits;     // multi-pass counting: log2(passes) (kc_table_set_pass)
     uint32_t pass;
+    // Read-only index of the FIRST list for point lookups: an open-addressing
+    // set of its keys (idx_cap slots of nw words, then one error word), built
+    // when a lookup finds it older than the table's counts (idx_epoch != epoch).
+    uint64_t *low_idx;
+    uint64_t low_idx_cap;   // allocated slots
+    uint64_t low_idx_mask;  // slots in use - 1 (0: no index, the list is empty)
+    uint64_t low_idx_epoch;
+    // host copy of the device counters (read_counters), sized at creation
+    std::vector<uint64_t> hcounters, hseg_off;
+    // test-only fault injection (KC_TEST_SKEW_COUNTS): skews one digit count
+    // of the partitioned pass so that the scatter guards must report it
+    uint32_t fault_skew;
 };
+
+// Every call that changes a table's counts marks the table (read-only
+// indexes built before are then rebuilt on their next use).
+inline void kc_table_touch(kc_table *t) { ++t->epoch; }
 
 // Slots / Bloom blocks of one partition that the LDS counting pass holds.
 constexpr uint32_t kc_lds_part_slots(uint32_t nw, bool bloom) { return (nw == 1 ? 4096u : 2048u) << (bloom ? 0 : 1); }

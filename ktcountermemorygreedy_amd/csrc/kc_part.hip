@@ -157,17 +157,30 @@ __global__ __launch_bounds__(kParseBlock) void k_front_count_seq(SeqSrc src, uin
     if (threadIdx.x < (1u << d.l1)) m1[(uint64_t)threadIdx.x * gridDim.x + blockIdx.x] = hist[threadIdx.x];
 }
 
+// Scatter guard.  A scatter writes each key at its digit region's offset
+// plus its rank, the region's size coming from a separate count pass.  Each
+// block keeps how many keys its region of every digit has left: a key past
+// the end is not written (it would land in the next region) and an error is
+// counted, and so is a region left with holes at the end.  Either makes the
+// counting call report KC_EFULL instead of returning wrong counts.
+__device__ __forceinline__ void guard_report(unsigned long long *counters, uint32_t n_err)
+{
+    stat_add(counters, KC_STAT_ERRORS, n_err);
+}
+
 // Front scatter with write combining: each thread loads its 64-byte
 // segment (and k-1 <= 4*HW bytes of history) straight into registers and
 // rolls it in chunks of CB bytes; the block's keys of one chunk (up to
 // TB x CB) are placed in LDS grouped by coarse digit and written out as
 // contiguous runs (TB x CB / 256 keys per digit per chunk on average).  A
 // tile is TB x 64 bytes, the tiles of k_front_count_seq at TB = 256, so the
-// per-block offsets of the count pass apply.
+// per-block offsets and counts of the count pass apply.
 template <int NW, int TB, int CB, bool PASS>
 __global__ __launch_bounds__(TB) void k_front_scatter_seq_staged(SeqSrc src, uint32_t k, Digits d,
                                                                  const uint64_t *__restrict__ o1,
-                                                                 uint64_t *__restrict__ out)
+                                                                 const uint32_t *__restrict__ m1,
+                                                                 uint64_t *__restrict__ out,
+                                                                 unsigned long long *counters)
 {
     constexpr int HW = NW == 1 ? 8 : 16;
     constexpr int kChunks = kParseSeg / CB;
@@ -175,15 +188,17 @@ __global__ __launch_bounds__(TB) void k_front_scatter_seq_staged(SeqSrc src, uin
     constexpr int kStage = TB * CB;
     __shared__ __align__(16) uint64_t stage[kStage * NW];
     __shared__ uint8_t sbin[kStage];
-    __shared__ uint32_t hist[256], loff[256];
+    __shared__ uint32_t hist[256], loff[256], left[256];
     __shared__ uint64_t base[256];
     uint32_t *wsum = loff + 256 - TB / 64;  // the scan's wave sums: loff is written only after the scan
     const uint32_t t = threadIdx.x;
     const uint32_t nbins = 1u << d.l1;
+    uint32_t n_err = 0;
     uint64_t n, emit_from;
     seq_bounds(src, n, emit_from);
     for (uint32_t f = t; f < 256; f += TB) {
         base[f] = f < nbins ? o1[(uint64_t)f * gridDim.x + blockIdx.x] : 0;
+        left[f] = f < nbins ? m1[(uint64_t)f * gridDim.x + blockIdx.x] : 0;
         hist[f] = 0;
     }
     __syncthreads();
@@ -254,7 +269,12 @@ __global__ __launch_bounds__(TB) void k_front_scatter_seq_staged(SeqSrc src, uin
             __syncthreads();
             for (uint32_t i = t; i < total; i += TB) {
                 const uint32_t c = sbin[i];
-                const uint64_t at = base[c] + (i - loff[c]);
+                const uint32_t r = i - loff[c];
+                if (r >= left[c]) {  // past the region the count pass gave this block
+                    ++n_err;
+                    continue;
+                }
+                const uint64_t at = base[c] + r;
                 if constexpr (NW == 1) {
                     out[at] = stage[i];
                 } else {
@@ -264,11 +284,14 @@ __global__ __launch_bounds__(TB) void k_front_scatter_seq_staged(SeqSrc src, uin
             __syncthreads();
             if (t < 256) {
                 base[t] += h;
+                left[t] = h <= left[t] ? left[t] - h : 0;
                 hist[t] = 0;
             }
             __syncthreads();
         }
     }
+    for (uint32_t f = t; f < nbins; f += TB) n_err += left[f] != 0;  // keys the count pass saw and this one did not
+    guard_report(counters, n_err);
 }
 
 template <int NW>
@@ -292,22 +315,53 @@ __global__ __launch_bounds__(kKeyBlock) void k_front_count_keys(const uint64_t *
 template <int NW>
 __global__ __launch_bounds__(kKeyBlock) void k_front_scatter_keys(const uint64_t *__restrict__ keys, uint64_t n,
                                                                   Digits d, const uint64_t *__restrict__ o1,
-                                                                  uint64_t *__restrict__ out)
+                                                                  const uint32_t *__restrict__ m1,
+                                                                  uint64_t *__restrict__ out,
+                                                                  unsigned long long *counters)
 {
-    __shared__ unsigned long long cur[256];
-    if (threadIdx.x < (1u << d.l1)) cur[threadIdx.x] = o1[(uint64_t)threadIdx.x * gridDim.x + blockIdx.x];
+    __shared__ unsigned long long cur[256], lend[256];
+    const uint32_t nbins = 1u << d.l1;
+    if (threadIdx.x < nbins) {
+        const uint64_t j = (uint64_t)threadIdx.x * gridDim.x + blockIdx.x;
+        cur[threadIdx.x] = o1[j];
+        lend[threadIdx.x] = o1[j] + m1[j];
+    }
     __syncthreads();
+    uint32_t n_err = 0;
     const uint64_t ntiles = (n + kKeyTile - 1) / kKeyTile;
     for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         for (int j = 0; j < kKeysPerThread; ++j) {
             const uint64_t i = tile * kKeyTile + (uint64_t)j * kKeyBlock + threadIdx.x;
             if (i < n && in_pass_of<NW>(keys + i * NW, d)) {
-                const uint64_t pos = atomicAdd(&cur[coarse_of<NW>(keys + i * NW, d)], 1ull);
+                const uint32_t c = coarse_of<NW>(keys + i * NW, d);
+                const uint64_t pos = atomicAdd(&cur[c], 1ull);
+                if (pos >= lend[c]) {  // past this block's region (scatter guard)
+                    ++n_err;
+                    continue;
+                }
 #pragma unroll
                 for (int w = 0; w < NW; ++w) out[pos * NW + w] = keys[i * NW + w];
             }
         }
     }
+    __syncthreads();
+    if (threadIdx.x < nbins) n_err += cur[threadIdx.x] < lend[threadIdx.x];
+    guard_report(counters, n_err);
+}
+
+// Test-only fault injection (KC_TEST_SKEW_COUNTS): adds delta (+1 or -1) to
+// the first nonzero count of a count matrix, so that the scatter that uses
+// it has one region too large or too small.  Every write stays inside the
+// key arrays: a total one larger is still below key_cap (a pass has fewer
+// windows than input bytes).
+__global__ void k_skew_count(uint32_t *m, uint64_t n, int delta)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    for (uint64_t i = 0; i < n; ++i)
+        if (m[i]) {
+            m[i] = (uint32_t)((int64_t)m[i] + delta);
+            return;
+        }
 }
 
 // Coarse region starts: o1[c * G] for every coarse digit, plus the total.
@@ -383,13 +437,15 @@ template <int NW, int NF, int TB, int FPT, bool SB = true>
 __global__ __launch_bounds__(TB) void k_fine_scatter_staged(const uint64_t *__restrict__ keys,
                                                             const uint64_t *__restrict__ starts, Digits d,
                                                             uint32_t S, const uint64_t *__restrict__ o2,
-                                                            uint64_t *__restrict__ out)
+                                                            const uint32_t *__restrict__ m2,
+                                                            uint64_t *__restrict__ out,
+                                                            unsigned long long *counters)
 {
     using BinT = typename std::conditional<NF <= 256, uint8_t, uint16_t>::type;
     constexpr uint32_t CH = (uint32_t)TB * FPT;
     __shared__ __align__(16) uint64_t stage[CH * NW];
     __shared__ BinT sbin[SB ? CH : 1];
-    __shared__ uint32_t hist[NF], loff[NF];
+    __shared__ uint32_t hist[NF], loff[NF], left[NF];
     __shared__ uint64_t base[NF];
     uint32_t *tmp = loff + NF - TB / 64;  // the scan's wave sums: loff is rewritten only after the scan
     uint32_t c, s;
@@ -398,8 +454,10 @@ __global__ __launch_bounds__(TB) void k_fine_scatter_staged(const uint64_t *__re
     const uint32_t nf = 1u << d.l2;
     const uint32_t t = threadIdx.x;
     const uint64_t row = (uint64_t)c << d.l2;
+    uint32_t n_err = 0;
     for (uint32_t f = t; f < nf; f += TB) {
         base[f] = o2[(row + f) * S + s];
+        left[f] = m2[(row + f) * S + s];
         hist[f] = 0;
     }
     __syncthreads();
@@ -462,7 +520,12 @@ __global__ __launch_bounds__(TB) void k_fine_scatter_staged(const uint64_t *__re
             uint32_t f;
             if constexpr (SB) f = sbin[p];
             else f = fine_of(xxh3_64<NW>(stage + p * NW), d);
-            const uint64_t g = base[f] + (p - loff[f]);
+            const uint32_t r = p - loff[f];
+            if (r >= left[f]) {  // past this slice's region (scatter guard)
+                ++n_err;
+                continue;
+            }
+            const uint64_t g = base[f] + r;
             if constexpr (NW == 2) {
                 *reinterpret_cast<ulonglong2 *>(out + g * 2) = *reinterpret_cast<const ulonglong2 *>(stage + p * 2);
             } else {
@@ -472,10 +535,13 @@ __global__ __launch_bounds__(TB) void k_fine_scatter_staged(const uint64_t *__re
         __syncthreads();
         for (uint32_t f = t; f < nf; f += TB) {
             base[f] += hist[f];
+            left[f] = hist[f] <= left[f] ? left[f] - hist[f] : 0;
             hist[f] = 0;
         }
         __syncthreads();
     }
+    for (uint32_t f = t; f < nf; f += TB) n_err += left[f] != 0;
+    guard_report(counters, n_err);
 }
 
 // Partition p starts at o2[p * S] (two levels) or at the coarse start p (one
@@ -732,7 +798,7 @@ bool dev_alloc(T *&p, uint64_t n)
 // Whether the LDS partition pass can hold this table's partitions.
 bool kc_part_eligible(const kc_table *t)
 {
-    const bool bloom = t->bloom && !t->sealed;
+    const bool bloom = t->bloom != nullptr;
     if ((1ull << t->part_log2) > kc_lds_part_slots(t->nw, bloom)) return false;
     if (bloom && (1ull << t->bloom_part_log2) > kc_lds_part_bloom(t->nw)) return false;
     return t->part_bits <= 18;
@@ -830,6 +896,8 @@ int kc_part_enqueue(const kc_table *t, kc_part_scratch &s, const kc_pipe_shape &
         else hipLaunchKernelGGL(k_front_count_keys<2>, dim3(G), dim3(kKeyBlock), 0, st, d_keys, n_keys, d, s.m1);
     }
     KC_HIP(hipGetLastError());
+    if (t->fault_skew == 1 || t->fault_skew == 2)
+        hipLaunchKernelGGL(k_skew_count, dim3(1), dim3(64), 0, st, s.m1, m1n, t->fault_skew == 1 ? -1 : 1);
     int rc;
     if ((rc = kc_scan_u32_on(st, s.m1, m1n, s.o1, s.sums))) return rc;
     hipLaunchKernelGGL(k_coarse_starts, dim3(1), dim3(256), 0, st, s.o1, s.m1, G, n_coarse, s.starts);
@@ -839,11 +907,11 @@ int kc_part_enqueue(const kc_table *t, kc_part_scratch &s, const kc_pipe_shape &
                        : &k_front_scatter_seq_staged<1, kFrontBlock1, kFrontChunk1, false>;
         auto f2 = pass ? &k_front_scatter_seq_staged<2, kFrontBlock2, kFrontChunk2, true>
                        : &k_front_scatter_seq_staged<2, kFrontBlock2, kFrontChunk2, false>;
-        if (nw == 1) hipLaunchKernelGGL(f1, dim3(G), dim3(kFrontBlock1), 0, st, seq, t->cfg.k, d, s.o1, s.keys_a);
-        else hipLaunchKernelGGL(f2, dim3(G), dim3(kFrontBlock2), 0, st, seq, t->cfg.k, d, s.o1, s.keys_a);
+        if (nw == 1) hipLaunchKernelGGL(f1, dim3(G), dim3(kFrontBlock1), 0, st, seq, t->cfg.k, d, s.o1, s.m1, s.keys_a, tv.counters);
+        else hipLaunchKernelGGL(f2, dim3(G), dim3(kFrontBlock2), 0, st, seq, t->cfg.k, d, s.o1, s.m1, s.keys_a, tv.counters);
     } else {
-        if (nw == 1) hipLaunchKernelGGL(k_front_scatter_keys<1>, dim3(G), dim3(kKeyBlock), 0, st, d_keys, n_keys, d, s.o1, s.keys_a);
-        else hipLaunchKernelGGL(k_front_scatter_keys<2>, dim3(G), dim3(kKeyBlock), 0, st, d_keys, n_keys, d, s.o1, s.keys_a);
+        if (nw == 1) hipLaunchKernelGGL(k_front_scatter_keys<1>, dim3(G), dim3(kKeyBlock), 0, st, d_keys, n_keys, d, s.o1, s.m1, s.keys_a, tv.counters);
+        else hipLaunchKernelGGL(k_front_scatter_keys<2>, dim3(G), dim3(kKeyBlock), 0, st, d_keys, n_keys, d, s.o1, s.m1, s.keys_a, tv.counters);
     }
     KC_HIP(hipGetLastError());
 
@@ -854,13 +922,16 @@ int kc_part_enqueue(const kc_table *t, kc_part_scratch &s, const kc_pipe_shape &
         if (nw == 1) hipLaunchKernelGGL(k_fine_count<1>, g2, dim3(kKeyBlock), 0, st, s.keys_a, s.starts, d, S, s.m2);
         else hipLaunchKernelGGL(k_fine_count<2>, g2, dim3(kKeyBlock), 0, st, s.keys_a, s.starts, d, S, s.m2);
         KC_HIP(hipGetLastError());
+        if (t->fault_skew == 3 || t->fault_skew == 4)
+            hipLaunchKernelGGL(k_skew_count, dim3(1), dim3(64), 0, st, s.m2, m2n, t->fault_skew == 3 ? -1 : 1);
         if ((rc = kc_scan_u32_on(st, s.m2, m2n, s.o2, s.sums))) return rc;
+        unsigned long long *ctr = tv.counters;
         if (d.l2 <= 8) {
-            if (nw == 1) hipLaunchKernelGGL((k_fine_scatter_staged<1, 256, 1024, 16>), g2, dim3(1024), 0, st, s.keys_a, s.starts, d, S, s.o2, s.keys_b);
-            else hipLaunchKernelGGL((k_fine_scatter_staged<2, 256, 1024, 8>), g2, dim3(1024), 0, st, s.keys_a, s.starts, d, S, s.o2, s.keys_b);
+            if (nw == 1) hipLaunchKernelGGL((k_fine_scatter_staged<1, 256, 1024, 16>), g2, dim3(1024), 0, st, s.keys_a, s.starts, d, S, s.o2, s.m2, s.keys_b, ctr);
+            else hipLaunchKernelGGL((k_fine_scatter_staged<2, 256, 1024, 8>), g2, dim3(1024), 0, st, s.keys_a, s.starts, d, S, s.o2, s.m2, s.keys_b, ctr);
         } else {
-            if (nw == 1) hipLaunchKernelGGL((k_fine_scatter_staged<1, 1024, 1024, 16, false>), g2, dim3(1024), 0, st, s.keys_a, s.starts, d, S, s.o2, s.keys_b);
-            else hipLaunchKernelGGL((k_fine_scatter_staged<2, 1024, 1024, 8, false>), g2, dim3(1024), 0, st, s.keys_a, s.starts, d, S, s.o2, s.keys_b);
+            if (nw == 1) hipLaunchKernelGGL((k_fine_scatter_staged<1, 1024, 1024, 16, false>), g2, dim3(1024), 0, st, s.keys_a, s.starts, d, S, s.o2, s.m2, s.keys_b, ctr);
+            else hipLaunchKernelGGL((k_fine_scatter_staged<2, 1024, 1024, 8, false>), g2, dim3(1024), 0, st, s.keys_a, s.starts, d, S, s.o2, s.m2, s.keys_b, ctr);
         }
         KC_HIP(hipGetLastError());
         part_keys = s.keys_b;
@@ -870,7 +941,7 @@ int kc_part_enqueue(const kc_table *t, kc_part_scratch &s, const kc_pipe_shape &
     KC_HIP(hipGetLastError());
 
     // per-partition LDS counting
-    const bool bloom = t->bloom && !t->sealed;
+    const bool bloom = t->bloom != nullptr;
     const dim3 g((uint32_t)n_parts);
     uint64_t *first_out = part_keys == s.keys_b ? s.keys_a : s.keys_b;  // the idle key array
     if (nw == 1) {

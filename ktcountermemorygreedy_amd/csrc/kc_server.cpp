@@ -7,11 +7,19 @@
 //   work of different tables overlaps on the device; requests on one table
 //   are serialized by the table's mutex.
 // * Worker threads take requests from one FIFO.  A count job is admitted
-//   only when its device footprint fits the HBM budget (what is left after
-//   resident tables and running jobs); a job that does not fit waits while
-//   later requests that fit (queries, smaller jobs) go ahead.  When nothing
-//   runs and a job still does not fit, least recently used tables are
-//   dropped until it does.
+//   only when its device footprint fits the HBM measured free at that moment
+//   (kc_device_mem_info) next to the running jobs, and the optional budget
+//   next to the resident tables (kc_admit.h); a job that does not fit waits
+//   while later requests that fit (queries, smaller jobs) go ahead.  When
+//   nothing runs and a job still does not fit, least recently used tables are
+//   dropped until it does; a job that cannot fit even then fails with
+//   KC_ENOMEM instead of waiting forever.
+// * Lookups are batched continuously: a worker that takes a query batch of a
+//   table also takes every other query batch of that table waiting in the
+//   queue (up to kMergeKeys keys) and answers them with one copy in, one
+//   lookup launch and one copy out, through page-locked staging.
+// * A count job with the Bloom split builds its table's FIRST-list index
+//   before it completes, so lookups never pay for it.
 // * Latency = completion time - submit time, per request.
 #include <algorithm>
 #include <atomic>
@@ -19,7 +27,6 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
-#include <list>
 #include <memory>
 #include <mutex>
 #include <new>
@@ -29,14 +36,18 @@
 
 #include "../../include/kc_api.h"
 #include "../../include/kc_server.h"
+#include "kc_admit.h"
 #include "kc_guard.h"
 
 namespace {
 
 using Clock = std::chrono::steady_clock;
 
+// Most keys one merged lookup launch answers.
+constexpr uint64_t kMergeKeys = 4ull << 20;
+
 struct Table {
-    uint64_t id;
+    uint64_t id = 0;
     int nw = 1;
     kc_ctx *ctx = nullptr;
     kc_table *t = nullptr;
@@ -44,26 +55,37 @@ struct Table {
     std::mutex mu;
     std::atomic<int> busy{0};
     Clock::time_point last_use;
-    void *qkeys = nullptr, *qcounts = nullptr;  // query staging, grown on demand
+    // lookup staging, grown on demand: device keys / counts and their
+    // page-locked host images
+    void *dkeys = nullptr, *dcounts = nullptr, *hkeys = nullptr, *hcounts = nullptr;
     uint64_t qcap = 0;
+    void free_staging()
+    {
+        if (dkeys) kc_dev_free(ctx, dkeys);
+        if (dcounts) kc_dev_free(ctx, dcounts);
+        if (hkeys) kc_host_free_pinned(ctx, hkeys);
+        if (hcounts) kc_host_free_pinned(ctx, hcounts);
+        dkeys = dcounts = hkeys = hcounts = nullptr;
+        qcap = 0;
+    }
     ~Table()
     {
-        if (qkeys) kc_dev_free(ctx, qkeys);
-        if (qcounts) kc_dev_free(ctx, qcounts);
+        if (ctx) free_staging();
         if (t) kc_table_destroy(t);
         if (ctx) kc_ctx_destroy(ctx);
     }
 };
 
 struct Request {
-    uint64_t id;
-    int kind;  // 0 count, 1 query
-    uint64_t table_id;
+    uint64_t id = 0;
+    int kind = 0;  // 0 count, 1 query
+    uint64_t table_id = 0;
     // count
     uint32_t k = 31, bloom = 0;
     const uint8_t *seq = nullptr;
-    uint64_t max_distinct = 0, footprint = 0, resident = 0;
-    uint64_t nbytes = 0;
+    uint64_t nbytes = 0, max_distinct = 0, capacity = 0, resident = 0;
+    kc_ticket ticket;
+    int refused = KC_OK;  // set when the request is taken from the queue without running
     // query
     const uint64_t *keys = nullptr;
     uint64_t n = 0;
@@ -80,11 +102,19 @@ uint64_t next_pow2(uint64_t v)
     return p;
 }
 
+// What a worker takes from the queue: one count job, or the query batches
+// of one table.
+struct Work {
+    std::vector<Request *> reqs;
+    std::shared_ptr<Table> tab;  // queries: the table (marked busy)
+};
+
 }  // namespace
 
 struct kc_server {
-    int device;
-    uint64_t budget;
+    int device = 0;
+    uint64_t total_hbm = 0;
+    kc_admission adm;
     std::vector<std::thread> workers;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
@@ -92,96 +122,122 @@ struct kc_server {
     std::unordered_map<uint64_t, std::unique_ptr<Request>> reqs;
     std::unordered_map<uint64_t, std::shared_ptr<Table>> tables;
     std::unordered_map<uint64_t, int> pending_counts;  // queued or running count jobs per table id
-    uint64_t used = 0;      // bytes of resident tables + reservations
-    uint64_t running = 0;   // requests being executed
+    uint64_t running = 0;  // requests being executed
     uint64_t next_id = 1;
     bool stop = false;
 
-    // Device bytes of a count job while it runs (footprint) and of its
-    // table afterwards (resident): slots (16/32 B at the sizing below),
-    // Bloom filter and FIRST list when enabled; while counting also the
-    // device staging and the partitioned pass's two key arrays.
-    static uint64_t footprint(const Request &r, uint64_t &cap, uint64_t &resident)
+    // Device bytes of a count job while it runs (footprint) and of its table
+    // afterwards (resident): slots (16/32 B), Bloom filter, FIRST list and
+    // its lookup index when the split is on; while counting also the device
+    // staging and the partitioned pass's two key arrays.
+    static uint64_t footprint(Request &r)
     {
         const uint64_t nw = r.k <= 32 ? 1 : 2;
         const uint64_t want = std::max<uint64_t>(r.max_distinct, 1024);
-        cap = std::max<uint64_t>(1024, next_pow2(want + want / 2));
-        resident = cap * 16 * nw + (1 << 20);
-        if (r.bloom) resident += 16 * (cap / 2) + 8 * nw * (want + want / 16);
+        r.capacity = std::max<uint64_t>(1024, next_pow2(want + want / 2));
+        r.resident = r.capacity * 16 * nw + (1 << 20);
+        if (r.bloom) {
+            const uint64_t low = want + want / 8 + 4096 * 1024;
+            r.resident += 16 * (r.capacity / 2) + 8 * nw * low + 8 * nw * next_pow2(2 * want);
+        }
         const uint64_t chunk = std::min<uint64_t>(r.nbytes + 64, 256ull << 20);
-        return resident + 2 * chunk + 20 * nw * chunk;
+        return r.resident + 2 * chunk + 20 * nw * chunk;
     }
 
-    // Picks the first request that can run now (queries of tables that are
-    // not busy, count jobs whose footprint fits).  Called with mu held.
-    Request *pick()
+    uint64_t free_hbm() const
     {
+        uint64_t f = 0, t = 0;
+        return kc_device_mem_info(device, &f, &t) == KC_OK ? f : 0;
+    }
+
+    // Takes the first request that can run now; mu held.
+    bool pick(Work &w)
+    {
+        bool measured = false;
+        uint64_t free_now = 0;
         for (auto it = queue.begin(); it != queue.end(); ++it) {
             Request *r = *it;
             if (r->kind == 1) {
-                // a query waits for the count jobs of its table submitted before it
+                // a query waits for the count jobs of its table
                 auto pc = pending_counts.find(r->table_id);
                 if (pc != pending_counts.end() && pc->second > 0) continue;
                 auto t = tables.find(r->table_id);
                 if (t != tables.end() && t->second->busy.load()) continue;
-                queue.erase(it);
-                return r;
+                w.tab = t != tables.end() ? t->second : nullptr;
+                // this batch and every later one of the same table, up to kMergeKeys keys
+                const uint64_t id = r->table_id;
+                uint64_t keys = 0;
+                for (auto j = it; j != queue.end();) {
+                    Request *q = *j;
+                    if (q->kind == 1 && q->table_id == id && (w.reqs.empty() || keys + q->n <= kMergeKeys)) {
+                        keys += q->n;
+                        w.reqs.push_back(q);
+                        j = queue.erase(j);
+                    } else {
+                        ++j;
+                    }
+                }
+                return true;
             }
-            if (tables.count(r->table_id)) {  // id in use: fails without a reservation
+            if (tables.count(r->table_id)) {  // id in use: refused without a reservation
+                r->refused = KC_EINVAL;
+                w.reqs.push_back(r);
                 queue.erase(it);
-                return r;
+                return true;
             }
-            if (used + r->footprint <= budget) {
-                used += r->footprint;
-                queue.erase(it);
-                return r;
+            if (!measured) {
+                free_now = free_hbm();
+                measured = true;
             }
-            if (running == 0 && evict_for(r->footprint)) {
-                used += r->footprint;
+            if (!adm.admit(r->ticket, free_now) && running == 0) {
+                // nothing runs: drop idle tables, least recently used first
+                while (!adm.admit(r->ticket, free_now) && evict_one()) free_now = free_hbm();
+                if (!r->ticket.reserved) r->refused = KC_ENOMEM;  // cannot fit even with every table dropped
+            }
+            if (r->ticket.reserved || r->refused) {
+                w.reqs.push_back(r);
                 queue.erase(it);
-                return r;
+                return true;
             }
         }
-        return nullptr;
+        return false;
     }
 
-    // LRU eviction of idle tables until need bytes fit.  mu held.
-    bool evict_for(uint64_t need)
+    // Drops the least recently used idle table; mu held.
+    bool evict_one()
     {
-        while (used + need > budget) {
-            std::shared_ptr<Table> lru;
-            for (auto &kv : tables)
-                if (!kv.second->busy.load() && (!lru || kv.second->last_use < lru->last_use)) lru = kv.second;
-            if (!lru) return false;
-            used -= lru->bytes;
-            tables.erase(lru->id);
-        }
-        return true;
+        std::shared_ptr<Table> lru;
+        for (auto &kv : tables)
+            if (!kv.second->busy.load() && (!lru || kv.second->last_use < lru->last_use)) lru = kv.second;
+        if (!lru) return false;
+        adm.drop(lru->bytes);
+        tables.erase(lru->id);
+        return true;  // freed when the last holder lets go (a worker holding it has it busy)
     }
 
     void run_count(Request *r)
     {
-        uint64_t cap = 0, resident = 0;
-        footprint(*r, cap, resident);
         std::shared_ptr<Table> tab;
+        int rc = KC_OK;
         try {
             tab = std::make_shared<Table>();
         } catch (...) {
-            std::lock_guard<std::mutex> g(mu);
-            r->res.status = KC_ENOMEM;
-            used -= r->footprint;
-            return;
+            rc = KC_ENOMEM;
         }
-        tab->id = r->table_id;
-        tab->nw = r->k <= 32 ? 1 : 2;
-        int rc = kc_ctx_create(device, &tab->ctx);
+        kc_stats st{};
+        uint64_t table_bytes = 0;
+        if (rc == KC_OK) {
+            tab->id = r->table_id;
+            tab->nw = r->k <= 32 ? 1 : 2;
+            rc = kc_ctx_create(device, &tab->ctx);
+        }
         if (rc == KC_OK) {
             kc_table_config cfg{};
             cfg.k = r->k;
             cfg.count_max = 0xFFFFFFFFu;
             cfg.filter_min = 0;
             cfg.filter_max = 0xFFFFFFFFu;
-            cfg.capacity = cap;
+            cfg.capacity = r->capacity;
             cfg.max_distinct = std::max<uint64_t>(r->max_distinct, 1024);
             cfg.bloom = r->bloom;
             rc = kc_table_create(tab->ctx, &cfg, &tab->t);
@@ -189,16 +245,16 @@ struct kc_server {
         if (rc == KC_OK) rc = kc_table_count_seq_host(tab->t, r->seq, r->nbytes);
         if (rc == KC_OK) rc = kc_ctx_sync(tab->ctx);
         if (rc == KC_OK) rc = kc_table_trim(tab->t);
-        kc_stats st{};
         if (rc == KC_OK) rc = kc_table_stats(tab->t, &st);
         if (rc == KC_OK && st.errors) rc = KC_EFULL;
+        if (rc == KC_OK && r->bloom) rc = kc_table_query(tab->t, nullptr, 0, nullptr);  // builds the FIRST-list index
+        if (rc == KC_OK) table_bytes = r->resident;
         r->res.kmers = st.kmers;
         r->res.distinct = st.distinct;
-        r->res.status = rc;
         std::lock_guard<std::mutex> g(mu);
-        if (rc == KC_OK && tables.count(tab->id)) rc = KC_EINVAL;  // id taken meanwhile
+        if (rc == KC_OK && tables.count(r->table_id)) rc = KC_EINVAL;  // id taken meanwhile
         if (rc == KC_OK) {
-            tab->bytes = resident;
+            tab->bytes = table_bytes;
             tab->last_use = Clock::now();
             try {
                 tables.emplace(tab->id, tab);
@@ -206,88 +262,88 @@ struct kc_server {
                 rc = KC_ENOMEM;
             }
         }
+        adm.finish(r->ticket, rc == KC_OK ? table_bytes : 0);
         r->res.status = rc;
-        if (rc == KC_OK) used -= r->footprint - resident;
-        else used -= r->footprint;
     }
 
-    void run_query(Request *r)
+    // One copy in, one lookup launch and one copy out for every batch.
+    void run_queries(const std::shared_ptr<Table> &tab, const std::vector<Request *> &batch)
     {
-        std::shared_ptr<Table> tab;
-        {
-            std::lock_guard<std::mutex> g(mu);
-            auto it = tables.find(r->table_id);
-            if (it != tables.end()) tab = it->second;
-        }
         if (!tab) {
-            r->res.status = KC_EINVAL;
+            for (Request *r : batch) r->res.status = KC_EINVAL;
             return;
         }
         std::lock_guard<std::mutex> tg(tab->mu);
-        const int nw = tab->nw;
+        const uint64_t nw = tab->nw;
+        uint64_t total = 0;
+        for (Request *r : batch) total += r->n;
         int rc = KC_OK;
-        if (r->n > tab->qcap) {
-            kc_dev_free(tab->ctx, tab->qkeys);
-            kc_dev_free(tab->ctx, tab->qcounts);
-            tab->qkeys = tab->qcounts = nullptr;
-            tab->qcap = std::max<uint64_t>(r->n, 65536);
-            rc = kc_dev_alloc(tab->ctx, tab->qcap * 8 * nw, &tab->qkeys);
-            if (rc == KC_OK) rc = kc_dev_alloc(tab->ctx, tab->qcap * 4, &tab->qcounts);
-            if (rc != KC_OK) tab->qcap = 0;
+        if (total > tab->qcap) {
+            tab->free_staging();
+            const uint64_t cap = std::max<uint64_t>(next_pow2(total), 65536);
+            rc = kc_dev_alloc(tab->ctx, cap * 8 * nw, &tab->dkeys);
+            if (rc == KC_OK) rc = kc_dev_alloc(tab->ctx, cap * 4, &tab->dcounts);
+            if (rc == KC_OK) rc = kc_host_alloc_pinned(tab->ctx, cap * 8 * nw, &tab->hkeys);
+            if (rc == KC_OK) rc = kc_host_alloc_pinned(tab->ctx, cap * 4, &tab->hcounts);
+            if (rc == KC_OK) tab->qcap = cap;
+            else tab->free_staging();
         }
-        if (rc == KC_OK) rc = kc_copy_h2d(tab->ctx, tab->qkeys, r->keys, r->n * 8 * nw);
-        if (rc == KC_OK)
-            rc = kc_table_query(tab->t, static_cast<const uint64_t *>(tab->qkeys), r->n,
-                                static_cast<uint32_t *>(tab->qcounts));
-        if (rc == KC_OK) rc = kc_copy_d2h(tab->ctx, r->counts, tab->qcounts, r->n * 4);
-        r->res.status = rc;
+        if (rc == KC_OK && total) {
+            uint64_t at = 0;
+            for (Request *r : batch) {
+                if (r->n) std::memcpy(static_cast<uint64_t *>(tab->hkeys) + at * nw, r->keys, r->n * 8 * nw);
+                at += r->n;
+            }
+            rc = kc_copy_h2d(tab->ctx, tab->dkeys, tab->hkeys, total * 8 * nw);
+            if (rc == KC_OK)
+                rc = kc_table_query(tab->t, static_cast<const uint64_t *>(tab->dkeys), total,
+                                    static_cast<uint32_t *>(tab->dcounts));
+            if (rc == KC_OK) rc = kc_copy_d2h(tab->ctx, tab->hcounts, tab->dcounts, total * 4);
+            if (rc == KC_OK) {
+                at = 0;
+                for (Request *r : batch) {
+                    if (r->n) std::memcpy(r->counts, static_cast<const uint32_t *>(tab->hcounts) + at, r->n * 4);
+                    at += r->n;
+                }
+            }
+        }
+        for (Request *r : batch) r->res.status = rc;
         tab->last_use = Clock::now();
     }
 
     void worker()
     {
         for (;;) {
-            Request *r = nullptr;
-            std::shared_ptr<Table> busy_tab;
+            Work w;
             {
                 std::unique_lock<std::mutex> lk(mu);
-                cv_work.wait(lk, [&] { return stop || (r = pick()) != nullptr; });
-                if (!r) return;
+                cv_work.wait(lk, [&] { return stop || pick(w); });
+                if (w.reqs.empty()) return;
                 ++running;
-                if (r->kind == 1) {
-                    auto it = tables.find(r->table_id);
-                    if (it != tables.end()) {
-                        busy_tab = it->second;
-                        busy_tab->busy.fetch_add(1);
-                    }
-                }
+                if (w.tab) w.tab->busy.fetch_add(1);
             }
-            r->start = Clock::now();
-            if (r->kind == 0) {
-                if (tables_has(r->table_id)) r->res.status = KC_EINVAL;  // picked without a reservation
-                else run_count(r);
-            } else {
-                run_query(r);
-            }
-            r->done = Clock::now();
-            r->res.latency_ms = std::chrono::duration<double, std::milli>(r->done - r->submit).count();
-            r->res.queue_ms = std::chrono::duration<double, std::milli>(r->start - r->submit).count();
+            const Clock::time_point start = Clock::now();
+            for (Request *r : w.reqs) r->start = start;
+            Request *first = w.reqs[0];
+            if (first->kind == 1) run_queries(w.tab, w.reqs);
+            else if (first->refused) first->res.status = first->refused;  // holds no reservation
+            else run_count(first);
+            const Clock::time_point done = Clock::now();
             {
                 std::lock_guard<std::mutex> g(mu);
-                if (busy_tab) busy_tab->busy.fetch_sub(1);
-                if (r->kind == 0 && --pending_counts[r->table_id] == 0) pending_counts.erase(r->table_id);
+                if (w.tab) w.tab->busy.fetch_sub(1);
+                for (Request *r : w.reqs) {
+                    r->done = done;
+                    r->res.latency_ms = std::chrono::duration<double, std::milli>(r->done - r->submit).count();
+                    r->res.queue_ms = std::chrono::duration<double, std::milli>(r->start - r->submit).count();
+                    if (r->kind == 0 && --pending_counts[r->table_id] == 0) pending_counts.erase(r->table_id);
+                    r->finished = true;
+                }
                 --running;
-                r->finished = true;
             }
             cv_done.notify_all();
             cv_work.notify_all();
         }
-    }
-
-    bool tables_has(uint64_t id)
-    {
-        std::lock_guard<std::mutex> g(mu);
-        return tables.count(id) != 0;
     }
 };
 
@@ -295,14 +351,15 @@ extern "C" int kc_server_create(int device, uint32_t workers, uint64_t hbm_budge
 {
     return kc_guard([&]() -> int {
         if (!out || workers == 0 || workers > 64) return KC_EINVAL;
-        kc_ctx *probe = nullptr;
-        int rc = kc_ctx_create(device, &probe);  // validates the device
+        uint64_t free_b = 0, total_b = 0;
+        int rc = kc_device_mem_info(device, &free_b, &total_b);  // validates the device
         if (rc != KC_OK) return rc;
-        kc_ctx_destroy(probe);
         auto *s = new (std::nothrow) kc_server();
         if (!s) return KC_ENOMEM;
         s->device = device;
-        s->budget = hbm_budget_bytes;
+        s->total_hbm = total_b;
+        s->adm.budget = hbm_budget_bytes;
+        s->adm.margin = std::min<uint64_t>(2ull << 30, total_b / 16);
         try {
             s->workers.reserve(workers);
             for (uint32_t i = 0; i < workers; ++i) s->workers.emplace_back([s] { s->worker(); });
@@ -369,9 +426,8 @@ extern "C" int kc_server_submit_count(kc_server *s, uint64_t table_id, uint32_t 
         r->seq = h_seq;
         r->nbytes = nbytes;
         r->max_distinct = max_distinct ? max_distinct : std::max<uint64_t>(nbytes, 1024);
-        uint64_t cap;
-        r->footprint = kc_server::footprint(*r, cap, r->resident);
-        if (r->footprint > s->budget) return KC_ENOMEM;
+        r->ticket.footprint = kc_server::footprint(*r);
+        if (!s->adm.could_fit(r->ticket.footprint, s->total_hbm)) return KC_ENOMEM;
         *req_id = enqueue(s, std::move(r));
         return KC_OK;
     });
@@ -413,7 +469,7 @@ extern "C" int kc_server_drop_table(kc_server *s, uint64_t table_id)
     auto it = s->tables.find(table_id);
     if (it == s->tables.end()) return KC_EINVAL;
     if (it->second->busy.load()) return KC_EINVAL;
-    s->used -= it->second->bytes;
+    s->adm.drop(it->second->bytes);
     s->tables.erase(it);
     s->cv_work.notify_all();
     return KC_OK;
@@ -439,6 +495,6 @@ extern "C" int kc_server_resident(kc_server *s, uint64_t *tables, uint64_t *byte
     if (!s || !tables || !bytes) return KC_EINVAL;
     std::lock_guard<std::mutex> g(s->mu);
     *tables = s->tables.size();
-    *bytes = s->used;
+    *bytes = s->adm.resident + s->adm.reserved;
     return KC_OK;
 }

@@ -107,6 +107,7 @@ EXPORTS = {
     "kc_copy_d2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
     "kc_event_record": (C.c_int, [C.c_void_p, C.c_int]),
     "kc_event_elapsed_ms": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_float)]),
+    "kc_device_mem_info": (C.c_int, [C.c_int, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "kc_table_create": (C.c_int, [C.c_void_p, C.POINTER(TableConfig), C.POINTER(C.c_void_p)]),
     "kc_table_destroy": (None, [C.c_void_p]),
     "kc_table_clear": (C.c_int, [C.c_void_p]),
@@ -572,6 +573,13 @@ class Batch:
 
 
 # ---- host helpers ----------------------------------------------------------
+
+def device_mem_info(device=0):
+    """(free, total) HBM bytes of a device as the HIP runtime reports them."""
+    f, t = C.c_uint64(), C.c_uint64()
+    check(lib().kc_device_mem_info(device, C.byref(f), C.byref(t)), "kc_device_mem_info")
+    return f.value, t.value
+
 
 def fastq_frame(fq: bytes) -> bytes:
     src = np.frombuffer(fq, dtype=np.uint8)

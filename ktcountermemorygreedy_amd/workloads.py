@@ -11,6 +11,11 @@ C++ routine in csrc/kc_host.cpp (SplitMix64 streams, one per read).
   uneven_50mbp    50 Mbp genome (seed 3); 80% 150 bp reads (0.5% error) and
                   20% long reads (lognormal mean 8 kb, sd 3 kb, 5% error) by
                   read count, 40x coverage; k = 21, 31, 63
+  repeat_4mbp     repeat_100mbp's generator on a 4 Mbp genome (same seeds,
+                  500 x 1 kb segments, 30% copies, 1% divergence), 300k reads
+                  (11x): the repeat-rich parity case
+  repeat_4mbp_hot the same with 20 segments, so each is copied about 60 times
+                  as in repeat_100mbp (hot keys counted in the hundreds)
 """
 from __future__ import annotations
 
@@ -68,3 +73,7 @@ REPEAT_100MBP = ReadSet("repeat_100mbp", 2, 100_000_000, 12, 20_000_000, n_segme
                         segment_len=1000, repeat_fraction=0.30, segment_divergence=0.01)
 UNEVEN_50MBP = ReadSet("uneven_50mbp", 3, 50_000_000, 13, 1_163_000, long_fraction=0.2,
                        long_mean=8000.0, long_sd=3000.0, long_sub_rate=0.05)
+REPEAT_4MBP = ReadSet("repeat_4mbp", 2, 4_000_000, 12, 300_000, n_segments=500, segment_len=1000,
+                      repeat_fraction=0.30, segment_divergence=0.01)
+REPEAT_4MBP_HOT = ReadSet("repeat_4mbp_hot", 2, 4_000_000, 12, 300_000, n_segments=20, segment_len=1000,
+                          repeat_fraction=0.30, segment_divergence=0.01)

@@ -21,6 +21,9 @@
  *    masked key computed independently in the tests.
  *  - hashes and Bloom probe: oracle/_ref/hash_kat_ref (the vendored headers
  *    and BloomFilter.h compiled into a harness) on 8/16/32-byte keys.
+ *  - kco_extract_slice: kco_extract's windows filtered by the top bits of
+ *    kco_xxh3_64 computed per key outside the oracle (tests/test_oracle.py);
+ *    both are pinned above.
  */
 #include "kc_oracle.h"
 
@@ -85,8 +88,11 @@ static int cmp_words(const uint64_t *a, const uint64_t *b, uint32_t nw)
     return 0;
 }
 
-uint64_t kco_extract(const uint8_t *bytes, uint64_t len, uint32_t k,
-                     uint64_t *out, uint64_t out_cap)
+/* Rolling extraction of every window; with pass_bits > 0 only the windows
+ * whose key is in hash slice `pass` are kept (the engine's multi-pass
+ * filter: the top pass_bits of XXH3_64 of the 8N key bytes). */
+static uint64_t extract_impl(const uint8_t *bytes, uint64_t len, uint32_t k, uint32_t pass_bits,
+                             uint32_t pass, uint64_t *out, uint64_t out_cap)
 {
     const uint32_t nw = kco_words_for_k(k);
     const uint32_t last = (k - 1) / 32;
@@ -110,14 +116,25 @@ uint64_t kco_extract(const uint8_t *bytes, uint64_t len, uint32_t k,
         rev[nw - 1] <<= 2;
         rev[last] |= (uint64_t)(c ^ 3) << rev_shift;
         if (++have >= k) {
-            if (n < out_cap) {
-                const uint64_t *m = cmp_words(fwd, rev, nw) < 0 ? fwd : rev;
-                memcpy(out + n * nw, m, nw * sizeof(uint64_t));
-            }
+            const uint64_t *m = cmp_words(fwd, rev, nw) < 0 ? fwd : rev;
+            if (pass_bits && (uint32_t)(kco_xxh3_64(m, 8ULL * nw) >> (64 - pass_bits)) != pass) continue;
+            if (n < out_cap) memcpy(out + n * nw, m, nw * sizeof(uint64_t));
             ++n;
         }
     }
     return n;
+}
+
+uint64_t kco_extract(const uint8_t *bytes, uint64_t len, uint32_t k,
+                     uint64_t *out, uint64_t out_cap)
+{
+    return extract_impl(bytes, len, k, 0, 0, out, out_cap);
+}
+
+uint64_t kco_extract_slice(const uint8_t *bytes, uint64_t len, uint32_t k, uint32_t pass_bits,
+                           uint32_t pass, uint64_t *out, uint64_t out_cap)
+{
+    return extract_impl(bytes, len, k, pass_bits, pass, out, out_cap);
 }
 
 /* Closed form for one window: position j sits at bit 64nw-2k+2j of the
@@ -767,18 +784,4 @@ void kco_bloom_probe(const uint64_t *key, uint32_t nw, uint64_t capacity,
     for (uint64_t i = 0; i < 3; ++i) mask |= 1ULL << ((lo + i * h2) & 63);
     *block_out = lo & (capacity - 1);
     *mask_out = mask;
-}
-
-int kco_bloom_insert(uint64_t *bins, const uint64_t *key, uint32_t nw, uint64_t capacity)
-{
-    uint64_t blk, mask;
-    kco_bloom_probe(key, nw, capacity, &blk, &mask);
-    uint64_t *cell = bins + 2 * blk;
-    if ((cell[0] & mask) == mask) return 2;
-    if ((cell[1] & mask) == mask) {
-        cell[0] |= mask;
-        return 1;
-    }
-    cell[1] |= mask;
-    return 0;
 }

@@ -29,6 +29,10 @@ uint64_t kco_fastq_frame(const uint8_t *fq, uint64_t len, uint8_t *out);
 /* Rolling canonical extraction over a framed byte stream. */
 uint64_t kco_extract(const uint8_t *bytes, uint64_t len, uint32_t k,
                      uint64_t *out, uint64_t out_cap);
+/* The windows of kco_extract whose key is in hash slice `pass` of
+ * 2^pass_bits (top pass_bits of kco_xxh3_64 of the key bytes). */
+uint64_t kco_extract_slice(const uint8_t *bytes, uint64_t len, uint32_t k, uint32_t pass_bits,
+                           uint32_t pass, uint64_t *out, uint64_t out_cap);
 /* Closed-form encoder of one window (independent of the rolling code). */
 int kco_encode_window(const uint8_t *s, uint32_t k, uint64_t *canon_out);
 
@@ -73,11 +77,9 @@ uint64_t kco_mix_hash(uint64_t h1, uint64_t h2);
 uint64_t kco_hash_func(const uint64_t *words, uint32_t nw);
 uint64_t kco_concurrent_map_hash(const uint64_t *words, uint32_t nw);
 
-/* Bloom singleton split (sequential). */
+/* Bloom probe (block, 3-bit mask) of BloomFilter.h:215-222, 287-297. */
 void kco_bloom_probe(const uint64_t *key, uint32_t nw, uint64_t capacity,
                      uint64_t *block_out, uint64_t *mask_out);
-int kco_bloom_insert(uint64_t *bins, const uint64_t *key, uint32_t nw,
-                     uint64_t capacity);
 
 #ifdef __cplusplus
 }

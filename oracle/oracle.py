@@ -60,7 +60,7 @@ def _declare(L):
         "kco_hash_func": (u64, [P, u32]),
         "kco_concurrent_map_hash": (u64, [P, u32]),
         "kco_bloom_probe": (None, [P, u32, u64, P, P]),
-        "kco_bloom_insert": (i32, [P, P, u32, u64]),
+        "kco_extract_slice": (u64, [P, u64, u32, u32, u32, P, u64]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -91,6 +91,41 @@ def extract(seq: bytes, k: int) -> np.ndarray:
     out = np.empty((cap, nw), dtype=np.uint64)
     n = lib().kco_extract(_p(src), len(seq), k, _p(out), cap)
     return out[:n].copy()
+
+
+def extract_slice(seq, k: int, pass_bits: int, pass_id: int, threads: int = 1) -> np.ndarray:
+    """Canonical k-mers of hash slice pass_id of 2^pass_bits (the engine's
+    multi-pass filter), in stream order.  seq: bytes or a uint8 array; with
+    threads > 1 the input is cut at newlines and the parts run in parallel
+    (ctypes releases the GIL)."""
+    nw = words_for_k(k)
+    src = np.frombuffer(seq, dtype=np.uint8) if isinstance(seq, (bytes, bytearray)) else \
+        np.ascontiguousarray(seq, dtype=np.uint8)
+    cuts = [0]
+    for i in range(1, max(1, threads)):
+        c = int(src.size * i // threads)
+        nl = np.flatnonzero(src[c:c + (1 << 20)] == 10)
+        if nl.size:
+            cuts.append(max(cuts[-1], c + int(nl[0]) + 1))
+    cuts.append(src.size)
+
+    def run(a, b):
+        part = src[a:b]
+        cap = max(1, part.size // (1 << pass_bits) // 16 + 4096)
+        while True:
+            out = np.empty((cap, nw), dtype=np.uint64)
+            n = lib().kco_extract_slice(C.c_void_p(part.ctypes.data), part.size, k, pass_bits, pass_id,
+                                        _p(out), cap)
+            if n <= cap:
+                return out[:n].copy()
+            cap = int(n)
+
+    if len(cuts) == 2:
+        return run(0, src.size)
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(len(cuts) - 1) as ex:
+        parts = list(ex.map(lambda ab: run(*ab), zip(cuts[:-1], cuts[1:])))
+    return np.concatenate(parts) if parts else np.zeros((0, nw), np.uint64)
 
 
 def encode_window(s: bytes, k: int):
@@ -271,11 +306,6 @@ def bloom_probe(words, capacity):
     b, m = C.c_uint64(), C.c_uint64()
     lib().kco_bloom_probe(_p(a), a.size, capacity, C.byref(b), C.byref(m))
     return b.value, m.value
-
-
-def bloom_insert(bins, words, capacity):
-    a, _ = _kb(words)
-    return lib().kco_bloom_insert(_p(bins), _p(a), a.size, capacity)
 
 
 def count_fastq_views(fq: bytes, k: int, count_max: int, hist_min=1, hist_max=10000):

@@ -8,6 +8,13 @@ Outputs (all data, no reference source):
   tiny_<name>.fastq / .k<k>.c<count_max>.dump.txt / .hist.txt
   count_5mbp.json          sha256 of the reference dump/histogram of the
                            seeded 5 Mbp workload plus its k-mer counts
+  repeat_4mbp.json         the same for the repeat-rich workloads
+                           repeat_4mbp and repeat_4mbp_hot
+  tiny_counts.*            keys at counts 1..8 and 50: Tree run with non-default
+                           filter_min / filter_max (argv 7-8), and dump_tool /
+                           histogram_tool min/max ranges over two exports of it
+                           (the plain split, and every key in low.bin with
+                           count - 1 in high.bin), listed in tiny_counts.json
   hash_kat.txt             hashes and Bloom probes of fixed keys
   sort_kat_<nw>_<bits>.npz records before/after the reference sorts
 """
@@ -133,7 +140,20 @@ def write_tiny():
 
 def write_count_5mbp():
     from ktcountermemorygreedy_amd import workloads as W
-    fq = W.COUNT_5MBP.reads(fastq=True)
+    rec = workload_golden(W.COUNT_5MBP)
+    with open(os.path.join(HERE, "count_5mbp.json"), "w") as f:
+        json.dump(rec, f, indent=1)
+
+
+def write_repeat():
+    from ktcountermemorygreedy_amd import workloads as W
+    recs = {w.name: workload_golden(w) for w in (W.REPEAT_4MBP, W.REPEAT_4MBP_HOT)}
+    with open(os.path.join(HERE, "repeat_4mbp.json"), "w") as f:
+        json.dump(recs, f, indent=1)
+
+
+def workload_golden(w):
+    fq = w.reads(fastq=True)
     with tempfile.NamedTemporaryFile(suffix=".fastq", delete=False) as f:
         f.write(fq.tobytes())
         path = f.name
@@ -143,14 +163,101 @@ def write_count_5mbp():
         os.unlink(path)
     kmers = [int(l.split(":")[1]) for l in log.splitlines() if l.startswith("Total read k-mer count")][0]
     rec = {
-        "workload": "count_5mbp", "k": 31, "count_max": 4294967295,
+        "workload": w.name, "k": 31, "count_max": 4294967295,
         "fastq_sha256": hashlib.sha256(fq.tobytes()).hexdigest(),
         "kmers": kmers, "distinct": dump.count(b"\n"),
         "dump_sha256": hashlib.sha256(dump).hexdigest(), "dump_bytes": len(dump),
         "hist_sha256": hashlib.sha256(hist).hexdigest(),
         "hist_head": hist.decode().splitlines()[:40],
     }
-    with open(os.path.join(HERE, "count_5mbp.json"), "w") as f:
+    return rec
+
+
+def tiny_counts_fastq():
+    """258 reads: three random 40-base reads at each count 1..8 and 50 (ten
+    31-mers each), shuffled."""
+    rng = np.random.default_rng(77)
+
+    def rs(n):
+        return "".join("ACGT"[i] for i in rng.integers(0, 4, n))
+
+    recs = []
+    for r in list(range(1, 9)) + [50]:
+        for _ in range(3):
+            s = rs(40)
+            recs += [s] * r
+    rng.shuffle(recs)
+    return "".join("@c%d\n%s\n+\n%s\n" % (i, s, "I" * len(s)) for i, s in enumerate(recs))
+
+
+FILTERS = [(3, 6), (5, 50)]
+DUMP_RANGES = [(1, 1), (2, 6), (3, 50), (7, 4294967295)]
+HIST_RANGES = [(1, 8), (2, 6), (5, 60)]
+
+
+def _tools_on(d, threads=6):
+    """dump --sort and histogram outputs of an export dir over the ranges."""
+    out = {"dump": {}, "hist": {}}
+    for lo, hi in DUMP_RANGES:
+        subprocess.run([os.path.join(REF, "dump_tool_ref"), "precise", d + "/", str(threads), "4", d + "/o.txt",
+                        str(lo), str(hi), "--sort"], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, check=True)
+        with open(d + "/o.txt") as f:
+            out["dump"]["%d-%d" % (lo, hi)] = f.read()
+    for lo, hi in HIST_RANGES:
+        subprocess.run([os.path.join(REF, "histogram_tool_ref"), "precise", d + "/", str(threads), "4",
+                        d + "/o.txt", str(lo), str(hi)], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, check=True)
+        with open(d + "/o.txt") as f:
+            out["hist"]["%d-%d" % (lo, hi)] = f.read()
+    return out
+
+
+def write_tiny_counts():
+    from oracle import oracle as O
+    text = tiny_counts_fastq()
+    fq_path = os.path.join(HERE, "tiny_counts.fastq")
+    with open(fq_path, "w") as f:
+        f.write(text)
+    rec = {"input": "tiny_counts.fastq", "k": 31, "count_max": 4294967295, "filters": {}, "splits": {}}
+    # the counter's own filter (Tree argv 7-8); merged views over the full range
+    for fmin, fmax in FILTERS:
+        tmp = tempfile.mkdtemp(prefix="kcgold_")
+        try:
+            subprocess.run([os.path.join(REF, "kmc_ref"), fq_path, "31", "6", "4", tmp + "/", "1024", str(fmin),
+                            str(fmax), "4294967295"], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, check=True)
+            subprocess.run([os.path.join(REF, "dump_tool_ref"), "precise", tmp + "/", "6", "4", tmp + "/d.txt", "1",
+                            "4294967295", "--sort"], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, check=True)
+            subprocess.run([os.path.join(REF, "histogram_tool_ref"), "precise", tmp + "/", "6", "4", tmp + "/h.txt",
+                            "1", "60"], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, check=True)
+            with open(tmp + "/d.txt") as f:
+                dump = f.read()
+            with open(tmp + "/h.txt") as f:
+                hist = f.read()
+        finally:
+            shutil.rmtree(tmp, ignore_errors=True)
+        rec["filters"]["%d-%d" % (fmin, fmax)] = {"dump": dump, "hist": hist}
+    # two exports of the same counts, and the tools' range rules over each
+    seq = O.fastq_frame(text.encode())
+    uniq, cnt = O.count(O.extract(seq, 31), 1)
+    infos, high, low = O.split_export(uniq, cnt, 31, 0xFFFFFFFF)
+    rep = cnt >= 2
+    _, high_f, _ = O.split_export(uniq[rep], cnt[rep] - 1, 31, 0xFFFFFFFF)
+    high_f = b"".join(high_f[i:i + 12] for i in range(0, len(high_f), 12) if
+                      int.from_bytes(high_f[i + 8:i + 12], "little") >= 1)
+    low_f = O.pack(uniq, 31)
+    for name, h, l in (("plain", high, low), ("first", high_f, low_f)):
+        with open(os.path.join(HERE, "tiny_counts.%s.high.bin" % name), "wb") as f:
+            f.write(h)
+        with open(os.path.join(HERE, "tiny_counts.%s.low.bin" % name), "wb") as f:
+            f.write(l)
+        tmp = tempfile.mkdtemp(prefix="kcgold_")
+        try:
+            for fn, b in (("infos.bin", infos), ("high.bin", h), ("low.bin", l)):
+                with open(os.path.join(tmp, fn), "wb") as f:
+                    f.write(b)
+            rec["splits"][name] = _tools_on(tmp)
+        finally:
+            shutil.rmtree(tmp, ignore_errors=True)
+    with open(os.path.join(HERE, "tiny_counts.json"), "w") as f:
         json.dump(rec, f, indent=1)
 
 
@@ -208,4 +315,6 @@ if __name__ == "__main__":
     write_sort_kat()
     write_tiny()
     write_count_5mbp()
+    write_repeat()
+    write_tiny_counts()
     print("goldens written to", HERE)

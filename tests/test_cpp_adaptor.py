@@ -26,6 +26,19 @@ def test_guard_maps_exceptions_to_status(tmp_path):
     assert out.returncode == 0 and out.stdout.strip() == "ok", out.stdout
 
 
+def test_admission_ledger(tmp_path):
+    """The serving loop's HBM admission (kc_admit.h): reservations are
+    returned exactly once, a request refused before it ran releases nothing,
+    and resident bytes return to their start after the tables are dropped
+    (host only)."""
+    src = os.path.join(ROOT, "tests", "cpp", "admit_check.cpp")
+    exe = str(tmp_path / "admit_check")
+    subprocess.check_call(["g++", "-std=c++17", "-Wall", "-Werror", "-O1", "-I", os.path.join(PKG, "csrc"), src,
+                           "-o", exe])
+    out = subprocess.run([exe], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=60)
+    assert out.returncode == 0 and out.stdout.strip() == "ok", out.stdout
+
+
 @pytest.mark.gpu
 def test_adaptor_runs(tmp_path):
     exe = str(tmp_path / "adaptor_check")

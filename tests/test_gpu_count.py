@@ -214,3 +214,27 @@ def test_first_list_spill_exact(ctx, k, path):
     assert np.array_equal(got, cnt)
     t.close()
     d.free()
+
+
+@pytest.mark.parametrize("filter_min", [0, 1, 2])
+def test_export_filter_wraps_at_u32_max(ctx, filter_min):
+    """The counter's export filter tests count + 1 < filter_min in u32
+    (NewKmerTree.h:1305, FinalDrainWriter.h:101): a count saturated at
+    2^32 - 1 wraps to 0, so its record is left out of high.bin whenever
+    filter_min >= 1 (the reference default is 2) and kept at 0.  Lookups
+    are not filtered."""
+    from ktcountermemorygreedy_amd import kc
+    keys = np.array([11, 22, 33, 44], dtype=np.uint64) << np.uint64(2)
+    t = kc.Table(ctx, 31, count_max=0xFFFFFFFF, capacity=1 << 12, filter_min=filter_min)
+    t.increment(keys, np.array([0xFFFFFFFF, 0xFFFFFFFE, 7, 0xFFFFFFFF], np.uint32))
+    t.increment(keys[3:], np.array([9], np.uint32))  # 2^32 + 8: saturates at 2^32 - 1
+    high, low = t.export_device()
+    hk, hc = O.parse_high(high, 31, 0xFFFFFFFF)
+    got = dict(zip(hk[:, 0].tolist(), hc.tolist()))
+    want = {22 << 2: 0xFFFFFFFE, 33 << 2: 7}
+    if filter_min == 0:
+        want.update({11 << 2: 0xFFFFFFFF, 44 << 2: 0xFFFFFFFF})
+    assert got == want and low == b""
+    assert t.dump().count(b"\n") == len(want)
+    assert t.query(keys).tolist() == [0xFFFFFFFF, 0xFFFFFFFE, 7, 0xFFFFFFFF]
+    t.close()

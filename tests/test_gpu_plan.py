@@ -48,15 +48,22 @@ def test_plan_estimates_and_table(ctx, k, bloom):
     high, low = t.export_device()
     dump, _ = views_from_export(high, low, k, 0xFFFFFFFF, 1000)
     assert dump == O.dump_text(uniq, cnt, k)
-    if bloom:
-        # the planned table holds the keys seen twice; point lookups fold
-        # every FIRST into it and need room for all distinct keys
-        with pytest.raises(kc.KCError):
-            t.query(uniq[:16])
-    else:
-        got = t.query(uniq).astype(np.uint64)
-        assert np.array_equal(got, cnt)
+    # lookups are read-only: the planned Bloom table holds only the keys
+    # seen twice and still answers every key, and counts on afterwards
+    st = t.stats()
+    got = t.query(uniq).astype(np.uint64)
+    assert np.array_equal(got, cnt)
+    assert t.stats() == st
+    extra = seq[:seq.index(b"\n", 20_000) + 1]
+    de = ctx.upload(np.frombuffer(extra, dtype=np.uint8))
+    t.count_seq(de, len(extra))
+    ctx.sync()
+    assert t.stats()["errors"] == 0
+    uniq2, cnt2 = O.count(np.concatenate([keys, O.extract(extra, k)]), nw)
+    assert np.array_equal(uniq2, uniq)
+    assert np.array_equal(t.query(uniq).astype(np.uint64), cnt2)
     t.close()
+    de.free()
     d.free()
 
 

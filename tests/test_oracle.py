@@ -35,6 +35,21 @@ def test_count_5mbp_golden():
     assert hashlib.sha256(hist).hexdigest() == g["hist_sha256"]
 
 
+@pytest.mark.parametrize("name", ["repeat_4mbp", "repeat_4mbp_hot"])
+def test_repeat_golden(name):
+    """The repeat-rich workloads (the headline's generator at 4 Mbp): the
+    oracle's merged views equal the reference binaries' dump and histogram."""
+    from ktcountermemorygreedy_amd import workloads as W
+    g = load_json("repeat_4mbp.json")[name]
+    w = {"repeat_4mbp": W.REPEAT_4MBP, "repeat_4mbp_hot": W.REPEAT_4MBP_HOT}[name]
+    fq = w.reads(fastq=True).tobytes()
+    assert hashlib.sha256(fq).hexdigest() == g["fastq_sha256"]
+    dump, hist, nk = O.count_fastq_views(fq, 31, g["count_max"])
+    assert nk == g["kmers"]
+    assert hashlib.sha256(dump).hexdigest() == g["dump_sha256"]
+    assert hashlib.sha256(hist).hexdigest() == g["hist_sha256"]
+
+
 def _hash_kat():
     rows = []
     for line in read_golden("hash_kat.txt", "r").splitlines():
@@ -170,3 +185,42 @@ def test_canonical_is_lexicographic_min():
 def test_fastq_framing():
     fq = b"@a\nACGT\n+\nIIII\n@b\nNNAC\n+\nIIII\n@c\nGG"
     assert O.fastq_frame(fq) == b"ACGT\nNNAC\nGG"
+
+
+def test_extract_slice_is_extract_filtered_by_xxh3():
+    """kco_extract_slice (the oracle of one multi-pass hash slice) equals the
+    full extraction filtered by the top bits of XXH3_64 of each key, computed
+    here key by key, on one thread and on input cut at newlines."""
+    from ktcountermemorygreedy_amd import kc
+    g = kc.synth_genome(5, 60_000)
+    seq = kc.synth_reads(g, 6, 1500, 150, True, 0.01, 1e-3).tobytes()
+    for k in (21, 31, 45):
+        keys = O.extract(seq, k)
+        h = np.array([O.xxh3_64(x) for x in keys], dtype=np.uint64)
+        for bits, p in [(1, 1), (3, 5), (6, 0), (6, 63)]:
+            want = keys[(h >> np.uint64(64 - bits)) == p]
+            for threads in (1, 3):
+                assert np.array_equal(O.extract_slice(seq, k, bits, p, threads=threads), want), (k, bits, p)
+        assert np.array_equal(O.extract_slice(seq, k, 0, 0), keys)
+
+
+@pytest.mark.parametrize("split", ["plain", "first"])
+def test_tool_ranges_golden(split):
+    """dump_tool / histogram_tool min-max ranges (dump_tool.cpp:717-718,
+    884-892; HighFrequencyInsertThreadPool.h:161-170,
+    LowFrequencyQueryThreadPool.h:213-232) over two exports of the same
+    counts: the oracle's merge reproduces the reference tools' output bytes,
+    including the split-dependent results of narrow dump ranges."""
+    g = load_json("tiny_counts.json")
+    high = read_golden("tiny_counts.%s.high.bin" % split)
+    low = read_golden("tiny_counts.%s.low.bin" % split)
+    hk, hc = O.parse_high(high, 31, g["count_max"])
+    lk = O.unpack(low, 31)
+    for r, want in g["splits"][split]["dump"].items():
+        lo, hi = map(int, r.split("-"))
+        mk, mc = O.merge(hk, hc, lk, 1, g["count_max"], lo, hi)
+        assert O.dump_text(mk, mc, 31).decode() == want, r
+    for r, want in g["splits"][split]["hist"].items():
+        lo, hi = map(int, r.split("-"))
+        bins = O.histogram(hk, hc, lk, 1, g["count_max"], lo, hi)
+        assert O.histogram_text(bins, lo, hi).decode() == want, r
