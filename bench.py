@@ -62,11 +62,44 @@ def parse_args():
     return ap.parse_args()
 
 
+# Host threads the CPU baselines use: nproc (the CPUs this process may run
+# on), at most 16 -- the host CPU share of one GPU on the GPU boxes this runs
+# on -- and at least 6, the reference counter's minimum (main.cpp:556-560).
+def host_threads():
+    try:
+        nproc = len(os.sched_getaffinity(0))
+    except AttributeError:
+        nproc = os.cpu_count() or 6
+    return max(6, min(16, nproc))
+
+
+def host_info():
+    """nproc, the CPU model and the RAM of this host (recorded beside every
+    CPU baseline)."""
+    info = {"nproc": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None,
+            "cpu_count": os.cpu_count()}
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    info["cpu_model"] = line.split(":", 1)[1].strip()
+                    break
+        with open("/proc/meminfo") as f:
+            for line in f:
+                if line.startswith("MemTotal"):
+                    info["ram_gb"] = round(int(line.split()[1]) / 1e6, 1)
+                    break
+    except OSError:
+        pass
+    info["threads_used"] = host_threads()
+    return info
+
+
 def cpu_baseline(ws, sample_reads, k, genome):
     """Reference Tree binary on the first sample_reads reads (FASTQ)."""
     from ktcountermemorygreedy_amd import kc  # noqa: F401
     ref = os.path.join(ROOT, "oracle", "_ref", "kmc_ref")
-    threads = max(6, min(16, os.cpu_count() or 6))
+    threads = host_threads()
     sample = ws.scaled(sample_reads)
     fq = sample.reads(fastq=True, genome=genome)
     if os.path.exists(ref):
@@ -84,6 +117,7 @@ def cpu_baseline(ws, sample_reads, k, genome):
         kmers = int(re.search(r"Total read k-mer count: (\d+)", out).group(1))
         us = int(re.search(r"Total elapsed us: (\d+)", out).group(1))
         return {"value": kmers / (us * 1e-6), "unit": "k-mers/s", "cores": threads, "kind": "reference",
+                "host": host_info(),
                 "sample": "first %d reads of the workload as FASTQ (%d k-mers), Tree <fastq> %d %d 16 <tmp> "
                           "1024 2 4294967295 4294967295; elapsed %.2f s (wall %.2f s)"
                           % (sample_reads, kmers, k, threads, us * 1e-6, wall)}
@@ -344,7 +378,7 @@ def bench_tools(args):
     if not args.no_cpu_baseline and os.path.exists(d_ref) and os.path.exists(h_ref):
         tmp = tempfile.mkdtemp(prefix="kctools_")
         table.export_dir(tmp)
-        threads = "16"
+        threads = str(host_threads())
         t0 = time.time()
         subprocess.run([d_ref, "precise", tmp + "/", threads, "16", tmp + "/dump.txt", "1", "4294967295", "--sort"],
                        stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, check=True, timeout=900)
@@ -357,7 +391,8 @@ def bench_tools(args):
         with open(tmp + "/hist.txt", "rb") as f:
             same_hist = f.read() == hist
         shutil.rmtree(tmp, ignore_errors=True)
-        cpu = {"value": lines / (t2 - t0), "unit": "dump lines/s (dump + histogram)", "cores": 16,
+        cpu = {"value": lines / (t2 - t0), "unit": "dump lines/s (dump + histogram)", "cores": host_threads(),
+               "host": host_info(),
                "kind": "reference",
                "sample": "dump_tool precise <export> 16 16 dump.txt 1 4294967295 --sort (%.2f s) + histogram_tool "
                          "precise <export> 16 16 hist.txt 1 10000 (%.2f s) on the same export; outputs %s" %
@@ -378,22 +413,24 @@ def bench_tools(args):
     ctx.close()
 
 
-def serving_cpu_baseline(n_keys, n_batches):
+def serving_cpu_baseline(n_keys, n_batches, batch_keys=0):
     """The reference's point lookup (FlatConcurrentHashMap find_prepared with
-    its query pool's prefetch loop) on this host: oracle/_ref/query_bench_ref."""
+    its query pool's prefetch loop) on this host: oracle/_ref/query_bench_ref,
+    batches of log-uniform size in [1, 65536] or of batch_keys keys."""
     ref = os.path.join(ROOT, "oracle", "_ref", "query_bench_ref")
     if not os.path.exists(ref):
         return None
-    threads = max(1, min(16, os.cpu_count() or 1))
-    out = subprocess.run([ref, str(n_keys), str(n_batches), str(threads), "5"], capture_output=True, text=True,
-                         timeout=600)
+    threads = host_threads()
+    args = [ref, str(n_keys), str(n_batches), str(threads), "5"] + ([str(batch_keys)] if batch_keys else [])
+    out = subprocess.run(args, capture_output=True, text=True, timeout=600)
     if out.returncode != 0:
         return {"error": out.stderr[-500:]}
     r = json.loads(out.stdout.strip().splitlines()[-1])
+    size = "%d keys" % batch_keys if batch_keys else "log-uniform size in [1, 65536]"
     return {"value": r["batch_p99_ms"], "unit": "ms", "cores": threads, "kind": "reference",
-            "keys_per_s": r["keys_per_s"], "batch_p50_ms": r["batch_p50_ms"],
-            "sample": "FlatConcurrentHashMap<1> of %d keys, %d batches of log-uniform size in [1, 65536] "
-                      "(half present), one batch per thread, prefetch distance 8" % (n_keys, n_batches)}
+            "keys_per_s": r["keys_per_s"], "batch_p50_ms": r["batch_p50_ms"], "host": host_info(),
+            "sample": "FlatConcurrentHashMap<1> of %d keys, %d batches of %s (half present), one batch per "
+                      "thread, prefetch distance 8, no job running" % (n_keys, n_batches, size)}
 
 
 def bench_serving(args):
@@ -417,18 +454,21 @@ def bench_serving(args):
     summ = run.run(srv)
     srv.close()
     line = {
-        "metric": "p99 query-batch latency under a Poisson job + query stream (serving)",
-        "value": summ["query_p99_ms"], "unit": "ms", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "metric": "p99 latency of 65536-key query batches under a Poisson job + query stream (serving)",
+        "value": summ["q64k_p99_ms"], "unit": "ms", "n_gpus": 1, "steps": 1, "warmup": 0,
         "ms_per_step": summ["wall_s"] * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
         "dtype": "u64", "data": "synthetic",
         "config": {"workload": "serving", "job_rate_per_s": spec.job_rate, "query_rate_per_s": spec.query_rate,
                    "seconds": spec.duration_s, "genomes": spec.n_genomes, "genome_len": spec.genome_len,
                    "job_reads_lognormal": [spec.reads_mu, spec.reads_sigma, spec.min_reads, spec.max_reads],
-                   "query_batch": [1, spec.max_query], "k": spec.k, "workers": spec.workers,
+                   "query_batch": [1, spec.max_query], "q64k_rate_per_s": spec.q64_rate,
+                   "q64k_idle_batches": spec.idle_q64, "k": spec.k, "workers": spec.workers,
                    "hbm_budget_gb": spec.hbm_budget / 1e9},
         "serving": summ,
         "roofline": None,
         "cpu_baseline": None if args.no_cpu_baseline else serving_cpu_baseline(
+            job_distinct_estimate(spec, int(np.median(run.job_reads))), 2000, spec.max_query),
+        "cpu_baseline_mixed_sizes": None if args.no_cpu_baseline else serving_cpu_baseline(
             job_distinct_estimate(spec, int(np.median(run.job_reads))), 20000),
         "prep_seconds": prep_s,
     }

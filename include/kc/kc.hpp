@@ -8,6 +8,8 @@
 //   kc::export_record_radix_sort<N>      ExportRecordRadixSort.h:256-269
 //   kc::radix_sort_export_records_12bit  ExportRecordRadixSort12.h:639-646
 //   kc::ExportWriter<N>                  ExportWriter.h:70-313 (low.bin)
+//   kc::FinalDrainWriter<N>              FinalDrainWriter.h:43-146 (high.bin)
+//   kc::write_infos                      main.cpp:590-597 (infos.bin)
 //   kc::ReadBatch                        batched counting of uneven read
 //                                        chunks (kc_batch; no reference
 //                                        counterpart: FastqReader.h:542-557
@@ -18,6 +20,7 @@
 
 #include <array>
 #include <cstdint>
+#include <cstdio>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -229,6 +232,191 @@ public:
 
 private:
     kc_batch *b_ = nullptr;
+};
+
+// infos.bin of a temp_dir: {u32 k, u32 count_max} (main.cpp:590-597).
+inline void write_infos(const std::string &temp_dir, uint32_t k, uint32_t count_max)
+{
+    const uint32_t v[2] = {k, count_max};
+    FILE *f = std::fopen((temp_dir + "infos.bin").c_str(), "wb");
+    if (!f) throw Error(KC_EIO, "write_infos");
+    const bool ok = std::fwrite(v, 4, 2, f) == 2;
+    if (std::fclose(f) != 0 || !ok) throw Error(KC_EIO, "write_infos");
+}
+
+namespace detail {
+// An output file of a temp_dir, created (truncated) when opened.
+class OutFile {
+public:
+    explicit OutFile(const std::string &path) : f_(std::fopen(path.c_str(), "wb"))
+    {
+        if (!f_) throw Error(KC_EIO, path.c_str());
+    }
+    ~OutFile() { close(); }
+    OutFile(const OutFile &) = delete;
+    OutFile &operator=(const OutFile &) = delete;
+    void write(const std::vector<uint8_t> &b)
+    {
+        if (!b.empty() && std::fwrite(b.data(), 1, b.size(), f_) != b.size()) throw Error(KC_EIO, "fwrite");
+    }
+    void close()
+    {
+        if (f_ && std::fclose(f_) != 0) {
+            f_ = nullptr;
+            throw Error(KC_EIO, "fclose");
+        }
+        f_ = nullptr;
+    }
+
+private:
+    FILE *f_;
+};
+}  // namespace detail
+
+// ExportWriter.h:70-313: writes temp_dir + "low.bin".  The constructor
+// creates the file (truncated), as the reference's does; write() takes the
+// k-mers of one export block, which are packed on the device
+// (kc_export_low, the byte_pack_to_buffer layout) in batches of up to
+// `batch` keys and appended; join() writes what is buffered and closes the
+// file.  start() is kept for the reference's call sequence (no writer thread
+// is needed).  Errors throw kc::Error where the reference exits.
+template <uint32_t N>
+class ExportWriter {
+    static_assert(N == 1 || N == 2, "k up to 63");
+
+public:
+    ExportWriter(Context &c, uint32_t k, const std::string &temp_dir, uint64_t batch = 1u << 20)
+        : c_(c), k_(k), batch_(batch ? batch : 1), file_(temp_dir + "low.bin")
+    {
+        if (k < 1 || k > 32 * N) throw Error(KC_EINVAL, "ExportWriter k");
+    }
+    void start() {}
+    void write(const kmer<N> *kmers, uint64_t n)
+    {
+        pending_.insert(pending_.end(), kmers, kmers + n);
+        if (pending_.size() >= batch_) flush();
+    }
+    void join()
+    {
+        flush();
+        file_.close();
+    }
+    uint64_t written() const { return written_; }
+
+private:
+    void flush()
+    {
+        const uint64_t n = pending_.size();
+        if (!n) return;
+        const uint64_t kb = (k_ / 32) * 8 + (2 * (k_ % 32) + 7) / 8;
+        DeviceBuffer dk(c_, n * sizeof(kmer<N>)), out(c_, n * kb + 16);
+        dk.upload(pending_.data(), n * sizeof(kmer<N>));
+        check(kc_export_low(c_.get(), static_cast<const uint64_t *>(dk.get()), n, k_,
+                            static_cast<uint8_t *>(out.get())), "kc_export_low");
+        std::vector<uint8_t> bytes(n * kb);
+        out.download(bytes.data(), bytes.size());
+        file_.write(bytes);
+        written_ += n;
+        pending_.clear();
+    }
+
+    Context &c_;
+    uint32_t k_;
+    uint64_t batch_;
+    detail::OutFile file_;
+    std::vector<kmer<N>> pending_;
+    uint64_t written_ = 0;
+};
+
+// FinalDrainWriter.h:43-146 (with the high.bin file of
+// FinalDrainWriterThread.h): write_kmer_record() packs a (k-mer, count)
+// record as given -- its caller filters, as NewKmerTree.h:1303-1311 does --
+// and write_map_record() leaves out records whose count fails the counter's
+// filter (count + 1 < filter_min in u32, or count > filter_max;
+// FinalDrainWriter.h:95-104).  Counts are written in count_bytes(count_max)
+// bytes (main.cpp:599-615).  The reference takes count_max, filter_min and
+// filter_max from its globals (definition.h:93-96); here they are arguments.
+// Records are packed on the device (kc_export_high) in batches; close()
+// writes what is buffered and closes temp_dir + "high.bin".
+template <uint32_t N>
+class FinalDrainWriter {
+    static_assert(N == 1 || N == 2, "k up to 63");
+
+public:
+    FinalDrainWriter(Context &c, uint32_t k, uint32_t count_max, const std::string &temp_dir,
+                     uint32_t filter_min = 2, uint32_t filter_max = 0xFFFFFFFFu, uint64_t batch = 1u << 20)
+        : c_(c), k_(k), count_max_(count_max), fmin_(filter_min), fmax_(filter_max), batch_(batch ? batch : 1),
+          file_(temp_dir + "high.bin")
+    {
+        if (k < 1 || k > 32 * N || count_max == 0) throw Error(KC_EINVAL, "FinalDrainWriter");
+    }
+    ~FinalDrainWriter()
+    {
+        try {
+            close();
+        } catch (...) {
+        }
+    }
+    void write_kmer_record(const uint64_t *kmer_data, uint32_t count)
+    {
+        kmer<N> key;
+        for (uint32_t w = 0; w < N; ++w) key.data[w] = kmer_data[w];
+        plain_.push_back({key, count});
+        if (plain_.size() >= batch_) flush(plain_, 0, 0xFFFFFFFFu);
+    }
+    void write_map_record(const ExportRecord<N> *records, uint32_t count)
+    {
+        filtered_.insert(filtered_.end(), records, records + count);
+        if (filtered_.size() >= batch_) flush(filtered_, fmin_, fmax_);
+    }
+    void close()
+    {
+        if (closed_) return;
+        flush(plain_, 0, 0xFFFFFFFFu);
+        flush(filtered_, fmin_, fmax_);
+        closed_ = true;
+        file_.close();
+    }
+    uint64_t written() const { return written_; }
+
+private:
+    void flush(std::vector<ExportRecord<N>> &recs, uint32_t fmin, uint32_t fmax)
+    {
+        const uint64_t n = recs.size();
+        if (!n) return;
+        std::vector<kmer<N>> keys(n);
+        std::vector<uint32_t> counts(n);
+        for (uint64_t i = 0; i < n; ++i) {
+            keys[i] = recs[i].key;
+            counts[i] = recs[i].count;
+        }
+        DeviceBuffer dk(c_, n * sizeof(kmer<N>)), dc(c_, n * 4);
+        dk.upload(keys.data(), n * sizeof(kmer<N>));
+        dc.upload(counts.data(), n * 4);
+        uint64_t bytes = 0;
+        const auto *kp = static_cast<const uint64_t *>(dk.get());
+        const auto *cp = static_cast<const uint32_t *>(dc.get());
+        check(kc_export_high(c_.get(), kp, cp, n, k_, count_max_, fmin, fmax, nullptr, 0, &bytes), "kc_export_high");
+        DeviceBuffer out(c_, bytes + 16);
+        check(kc_export_high(c_.get(), kp, cp, n, k_, count_max_, fmin, fmax, static_cast<uint8_t *>(out.get()),
+                             bytes + 16, &bytes),
+              "kc_export_high");
+        std::vector<uint8_t> host(bytes);
+        out.download(host.data(), bytes);
+        file_.write(host);
+        const uint64_t rec = (k_ / 32) * 8 + (2 * (k_ % 32) + 7) / 8 +
+                             (count_max_ <= 0xFF ? 1 : count_max_ <= 0xFFFF ? 2 : count_max_ <= 0xFFFFFF ? 3 : 4);
+        written_ += bytes / rec;
+        recs.clear();
+    }
+
+    Context &c_;
+    uint32_t k_, count_max_, fmin_, fmax_;
+    uint64_t batch_;
+    detail::OutFile file_;
+    std::vector<ExportRecord<N>> plain_, filtered_;
+    uint64_t written_ = 0;
+    bool closed_ = false;
 };
 
 namespace detail {

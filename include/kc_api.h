@@ -189,6 +189,48 @@ int kc_table_dump_device(kc_table *t, uint32_t min_freq, uint32_t max_freq, int 
 int kc_table_dump_file(kc_table *t, uint32_t min_freq, uint32_t max_freq, int sorted,
                        const char *path);
 
+/* ---- the on-disk format for caller keys: ExportWriter<N>
+ *      (ExportWriter.h:70-313) and FinalDrainWriter<N>
+ *      (FinalDrainWriter.h:43-146).  Device keys in, device bytes out,
+ *      asynchronous on ctx's stream except where a size is returned. ---- */
+/* low.bin records of n keys (ExportWriter::byte_pack_to_buffer,
+ * ExportWriter.h:296-313): k/32 full words as LE u64, then the top
+ * ceil(2(k%32)/8) bytes of the masked tail word.  d_out holds
+ * n * packed-k-mer bytes. */
+int kc_export_low(kc_ctx *ctx, const uint64_t *d_keys, uint64_t n, uint32_t k, uint8_t *d_out);
+/* high.bin records of n (key, count) pairs (write_kmer_record /
+ * write_map_record, FinalDrainWriter.h:71-146): the packed key, then the
+ * low count_bytes(count_max) bytes of the count (main.cpp:599-615; counts
+ * are written as given, not clamped).  A pair is left out when
+ * count + 1 < filter_min (in u32, so 2^32-1 wraps to 0) or count >
+ * filter_max (NewKmerTree.h:1303-1311, FinalDrainWriter.h:95-104); pass 0
+ * and 0xFFFFFFFF to keep every pair.  Kept records stay in input order.
+ * d_out NULL: *out_bytes only.  Synchronous. */
+int kc_export_high(kc_ctx *ctx, const uint64_t *d_keys, const uint32_t *d_counts, uint64_t n, uint32_t k,
+                   uint32_t count_max, uint32_t filter_min, uint32_t filter_max, uint8_t *d_out,
+                   uint64_t out_cap, uint64_t *out_bytes);
+
+/* ---- the reference's tools over a temp_dir (infos.bin, high.bin, low.bin
+ *      as written by the reference's counter or by kc_table_export_dir) ---- */
+/* Loads a temp_dir into a new table on ctx: k and count_max from infos.bin
+ * (main.cpp:590-597), high.bin records as table counts and, with with_low,
+ * low.bin keys as the table's FIRST list.  The table's histogram, dump and
+ * lookups are then histogram_tool's / dump_tool's precise merges of the
+ * files (histogram_tool.cpp:734-816, dump_tool.cpp:1202-1270); without
+ * low.bin, their approximate modes (high.bin alone: histogram_tool.cpp:
+ * 840-880, dump_tool.cpp:1294-1400).  Empty files are an empty export (the
+ * reference tools refuse an empty high.bin).  KC_ENOTSUP for k > 63. */
+int kc_table_load_dir(kc_ctx *ctx, const char *dir, int with_low, kc_table **out);
+/* histogram_tool <precise|approximate> <dir> ... <min> <max>: bins[f-min]
+ * for f = min..max, host int64. */
+int kc_histogram_dir(kc_ctx *ctx, const char *dir, int precise, uint32_t min_freq, uint32_t max_freq,
+                     int64_t *h_bins);
+/* dump_tool <precise|approximate> <dir> ... <path> <min> <max> [--sort]:
+ * "<bases>\t<count>\n" lines into path, sorted lexicographically when
+ * sorted != 0. */
+int kc_dump_dir(kc_ctx *ctx, const char *dir, int precise, uint32_t min_freq, uint32_t max_freq, int sorted,
+                const char *path);
+
 /* ---- table sizing from a device sample.  The reference plans its Bloom
  *      filters and maps from the file size, a pre-read quality average and a
  *      prefix sample (main.cpp:41-217, FastqPrefixCounter.h:28-133); here one

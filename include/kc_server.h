@@ -24,7 +24,7 @@ typedef struct kc_server kc_server;
 
 typedef struct kc_request_result {
     int status;          /* KC_OK or a KC_E* code */
-    int reserved;
+    int batched;         /* query batches answered by the same lookup launch (1: alone) */
     double latency_ms;   /* submit -> completion */
     double queue_ms;     /* submit -> start of execution */
     uint64_t kmers;      /* count jobs: windows counted */

@@ -19,8 +19,18 @@ HIP_FLAGS = [
     # packed FP32 ops stay off for all device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
-HIP_SOURCES = ["kc_count.hip", "kc_sort.hip", "kc_part.hip", "kc_dump.hip", "kc_batch.hip"]
+HIP_SOURCES = [
+    "kc_count.hip",
+    "kc_sort.hip",
+    "kc_part.hip",
+    "kc_dump.hip",
+    "kc_batch.hip",
+    "kc_tools.hip",
+]
 CXX_SOURCES = ["kc_host.cpp", "kc_server.cpp"]
+# command-line tools over libkc.so, with the reference tools' argv
+TOOLS = ["histogram_tool", "dump_tool"]
+BIN = os.path.join(PKG, "bin")
 
 
 def _run(cmd):
@@ -56,6 +66,12 @@ def build(jobs: int = 4) -> str:
     tmp = LIB + ".tmp"
     _run(["hipcc", "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs, "-lz", "-pthread"])
     os.replace(tmp, LIB)
+    os.makedirs(BIN, exist_ok=True)
+    for tool in TOOLS:
+        exe = os.path.join(BIN, tool)
+        _run(["g++", "-O2", "-std=c++17", "-Wall", "-o", exe + ".tmp", os.path.join(PKG, "tools", tool + ".cpp"),
+              "-L", PKG, "-l:libkc.so", "-Wl,-rpath,$ORIGIN/.."])
+        os.replace(exe + ".tmp", exe)
     return LIB
 
 

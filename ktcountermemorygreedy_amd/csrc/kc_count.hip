@@ -15,6 +15,7 @@
 
 #include "kc_dev_common.h"
 #include "kc_guard.h"
+#include "kc_pack_dev.h"
 
 namespace {
 thread_local char g_last_error[256];
@@ -201,57 +202,7 @@ __global__ void k_fill(uint64_t *p, uint64_t n, uint64_t v)
         p[i] = v;
 }
 
-// ---- export ------------------------------------------------------------
-// Packed k-mer bytes (ExportWriter.h:296-313): full words as LE u64, then
-// the top tail_bytes of the masked tail word; high records append the low
-// count_bytes of the count (FinalDrainWriter.h:71-88).
-struct PackSpec {
-    uint32_t full_words, tail_bytes, kmer_bytes, count_bytes;
-    uint64_t tail_mask;
-};
-
-__device__ __forceinline__ void put_bytes(uint8_t *dst, uint64_t v, uint32_t nbytes, uint32_t skip)
-{
-    for (uint32_t b = 0; b < nbytes; ++b) dst[b] = (uint8_t)(v >> (8 * (skip + b)));
-}
-
-template <int NW>
-__device__ __forceinline__ void pack_key(uint8_t *dst, const uint64_t *key, const PackSpec &ps)
-{
-    for (uint32_t w = 0; w < ps.full_words; ++w) put_bytes(dst + 8 * w, key[w], 8, 0);
-    if (ps.tail_bytes) {
-        const uint64_t tail = key[ps.full_words] & ps.tail_mask;
-        put_bytes(dst + 8 * ps.full_words, tail, ps.tail_bytes, 8 - ps.tail_bytes);
-    }
-}
-
-// Export compaction is deterministic: a counting pass writes per-tile
-// record counts, one block scans them, and the writing pass places each
-// tile's records at its scanned offset.  Records are packed into an LDS
-// staging buffer first and the tile's contiguous output bytes are then
-// written with coalesced stores.
-constexpr int kExportPer = 8;                        // slots per thread
-constexpr uint64_t kExportTile = (uint64_t)kBlock * kExportPer;
-constexpr uint32_t kExportStage = (uint32_t)kExportTile * 20;  // bytes: k=63 high record
-
-// Copies nbytes of LDS staging to dst (any alignment): byte stores up to a
-// 4-byte boundary, then 4-byte stores, then the tail; consecutive threads
-// take consecutive words, so every wave store is one contiguous segment.
-__device__ __forceinline__ void stage_copy(uint8_t *dst, const uint8_t *lds, uint32_t nbytes)
-{
-    const uint32_t head = (uint32_t)((4 - (reinterpret_cast<uintptr_t>(dst) & 3)) & 3);
-    const uint32_t h = head < nbytes ? head : nbytes;
-    if (threadIdx.x < h) dst[threadIdx.x] = lds[threadIdx.x];
-    const uint32_t words = (nbytes - h) / 4;
-    uint32_t *dw = reinterpret_cast<uint32_t *>(dst + h);
-    for (uint32_t w = threadIdx.x; w < words; w += kBlock) {
-        const uint8_t *b = lds + h + 4 * w;
-        dw[w] = (uint32_t)b[0] | ((uint32_t)b[1] << 8) | ((uint32_t)b[2] << 16) | ((uint32_t)b[3] << 24);
-    }
-    const uint32_t done = h + 4 * words;
-    if (threadIdx.x < nbytes - done) dst[done + threadIdx.x] = lds[done + threadIdx.x];
-}
-
+// ---- export (packing helpers: kc_pack_dev.h) ---------------------------
 // Classifies slot i: 1 = high record (count in cc), 2 = low record
 // (singleton without an active FIRST list), 0 = nothing.
 template <int NW>
@@ -613,19 +564,7 @@ TableView view_of(kc_table *t)
     return v;
 }
 
-PackSpec pack_spec(const kc_table *t)
-{
-    PackSpec p;
-    const uint32_t k = t->cfg.k;
-    p.full_words = k / 32;
-    const uint32_t tail_bits = 2 * (k % 32);
-    p.tail_bytes = (tail_bits + 7) / 8;
-    p.kmer_bytes = p.full_words * 8 + p.tail_bytes;
-    const uint32_t cm = t->cfg.count_max;
-    p.count_bytes = cm <= 0xFF ? 1 : cm <= 0xFFFF ? 2 : cm <= 0xFFFFFF ? 3 : 4;
-    p.tail_mask = tail_bits ? (~0ULL << (64 - tail_bits)) : 0;
-    return p;
-}
+PackSpec pack_spec(const kc_table *t) { return kc_pack_spec(t->cfg.k, t->cfg.count_max); }
 
 bool low_list_active(const kc_table *t) { return t->bloom != nullptr; }
 
@@ -1278,37 +1217,46 @@ extern "C" int kc_table_set_pass(kc_table *t, uint32_t pass, uint32_t n_passes)
     return KC_OK;
 }
 
-extern "C" int kc_table_histogram(kc_table *t, uint32_t min_freq, uint32_t max_freq, int64_t *h_bins)
+// histogram_tool precise over the table's exported view into host bins,
+// with the device bins in gb (at least max - min + 1 words; cleared here by
+// a fill kernel).  Allocates nothing: the host counters are the table's own.
+int kc_histogram_into(kc_table *t, uint32_t min_freq, uint32_t max_freq, int64_t *h_bins, unsigned long long *gb)
 {
-    if (!t || !h_bins || max_freq < min_freq) return KC_EINVAL;
     kc_ctx *c = t->ctx;
     HostCounters hc;
     int rc = read_counters(t, hc);
     if (rc != KC_OK) return rc;
     if (hc.stat[KC_STAT_ERRORS]) return KC_EFULL;
     const uint64_t nb = (uint64_t)max_freq - min_freq + 1;
-    unsigned long long *gb = nullptr;
-    KC_HIP(hipMalloc((void **)&gb, nb * 8));
-    rc = kc_launch_fill(c, reinterpret_cast<uint64_t *>(gb), nb, 0);
+    if ((rc = kc_launch_fill(c, reinterpret_cast<uint64_t *>(gb), nb, 0)) != KC_OK) return rc;
     const bool list = low_list_active(t);
     HistSpec hs{min_freq, max_freq, !list};
     const TableView v = view_of(t);
-    if (rc == KC_OK) {
-        const uint32_t g = grid_for(t->cap, c->cu_count);
-        if (t->nw == 1) hipLaunchKernelGGL(k_hist_table<1>, dim3(g), dim3(kBlock), 0, c->stream, v, hs, gb);
-        else hipLaunchKernelGGL(k_hist_table<2>, dim3(g), dim3(kBlock), 0, c->stream, v, hs, gb);
-        if (list && hc.low) {
-            uint64_t most = 0;
-            for (int sgi = 0; sgi < KC_LOW_SEGS; ++sgi) most = std::max(most, hc.seg_fill[sgi]);
-            const dim3 g2(kc_low_chunks(most, 4 * kBlock), KC_LOW_SEGS);
-            if (t->nw == 1) hipLaunchKernelGGL(k_hist_low_list<1>, g2, dim3(kBlock), 0, c->stream, v, hs, gb);
-            else hipLaunchKernelGGL(k_hist_low_list<2>, g2, dim3(kBlock), 0, c->stream, v, hs, gb);
-        }
-        if (hipGetLastError() != hipSuccess ||
-            hipMemcpyAsync(h_bins, gb, nb * 8, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
-            hipStreamSynchronize(c->stream) != hipSuccess)
-            rc = KC_EDEVICE;
+    const uint32_t g = grid_for(t->cap, c->cu_count);
+    if (t->nw == 1) hipLaunchKernelGGL(k_hist_table<1>, dim3(g), dim3(kBlock), 0, c->stream, v, hs, gb);
+    else hipLaunchKernelGGL(k_hist_table<2>, dim3(g), dim3(kBlock), 0, c->stream, v, hs, gb);
+    if (list && hc.low) {
+        uint64_t most = 0;
+        for (int sgi = 0; sgi < KC_LOW_SEGS; ++sgi) most = std::max(most, hc.seg_fill[sgi]);
+        const dim3 g2(kc_low_chunks(most, 4 * kBlock), KC_LOW_SEGS);
+        if (t->nw == 1) hipLaunchKernelGGL(k_hist_low_list<1>, g2, dim3(kBlock), 0, c->stream, v, hs, gb);
+        else hipLaunchKernelGGL(k_hist_low_list<2>, g2, dim3(kBlock), 0, c->stream, v, hs, gb);
     }
+    KC_HIP(hipGetLastError());
+    KC_HIP(hipMemcpyAsync(h_bins, gb, nb * 8, hipMemcpyDeviceToHost, c->stream));
+    KC_HIP(hipStreamSynchronize(c->stream));
+    return KC_OK;
+}
+
+extern "C" int kc_table_histogram(kc_table *t, uint32_t min_freq, uint32_t max_freq, int64_t *h_bins)
+{
+    if (!t || !h_bins || max_freq < min_freq) return KC_EINVAL;
+    KC_HIP(hipSetDevice(t->ctx->device));
+    const uint64_t nb = (uint64_t)max_freq - min_freq + 1;
+    unsigned long long *gb = nullptr;
+    if (hipMalloc((void **)&gb, nb * 8) != hipSuccess) return KC_ENOMEM;
+    const int rc = kc_histogram_into(t, min_freq, max_freq, h_bins, gb);
+    (void)hipStreamSynchronize(t->ctx->stream);
     (void)hipFree(gb);
     return rc;
 }

@@ -139,3 +139,6 @@ int kc_scan_u32(kc_ctx *ctx, const uint32_t *in, uint64_t n, uint64_t *out, uint
 int kc_scan_u32_on(hipStream_t st, const uint32_t *in, uint64_t n, uint64_t *out, uint64_t *sums);
 int kc_sort_range(kc_ctx *ctx, uint32_t *a, uint32_t *b, uint64_t n, uint32_t rw, uint32_t lo, uint32_t hi,
                   int *in_scratch);
+// histogram_tool precise over a table's exported view with caller-provided
+// device bins (kc_count.hip); allocates nothing.
+int kc_histogram_into(kc_table *t, uint32_t min_freq, uint32_t max_freq, int64_t *h_bins, unsigned long long *gb);

@@ -307,7 +307,10 @@ struct kc_server {
                 }
             }
         }
-        for (Request *r : batch) r->res.status = rc;
+        for (Request *r : batch) {
+            r->res.status = rc;
+            r->res.batched = (int)batch.size();
+        }
         tab->last_use = Clock::now();
     }
 

@@ -85,7 +85,7 @@ class Plan(C.Structure):
 
 
 class RequestResult(C.Structure):
-    _fields_ = [("status", C.c_int), ("reserved", C.c_int), ("latency_ms", C.c_double),
+    _fields_ = [("status", C.c_int), ("batched", C.c_int), ("latency_ms", C.c_double),
                 ("queue_ms", C.c_double), ("kmers", C.c_uint64), ("distinct", C.c_uint64)]
 
 
@@ -126,6 +126,12 @@ EXPORTS = {
     "kc_table_dump_device": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_int, C.c_void_p, C.c_uint64,
                                        C.POINTER(C.c_uint64)]),
     "kc_table_dump_file": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_int, C.c_char_p]),
+    "kc_export_low": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32, C.c_void_p]),
+    "kc_export_high": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32,
+                                 C.c_uint32, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "kc_table_load_dir": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, C.POINTER(C.c_void_p)]),
+    "kc_histogram_dir": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, C.c_uint32, C.c_uint32, C.c_void_p]),
+    "kc_dump_dir": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, C.c_uint32, C.c_uint32, C.c_int, C.c_char_p]),
     "kc_plan_table": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint64, C.POINTER(TableConfig),
                                 C.POINTER(Plan)]),
     "kc_plan_sizes": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint64, C.POINTER(TableConfig), C.POINTER(Plan)]),
@@ -373,6 +379,43 @@ class Context:
                                          _ptr(bins)), "kc_records_histogram")
         return bins
 
+    def export_low(self, keys: np.ndarray, k) -> bytes:
+        """low.bin records of host keys (n x nw uint64), packed on the device."""
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        n = keys.size // words_for_k(k)
+        kb = (k // 32) * 8 + (2 * (k % 32) + 7) // 8
+        dk, out = self.upload(keys), self.alloc(max(16, n * kb))
+        check(lib().kc_export_low(self.h, _ptr(dk), n, k, _ptr(out)), "kc_export_low")
+        self.sync()
+        return out.numpy(np.uint8, n * kb).tobytes()
+
+    def export_high(self, keys: np.ndarray, counts: np.ndarray, k, count_max=0xFFFFFFFF, filter_min=0,
+                    filter_max=0xFFFFFFFF) -> bytes:
+        """high.bin records of host (key, count) pairs, filtered as the counter does."""
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        counts = np.ascontiguousarray(counts, dtype=np.uint32)
+        n = counts.size
+        dk, dc = self.upload(keys), self.upload(counts)
+        size = C.c_uint64()
+        check(lib().kc_export_high(self.h, _ptr(dk), _ptr(dc), n, k, count_max, filter_min, filter_max, None, 0,
+                                   C.byref(size)), "kc_export_high(size)")
+        out = self.alloc(max(16, size.value))
+        check(lib().kc_export_high(self.h, _ptr(dk), _ptr(dc), n, k, count_max, filter_min, filter_max, _ptr(out),
+                                   out.nbytes, C.byref(size)), "kc_export_high")
+        return out.numpy(np.uint8, size.value).tobytes()
+
+    def histogram_dir(self, path, min_freq=1, max_freq=10000, precise=True):
+        """histogram_tool over a temp_dir, on the device."""
+        bins = np.zeros(max_freq - min_freq + 1, dtype=np.int64)
+        check(lib().kc_histogram_dir(self.h, os.fsencode(path), 1 if precise else 0, min_freq, max_freq, _ptr(bins)),
+              "kc_histogram_dir")
+        return bins
+
+    def dump_dir(self, path, out_path, min_freq=1, max_freq=0xFFFFFFFF, sorted=True, precise=True):
+        """dump_tool over a temp_dir into out_path, on the device."""
+        check(lib().kc_dump_dir(self.h, os.fsencode(path), 1 if precise else 0, min_freq, max_freq,
+                                1 if sorted else 0, os.fsencode(out_path)), "kc_dump_dir")
+
     def synth_records(self, recs, n, record_bytes=12, seed=4, zipf_s=1.2):
         check(lib().kc_synth_records(self.h, _ptr(recs), n, record_bytes, seed, zipf_s), "kc_synth_records")
 
@@ -382,17 +425,31 @@ class Table:
 
     def __init__(self, ctx: Context, k, count_max=0xFFFFFFFF, capacity=0, max_distinct=0,
                  bloom=False, bloom_blocks=0, low_capacity=0, filter_min=0, filter_max=0xFFFFFFFF,
-                 path="auto"):
-        paths = {"auto": 0, "direct": 1, "partitioned": 2}
-        cfg = TableConfig(k=k, count_max=count_max, filter_min=filter_min, filter_max=filter_max,
-                          capacity=capacity, max_distinct=max_distinct, bloom=1 if bloom else 0,
-                          count_path=paths[path], bloom_blocks=bloom_blocks, low_capacity=low_capacity)
-        h = C.c_void_p()
-        check(lib().kc_table_create(ctx.h, C.byref(cfg), C.byref(h)), "kc_table_create")
+                 path="auto", _handle=None):
+        if _handle is None:
+            paths = {"auto": 0, "direct": 1, "partitioned": 2}
+            cfg = TableConfig(k=k, count_max=count_max, filter_min=filter_min, filter_max=filter_max,
+                              capacity=capacity, max_distinct=max_distinct, bloom=1 if bloom else 0,
+                              count_path=paths[path], bloom_blocks=bloom_blocks, low_capacity=low_capacity)
+            h = C.c_void_p()
+            check(lib().kc_table_create(ctx.h, C.byref(cfg), C.byref(h)), "kc_table_create")
+        else:
+            h = _handle
         self.h, self.ctx, self.k, self.nw, self.count_max = h, ctx, k, words_for_k(k), count_max
         self.bloom = bool(bloom)
         self._batches = weakref.WeakSet()  # destroyed before the table: they count into it
         ctx._tables.add(self)
+
+    @classmethod
+    def load_dir(cls, ctx: Context, path, with_low=True):
+        """A temp_dir (infos.bin, high.bin, low.bin) loaded into a table whose
+        histogram / dump / lookups are the tools' merges of the files."""
+        h = C.c_void_p()
+        check(lib().kc_table_load_dir(ctx.h, os.fsencode(path), 1 if with_low else 0, C.byref(h)),
+              "kc_table_load_dir")
+        with open(os.path.join(path, "infos.bin"), "rb") as f:
+            k, count_max = np.frombuffer(f.read(8), dtype=np.uint32)
+        return cls(ctx, int(k), int(count_max), bloom=with_low, _handle=h)
 
     def close(self):
         if self.h:

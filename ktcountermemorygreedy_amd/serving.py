@@ -16,6 +16,10 @@ This driver replays a seeded schedule:
            completed tables: half of the keys are canonical 31-mers sampled
            from the job's reads (present), half are canonical 31-mers of
            random bases (absent with probability above 1 - 1e-9 each).
+  q64k     a second Poisson stream, `q64_rate` per second, of batches of
+           exactly 65536 keys (the batch size the latency target is quoted
+           for), built the same way; after the schedule, `idle_q64` more such
+           batches are sent one at a time with no job running (idle latency).
 
 Every query answer is checked: present keys >= 1, absent keys 0.
 """
@@ -76,6 +80,8 @@ class ServingSpec:
     recent_tables: int = 16
     workers: int = 8
     hbm_budget: int = 160 << 30
+    q64_rate: float = 10.0
+    idle_q64: int = 64
 
 
 def job_distinct_estimate(spec: ServingSpec, n_reads: int) -> int:
@@ -96,6 +102,8 @@ class Schedule:
     q_t: np.ndarray         # query batch arrival times (s)
     q_n: np.ndarray         # keys per batch
     q_pick: np.ndarray      # picks the target among the recent tables
+    q64_t: np.ndarray       # arrival times of the 65536-key batches
+    q64_pick: np.ndarray
 
 
 def make_schedule(spec: ServingSpec, rng: np.random.Generator) -> Schedule:
@@ -112,7 +120,10 @@ def make_schedule(spec: ServingSpec, rng: np.random.Generator) -> Schedule:
     q_n = np.clip(np.rint(np.exp(rng.uniform(0.0, math.log(spec.max_query), n_q))), 1,
                   spec.max_query).astype(np.int64)
     q_pick = rng.integers(0, 1 << 30, n_q)
-    return Schedule(job_t, job_reads, job_genome, job_off, q_t, q_n, q_pick)
+    n_q64 = int(rng.poisson(spec.q64_rate * spec.duration_s))
+    q64_t = np.sort(rng.uniform(0.0, spec.duration_s, n_q64))
+    q64_pick = rng.integers(0, 1 << 30, n_q64)
+    return Schedule(job_t, job_reads, job_genome, job_off, q_t, q_n, q_pick, q64_t, q64_pick)
 
 
 class ServingRun:
@@ -126,6 +137,7 @@ class ServingRun:
         self.schedule = sc
         self.job_t, self.job_reads, self.job_genome, self.job_off = sc.job_t, sc.job_reads, sc.job_genome, sc.job_off
         self.q_t, self.q_n, self.q_pick = sc.q_t, sc.q_n, sc.q_pick
+        self.q64_t, self.q64_pick = sc.q64_t, sc.q64_pick
         n_jobs = self.job_t.size
         # pools: only the genomes the schedule uses
         self.pools = {}
@@ -161,8 +173,9 @@ class ServingRun:
         spec = self.spec
         lock = threading.Lock()
         completed = []     # job indices in completion order
-        jobs, queries = [], []
-        events = sorted([(t, 0, j) for j, t in enumerate(self.job_t)] + [(t, 1, q) for q, t in enumerate(self.q_t)])
+        jobs, queries, q64 = [], [], []
+        events = sorted([(t, 0, j) for j, t in enumerate(self.job_t)] + [(t, 1, q) for q, t in enumerate(self.q_t)] +
+                        [(t, 2, q) for q, t in enumerate(self.q64_t)])
         ex = ThreadPoolExecutor(max_workers=64)
 
         def wait_job(j, rid):
@@ -172,14 +185,14 @@ class ServingRun:
                 if res["status"] == kc.KC_OK:
                     completed.append(j)
 
-        def wait_query(j, n_present, rid):
+        def wait_query(j, n_present, rid, big):
             res, counts = server.wait(rid)
             ok = res["status"] == kc.KC_OK
             bad = 0
             if ok:
                 bad = int((counts[:n_present] == 0).sum() + (counts[n_present:] != 0).sum())
             with lock:
-                queries.append((j, int(counts.size) if counts is not None else 0, res, bad))
+                (q64 if big else queries).append((j, int(counts.size) if counts is not None else 0, res, bad))
 
         futs = []
         t0 = time.monotonic()
@@ -199,21 +212,44 @@ class ServingRun:
                 if not recent:
                     skipped += 1
                     continue
-                j = recent[int(self.q_pick[i]) % len(recent)]
-                n = int(self.q_n[i])
-                n_present = min(n // 2, self.present[j].size)
-                keys = np.concatenate([self.present[j][:n_present], self.absent[:n - n_present]])
+                big = kind == 2
+                j = recent[int((self.q64_pick if big else self.q_pick)[i]) % len(recent)]
+                n = spec.max_query if big else int(self.q_n[i])
+                keys, n_present = self.query_keys(j, n)
                 rid = server.submit_query(table_base + j, keys)
-                futs.append(ex.submit(wait_query, j, n_present, rid))
+                futs.append(ex.submit(wait_query, j, n_present, rid, big))
         for f in futs:
             f.result()
         wall = time.monotonic() - t0
         ex.shutdown()
-        return self._summary(jobs, queries, skipped, wall, server)
+        idle = self.run_idle(server, table_base, completed[-spec.recent_tables:])
+        return self._summary(jobs, queries, q64, idle, skipped, wall, server)
 
-    def _summary(self, jobs, queries, skipped, wall, server):
+    def query_keys(self, j, n):
+        """n keys for job j's table: half present, half absent."""
+        n_present = min(n // 2, self.present[j].size)
+        return np.concatenate([self.present[j][:n_present], self.absent[:n - n_present]]), n_present
+
+    def run_idle(self, server, table_base, recent):
+        """65536-key batches one at a time with no job running."""
+        out = []
+        if not recent:
+            return out
+        for i in range(self.spec.idle_q64):
+            j = recent[i % len(recent)]
+            keys, n_present = self.query_keys(j, self.spec.max_query)
+            res, counts = server.wait(server.submit_query(table_base + j, keys))
+            bad = int((counts[:n_present] == 0).sum() + (counts[n_present:] != 0).sum()) \
+                if res["status"] == kc.KC_OK else 0
+            out.append((j, keys.size, res, bad))
+        return out
+
+    def _summary(self, jobs, queries, q64, idle, skipped, wall, server):
         def pct(v, p):
             return float(np.percentile(v, p)) if len(v) else None
+
+        def lat(qs):
+            return [r["latency_ms"] for _, _, r, _ in qs if r["status"] == kc.KC_OK]
 
         jl = [r["latency_ms"] for _, r in jobs if r["status"] == kc.KC_OK]
         ql = [r["latency_ms"] for _, _, r, _ in queries if r["status"] == kc.KC_OK]
@@ -231,5 +267,13 @@ class ServingRun:
             "query_keys": int(sum(n for _, n, _, _ in queries)),
             "query_p50_ms": pct(ql, 50), "query_p99_ms": pct(ql, 99),
             "query_wrong": int(sum(b for _, _, _, b in queries)),
+            "query_merged_share": float(np.mean([r["batched"] > 1 for _, _, r, _ in queries])) if queries else None,
+            "q64k": len(q64), "q64k_failed": sum(1 for _, _, r, _ in q64 if r["status"] != kc.KC_OK),
+            "q64k_wrong": int(sum(b for _, _, _, b in q64)),
+            "q64k_p50_ms": pct(lat(q64), 50), "q64k_p99_ms": pct(lat(q64), 99),
+            "q64k_max_ms": max(lat(q64)) if q64 else None,
+            "q64k_idle": len(idle), "q64k_idle_wrong": int(sum(b for _, _, _, b in idle)),
+            "q64k_idle_failed": sum(1 for _, _, r, _ in idle if r["status"] != kc.KC_OK),
+            "q64k_idle_p50_ms": pct(lat(idle), 50), "q64k_idle_p99_ms": pct(lat(idle), 99),
             "resident_tables": resident, "resident_gb": resident_bytes / 1e9,
         }

@@ -3,12 +3,13 @@
 // LowFrequencyQueryThreadPool.h:196-244: prepare_lookup, prefetch a fixed
 // distance ahead (QUERY_PREFETCH_DISTANCE = 8, :25), find_prepared.
 //
-//   query_bench_ref <n_keys> <n_batches> <threads> <seed>
+//   query_bench_ref <n_keys> <n_batches> <threads> <seed> [batch_keys]
 //
 // Inserts n_keys distinct k=31 keys (a 62-bit SplitMix64 stream shifted left
 // by 2, the one-word layout of kmer.h) with counts 1.., seals, then answers
-// n_batches query batches of log-uniform size in [1, 65536], half present
-// keys and half absent ones, one batch per thread at a time.  Prints one JSON
+// n_batches query batches of log-uniform size in [1, 65536] (or of
+// batch_keys keys each), half present keys and half absent ones, one batch
+// per thread at a time.  Prints one JSON
 // line: keys/s and the p50/p99 batch latency.  Exit 1 if any answer is wrong.
 #include "FlatConcurrentHashMap.h"
 
@@ -43,10 +44,11 @@ kmer<1> key_of(uint64_t stream, uint64_t i)
 
 int main(int argc, char **argv)
 {
-    if (argc != 5) {
-        std::fprintf(stderr, "usage: %s n_keys n_batches threads seed\n", argv[0]);
+    if (argc != 5 && argc != 6) {
+        std::fprintf(stderr, "usage: %s n_keys n_batches threads seed [batch_keys]\n", argv[0]);
         return 2;
     }
+    const uint64_t fixed = argc == 6 ? std::strtoull(argv[5], nullptr, 10) : 0;
     const uint64_t n_keys = std::strtoull(argv[1], nullptr, 10);
     const uint64_t n_batches = std::strtoull(argv[2], nullptr, 10);
     const uint32_t threads = (uint32_t)std::strtoul(argv[3], nullptr, 10);
@@ -72,7 +74,8 @@ int main(int argc, char **argv)
     uint64_t total = 0;
     for (uint64_t b = 0; b < n_batches; ++b) {
         const double u = (double)(splitmix(rs) >> 11) * 0x1.0p-53;
-        size[b] = std::min<uint64_t>(65536, std::max<uint64_t>(1, (uint64_t)std::llround(std::exp(u * std::log(65536.0)))));
+        size[b] = fixed ? fixed
+                        : std::min<uint64_t>(65536, std::max<uint64_t>(1, (uint64_t)std::llround(std::exp(u * std::log(65536.0)))));
         first[b] = splitmix(rs) % n_keys;
         total += size[b];
     }

@@ -92,6 +92,46 @@ int main()
         }
     }
 
+    // a temp_dir written with the writer adaptors: the FIRST occurrence of
+    // each key in low.bin, its other occurrences as the high.bin count;
+    // loading it back merges to the true counts (histogram_tool precise)
+    {
+        const char *dir_env = std::getenv("KC_ADAPTOR_DIR");
+        if (!dir_env) { std::printf("KC_ADAPTOR_DIR not set\n"); return 1; }
+        const std::string dir = std::string(dir_env) + "/";
+        kc::write_infos(dir, 31, 255);
+        {
+            kc::ExportWriter<1> low(ctx, 31, dir, 300);
+            low.start();
+            low.write(keys.data(), keys.size());
+            low.join();
+            if (low.written() != keys.size()) { std::printf("low records\n"); return 1; }
+            kc::FinalDrainWriter<1> high(ctx, 31, 255, dir, 2, 5, 256);
+            std::vector<kc::ExportRecord<1>> rs;
+            for (size_t i = 0; i < keys.size(); ++i)
+                if (i % 7) rs.push_back({keys[i], (uint32_t)(i % 7)});  // total count i % 7 + 1
+            // the map path filters (count 1 < 2 - 1 is kept, count 6 > 5 is dropped)...
+            high.write_map_record(rs.data(), (uint32_t)rs.size());
+            // ...the k-mer path does not
+            high.write_kmer_record(keys[6].data.data(), 6);
+            high.close();
+        }
+        kc_table *t = nullptr;
+        kc::check(kc_table_load_dir(ctx.get(), dir.c_str(), 1, &t), "kc_table_load_dir");
+        std::vector<int64_t> bins(10);
+        kc::check(kc_table_histogram(t, 1, 10, bins.data()), "kc_table_histogram");
+        kc_table_destroy(t);
+        // keys i with i % 7 in 1..5 merge to i % 7 + 1; i % 7 == 6 (dropped from
+        // high.bin but for key 6) read 1; i % 7 == 0 read 1
+        std::vector<int64_t> want_bins(10, 0);
+        for (size_t i = 0; i < keys.size(); ++i) {
+            const size_t r = i % 7;
+            const uint32_t c = (r >= 1 && r <= 5) || i == 6 ? (uint32_t)r + 1 : 1;
+            ++want_bins[c - 1];
+        }
+        if (bins != want_bins) { std::printf("writer round trip histogram differs\n"); return 1; }
+    }
+
     bool threw = false;
     try { kc::radix_sort_export_records_12bit<1, 10>(ctx, recs, scratch, 9); } catch (const std::invalid_argument &) { threw = true; }
     if (!threw) { std::printf("missing invalid_argument\n"); return 1; }

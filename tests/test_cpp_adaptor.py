@@ -44,5 +44,8 @@ def test_adaptor_runs(tmp_path):
     exe = str(tmp_path / "adaptor_check")
     subprocess.check_call(["g++", "-std=c++17", "-O2", "-I", os.path.join(ROOT, "include"), SRC, "-o", exe,
                            "-L", PKG, "-l:libkc.so", "-Wl,-rpath," + PKG])
-    out = subprocess.run([exe], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=120)
+    d = tmp_path / "tmpdir"
+    d.mkdir()
+    env = dict(os.environ, KC_ADAPTOR_DIR=str(d))
+    out = subprocess.run([exe], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=120, env=env)
     assert out.returncode == 0 and out.stdout.strip().endswith("ok"), out.stdout
