@@ -180,6 +180,17 @@ int kc_table_export_device(kc_table *t, uint8_t *d_high, uint64_t high_cap,
 int kc_table_histogram(kc_table *t, uint32_t min_freq, uint32_t max_freq,
                        int64_t *h_bins);
 
+/* kc_table_histogram without allocating: the device bins live in a static
+ * device array of KC_NOALLOC_HIST_BINS bins per device (512 KiB, held until
+ * the process ends), cleared at the start of each call.  No counterpart in
+ * the reference; for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid allocation).
+ * KC_EINVAL when max - min + 1 > KC_NOALLOC_HIST_BINS.  Calls that overlap
+ * on one device race on the shared array. */
+#define KC_NOALLOC_HIST_BINS 65536
+int kc_table_histogram_noalloc(kc_table *t, uint32_t min_freq, uint32_t max_freq,
+                               int64_t *h_bins);
+
 /* dump_tool precise over the exported view (dump_tool.cpp:1202-1270):
  * "<bases>\t<count>\n" lines of merged counts in [min, max], sorted
  * lexicographically when sorted != 0 (the --sort order), built on the

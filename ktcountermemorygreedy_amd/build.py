@@ -26,6 +26,7 @@ HIP_SOURCES = [
     "kc_dump.hip",
     "kc_batch.hip",
     "kc_tools.hip",
+    "kc_noalloc.hip",
 ]
 CXX_SOURCES = ["kc_host.cpp", "kc_server.cpp"]
 # command-line tools over libkc.so, with the reference tools' argv

@@ -123,6 +123,7 @@ EXPORTS = {
     "kc_table_export_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64),
                                          C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
     "kc_table_histogram": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]),
+    "kc_table_histogram_noalloc": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]),
     "kc_table_dump_device": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_int, C.c_void_p, C.c_uint64,
                                        C.POINTER(C.c_uint64)]),
     "kc_table_dump_file": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_int, C.c_char_p]),
@@ -548,6 +549,13 @@ class Table:
     def histogram(self, min_freq=1, max_freq=10000):
         bins = np.zeros(max_freq - min_freq + 1, dtype=np.int64)
         check(lib().kc_table_histogram(self.h, min_freq, max_freq, _ptr(bins)), "kc_table_histogram")
+        return bins
+
+    def histogram_noalloc(self, min_freq=1, max_freq=10000):
+        """kc_table_histogram_noalloc: the same bins, no device allocation."""
+        bins = np.zeros(max_freq - min_freq + 1, dtype=np.int64)
+        check(lib().kc_table_histogram_noalloc(self.h, min_freq, max_freq, _ptr(bins)),
+              "kc_table_histogram_noalloc")
         return bins
 
     def set_tune(self, **kw):
