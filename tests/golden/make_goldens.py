@@ -239,10 +239,13 @@ def write_tiny_counts():
     seq = O.fastq_frame(text.encode())
     uniq, cnt = O.count(O.extract(seq, 31), 1)
     infos, high, low = O.split_export(uniq, cnt, 31, 0xFFFFFFFF)
+    # FIRST split: every key in low.bin, count - 1 in high.bin for the keys
+    # seen at least twice (what the Bloom split writes when no first
+    # occurrence is mistaken for a second one)
     rep = cnt >= 2
-    _, high_f, _ = O.split_export(uniq[rep], cnt[rep] - 1, 31, 0xFFFFFFFF)
-    high_f = b"".join(high_f[i:i + 12] for i in range(0, len(high_f), 12) if
-                      int.from_bytes(high_f[i + 8:i + 12], "little") >= 1)
+    kb = np.frombuffer(O.pack(uniq[rep], 31), dtype=np.uint8).reshape(-1, 8)
+    cb = (cnt[rep] - 1).astype(np.uint32).view(np.uint8).reshape(-1, 4)
+    high_f = np.concatenate([kb, cb], axis=1).tobytes()
     low_f = O.pack(uniq, 31)
     for name, h, l in (("plain", high, low), ("first", high_f, low_f)):
         with open(os.path.join(HERE, "tiny_counts.%s.high.bin" % name), "wb") as f:
