@@ -158,7 +158,8 @@ int sync_pass(kc_batch *b)
 int check_table(kc_batch *b)
 {
     kc_table_touch(b->t);
-    return sync_pass(b);
+    const int rc = kc_table_materialize(b->t);  // the captured passes read the table
+    return rc != KC_OK ? rc : sync_pass(b);
 }
 
 // Launches slot i over its packed bytes (emit_from = halo of a piece).
@@ -549,11 +550,13 @@ extern "C" int kc_autotune(kc_table *t, const uint8_t *d_sample, uint64_t nbytes
             win = t->tune;
         }
     }
-    // the other FIRST hand-over of the partition pass on the winning shape
-    if (rc == KC_OK && best_t > 0 && win.path == KC_PATH_PARTITIONED && t->bloom) {
-        const bool staged = kc_part_shape(t, win, 0).stage_first;
+    // the other FIRST hand-overs of the partition pass on the winning shape
+    const uint32_t first_default = kc_part_shape(t, win, 0).first_mode;
+    for (uint32_t mode = 1; mode <= 3; ++mode) {
+        if (rc != KC_OK || best_t <= 0 || win.path != KC_PATH_PARTITIONED || !t->bloom) break;
+        if (mode == first_default) continue;
         t->tune = win;
-        t->tune.first_mode = staged ? 2 : 1;
+        t->tune.first_mode = mode;
         float tmin = 0;
         for (uint32_t r = 0; r <= reps && rc == KC_OK; ++r) {
             if ((rc = kc_table_clear(t)) != KC_OK) break;

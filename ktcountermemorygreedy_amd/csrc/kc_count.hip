@@ -747,6 +747,27 @@ extern "C" int kc_ctx_sync(kc_ctx *ctx)
 
 extern "C" void *kc_ctx_stream(kc_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
 
+// Empty slots and a zero Bloom filter.
+static int init_slots_bloom(kc_table *t)
+{
+    kc_ctx *c = t->ctx;
+    if (t->nw == 1)
+        hipLaunchKernelGGL(k_init_slots<1>, dim3(grid_for(t->cap, c->cu_count)), dim3(kBlock), 0, c->stream, t->slots, t->cap);
+    else
+        hipLaunchKernelGGL(k_init_slots<2>, dim3(grid_for(t->cap, c->cu_count)), dim3(kBlock), 0, c->stream, t->slots, t->cap);
+    KC_HIP(hipGetLastError());
+    if (t->bloom) return kc_launch_fill(c, t->bloom, 2 * t->bloom_blocks, 0);
+    return KC_OK;
+}
+
+int kc_table_materialize(kc_table *t)
+{
+    if (!t->fresh) return KC_OK;
+    const int rc = init_slots_bloom(t);
+    if (rc == KC_OK) t->fresh = false;
+    return rc;
+}
+
 static int table_init_device(kc_table *t)
 {
     kc_ctx *c = t->ctx;
@@ -758,6 +779,7 @@ static int table_init_device(kc_table *t)
     int rc;
     if (t->bloom && (rc = kc_launch_fill(c, t->bloom, 2 * t->bloom_blocks, 0)) != KC_OK) return rc;
     if ((rc = kc_launch_fill(c, t->counters, KC_COUNTER_WORDS, 0)) != KC_OK) return rc;
+    t->fresh = false;
     kc_table_touch(t);
     return KC_OK;
 }
@@ -890,11 +912,21 @@ extern "C" int kc_table_trim(kc_table *t)
     return KC_OK;
 }
 
+// A table the partitioned pass can count is cleared lazily: only the
+// counters are zeroed now; the next partitioned pass starts its partitions
+// empty in LDS and writes every slot and Bloom word (no read of the old
+// ones), and any other access writes the empty table first
+// (kc_table_materialize).
 extern "C" int kc_table_clear(kc_table *t)
 {
     if (!t) return KC_EINVAL;
     KC_HIP(hipSetDevice(t->ctx->device));
-    return table_init_device(t);
+    if (!kc_part_eligible(t)) return table_init_device(t);
+    const int rc = kc_launch_fill(t->ctx, t->counters, KC_COUNTER_WORDS, 0);
+    if (rc != KC_OK) return rc;
+    t->fresh = true;
+    kc_table_touch(t);
+    return KC_OK;
 }
 
 // Batches of at least this many bytes / keys take the partitioned path
@@ -932,6 +964,8 @@ static int launch_count(kc_table *t, const uint8_t *d_seq, uint64_t nbytes, uint
     kc_table_touch(t);
     const TableView v = view_of(t);
     if (use_partitioned(t, nbytes, kPartMinBytes)) return kc_part_count(t, d_seq, nbytes, emit_from, nullptr, 0, v);
+    int rc = kc_table_materialize(t);
+    if (rc != KC_OK) return rc;
     const uint64_t blocks = (nbytes + kTile - 1) / kTile;
     return kc_direct_enqueue(t, SeqSrc{d_seq, nbytes, emit_from, nullptr}, (uint32_t)std::min<uint64_t>(blocks, 1u << 30),
                              t->ctx->stream);
@@ -1014,6 +1048,8 @@ extern "C" int kc_table_increment(kc_table *t, const uint64_t *d_keys, uint64_t 
     // of the reference's maps does
     if (!d_inc && !t->bloom && use_partitioned(t, n, kPartMinKeys))
         return kc_part_count(t, nullptr, 0, 0, d_keys, n, v);
+    int rc = kc_table_materialize(t);
+    if (rc != KC_OK) return rc;
     const uint32_t g = grid_for(n, t->ctx->cu_count);
     if (t->nw == 1) hipLaunchKernelGGL(k_increment<1>, dim3(g), dim3(kBlock), 0, t->ctx->stream, d_keys, n, d_inc, v);
     else hipLaunchKernelGGL(k_increment<2>, dim3(g), dim3(kBlock), 0, t->ctx->stream, d_keys, n, d_inc, v);
@@ -1078,7 +1114,8 @@ extern "C" int kc_table_query(kc_table *t, const uint64_t *d_keys, uint64_t n, u
     if (!t || (n && (!d_keys || !d_counts))) return KC_EINVAL;
     KC_HIP(hipSetDevice(t->ctx->device));
     uint64_t imask = 0;
-    int rc = ensure_low_index(t, &imask);
+    int rc = kc_table_materialize(t);
+    if (rc == KC_OK) rc = ensure_low_index(t, &imask);
     if (rc != KC_OK) return rc;
     if (!n) return KC_OK;
     const TableView v = view_of(t);
@@ -1116,7 +1153,8 @@ extern "C" int kc_table_export_device(kc_table *t, uint8_t *d_high, uint64_t hig
     if (!t || !high_bytes || !low_bytes) return KC_EINVAL;
     kc_ctx *c = t->ctx;
     HostCounters hc;
-    int rc = read_counters(t, hc);
+    int rc = kc_table_materialize(t);
+    if (rc == KC_OK) rc = read_counters(t, hc);
     if (rc != KC_OK) return rc;
     if (hc.stat[KC_STAT_ERRORS]) return KC_EFULL;
     const PackSpec ps = pack_spec(t);
@@ -1224,7 +1262,8 @@ int kc_histogram_into(kc_table *t, uint32_t min_freq, uint32_t max_freq, int64_t
 {
     kc_ctx *c = t->ctx;
     HostCounters hc;
-    int rc = read_counters(t, hc);
+    int rc = kc_table_materialize(t);
+    if (rc == KC_OK) rc = read_counters(t, hc);
     if (rc != KC_OK) return rc;
     if (hc.stat[KC_STAT_ERRORS]) return KC_EFULL;
     const uint64_t nb = (uint64_t)max_freq - min_freq + 1;
@@ -1299,7 +1338,8 @@ kc::TableView kc_table_view(kc_table *t) { return view_of(t); }
 int kc_table_low_count(kc_table *t, uint64_t *n_low, bool *list_active)
 {
     HostCounters hc;
-    int rc = read_counters(t, hc);
+    int rc = kc_table_materialize(t);
+    if (rc == KC_OK) rc = read_counters(t, hc);
     if (rc != KC_OK) return rc;
     if (hc.stat[KC_STAT_ERRORS]) return KC_EFULL;
     *list_active = low_list_active(t);

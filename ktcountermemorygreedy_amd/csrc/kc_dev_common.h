@@ -14,9 +14,10 @@ constexpr uint32_t kValidMask = (1u << 0) | (1u << 2) | (1u << 6) | (1u << 19) |
 
 // The table is cut into 2^part_bits partitions of part_slots slots (and
 // the Bloom filter into partitions of bloom_part blocks).  A key's partition
-// is the top part_bits of its XXH3_64 hash; inside the partition the slot
-// probe starts at the hash's low bits and wraps within the partition, and
-// the Bloom block is the reference's h1 & (blocks - 1) within the partition.
+// is the top part_bits of a multiply-shift mix of the key; inside the
+// partition the slot probe starts at bits of XXH3_128 and wraps within the
+// partition, and the Bloom block is the reference's h1 & (blocks - 1) within
+// the partition.
 struct TableView {
     uint64_t *slots;
     uint64_t mask;               // total slots - 1
@@ -52,9 +53,11 @@ __device__ __forceinline__ bool export_filtered(uint32_t c, const TableView &tv)
 
 // A table of 2^L partitions splits L into a coarse digit (the top
 // min(L, 8) bits) and a fine digit (the other L - min(L, 8) bits); the two
-// scatter levels sort by them in turn.  The coarse digit comes from a cheap
-// multiply-shift mix (the two parse passes compute it for every window),
-// the fine digit from XXH3_64 bits below its top byte.
+// scatter levels sort by them in turn.  Both come from one cheap
+// multiply-shift mix of the key (32 bits; L <= 18): the coarse digit is its
+// top bits, the fine digit the bits below.  Its top 18 bits spread the keys
+// of repeat-rich input over the partitions as evenly as XXH3 does
+// (Poisson-distributed loads).
 constexpr uint32_t kCoarseMaxBits = 8;
 
 template <int NW>
@@ -73,16 +76,16 @@ __device__ __forceinline__ uint32_t coarse_bits(uint32_t part_bits)
 
 __device__ __forceinline__ uint32_t coarse_digit(uint32_t mix, uint32_t l1) { return l1 ? mix >> (32 - l1) : 0; }
 
-__device__ __forceinline__ uint32_t fine_digit(uint64_t h64, uint32_t part_bits, uint32_t l2)
+// The fine digit: the l2 bits of the mix below the coarse digit's l1.
+__device__ __forceinline__ uint32_t fine_digit(uint32_t mix, uint32_t l1, uint32_t l2)
 {
-    return l2 ? (uint32_t)(h64 >> (64 - part_bits)) & ((1u << l2) - 1) : 0;
+    return l2 ? (mix >> (32 - l1 - l2)) & ((1u << l2) - 1) : 0;
 }
 
 // Multi-pass counting (tables larger than HBM are counted in 2^pass_bits
 // passes over the input, each holding one hash slice of the k-mers): a key
-// belongs to pass (XXH3_64 >> (64 - pass_bits)).  The fine digit uses
-// XXH3_64 bits below the top byte, so the slices and the partitions are
-// independent.
+// belongs to pass (XXH3_64 >> (64 - pass_bits)), a hash independent of the
+// partition mix, so every slice spreads over all partitions.
 template <int NW>
 __device__ __forceinline__ bool in_pass(const uint64_t *key, uint32_t pass_bits, uint32_t pass)
 {
@@ -93,8 +96,9 @@ template <int NW>
 __device__ __forceinline__ uint64_t part_of_key(const uint64_t *key, uint32_t part_bits)
 {
     const uint32_t l1 = coarse_bits(part_bits), l2 = part_bits - l1;
-    const uint64_t c = coarse_digit(coarse_mix<NW>(key), l1);
-    return l2 ? (c << l2) | fine_digit(xxh3_64<NW>(key), part_bits, l2) : c;
+    const uint32_t mix = coarse_mix<NW>(key);
+    const uint64_t c = coarse_digit(mix, l1);
+    return l2 ? (c << l2) | fine_digit(mix, l1, l2) : c;
 }
 
 // Where a key lives.  Its partition is (coarse digit, fine digit) as
@@ -534,8 +538,11 @@ kc_pipe_shape kc_part_shape(const kc_table *t, const kc_tune &tune, uint64_t key
 int kc_part_reserve(const kc_table *t, kc_part_scratch &s, const kc_pipe_shape &sh);
 // Enqueues the whole pass on st without host synchronisation: over the
 // framed input `seq` (when d_keys is null) or the n_keys keys of d_keys.
+// fresh: the table is lazily cleared (kc_table::fresh) and this pass writes
+// every slot and Bloom word.
 int kc_part_enqueue(const kc_table *t, kc_part_scratch &s, const kc_pipe_shape &sh, const kc::SeqSrc &seq,
-                    const uint64_t *d_keys, uint64_t n_keys, const kc::TableView &tv, hipStream_t st);
+                    const uint64_t *d_keys, uint64_t n_keys, const kc::TableView &tv, hipStream_t st,
+                    bool fresh = false);
 int kc_part_count(kc_table *t, const uint8_t *d_seq, uint64_t nbytes, uint64_t emit_from, const uint64_t *d_keys,
                   uint64_t n_keys, const kc::TableView &tv);
 // Direct path (kc_count.hip): the fused count kernel with `grid` blocks.

@@ -60,7 +60,7 @@ struct kc_pipe_shape {
     uint32_t front_blocks;  // persistent blocks of the parse / scatter front end
     uint32_t fine_slices;   // blocks per coarse region in the second scatter level
     uint64_t key_cap;
-    bool stage_first;       // partition pass: FIRSTs staged per wave (else appended per wave step)
+    uint32_t first_mode;    // partition pass FIRST hand-over: 1 staged, 2 per wave step, 3 per wave group of steps
 };
 
 // Device scratch of the partitioned counting pass, sized for a shape.
@@ -69,8 +69,9 @@ struct kc_part_scratch {
     uint32_t *m1 = nullptr, *m2 = nullptr;          // digit counts per block / slice
     uint64_t *o1 = nullptr, *o2 = nullptr;          // their exclusive offsets
     uint64_t *sums = nullptr, *starts = nullptr, *part_off = nullptr;
+    uint8_t *dig = nullptr;                         // each key's fine digit (u8, or u16 above 256 digits)
     uint64_t key_cap = 0;
-    uint64_t m1_n = 0, m2_n = 0, sums_n = 0, part_n = 0;
+    uint64_t m1_n = 0, m2_n = 0, sums_n = 0, part_n = 0, dig_bytes = 0;
 };
 
 struct kc_table {
@@ -114,11 +115,19 @@ struct kc_table {
     // test-only fault injection (KC_TEST_SKEW_COUNTS): skews one digit count
     // of the partitioned pass so that the scatter guards must report it
     uint32_t fault_skew;
+    // lazy clear: the table is logically empty but its slots and Bloom filter
+    // have not been written; the next partitioned pass initialises them, and
+    // any other access writes them first (kc_table_materialize)
+    bool fresh;
 };
 
 // Every call that changes a table's counts marks the table (read-only
 // indexes built before are then rebuilt on their next use).
 inline void kc_table_touch(kc_table *t) { ++t->epoch; }
+// Writes the empty slots and Bloom filter of a lazily cleared table (no-op
+// otherwise); every access to the table other than a fresh partitioned pass
+// comes after it.
+int kc_table_materialize(kc_table *t);
 
 // Slots / Bloom blocks of one partition that the LDS counting pass holds.
 constexpr uint32_t kc_lds_part_slots(uint32_t nw, bool bloom) { return (nw == 1 ? 4096u : 2048u) << (bloom ? 0 : 1); }

@@ -3,9 +3,9 @@
 // (NewKmerTree.h:526-570).  Where the reference moves 4 KiB blocks down a
 // 256 x 16 x 16 x 16 tree by the top bits of the key, this pass sends every
 // k-mer to one of 2^L table partitions by hash digits (a coarse digit of up
-// to 8 bits from a multiply-shift mix, a fine digit from XXH3_64), in two
-// exact scatter levels, and then counts
-// each partition in an LDS-resident copy of its sub-table (and Bloom part):
+// to 8 bits and a fine digit from the bits below it, of one multiply-shift
+// mix of the key), in two exact scatter levels, and then counts each
+// partition in an LDS-resident copy of its sub-table (and Bloom part):
 //
 //   front count    G persistent blocks; block b parses tiles b, b+G, ... and
 //                  builds an LDS histogram of the coarse digit (top L1 <= 8
@@ -42,13 +42,9 @@ constexpr int kKeysPerThread = 16;
 constexpr uint64_t kKeyTile = (uint64_t)kKeyBlock * kKeysPerThread;  // keys per front-end tile
 constexpr int kPartThreads = 1024;
 // How the partition pass hands FIRST occurrences to the FIRST list by
-// default (kc_tune.first_mode 0): staged per wave for one-word keys (fewer
-// cursor atomics; faster on repeat-rich input, 37.0 vs 38.3 ms per count
-// call), appended per wave step for two-word keys (moves 16-byte FIRSTs once;
-// 57 vs 70 ms on 1.9e9 63-mers, 86% FIRSTs).  FIRST-rich one-word input is
-// faster appended (36.4 vs 38.2 ms at 73% FIRSTs): the autotuner tries both.
-constexpr bool kStageFirst1 = true;
-constexpr bool kStageFirst2 = false;
+// default (kc_tune.first_mode 0): appended per wave every few steps from
+// registers (kFirstGroup below); the autotuner also tries the other modes.
+constexpr uint32_t kFirstModeDefault = 3;
 // Steps of keys the partition pass keeps in flight ahead of the one counted.
 constexpr int kPartPrefetch = 1;
 // Front scatter blocks of one-word keys: 256 threads (the count pass's
@@ -81,7 +77,20 @@ __device__ __forceinline__ uint32_t coarse_of(const uint64_t *key, const Digits 
     return coarse_digit(coarse_mix<NW>(key), d.l1);
 }
 
-__device__ __forceinline__ uint32_t fine_of(uint64_t h, const Digits &d) { return fine_digit(h, d.part_bits, d.l2); }
+template <int NW>
+__device__ __forceinline__ uint32_t fine_of(const uint64_t *key, const Digits &d)
+{
+    return fine_digit(coarse_mix<NW>(key), d.l1, d.l2);
+}
+
+// The front scatters write each key's fine digit beside it (u8, or u16 above
+// 256 fine digits), so that the fine count reads 1-2 bytes per key instead
+// of the key.  dig is null when there is no fine level.
+__device__ __forceinline__ void put_digit(uint8_t *dig, uint64_t at, uint32_t f, const Digits &d)
+{
+    if (d.l2 <= 8) dig[at] = (uint8_t)f;
+    else reinterpret_cast<uint16_t *>(dig)[at] = (uint16_t)f;
+}
 
 // Exclusive scan of one value per thread of a 256-thread block: a shuffle
 // scan inside each wave, then the four wave totals (two barriers).
@@ -179,7 +188,7 @@ template <int NW, int TB, int CB, bool PASS>
 __global__ __launch_bounds__(TB) void k_front_scatter_seq_staged(SeqSrc src, uint32_t k, Digits d,
                                                                  const uint64_t *__restrict__ o1,
                                                                  const uint32_t *__restrict__ m1,
-                                                                 uint64_t *__restrict__ out,
+                                                                 uint64_t *__restrict__ out, uint8_t *__restrict__ dig,
                                                                  unsigned long long *counters)
 {
     constexpr int HW = NW == 1 ? 8 : 16;
@@ -280,6 +289,7 @@ __global__ __launch_bounds__(TB) void k_front_scatter_seq_staged(SeqSrc src, uin
                 } else {
                     *reinterpret_cast<ulonglong2 *>(out + at * 2) = *reinterpret_cast<const ulonglong2 *>(stage + i * 2);
                 }
+                if (dig) put_digit(dig, at, fine_of<NW>(stage + i * NW, d), d);
             }
             __syncthreads();
             if (t < 256) {
@@ -316,7 +326,7 @@ template <int NW>
 __global__ __launch_bounds__(kKeyBlock) void k_front_scatter_keys(const uint64_t *__restrict__ keys, uint64_t n,
                                                                   Digits d, const uint64_t *__restrict__ o1,
                                                                   const uint32_t *__restrict__ m1,
-                                                                  uint64_t *__restrict__ out,
+                                                                  uint64_t *__restrict__ out, uint8_t *__restrict__ dig,
                                                                   unsigned long long *counters)
 {
     __shared__ unsigned long long cur[256], lend[256];
@@ -341,6 +351,7 @@ __global__ __launch_bounds__(kKeyBlock) void k_front_scatter_keys(const uint64_t
                 }
 #pragma unroll
                 for (int w = 0; w < NW; ++w) out[pos * NW + w] = keys[i * NW + w];
+                if (dig) put_digit(dig, pos, fine_of<NW>(keys + i * NW, d), d);
             }
         }
     }
@@ -391,32 +402,46 @@ __device__ __forceinline__ void slice_of(const uint64_t *starts, uint32_t S, uin
     end = b0 + len * (s + 1) / S;
 }
 
-template <int NW>
-__global__ __launch_bounds__(kKeyBlock) void k_fine_count(const uint64_t *__restrict__ keys,
-                                                          const uint64_t *__restrict__ starts, Digits d, uint32_t S,
-                                                          uint32_t *__restrict__ m2)
+// Fine count over the digits the front scatter wrote: 16-byte loads (16 u8
+// or 8 u16 digits), U of them per thread in flight, LDS histogram.
+template <typename DT>
+__global__ __launch_bounds__(kKeyBlock) void k_fine_count_dig(const DT *__restrict__ dig,
+                                                              const uint64_t *__restrict__ starts, Digits d,
+                                                              uint32_t S, uint32_t *__restrict__ m2)
 {
     __shared__ uint32_t hist[1024];
     uint32_t c, s;
     uint64_t beg, end;
     slice_of(starts, S, c, s, beg, end);
-    const uint32_t nf = 1u << d.l2;
+    const uint32_t nf = 1u << d.l2, fmask = nf - 1;  // the mask keeps any stored value inside hist
     for (uint32_t f = threadIdx.x; f < nf; f += kKeyBlock) hist[f] = 0;
     __syncthreads();
-    // U keys per thread per step, all loads issued before the first use
-    constexpr uint32_t U = 16;
-    for (uint64_t i0 = beg; i0 < end; i0 += (uint64_t)U * kKeyBlock) {
-        uint64_t key[U][NW];
+    constexpr uint32_t PER = 16 / sizeof(DT);  // digits per 16-byte load
+    const uint64_t a0 = (beg + PER - 1) / PER * PER < end ? (beg + PER - 1) / PER * PER : end;
+    const uint64_t a1 = end / PER * PER > a0 ? end / PER * PER : a0;
+    for (uint64_t i = beg + threadIdx.x; i < a0; i += kKeyBlock) atomicAdd(&hist[dig[i] & fmask], 1u);
+    for (uint64_t i = a1 + threadIdx.x; i < end; i += kKeyBlock) atomicAdd(&hist[dig[i] & fmask], 1u);
+    const uint4 *v = reinterpret_cast<const uint4 *>(dig);
+    const uint64_t q0 = a0 / PER, q1 = a1 / PER;
+    constexpr uint32_t U = 4;
+    for (uint64_t qb = q0; qb < q1; qb += (uint64_t)U * kKeyBlock) {
+        uint4 x[U];
 #pragma unroll
         for (uint32_t u = 0; u < U; ++u) {
-            const uint64_t i = i0 + u * kKeyBlock + threadIdx.x;
-            const uint64_t *src = keys + (i < end ? i : beg) * NW;
-#pragma unroll
-            for (int w = 0; w < NW; ++w) key[u][w] = src[w];
+            const uint64_t q = qb + u * kKeyBlock + threadIdx.x;
+            x[u] = v[q < q1 ? q : q0];
         }
 #pragma unroll
-        for (uint32_t u = 0; u < U; ++u)
-            if (i0 + u * kKeyBlock + threadIdx.x < end) atomicAdd(&hist[fine_of(xxh3_64<NW>(key[u]), d)], 1u);
+        for (uint32_t u = 0; u < U; ++u) {
+            if (qb + u * kKeyBlock + threadIdx.x >= q1) continue;
+            const uint32_t w[4] = {x[u].x, x[u].y, x[u].z, x[u].w};
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+#pragma unroll
+                for (uint32_t b = 0; b < 4 / sizeof(DT); ++b)
+                    atomicAdd(&hist[(w[j] >> (8 * sizeof(DT) * b)) & fmask], 1u);
+            }
+        }
     }
     __syncthreads();
     const uint64_t row = (uint64_t)c << d.l2;
@@ -486,7 +511,7 @@ __global__ __launch_bounds__(TB) void k_fine_scatter_staged(const uint64_t *__re
         for (uint32_t j = 0; j < (uint32_t)FPT; ++j) {
             const uint32_t i = j * TB + t;
             if (i < cn) {
-                bin[j] = fine_of(xxh3_64<NW>(key[j]), d);
+                bin[j] = fine_of<NW>(key[j], d);
                 rank[j] = atomicAdd(&hist[bin[j]], 1u);
             }
         }
@@ -519,7 +544,7 @@ __global__ __launch_bounds__(TB) void k_fine_scatter_staged(const uint64_t *__re
         for (uint32_t p = t; p < cn; p += TB) {
             uint32_t f;
             if constexpr (SB) f = sbin[p];
-            else f = fine_of(xxh3_64<NW>(stage + p * NW), d);
+            else f = fine_of<NW>(stage + p * NW, d);
             const uint32_t r = p - loff[f];
             if (r >= left[f]) {  // past this slice's region (scatter guard)
                 ++n_err;
@@ -639,16 +664,27 @@ __device__ __forceinline__ uint64_t wave_pos(uint64_t beg, uint32_t wave, uint32
     return beg + (uint64_t)(n >> 6) * kPartThreads + wave * 64u + (n & 63u);
 }
 
-// FIRST occurrences, STAGE = true: staged in `first_out` (the pipeline's
-// other key array, idle during this pass) at the positions of the wave's own
-// keys: the wave's n-th FIRST goes where its n-th key was, so runs of
-// different waves and partitions never overlap and no atomic is needed while
-// counting.  After the loop each wave moves its run to the FIRST list in
-// pieces of up to 1024 keys, one cursor atomic per piece, on segments that
-// rotate with the partition, the wave and the piece.  STAGE = false: each
-// wave step appends its FIRSTs to the FIRST list at once (one cursor atomic
-// per wave step), which moves the FIRST bytes once instead of three times.
-template <int NW, bool BLOOM, bool STAGE>
+// FIRST occurrences (the first occurrence of a key, for the FIRST list):
+//   FM = kFirstStaged: staged in `first_out` (the pipeline's other key
+//     array, idle during this pass) at the positions of the wave's own keys:
+//     the wave's n-th FIRST goes where its n-th key was, so runs of different
+//     waves and partitions never overlap and no atomic is needed while
+//     counting.  After the loop each wave moves its run to the FIRST list in
+//     pieces of up to 1024 keys, one cursor atomic per piece.
+//   FM = kFirstStep: each wave step appends its FIRSTs at once (one cursor
+//     atomic per wave step).
+//   FM = kFirstGroup: each lane keeps its FIRSTs of kGroup consecutive
+//     steps in registers and the wave appends them together (one cursor
+//     atomic and one contiguous run per kGroup steps): the FIRST bytes move
+//     once, with few atomics and no LDS.
+// Segments rotate with the partition, the wave and the step or piece.
+// FRESH: the table is logically empty (a lazy clear): partitions start from
+// empty slots and zero Bloom words in LDS instead of loading them, and every
+// slot and Bloom word is written back, also for partitions with no keys.
+constexpr int kFirstStaged = 1, kFirstStep = 2, kFirstGroup = 3;
+template <int NW> constexpr int kGroup = 4;  // 4 steps: <= 64 VGPRs, two 1024-thread blocks per CU
+
+template <int NW, bool BLOOM, int FM, bool FRESH>
 __global__ __launch_bounds__(kPartThreads) void k_count_parts(const uint64_t *__restrict__ keys,
                                                               const uint64_t *__restrict__ part_off, TableView tv,
                                                               uint64_t *__restrict__ first_out)
@@ -658,13 +694,18 @@ __global__ __launch_bounds__(kPartThreads) void k_count_parts(const uint64_t *__
     __shared__ PartLds<NW, BLOOM> S;
     const uint64_t part = blockIdx.x;
     const uint64_t beg = part_off[part], end = part_off[part + 1];
-    if (beg == end) return;  // nothing to count: the partition stays as it is
+    if (!FRESH && beg == end) return;  // nothing to count: the partition stays as it is
     const uint32_t psz = 1u << tv.part_log2;
     const uint64_t sbase = part << tv.part_log2;
     for (uint32_t s = threadIdx.x; s < psz; s += kPartThreads) {
-        const uint64_t *g = tv.slots + (sbase + s) * SW;
-        S.key[s] = g[0];
-        if constexpr (NW == 2) S.key[PS + s] = g[1];
+        if constexpr (FRESH) {
+            S.key[s] = KC_EMPTY;
+            if constexpr (NW == 2) S.key[PS + s] = KC_EMPTY;
+        } else {
+            const uint64_t *g = tv.slots + (sbase + s) * SW;
+            S.key[s] = g[0];
+            if constexpr (NW == 2) S.key[PS + s] = g[1];
+        }
         S.cnt[s] = 0;
     }
     uint32_t bsz = 0;
@@ -672,13 +713,51 @@ __global__ __launch_bounds__(kPartThreads) void k_count_parts(const uint64_t *__
     if constexpr (BLOOM) {
         bsz = 1u << tv.bloom_part_log2;
         bbase = part << tv.bloom_part_log2;
-        for (uint32_t b = threadIdx.x; b < 2 * bsz; b += kPartThreads) S.bloom[b] = tv.bloom[2 * bbase + b];
+        for (uint32_t b = threadIdx.x; b < 2 * bsz; b += kPartThreads) S.bloom[b] = FRESH ? 0 : tv.bloom[2 * bbase + b];
     }
     __syncthreads();
 
     uint32_t n_k = 0, n_new = 0, n_err = 0;
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     uint32_t fcur = 0;  // FIRSTs staged by this wave (wave-uniform)
+    // kFirstGroup: this lane's FIRSTs of the group's steps and the waves' ballots
+    constexpr int G = FM == kFirstGroup ? kGroup<NW> : 1;
+    uint64_t fk[G][NW];
+    uint64_t fbs[G];
+    uint32_t held = 0, group = 0;  // steps held (wave-uniform), groups appended
+    auto append_group = [&]() {
+        uint32_t tot = 0;
+#pragma unroll
+        for (int j = 0; j < G; ++j)
+            if ((uint32_t)j < held) tot += (uint32_t)__popcll(fbs[j]);
+        if (tot) {
+            const uint32_t seg = first_segment((uint32_t)part, wave, group);
+            unsigned long long at0 = 0;
+            if (lane == 0) at0 = atomicAdd(tv.counters + KC_CURSOR_BASE + (uint64_t)seg * KC_LINE_WORDS, (unsigned long long)tot);
+            at0 = __shfl(at0, 0, 64);
+            uint32_t pre = 0;
+#pragma unroll
+            for (int j = 0; j < G; ++j) {
+                if ((uint32_t)j >= held) break;
+                if ((fbs[j] >> lane) & 1) {
+                    const uint64_t at = at0 + pre + (uint32_t)__popcll(fbs[j] & lanemask_lt());
+                    if (at < tv.seg_cap) {
+                        uint64_t *dst = tv.low + ((uint64_t)seg * tv.seg_cap + at) * NW;
+#pragma unroll
+                        for (int w = 0; w < NW; ++w) dst[w] = fk[j][w];
+                    } else {
+                        // no room on the list: counted in the partition's table instead
+                        uint64_t lo, hi;
+                        xxh3_128<NW>(fk[j], lo, hi);
+                        lds_insert<NW, BLOOM>(S, fk[j], hi, psz - 1, n_new, n_err);
+                    }
+                }
+                pre += (uint32_t)__popcll(fbs[j]);
+            }
+        }
+        ++group;
+        held = 0;
+    };
     // the keys of the next PF steps are in flight while this step's key is
     // counted (indexes past the end read the partition's first key)
     constexpr int PF = kPartPrefetch;
@@ -689,44 +768,56 @@ __global__ __launch_bounds__(kPartThreads) void k_count_parts(const uint64_t *__
 #pragma unroll
         for (int w = 0; w < NW; ++w) nxt[p][w] = keys[(i < end ? i : beg) * NW + w];
     }
-    for (uint64_t base = beg; base < end; base += kPartThreads) {
-        const uint64_t i = base + threadIdx.x;
-        const bool valid = i < end;
-        uint64_t key[NW];
+    for (uint64_t gbase = beg; gbase < end; gbase += (uint64_t)G * kPartThreads) {
 #pragma unroll
-        for (int w = 0; w < NW; ++w) key[w] = nxt[0][w];
+        for (int j = 0; j < G; ++j) {
+            const uint64_t base = gbase + (uint64_t)j * kPartThreads;
+            if (base >= end) break;
+            const uint64_t i = base + threadIdx.x;
+            const bool valid = i < end;
+            uint64_t key[NW];
 #pragma unroll
-        for (int p = 0; p + 1 < PF; ++p)
+            for (int w = 0; w < NW; ++w) key[w] = nxt[0][w];
 #pragma unroll
-            for (int w = 0; w < NW; ++w) nxt[p][w] = nxt[p + 1][w];
-        if (base + (uint64_t)PF * kPartThreads < end) {
-            const uint64_t j = i + (uint64_t)PF * kPartThreads;
+            for (int p = 0; p + 1 < PF; ++p)
 #pragma unroll
-            for (int w = 0; w < NW; ++w) nxt[PF - 1][w] = keys[(j < end ? j : beg) * NW + w];
-        }
-        n_k += valid;
-        uint64_t lo, hi;  // one XXH3_128: Bloom probe and slot probe start
-        xxh3_128<NW>(key, lo, hi);
-        bool to_table = valid;
-        if constexpr (BLOOM) {
-            const bool first = valid && lds_bloom<NW, BLOOM>(S, lo, hi, bsz - 1) == 0;
-            if constexpr (STAGE) {
-                const uint64_t fb = __ballot(first);
-                if (first) {
-                    uint64_t *dst = first_out + wave_pos(beg, wave, fcur + (uint32_t)__popcll(fb & lanemask_lt())) * NW;
+                for (int w = 0; w < NW; ++w) nxt[p][w] = nxt[p + 1][w];
+            if (base + (uint64_t)PF * kPartThreads < end) {
+                const uint64_t jj = i + (uint64_t)PF * kPartThreads;
 #pragma unroll
-                    for (int w = 0; w < NW; ++w) dst[w] = key[w];
-                }
-                fcur += (uint32_t)__popcll(fb);
-                to_table = valid && !first;
-            } else {
-                const bool kept = first_append<NW>(tv, first, key, (uint32_t)((base - beg) >> 10));
-                to_table = valid && (!first || !kept);  // no room on the list: counted in the table
+                for (int w = 0; w < NW; ++w) nxt[PF - 1][w] = keys[(jj < end ? jj : beg) * NW + w];
             }
+            n_k += valid;
+            uint64_t lo, hi;  // one XXH3_128: Bloom probe and slot probe start
+            xxh3_128<NW>(key, lo, hi);
+            bool to_table = valid;
+            if constexpr (BLOOM) {
+                const bool first = valid && lds_bloom<NW, BLOOM>(S, lo, hi, bsz - 1) == 0;
+                if constexpr (FM == kFirstStaged) {
+                    const uint64_t fb = __ballot(first);
+                    if (first) {
+                        uint64_t *dst = first_out + wave_pos(beg, wave, fcur + (uint32_t)__popcll(fb & lanemask_lt())) * NW;
+#pragma unroll
+                        for (int w = 0; w < NW; ++w) dst[w] = key[w];
+                    }
+                    fcur += (uint32_t)__popcll(fb);
+                    to_table = valid && !first;
+                } else if constexpr (FM == kFirstStep) {
+                    const bool kept = first_append<NW>(tv, first, key, (uint32_t)((base - beg) >> 10));
+                    to_table = valid && (!first || !kept);  // no room on the list: counted in the table
+                } else {
+                    fbs[j] = __ballot(first);
+#pragma unroll
+                    for (int w = 0; w < NW; ++w) fk[j][w] = key[w];
+                    held = (uint32_t)j + 1;
+                    to_table = valid && !first;
+                }
+            }
+            if (to_table) lds_insert<NW, BLOOM>(S, key, hi, psz - 1, n_new, n_err);
         }
-        if (to_table) lds_insert<NW, BLOOM>(S, key, hi, psz - 1, n_new, n_err);
+        if constexpr (BLOOM && FM == kFirstGroup) append_group();
     }
-    if constexpr (BLOOM && STAGE) {
+    if constexpr (BLOOM && FM == kFirstStaged) {
         __threadfence_block();  // the wave's staged FIRSTs are read back by other lanes
         for (uint32_t p0 = 0; p0 < fcur; p0 += 1024) {
             const uint32_t cnt = fcur - p0 < 1024 ? fcur - p0 : 1024;
@@ -756,11 +847,20 @@ __global__ __launch_bounds__(kPartThreads) void k_count_parts(const uint64_t *__
     __syncthreads();
     for (uint32_t s = threadIdx.x; s < psz; s += kPartThreads) {
         const uint32_t c = S.cnt[s];
-        if (!c) continue;
         uint64_t *g = tv.slots + (sbase + s) * SW;
-        g[0] = S.key[s];
-        if constexpr (NW == 2) g[1] = S.key[PS + s];
-        g[NW == 1 ? 1 : 2] += c;
+        if constexpr (FRESH) {  // every slot: the memory held no table before
+            g[0] = S.key[s];
+            if constexpr (NW == 2) {
+                g[1] = S.key[PS + s];
+                g[3] = 0;
+            }
+            g[NW == 1 ? 1 : 2] = c;
+        } else {
+            if (!c) continue;
+            g[0] = S.key[s];
+            if constexpr (NW == 2) g[1] = S.key[PS + s];
+            g[NW == 1 ? 1 : 2] += c;
+        }
     }
     if constexpr (BLOOM) {
         for (uint32_t b = threadIdx.x; b < 2 * bsz; b += kPartThreads) tv.bloom[2 * bbase + b] = S.bloom[b];
@@ -807,7 +907,7 @@ bool kc_part_eligible(const kc_table *t)
 void kc_part_scratch_free(kc_part_scratch &s)
 {
     for (void *p : {(void *)s.keys_a, (void *)s.keys_b, (void *)s.m1, (void *)s.o1, (void *)s.m2, (void *)s.o2,
-                    (void *)s.sums, (void *)s.part_off, (void *)s.starts})
+                    (void *)s.sums, (void *)s.part_off, (void *)s.starts, (void *)s.dig})
         if (p) (void)hipFree(p);
     s = kc_part_scratch();
 }
@@ -825,7 +925,7 @@ kc_pipe_shape kc_part_shape(const kc_table *t, const kc_tune &tune, uint64_t key
     sh.front_blocks = (g + 63) / 64 * 64;
     sh.fine_slices = std::max<uint32_t>(1, std::min<uint32_t>(tune.fine_slices ? tune.fine_slices : 8, 64));
     sh.key_cap = key_cap;
-    sh.stage_first = tune.first_mode == 1 || (tune.first_mode != 2 && (t->nw == 1 ? kStageFirst1 : kStageFirst2));
+    sh.first_mode = tune.first_mode >= 1 && tune.first_mode <= 3 ? tune.first_mode : kFirstModeDefault;
     return sh;
 }
 
@@ -848,14 +948,17 @@ int kc_part_reserve(const kc_table *t, kc_part_scratch &s, const kc_pipe_shape &
         return true;
     };
     uint64_t kc_a = s.key_cap, kc_b = s.key_cap, m1c = s.m1_n, o1c = s.m1_n, m2c = s.m2_n, o2c = s.m2_n;
-    uint64_t sc = s.sums_n, pc = s.part_n, stc = s.starts ? 257 : 0;
+    uint64_t sc = s.sums_n, pc = s.part_n, stc = s.starts ? 257 : 0, dgc = s.dig_bytes;
     const uint64_t keys_words = std::max<uint64_t>(sh.key_cap, 1) * nw;
+    // the digit array (+16: the fine count's last 16-byte load may reach past the final digit)
+    const uint64_t dig_need = d.l2 ? std::max<uint64_t>(sh.key_cap, 1) * (d.l2 > 8 ? 2 : 1) + 16 : 0;
     kc_a *= nw;
     kc_b *= nw;
     const bool ok = grow(s.keys_a, kc_a, keys_words) && grow(s.keys_b, kc_b, keys_words) &&
                     grow(s.m1, m1c, m1n) && grow(s.o1, o1c, m1n) && (d.l2 == 0 || grow(s.m2, m2c, m2n)) &&
                     (d.l2 == 0 || grow(s.o2, o2c, m2n)) && grow(s.sums, sc, sumsn) &&
-                    grow(s.part_off, pc, n_parts + 1) && grow(s.starts, stc, 257);
+                    grow(s.part_off, pc, n_parts + 1) && grow(s.starts, stc, 257) &&
+                    (d.l2 == 0 || grow(s.dig, dgc, dig_need));
     if (!ok) {
         kc_part_scratch_free(s);
         return KC_ENOMEM;
@@ -865,11 +968,12 @@ int kc_part_reserve(const kc_table *t, kc_part_scratch &s, const kc_pipe_shape &
     s.m2_n = d.l2 ? std::min(m2c, o2c) : s.m2_n;
     s.sums_n = sc;
     s.part_n = pc;
+    s.dig_bytes = dgc;
     return KC_OK;
 }
 
 int kc_part_enqueue(const kc_table *t, kc_part_scratch &s, const kc_pipe_shape &sh, const SeqSrc &seq,
-                    const uint64_t *d_keys, uint64_t n_keys, const TableView &tv, hipStream_t st)
+                    const uint64_t *d_keys, uint64_t n_keys, const TableView &tv, hipStream_t st, bool fresh)
 {
     const int nw = (int)t->nw;
     const Digits d = digits_of(t);
@@ -880,9 +984,12 @@ int kc_part_enqueue(const kc_table *t, kc_part_scratch &s, const kc_pipe_shape &
     // the shape the scratch was sized for must cover this launch
     const uint64_t m1n = (uint64_t)n_coarse * G, m2n = d.l2 ? n_parts * S : 0;
     const uint64_t need_keys = from_seq ? seq.n - std::min(seq.n, seq.hdr ? 0 : seq.emit_from) : n_keys;
+    const uint64_t dig_need = d.l2 ? s.key_cap * (d.l2 > 8 ? 2 : 1) + 16 : 0;
     if (s.key_cap < need_keys || s.key_cap < sh.key_cap || s.m1_n < m1n || s.m2_n < m2n ||
-        s.sums_n < kc_scan_tmp_words(std::max(m1n, m2n)) || s.part_n < n_parts + 1 || !s.starts)
+        s.sums_n < kc_scan_tmp_words(std::max(m1n, m2n)) || s.part_n < n_parts + 1 || !s.starts ||
+        (d.l2 && (!s.dig || s.dig_bytes < dig_need)))
         return KC_EINVAL;
+    uint8_t *dig = d.l2 ? s.dig : nullptr;
 
     // level 1: count, scan, coarse starts, scatter
     if (from_seq) {
@@ -907,11 +1014,11 @@ int kc_part_enqueue(const kc_table *t, kc_part_scratch &s, const kc_pipe_shape &
                        : &k_front_scatter_seq_staged<1, kFrontBlock1, kFrontChunk1, false>;
         auto f2 = pass ? &k_front_scatter_seq_staged<2, kFrontBlock2, kFrontChunk2, true>
                        : &k_front_scatter_seq_staged<2, kFrontBlock2, kFrontChunk2, false>;
-        if (nw == 1) hipLaunchKernelGGL(f1, dim3(G), dim3(kFrontBlock1), 0, st, seq, t->cfg.k, d, s.o1, s.m1, s.keys_a, tv.counters);
-        else hipLaunchKernelGGL(f2, dim3(G), dim3(kFrontBlock2), 0, st, seq, t->cfg.k, d, s.o1, s.m1, s.keys_a, tv.counters);
+        if (nw == 1) hipLaunchKernelGGL(f1, dim3(G), dim3(kFrontBlock1), 0, st, seq, t->cfg.k, d, s.o1, s.m1, s.keys_a, dig, tv.counters);
+        else hipLaunchKernelGGL(f2, dim3(G), dim3(kFrontBlock2), 0, st, seq, t->cfg.k, d, s.o1, s.m1, s.keys_a, dig, tv.counters);
     } else {
-        if (nw == 1) hipLaunchKernelGGL(k_front_scatter_keys<1>, dim3(G), dim3(kKeyBlock), 0, st, d_keys, n_keys, d, s.o1, s.m1, s.keys_a, tv.counters);
-        else hipLaunchKernelGGL(k_front_scatter_keys<2>, dim3(G), dim3(kKeyBlock), 0, st, d_keys, n_keys, d, s.o1, s.m1, s.keys_a, tv.counters);
+        if (nw == 1) hipLaunchKernelGGL(k_front_scatter_keys<1>, dim3(G), dim3(kKeyBlock), 0, st, d_keys, n_keys, d, s.o1, s.m1, s.keys_a, dig, tv.counters);
+        else hipLaunchKernelGGL(k_front_scatter_keys<2>, dim3(G), dim3(kKeyBlock), 0, st, d_keys, n_keys, d, s.o1, s.m1, s.keys_a, dig, tv.counters);
     }
     KC_HIP(hipGetLastError());
 
@@ -919,8 +1026,8 @@ int kc_part_enqueue(const kc_table *t, kc_part_scratch &s, const kc_pipe_shape &
     const uint64_t *part_keys = s.keys_a;
     if (d.l2) {
         const dim3 g2(n_coarse * S);
-        if (nw == 1) hipLaunchKernelGGL(k_fine_count<1>, g2, dim3(kKeyBlock), 0, st, s.keys_a, s.starts, d, S, s.m2);
-        else hipLaunchKernelGGL(k_fine_count<2>, g2, dim3(kKeyBlock), 0, st, s.keys_a, s.starts, d, S, s.m2);
+        if (d.l2 <= 8) hipLaunchKernelGGL(k_fine_count_dig<uint8_t>, g2, dim3(kKeyBlock), 0, st, s.dig, s.starts, d, S, s.m2);
+        else hipLaunchKernelGGL(k_fine_count_dig<uint16_t>, g2, dim3(kKeyBlock), 0, st, reinterpret_cast<const uint16_t *>(s.dig), s.starts, d, S, s.m2);
         KC_HIP(hipGetLastError());
         if (t->fault_skew == 3 || t->fault_skew == 4)
             hipLaunchKernelGGL(k_skew_count, dim3(1), dim3(64), 0, st, s.m2, m2n, t->fault_skew == 3 ? -1 : 1);
@@ -944,15 +1051,20 @@ int kc_part_enqueue(const kc_table *t, kc_part_scratch &s, const kc_pipe_shape &
     const bool bloom = t->bloom != nullptr;
     const dim3 g((uint32_t)n_parts);
     uint64_t *first_out = part_keys == s.keys_b ? s.keys_a : s.keys_b;  // the idle key array
-    if (nw == 1) {
-        auto f = sh.stage_first ? &k_count_parts<1, true, true> : &k_count_parts<1, true, false>;
-        if (bloom) hipLaunchKernelGGL(f, g, dim3(kPartThreads), 0, st, part_keys, s.part_off, tv, first_out);
-        else hipLaunchKernelGGL((k_count_parts<1, false, true>), g, dim3(kPartThreads), 0, st, part_keys, s.part_off, tv, first_out);
-    } else {
-        auto f = sh.stage_first ? &k_count_parts<2, true, true> : &k_count_parts<2, true, false>;
-        if (bloom) hipLaunchKernelGGL(f, g, dim3(kPartThreads), 0, st, part_keys, s.part_off, tv, first_out);
-        else hipLaunchKernelGGL((k_count_parts<2, false, true>), g, dim3(kPartThreads), 0, st, part_keys, s.part_off, tv, first_out);
-    }
+    using PartKernel = void (*)(const uint64_t *, const uint64_t *, TableView, uint64_t *);
+    // [nw - 1][fresh][mode]: mode 0 = no Bloom split, 1..3 = the FIRST hand-over of the split
+    static const PartKernel kernels[2][2][4] = {
+        {{&k_count_parts<1, false, kFirstStaged, false>, &k_count_parts<1, true, kFirstStaged, false>,
+          &k_count_parts<1, true, kFirstStep, false>, &k_count_parts<1, true, kFirstGroup, false>},
+         {&k_count_parts<1, false, kFirstStaged, true>, &k_count_parts<1, true, kFirstStaged, true>,
+          &k_count_parts<1, true, kFirstStep, true>, &k_count_parts<1, true, kFirstGroup, true>}},
+        {{&k_count_parts<2, false, kFirstStaged, false>, &k_count_parts<2, true, kFirstStaged, false>,
+          &k_count_parts<2, true, kFirstStep, false>, &k_count_parts<2, true, kFirstGroup, false>},
+         {&k_count_parts<2, false, kFirstStaged, true>, &k_count_parts<2, true, kFirstStaged, true>,
+          &k_count_parts<2, true, kFirstStep, true>, &k_count_parts<2, true, kFirstGroup, true>}}};
+    const uint32_t mode = bloom ? std::min<uint32_t>(std::max<uint32_t>(sh.first_mode, 1), 3) : 0;
+    hipLaunchKernelGGL(kernels[nw - 1][fresh ? 1 : 0][mode], g, dim3(kPartThreads), 0, st, part_keys, s.part_off, tv,
+                       first_out);
     KC_HIP(hipGetLastError());
     return KC_OK;
 }
@@ -970,6 +1082,7 @@ constexpr uint64_t kPartHalo = 64;
 int kc_part_count(kc_table *t, const uint8_t *d_seq, uint64_t nbytes, uint64_t emit_from, const uint64_t *d_keys,
                   uint64_t n_keys, const TableView &tv)
 {
+    // a lazily cleared table (t->fresh) is written whole by the first piece's pass
     const bool from_seq = d_keys == nullptr;
     const uint64_t total = from_seq ? nbytes : n_keys;
     uint64_t pos = from_seq ? std::min(nbytes, emit_from) : 0;
@@ -984,14 +1097,15 @@ int kc_part_count(kc_table *t, const uint8_t *d_seq, uint64_t nbytes, uint64_t e
             const kc_pipe_shape sh = kc_part_shape(t, t->tune, take);
             if ((rc = kc_part_reserve(t, t->ps, sh)) != KC_OK) return rc;
             SeqSrc src{d_seq + pos - halo, take + halo, halo, nullptr};
-            rc = kc_part_enqueue(t, t->ps, sh, src, nullptr, 0, tv, t->ctx->stream);
+            rc = kc_part_enqueue(t, t->ps, sh, src, nullptr, 0, tv, t->ctx->stream, t->fresh);
         } else {
             const kc_pipe_shape sh = kc_part_shape(t, t->tune, take);
             if ((rc = kc_part_reserve(t, t->ps, sh)) != KC_OK) return rc;
             SeqSrc none{nullptr, 0, 0, nullptr};
-            rc = kc_part_enqueue(t, t->ps, sh, none, d_keys + pos * t->nw, take, tv, t->ctx->stream);
+            rc = kc_part_enqueue(t, t->ps, sh, none, d_keys + pos * t->nw, take, tv, t->ctx->stream, t->fresh);
         }
         if (rc != KC_OK) return rc;
+        t->fresh = false;  // the first pass wrote every slot
         pos += take;
     }
     return KC_OK;
