@@ -47,7 +47,7 @@ struct Slot {
     uint8_t *pinned = nullptr;  // cap bytes
     uint8_t *dbuf = nullptr;    // cap + 16 bytes on the device
     uint64_t *hdr = nullptr;    // {n, emit_from} on the device
-    hipGraphExec_t exec = nullptr;
+    hipGraphExec_t exec[2] = {nullptr, nullptr};  // [fresh]: see check_table
     hipEvent_t copied = nullptr, done = nullptr;
     uint64_t fill = 0;
     bool used = false;          // copied / done have been recorded
@@ -57,7 +57,7 @@ struct DevGraph {
     const uint8_t *ptr;
     uint64_t cap;
     uint64_t *hdr;
-    hipGraphExec_t exec;
+    hipGraphExec_t exec[2];  // [fresh]
     uint64_t last_use;
 };
 
@@ -86,12 +86,14 @@ namespace {
 void destroy_graphs(kc_batch *b)
 {
     for (uint32_t i = 0; i < b->depth; ++i)
-        if (b->slot[i].exec) {
-            (void)hipGraphExecDestroy(b->slot[i].exec);
-            b->slot[i].exec = nullptr;
-        }
+        for (hipGraphExec_t &e : b->slot[i].exec)
+            if (e) {
+                (void)hipGraphExecDestroy(e);
+                e = nullptr;
+            }
     for (DevGraph &g : b->dev) {
-        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        for (hipGraphExec_t e : g.exec)
+            if (e) (void)hipGraphExecDestroy(e);
         if (g.hdr) (void)hipFree(g.hdr);
     }
     b->dev.clear();
@@ -108,8 +110,9 @@ int ensure_scratch(kc_batch *b, uint64_t key_cap)
     return kc_part_reserve(b->t, b->ps, b->sh);
 }
 
-// Captures the counting pass over seq[0, cap) with its size in *hdr.
-int capture(kc_batch *b, const uint8_t *seq, uint64_t cap, const uint64_t *hdr, hipGraphExec_t *out)
+// Captures the counting pass over seq[0, cap) with its size in *hdr; fresh:
+// the partitioned pass that writes a lazily cleared table whole.
+int capture(kc_batch *b, const uint8_t *seq, uint64_t cap, const uint64_t *hdr, bool fresh, hipGraphExec_t *out)
 {
     const kc::SeqSrc src{seq, cap, 0, hdr};
     KC_HIP(hipStreamBeginCapture(b->cap_st, hipStreamCaptureModeThreadLocal));
@@ -117,7 +120,7 @@ int capture(kc_batch *b, const uint8_t *seq, uint64_t cap, const uint64_t *hdr, 
     if (b->path == KC_PATH_PARTITIONED) {
         kc_pipe_shape sh = b->sh;
         sh.key_cap = cap;
-        rc = kc_part_enqueue(b->t, b->ps, sh, src, nullptr, 0, kc_table_view(b->t), b->cap_st);
+        rc = kc_part_enqueue(b->t, b->ps, sh, src, nullptr, 0, kc_table_view(b->t), b->cap_st, fresh);
     } else {
         rc = kc_direct_enqueue(b->t, src, b->direct_grid, b->cap_st);
     }
@@ -154,11 +157,14 @@ int sync_pass(kc_batch *b)
 }
 
 // Before a launch: the table's counts change (see kc_table_touch), and the
-// graphs must carry the table's current pass.
-int check_table(kc_batch *b)
+// graphs must carry the table's current pass.  A lazily cleared table
+// (kc_table::fresh) is counted by the fresh partitioned graph, which writes
+// it whole; every other graph reads the table, so it is materialized first.
+int check_table(kc_batch *b, bool &fresh)
 {
     kc_table_touch(b->t);
-    const int rc = kc_table_materialize(b->t);  // the captured passes read the table
+    fresh = b->path == KC_PATH_PARTITIONED && b->t->fresh;
+    const int rc = fresh ? KC_OK : kc_table_materialize(b->t);
     return rc != KC_OK ? rc : sync_pass(b);
 }
 
@@ -167,10 +173,11 @@ int launch_slot(kc_batch *b, uint32_t i, uint64_t emit_from)
 {
     Slot &s = b->slot[i];
     if (!s.fill) return KC_OK;
-    int rc = check_table(b);
+    bool fresh = false;
+    int rc = check_table(b, fresh);
     if (rc != KC_OK) return rc;
     if ((rc = ensure_scratch(b, b->cap)) != KC_OK) return rc;
-    if (!s.exec && (rc = capture(b, s.dbuf, b->cap, s.hdr, &s.exec)) != KC_OK) return rc;
+    if (!s.exec[fresh] && (rc = capture(b, s.dbuf, b->cap, s.hdr, fresh, &s.exec[fresh])) != KC_OK) return rc;
     hipStream_t ts = b->t->ctx->stream;
     // the device buffer is free once the previous launch of this slot is done
     if (s.used) KC_HIP(hipStreamWaitEvent(b->copy_st, s.done, 0));
@@ -179,7 +186,8 @@ int launch_slot(kc_batch *b, uint32_t i, uint64_t emit_from)
     KC_HIP(hipStreamWaitEvent(ts, s.copied, 0));
     hipLaunchKernelGGL(k_set_hdr, dim3(1), dim3(64), 0, ts, s.hdr, s.fill, emit_from);
     KC_HIP(hipGetLastError());
-    KC_HIP(hipGraphLaunch(s.exec, ts));
+    KC_HIP(hipGraphLaunch(s.exec[fresh], ts));
+    if (fresh) b->t->fresh = false;  // this pass wrote every slot
     KC_HIP(hipEventRecord(s.done, ts));
     s.used = true;
     ++b->info.launches;
@@ -339,7 +347,8 @@ extern "C" int kc_batch_add(kc_batch *b, const uint8_t *h_chunk, uint64_t len)
 {
     if (!b || (!h_chunk && len)) return KC_EINVAL;
     if (!len) return KC_OK;
-    int rc = check_table(b);
+    bool fresh;
+    int rc = check_table(b, fresh);
     if (rc != KC_OK) return rc;
     KC_HIP(hipSetDevice(b->t->ctx->device));
     ++b->info.chunks;
@@ -381,7 +390,8 @@ extern "C" int kc_batch_add_chunks(kc_batch *b, const uint8_t *h_base, const uin
     }
     if (prev && !h_base) return KC_EINVAL;
     if (!prev) return KC_OK;
-    int rc = check_table(b);
+    bool fresh;
+    int rc = check_table(b, fresh);
     if (rc != KC_OK) return rc;
     KC_HIP(hipSetDevice(b->t->ctx->device));
     uint64_t i = 0, pos = 0;  // chunk i is h_base[pos, ends[i])
@@ -447,9 +457,10 @@ extern "C" int kc_batch_launch_device(kc_batch *b, const uint8_t *d_seq, uint64_
     if (!b || (!d_seq && nbytes)) return KC_EINVAL;
     if (reinterpret_cast<uintptr_t>(d_seq) % 16) return KC_EINVAL;
     if (!nbytes) return KC_OK;
-    int rc = check_table(b);
-    if (rc != KC_OK) return rc;
     KC_HIP(hipSetDevice(b->t->ctx->device));
+    bool fresh = false;
+    int rc = check_table(b, fresh);
+    if (rc != KC_OK) return rc;
     if ((rc = ensure_scratch(b, nbytes)) != KC_OK) return rc;
     DevGraph *g = nullptr;
     for (DevGraph &x : b->dev)
@@ -459,24 +470,23 @@ extern "C" int kc_batch_launch_device(kc_batch *b, const uint8_t *d_seq, uint64_
             auto lru = std::min_element(b->dev.begin(), b->dev.end(),
                                         [](const DevGraph &a, const DevGraph &c) { return a.last_use < c.last_use; });
             KC_HIP(hipStreamSynchronize(b->t->ctx->stream));
-            (void)hipGraphExecDestroy(lru->exec);
+            for (hipGraphExec_t e : lru->exec)
+                if (e) (void)hipGraphExecDestroy(e);
             (void)hipFree(lru->hdr);
             b->dev.erase(lru);
         }
-        DevGraph ng{d_seq, nbytes, nullptr, nullptr, 0};
+        DevGraph ng{d_seq, nbytes, nullptr, {nullptr, nullptr}, 0};
         if (hipMalloc((void **)&ng.hdr, 16) != hipSuccess) return KC_ENOMEM;
-        if ((rc = capture(b, d_seq, nbytes, ng.hdr, &ng.exec)) != KC_OK) {
-            (void)hipFree(ng.hdr);
-            return rc;
-        }
         b->dev.push_back(ng);
         g = &b->dev.back();
     }
+    if (!g->exec[fresh] && (rc = capture(b, g->ptr, g->cap, g->hdr, fresh, &g->exec[fresh])) != KC_OK) return rc;
     g->last_use = ++b->tick;
     hipStream_t ts = b->t->ctx->stream;
     hipLaunchKernelGGL(k_set_hdr, dim3(1), dim3(64), 0, ts, g->hdr, nbytes, 0);
     KC_HIP(hipGetLastError());
-    KC_HIP(hipGraphLaunch(g->exec, ts));
+    KC_HIP(hipGraphLaunch(g->exec[fresh], ts));
+    if (fresh) b->t->fresh = false;  // this pass wrote every slot
     ++b->info.launches;
     b->info.bytes += nbytes;
     return KC_OK;

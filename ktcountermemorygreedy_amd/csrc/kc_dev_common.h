@@ -102,13 +102,12 @@ __device__ __forceinline__ uint64_t part_of_key(const uint64_t *key, uint32_t pa
 }
 
 // Where a key lives.  Its partition is (coarse digit, fine digit) as
-// above.  Inside the partition, one XXH3_128
-// gives both the Bloom probe (block lo & (blocks - 1), mask bits from lo and
-// hi | 1: BloomFilter.h:215-222, 287-297, on the low bits) and the linear
-// probe start (hi >> 40, bits the Bloom probe does not use).
+// above.  Inside the partition, one slot_mix gives the Bloom block and mask
+// (bloom_in_part) and the linear probe start (bits 44..63, which the Bloom
+// probe does not use).
 struct KeyHash {
     uint64_t part;
-    uint64_t lo, hi;
+    uint64_t h;
 };
 
 template <int NW>
@@ -116,17 +115,17 @@ __device__ __forceinline__ KeyHash key_hash(const TableView &tv, const uint64_t 
 {
     KeyHash kh;
     kh.part = part_of_key<NW>(key, tv.part_bits);
-    xxh3_128<NW>(key, kh.lo, kh.hi);
+    kh.h = slot_mix<NW>(key);
     return kh;
 }
 
-__device__ __forceinline__ uint32_t probe_start(uint64_t hi) { return (uint32_t)(hi >> 40); }
+__device__ __forceinline__ uint32_t probe_start(uint64_t h) { return (uint32_t)(h >> 44); }
 
 // Slot index of probe step `probe` of a key.
 __device__ __forceinline__ uint64_t slot_index(const TableView &tv, const KeyHash &kh, uint64_t probe)
 {
     const uint64_t pmask = (1ULL << tv.part_log2) - 1;
-    return (kh.part << tv.part_log2) | ((probe_start(kh.hi) + probe) & pmask);
+    return (kh.part << tv.part_log2) | ((probe_start(kh.h) + probe) & pmask);
 }
 
 __device__ __forceinline__ uint64_t lanemask_lt()
@@ -254,7 +253,7 @@ __device__ __forceinline__ uint64_t table_find(const TableView &tv, const uint64
 __device__ __forceinline__ int bloom_classify(const TableView &tv, const KeyHash &kh)
 {
     uint64_t blk, m;
-    bloom_from(kh.lo, kh.hi, (1ULL << tv.bloom_part_log2) - 1, blk, m);
+    bloom_in_part(kh.h, (1ULL << tv.bloom_part_log2) - 1, blk, m);
     blk |= kh.part << tv.bloom_part_log2;
     uint64_t *cell = tv.bloom + 2 * blk;
     if ((ld_agent(cell) & m) == m) return 2;

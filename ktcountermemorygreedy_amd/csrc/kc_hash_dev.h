@@ -227,4 +227,35 @@ __device__ __forceinline__ void bloom_probe(const uint64_t *w, uint64_t cap_mask
     bloom_from(lo, hi, cap_mask, block, mask);
 }
 
+// ---- the table's own key hash (device layout, not a reference hash) ------
+// Where a key sits inside its partition: its Bloom block (low bits), the
+// three Bloom bits (bits 32..43) and the slot probe start (bits 44..63), all
+// from one 64-bit mix of the key words: two multiply rounds with 32-bit
+// xor-shifts, which a 32-bit ALU does as a word swap.  The reference's
+// probe (bloom_from) takes block and mask from XXH3_128, whose first mask
+// bit is the block's low bits, so a key's mask in its block depends on 6
+// bits only; three positions a, a+b, a+2b (b odd, so distinct) from 12 free
+// bits give ~2^11 masks per block and about a third of the false positives
+// at the same size, for a third of the multiplies.
+constexpr uint64_t kSlotMixC = 0xD6E8FEB86659FD93ULL;
+constexpr uint64_t kSlotFoldC = 0x9E3779B97F4A7C15ULL;
+
+template <int NW>
+__device__ __forceinline__ uint64_t slot_mix(const uint64_t *w)
+{
+    uint64_t x = w[0];
+    if constexpr (NW == 2) x ^= w[1] * kSlotFoldC;
+    x = (x ^ (x >> 32)) * kSlotMixC;
+    x = (x ^ (x >> 32)) * kSlotMixC;
+    return x ^ (x >> 32);
+}
+
+// Bloom block (in a part of cap_mask + 1 blocks) and 3-bit mask of a key.
+__device__ __forceinline__ void bloom_in_part(uint64_t h, uint64_t cap_mask, uint64_t &block, uint64_t &mask)
+{
+    const uint32_t a = (uint32_t)(h >> 32) & 63u, b = ((uint32_t)(h >> 38) & 63u) | 1u;
+    mask = (1ULL << a) | (1ULL << ((a + b) & 63u)) | (1ULL << ((a + 2u * b) & 63u));
+    block = h & cap_mask;
+}
+
 } // namespace kc

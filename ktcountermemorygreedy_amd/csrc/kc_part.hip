@@ -591,17 +591,18 @@ struct PartLds {
     unsigned long long bloom[2 * BS]; // {BF1, BF2} per block
 };
 
-// Bloom state machine of BloomFilter.h:233-273 on the LDS copy: 0 FIRST,
+// Bloom state machine of BloomFilter.h:233-273 on the LDS copy (block and
+// mask from bloom_in_part): 0 FIRST,
 // 1 SECOND, 2 THIRD_PLUS.  BF1 only ever receives bits that BF2 holds, so
 // the reference's plain read of BF1 before its RMWs can be folded into the
 // RMWs: fetch_or(BF2) decides FIRST (the one caller that completes the
 // mask), fetch_or(BF1) decides SECOND vs THIRD_PLUS -- the same outcomes for
 // every interleaving, with one branch instead of three.
 template <int NW, bool BLOOM>
-__device__ __forceinline__ int lds_bloom(PartLds<NW, BLOOM> &S, uint64_t lo, uint64_t hi, uint32_t bmask)
+__device__ __forceinline__ int lds_bloom(PartLds<NW, BLOOM> &S, uint64_t h, uint32_t bmask)
 {
     uint64_t blk, m;
-    bloom_from(lo, hi, bmask, blk, m);
+    bloom_in_part(h, bmask, blk, m);
     unsigned long long *cell = S.bloom + 2 * blk;
     const uint64_t b2 = atomicOr(cell + 1, (unsigned long long)m);
     if ((b2 & m) != m) return 0;
@@ -610,11 +611,11 @@ __device__ __forceinline__ int lds_bloom(PartLds<NW, BLOOM> &S, uint64_t lo, uin
 }
 
 template <int NW, bool BLOOM>
-__device__ __forceinline__ bool lds_insert(PartLds<NW, BLOOM> &S, const uint64_t *key, uint64_t hi, uint32_t pmask,
+__device__ __forceinline__ bool lds_insert(PartLds<NW, BLOOM> &S, const uint64_t *key, uint64_t h, uint32_t pmask,
                                            uint32_t &n_new, uint32_t &n_err)
 {
     constexpr uint32_t PS = PartLds<NW, BLOOM>::PS;
-    uint32_t s = probe_start(hi) & pmask;
+    uint32_t s = probe_start(h) & pmask;
     for (uint32_t probe = 0; probe <= pmask; ++probe, s = (s + 1) & pmask) {
         // one LDS CAS per probe: it claims an empty slot or returns the
         // slot's key (a present key is the common case and is not written)
@@ -747,9 +748,7 @@ __global__ __launch_bounds__(kPartThreads) void k_count_parts(const uint64_t *__
                         for (int w = 0; w < NW; ++w) dst[w] = fk[j][w];
                     } else {
                         // no room on the list: counted in the partition's table instead
-                        uint64_t lo, hi;
-                        xxh3_128<NW>(fk[j], lo, hi);
-                        lds_insert<NW, BLOOM>(S, fk[j], hi, psz - 1, n_new, n_err);
+                        lds_insert<NW, BLOOM>(S, fk[j], slot_mix<NW>(fk[j]), psz - 1, n_new, n_err);
                     }
                 }
                 pre += (uint32_t)__popcll(fbs[j]);
@@ -788,11 +787,10 @@ __global__ __launch_bounds__(kPartThreads) void k_count_parts(const uint64_t *__
                 for (int w = 0; w < NW; ++w) nxt[PF - 1][w] = keys[(jj < end ? jj : beg) * NW + w];
             }
             n_k += valid;
-            uint64_t lo, hi;  // one XXH3_128: Bloom probe and slot probe start
-            xxh3_128<NW>(key, lo, hi);
+            const uint64_t h = slot_mix<NW>(key);  // Bloom probe and slot probe start
             bool to_table = valid;
             if constexpr (BLOOM) {
-                const bool first = valid && lds_bloom<NW, BLOOM>(S, lo, hi, bsz - 1) == 0;
+                const bool first = valid && lds_bloom<NW, BLOOM>(S, h, bsz - 1) == 0;
                 if constexpr (FM == kFirstStaged) {
                     const uint64_t fb = __ballot(first);
                     if (first) {
@@ -813,7 +811,7 @@ __global__ __launch_bounds__(kPartThreads) void k_count_parts(const uint64_t *__
                     to_table = valid && !first;
                 }
             }
-            if (to_table) lds_insert<NW, BLOOM>(S, key, hi, psz - 1, n_new, n_err);
+            if (to_table) lds_insert<NW, BLOOM>(S, key, h, psz - 1, n_new, n_err);
         }
         if constexpr (BLOOM && FM == kFirstGroup) append_group();
     }
@@ -835,11 +833,10 @@ __global__ __launch_bounds__(kPartThreads) void k_count_parts(const uint64_t *__
                 } else {
                     // no room on the list: the occurrence is counted in the
                     // partition's table (still in LDS) instead
-                    uint64_t key[NW], lo, hi;
+                    uint64_t key[NW];
 #pragma unroll
                     for (int w = 0; w < NW; ++w) key[w] = src[w];
-                    xxh3_128<NW>(key, lo, hi);
-                    lds_insert<NW, BLOOM>(S, key, hi, psz - 1, n_new, n_err);
+                    lds_insert<NW, BLOOM>(S, key, slot_mix<NW>(key), psz - 1, n_new, n_err);
                 }
             }
         }

@@ -206,6 +206,14 @@ def test_batch_device_launch_replays(ctx, k, path):
     b.sync()
     _check_exact(t, cut, k)
     assert b.info()["graphs_captured"] == 1
+    # after a clear the partitioned graph writes the lazily cleared table
+    # whole; the second launch adds to it with the graph that reads it
+    t.clear()
+    b.launch_device(d, len(seq))
+    b.launch_device(d, len(seq))
+    b.sync()
+    _check_exact(t, seq + seq, k)
+    assert b.info()["graphs_captured"] == (2 if path == "partitioned" else 1)
     b.close()
     t.close()
 

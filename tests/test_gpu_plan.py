@@ -36,7 +36,7 @@ def test_plan_estimates_and_table(ctx, k, bloom):
     assert abs(plan["repeated"] - repeated) <= 0.10 * repeated + 2048
     assert kw["capacity"] >= 1.5 * (repeated if bloom else distinct)
     if bloom:
-        assert kw["bloom_blocks"] * 6 >= distinct and kw["low_capacity"] >= distinct
+        assert kw["bloom_blocks"] * 8 >= distinct and kw["low_capacity"] >= distinct
     t = kc.Table(ctx, k, **kw)
     t.count_seq(d, len(seq))
     ctx.sync()

@@ -83,8 +83,8 @@ def test_plan_sizes_reproduces_device_plans():
     (table keys, FIRST occurrences).  Totals and outcomes are those runs'."""
     from ktcountermemorygreedy_amd import kc
     runs = [  # k, distinct, repeated, device_bytes, measured table keys, FIRST occurrences
-        (31, 391599168, 101114048, 9732416840, 116714803, 374634073),    # 20M x 150 bp over 100 Mbp
-        (63, 1836491392, 40757888, 48033990336, 134327905, 1741620449),  # 1.16M uneven reads over 50 Mbp
+        (31, 391599168, 101114048, 9732416840, 101989117, 391114715),    # 20M x 150 bp over 100 Mbp
+        (63, 1836491392, 40757888, 39444055744, 58607250, 1818313084),  # 1.16M uneven reads over 50 Mbp
     ]
     for k, distinct, repeated, dev_bytes, table_keys, firsts in runs:
         kw, plan = kc.plan_sizes(k, distinct, repeated, bloom=True)
