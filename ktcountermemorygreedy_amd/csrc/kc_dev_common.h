@@ -270,6 +270,55 @@ __device__ __forceinline__ int bloom_classify(const TableView &tv, const KeyHash
 }
 
 // ---- rolling canonical state (GetKmer.h:58-82) -------------------------
+// Four input bytes at once: `codes` gets base_code's 2-bit code of every
+// byte in that byte's low bits; the result has bit 7 set in every byte
+// base_code calls invalid.  A byte is valid when it equals (case folded) the
+// base its code names (A C G T) or is U, which shares T's code.
+__device__ __forceinline__ uint32_t invalid_bytes(uint32_t w, uint32_t &codes)
+{
+    codes = ((w >> 1) ^ (w >> 2)) & 0x03030303u;
+    const uint32_t c = w & 0xDFDFDFDFu;
+    const uint32_t named = __builtin_amdgcn_perm(0u, 0x54474341u, codes);  // byte i = "ACGT"[code i]
+    const auto nonzero = [](uint32_t z) { return (((z & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | z) & 0x80808080u; };
+    return nonzero(c ^ named) & nonzero(c ^ 0x55555555u);
+}
+
+// The codes of invalid_bytes packed to 8 bits, byte 0's code lowest.
+__device__ __forceinline__ uint32_t pack_codes(uint32_t codes)
+{
+    const uint32_t u = codes | (codes >> 6);
+    return (u | (u >> 12)) & 0xFFu;
+}
+
+// Bit 7 of each byte of invalid_bytes gathered to bits 0..3.
+__device__ __forceinline__ uint32_t gather_invalid(uint32_t iv)
+{
+    return (((iv >> 7) * 0x00204081u) >> 21) & 0xFu;
+}
+
+// Reverses the order of the 32 two-bit fields of x.
+__device__ __forceinline__ uint64_t pair_reverse(uint64_t x)
+{
+    x = __builtin_bitreverse64(x);
+    return ((x >> 1) & 0x5555555555555555ULL) | ((x & 0x5555555555555555ULL) << 1);
+}
+
+// History bytes h[0..4*HW) (byte 0 oldest) packed 4 words per u64 (byte
+// j's code at bits 2j of its word) with their invalid flags (bit j).
+template <int HW>
+__device__ __forceinline__ void pack_history(const uint32_t *h, uint64_t *packed, uint64_t &invalid)
+{
+    invalid = 0;
+#pragma unroll
+    for (int q = 0; q < HW; ++q) {
+        uint32_t codes;
+        const uint32_t iv = invalid_bytes(h[q], codes);
+        if (q % 8 == 0) packed[q / 8] = 0;
+        packed[q / 8] |= (uint64_t)pack_codes(codes) << (8 * (q % 8));
+        invalid |= (uint64_t)gather_invalid(iv) << (4 * q);
+    }
+}
+
 template <int NW> struct Roll;
 
 template <> struct Roll<1> {
@@ -287,6 +336,18 @@ template <> struct Roll<1> {
         r = (r << 2) | ((code ^ 3) << rev_shift);
     }
     __device__ __forceinline__ void canon(uint64_t *out) const { out[0] = f < r ? f : r; }
+    // The state push() leaves after the 32 history bytes h[0..8) (byte 0
+    // first), without rolling them: f holds the last k codes, newest at the
+    // top, which is the packed history masked; r the complements in
+    // reverse order.  `have` = the trailing run of valid bytes.
+    __device__ __forceinline__ void seed(const uint32_t *h, uint32_t &have)
+    {
+        uint64_t p[1], inv;
+        pack_history<8>(h, p, inv);
+        f = p[0] & back_mask;
+        r = pair_reverse(~p[0]) << rev_shift;
+        have = (uint32_t)__clz((uint32_t)inv);
+    }
 };
 
 template <> struct Roll<2> {
@@ -311,6 +372,20 @@ template <> struct Roll<2> {
         const bool fwd = (f0 < r0) || (f0 == r0 && f1 < r1);
         out[0] = fwd ? f0 : r0;
         out[1] = fwd ? f1 : r1;
+    }
+    // As Roll<1>::seed over the 64 history bytes h[0..16): the newer 32
+    // codes are the head word, the older 32 (masked) the tail word, and the
+    // 128-bit reverse complement is shifted up by rev_shift.
+    __device__ __forceinline__ void seed(const uint32_t *h, uint32_t &have)
+    {
+        uint64_t p[2], inv;
+        pack_history<16>(h, p, inv);
+        f0 = p[1];
+        f1 = p[0] & back_mask;
+        const uint64_t ylo = pair_reverse(~p[1]), yhi = pair_reverse(~p[0]);
+        r1 = ylo << rev_shift;
+        r0 = rev_shift ? (yhi << rev_shift) | (ylo >> (64 - rev_shift)) : yhi;
+        have = (uint32_t)__clzll((long long)inv);
     }
 };
 
@@ -499,16 +574,10 @@ __device__ __forceinline__ void roll_segment(const ParseLds &L, uint64_t tile, u
     const uint64_t start = tile * kParseTile + (uint64_t)t * kParseSeg;
     if (start >= n) return;
     Roll<NW> roll(k);
-    uint32_t have = 0;
-    for (int j = 16 - kHaloWords; j < 16; ++j) {
-        const uint32_t v = L.w[j][t];
+    uint32_t have, hist[kHaloWords];
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            bool valid;
-            roll.push(base_code(v >> (8 * b), valid));
-            have = valid ? have + 1 : 0;
-        }
-    }
+    for (int j = 0; j < kHaloWords; ++j) hist[j] = L.w[16 - kHaloWords + j][t];
+    roll.seed(hist, have);
     // bytes of this segment before emit_from emit nothing
     const uint32_t skip = emit_from > start ? (uint32_t)(emit_from - start < 64 ? emit_from - start : 64) : 0;
     for (int j = 0; j < 16; ++j) {

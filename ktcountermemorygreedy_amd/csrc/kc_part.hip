@@ -223,16 +223,8 @@ __global__ __launch_bounds__(TB) void k_front_scatter_seq_staged(SeqSrc src, uin
         const uint32_t skip = start >= n ? 64u
                             : emit_from > start ? (uint32_t)(emit_from - start < 64 ? emit_from - start : 64) : 0u;
         Roll<NW> roll(k);
-        uint32_t have = 0;
-#pragma unroll
-        for (int j = 0; j < HW; ++j) {  // history from the previous segment
-#pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                bool valid;
-                roll.push(base_code(w[j] >> (8 * b), valid));
-                have = valid ? have + 1 : 0;
-            }
-        }
+        uint32_t have;
+        roll.seed(w, have);  // the history: the previous segment's last 4*HW bytes
 #pragma unroll
         for (int q = 0; q < kChunks; ++q) {
             uint64_t key[CB * NW];
