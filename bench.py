@@ -524,7 +524,7 @@ def main():
     tune = None
     if not args.no_autotune:
         t0 = time.time()
-        tune = AT.tune_shape(table, d_seq, S, reps=1, use_cache=False)
+        tune = AT.tune_shape(table, d_seq, S, reps=3, use_cache=False)
         tune["seconds"] = time.time() - t0
         print("autotune: %s" % tune, file=sys.stderr, flush=True)
     launch = args.launch if args.launch != "auto" else ("graph" if args.workload == "uneven_50mbp" else "eager")

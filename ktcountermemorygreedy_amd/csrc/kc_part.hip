@@ -1027,6 +1027,8 @@ int kc_part_enqueue(const kc_table *t, kc_part_scratch &s, const kc_pipe_shape &
             else hipLaunchKernelGGL((k_fine_scatter_staged<2, 256, 1024, 8>), g2, dim3(1024), 0, st, s.keys_a, s.starts, d, S, s.o2, s.m2, s.keys_b, ctr);
         } else {
             if (nw == 1) hipLaunchKernelGGL((k_fine_scatter_staged<1, 1024, 1024, 16, false>), g2, dim3(1024), 0, st, s.keys_a, s.starts, d, S, s.o2, s.m2, s.keys_b, ctr);
+            else if (d.l2 == 9)  // 512 digits: their u16 copies fit next to the 8192 two-word keys
+                hipLaunchKernelGGL((k_fine_scatter_staged<2, 512, 1024, 8, true>), g2, dim3(1024), 0, st, s.keys_a, s.starts, d, S, s.o2, s.m2, s.keys_b, ctr);
             else hipLaunchKernelGGL((k_fine_scatter_staged<2, 1024, 1024, 8, false>), g2, dim3(1024), 0, st, s.keys_a, s.starts, d, S, s.o2, s.m2, s.keys_b, ctr);
         }
         KC_HIP(hipGetLastError());
