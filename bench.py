@@ -27,7 +27,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 # one kc_table_count_seq call on the partitioned path: these kernels, in order
-COUNT_KERNELS = ("count pipeline: k_front_count_seq + k_front_scatter_seq_staged + k_fine_count + "
+COUNT_KERNELS = ("count pipeline: k_front_count_seq + k_front_scatter_seq_staged + k_fine_count_dig + "
                  "k_fine_scatter_staged + k_count_parts (timed together with HIP events on the table's stream)")
 
 
@@ -132,14 +132,17 @@ def cpu_baseline(ws, sample_reads, k, genome):
             "sample": "first %d reads, oracle extract+count" % sample_reads}
 
 
-def measured_traffic(kernel, workload):
+def measured_traffic(kernel, workload, k=None):
     """HBM bytes per launch of the kernel from the committed rocprof --pmc
-    summary (profiles/traffic.json), when it covers this kernel and workload."""
+    summary (profiles/traffic.json), when it covers this kernel, workload
+    and k (entries "kernel/workload/k<k>", or "kernel" for one workload)."""
     try:
         with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
-            rec = json.load(f).get(kernel)
-        if rec and rec["workload"] == workload:
-            return rec["bytes_per_launch"]
+            data = json.load(f)
+        for key in ("%s/%s/k%s" % (kernel, workload, k), kernel):
+            rec = data.get(key)
+            if rec and rec["workload"] == workload and rec.get("k", k) == k:
+                return rec["bytes_per_launch"]
     except (OSError, ValueError, KeyError):
         pass
     return None
@@ -627,7 +630,7 @@ def main():
                    "step": "clear + fused parse/canonical/Bloom/count + export packing (device)"},
         "roofline": {"bound": "hbm", "kernel": COUNT_KERNELS, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic("count_pipeline", ws.name),
+                     "traffic": measured_traffic("count_pipeline", ws.name, k),
                      "alg_bytes_per_launch": alg_bytes, "avg_launch_ms": avg_count_s * 1e3},
         "cpu_baseline": cpu,
         "checks": {"windows_match": windows == K, "hist_mass_equals_kmers": mass == K},
