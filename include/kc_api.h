@@ -101,6 +101,10 @@ int kc_dev_alloc(kc_ctx *ctx, uint64_t bytes, void **out);
 int kc_dev_free(kc_ctx *ctx, void *d);
 int kc_host_alloc_pinned(kc_ctx *ctx, uint64_t bytes, void **out);
 int kc_host_free_pinned(kc_ctx *ctx, void *h);
+/* The device address of page-locked memory from kc_host_alloc_pinned (it is
+ * mapped into the device's address space), for kernels that read or write
+ * it in place instead of copying it.  No reference counterpart. */
+int kc_host_device_ptr(kc_ctx *ctx, void *h, void **d);
 int kc_copy_h2d(kc_ctx *ctx, void *d, const void *h, uint64_t bytes);
 int kc_copy_d2h(kc_ctx *ctx, void *h, const void *d, uint64_t bytes);
 int kc_copy_d2d(kc_ctx *ctx, void *dst, const void *src, uint64_t bytes);
@@ -148,6 +152,17 @@ int kc_table_increment(kc_table *t, const uint64_t *d_keys, uint64_t n,
  * Asynchronous, except for that index build. */
 int kc_table_query(kc_table *t, const uint64_t *d_keys, uint64_t n,
                    uint32_t *d_counts);
+/* The same lookup launched on ctx's stream instead of the table's (same
+ * device), for lookups that must not queue behind other work on the table's
+ * stream: a serving loop's lookup stream next to job streams that several
+ * tables share.  The caller makes sure no count of this table is still in
+ * flight (kc_ctx_sync of the table's context after its last count): the
+ * lookup does not wait for the table's stream, which may be busy with other
+ * tables.  Later work enqueued on the table's stream (counts,
+ * kc_table_destroy) waits for the lookup.  No reference counterpart beyond
+ * kc_table_query's. */
+int kc_table_query_on(kc_table *t, kc_ctx *ctx, const uint64_t *d_keys, uint64_t n,
+                      uint32_t *d_counts);
 int kc_table_stats(kc_table *t, kc_stats *out);
 
 /* Multi-pass counting for tables larger than HBM: the k-mers are split into

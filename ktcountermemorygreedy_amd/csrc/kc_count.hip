@@ -659,7 +659,16 @@ extern "C" int kc_host_alloc_pinned(kc_ctx *ctx, uint64_t bytes, void **out)
 {
     if (!ctx || !out) return KC_EINVAL;
     *out = nullptr;
-    if (hipHostMalloc(out, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) return KC_ENOMEM;
+    if (hipHostMalloc(out, bytes ? bytes : 1, hipHostMallocMapped) != hipSuccess) return KC_ENOMEM;
+    return KC_OK;
+}
+
+extern "C" int kc_host_device_ptr(kc_ctx *ctx, void *h, void **d)
+{
+    if (!ctx || !h || !d) return KC_EINVAL;
+    *d = nullptr;
+    KC_HIP(hipSetDevice(ctx->device));
+    KC_HIP(hipHostGetDevicePointer(d, h, 0));
     return KC_OK;
 }
 
@@ -887,6 +896,8 @@ extern "C" void kc_table_destroy(kc_table *t)
         if (t->stage_done[i]) (void)hipEventDestroy(t->stage_done[i]);
     }
     if (t->copy_done) (void)hipEventDestroy(t->copy_done);
+    if (t->query_ready) (void)hipEventDestroy(t->query_ready);
+    if (t->query_done) (void)hipEventDestroy(t->query_done);
     delete t;
 }
 
@@ -1109,19 +1120,29 @@ static int ensure_low_index(kc_table *t, uint64_t *mask)
     return KC_OK;
 }
 
-extern "C" int kc_table_query(kc_table *t, const uint64_t *d_keys, uint64_t n, uint32_t *d_counts)
+extern "C" int kc_table_query_on(kc_table *t, kc_ctx *ctx, const uint64_t *d_keys, uint64_t n, uint32_t *d_counts)
 {
-    if (!t || (n && (!d_keys || !d_counts))) return KC_EINVAL;
+    if (!t || !ctx || ctx->device != t->ctx->device || (n && (!d_keys || !d_counts))) return KC_EINVAL;
     KC_HIP(hipSetDevice(t->ctx->device));
+    const bool other = ctx != t->ctx;
+    if (other && !t->query_done) KC_HIP(hipEventCreateWithFlags(&t->query_done, hipEventDisableTiming));
+    // a lazily cleared table is written empty on its own stream first, and
+    // only then does the lookup stream wait for that stream
+    const bool was_fresh = t->fresh;
     uint64_t imask = 0;
     int rc = kc_table_materialize(t);
-    if (rc == KC_OK) rc = ensure_low_index(t, &imask);
+    if (rc == KC_OK) rc = ensure_low_index(t, &imask);  // synchronous when it builds the index
     if (rc != KC_OK) return rc;
     if (!n) return KC_OK;
+    hipStream_t st = ctx->stream;
+    if (other && was_fresh) {
+        if (!t->query_ready) KC_HIP(hipEventCreateWithFlags(&t->query_ready, hipEventDisableTiming));
+        KC_HIP(hipEventRecord(t->query_ready, t->ctx->stream));
+        KC_HIP(hipStreamWaitEvent(st, t->query_ready, 0));
+    }
     const TableView v = view_of(t);
-    const uint32_t g = grid_for(n, t->ctx->cu_count);
+    const uint32_t g = grid_for(n, ctx->cu_count);
     const uint64_t *idx = t->low_idx;
-    hipStream_t st = t->ctx->stream;
     if (t->nw == 1) {
         if (imask) hipLaunchKernelGGL((k_query<1, true>), dim3(g), dim3(kBlock), 0, st, d_keys, n, d_counts, v, idx, imask);
         else hipLaunchKernelGGL((k_query<1, false>), dim3(g), dim3(kBlock), 0, st, d_keys, n, d_counts, v, idx, imask);
@@ -1130,7 +1151,17 @@ extern "C" int kc_table_query(kc_table *t, const uint64_t *d_keys, uint64_t n, u
         else hipLaunchKernelGGL((k_query<2, false>), dim3(g), dim3(kBlock), 0, st, d_keys, n, d_counts, v, idx, imask);
     }
     KC_HIP(hipGetLastError());
+    if (other) {  // later work on the table (a count, its destruction) waits for the lookup
+        KC_HIP(hipEventRecord(t->query_done, st));
+        KC_HIP(hipStreamWaitEvent(t->ctx->stream, t->query_done, 0));
+    }
     return KC_OK;
+}
+
+extern "C" int kc_table_query(kc_table *t, const uint64_t *d_keys, uint64_t n, uint32_t *d_counts)
+{
+    if (!t) return KC_EINVAL;
+    return kc_table_query_on(t, t->ctx, d_keys, n, d_counts);
 }
 
 extern "C" int kc_table_stats(kc_table *t, kc_stats *out)

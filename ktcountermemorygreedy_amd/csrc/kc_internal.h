@@ -110,6 +110,9 @@ struct kc_table {
     uint64_t low_idx_cap;   // allocated slots
     uint64_t low_idx_mask;  // slots in use - 1 (0: no index, the list is empty)
     uint64_t low_idx_epoch;
+    // lookups launched on another context (kc_table_query_on): the table's
+    // stream hands over to that stream and waits for the lookup after it
+    hipEvent_t query_ready, query_done;
     // host copy of the device counters (read_counters), sized at creation
     std::vector<uint64_t> hcounters, hseg_off;
     // test-only fault injection (KC_TEST_SKEW_COUNTS): skews one digit count

@@ -3,9 +3,15 @@
 // only point lookup is FlatConcurrentHashMap::find_prepared in the tools,
 // LowFrequencyQueryThreadPool.h:196-244).
 //
-// * Every table owns a kc_ctx (its own HIP stream, default priority), so the
-//   work of different tables overlaps on the device; requests on one table
-//   are serialized by the table's mutex.
+// * Streams (all at default priority): count jobs run on a small pool of job
+//   contexts (kJobStreams, one job per context at a time), lookups on one
+//   lookup context of their own (kc_table_query_on), all created when the
+//   server starts.  The process's streams share a few hardware queues (4 by
+//   default) and work on one hardware queue runs in order, so a lookup on a
+//   stream that shares its queue with a running job's stream waits for that
+//   job's kernels; a lookup stream next to a bounded number of job streams
+//   keeps lookups off the jobs' queues.  Requests on one table are
+//   serialized by the table's mutex.
 // * Worker threads take requests from one FIFO.  A count job is admitted
 //   only when its device footprint fits the HBM measured free at that moment
 //   (kc_device_mem_info) next to the running jobs, and the optional budget
@@ -16,8 +22,10 @@
 //   KC_ENOMEM instead of waiting forever.
 // * Lookups are batched continuously: a worker that takes a query batch of a
 //   table also takes every other query batch of that table waiting in the
-//   queue (up to kMergeKeys keys) and answers them with one copy in, one
-//   lookup launch and one copy out, through page-locked staging.
+//   queue (up to kMergeKeys keys) and answers them with one lookup launch
+//   that reads the keys from and writes the counts to page-locked host
+//   staging directly: no copy-engine transfer, which would queue behind the
+//   count jobs' host-to-device streaming.
 // * A count job with the Bloom split builds its table's FIRST-list index
 //   before it completes, so lookups never pay for it.
 // * Latency = completion time - submit time, per request.
@@ -45,34 +53,34 @@ using Clock = std::chrono::steady_clock;
 
 // Most keys one merged lookup launch answers.
 constexpr uint64_t kMergeKeys = 4ull << 20;
+// Count jobs running at once (one job context each).
+constexpr uint32_t kJobStreams = 3;
 
 struct Table {
     uint64_t id = 0;
     int nw = 1;
-    kc_ctx *ctx = nullptr;
+    kc_ctx *ctx = nullptr;   // the job context the table was counted on (the server's)
+    kc_ctx *sctx = nullptr;  // the lookup context its staging belongs to (the server's)
     kc_table *t = nullptr;
     uint64_t bytes = 0;
     std::mutex mu;
     std::atomic<int> busy{0};
     Clock::time_point last_use;
-    // lookup staging, grown on demand: device keys / counts and their
-    // page-locked host images
-    void *dkeys = nullptr, *dcounts = nullptr, *hkeys = nullptr, *hcounts = nullptr;
+    // lookup staging, grown on demand: page-locked host keys / counts that
+    // the lookup kernel reads and writes in place (at their device addresses)
+    void *hkeys = nullptr, *hcounts = nullptr, *dkeys = nullptr, *dcounts = nullptr;
     uint64_t qcap = 0;
     void free_staging()
     {
-        if (dkeys) kc_dev_free(ctx, dkeys);
-        if (dcounts) kc_dev_free(ctx, dcounts);
-        if (hkeys) kc_host_free_pinned(ctx, hkeys);
-        if (hcounts) kc_host_free_pinned(ctx, hcounts);
-        dkeys = dcounts = hkeys = hcounts = nullptr;
+        if (hkeys) kc_host_free_pinned(sctx, hkeys);
+        if (hcounts) kc_host_free_pinned(sctx, hcounts);
+        hkeys = hcounts = dkeys = dcounts = nullptr;
         qcap = 0;
     }
     ~Table()
     {
-        if (ctx) free_staging();
-        if (t) kc_table_destroy(t);
-        if (ctx) kc_ctx_destroy(ctx);
+        if (sctx) free_staging();
+        if (t) kc_table_destroy(t);  // after every lookup (kc_table_query_on orders them)
     }
 };
 
@@ -86,6 +94,7 @@ struct Request {
     uint64_t nbytes = 0, max_distinct = 0, capacity = 0, resident = 0;
     kc_ticket ticket;
     int refused = KC_OK;  // set when the request is taken from the queue without running
+    int job_slot = -1;    // the job context a running count holds
     // query
     const uint64_t *keys = nullptr;
     uint64_t n = 0;
@@ -122,6 +131,13 @@ struct kc_server {
     std::unordered_map<uint64_t, std::unique_ptr<Request>> reqs;
     std::unordered_map<uint64_t, std::shared_ptr<Table>> tables;
     std::unordered_map<uint64_t, int> pending_counts;  // queued or running count jobs per table id
+    kc_ctx *lookup_ctx = nullptr;
+    std::vector<kc_ctx *> job_ctx;
+    std::vector<char> job_busy;
+    // tables dropped while jobs run are destroyed by a worker outside the
+    // lock (kc_table_destroy waits for its job context's stream, which may be
+    // running another job), or here once nothing runs
+    std::vector<std::shared_ptr<Table>> dead;
     uint64_t running = 0;  // requests being executed
     uint64_t next_id = 1;
     bool stop = false;
@@ -153,6 +169,7 @@ struct kc_server {
     // Takes the first request that can run now; mu held.
     bool pick(Work &w)
     {
+        if (running == 0) dead.clear();  // every stream is idle: freeing them waits for nothing
         bool measured = false;
         uint64_t free_now = 0;
         for (auto it = queue.begin(); it != queue.end(); ++it) {
@@ -185,6 +202,10 @@ struct kc_server {
                 queue.erase(it);
                 return true;
             }
+            int slot = -1;  // a job context to run on; none free: later queries go ahead
+            for (size_t j = 0; j < job_busy.size() && slot < 0; ++j)
+                if (!job_busy[j]) slot = (int)j;
+            if (slot < 0) continue;
             if (!measured) {
                 free_now = free_hbm();
                 measured = true;
@@ -195,6 +216,10 @@ struct kc_server {
                 if (!r->ticket.reserved) r->refused = KC_ENOMEM;  // cannot fit even with every table dropped
             }
             if (r->ticket.reserved || r->refused) {
+                if (r->ticket.reserved) {
+                    r->job_slot = slot;
+                    job_busy[slot] = 1;
+                }
                 w.reqs.push_back(r);
                 queue.erase(it);
                 return true;
@@ -212,7 +237,7 @@ struct kc_server {
         if (!lru) return false;
         adm.drop(lru->bytes);
         tables.erase(lru->id);
-        return true;  // freed when the last holder lets go (a worker holding it has it busy)
+        return true;  // freed now (nothing runs) or by the last holder
     }
 
     void run_count(Request *r)
@@ -229,7 +254,8 @@ struct kc_server {
         if (rc == KC_OK) {
             tab->id = r->table_id;
             tab->nw = r->k <= 32 ? 1 : 2;
-            rc = kc_ctx_create(device, &tab->ctx);
+            tab->ctx = job_ctx[r->job_slot];
+            tab->sctx = lookup_ctx;
         }
         if (rc == KC_OK) {
             kc_table_config cfg{};
@@ -266,7 +292,7 @@ struct kc_server {
         r->res.status = rc;
     }
 
-    // One copy in, one lookup launch and one copy out for every batch.
+    // One lookup launch over page-locked staging for every batch.
     void run_queries(const std::shared_ptr<Table> &tab, const std::vector<Request *> &batch)
     {
         if (!tab) {
@@ -281,10 +307,10 @@ struct kc_server {
         if (total > tab->qcap) {
             tab->free_staging();
             const uint64_t cap = std::max<uint64_t>(next_pow2(total), 65536);
-            rc = kc_dev_alloc(tab->ctx, cap * 8 * nw, &tab->dkeys);
-            if (rc == KC_OK) rc = kc_dev_alloc(tab->ctx, cap * 4, &tab->dcounts);
-            if (rc == KC_OK) rc = kc_host_alloc_pinned(tab->ctx, cap * 8 * nw, &tab->hkeys);
-            if (rc == KC_OK) rc = kc_host_alloc_pinned(tab->ctx, cap * 4, &tab->hcounts);
+            rc = kc_host_alloc_pinned(tab->sctx, cap * 8 * nw, &tab->hkeys);
+            if (rc == KC_OK) rc = kc_host_alloc_pinned(tab->sctx, cap * 4, &tab->hcounts);
+            if (rc == KC_OK) rc = kc_host_device_ptr(tab->sctx, tab->hkeys, &tab->dkeys);
+            if (rc == KC_OK) rc = kc_host_device_ptr(tab->sctx, tab->hcounts, &tab->dcounts);
             if (rc == KC_OK) tab->qcap = cap;
             else tab->free_staging();
         }
@@ -294,11 +320,9 @@ struct kc_server {
                 if (r->n) std::memcpy(static_cast<uint64_t *>(tab->hkeys) + at * nw, r->keys, r->n * 8 * nw);
                 at += r->n;
             }
-            rc = kc_copy_h2d(tab->ctx, tab->dkeys, tab->hkeys, total * 8 * nw);
-            if (rc == KC_OK)
-                rc = kc_table_query(tab->t, static_cast<const uint64_t *>(tab->dkeys), total,
-                                    static_cast<uint32_t *>(tab->dcounts));
-            if (rc == KC_OK) rc = kc_copy_d2h(tab->ctx, tab->hcounts, tab->dcounts, total * 4);
+            rc = kc_table_query_on(tab->t, tab->sctx, static_cast<const uint64_t *>(tab->dkeys), total,
+                                   static_cast<uint32_t *>(tab->dcounts));
+            if (rc == KC_OK) rc = kc_ctx_sync(tab->sctx);
             if (rc == KC_OK) {
                 at = 0;
                 for (Request *r : batch) {
@@ -318,13 +342,16 @@ struct kc_server {
     {
         for (;;) {
             Work w;
+            std::vector<std::shared_ptr<Table>> corpses;  // destroyed here, after the lock
             {
                 std::unique_lock<std::mutex> lk(mu);
                 cv_work.wait(lk, [&] { return stop || pick(w); });
+                corpses.swap(dead);
                 if (w.reqs.empty()) return;
                 ++running;
                 if (w.tab) w.tab->busy.fetch_add(1);
             }
+            corpses.clear();
             const Clock::time_point start = Clock::now();
             for (Request *r : w.reqs) r->start = start;
             Request *first = w.reqs[0];
@@ -340,6 +367,7 @@ struct kc_server {
                     r->res.latency_ms = std::chrono::duration<double, std::milli>(r->done - r->submit).count();
                     r->res.queue_ms = std::chrono::duration<double, std::milli>(r->start - r->submit).count();
                     if (r->kind == 0 && --pending_counts[r->table_id] == 0) pending_counts.erase(r->table_id);
+                    if (r->job_slot >= 0) job_busy[r->job_slot] = 0;
                     r->finished = true;
                 }
                 --running;
@@ -363,6 +391,26 @@ extern "C" int kc_server_create(int device, uint32_t workers, uint64_t hbm_budge
         s->total_hbm = total_b;
         s->adm.budget = hbm_budget_bytes;
         s->adm.margin = std::min<uint64_t>(2ull << 30, total_b / 16);
+        // the lookup context first, then the job contexts
+        const uint32_t jobs = std::min(workers, kJobStreams);
+        rc = kc_ctx_create(device, &s->lookup_ctx);
+        for (uint32_t i = 0; i < jobs && rc == KC_OK; ++i) {
+            kc_ctx *c = nullptr;
+            rc = kc_ctx_create(device, &c);
+            if (rc == KC_OK) {
+                try {
+                    s->job_ctx.push_back(c);
+                    s->job_busy.push_back(0);
+                } catch (...) {
+                    kc_ctx_destroy(c);
+                    rc = KC_ENOMEM;
+                }
+            }
+        }
+        if (rc != KC_OK) {
+            kc_server_destroy(s);
+            return rc;
+        }
         try {
             s->workers.reserve(workers);
             for (uint32_t i = 0; i < workers; ++i) s->workers.emplace_back([s] { s->worker(); });
@@ -385,6 +433,9 @@ extern "C" void kc_server_destroy(kc_server *s)
     s->cv_work.notify_all();
     for (auto &w : s->workers) w.join();
     s->tables.clear();
+    s->dead.clear();
+    for (kc_ctx *c : s->job_ctx) kc_ctx_destroy(c);
+    if (s->lookup_ctx) kc_ctx_destroy(s->lookup_ctx);
     delete s;
 }
 
@@ -473,6 +524,7 @@ extern "C" int kc_server_drop_table(kc_server *s, uint64_t table_id)
     if (it == s->tables.end()) return KC_EINVAL;
     if (it->second->busy.load()) return KC_EINVAL;
     s->adm.drop(it->second->bytes);
+    if (s->running) s->dead.push_back(it->second);  // destroyed by a worker, outside the lock
     s->tables.erase(it);
     s->cv_work.notify_all();
     return KC_OK;

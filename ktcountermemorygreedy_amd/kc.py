@@ -102,6 +102,7 @@ EXPORTS = {
     "kc_dev_free": (C.c_int, [C.c_void_p, C.c_void_p]),
     "kc_host_alloc_pinned": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p)]),
     "kc_host_free_pinned": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "kc_host_device_ptr": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]),
     "kc_copy_h2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
     "kc_copy_d2h": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
     "kc_copy_d2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
@@ -116,6 +117,7 @@ EXPORTS = {
     "kc_table_count_seq_host": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
     "kc_table_increment": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
     "kc_table_query": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
+    "kc_table_query_on": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
     "kc_table_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
     "kc_table_set_pass": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32]),
     "kc_table_export_dir": (C.c_int, [C.c_void_p, C.c_char_p]),
@@ -489,12 +491,18 @@ class Table:
         check(lib().kc_table_increment(self.h, _ptr(dk), n, _ptr(di)), "kc_table_increment")
         self.ctx.sync()
 
-    def query(self, keys: np.ndarray) -> np.ndarray:
+    def query(self, keys: np.ndarray, ctx: "Context" = None) -> np.ndarray:
+        """Point lookups; with ctx (another context on the same device) the
+        lookup runs on that context's stream (kc_table_query_on)."""
         keys = np.ascontiguousarray(keys, dtype=np.uint64)
         n = keys.size // self.nw
-        dk = self.ctx.upload(keys)
-        out = self.ctx.alloc(4 * n)
-        check(lib().kc_table_query(self.h, _ptr(dk), n, _ptr(out)), "kc_table_query")
+        c = ctx or self.ctx
+        dk = c.upload(keys)
+        out = c.alloc(4 * n)
+        if ctx is None:
+            check(lib().kc_table_query(self.h, _ptr(dk), n, _ptr(out)), "kc_table_query")
+        else:
+            check(lib().kc_table_query_on(self.h, ctx.h, _ptr(dk), n, _ptr(out)), "kc_table_query_on")
         return out.numpy(np.uint32, n)
 
     def stats(self):

@@ -272,6 +272,9 @@ class ServingRun:
             "q64k_wrong": int(sum(b for _, _, _, b in q64)),
             "q64k_p50_ms": pct(lat(q64), 50), "q64k_p99_ms": pct(lat(q64), 99),
             "q64k_max_ms": max(lat(q64)) if q64 else None,
+            # of which waiting in the server's queue (submit to start)
+            "q64k_queue_p50_ms": pct([r["queue_ms"] for _, _, r, _ in q64 if r["status"] == kc.KC_OK], 50),
+            "q64k_queue_p99_ms": pct([r["queue_ms"] for _, _, r, _ in q64 if r["status"] == kc.KC_OK], 99),
             "q64k_idle": len(idle), "q64k_idle_wrong": int(sum(b for _, _, _, b in idle)),
             "q64k_idle_failed": sum(1 for _, _, r, _ in idle if r["status"] != kc.KC_OK),
             "q64k_idle_p50_ms": pct(lat(idle), 50), "q64k_idle_p99_ms": pct(lat(idle), 99),

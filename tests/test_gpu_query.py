@@ -81,3 +81,61 @@ def test_query_saturates_with_first_list(ctx):
     assert (cnt == 9).all() and t.stats()["low"] == uniq.shape[0]
     assert (t.query(uniq) == 5).all()
     t.close()
+
+
+@pytest.mark.parametrize("path", ["direct", "partitioned"])
+@pytest.mark.parametrize("k", [31, 63])
+def test_query_on_another_context(ctx, k, path):
+    """kc_table_query_on: lookups on a second context's stream after the
+    table's context was synchronized, and a count enqueued on the table's
+    stream right after them waits for them."""
+    from ktcountermemorygreedy_amd import kc
+    a, b = _halves()
+    nw = O.words_for_k(k)
+    ua, ca = O.count(O.extract(a, k), nw)
+    uab, cab = O.count(O.extract(a + b, k), nw)
+    other = kc.Context(0)
+    t = kc.Table(ctx, k, capacity=1 << 23, bloom=True, max_distinct=uab.shape[0], path=path)
+    da = ctx.upload(np.frombuffer(a, dtype=np.uint8))
+    db = ctx.upload(np.frombuffer(b, dtype=np.uint8))
+    t.count_seq(da, len(a))
+    ctx.sync()
+    got = t.query(ua, ctx=other)
+    t.count_seq(db, len(b))
+    assert np.array_equal(got.astype(np.uint64), ca)
+    assert np.array_equal(t.query(uab, ctx=other).astype(np.uint64), cab)
+    assert np.array_equal(t.query(uab).astype(np.uint64), cab)
+    t.close()
+    da.free()
+    db.free()
+    other.close()
+
+
+def test_query_on_mapped_host_memory(ctx):
+    """A lookup that reads its keys from and writes its counts to page-locked
+    host memory in place (kc_host_device_ptr), as the serving loop does."""
+    import ctypes as C
+    from ktcountermemorygreedy_amd import kc
+    a, _ = _halves()
+    ua, ca = O.count(O.extract(a, 31), 1)
+    t = kc.Table(ctx, 31, capacity=1 << 22, bloom=True, max_distinct=ua.shape[0])
+    da = ctx.upload(np.frombuffer(a, dtype=np.uint8))
+    t.count_seq(da, len(a))
+    ctx.sync()
+    other = kc.Context(0)
+    n = ua.shape[0]
+    hk = kc.PinnedBuffer(other, 8 * n)
+    hc = kc.PinnedBuffer(other, 4 * n)
+    hk.array[:] = np.ascontiguousarray(ua[:, 0]).view(np.uint8)
+    hc.array[:] = 0xFF
+    dk, dc = C.c_void_p(), C.c_void_p()
+    kc.check(kc.lib().kc_host_device_ptr(other.h, C.c_void_p(hk.ptr), C.byref(dk)), "kc_host_device_ptr")
+    kc.check(kc.lib().kc_host_device_ptr(other.h, C.c_void_p(hc.ptr), C.byref(dc)), "kc_host_device_ptr")
+    kc.check(kc.lib().kc_table_query_on(t.h, other.h, dk, n, dc), "kc_table_query_on")
+    other.sync()
+    assert np.array_equal(hc.array.view(np.uint32).astype(np.uint64), ca)
+    hk.free()
+    hc.free()
+    t.close()
+    da.free()
+    other.close()
