@@ -53,7 +53,7 @@ typedef struct kc_table_config {
     uint64_t max_distinct; /* sizing hint when capacity == 0 */
     uint32_t bloom;        /* 1 = Bloom singleton split (BloomFilter.h:149-298) */
     uint32_t count_path;   /* KC_PATH_AUTO / _DIRECT / _PARTITIONED */
-    uint64_t bloom_blocks; /* {BF1,BF2} blocks, power of two; 0 = derive */
+    uint64_t bloom_blocks; /* 64-bit Bloom blocks, power of two; 0 = derive */
     uint64_t low_capacity; /* FIRST-occurrence list capacity; 0 = derive */
 } kc_table_config;
 
@@ -273,8 +273,8 @@ typedef struct kc_plan {
  * large inputs so the probe table stays below 2^26 keys) of the framed input
  * d_seq exactly in a scratch table, scales its distinct and repeated keys,
  * and fills cfg (k, count_max, filters, bloom, count_path as given) with
- * capacity (table keys at <= ~62% load), bloom_blocks (2-4 distinct keys per
- * block), low_capacity and max_distinct for ONE of plan->passes passes
+ * capacity (table keys at <= ~62% load), bloom_blocks (3.7-7.4 distinct keys per
+ * 64-bit block), low_capacity and max_distinct for ONE of plan->passes passes
  * (hbm_budget = 0: no limit, one pass).  Synchronous; the scratch table is
  * freed before returning. */
 int kc_plan_table(kc_ctx *ctx, const uint8_t *d_seq, uint64_t nbytes, uint32_t sample_slices, uint64_t hbm_budget,

@@ -69,7 +69,7 @@ __global__ __launch_bounds__(kBlock) void k_count_seq(SeqSrc src, TableView tv)
             const KeyHash kh = key_hash<NW>(tv, key);
             bool to_table = true;
             if constexpr (BLOOM) {
-                const bool first = bloom_classify(tv, kh) == 0;
+                const bool first = bloom_first(tv, kh);
                 // each byte position of the tile appends to its own segment,
                 // so that even a grid of a few blocks fills every segment
                 const bool kept = first_append<NW>(tv, first, key, step * kParseSeg + e);
@@ -765,7 +765,7 @@ static int init_slots_bloom(kc_table *t)
     else
         hipLaunchKernelGGL(k_init_slots<2>, dim3(grid_for(t->cap, c->cu_count)), dim3(kBlock), 0, c->stream, t->slots, t->cap);
     KC_HIP(hipGetLastError());
-    if (t->bloom) return kc_launch_fill(c, t->bloom, 2 * t->bloom_blocks, 0);
+    if (t->bloom) return kc_launch_fill(c, t->bloom, t->bloom_blocks, 0);
     return KC_OK;
 }
 
@@ -786,7 +786,7 @@ static int table_init_device(kc_table *t)
         hipLaunchKernelGGL(k_init_slots<2>, dim3(grid_for(t->cap, c->cu_count)), dim3(kBlock), 0, c->stream, t->slots, t->cap);
     KC_HIP(hipGetLastError());
     int rc;
-    if (t->bloom && (rc = kc_launch_fill(c, t->bloom, 2 * t->bloom_blocks, 0)) != KC_OK) return rc;
+    if (t->bloom && (rc = kc_launch_fill(c, t->bloom, t->bloom_blocks, 0)) != KC_OK) return rc;
     if ((rc = kc_launch_fill(c, t->counters, KC_COUNTER_WORDS, 0)) != KC_OK) return rc;
     t->fresh = false;
     kc_table_touch(t);
@@ -854,7 +854,7 @@ extern "C" int kc_table_create(kc_ctx *ctx, const kc_table_config *cfg, kc_table
     }
     t->n_tiles = (t->cap + kExportTile - 1) / kExportTile;
     const uint64_t slot_bytes = t->nw == 1 ? 16 : 32;
-    const uint64_t need = t->cap * slot_bytes + 16 * t->bloom_blocks + 8ull * t->nw * t->low_cap +
+    const uint64_t need = t->cap * slot_bytes + 8 * t->bloom_blocks + 8ull * t->nw * t->low_cap +
                           8 * KC_COUNTER_WORDS + 16 * (t->n_tiles + 1);
     size_t free_b = 0, total_b = 0;
     KC_HIP(hipMemGetInfo(&free_b, &total_b));
@@ -866,7 +866,7 @@ extern "C" int kc_table_create(kc_ctx *ctx, const kc_table_config *cfg, kc_table
               hipMalloc((void **)&t->counters, 8 * KC_COUNTER_WORDS) == hipSuccess &&
               hipMalloc((void **)&t->tile_counts, 16 * (t->n_tiles + 1)) == hipSuccess &&
               hipMalloc((void **)&t->seg_off, 8 * (KC_LOW_SEGS + 1)) == hipSuccess;
-    if (ok && t->bloom_blocks) ok = hipMalloc((void **)&t->bloom, 16 * t->bloom_blocks) == hipSuccess;
+    if (ok && t->bloom_blocks) ok = hipMalloc((void **)&t->bloom, 8 * t->bloom_blocks) == hipSuccess;
     if (ok && t->low_cap) ok = hipMalloc((void **)&t->low, 8ull * t->nw * t->low_cap) == hipSuccess;
     if (!ok) {
         kc_table_destroy(t);

@@ -1,6 +1,6 @@
 // Device-side pieces shared by the counting kernels: the table view and its
 // partitioned slot indexing, the global-atomic insert/lookup, the Bloom
-// state machine, the rolling canonical encoder and counter helpers.
+// singleton split, the rolling canonical encoder and counter helpers.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -13,11 +13,10 @@ namespace kc {
 constexpr uint32_t kValidMask = (1u << 0) | (1u << 2) | (1u << 6) | (1u << 19) | (1u << 20);
 
 // The table is cut into 2^part_bits partitions of part_slots slots (and
-// the Bloom filter into partitions of bloom_part blocks).  A key's partition
-// is the top part_bits of a multiply-shift mix of the key; inside the
-// partition the slot probe starts at bits of XXH3_128 and wraps within the
-// partition, and the Bloom block is the reference's h1 & (blocks - 1) within
-// the partition.
+// the Bloom filter into partitions of bloom_part 64-bit blocks).  A key's
+// partition is the top part_bits of coarse_mix; inside the partition the
+// slot probe start, the Bloom block and the Bloom mask come from slot_mix
+// (KeyHash below), and the probe wraps within the partition.
 struct TableView {
     uint64_t *slots;
     uint64_t mask;               // total slots - 1
@@ -53,20 +52,31 @@ __device__ __forceinline__ bool export_filtered(uint32_t c, const TableView &tv)
 
 // A table of 2^L partitions splits L into a coarse digit (the top
 // min(L, 8) bits) and a fine digit (the other L - min(L, 8) bits); the two
-// scatter levels sort by them in turn.  Both come from one cheap
-// multiply-shift mix of the key (32 bits; L <= 18): the coarse digit is its
-// top bits, the fine digit the bits below.  Its top 18 bits spread the keys
-// of repeat-rich input over the partitions as evenly as XXH3 does
-// (Poisson-distributed loads).
+// scatter levels sort by them in turn.  Both come from one cheap mix of the
+// key (32 bits; L <= 18): the coarse digit is its top bits, the fine digit
+// the bits below.  The parse passes compute it for every window, so it is
+// kept to one 32-bit multiply (two for two-word keys): each key word is
+// folded to 32 bits (low half xor the high half rotated by 13), the fold is
+// xor-shifted by 16 so that its high bits reach the product's top bits, and
+// multiplied by an odd constant.  On repeat-rich and error-rich k = 21, 31
+// and 63 keys its top 16 and 18 bits load the partitions as a Poisson law
+// does (standard deviation / sqrt(mean) within 1% of 1, as for the 64-bit
+// multiply-shift it replaced).
 constexpr uint32_t kCoarseMaxBits = 8;
+
+__device__ __forceinline__ uint32_t fold32(uint64_t w)
+{
+    const uint32_t hi = (uint32_t)(w >> 32);
+    return (uint32_t)w ^ ((hi << 13) | (hi >> 19));
+}
 
 template <int NW>
 __device__ __forceinline__ uint32_t coarse_mix(const uint64_t *key)
 {
-    uint64_t x = key[0];
-    if constexpr (NW == 2) x ^= ((key[1] >> 32) | (key[1] << 32)) * 0xC2B2AE3D27D4EB4FULL;
-    x ^= x >> 29;
-    return (uint32_t)((x * 0x9E3779B97F4A7C15ULL) >> 32);
+    uint32_t y = fold32(key[0]);
+    if constexpr (NW == 2) y = (y * 0x85EBCA77u) ^ fold32(key[1]);
+    y ^= y >> 16;
+    return y * 0x9E3779B1u;
 }
 
 __device__ __forceinline__ uint32_t coarse_bits(uint32_t part_bits)
@@ -247,26 +257,24 @@ __device__ __forceinline__ uint64_t table_find(const TableView &tv, const uint64
     return s ? s[NW == 1 ? 1 : 2] : 0;
 }
 
-// Bloom singleton split state machine (BloomFilter.h:233-273): returns
-// 0 FIRST, 1 SECOND, 2 THIRD_PLUS.  Only atomic results decide a FIRST, so
-// each distinct key gets at most one FIRST.
-__device__ __forceinline__ int bloom_classify(const TableView &tv, const KeyHash &kh)
+// Bloom singleton split (BloomFilter.h:233-273, FastqClassifier.h:641-650):
+// true when this occurrence is the key's FIRST.  The reference keeps two
+// words per block, BF2 (seen once) and BF1 (seen twice), and classifies
+// FIRST / SECOND / THIRD_PLUS; its counter sends SECOND and THIRD_PLUS down
+// the same path, and BF1 only ever receives bits BF2 holds, so the FIRST
+// decision depends on BF2 alone: FIRST exactly when the fetch_or of the
+// key's mask into BF2 finds a bit of the mask missing.  One word per block
+// is kept here.  Only atomic results decide a FIRST, so each distinct key
+// gets at most one FIRST; a plain read settles keys already seen.
+__device__ __forceinline__ bool bloom_first(const TableView &tv, const KeyHash &kh)
 {
     uint64_t blk, m;
     bloom_in_part(kh.h, (1ULL << tv.bloom_part_log2) - 1, blk, m);
     blk |= kh.part << tv.bloom_part_log2;
-    uint64_t *cell = tv.bloom + 2 * blk;
-    if ((ld_agent(cell) & m) == m) return 2;
-    if ((ld_agent(cell + 1) & m) == m) {
-        const uint64_t b1 = __hip_atomic_fetch_or(cell, m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        return ((b1 & m) == m) ? 2 : 1;
-    }
-    const uint64_t b2 = __hip_atomic_fetch_or(cell + 1, m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if ((b2 & m) == m) {
-        const uint64_t b1 = __hip_atomic_fetch_or(cell, m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        return ((b1 & m) == m) ? 2 : 1;
-    }
-    return 0;
+    uint64_t *cell = tv.bloom + blk;
+    if ((ld_agent(cell) & m) == m) return false;
+    const uint64_t b = __hip_atomic_fetch_or(cell, m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return (b & m) != m;
 }
 
 // ---- rolling canonical state (GetKmer.h:58-82) -------------------------

@@ -230,13 +230,16 @@ __device__ __forceinline__ void bloom_probe(const uint64_t *w, uint64_t cap_mask
 // ---- the table's own key hash (device layout, not a reference hash) ------
 // Where a key sits inside its partition: its Bloom block (low bits), the
 // three Bloom bits (bits 32..43) and the slot probe start (bits 44..63), all
-// from one 64-bit mix of the key words: two multiply rounds with 32-bit
-// xor-shifts, which a 32-bit ALU does as a word swap.  The reference's
-// probe (bloom_from) takes block and mask from XXH3_128, whose first mask
-// bit is the block's low bits, so a key's mask in its block depends on 6
-// bits only; three positions a, a+b, a+2b (b odd, so distinct) from 12 free
-// bits give ~2^11 masks per block and about a third of the false positives
-// at the same size, for a third of the multiplies.
+// from one 64-bit mix of the key words: a 32-bit xor-shift (a word swap on a
+// 32-bit ALU), one 64-bit multiply and a final 32-bit xor-shift.  (A second
+// multiply round left the block, mask and probe-start bits of repeat-rich
+// k = 21 and 31 keys inside each partition no more uniform: both give
+// Poisson counts.)  The reference's probe (bloom_from) takes block and mask
+// from XXH3_128, whose first mask bit is the block's low bits, so a key's
+// mask in its block depends on 6 bits only; three positions a, a+b, a+2b
+// (b odd, so distinct) from 12 free bits give ~2^11 masks per block and
+// about a third of the false positives at the same size, for a fraction of
+// the multiplies.
 constexpr uint64_t kSlotMixC = 0xD6E8FEB86659FD93ULL;
 constexpr uint64_t kSlotFoldC = 0x9E3779B97F4A7C15ULL;
 
@@ -245,7 +248,6 @@ __device__ __forceinline__ uint64_t slot_mix(const uint64_t *w)
 {
     uint64_t x = w[0];
     if constexpr (NW == 2) x ^= w[1] * kSlotFoldC;
-    x = (x ^ (x >> 32)) * kSlotMixC;
     x = (x ^ (x >> 32)) * kSlotMixC;
     return x ^ (x >> 32);
 }

@@ -627,8 +627,10 @@ extern "C" int kc_count_fastq_files_passes(kc_table *t, const char *const *paths
 // ---- table sizing from a device sample -----------------------------------
 namespace {
 
-// Slots of one table partition of the LDS counting pass (Bloom split on).
+// Slots and 64-bit Bloom blocks of one table partition of the LDS counting
+// pass (Bloom split on).
 uint64_t kc_lds_slots(uint32_t nw) { return nw == 1 ? 4096 : 2048; }
+uint64_t kc_lds_bloom(uint32_t) { return 4096; }
 
 uint64_t pow2_at_most(uint64_t v)
 {
@@ -686,14 +688,14 @@ extern "C" int kc_plan_sizes(uint64_t distinct_total, uint64_t repeated_total, u
     plan->repeated = std::min(repeated_total, distinct_total);
     // one pass, then halve per pass until the device bytes fit the budget
     const uint64_t slot_bytes = nw == 1 ? 16 : 32;
-    const uint64_t lds_bloom = 2048;  // Bloom blocks of one partition in the LDS pass
+    const uint64_t lds_bloom = kc_lds_bloom(nw);  // Bloom blocks of one partition in the LDS pass
     const uint64_t max_bloom = lds_bloom << 18;  // the partition pass takes 2^18 Bloom parts
     uint32_t P = 1;
     for (;;) {
         const uint64_t distinct = plan->distinct / P + 1024, repeated = plan->repeated / P + 1024;
         uint64_t cap, bloom = 0, low = 0;
         if (cfg->bloom) {
-            // 3.7-7.4 distinct keys per {BF1, BF2} block, up to what the
+            // 3.7-7.4 distinct keys per 64-bit block, up to what the
             // partition pass takes (then more keys per block, at most 8);
             // its false positives send singletons to the table as well
             bloom = std::min(max_bloom, std::max<uint64_t>(pow2_at_least((uint64_t)((double)distinct / 7.4)), 1 << 12));
@@ -703,12 +705,15 @@ extern "C" int kc_plan_sizes(uint64_t distinct_total, uint64_t repeated_total, u
             // and at least the LDS-sized partitions the filter's parts need
             // (the table is split into more, smaller partitions otherwise)
             bloom = std::max(bloom, cap / kc_lds_slots(nw) * lds_bloom);
-            cap = std::max(cap, bloom * 256 / lds_bloom);  // partitions of >= 256 slots
+            // partitions of >= 512 slots: the load of a small partition
+            // strays further from the table's mean (Poisson), and a full
+            // partition cannot borrow from its neighbours
+            cap = std::max(cap, bloom * 512 / lds_bloom);
             low = distinct + distinct / 20 + (1 << 16);
         } else {
             cap = pow2_at_least(std::max<uint64_t>(distinct + distinct * 3 / 5, 1 << 16));
         }
-        const uint64_t bytes = cap * slot_bytes + 16 * bloom + 8ull * nw * low;
+        const uint64_t bytes = cap * slot_bytes + 8 * bloom + 8ull * nw * low;
         const bool fits = (!hbm_budget || bytes <= hbm_budget) && (!cfg->bloom || distinct <= 8 * bloom);
         if (fits || P == 256) {
             cfg->capacity = cap;

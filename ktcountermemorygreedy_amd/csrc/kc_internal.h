@@ -85,7 +85,7 @@ struct kc_table {
     uint32_t bloom_part_log2; // log2(Bloom blocks per partition)
     uint64_t *slots;        // cap * (nw == 1 ? 2 : 4) words
     uint64_t bloom_blocks;  // 0 when Bloom is off
-    uint64_t *bloom;        // 2 * bloom_blocks words {BF1, BF2}
+    uint64_t *bloom;        // bloom_blocks words: one 64-bit block each
     uint64_t low_cap;       // KC_LOW_SEGS * seg_cap
     uint64_t seg_cap;       // keys per FIRST-list segment
     uint64_t *low;          // low_cap * nw words, segment s at s * seg_cap
@@ -134,7 +134,7 @@ int kc_table_materialize(kc_table *t);
 
 // Slots / Bloom blocks of one partition that the LDS counting pass holds.
 constexpr uint32_t kc_lds_part_slots(uint32_t nw, bool bloom) { return (nw == 1 ? 4096u : 2048u) << (bloom ? 0 : 1); }
-constexpr uint32_t kc_lds_part_bloom(uint32_t) { return 2048u; }
+constexpr uint32_t kc_lds_part_bloom(uint32_t) { return 4096u; }
 
 // Grid of a FIRST-list walk (see low_seg_key): enough chunks of `per_block`
 // keys for the fullest segment, at most 64.

@@ -3,8 +3,8 @@
 // (NewKmerTree.h:526-570).  Where the reference moves 4 KiB blocks down a
 // 256 x 16 x 16 x 16 tree by the top bits of the key, this pass sends every
 // k-mer to one of 2^L table partitions by hash digits (a coarse digit of up
-// to 8 bits and a fine digit from the bits below it, of one multiply-shift
-// mix of the key), in two exact scatter levels, and then counts each
+// to 8 bits and a fine digit from the bits below it, of one cheap mix of
+// the key), in two exact scatter levels, and then counts each
 // partition in an LDS-resident copy of its sub-table (and Bloom part):
 //
 //   front count    G persistent blocks; block b parses tiles b, b+G, ... and
@@ -63,6 +63,7 @@ struct Digits {
     uint32_t l2;         // fine bits = L - l1
     uint32_t pass_bits;  // multi-pass counting: only keys of `pass` enter
     uint32_t pass;
+    uint32_t dig16;      // the front scatter writes fine digits as u16 (else u8)
 };
 
 template <int NW>
@@ -83,14 +84,23 @@ __device__ __forceinline__ uint32_t fine_of(const uint64_t *key, const Digits &d
     return fine_digit(coarse_mix<NW>(key), d.l1, d.l2);
 }
 
-// The front scatters write each key's fine digit beside it (u8, or u16 above
-// 256 fine digits), so that the fine count reads 1-2 bytes per key instead
-// of the key.  dig is null when there is no fine level.
+// The front scatters write each key's fine digit beside it, so that the
+// fine count reads 1-2 bytes per key instead of the key.  The digits of one
+// chunk land as one short run per coarse digit, and runs under 32 bytes
+// cost far more per byte than their size (the front scatter of 4096-key
+// chunks, 16-key runs, took 13.5 ms with u8 digits and 11.7 ms with u16 on
+// 1.9e9 63-mers; 8192-key chunks took 11.8 ms with u8 and 13.7 ms with u16 on
+// 2.4e9 31-mers), so the digits are u16 above 256 fine digits and for
+// two-word keys, u8 otherwise.  dig is null when there is no fine level.
 __device__ __forceinline__ void put_digit(uint8_t *dig, uint64_t at, uint32_t f, const Digits &d)
 {
-    if (d.l2 <= 8) dig[at] = (uint8_t)f;
+    if (!d.dig16) dig[at] = (uint8_t)f;
     else reinterpret_cast<uint16_t *>(dig)[at] = (uint16_t)f;
 }
+
+// A digit whose chunk does not fit its region (scatter guard): no staged
+// key of it is written.  Real offsets differences stay far from it.
+constexpr uint64_t kNoDelta = 0x8000000000000000ull;
 
 // Exclusive scan of one value per thread of a 256-thread block: a shuffle
 // scan inside each wave, then the four wave totals (two barriers).
@@ -168,10 +178,11 @@ __global__ __launch_bounds__(kParseBlock) void k_front_count_seq(SeqSrc src, uin
 
 // Scatter guard.  A scatter writes each key at its digit region's offset
 // plus its rank, the region's size coming from a separate count pass.  Each
-// block keeps how many keys its region of every digit has left: a key past
-// the end is not written (it would land in the next region) and an error is
-// counted, and so is a region left with holes at the end.  Either makes the
-// counting call report KC_EFULL instead of returning wrong counts.
+// block keeps how many keys its region of every digit has left: when a chunk
+// holds more keys of a digit than that, none of them is written (they would
+// run into the next region) and an error is counted, and so is a region left
+// with holes at the end.  Either makes the counting call report KC_EFULL
+// instead of returning wrong counts.
 __device__ __forceinline__ void guard_report(unsigned long long *counters, uint32_t n_err)
 {
     stat_add(counters, KC_STAT_ERRORS, n_err);
@@ -195,21 +206,21 @@ __global__ __launch_bounds__(TB) void k_front_scatter_seq_staged(SeqSrc src, uin
     constexpr int kChunks = kParseSeg / CB;
     constexpr int kWords = CB / 4;
     constexpr int kStage = TB * CB;
+    static_assert(TB == 256, "thread t keeps the region of coarse digit t");
     __shared__ __align__(16) uint64_t stage[kStage * NW];
     __shared__ uint8_t sbin[kStage];
-    __shared__ uint32_t hist[256], loff[256], left[256];
-    __shared__ uint64_t base[256];
+    __shared__ uint32_t hist[256], loff[256];
+    __shared__ uint64_t delta[256];
     uint32_t *wsum = loff + 256 - TB / 64;  // the scan's wave sums: loff is written only after the scan
     const uint32_t t = threadIdx.x;
     const uint32_t nbins = 1u << d.l1;
     uint32_t n_err = 0;
     uint64_t n, emit_from;
     seq_bounds(src, n, emit_from);
-    for (uint32_t f = t; f < 256; f += TB) {
-        base[f] = f < nbins ? o1[(uint64_t)f * gridDim.x + blockIdx.x] : 0;
-        left[f] = f < nbins ? m1[(uint64_t)f * gridDim.x + blockIdx.x] : 0;
-        hist[f] = 0;
-    }
+    // thread t's digit: where its region continues and how many keys it has left
+    uint64_t my_base = t < nbins ? o1[(uint64_t)t * gridDim.x + blockIdx.x] : 0;
+    uint32_t my_left = t < nbins ? m1[(uint64_t)t * gridDim.x + blockIdx.x] : 0;
+    hist[t] = 0;
     __syncthreads();
     constexpr uint64_t kTileBytes = (uint64_t)TB * kParseSeg;
     const uint64_t ntiles = (n + kTileBytes - 1) / kTileBytes;
@@ -254,9 +265,14 @@ __global__ __launch_bounds__(TB) void k_front_scatter_seq_staged(SeqSrc src, uin
                 load_segment<HW>(src.seq, n, start + (uint64_t)gridDim.x * kTileBytes, w);
             __syncthreads();
             uint32_t total;
-            const uint32_t h = t < 256 ? hist[t] : 0;
+            const uint32_t h = hist[t];
             const uint32_t off = scan_block<TB>(h, wsum, total);
-            if (t < 256) loff[t] = off;
+            loff[t] = off;
+            // staged key i of digit t goes to i + delta[t]; a chunk with more
+            // keys of the digit than its region has left is not written
+            const bool fits = h <= my_left;
+            n_err += !fits;
+            delta[t] = fits ? my_base - off : kNoDelta;
             __syncthreads();
 #pragma unroll
             for (int e = 0; e < CB; ++e) {
@@ -269,13 +285,9 @@ __global__ __launch_bounds__(TB) void k_front_scatter_seq_staged(SeqSrc src, uin
             }
             __syncthreads();
             for (uint32_t i = t; i < total; i += TB) {
-                const uint32_t c = sbin[i];
-                const uint32_t r = i - loff[c];
-                if (r >= left[c]) {  // past the region the count pass gave this block
-                    ++n_err;
-                    continue;
-                }
-                const uint64_t at = base[c] + r;
+                const uint64_t dl = delta[sbin[i]];
+                if (dl == kNoDelta) continue;
+                const uint64_t at = i + dl;
                 if constexpr (NW == 1) {
                     out[at] = stage[i];
                 } else {
@@ -284,15 +296,13 @@ __global__ __launch_bounds__(TB) void k_front_scatter_seq_staged(SeqSrc src, uin
                 if (dig) put_digit(dig, at, fine_of<NW>(stage + i * NW, d), d);
             }
             __syncthreads();
-            if (t < 256) {
-                base[t] += h;
-                left[t] = h <= left[t] ? left[t] - h : 0;
-                hist[t] = 0;
-            }
+            my_base += h;
+            my_left = fits ? my_left - h : 0;
+            hist[t] = 0;
             __syncthreads();
         }
     }
-    for (uint32_t f = t; f < nbins; f += TB) n_err += left[f] != 0;  // keys the count pass saw and this one did not
+    n_err += my_left != 0;  // keys the count pass saw and this one did not
     guard_report(counters, n_err);
 }
 
@@ -461,9 +471,10 @@ __global__ __launch_bounds__(TB) void k_fine_scatter_staged(const uint64_t *__re
     using BinT = typename std::conditional<NF <= 256, uint8_t, uint16_t>::type;
     constexpr uint32_t CH = (uint32_t)TB * FPT;
     __shared__ __align__(16) uint64_t stage[CH * NW];
+    static_assert(NF <= TB, "thread t keeps the region of fine digit t");
     __shared__ BinT sbin[SB ? CH : 1];
-    __shared__ uint32_t hist[NF], loff[NF], left[NF];
-    __shared__ uint64_t base[NF];
+    __shared__ uint32_t hist[NF], loff[NF];
+    __shared__ uint64_t delta[NF];
     uint32_t *tmp = loff + NF - TB / 64;  // the scan's wave sums: loff is rewritten only after the scan
     uint32_t c, s;
     uint64_t beg, end;
@@ -472,11 +483,11 @@ __global__ __launch_bounds__(TB) void k_fine_scatter_staged(const uint64_t *__re
     const uint32_t t = threadIdx.x;
     const uint64_t row = (uint64_t)c << d.l2;
     uint32_t n_err = 0;
-    for (uint32_t f = t; f < nf; f += TB) {
-        base[f] = o2[(row + f) * S + s];
-        left[f] = m2[(row + f) * S + s];
-        hist[f] = 0;
-    }
+    // thread t's digit: where its region continues and how many keys it has left
+    const bool mine = t < nf;
+    uint64_t my_base = mine ? o2[(row + t) * S + s] : 0;
+    uint32_t my_left = mine ? m2[(row + t) * S + s] : 0;
+    if (mine) hist[t] = 0;
     __syncthreads();
     // a chunk's FPT keys per thread (index clamped into the chunk)
     uint64_t key[FPT][NW];
@@ -508,15 +519,16 @@ __global__ __launch_bounds__(TB) void k_fine_scatter_staged(const uint64_t *__re
             }
         }
         __syncthreads();
-        // exclusive scan of hist[0..nf): each thread owns nf/TB consecutive digits
-        const uint32_t per = (nf + TB - 1) / TB;
-        uint32_t acc = 0;
-        for (uint32_t f = t * per; f < (t + 1) * per && f < nf; ++f) acc += hist[f];
+        const uint32_t h = mine ? hist[t] : 0;
         uint32_t total;
-        uint32_t run = scan_block<TB>(acc, tmp, total);
-        for (uint32_t f = t * per; f < (t + 1) * per && f < nf; ++f) {
-            loff[f] = run;
-            run += hist[f];
+        const uint32_t off = scan_block<TB>(h, tmp, total);
+        // staged key p of digit t goes to p + delta[t]; a chunk with more
+        // keys of the digit than its region has left is not written
+        const bool fits = h <= my_left;
+        if (mine) {
+            loff[t] = off;
+            n_err += !fits;
+            delta[t] = fits ? my_base - off : kNoDelta;
         }
         __syncthreads();
 #pragma unroll
@@ -537,12 +549,9 @@ __global__ __launch_bounds__(TB) void k_fine_scatter_staged(const uint64_t *__re
             uint32_t f;
             if constexpr (SB) f = sbin[p];
             else f = fine_of<NW>(stage + p * NW, d);
-            const uint32_t r = p - loff[f];
-            if (r >= left[f]) {  // past this slice's region (scatter guard)
-                ++n_err;
-                continue;
-            }
-            const uint64_t g = base[f] + r;
+            const uint64_t dl = delta[f];
+            if (dl == kNoDelta) continue;
+            const uint64_t g = p + dl;
             if constexpr (NW == 2) {
                 *reinterpret_cast<ulonglong2 *>(out + g * 2) = *reinterpret_cast<const ulonglong2 *>(stage + p * 2);
             } else {
@@ -550,14 +559,14 @@ __global__ __launch_bounds__(TB) void k_fine_scatter_staged(const uint64_t *__re
             }
         }
         __syncthreads();
-        for (uint32_t f = t; f < nf; f += TB) {
-            base[f] += hist[f];
-            left[f] = hist[f] <= left[f] ? left[f] - hist[f] : 0;
-            hist[f] = 0;
+        if (mine) {
+            my_base += h;
+            my_left = fits ? my_left - h : 0;
+            hist[t] = 0;
         }
         __syncthreads();
     }
-    for (uint32_t f = t; f < nf; f += TB) n_err += left[f] != 0;
+    n_err += mine && my_left != 0;  // keys the count pass saw and this one did not
     guard_report(counters, n_err);
 }
 
@@ -580,26 +589,20 @@ struct PartLds {
     static constexpr uint32_t BS = BLOOM ? kc_lds_part_bloom(NW) : 1;
     unsigned long long key[PS * NW];  // word 0 at [s], word 1 at [PS + s]
     uint32_t cnt[PS];                 // occurrences added in this pass
-    unsigned long long bloom[2 * BS]; // {BF1, BF2} per block
+    unsigned long long bloom[BS];     // one 64-bit Bloom block each
 };
 
-// Bloom state machine of BloomFilter.h:233-273 on the LDS copy (block and
-// mask from bloom_in_part): 0 FIRST,
-// 1 SECOND, 2 THIRD_PLUS.  BF1 only ever receives bits that BF2 holds, so
-// the reference's plain read of BF1 before its RMWs can be folded into the
-// RMWs: fetch_or(BF2) decides FIRST (the one caller that completes the
-// mask), fetch_or(BF1) decides SECOND vs THIRD_PLUS -- the same outcomes for
-// every interleaving, with one branch instead of three.
+// The Bloom singleton split on the LDS copy (block and mask from
+// bloom_in_part): FIRST exactly when the fetch_or of the key's mask into its
+// block finds a bit missing (bloom_first in kc_dev_common.h: the reference's
+// BF1 word does not change that decision and is not kept).
 template <int NW, bool BLOOM>
-__device__ __forceinline__ int lds_bloom(PartLds<NW, BLOOM> &S, uint64_t h, uint32_t bmask)
+__device__ __forceinline__ bool lds_bloom_first(PartLds<NW, BLOOM> &S, uint64_t h, uint32_t bmask)
 {
     uint64_t blk, m;
     bloom_in_part(h, bmask, blk, m);
-    unsigned long long *cell = S.bloom + 2 * blk;
-    const uint64_t b2 = atomicOr(cell + 1, (unsigned long long)m);
-    if ((b2 & m) != m) return 0;
-    const uint64_t b1 = atomicOr(cell, (unsigned long long)m);
-    return ((b1 & m) == m) ? 2 : 1;
+    const uint64_t b = atomicOr(&S.bloom[blk], (unsigned long long)m);
+    return (b & m) != m;
 }
 
 template <int NW, bool BLOOM>
@@ -608,18 +611,33 @@ __device__ __forceinline__ bool lds_insert(PartLds<NW, BLOOM> &S, const uint64_t
 {
     constexpr uint32_t PS = PartLds<NW, BLOOM>::PS;
     uint32_t s = probe_start(h) & pmask;
-    for (uint32_t probe = 0; probe <= pmask; ++probe, s = (s + 1) & pmask) {
-        // one LDS CAS per probe: it claims an empty slot or returns the
-        // slot's key (a present key is the common case and is not written)
-        const uint64_t w0 = atomicCAS(&S.key[s], (unsigned long long)KC_EMPTY, (unsigned long long)key[0]);
-        const bool owner = w0 == KC_EMPTY;
-        if constexpr (NW == 1) {
-            if (owner || w0 == key[0]) {
-                n_new += owner;
-                atomicAdd(&S.cnt[s], 1u);
-                return true;
-            }
-        } else {
+    if constexpr (NW == 1) {
+        // One exit condition for the whole probe loop (claimed, found, or
+        // every slot tried) keeps its exec-mask bookkeeping short; the count
+        // is added once the loop has settled.
+        uint64_t w0;
+        uint32_t left = pmask + 1;
+        bool go;
+        do {
+            w0 = atomicCAS(&S.key[s], (unsigned long long)KC_EMPTY, (unsigned long long)key[0]);
+            const bool settled = (w0 == KC_EMPTY) | (w0 == key[0]);
+            --left;
+            go = !settled & (left != 0);
+            s = go ? (s + 1) & pmask : s;
+        } while (go);
+        if ((w0 == KC_EMPTY) | (w0 == key[0])) {
+            n_new += w0 == KC_EMPTY;
+            atomicAdd(&S.cnt[s], 1u);
+            return true;
+        }
+        ++n_err;
+        return false;
+    } else {
+        for (uint32_t probe = 0; probe <= pmask; ++probe, s = (s + 1) & pmask) {
+            // one LDS CAS per probe: it claims an empty slot or returns the
+            // slot's key (a present key is the common case and is not written)
+            const uint64_t w0 = atomicCAS(&S.key[s], (unsigned long long)KC_EMPTY, (unsigned long long)key[0]);
+            const bool owner = w0 == KC_EMPTY;
             if (owner)
                 __hip_atomic_store(&S.key[PS + s], (unsigned long long)key[1], __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -645,9 +663,9 @@ __device__ __forceinline__ bool lds_insert(PartLds<NW, BLOOM> &S, const uint64_t
                 }
             }
         }
+        ++n_err;
+        return false;
     }
-    ++n_err;
-    return false;
 }
 
 // Position of the n-th key counted by wave `wave` of a partition block (the
@@ -706,7 +724,7 @@ __global__ __launch_bounds__(kPartThreads) void k_count_parts(const uint64_t *__
     if constexpr (BLOOM) {
         bsz = 1u << tv.bloom_part_log2;
         bbase = part << tv.bloom_part_log2;
-        for (uint32_t b = threadIdx.x; b < 2 * bsz; b += kPartThreads) S.bloom[b] = FRESH ? 0 : tv.bloom[2 * bbase + b];
+        for (uint32_t b = threadIdx.x; b < bsz; b += kPartThreads) S.bloom[b] = FRESH ? 0 : tv.bloom[bbase + b];
     }
     __syncthreads();
 
@@ -782,7 +800,7 @@ __global__ __launch_bounds__(kPartThreads) void k_count_parts(const uint64_t *__
             const uint64_t h = slot_mix<NW>(key);  // Bloom probe and slot probe start
             bool to_table = valid;
             if constexpr (BLOOM) {
-                const bool first = valid && lds_bloom<NW, BLOOM>(S, h, bsz - 1) == 0;
+                const bool first = valid && lds_bloom_first<NW, BLOOM>(S, h, bsz - 1);
                 if constexpr (FM == kFirstStaged) {
                     const uint64_t fb = __ballot(first);
                     if (first) {
@@ -852,7 +870,7 @@ __global__ __launch_bounds__(kPartThreads) void k_count_parts(const uint64_t *__
         }
     }
     if constexpr (BLOOM) {
-        for (uint32_t b = threadIdx.x; b < 2 * bsz; b += kPartThreads) tv.bloom[2 * bbase + b] = S.bloom[b];
+        for (uint32_t b = threadIdx.x; b < bsz; b += kPartThreads) tv.bloom[bbase + b] = S.bloom[b];
     }
     stat_add(tv.counters, KC_STAT_KMERS, n_k);
     stat_add(tv.counters, KC_STAT_DISTINCT, n_new);
@@ -873,6 +891,7 @@ Digits digits_of(const kc_table *t)
     d.l2 = d.part_bits - d.l1;
     d.pass_bits = t->pass_bits;
     d.pass = t->pass;
+    d.dig16 = d.l2 > 8 || t->nw == 2;
     return d;
 }
 
@@ -940,7 +959,7 @@ int kc_part_reserve(const kc_table *t, kc_part_scratch &s, const kc_pipe_shape &
     uint64_t sc = s.sums_n, pc = s.part_n, stc = s.starts ? 257 : 0, dgc = s.dig_bytes;
     const uint64_t keys_words = std::max<uint64_t>(sh.key_cap, 1) * nw;
     // the digit array (+16: the fine count's last 16-byte load may reach past the final digit)
-    const uint64_t dig_need = d.l2 ? std::max<uint64_t>(sh.key_cap, 1) * (d.l2 > 8 ? 2 : 1) + 16 : 0;
+    const uint64_t dig_need = d.l2 ? std::max<uint64_t>(sh.key_cap, 1) * (d.dig16 ? 2 : 1) + 16 : 0;
     kc_a *= nw;
     kc_b *= nw;
     const bool ok = grow(s.keys_a, kc_a, keys_words) && grow(s.keys_b, kc_b, keys_words) &&
@@ -973,7 +992,7 @@ int kc_part_enqueue(const kc_table *t, kc_part_scratch &s, const kc_pipe_shape &
     // the shape the scratch was sized for must cover this launch
     const uint64_t m1n = (uint64_t)n_coarse * G, m2n = d.l2 ? n_parts * S : 0;
     const uint64_t need_keys = from_seq ? seq.n - std::min(seq.n, seq.hdr ? 0 : seq.emit_from) : n_keys;
-    const uint64_t dig_need = d.l2 ? s.key_cap * (d.l2 > 8 ? 2 : 1) + 16 : 0;
+    const uint64_t dig_need = d.l2 ? s.key_cap * (d.dig16 ? 2 : 1) + 16 : 0;
     if (s.key_cap < need_keys || s.key_cap < sh.key_cap || s.m1_n < m1n || s.m2_n < m2n ||
         s.sums_n < kc_scan_tmp_words(std::max(m1n, m2n)) || s.part_n < n_parts + 1 || !s.starts ||
         (d.l2 && (!s.dig || s.dig_bytes < dig_need)))
@@ -1015,7 +1034,7 @@ int kc_part_enqueue(const kc_table *t, kc_part_scratch &s, const kc_pipe_shape &
     const uint64_t *part_keys = s.keys_a;
     if (d.l2) {
         const dim3 g2(n_coarse * S);
-        if (d.l2 <= 8) hipLaunchKernelGGL(k_fine_count_dig<uint8_t>, g2, dim3(kKeyBlock), 0, st, s.dig, s.starts, d, S, s.m2);
+        if (!d.dig16) hipLaunchKernelGGL(k_fine_count_dig<uint8_t>, g2, dim3(kKeyBlock), 0, st, s.dig, s.starts, d, S, s.m2);
         else hipLaunchKernelGGL(k_fine_count_dig<uint16_t>, g2, dim3(kKeyBlock), 0, st, reinterpret_cast<const uint16_t *>(s.dig), s.starts, d, S, s.m2);
         KC_HIP(hipGetLastError());
         if (t->fault_skew == 3 || t->fault_skew == 4)

@@ -154,7 +154,7 @@ struct kc_server {
         r.resident = r.capacity * 16 * nw + (1 << 20);
         if (r.bloom) {
             const uint64_t low = want + want / 8 + 4096 * 1024;
-            r.resident += 16 * (r.capacity / 2) + 8 * nw * low + 8 * nw * next_pow2(2 * want);
+            r.resident += 8 * r.capacity + 8 * nw * low + 8 * nw * next_pow2(2 * want);
         }
         const uint64_t chunk = std::min<uint64_t>(r.nbytes + 64, 256ull << 20);
         return r.resident + 2 * chunk + 20 * nw * chunk;

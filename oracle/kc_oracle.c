@@ -773,8 +773,7 @@ uint64_t kco_concurrent_map_hash(const uint64_t *w, uint32_t nw)
 }
 
 /* ------------------------------------------------------------------ */
-/* Bloom: BloomFilter.h:215-222 (probe), :287-297 (3-bit mask), :233-273
- * (FIRST / SECOND / THIRD_PLUS state machine over {BF1, BF2} words). */
+/* Bloom probe: BloomFilter.h:215-222 (block), :287-297 (3-bit mask). */
 void kco_bloom_probe(const uint64_t *key, uint32_t nw, uint64_t capacity,
                      uint64_t *block_out, uint64_t *mask_out)
 {

@@ -90,7 +90,7 @@ def test_bloom_splits_partitions(ctx, k, path):
     keys = O.extract(seq, k)
     uniq, cnt = O.count(keys, nw)
     cap = 1 << 23  # room for every distinct key: point lookups fold the FIRST list in
-    lds_slots, lds_bloom = (4096, 2048) if nw == 1 else (2048, 2048)
+    lds_slots, lds_bloom = (4096, 4096) if nw == 1 else (2048, 4096)
     bloom_blocks = 8 * (cap // lds_slots) * lds_bloom
     t = kc.Table(ctx, k, capacity=cap, bloom=True, bloom_blocks=bloom_blocks, max_distinct=len(seq), path=path)
     d = ctx.upload(np.frombuffer(seq, dtype=np.uint8))

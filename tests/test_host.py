@@ -79,12 +79,13 @@ def test_merge_sorted_dumps(tmp_path):
 def test_plan_sizes_reproduces_device_plans():
     """kc_plan_sizes is the sizing half of kc_plan_table: fed the scaled
     totals the device probe reported on two MI355X workloads it returns the
-    same device bytes, and those sizes held what the full count then found
-    (table keys, FIRST occurrences).  Totals and outcomes are those runs'."""
+    device bytes of those plans (Bloom blocks of 8 bytes), and those sizes
+    held what the full count then found (table keys, FIRST occurrences).
+    Totals and outcomes are those runs'."""
     from ktcountermemorygreedy_amd import kc
     runs = [  # k, distinct, repeated, device_bytes, measured table keys, FIRST occurrences
         (31, 391599168, 101114048, 9732416840, 101989117, 391114715),    # 20M x 150 bp over 100 Mbp
-        (63, 1836491392, 40757888, 39444055744, 58607250, 1818313084),  # 1.16M uneven reads over 50 Mbp
+        (63, 1836491392, 40757888, 37296572096, 58607250, 1818313084),  # 1.16M uneven reads over 50 Mbp
     ]
     for k, distinct, repeated, dev_bytes, table_keys, firsts in runs:
         kw, plan = kc.plan_sizes(k, distinct, repeated, bloom=True)
@@ -95,7 +96,7 @@ def test_plan_sizes_reproduces_device_plans():
         assert firsts <= kw["low_capacity"]
         assert distinct <= 8 * bloom
         slot = 16 if k <= 32 else 32
-        assert dev_bytes == cap * slot + 16 * bloom + 8 * (k > 32) * kw["low_capacity"] + 8 * kw["low_capacity"]
+        assert dev_bytes == cap * slot + 8 * bloom + 8 * (k > 32) * kw["low_capacity"] + 8 * kw["low_capacity"]
 
 
 def test_plan_sizes_passes_and_budget():
