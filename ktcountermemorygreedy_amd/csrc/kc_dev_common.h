@@ -10,8 +10,6 @@
 
 namespace kc {
 
-constexpr uint32_t kValidMask = (1u << 0) | (1u << 2) | (1u << 6) | (1u << 19) | (1u << 20);
-
 // The table is cut into 2^part_bits partitions of part_slots slots (and
 // the Bloom filter into partitions of bloom_part 64-bit blocks).  A key's
 // partition is the top part_bits of coarse_mix; inside the partition the
@@ -278,10 +276,11 @@ __device__ __forceinline__ bool bloom_first(const TableView &tv, const KeyHash &
 }
 
 // ---- rolling canonical state (GetKmer.h:58-82) -------------------------
-// Four input bytes at once: `codes` gets base_code's 2-bit code of every
-// byte in that byte's low bits; the result has bit 7 set in every byte
-// base_code calls invalid.  A byte is valid when it equals (case folded) the
-// base its code names (A C G T) or is U, which shares T's code.
+// Four input bytes at once (FastqParser.h:280-305 semantics, GetKmer.h
+// table: A/a=0 C/c=1 G/g=2 T/t/U/u=3, anything else invalid): `codes` gets
+// the 2-bit code of every byte in that byte's low bits; the result has bit 7
+// set in every invalid byte.  A byte is valid when it equals (case folded)
+// the base its code names (A C G T) or is U, which shares T's code.
 __device__ __forceinline__ uint32_t invalid_bytes(uint32_t w, uint32_t &codes)
 {
     codes = ((w >> 1) ^ (w >> 2)) & 0x03030303u;
@@ -327,10 +326,34 @@ __device__ __forceinline__ void pack_history(const uint32_t *h, uint64_t *packed
     }
 }
 
+// The codes, complements and invalid flags of four input bytes
+// (invalid_bytes): byte b's code at bits 8b..8b+1 of codes, code ^ 3 at the
+// same bits of comps, bit 8b+7 of iv set when the byte is not a base.
+struct Word4 {
+    uint32_t codes, comps, iv;
+};
+
+__device__ __forceinline__ Word4 word4(uint32_t v)
+{
+    Word4 x;
+    x.iv = invalid_bytes(v, x.codes);
+    x.comps = x.codes ^ 0x03030303u;
+    return x;
+}
+
 template <int NW> struct Roll;
 
+// The rolling state of GetKmer.h:58-82, kept so that a push is four 32-bit
+// operations per key word: the forward codes with the newest at the top and
+// the older ones below it, not masked (canon() keeps the top 2k bits), and
+// the complements bottom-aligned, newest at bits 0-1 (canon() shifts them up
+// by rev_shift, which drops the bases older than the window).  Shifts across
+// words are funnel shifts (v_alignbit).  canon() returns the key layout of
+// kmer<N>: the forward word(s) masked and the reverse complement, whichever
+// is numerically smaller.
 template <> struct Roll<1> {
-    uint64_t f = 0, r = 0;
+    uint32_t f0 = 0, f1 = 0;  // forward, f1:f0
+    uint32_t r0 = 0, r1 = 0;  // complement, r1:r0, bottom-aligned
     uint64_t back_mask;
     uint32_t rev_shift;
     __device__ Roll(uint32_t k)
@@ -338,28 +361,40 @@ template <> struct Roll<1> {
         back_mask = (k % 32) ? (((1ULL << (2 * (k % 32))) - 1ULL) << (64 - 2 * (k % 32))) : ~0ULL;
         rev_shift = (k % 32) ? 2 * (32 - k % 32) : 0;
     }
-    __device__ __forceinline__ void push(uint64_t code)
+    // code: the new base's code in bits 0-1 (higher bits ignored); comp:
+    // code ^ 3, exactly two bits.
+    __device__ __forceinline__ void push(uint32_t code, uint32_t comp)
     {
-        f = ((f >> 2) | (code << 62)) & back_mask;
-        r = (r << 2) | ((code ^ 3) << rev_shift);
+        f0 = __builtin_amdgcn_alignbit(f1, f0, 2);
+        f1 = __builtin_amdgcn_alignbit(code, f1, 2);
+        r1 = __builtin_amdgcn_alignbit(r1, r0, 30);
+        r0 = (r0 << 2) | comp;
     }
-    __device__ __forceinline__ void canon(uint64_t *out) const { out[0] = f < r ? f : r; }
-    // The state push() leaves after the 32 history bytes h[0..8) (byte 0
-    // first), without rolling them: f holds the last k codes, newest at the
-    // top, which is the packed history masked; r the complements in
-    // reverse order.  `have` = the trailing run of valid bytes.
+    __device__ __forceinline__ void canon(uint64_t *out) const
+    {
+        const uint64_t f = (((uint64_t)f1 << 32) | f0) & back_mask;
+        const uint64_t r = (((uint64_t)r1 << 32) | r0) << rev_shift;
+        out[0] = f < r ? f : r;
+    }
+    // The state pushes of the 32 history bytes h[0..8) (byte 0 first) leave,
+    // without rolling them: the forward word is the packed history, the
+    // complement its reverse.  `have` = the trailing run of valid bytes.
     __device__ __forceinline__ void seed(const uint32_t *h, uint32_t &have)
     {
         uint64_t p[1], inv;
         pack_history<8>(h, p, inv);
-        f = p[0] & back_mask;
-        r = pair_reverse(~p[0]) << rev_shift;
+        f0 = (uint32_t)p[0];
+        f1 = (uint32_t)(p[0] >> 32);
+        const uint64_t y = pair_reverse(~p[0]);
+        r0 = (uint32_t)y;
+        r1 = (uint32_t)(y >> 32);
         have = (uint32_t)__clz((uint32_t)inv);
     }
 };
 
 template <> struct Roll<2> {
-    uint64_t f0 = 0, f1 = 0, r0 = 0, r1 = 0;
+    uint32_t f[4] = {0, 0, 0, 0};  // forward: f[3]:f[2] = key word 0 (newest codes), f[1]:f[0] = word 1
+    uint32_t r[4] = {0, 0, 0, 0};  // complement, bottom-aligned 128 bits
     uint64_t back_mask;
     uint32_t rev_shift;
     __device__ Roll(uint32_t k)
@@ -367,45 +402,56 @@ template <> struct Roll<2> {
         back_mask = (k % 32) ? (((1ULL << (2 * (k % 32))) - 1ULL) << (64 - 2 * (k % 32))) : ~0ULL;
         rev_shift = (k % 32) ? 2 * (32 - k % 32) : 0;
     }
-    __device__ __forceinline__ void push(uint64_t code)
+    __device__ __forceinline__ void push(uint32_t code, uint32_t comp)
     {
-        // data[0] most significant; the tail word (data[1]) is masked.
-        f1 = ((f1 >> 2) | (f0 << 62)) & back_mask;
-        f0 = (f0 >> 2) | (code << 62);
-        r0 = (r0 << 2) | (r1 >> 62);
-        r1 = (r1 << 2) | ((code ^ 3) << rev_shift);
+        f[0] = __builtin_amdgcn_alignbit(f[1], f[0], 2);
+        f[1] = __builtin_amdgcn_alignbit(f[2], f[1], 2);
+        f[2] = __builtin_amdgcn_alignbit(f[3], f[2], 2);
+        f[3] = __builtin_amdgcn_alignbit(code, f[3], 2);
+        r[3] = __builtin_amdgcn_alignbit(r[3], r[2], 30);
+        r[2] = __builtin_amdgcn_alignbit(r[2], r[1], 30);
+        r[1] = __builtin_amdgcn_alignbit(r[1], r[0], 30);
+        r[0] = (r[0] << 2) | comp;
     }
     __device__ __forceinline__ void canon(uint64_t *out) const
     {
+        // word 0 most significant; the tail word (word 1) is masked
+        const uint64_t f0 = ((uint64_t)f[3] << 32) | f[2];
+        const uint64_t f1 = (((uint64_t)f[1] << 32) | f[0]) & back_mask;
+        const uint64_t ylo = ((uint64_t)r[1] << 32) | r[0], yhi = ((uint64_t)r[3] << 32) | r[2];
+        const uint64_t r1 = ylo << rev_shift;
+        const uint64_t r0 = rev_shift ? (yhi << rev_shift) | (ylo >> (64 - rev_shift)) : yhi;
         const bool fwd = (f0 < r0) || (f0 == r0 && f1 < r1);
         out[0] = fwd ? f0 : r0;
         out[1] = fwd ? f1 : r1;
     }
     // As Roll<1>::seed over the 64 history bytes h[0..16): the newer 32
-    // codes are the head word, the older 32 (masked) the tail word, and the
-    // 128-bit reverse complement is shifted up by rev_shift.
+    // codes are the head word, the older 32 the tail word.
     __device__ __forceinline__ void seed(const uint32_t *h, uint32_t &have)
     {
         uint64_t p[2], inv;
         pack_history<16>(h, p, inv);
-        f0 = p[1];
-        f1 = p[0] & back_mask;
+        f[0] = (uint32_t)p[0];
+        f[1] = (uint32_t)(p[0] >> 32);
+        f[2] = (uint32_t)p[1];
+        f[3] = (uint32_t)(p[1] >> 32);
         const uint64_t ylo = pair_reverse(~p[1]), yhi = pair_reverse(~p[0]);
-        r1 = ylo << rev_shift;
-        r0 = rev_shift ? (yhi << rev_shift) | (ylo >> (64 - rev_shift)) : yhi;
+        r[0] = (uint32_t)ylo;
+        r[1] = (uint32_t)(ylo >> 32);
+        r[2] = (uint32_t)yhi;
+        r[3] = (uint32_t)(yhi >> 32);
         have = (uint32_t)__clzll((long long)inv);
     }
 };
 
-
-// Classifies one input byte (FastqParser.h:280-305 semantics, GetKmer.h
-// table): A/a=0 C/c=1 G/g=2 T/t/U/u=3, anything else invalid.
-__device__ __forceinline__ uint32_t base_code(uint32_t byte, bool &valid)
+// Pushes byte b of a Word4 and updates the trailing run of valid bytes
+// (any invalid byte resets the window, FastqParser.h:280-305 / GetKmer.h).
+template <int NW>
+__device__ __forceinline__ void push_byte(Roll<NW> &roll, const Word4 &x, int b, uint32_t &have)
 {
-    const uint32_t c = byte & 0xDF;
-    const uint32_t d = c - 0x41;
-    valid = d < 32 && ((kValidMask >> d) & 1);
-    return ((c >> 1) ^ (c >> 2)) & 3;
+    roll.push(x.codes >> (8 * b), (x.comps >> (8 * b)) & 3u);
+    const bool valid = !((x.iv >> (8 * b + 7)) & 1u);
+    have = valid ? have + 1 : 0;
 }
 
 // FIRST-list segment of an append by (block, wave, step): a mix of the three,
@@ -589,12 +635,10 @@ __device__ __forceinline__ void roll_segment(const ParseLds &L, uint64_t tile, u
     // bytes of this segment before emit_from emit nothing
     const uint32_t skip = emit_from > start ? (uint32_t)(emit_from - start < 64 ? emit_from - start : 64) : 0;
     for (int j = 0; j < 16; ++j) {
-        const uint32_t v = L.w[j][t + 1];
+        const Word4 x = word4(L.w[j][t + 1]);
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
-            bool valid;
-            roll.push(base_code(v >> (8 * b), valid));
-            have = valid ? have + 1 : 0;
+            push_byte<NW>(roll, x, b, have);
             if (have >= k && (uint32_t)(4 * j + b) >= skip) {
                 uint64_t key[NW];
                 roll.canon(key);

@@ -70,6 +70,7 @@ struct kc_part_scratch {
     uint64_t *o1 = nullptr, *o2 = nullptr;          // their exclusive offsets
     uint64_t *sums = nullptr, *starts = nullptr, *part_off = nullptr;
     uint8_t *dig = nullptr;                         // each key's fine digit (u8, or u16 above 256 digits)
+    uint64_t *gcur = nullptr;                       // front scatter: [coarse][group] cursors, then region ends
     uint64_t key_cap = 0;
     uint64_t m1_n = 0, m2_n = 0, sums_n = 0, part_n = 0, dig_bytes = 0;
 };

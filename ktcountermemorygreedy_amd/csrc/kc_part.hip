@@ -56,6 +56,15 @@ constexpr int kFrontChunk1 = 32;
 // Two-word keys: 16-byte chunks (4096 keys of 16 bytes = 64 KB of stage).
 constexpr int kFrontBlock2 = kParseBlock;
 constexpr int kFrontChunk2 = 16;
+// The front scatter's blocks form kFrontGroups groups (block b in group
+// b % kFrontGroups: the XCD the hardware dispatches it to); each group has
+// one region per coarse digit and its blocks append their chunk runs to it
+// through a cursor.  A block's open (partly written) lines then continue
+// where another block of its XCD stopped, so the L2 of the XCD completes
+// them; with a region per block, 64 blocks x 512 open runs filled the 4 MB
+// L2 of an XCD and partial lines reached memory.
+constexpr uint32_t kFrontGroups = 8;
+constexpr uint64_t kGroupWords = 2ull * 256 * kFrontGroups;  // cursors, then region ends
 
 struct Digits {
     uint32_t part_bits;  // L
@@ -84,14 +93,9 @@ __device__ __forceinline__ uint32_t fine_of(const uint64_t *key, const Digits &d
     return fine_digit(coarse_mix<NW>(key), d.l1, d.l2);
 }
 
-// The front scatters write each key's fine digit beside it, so that the
-// fine count reads 1-2 bytes per key instead of the key.  The digits of one
-// chunk land as one short run per coarse digit, and runs under 32 bytes
-// cost far more per byte than their size (the front scatter of 4096-key
-// chunks, 16-key runs, took 13.5 ms with u8 digits and 11.7 ms with u16 on
-// 1.9e9 63-mers; 8192-key chunks took 11.8 ms with u8 and 13.7 ms with u16 on
-// 2.4e9 31-mers), so the digits are u16 above 256 fine digits and for
-// two-word keys, u8 otherwise.  dig is null when there is no fine level.
+// The front scatters write each key's fine digit beside it (u8, or u16
+// above 256 fine digits), so that the fine count reads 1-2 bytes per key
+// instead of the key.  dig is null when there is no fine level.
 __device__ __forceinline__ void put_digit(uint8_t *dig, uint64_t at, uint32_t f, const Digits &d)
 {
     if (!d.dig16) dig[at] = (uint8_t)f;
@@ -173,7 +177,11 @@ __global__ __launch_bounds__(kParseBlock) void k_front_count_seq(SeqSrc src, uin
         __syncthreads();  // the next tile overwrites lds
     }
     __syncthreads();
-    if (threadIdx.x < (1u << d.l1)) m1[(uint64_t)threadIdx.x * gridDim.x + blockIdx.x] = hist[threadIdx.x];
+    // counts in [coarse][group][block of the group] order: the scan lays each
+    // group's blocks out as one region per coarse digit
+    const uint32_t J = gridDim.x / kFrontGroups;
+    const uint32_t col = (blockIdx.x % kFrontGroups) * J + blockIdx.x / kFrontGroups;
+    if (threadIdx.x < (1u << d.l1)) m1[(uint64_t)threadIdx.x * gridDim.x + col] = hist[threadIdx.x];
 }
 
 // Scatter guard.  A scatter writes each key at its digit region's offset
@@ -192,13 +200,14 @@ __device__ __forceinline__ void guard_report(unsigned long long *counters, uint3
 // segment (and k-1 <= 4*HW bytes of history) straight into registers and
 // rolls it in chunks of CB bytes; the block's keys of one chunk (up to
 // TB x CB) are placed in LDS grouped by coarse digit and written out as
-// contiguous runs (TB x CB / 256 keys per digit per chunk on average).  A
-// tile is TB x 64 bytes, the tiles of k_front_count_seq at TB = 256, so the
-// per-block offsets and counts of the count pass apply.
+// contiguous runs (TB x CB / 256 keys per digit per chunk on average), each
+// run at a place taken from its group's cursor for the digit (one atomic per
+// digit and chunk).  A tile is TB x 64 bytes, the tiles of
+// k_front_count_seq at TB = 256, so the group regions of the count pass hold
+// exactly the keys of their blocks.
 template <int NW, int TB, int CB, bool PASS>
 __global__ __launch_bounds__(TB) void k_front_scatter_seq_staged(SeqSrc src, uint32_t k, Digits d,
-                                                                 const uint64_t *__restrict__ o1,
-                                                                 const uint32_t *__restrict__ m1,
+                                                                 unsigned long long *__restrict__ gcur,
                                                                  uint64_t *__restrict__ out, uint8_t *__restrict__ dig,
                                                                  unsigned long long *counters)
 {
@@ -206,7 +215,7 @@ __global__ __launch_bounds__(TB) void k_front_scatter_seq_staged(SeqSrc src, uin
     constexpr int kChunks = kParseSeg / CB;
     constexpr int kWords = CB / 4;
     constexpr int kStage = TB * CB;
-    static_assert(TB == 256, "thread t keeps the region of coarse digit t");
+    static_assert(TB == 256, "thread t takes the cursor of coarse digit t");
     __shared__ __align__(16) uint64_t stage[kStage * NW];
     __shared__ uint8_t sbin[kStage];
     __shared__ uint32_t hist[256], loff[256];
@@ -217,9 +226,9 @@ __global__ __launch_bounds__(TB) void k_front_scatter_seq_staged(SeqSrc src, uin
     uint32_t n_err = 0;
     uint64_t n, emit_from;
     seq_bounds(src, n, emit_from);
-    // thread t's digit: where its region continues and how many keys it has left
-    uint64_t my_base = t < nbins ? o1[(uint64_t)t * gridDim.x + blockIdx.x] : 0;
-    uint32_t my_left = t < nbins ? m1[(uint64_t)t * gridDim.x + blockIdx.x] : 0;
+    // thread t's digit: its group's cursor and region end
+    const uint32_t gslot = t * kFrontGroups + blockIdx.x % kFrontGroups;
+    const uint64_t my_end = t < nbins ? gcur[256 * kFrontGroups + gslot] : 0;
     hist[t] = 0;
     __syncthreads();
     constexpr uint64_t kTileBytes = (uint64_t)TB * kParseSeg;
@@ -243,13 +252,11 @@ __global__ __launch_bounds__(TB) void k_front_scatter_seq_staged(SeqSrc src, uin
             uint32_t emitted = 0;  // bit e: byte e of the chunk emitted a key
 #pragma unroll
             for (int jj = 0; jj < kWords; ++jj) {
-                const uint32_t v = w[HW + q * kWords + jj];
+                const Word4 x = word4(w[HW + q * kWords + jj]);
 #pragma unroll
                 for (int b = 0; b < 4; ++b) {
                     const int e = jj * 4 + b;
-                    bool valid;
-                    roll.push(base_code(v >> (8 * b), valid));
-                    have = valid ? have + 1 : 0;
+                    push_byte<NW>(roll, x, b, have);
                     const bool emit = have >= k && (uint32_t)(4 * (q * kWords + jj) + b) >= skip;
                     if (emit) {
                         roll.canon(&key[e * NW]);
@@ -268,11 +275,13 @@ __global__ __launch_bounds__(TB) void k_front_scatter_seq_staged(SeqSrc src, uin
             const uint32_t h = hist[t];
             const uint32_t off = scan_block<TB>(h, wsum, total);
             loff[t] = off;
-            // staged key i of digit t goes to i + delta[t]; a chunk with more
-            // keys of the digit than its region has left is not written
-            const bool fits = h <= my_left;
+            // staged key i of digit t goes to i + delta[t]; a run that would
+            // pass the end of its group's region is not written
+            uint64_t pos = 0;
+            if (h) pos = atomicAdd(gcur + gslot, (unsigned long long)h);
+            const bool fits = pos + h <= my_end;
             n_err += !fits;
-            delta[t] = fits ? my_base - off : kNoDelta;
+            delta[t] = fits ? pos - off : kNoDelta;
             __syncthreads();
 #pragma unroll
             for (int e = 0; e < CB; ++e) {
@@ -296,13 +305,37 @@ __global__ __launch_bounds__(TB) void k_front_scatter_seq_staged(SeqSrc src, uin
                 if (dig) put_digit(dig, at, fine_of<NW>(stage + i * NW, d), d);
             }
             __syncthreads();
-            my_base += h;
-            my_left = fits ? my_left - h : 0;
             hist[t] = 0;
             __syncthreads();
         }
     }
-    n_err += my_left != 0;  // keys the count pass saw and this one did not
+    guard_report(counters, n_err);
+}
+
+// Group cursors of the front scatter: for coarse digit c and group x the
+// region [o1[c][x][0], o1[c][x + 1][0]) of the scan (the next group's start,
+// or the next digit's, or the total), cursors first, region ends after them.
+__global__ void k_group_cursors(const uint64_t *o1, const uint32_t *m1, uint32_t G, uint32_t n_coarse,
+                                unsigned long long *gcur)
+{
+    const uint32_t c = threadIdx.x;
+    if (c >= n_coarse) return;
+    const uint32_t J = G / kFrontGroups;
+    const uint64_t last = (uint64_t)n_coarse * G - 1;
+    for (uint32_t x = 0; x < kFrontGroups; ++x) {
+        const uint64_t at = (uint64_t)c * G + (uint64_t)x * J, next = at + J;
+        gcur[c * kFrontGroups + x] = o1[at];
+        gcur[256 * kFrontGroups + c * kFrontGroups + x] = next <= last ? o1[next] : o1[last] + m1[last];
+    }
+}
+
+// Scatter guard of the grouped front scatter: a region whose cursor did not
+// reach its end holds keys the count pass saw and the scatter did not.
+__global__ void k_group_check(const unsigned long long *gcur, uint32_t n_coarse, unsigned long long *counters)
+{
+    uint32_t n_err = 0;
+    for (uint32_t i = threadIdx.x; i < n_coarse * kFrontGroups; i += blockDim.x)
+        n_err += gcur[i] != gcur[256 * kFrontGroups + i];
     guard_report(counters, n_err);
 }
 
@@ -595,7 +628,10 @@ struct PartLds {
 // The Bloom singleton split on the LDS copy (block and mask from
 // bloom_in_part): FIRST exactly when the fetch_or of the key's mask into its
 // block finds a bit missing (bloom_first in kc_dev_common.h: the reference's
-// BF1 word does not change that decision and is not kept).
+// BF1 word does not change that decision and is not kept).  (A plain read
+// first, with the atomic only for keys that miss a bit, measured slower: a
+// wave of 64 keys nearly always holds one that needs the atomic, so the
+// wave issues both.)
 template <int NW, bool BLOOM>
 __device__ __forceinline__ bool lds_bloom_first(PartLds<NW, BLOOM> &S, uint64_t h, uint32_t bmask)
 {
@@ -614,7 +650,8 @@ __device__ __forceinline__ bool lds_insert(PartLds<NW, BLOOM> &S, const uint64_t
     if constexpr (NW == 1) {
         // One exit condition for the whole probe loop (claimed, found, or
         // every slot tried) keeps its exec-mask bookkeeping short; the count
-        // is added once the loop has settled.
+        // is added once the loop has settled.  (A plain read of the first
+        // probe slot before the CAS measured slower, as for the Bloom word.)
         uint64_t w0;
         uint32_t left = pmask + 1;
         bool go;
@@ -891,7 +928,7 @@ Digits digits_of(const kc_table *t)
     d.l2 = d.part_bits - d.l1;
     d.pass_bits = t->pass_bits;
     d.pass = t->pass;
-    d.dig16 = d.l2 > 8 || t->nw == 2;
+    d.dig16 = d.l2 > 8;
     return d;
 }
 
@@ -915,7 +952,7 @@ bool kc_part_eligible(const kc_table *t)
 void kc_part_scratch_free(kc_part_scratch &s)
 {
     for (void *p : {(void *)s.keys_a, (void *)s.keys_b, (void *)s.m1, (void *)s.o1, (void *)s.m2, (void *)s.o2,
-                    (void *)s.sums, (void *)s.part_off, (void *)s.starts, (void *)s.dig})
+                    (void *)s.sums, (void *)s.part_off, (void *)s.starts, (void *)s.dig, (void *)s.gcur})
         if (p) (void)hipFree(p);
     s = kc_part_scratch();
 }
@@ -956,7 +993,7 @@ int kc_part_reserve(const kc_table *t, kc_part_scratch &s, const kc_pipe_shape &
         return true;
     };
     uint64_t kc_a = s.key_cap, kc_b = s.key_cap, m1c = s.m1_n, o1c = s.m1_n, m2c = s.m2_n, o2c = s.m2_n;
-    uint64_t sc = s.sums_n, pc = s.part_n, stc = s.starts ? 257 : 0, dgc = s.dig_bytes;
+    uint64_t sc = s.sums_n, pc = s.part_n, stc = s.starts ? 257 : 0, dgc = s.dig_bytes, gcc = s.gcur ? kGroupWords : 0;
     const uint64_t keys_words = std::max<uint64_t>(sh.key_cap, 1) * nw;
     // the digit array (+16: the fine count's last 16-byte load may reach past the final digit)
     const uint64_t dig_need = d.l2 ? std::max<uint64_t>(sh.key_cap, 1) * (d.dig16 ? 2 : 1) + 16 : 0;
@@ -965,7 +1002,7 @@ int kc_part_reserve(const kc_table *t, kc_part_scratch &s, const kc_pipe_shape &
     const bool ok = grow(s.keys_a, kc_a, keys_words) && grow(s.keys_b, kc_b, keys_words) &&
                     grow(s.m1, m1c, m1n) && grow(s.o1, o1c, m1n) && (d.l2 == 0 || grow(s.m2, m2c, m2n)) &&
                     (d.l2 == 0 || grow(s.o2, o2c, m2n)) && grow(s.sums, sc, sumsn) &&
-                    grow(s.part_off, pc, n_parts + 1) && grow(s.starts, stc, 257) &&
+                    grow(s.part_off, pc, n_parts + 1) && grow(s.starts, stc, 257) && grow(s.gcur, gcc, kGroupWords) &&
                     (d.l2 == 0 || grow(s.dig, dgc, dig_need));
     if (!ok) {
         kc_part_scratch_free(s);
@@ -989,12 +1026,13 @@ int kc_part_enqueue(const kc_table *t, kc_part_scratch &s, const kc_pipe_shape &
     const uint64_t n_parts = 1ull << d.part_bits;
     const uint32_t G = sh.front_blocks, S = sh.fine_slices;
     const bool from_seq = d_keys == nullptr;
+    if (G % kFrontGroups) return KC_EINVAL;
     // the shape the scratch was sized for must cover this launch
     const uint64_t m1n = (uint64_t)n_coarse * G, m2n = d.l2 ? n_parts * S : 0;
     const uint64_t need_keys = from_seq ? seq.n - std::min(seq.n, seq.hdr ? 0 : seq.emit_from) : n_keys;
     const uint64_t dig_need = d.l2 ? s.key_cap * (d.dig16 ? 2 : 1) + 16 : 0;
     if (s.key_cap < need_keys || s.key_cap < sh.key_cap || s.m1_n < m1n || s.m2_n < m2n ||
-        s.sums_n < kc_scan_tmp_words(std::max(m1n, m2n)) || s.part_n < n_parts + 1 || !s.starts ||
+        s.sums_n < kc_scan_tmp_words(std::max(m1n, m2n)) || s.part_n < n_parts + 1 || !s.starts || !s.gcur ||
         (d.l2 && (!s.dig || s.dig_bytes < dig_need)))
         return KC_EINVAL;
     uint8_t *dig = d.l2 ? s.dig : nullptr;
@@ -1017,13 +1055,16 @@ int kc_part_enqueue(const kc_table *t, kc_part_scratch &s, const kc_pipe_shape &
     if ((rc = kc_scan_u32_on(st, s.m1, m1n, s.o1, s.sums))) return rc;
     hipLaunchKernelGGL(k_coarse_starts, dim3(1), dim3(256), 0, st, s.o1, s.m1, G, n_coarse, s.starts);
     if (from_seq) {
+        auto *gcur = reinterpret_cast<unsigned long long *>(s.gcur);
+        hipLaunchKernelGGL(k_group_cursors, dim3(1), dim3(256), 0, st, s.o1, s.m1, G, n_coarse, gcur);
         const bool pass = d.pass_bits != 0;
         auto f1 = pass ? &k_front_scatter_seq_staged<1, kFrontBlock1, kFrontChunk1, true>
                        : &k_front_scatter_seq_staged<1, kFrontBlock1, kFrontChunk1, false>;
         auto f2 = pass ? &k_front_scatter_seq_staged<2, kFrontBlock2, kFrontChunk2, true>
                        : &k_front_scatter_seq_staged<2, kFrontBlock2, kFrontChunk2, false>;
-        if (nw == 1) hipLaunchKernelGGL(f1, dim3(G), dim3(kFrontBlock1), 0, st, seq, t->cfg.k, d, s.o1, s.m1, s.keys_a, dig, tv.counters);
-        else hipLaunchKernelGGL(f2, dim3(G), dim3(kFrontBlock2), 0, st, seq, t->cfg.k, d, s.o1, s.m1, s.keys_a, dig, tv.counters);
+        if (nw == 1) hipLaunchKernelGGL(f1, dim3(G), dim3(kFrontBlock1), 0, st, seq, t->cfg.k, d, gcur, s.keys_a, dig, tv.counters);
+        else hipLaunchKernelGGL(f2, dim3(G), dim3(kFrontBlock2), 0, st, seq, t->cfg.k, d, gcur, s.keys_a, dig, tv.counters);
+        hipLaunchKernelGGL(k_group_check, dim3(1), dim3(256), 0, st, gcur, n_coarse, tv.counters);
     } else {
         if (nw == 1) hipLaunchKernelGGL(k_front_scatter_keys<1>, dim3(G), dim3(kKeyBlock), 0, st, d_keys, n_keys, d, s.o1, s.m1, s.keys_a, dig, tv.counters);
         else hipLaunchKernelGGL(k_front_scatter_keys<2>, dim3(G), dim3(kKeyBlock), 0, st, d_keys, n_keys, d, s.o1, s.m1, s.keys_a, dig, tv.counters);
