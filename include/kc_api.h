@@ -12,8 +12,9 @@
  * context's HIP stream; a kc_ctx must be used by one host thread at a time.
  *
  * Key layout (identical to kmer<N>, kmer.h:11-119 and GetKmer.h:71-77):
- * nw = 1 (k<=32) or 2 (k<=63) little-endian u64 words, word 0 most
- * significant, string position j at bit 64*nw-2k+2j; low padding bits zero.
+ * nw = 1 (k<=32), 2 (k<=64) or 4 (k<=128) little-endian u64 words, word 0
+ * most significant, the k-mer in words 0..(k-1)/32 with string position j
+ * at bit 64*m-2k+2j of those m words; padding bits and words zero.
  */
 #ifndef KC_API_H
 #define KC_API_H
@@ -30,7 +31,7 @@ extern "C" {
 #define KC_EFULL (-3)    /* table overflowed (a full FIRST-list segment spills into the table) */
 #define KC_EDEVICE (-4)  /* HIP runtime error */
 #define KC_EIO (-5)      /* file error */
-#define KC_ENOTSUP (-6)  /* configuration the library does not support (k > 63) */
+#define KC_ENOTSUP (-6)  /* configuration the library does not support */
 
 typedef struct kc_ctx kc_ctx;
 typedef struct kc_table kc_table;
@@ -45,7 +46,7 @@ typedef struct kc_table kc_table;
 #define KC_PATH_PARTITIONED 2
 
 typedef struct kc_table_config {
-    uint32_t k;            /* 1..63 */
+    uint32_t k;            /* 1..128 */
     uint32_t count_max;    /* saturation value (main.cpp:599-615), >= 1 */
     uint32_t filter_min;   /* high.bin filter (FinalDrainWriter.h:101-104) */
     uint32_t filter_max;
@@ -245,7 +246,7 @@ int kc_export_high(kc_ctx *ctx, const uint64_t *d_keys, const uint32_t *d_counts
  * files (histogram_tool.cpp:734-816, dump_tool.cpp:1202-1270); without
  * low.bin, their approximate modes (high.bin alone: histogram_tool.cpp:
  * 840-880, dump_tool.cpp:1294-1400).  Empty files are an empty export (the
- * reference tools refuse an empty high.bin).  KC_ENOTSUP for k > 63. */
+ * reference tools refuse an empty high.bin). */
 int kc_table_load_dir(kc_ctx *ctx, const char *dir, int with_low, kc_table **out);
 /* histogram_tool <precise|approximate> <dir> ... <min> <max>: bins[f-min]
  * for f = min..max, host int64. */
@@ -350,7 +351,7 @@ int kc_batch_get_info(kc_batch *b, kc_batch_info *out);
 /* ---- sorts: ExportRecordRadixSort (ExportRecordRadixSort.h:256-269) and
  *      radix_sort_export_records_12bit (ExportRecordRadixSort12.h:639-646).
  * Records are {nw u64 key words, u32 count} at record_bytes stride
- * (12 packed, or 16/24 as ExportRecord<N> in memory).  digit_bits 8 gives
+ * (8*nw+4 packed, or 16/24/40 as ExportRecord<N> in memory).  digit_bits 8 gives
  * the 8-bit sort's key range, 12 the 12-bit sort's (common_prefix_bases
  * used only there).  The result lands in d_recs or d_scratch;
  * *result_in_scratch says which.  Stable, like the reference. ---- */

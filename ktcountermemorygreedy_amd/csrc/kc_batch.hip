@@ -15,7 +15,7 @@
 // copy and the counting of batch i overlap.
 //
 // Chunks that do not end in '\n' get one, so no window joins two chunks.  A
-// chunk larger than a slot is sent in pieces of its own with a 64-byte halo
+// chunk larger than a slot is sent in pieces of its own with a 128-byte halo
 // and the halo excluded from emission (the reference's k-1 overlap rule).
 #include <hip/hip_runtime.h>
 
@@ -30,7 +30,7 @@
 
 namespace {
 
-constexpr uint64_t kHalo = 64;  // >= k-1 for k <= 63, multiple of 16
+constexpr uint64_t kHalo = 128;  // >= k-1 for k <= 128, multiple of 16
 constexpr uint32_t kMaxSlots = 8;
 constexpr size_t kMaxDevGraphs = 8;  // graphs cached by kc_batch_launch_device
 constexpr uint64_t kPartMinBatch = 32ull << 20;
@@ -528,7 +528,7 @@ extern "C" int kc_autotune(kc_table *t, const uint8_t *d_sample, uint64_t nbytes
     if (kc_part_eligible(t) && forced != KC_PATH_DIRECT)
         for (uint32_t bpc : {2u, 4u, 8u, 16u})
             for (uint32_t sl : {2u, 4u, 8u, 16u}) cands[n_cands++] = {(uint32_t)KC_PATH_PARTITIONED, cu * bpc, sl};
-    if (forced != KC_PATH_PARTITIONED) cands[n_cands++] = {(uint32_t)KC_PATH_DIRECT, 0, 0};
+    if (forced != KC_PATH_PARTITIONED || !kc_part_eligible(t)) cands[n_cands++] = {(uint32_t)KC_PATH_DIRECT, 0, 0};
     float best_t = 0;
     int rc = KC_OK;
     kc_tune win = saved;

@@ -62,9 +62,9 @@ __global__ __launch_bounds__(kBlock) void k_count_seq(SeqSrc src, TableView tv)
     const uint64_t ntiles = (n + kTile - 1) / kTile;
     uint32_t step = 0;
     for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x, ++step) {
-        stage_tile(src.seq, n, tile, lds);
+        stage_tile<halo_segs<NW>()>(src.seq, n, tile, lds);
         roll_segment<NW>(lds, tile, n, emit_from, tv.k, [&](const uint64_t *key, uint32_t e) {
-            if (!in_pass<NW>(key, tv.pass_bits, tv.pass)) return;
+            if (!in_pass<NW>(key, tv.pass_bits, tv.pass, tv.rw)) return;
             ++n_kmers;
             const KeyHash kh = key_hash<NW>(tv, key);
             bool to_table = true;
@@ -92,9 +92,8 @@ __global__ __launch_bounds__(kBlock) void k_increment(const uint64_t *__restrict
     uint32_t n_new = 0, n_err = 0;
     for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) {
         uint64_t key[NW];
-#pragma unroll
-        for (int w = 0; w < NW; ++w) key[w] = keys[i * NW + w];
-        if (!in_pass<NW>(key, tv.pass_bits, tv.pass)) continue;
+        caller_key<NW>(keys, i, tv.rw, key);
+        if (!in_pass<NW>(key, tv.pass_bits, tv.pass, tv.rw)) continue;
         table_add<NW>(tv, key, key_hash<NW>(tv, key), inc ? inc[i] : 1, n_new, n_err);
     }
     stat_add(tv.counters, KC_STAT_DISTINCT, n_new);
@@ -112,7 +111,8 @@ template <int NW>
 __device__ __forceinline__ uint64_t idx_hash(const uint64_t *key)
 {
     uint64_t x = key[0];
-    if constexpr (NW == 2) x ^= ((key[1] >> 29) | (key[1] << 35)) * 0x94D049BB133111EBULL;
+#pragma unroll
+    for (int w = 1; w < NW; ++w) x ^= ((key[w] >> (29 - 4 * w)) | (key[w] << (35 + 4 * w))) * 0x94D049BB133111EBULL;
     x ^= x >> 31;
     x *= 0xBF58476D1CE4E5B9ULL;
     x ^= x >> 29;
@@ -120,8 +120,13 @@ __device__ __forceinline__ uint64_t idx_hash(const uint64_t *key)
     return x ^ (x >> 32);
 }
 
+// Index slots: the key (one or two words, claimed with a CAS on word 0 from
+// KC_EMPTY), or four key words and a fifth word claimed from 0 to 1 (a
+// four-word key can have an all-ones word 0).
+template <int NW> constexpr int idx_words() { return NW == 4 ? 5 : NW; }
+
 // Inserts every FIRST-list key (block (chunk, segment) walks one segment).
-// idx[(imask + 1) * NW] counts keys that found no slot (never at load 1/2).
+// idx[(imask + 1) * idx_words] counts keys that found no slot (never at load 1/2).
 template <int NW>
 __global__ __launch_bounds__(kBlock) void k_low_index_build(TableView tv, uint64_t *__restrict__ idx, uint64_t imask)
 {
@@ -136,10 +141,16 @@ __global__ __launch_bounds__(kBlock) void k_low_index_build(TableView tv, uint64
         uint64_t s = idx_hash<NW>(key) & imask;
         bool done = false;
         for (uint64_t probe = 0; probe <= imask && !done; ++probe, s = (s + 1) & imask) {
-            uint64_t *slot = idx + s * NW;
+            uint64_t *slot = idx + s * idx_words<NW>();
             // the list's keys are unique: an occupied slot holds another key
-            if (atomicCAS((unsigned long long *)slot, (unsigned long long)KC_EMPTY, (unsigned long long)key[0]) ==
-                KC_EMPTY) {
+            if constexpr (NW == 4) {
+                if (atomicCAS((unsigned long long *)(slot + 4), 0ull, 1ull) == 0) {
+#pragma unroll
+                    for (int w = 0; w < 4; ++w) slot[w] = key[w];
+                    done = true;
+                }
+            } else if (atomicCAS((unsigned long long *)slot, (unsigned long long)KC_EMPTY, (unsigned long long)key[0]) ==
+                       KC_EMPTY) {
                 if constexpr (NW == 2) slot[1] = key[1];
                 done = true;
             }
@@ -149,7 +160,7 @@ __global__ __launch_bounds__(kBlock) void k_low_index_build(TableView tv, uint64
     uint64_t e = n_err;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) e += __shfl_xor(e, o, 64);
-    if (__lane_id() == 0 && e) atomicAdd((unsigned long long *)(idx + (imask + 1) * NW), (unsigned long long)e);
+    if (__lane_id() == 0 && e) atomicAdd((unsigned long long *)(idx + (imask + 1) * idx_words<NW>()), (unsigned long long)e);
 }
 
 template <int NW>
@@ -157,10 +168,15 @@ __device__ __forceinline__ bool idx_contains(const uint64_t *__restrict__ idx, u
 {
     uint64_t s = idx_hash<NW>(key) & imask;
     for (uint64_t probe = 0; probe <= imask; ++probe, s = (s + 1) & imask) {
-        const uint64_t *slot = idx + s * NW;
-        const uint64_t w0 = slot[0];
-        if (w0 == KC_EMPTY) return false;
-        if (w0 == key[0] && (NW == 1 || slot[NW - 1] == key[NW - 1])) return true;
+        const uint64_t *slot = idx + s * idx_words<NW>();
+        if constexpr (NW == 4) {
+            if (slot[4] == 0) return false;
+            if (slot[0] == key[0] && slot[1] == key[1] && slot[2] == key[2] && slot[3] == key[3]) return true;
+        } else {
+            const uint64_t w0 = slot[0];
+            if (w0 == KC_EMPTY) return false;
+            if (w0 == key[0] && (NW == 1 || slot[NW - 1] == key[NW - 1])) return true;
+        }
     }
     return false;
 }
@@ -174,8 +190,7 @@ __global__ __launch_bounds__(kBlock) void k_query(const uint64_t *__restrict__ k
 {
     for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) {
         uint64_t key[NW];
-#pragma unroll
-        for (int w = 0; w < NW; ++w) key[w] = keys[i * NW + w];
+        caller_key<NW>(keys, i, tv.rw, key);
         uint64_t c = table_find<NW>(tv, key);
         if constexpr (IDX) c += idx_contains<NW>(idx, imask, key);
         out[i] = (uint32_t)(c < tv.count_max ? c : tv.count_max);
@@ -187,11 +202,16 @@ __global__ void k_init_slots(uint64_t *slots, uint64_t cap)
 {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < cap; i += (uint64_t)gridDim.x * blockDim.x) {
         uint64_t *s = slots + i * slot_words<NW>();
-        s[0] = KC_EMPTY;
-        s[1] = NW == 1 ? 0 : KC_EMPTY;
-        if constexpr (NW == 2) {
-            s[2] = 0;
-            s[3] = 0;
+        if constexpr (NW == 4) {
+#pragma unroll
+            for (int w = 0; w < 8; ++w) s[w] = 0;  // state word 0: empty
+        } else {
+            s[0] = KC_EMPTY;
+            s[1] = NW == 1 ? 0 : KC_EMPTY;
+            if constexpr (NW == 2) {
+                s[2] = 0;
+                s[3] = 0;
+            }
         }
     }
 }
@@ -210,10 +230,10 @@ __device__ __forceinline__ int classify_slot(const TableView &tv, uint64_t i, bo
                                              uint64_t *key, uint32_t &cc)
 {
     const uint64_t *s = tv.slots + i * slot_words<NW>();
-    if (s[0] == KC_EMPTY) return 0;
+    if (!slot_used<NW>(s)) return 0;
 #pragma unroll
     for (int w = 0; w < NW; ++w) key[w] = s[w];
-    const uint64_t c = s[NW == 1 ? 1 : 2];
+    const uint64_t c = s[count_word<NW>()];
     if (singles_to_low && c == 1) return 2;
     if (c == 0) return 0;
     cc = (uint32_t)(c < tv.count_max ? c : tv.count_max);
@@ -319,7 +339,7 @@ __global__ __launch_bounds__(kBlock) void k_export_write(TableView tv, PackSpec 
                                                          uint64_t n_tiles, const uint64_t *tile_off,
                                                          uint8_t *out_high, uint8_t *out_low, uint64_t n_list)
 {
-    __shared__ uint8_t stage[kExportStage];
+    __shared__ uint8_t stage[export_stage<NW>()];
     const uint64_t cap = tv.mask + 1;
     const uint32_t hrec = ps.kmer_bytes + ps.count_bytes;
     for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
@@ -372,7 +392,7 @@ __global__ __launch_bounds__(kBlock) void k_export_write(TableView tv, PackSpec 
 template <int NW>
 __global__ __launch_bounds__(kBlock) void k_export_low_list(TableView tv, PackSpec ps, uint8_t *out_low)
 {
-    __shared__ uint8_t stage[kExportStage];
+    __shared__ uint8_t stage[export_stage<NW>()];
     const uint32_t seg = blockIdx.y;
     const uint64_t s0 = tv.seg_off[seg], cnt = tv.seg_off[seg + 1] - s0;
     for (uint64_t r0 = blockIdx.x * kExportTile; r0 < cnt; r0 += (uint64_t)gridDim.x * kExportTile) {
@@ -437,8 +457,8 @@ __global__ __launch_bounds__(kBlock) void k_hist_table(TableView tv, HistSpec h,
     const uint64_t cap = tv.mask + 1;
     for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < cap; i += (uint64_t)gridDim.x * kBlock) {
         const uint64_t *s = tv.slots + i * slot_words<NW>();
-        if (s[0] == KC_EMPTY) continue;
-        const uint64_t c = s[NW == 1 ? 1 : 2];
+        if (!slot_used<NW>(s)) continue;
+        const uint64_t c = s[count_word<NW>()];
         if (h.singles_low && c == 1) {
             // low record whose lookup misses the high set
             hist_add(lds_bins, gbins, h, 1, 1);
@@ -561,6 +581,7 @@ TableView view_of(kc_table *t)
     v.filter_max = t->cfg.filter_max;
     v.pass_bits = t->pass_bits;
     v.pass = t->pass;
+    v.rw = t->cfg.k <= 32 ? 1 : t->cfg.k <= 64 ? 2 : 4;
     return v;
 }
 
@@ -762,8 +783,10 @@ static int init_slots_bloom(kc_table *t)
     kc_ctx *c = t->ctx;
     if (t->nw == 1)
         hipLaunchKernelGGL(k_init_slots<1>, dim3(grid_for(t->cap, c->cu_count)), dim3(kBlock), 0, c->stream, t->slots, t->cap);
-    else
+    else if (t->nw == 2)
         hipLaunchKernelGGL(k_init_slots<2>, dim3(grid_for(t->cap, c->cu_count)), dim3(kBlock), 0, c->stream, t->slots, t->cap);
+    else
+        hipLaunchKernelGGL(k_init_slots<4>, dim3(grid_for(t->cap, c->cu_count)), dim3(kBlock), 0, c->stream, t->slots, t->cap);
     KC_HIP(hipGetLastError());
     if (t->bloom) return kc_launch_fill(c, t->bloom, t->bloom_blocks, 0);
     return KC_OK;
@@ -782,8 +805,10 @@ static int table_init_device(kc_table *t)
     kc_ctx *c = t->ctx;
     if (t->nw == 1)
         hipLaunchKernelGGL(k_init_slots<1>, dim3(grid_for(t->cap, c->cu_count)), dim3(kBlock), 0, c->stream, t->slots, t->cap);
-    else
+    else if (t->nw == 2)
         hipLaunchKernelGGL(k_init_slots<2>, dim3(grid_for(t->cap, c->cu_count)), dim3(kBlock), 0, c->stream, t->slots, t->cap);
+    else
+        hipLaunchKernelGGL(k_init_slots<4>, dim3(grid_for(t->cap, c->cu_count)), dim3(kBlock), 0, c->stream, t->slots, t->cap);
     KC_HIP(hipGetLastError());
     int rc;
     if (t->bloom && (rc = kc_launch_fill(c, t->bloom, t->bloom_blocks, 0)) != KC_OK) return rc;
@@ -800,8 +825,7 @@ extern "C" int kc_table_create(kc_ctx *ctx, const kc_table_config *cfg, kc_table
 {
     if (!ctx || !cfg || !out) return KC_EINVAL;
     *out = nullptr;
-    if (cfg->k > 63 && cfg->k <= 128) return KC_ENOTSUP;
-    if (cfg->k < 1 || cfg->k > 63 || cfg->count_max == 0 || cfg->filter_max < cfg->filter_min) return KC_EINVAL;
+    if (cfg->k < 1 || cfg->k > 128 || cfg->count_max == 0 || cfg->filter_max < cfg->filter_min) return KC_EINVAL;
     if (cfg->count_path > KC_PATH_PARTITIONED) return KC_EINVAL;
     if (cfg->capacity && !is_pow2(cfg->capacity)) return KC_EINVAL;
     if (cfg->bloom_blocks && !is_pow2(cfg->bloom_blocks)) return KC_EINVAL;
@@ -823,7 +847,8 @@ extern "C" int kc_table_create(kc_ctx *ctx, const kc_table_config *cfg, kc_table
     if (const char *f = std::getenv("KC_TEST_SKEW_COUNTS")) t->fault_skew = (uint32_t)std::atoi(f);
     t->ctx = ctx;
     t->cfg = *cfg;
-    t->nw = cfg->k <= 32 ? 1 : 2;
+    // key words inside the table: k = 64 keys (kmer<2>) take the four-word slots
+    t->nw = cfg->k <= 32 ? 1 : cfg->k <= 63 ? 2 : 4;
     t->cap = cfg->capacity ? cfg->capacity : std::max<uint64_t>(1024, next_pow2(cfg->max_distinct + cfg->max_distinct / 2));
     const uint64_t hint = cfg->max_distinct ? cfg->max_distinct : t->cap / 2;
     // partitions of the size the LDS counting pass holds; a Bloom filter
@@ -853,7 +878,7 @@ extern "C" int kc_table_create(kc_ctx *ctx, const kc_table_config *cfg, kc_table
         t->low_cap = t->seg_cap * KC_LOW_SEGS;
     }
     t->n_tiles = (t->cap + kExportTile - 1) / kExportTile;
-    const uint64_t slot_bytes = t->nw == 1 ? 16 : 32;
+    const uint64_t slot_bytes = 8ull * (t->nw == 1 ? 2 : t->nw == 2 ? 4 : 8);
     const uint64_t need = t->cap * slot_bytes + 8 * t->bloom_blocks + 8ull * t->nw * t->low_cap +
                           8 * KC_COUNTER_WORDS + 16 * (t->n_tiles + 1);
     size_t free_b = 0, total_b = 0;
@@ -961,9 +986,12 @@ int kc_direct_enqueue(const kc_table *t, const SeqSrc &src, uint32_t grid, hipSt
     if (t->nw == 1) {
         if (bloom) hipLaunchKernelGGL((k_count_seq<1, true>), g, dim3(kBlock), 0, st, src, v);
         else hipLaunchKernelGGL((k_count_seq<1, false>), g, dim3(kBlock), 0, st, src, v);
-    } else {
+    } else if (t->nw == 2) {
         if (bloom) hipLaunchKernelGGL((k_count_seq<2, true>), g, dim3(kBlock), 0, st, src, v);
         else hipLaunchKernelGGL((k_count_seq<2, false>), g, dim3(kBlock), 0, st, src, v);
+    } else {
+        if (bloom) hipLaunchKernelGGL((k_count_seq<4, true>), g, dim3(kBlock), 0, st, src, v);
+        else hipLaunchKernelGGL((k_count_seq<4, false>), g, dim3(kBlock), 0, st, src, v);
     }
     KC_HIP(hipGetLastError());
     return KC_OK;
@@ -994,7 +1022,7 @@ extern "C" int kc_table_count_seq_host(kc_table *t, const uint8_t *h_seq, uint64
     if (!t || (!h_seq && nbytes)) return KC_EINVAL;
     if (!nbytes) return KC_OK;
     constexpr uint64_t kStage = 256ull << 20;
-    constexpr uint64_t kHalo = 64;  // >= k-1, multiple of 16
+    constexpr uint64_t kHalo = 128;  // >= k-1 for k <= 128, multiple of 16
     hipStream_t s = t->ctx->stream;
     KC_HIP(hipSetDevice(t->ctx->device));
     // Page-locked sources are copied to the device directly; pageable ones
@@ -1063,7 +1091,8 @@ extern "C" int kc_table_increment(kc_table *t, const uint64_t *d_keys, uint64_t 
     if (rc != KC_OK) return rc;
     const uint32_t g = grid_for(n, t->ctx->cu_count);
     if (t->nw == 1) hipLaunchKernelGGL(k_increment<1>, dim3(g), dim3(kBlock), 0, t->ctx->stream, d_keys, n, d_inc, v);
-    else hipLaunchKernelGGL(k_increment<2>, dim3(g), dim3(kBlock), 0, t->ctx->stream, d_keys, n, d_inc, v);
+    else if (t->nw == 2) hipLaunchKernelGGL(k_increment<2>, dim3(g), dim3(kBlock), 0, t->ctx->stream, d_keys, n, d_inc, v);
+    else hipLaunchKernelGGL(k_increment<4>, dim3(g), dim3(kBlock), 0, t->ctx->stream, d_keys, n, d_inc, v);
     KC_HIP(hipGetLastError());
     return KC_OK;
 }
@@ -1088,7 +1117,8 @@ static int ensure_low_index(kc_table *t, uint64_t *mask)
     t->low_idx_mask = 0;
     if (hc.low) {
         const uint64_t cap = std::max<uint64_t>(1024, next_pow2(2 * hc.low));
-        const uint64_t words = cap * t->nw + 1;  // + the error word
+        const uint64_t iw = t->nw == 4 ? 5 : t->nw;  // index words per key (idx_words)
+        const uint64_t words = cap * iw + 1;  // + the error word
         if (cap > t->low_idx_cap) {
             KC_HIP(hipStreamSynchronize(t->ctx->stream));
             if (t->low_idx) KC_HIP(hipFree(t->low_idx));
@@ -1100,17 +1130,18 @@ static int ensure_low_index(kc_table *t, uint64_t *mask)
             }
             t->low_idx_cap = cap;
         }
-        if ((rc = kc_launch_fill(t->ctx, t->low_idx, cap * t->nw, KC_EMPTY)) != KC_OK) return rc;
-        if ((rc = kc_launch_fill(t->ctx, t->low_idx + cap * t->nw, 1, 0)) != KC_OK) return rc;
+        if ((rc = kc_launch_fill(t->ctx, t->low_idx, cap * iw, t->nw == 4 ? 0 : KC_EMPTY)) != KC_OK) return rc;
+        if ((rc = kc_launch_fill(t->ctx, t->low_idx + cap * iw, 1, 0)) != KC_OK) return rc;
         uint64_t most = 0;
         for (int sgi = 0; sgi < KC_LOW_SEGS; ++sgi) most = std::max(most, hc.seg_fill[sgi]);
         const TableView v = view_of(t);
         const dim3 g(kc_low_chunks(most, 4 * kBlock), KC_LOW_SEGS);
         if (t->nw == 1) hipLaunchKernelGGL(k_low_index_build<1>, g, dim3(kBlock), 0, t->ctx->stream, v, t->low_idx, cap - 1);
-        else hipLaunchKernelGGL(k_low_index_build<2>, g, dim3(kBlock), 0, t->ctx->stream, v, t->low_idx, cap - 1);
+        else if (t->nw == 2) hipLaunchKernelGGL(k_low_index_build<2>, g, dim3(kBlock), 0, t->ctx->stream, v, t->low_idx, cap - 1);
+        else hipLaunchKernelGGL(k_low_index_build<4>, g, dim3(kBlock), 0, t->ctx->stream, v, t->low_idx, cap - 1);
         KC_HIP(hipGetLastError());
         uint64_t err = 0;
-        KC_HIP(hipMemcpyAsync(&err, t->low_idx + cap * t->nw, 8, hipMemcpyDeviceToHost, t->ctx->stream));
+        KC_HIP(hipMemcpyAsync(&err, t->low_idx + cap * iw, 8, hipMemcpyDeviceToHost, t->ctx->stream));
         KC_HIP(hipStreamSynchronize(t->ctx->stream));
         if (err) return KC_EDEVICE;  // a set at load 1/2 always has room: a broken list
         t->low_idx_mask = cap - 1;
@@ -1146,9 +1177,12 @@ extern "C" int kc_table_query_on(kc_table *t, kc_ctx *ctx, const uint64_t *d_key
     if (t->nw == 1) {
         if (imask) hipLaunchKernelGGL((k_query<1, true>), dim3(g), dim3(kBlock), 0, st, d_keys, n, d_counts, v, idx, imask);
         else hipLaunchKernelGGL((k_query<1, false>), dim3(g), dim3(kBlock), 0, st, d_keys, n, d_counts, v, idx, imask);
-    } else {
+    } else if (t->nw == 2) {
         if (imask) hipLaunchKernelGGL((k_query<2, true>), dim3(g), dim3(kBlock), 0, st, d_keys, n, d_counts, v, idx, imask);
         else hipLaunchKernelGGL((k_query<2, false>), dim3(g), dim3(kBlock), 0, st, d_keys, n, d_counts, v, idx, imask);
+    } else {
+        if (imask) hipLaunchKernelGGL((k_query<4, true>), dim3(g), dim3(kBlock), 0, st, d_keys, n, d_counts, v, idx, imask);
+        else hipLaunchKernelGGL((k_query<4, false>), dim3(g), dim3(kBlock), 0, st, d_keys, n, d_counts, v, idx, imask);
     }
     KC_HIP(hipGetLastError());
     if (other) {  // later work on the table (a count, its destruction) waits for the lookup
@@ -1195,7 +1229,8 @@ extern "C" int kc_table_export_device(kc_table *t, uint8_t *d_high, uint64_t hig
     const TableView v = view_of(t);
     const uint32_t g = (uint32_t)std::min<uint64_t>(t->n_tiles, (uint64_t)c->cu_count * 8);
     if (t->nw == 1) hipLaunchKernelGGL(k_export_count<1>, dim3(g), dim3(kBlock), 0, c->stream, v, !list, t->n_tiles, t->tile_counts);
-    else hipLaunchKernelGGL(k_export_count<2>, dim3(g), dim3(kBlock), 0, c->stream, v, !list, t->n_tiles, t->tile_counts);
+    else if (t->nw == 2) hipLaunchKernelGGL(k_export_count<2>, dim3(g), dim3(kBlock), 0, c->stream, v, !list, t->n_tiles, t->tile_counts);
+    else hipLaunchKernelGGL(k_export_count<4>, dim3(g), dim3(kBlock), 0, c->stream, v, !list, t->n_tiles, t->tile_counts);
     hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, c->stream, t->tile_counts, t->n_tiles);
     KC_HIP(hipGetLastError());
     uint64_t tot[2];
@@ -1206,7 +1241,8 @@ extern "C" int kc_table_export_device(kc_table *t, uint8_t *d_high, uint64_t hig
     if (!d_high || !d_low) return KC_OK;
     if (*high_bytes > high_cap || *low_bytes > low_cap) return KC_EINVAL;
     if (t->nw == 1) hipLaunchKernelGGL(k_export_write<1>, dim3(g), dim3(kBlock), 0, c->stream, v, ps, !list, t->n_tiles, t->tile_counts, d_high, d_low, n_list);
-    else hipLaunchKernelGGL(k_export_write<2>, dim3(g), dim3(kBlock), 0, c->stream, v, ps, !list, t->n_tiles, t->tile_counts, d_high, d_low, n_list);
+    else if (t->nw == 2) hipLaunchKernelGGL(k_export_write<2>, dim3(g), dim3(kBlock), 0, c->stream, v, ps, !list, t->n_tiles, t->tile_counts, d_high, d_low, n_list);
+    else hipLaunchKernelGGL(k_export_write<4>, dim3(g), dim3(kBlock), 0, c->stream, v, ps, !list, t->n_tiles, t->tile_counts, d_high, d_low, n_list);
     KC_HIP(hipGetLastError());
     if (n_list) {
         // chunks per segment: enough blocks for the fullest segment, capped
@@ -1215,7 +1251,8 @@ extern "C" int kc_table_export_device(kc_table *t, uint8_t *d_high, uint64_t hig
         const dim3 g2((uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((most + kExportTile - 1) / kExportTile, 64)),
                       KC_LOW_SEGS);
         if (t->nw == 1) hipLaunchKernelGGL(k_export_low_list<1>, g2, dim3(kBlock), 0, c->stream, v, ps, d_low);
-        else hipLaunchKernelGGL(k_export_low_list<2>, g2, dim3(kBlock), 0, c->stream, v, ps, d_low);
+        else if (t->nw == 2) hipLaunchKernelGGL(k_export_low_list<2>, g2, dim3(kBlock), 0, c->stream, v, ps, d_low);
+        else hipLaunchKernelGGL(k_export_low_list<4>, g2, dim3(kBlock), 0, c->stream, v, ps, d_low);
         KC_HIP(hipGetLastError());
     }
     return KC_OK;
@@ -1254,8 +1291,13 @@ static int export_dir(kc_table *t, const char *dir, bool append)
             rc = KC_ENOMEM;
         }
     }
-    if (rc == KC_OK && (hipMemcpy(hh.data(), dh, hb2, hipMemcpyDeviceToHost) != hipSuccess ||
-                        hipMemcpy(hl.data(), dl, lb2, hipMemcpyDeviceToHost) != hipSuccess))
+    // on the table's stream: the context's stream does not order with the
+    // null stream, so a plain hipMemcpy could read the records before the
+    // export kernels have written them
+    hipStream_t st = t->ctx->stream;
+    if (rc == KC_OK && (hipMemcpyAsync(hh.data(), dh, hb2, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                        hipMemcpyAsync(hl.data(), dl, lb2, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                        hipStreamSynchronize(st) != hipSuccess))
         rc = KC_EDEVICE;
     (void)hipFree(dh);
     (void)hipFree(dl);
@@ -1304,13 +1346,15 @@ int kc_histogram_into(kc_table *t, uint32_t min_freq, uint32_t max_freq, int64_t
     const TableView v = view_of(t);
     const uint32_t g = grid_for(t->cap, c->cu_count);
     if (t->nw == 1) hipLaunchKernelGGL(k_hist_table<1>, dim3(g), dim3(kBlock), 0, c->stream, v, hs, gb);
-    else hipLaunchKernelGGL(k_hist_table<2>, dim3(g), dim3(kBlock), 0, c->stream, v, hs, gb);
+    else if (t->nw == 2) hipLaunchKernelGGL(k_hist_table<2>, dim3(g), dim3(kBlock), 0, c->stream, v, hs, gb);
+    else hipLaunchKernelGGL(k_hist_table<4>, dim3(g), dim3(kBlock), 0, c->stream, v, hs, gb);
     if (list && hc.low) {
         uint64_t most = 0;
         for (int sgi = 0; sgi < KC_LOW_SEGS; ++sgi) most = std::max(most, hc.seg_fill[sgi]);
         const dim3 g2(kc_low_chunks(most, 4 * kBlock), KC_LOW_SEGS);
         if (t->nw == 1) hipLaunchKernelGGL(k_hist_low_list<1>, g2, dim3(kBlock), 0, c->stream, v, hs, gb);
-        else hipLaunchKernelGGL(k_hist_low_list<2>, g2, dim3(kBlock), 0, c->stream, v, hs, gb);
+        else if (t->nw == 2) hipLaunchKernelGGL(k_hist_low_list<2>, g2, dim3(kBlock), 0, c->stream, v, hs, gb);
+        else hipLaunchKernelGGL(k_hist_low_list<4>, g2, dim3(kBlock), 0, c->stream, v, hs, gb);
     }
     KC_HIP(hipGetLastError());
     KC_HIP(hipMemcpyAsync(h_bins, gb, nb * 8, hipMemcpyDeviceToHost, c->stream));

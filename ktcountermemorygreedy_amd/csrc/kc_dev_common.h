@@ -33,9 +33,36 @@ struct TableView {
     uint32_t filter_max;
     uint32_t pass_bits;          // multi-pass counting: log2(passes), 0 = one pass
     uint32_t pass;               // the pass whose k-mers are counted
+    uint32_t rw;                 // words of a caller's key (kmer<N>: 1, 2 or 4; 2 at k = 64)
 };
 
-template <int NW> constexpr int slot_words() { return NW == 1 ? 2 : 4; }
+// Key i of a caller's array of rw-word keys (kmer<N>), widened to the
+// table's NW words (k = 64 keys are kmer<2> outside and four words inside).
+template <int NW>
+__device__ __forceinline__ void caller_key(const uint64_t *keys, uint64_t i, uint32_t rw, uint64_t *key)
+{
+#pragma unroll
+    for (int w = 0; w < NW; ++w) key[w] = (uint32_t)w < rw ? keys[i * rw + w] : 0;
+}
+
+// Slot layout by key words: {key, count} (k <= 32), {word 0, word 1, count,
+// 0} (k = 33..63) and {words 0..3, count, state, 0, 0} (k = 64..128).  One-
+// and two-word keys claim a slot with a CAS on word 0, whose all-ones value
+// (KC_EMPTY) no canonical key of k <= 63 has.  Four-word keys can have an
+// all-ones word 0 (the palindrome T^32 A^32 at k = 64, for one), so their
+// slots carry a state word instead: 0 empty, 1 claimed, 2 key published.
+template <int NW> constexpr int slot_words() { return NW == 1 ? 2 : NW == 2 ? 4 : 8; }
+template <int NW> constexpr int count_word() { return NW == 1 ? 1 : NW == 2 ? 2 : 4; }
+constexpr int kStateWord = 5;
+constexpr uint64_t kSlotClaimed = 1, kSlotReady = 2;
+
+// Whether a slot holds a key (after counting has finished).
+template <int NW>
+__device__ __forceinline__ bool slot_used(const uint64_t *s)
+{
+    if constexpr (NW == 4) return s[kStateWord] == kSlotReady;
+    else return s[0] != KC_EMPTY;
+}
 
 // The counter's export filter (NewKmerTree.h:1303-1311, FinalDrainWriter.h:
 // 95-104): a record with count c is left out of high.bin when c + 1 <
@@ -72,7 +99,8 @@ template <int NW>
 __device__ __forceinline__ uint32_t coarse_mix(const uint64_t *key)
 {
     uint32_t y = fold32(key[0]);
-    if constexpr (NW == 2) y = (y * 0x85EBCA77u) ^ fold32(key[1]);
+#pragma unroll
+    for (int w = 1; w < NW; ++w) y = (y * 0x85EBCA77u) ^ fold32(key[w]);
     y ^= y >> 16;
     return y * 0x9E3779B1u;
 }
@@ -93,11 +121,17 @@ __device__ __forceinline__ uint32_t fine_digit(uint32_t mix, uint32_t l1, uint32
 // Multi-pass counting (tables larger than HBM are counted in 2^pass_bits
 // passes over the input, each holding one hash slice of the k-mers): a key
 // belongs to pass (XXH3_64 >> (64 - pass_bits)), a hash independent of the
-// partition mix, so every slice spreads over all partitions.
+// partition mix, so every slice spreads over all partitions.  The hash is
+// over the caller's key words (rw: a k = 64 key hashes as the 16 bytes of a
+// kmer<2>, though the table keeps it in four words).
 template <int NW>
-__device__ __forceinline__ bool in_pass(const uint64_t *key, uint32_t pass_bits, uint32_t pass)
+__device__ __forceinline__ bool in_pass(const uint64_t *key, uint32_t pass_bits, uint32_t pass, uint32_t rw = NW)
 {
-    return pass_bits == 0 || (uint32_t)(xxh3_64<NW>(key) >> (64 - pass_bits)) == pass;
+    if (pass_bits == 0) return true;
+    uint64_t h;
+    if constexpr (NW == 4) h = rw == 2 ? xxh3_64<2>(key) : xxh3_64<4>(key);
+    else h = xxh3_64<NW>(key);
+    return (uint32_t)(h >> (64 - pass_bits)) == pass;
 }
 
 template <int NW>
@@ -185,6 +219,45 @@ __device__ __forceinline__ bool table_add(const TableView &tv, const uint64_t *k
                                           uint32_t &n_new, uint32_t &n_err)
 {
     const uint64_t psize = 1ULL << tv.part_log2;
+    if constexpr (NW == 4) {
+        // Four-word key: the state word's CAS claims a slot, the owner writes
+        // the key and publishes it with a release store; others wait for the
+        // publication (bounded) and compare the key.
+        for (uint64_t probe = 0; probe < psize; ++probe) {
+            uint64_t *s = tv.slots + slot_index(tv, kh, probe) * slot_words<NW>();
+            uint64_t st = __hip_atomic_load(s + kStateWord, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+            bool owner = false;
+            if (st == 0) {
+                st = atomicCAS((unsigned long long *)(s + kStateWord), 0ull, (unsigned long long)kSlotClaimed);
+                owner = st == 0;
+            }
+            if (owner) {
+#pragma unroll
+                for (int w = 0; w < 4; ++w) s[w] = key[w];
+                __hip_atomic_store(s + kStateWord, kSlotReady, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            __builtin_amdgcn_wave_barrier();
+            if (owner) {
+                ++n_new;
+                add_agent(s + count_word<NW>(), inc);
+                return true;
+            }
+            for (int spin = 0; st == kSlotClaimed && spin < (1 << 16); ++spin) {
+                __builtin_amdgcn_s_sleep(1);
+                st = __hip_atomic_load(s + kStateWord, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            if (st != kSlotReady) {
+                ++n_err;
+                return false;
+            }
+            if (s[0] == key[0] && s[1] == key[1] && s[2] == key[2] && s[3] == key[3]) {
+                add_agent(s + count_word<NW>(), inc);
+                return true;
+            }
+        }
+        ++n_err;
+        return false;
+    }
     for (uint64_t probe = 0; probe < psize; ++probe) {
         uint64_t *s = tv.slots + slot_index(tv, kh, probe) * slot_words<NW>();
         // A plain read settles the common case (key present: one atomic add).
@@ -240,9 +313,14 @@ __device__ __forceinline__ uint64_t *table_locate(const TableView &tv, const uin
     const uint64_t psize = 1ULL << tv.part_log2;
     for (uint64_t probe = 0; probe < psize; ++probe) {
         uint64_t *s = tv.slots + slot_index(tv, kh, probe) * slot_words<NW>();
-        const uint64_t w0 = s[0];
-        if (w0 == KC_EMPTY) return nullptr;
-        if (w0 == key[0] && (NW == 1 || s[1] == key[NW - 1])) return s;
+        if constexpr (NW == 4) {
+            if (s[kStateWord] == 0) return nullptr;
+            if (s[0] == key[0] && s[1] == key[1] && s[2] == key[2] && s[3] == key[3]) return s;
+        } else {
+            const uint64_t w0 = s[0];
+            if (w0 == KC_EMPTY) return nullptr;
+            if (w0 == key[0] && (NW == 1 || s[1] == key[NW - 1])) return s;
+        }
     }
     return nullptr;
 }
@@ -252,7 +330,7 @@ template <int NW>
 __device__ __forceinline__ uint64_t table_find(const TableView &tv, const uint64_t *key)
 {
     const uint64_t *s = table_locate<NW>(tv, key);
-    return s ? s[NW == 1 ? 1 : 2] : 0;
+    return s ? s[count_word<NW>()] : 0;
 }
 
 // Bloom singleton split (BloomFilter.h:233-273, FastqClassifier.h:641-650):
@@ -444,6 +522,84 @@ template <> struct Roll<2> {
     }
 };
 
+// Four-word keys (k = 64..128; k = 64 is kept in four words here and
+// written out as kmer<2>): the k-mer spans words 0..last, last = (k-1)/32,
+// word `last` masked to its top 2 * (k - 32 * last) bits and the words after
+// it zero, as the oracle and kmer<N> lay them out.  The forward codes are a
+// 256-bit number with the newest code at the top (f[7] holds the top 32
+// bits), the complements a 256-bit number with the newest at the bottom
+// (r[0] the lowest 32 bits).
+template <> struct Roll<4> {
+    uint32_t f[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    uint32_t r[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    uint64_t back_mask;
+    uint32_t last, pad;  // key words in use - 1; zero bits below the reverse complement
+    __device__ Roll(uint32_t k)
+    {
+        last = (k - 1) / 32;
+        const uint32_t rem = k - 32 * last;  // 1..32 bases in word `last`
+        back_mask = rem < 32 ? (((1ULL << (2 * rem)) - 1ULL) << (64 - 2 * rem)) : ~0ULL;
+        pad = 64 * (last + 1) - 2 * k;
+    }
+    __device__ __forceinline__ void push(uint32_t code, uint32_t comp)
+    {
+#pragma unroll
+        for (int i = 0; i < 7; ++i) f[i] = __builtin_amdgcn_alignbit(f[i + 1], f[i], 2);
+        f[7] = __builtin_amdgcn_alignbit(code, f[7], 2);
+#pragma unroll
+        for (int i = 7; i > 0; --i) r[i] = __builtin_amdgcn_alignbit(r[i], r[i - 1], 30);
+        r[0] = (r[0] << 2) | comp;
+    }
+    __device__ __forceinline__ void canon(uint64_t *out) const
+    {
+        uint64_t fk[4], rw[4], rs[4], rk[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            fk[j] = ((uint64_t)f[7 - 2 * j] << 32) | f[6 - 2 * j];  // word j from the top
+            rw[j] = ((uint64_t)r[2 * j + 1] << 32) | r[2 * j];      // word j from the bottom
+        }
+        // the reverse complement: the low 2k bits of rw, shifted up by pad
+#pragma unroll
+        for (int j = 0; j < 4; ++j) rs[j] = pad ? (rw[j] << pad) | (j ? rw[j - 1] >> (64 - pad) : 0) : rw[j];
+        // key word j of it is rs[last - j]
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const uint32_t src = last - j;  // wraps for j > last
+            rk[j] = src == 0 ? rs[0] : src == 1 ? rs[1] : src == 2 ? rs[2] : src == 3 ? rs[3] : 0;
+            if ((uint32_t)j > last) fk[j] = 0;
+            if ((uint32_t)j == last) fk[j] &= back_mask;
+        }
+        // numeric (lexicographic by word) minimum
+        bool fwd = false, decided = false;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (!decided && fk[j] != rk[j]) {
+                fwd = fk[j] < rk[j];
+                decided = true;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) out[j] = (fwd || !decided) ? fk[j] : rk[j];
+    }
+    // The state after the 128 history bytes h[0..32) (byte 0 first).
+    __device__ __forceinline__ void seed(const uint32_t *h, uint32_t &have)
+    {
+        uint64_t lo[2], hi[2], inv_lo, inv_hi;
+        pack_history<16>(h, lo, inv_lo);       // bytes 0..63 (older)
+        pack_history<16>(h + 16, hi, inv_hi);  // bytes 64..127 (newer)
+        const uint64_t p[4] = {lo[0], lo[1], hi[0], hi[1]};  // p[3] the newest 32 codes
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            f[2 * j] = (uint32_t)p[j];
+            f[2 * j + 1] = (uint32_t)(p[j] >> 32);
+            const uint64_t y = pair_reverse(~p[3 - j]);
+            r[2 * j] = (uint32_t)y;
+            r[2 * j + 1] = (uint32_t)(y >> 32);
+        }
+        have = inv_hi ? (uint32_t)__clzll((long long)inv_hi) : 64 + (inv_lo ? (uint32_t)__clzll((long long)inv_lo) : 64);
+    }
+};
+
 // Pushes byte b of a Word4 and updates the trailing run of valid bytes
 // (any invalid byte resets the window, FastqParser.h:280-305 / GetKmer.h).
 template <int NW>
@@ -521,15 +677,20 @@ constexpr int kParseBlock = 256;
 constexpr int kParseSeg = 64;
 constexpr int kParseTile = kParseBlock * kParseSeg;
 
-// LDS image of one tile (+ the previous segment as halo), transposed so
-// that lane t reading its own dword j touches consecutive banks.
+// LDS image of one tile (+ the previous H segments as halo: H = 1 for
+// k <= 64, 2 for up to 127 bytes of history), transposed so that lane t
+// reading its own dword j touches consecutive banks; segment s of the tile
+// is column s + H.
 struct ParseLds {
     uint32_t w[16][kParseBlock + 2];
 };
 
+template <int NW> constexpr int halo_segs() { return NW == 4 ? 2 : 1; }
+
 // Stages tile `tile` of seq[0, n) into L (all threads; ends with a barrier).
 // A whole tile issues its four 16-byte loads per thread back to back (no
 // wait in between); only the last, partial tile takes the byte-wise path.
+template <int H = 1>
 __device__ __forceinline__ void stage_tile(const uint8_t *__restrict__ seq, uint64_t n, uint64_t tile, ParseLds &L)
 {
     const uint32_t t = threadIdx.x;
@@ -554,20 +715,20 @@ __device__ __forceinline__ void stage_tile(const uint8_t *__restrict__ seq, uint
         }
     }
     uint32_t hv = 0;
-    if (t < 16 && base >= kParseSeg) {
-        const uint64_t off = base - kParseSeg + 4ull * t;
+    if (t < 16 * H && base >= (uint64_t)kParseSeg * H) {
+        const uint64_t off = base - (uint64_t)kParseSeg * H + 4ull * t;
         if (off + 4 <= n) hv = *reinterpret_cast<const uint32_t *>(seq + off);
     }
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const uint32_t q = t + kParseBlock * r;  // uint4 index inside the tile
         const uint32_t seg = q >> 2, j0 = (q & 3) * 4;
-        L.w[j0 + 0][seg + 1] = v[r].x;
-        L.w[j0 + 1][seg + 1] = v[r].y;
-        L.w[j0 + 2][seg + 1] = v[r].z;
-        L.w[j0 + 3][seg + 1] = v[r].w;
+        L.w[j0 + 0][seg + H] = v[r].x;
+        L.w[j0 + 1][seg + H] = v[r].y;
+        L.w[j0 + 2][seg + H] = v[r].z;
+        L.w[j0 + 3][seg + H] = v[r].w;
     }
-    if (t < 16) L.w[t][0] = hv;
+    if (t < 16 * H) L.w[t & 15][t >> 4] = hv;
     __syncthreads();
 }
 
@@ -623,19 +784,24 @@ template <int NW, typename F>
 __device__ __forceinline__ void roll_segment(const ParseLds &L, uint64_t tile, uint64_t n, uint64_t emit_from,
                                              uint32_t k, F &&emit)
 {
-    constexpr int kHaloWords = NW == 1 ? 8 : 16;  // >= k-1 bytes of history
+    constexpr int kHaloWords = NW == 1 ? 8 : NW == 2 ? 16 : 32;  // >= k-1 bytes of history
+    constexpr int H = halo_segs<NW>();
     const uint32_t t = threadIdx.x;
     const uint64_t start = tile * kParseTile + (uint64_t)t * kParseSeg;
     if (start >= n) return;
     Roll<NW> roll(k);
     uint32_t have, hist[kHaloWords];
+    // the last kHaloWords words before the segment (column t + H)
 #pragma unroll
-    for (int j = 0; j < kHaloWords; ++j) hist[j] = L.w[16 - kHaloWords + j][t];
+    for (int j = 0; j < kHaloWords; ++j) {
+        const int back = kHaloWords - j;  // words before the segment
+        hist[j] = L.w[16 - 1 - ((back - 1) & 15)][t + H - 1 - (back - 1) / 16];
+    }
     roll.seed(hist, have);
     // bytes of this segment before emit_from emit nothing
     const uint32_t skip = emit_from > start ? (uint32_t)(emit_from - start < 64 ? emit_from - start : 64) : 0;
     for (int j = 0; j < 16; ++j) {
-        const Word4 x = word4(L.w[j][t + 1]);
+        const Word4 x = word4(L.w[j][t + H]);
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
             push_byte<NW>(roll, x, b, have);

@@ -48,6 +48,11 @@ __device__ __forceinline__ void string_key(const uint64_t *key, uint32_t k, uint
     // stored: position j at bit 64*NW - 2k + 2j of the (64*NW)-bit number
     if constexpr (NW == 1) {
         sk[0] = rev2(key[0] >> (64 - 2 * k)) >> (64 - 2 * k);
+    } else if constexpr (NW == 4) {
+        // the bits below position 0 are zero, so reversing the whole 256-bit
+        // number by 2-bit groups leaves the string in its low 2k bits
+#pragma unroll
+        for (int w = 0; w < 4; ++w) sk[w] = rev2(key[3 - w]);
     } else {
         // 128-bit number hi:lo = key[0]:key[1]; shift right by 128-2k, reverse 2k bits
         const uint32_t sh = 128 - 2 * k;  // >= 2 for k <= 63
@@ -127,11 +132,11 @@ __global__ __launch_bounds__(kThreads) void k_dump_low_list(TableView tv, DumpSp
 #pragma unroll
             for (int w = 0; w < NW; ++w) key[w] = lk[w];
             uint64_t *slot = table_locate<NW>(tv, key);
-            const int64_t cc = slot ? slot_high<NW>(tv, slot[NW == 1 ? 1 : 2] & ~kFlag, false) : -1;
+            const int64_t cc = slot ? slot_high<NW>(tv, slot[count_word<NW>()] & ~kFlag, false) : -1;
             if (cc >= 0 && in_range((uint64_t)cc, ds)) {
                 uint64_t merged = (uint64_t)cc + 1;
                 if (merged > tv.count_max) merged = tv.count_max;
-                slot[NW == 1 ? 1 : 2] |= kFlag;
+                slot[count_word<NW>()] |= kFlag;
                 out = in_range(merged, ds);
                 cnt = (uint32_t)merged;
             } else {
@@ -161,13 +166,13 @@ __global__ __launch_bounds__(kThreads) void k_dump_table(TableView tv, DumpSpec 
         uint64_t key[NW] = {};
         if (i < cap) {
             uint64_t *s = tv.slots + i * SW;
-            if (s[0] != KC_EMPTY) {
+            if (slot_used<NW>(s)) {
 #pragma unroll
                 for (int w = 0; w < NW; ++w) key[w] = s[w];
-                const uint64_t raw = s[NW == 1 ? 1 : 2];
+                const uint64_t raw = s[count_word<NW>()];
                 const uint64_t c = raw & ~kFlag;
                 if (raw & kFlag) {
-                    s[NW == 1 ? 1 : 2] = c;  // merged with its low record
+                    s[count_word<NW>()] = c;  // merged with its low record
                 } else if (ds.singles_low && c == 1) {
                     out = in_range(1, ds);
                     cnt = 1;
@@ -219,6 +224,11 @@ __global__ __launch_bounds__(kThreads) void k_write_lines(const uint32_t *recs, 
             if constexpr (NW == 1) {
                 const uint64_t v = (uint64_t)r[0] | ((uint64_t)r[1] << 32);
                 code = (uint32_t)(v >> bit) & 3;
+            } else if constexpr (NW == 4) {
+                // word 3 - bit / 64 of the 256-bit key (word 0 most significant)
+                const uint32_t q = 2 * (3 - bit / 64);
+                const uint64_t v = (uint64_t)r[q] | ((uint64_t)r[q + 1] << 32);
+                code = (uint32_t)(v >> (bit % 64)) & 3;
             } else {
                 const uint64_t hi = (uint64_t)r[0] | ((uint64_t)r[1] << 32);
                 const uint64_t lo = (uint64_t)r[2] | ((uint64_t)r[3] << 32);
@@ -282,10 +292,12 @@ static int dump_device(kc_table *t, uint32_t min_freq, uint32_t max_freq, int so
         // segments fill evenly (hashed appends): twice the mean covers the fullest
         const dim3 gl(kc_low_chunks(2 * n_low / KC_LOW_SEGS + 1, 4 * kThreads), KC_LOW_SEGS);
         if (nw == 1) hipLaunchKernelGGL(k_dump_low_list<1>, gl, dim3(kThreads), 0, st, tv, ds, r, cur);
-        else hipLaunchKernelGGL(k_dump_low_list<2>, gl, dim3(kThreads), 0, st, tv, ds, r, cur);
+        else if (nw == 2) hipLaunchKernelGGL(k_dump_low_list<2>, gl, dim3(kThreads), 0, st, tv, ds, r, cur);
+        else hipLaunchKernelGGL(k_dump_low_list<4>, gl, dim3(kThreads), 0, st, tv, ds, r, cur);
     }
     if (nw == 1) hipLaunchKernelGGL(k_dump_table<1>, dim3(g), dim3(kThreads), 0, st, tv, ds, r, cur);
-    else hipLaunchKernelGGL(k_dump_table<2>, dim3(g), dim3(kThreads), 0, st, tv, ds, r, cur);
+    else if (nw == 2) hipLaunchKernelGGL(k_dump_table<2>, dim3(g), dim3(kThreads), 0, st, tv, ds, r, cur);
+    else hipLaunchKernelGGL(k_dump_table<4>, dim3(g), dim3(kThreads), 0, st, tv, ds, r, cur);
     KC_HIP(hipGetLastError());
     uint64_t n = 0;
     KC_HIP(hipMemcpyAsync(&n, cursor.p, 8, hipMemcpyDeviceToHost, st));
@@ -306,7 +318,8 @@ static int dump_device(kc_table *t, uint32_t min_freq, uint32_t max_freq, int so
     const uint32_t g2 = (uint32_t)std::min<uint64_t>((n + kThreads - 1) / kThreads + 1, (uint64_t)c->cu_count * 16);
     if (n) {
         if (nw == 1) hipLaunchKernelGGL(k_line_lengths<1>, dim3(g2), dim3(kThreads), 0, st, r, n, t->cfg.k, static_cast<uint32_t *>(len.p));
-        else hipLaunchKernelGGL(k_line_lengths<2>, dim3(g2), dim3(kThreads), 0, st, r, n, t->cfg.k, static_cast<uint32_t *>(len.p));
+        else if (nw == 2) hipLaunchKernelGGL(k_line_lengths<2>, dim3(g2), dim3(kThreads), 0, st, r, n, t->cfg.k, static_cast<uint32_t *>(len.p));
+        else hipLaunchKernelGGL(k_line_lengths<4>, dim3(g2), dim3(kThreads), 0, st, r, n, t->cfg.k, static_cast<uint32_t *>(len.p));
         if ((rc = kc_scan_u32(c, static_cast<uint32_t *>(len.p), n, static_cast<uint64_t *>(off.p),
                               static_cast<uint64_t *>(sums.p))) != KC_OK)
             return rc;
@@ -322,7 +335,8 @@ static int dump_device(kc_table *t, uint32_t min_freq, uint32_t max_freq, int so
     if (hipMalloc(&text.p, bytes + 16) != hipSuccess) return KC_ENOMEM;
     if (n) {
         if (nw == 1) hipLaunchKernelGGL(k_write_lines<1>, dim3(g2), dim3(kThreads), 0, st, r, n, t->cfg.k, static_cast<uint64_t *>(off.p), static_cast<uint8_t *>(text.p));
-        else hipLaunchKernelGGL(k_write_lines<2>, dim3(g2), dim3(kThreads), 0, st, r, n, t->cfg.k, static_cast<uint64_t *>(off.p), static_cast<uint8_t *>(text.p));
+        else if (nw == 2) hipLaunchKernelGGL(k_write_lines<2>, dim3(g2), dim3(kThreads), 0, st, r, n, t->cfg.k, static_cast<uint64_t *>(off.p), static_cast<uint8_t *>(text.p));
+        else hipLaunchKernelGGL(k_write_lines<4>, dim3(g2), dim3(kThreads), 0, st, r, n, t->cfg.k, static_cast<uint64_t *>(off.p), static_cast<uint8_t *>(text.p));
         KC_HIP(hipGetLastError());
     }
     KC_HIP(hipStreamSynchronize(st));

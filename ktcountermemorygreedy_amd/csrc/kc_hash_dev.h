@@ -247,7 +247,8 @@ template <int NW>
 __device__ __forceinline__ uint64_t slot_mix(const uint64_t *w)
 {
     uint64_t x = w[0];
-    if constexpr (NW == 2) x ^= w[1] * kSlotFoldC;
+    if constexpr (NW >= 2) x ^= w[1] * kSlotFoldC;
+    if constexpr (NW == 4) x ^= (w[2] ^ ((w[3] >> 32) | (w[3] << 32))) * kSlotMixC;
     x = (x ^ (x >> 32)) * kSlotMixC;
     return x ^ (x >> 32);
 }

@@ -682,12 +682,12 @@ extern "C" int kc_plan_sizes(uint64_t distinct_total, uint64_t repeated_total, u
                              kc_table_config *cfg, kc_plan *plan)
 {
     if (!cfg || !plan) return KC_EINVAL;
-    if (cfg->k < 1 || cfg->k > 63) return cfg->k > 63 ? KC_ENOTSUP : KC_EINVAL;
-    const uint32_t nw = cfg->k <= 32 ? 1 : 2;
+    if (cfg->k < 1 || cfg->k > 128) return KC_EINVAL;
+    const uint32_t nw = cfg->k <= 32 ? 1 : cfg->k <= 63 ? 2 : 4;  // the table's key words
     plan->distinct = distinct_total;
     plan->repeated = std::min(repeated_total, distinct_total);
     // one pass, then halve per pass until the device bytes fit the budget
-    const uint64_t slot_bytes = nw == 1 ? 16 : 32;
+    const uint64_t slot_bytes = 8ull * (nw == 1 ? 2 : nw == 2 ? 4 : 8);
     const uint64_t lds_bloom = kc_lds_bloom(nw);  // Bloom blocks of one partition in the LDS pass
     const uint64_t max_bloom = lds_bloom << 18;  // the partition pass takes 2^18 Bloom parts
     uint32_t P = 1;
@@ -732,7 +732,7 @@ extern "C" int kc_plan_table(kc_ctx *ctx, const uint8_t *d_seq, uint64_t nbytes,
                              uint64_t hbm_budget, kc_table_config *cfg, kc_plan *plan)
 {
     if (!ctx || !cfg || !plan || (nbytes && !d_seq)) return KC_EINVAL;
-    if (cfg->k < 1 || cfg->k > 63) return cfg->k > 63 ? KC_ENOTSUP : KC_EINVAL;
+    if (cfg->k < 1 || cfg->k > 128) return KC_EINVAL;
     uint32_t S = sample_slices ? sample_slices : 16;
     if (S > 256 || (S & (S - 1))) return KC_EINVAL;
     // the probe holds every distinct key of its slice: at most nbytes / S

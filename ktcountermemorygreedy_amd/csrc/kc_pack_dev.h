@@ -13,7 +13,7 @@ struct PackSpec {
     uint64_t tail_mask;
 };
 
-// Packing of a k (1..64) and count_max: high records are kmer_bytes +
+// Packing of a k (1..128) and count_max: high records are kmer_bytes +
 // count_bytes long (FinalDrainWriter.h:43-52), low records kmer_bytes.
 inline PackSpec kc_pack_spec(uint32_t k, uint32_t count_max)
 {
@@ -69,7 +69,9 @@ __device__ __forceinline__ void unpack_key(const uint8_t *src, const PackSpec &p
 constexpr int kPackBlock = 256;
 constexpr int kExportPer = 8;
 constexpr uint64_t kExportTile = (uint64_t)kPackBlock * kExportPer;
-constexpr uint32_t kExportStage = (uint32_t)kExportTile * 20;  // bytes: k=63 high record
+// LDS staging of a tile's records by key words: 20 bytes hold a k <= 64 high
+// record (16 key bytes + 4 count bytes), 36 a k <= 128 one.
+template <int NW> constexpr uint32_t export_stage() { return (uint32_t)kExportTile * (NW == 4 ? 36 : 20); }
 
 // Copies nbytes of LDS staging to dst (any alignment): byte stores up to a
 // 4-byte boundary, then 4-byte stores, then the tail; consecutive threads

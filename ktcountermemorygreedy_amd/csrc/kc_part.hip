@@ -943,6 +943,7 @@ bool dev_alloc(T *&p, uint64_t n)
 // Whether the LDS partition pass can hold this table's partitions.
 bool kc_part_eligible(const kc_table *t)
 {
+    if (t->nw > 2) return false;  // four-word keys (k >= 64) count on the direct path
     const bool bloom = t->bloom != nullptr;
     if ((1ull << t->part_log2) > kc_lds_part_slots(t->nw, bloom)) return false;
     if (bloom && (1ull << t->bloom_part_log2) > kc_lds_part_bloom(t->nw)) return false;

@@ -143,12 +143,12 @@ struct kc_server {
     bool stop = false;
 
     // Device bytes of a count job while it runs (footprint) and of its table
-    // afterwards (resident): slots (16/32 B), Bloom filter, FIRST list and
+    // afterwards (resident): slots (16/32/64 B), Bloom filter, FIRST list and
     // its lookup index when the split is on; while counting also the device
     // staging and the partitioned pass's two key arrays.
     static uint64_t footprint(Request &r)
     {
-        const uint64_t nw = r.k <= 32 ? 1 : 2;
+        const uint64_t nw = r.k <= 32 ? 1 : r.k <= 63 ? 2 : 4;  // the table's key words
         const uint64_t want = std::max<uint64_t>(r.max_distinct, 1024);
         r.capacity = std::max<uint64_t>(1024, next_pow2(want + want / 2));
         r.resident = r.capacity * 16 * nw + (1 << 20);
@@ -253,7 +253,7 @@ struct kc_server {
         uint64_t table_bytes = 0;
         if (rc == KC_OK) {
             tab->id = r->table_id;
-            tab->nw = r->k <= 32 ? 1 : 2;
+            tab->nw = r->k <= 32 ? 1 : r->k <= 64 ? 2 : 4;  // words of a lookup key (kmer<N>)
             tab->ctx = job_ctx[r->job_slot];
             tab->sctx = lookup_ctx;
         }
@@ -471,7 +471,7 @@ extern "C" int kc_server_submit_count(kc_server *s, uint64_t table_id, uint32_t 
                                       uint64_t nbytes, uint64_t max_distinct, int bloom, uint64_t *req_id)
 {
     return kc_guard([&]() -> int {
-        if (!s || !req_id || (!h_seq && nbytes) || k < 1 || k > 63) return KC_EINVAL;
+        if (!s || !req_id || (!h_seq && nbytes) || k < 1 || k > 128) return KC_EINVAL;
         auto r = std::make_unique<Request>();
         r->kind = 0;
         r->table_id = table_id;

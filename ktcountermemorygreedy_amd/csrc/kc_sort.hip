@@ -11,7 +11,7 @@
 //   downsweep  per-tile stable ranking (wave ballots, waves in order), the
 //              tile is placed digit-sorted in LDS and written out in runs.
 // Records are 32-bit-word arrays: {key words (2N u32), count u32, [pad]} at
-// 12, 16, 20 or 24 bytes.  Integer work only, HBM-bound: each pass reads and
+// 12, 16, 20, 24, 36 or 40 bytes.  Integer work only, HBM-bound: each pass reads and
 // writes every record once (plus the key words once in the upsweep).
 #include <hip/hip_runtime.h>
 
@@ -28,13 +28,15 @@ constexpr int kThreads = 256;
 constexpr int kSortDigitBits = 9;  // 5 passes over 42 bits; 11 (4 passes) measured slower: one block per CU
 
 // Downsweep tiles: TB threads x ITEMS records (8192 records of up to 16
-// bytes, 4096 of 20-24 bytes), staged in LDS (96-128 KB) so that a digit's
-// records of a tile leave as one run (16 records per digit at 9 bits).
+// bytes, 4096 of 20-24 bytes, 2048 of 36-40 bytes), staged in LDS (80-128
+// KB) so that a digit's records of a tile leave as one run (16 records per
+// digit at 9 bits, 4 for four-word keys).
 template <int RW> struct Layout {
-    static constexpr int NW = RW >= 5 ? 2 : 1;   // key words (u64)
+    static constexpr int NW = RW >= 9 ? 4 : RW >= 5 ? 2 : 1;       // key words (u64)
     static constexpr int TB = 1024;
-    static constexpr int ITEMS = RW >= 5 ? 4 : 8; // records per thread per tile
+    static constexpr int ITEMS = RW >= 9 ? 2 : RW >= 5 ? 4 : 8;    // records per thread per tile
     static constexpr int TILE = TB * ITEMS;
+    static constexpr int UP = RW >= 9 ? 8 : 16;                    // upsweep records per thread per chunk
 };
 
 // Digit of bits [bit, bit+width) of the key (LSB-numbered over the whole
@@ -49,6 +51,17 @@ __device__ __forceinline__ uint32_t digit_of(const uint32_t *rec, uint32_t bit, 
     uint64_t v;
     if constexpr (NW == 1) {
         v = k0 >> bit;
+    } else if constexpr (NW == 4) {
+        // key word 3 - bit / 64 holds the digit's low bits, the next more
+        // significant word any bits that straddle
+        const uint64_t k1 = (uint64_t)rec[2] | ((uint64_t)rec[3] << 32);
+        const uint64_t k2 = (uint64_t)rec[4] | ((uint64_t)rec[5] << 32);
+        const uint64_t k3 = (uint64_t)rec[6] | ((uint64_t)rec[7] << 32);
+        const uint32_t wi = bit >> 6, sh = bit & 63;
+        const uint64_t lo = wi == 0 ? k3 : wi == 1 ? k2 : wi == 2 ? k1 : k0;
+        const uint64_t hi = wi == 0 ? k2 : wi == 1 ? k1 : k0;
+        v = lo >> sh;
+        if (wi < 3 && sh + width > 64) v |= hi << (64 - sh);
     } else {
         const uint64_t k1 = (uint64_t)rec[2] | ((uint64_t)rec[3] << 32);
         const uint32_t sh = bit & 63;
@@ -80,8 +93,7 @@ __device__ __forceinline__ void block_tiles(uint64_t n_tiles, uint64_t &first, u
 }
 
 // Digit counts of block b's records: the records of its downsweep tiles,
-// read in chunks of kThreads x kUpItems.
-constexpr int kUpItems = 16;
+// read in chunks of kThreads x Layout::UP.
 
 template <int RW, int DB>
 __global__ __launch_bounds__(kThreads) void k_upsweep(const uint32_t *__restrict__ src, uint64_t n, uint32_t bit,
@@ -95,16 +107,16 @@ __global__ __launch_bounds__(kThreads) void k_upsweep(const uint32_t *__restrict
     uint64_t first, last;
     block_tiles(n_tiles, first, last);
     const uint64_t r0 = first * L::TILE, r1 = last * L::TILE < n ? last * L::TILE : n;
-    for (uint64_t c0 = r0; c0 < r1; c0 += (uint64_t)kThreads * kUpItems) {
-        uint32_t r[kUpItems][RW];
+    for (uint64_t c0 = r0; c0 < r1; c0 += (uint64_t)kThreads * L::UP) {
+        uint32_t r[L::UP][RW];
         // all loads first (index clamped into the range), then the counts
 #pragma unroll
-        for (int j = 0; j < kUpItems; ++j) {
+        for (int j = 0; j < L::UP; ++j) {
             const uint64_t i = c0 + (uint64_t)j * kThreads + threadIdx.x;
             load_rec<RW>(src, i < r1 ? i : r1 - 1, r[j]);
         }
 #pragma unroll
-        for (int j = 0; j < kUpItems; ++j)
+        for (int j = 0; j < L::UP; ++j)
             if (c0 + (uint64_t)j * kThreads + threadIdx.x < r1)
                 atomicAdd(&hist[digit_of<L::NW>(r[j], bit, width)], 1u);
     }
@@ -516,7 +528,7 @@ extern "C" int kc_sort_records(kc_ctx *ctx, void *d_recs, void *d_scratch, uint6
                                uint32_t cpb, uint32_t digit_bits, uint32_t record_bytes, int *result_in_scratch)
 {
     if (!ctx || !result_in_scratch || (n && (!d_recs || !d_scratch)) || d_recs == d_scratch) return KC_EINVAL;
-    if (nw != 1 && nw != 2) return KC_EINVAL;
+    if (nw != 1 && nw != 2 && nw != 4) return KC_EINVAL;
     if (record_bytes != 8 * nw + 4 && record_bytes != 8 * nw + 8) return KC_EINVAL;
     if ((reinterpret_cast<uintptr_t>(d_recs) | reinterpret_cast<uintptr_t>(d_scratch)) & 3) return KC_EINVAL;
     uint32_t lo, hi;
@@ -539,6 +551,8 @@ extern "C" int kc_sort_records(kc_ctx *ctx, void *d_recs, void *d_scratch, uint6
     case 4: return sort_impl<4>(ctx, a, b, n, lo, hi, result_in_scratch);
     case 5: return sort_impl<5>(ctx, a, b, n, lo, hi, result_in_scratch);
     case 6: return sort_impl<6>(ctx, a, b, n, lo, hi, result_in_scratch);
+    case 9: return sort_impl<9>(ctx, a, b, n, lo, hi, result_in_scratch);
+    case 10: return sort_impl<10>(ctx, a, b, n, lo, hi, result_in_scratch);
     default: return KC_EINVAL;
     }
 }
@@ -546,7 +560,7 @@ extern "C" int kc_sort_records(kc_ctx *ctx, void *d_recs, void *d_scratch, uint6
 extern "C" int kc_records_histogram(kc_ctx *ctx, const void *d_recs, uint64_t n, uint32_t nw, uint32_t record_bytes,
                                     uint32_t min_freq, uint32_t max_freq, int64_t *h_bins)
 {
-    if (!ctx || !h_bins || (n && !d_recs) || max_freq < min_freq || (nw != 1 && nw != 2)) return KC_EINVAL;
+    if (!ctx || !h_bins || (n && !d_recs) || max_freq < min_freq || (nw != 1 && nw != 2 && nw != 4)) return KC_EINVAL;
     if (record_bytes != 8 * nw + 4 && record_bytes != 8 * nw + 8) return KC_EINVAL;
     const uint64_t nb = (uint64_t)max_freq - min_freq + 1;
     unsigned long long *gb = nullptr;
@@ -559,7 +573,9 @@ extern "C" int kc_records_histogram(kc_ctx *ctx, const void *d_recs, uint64_t n,
         case 3: hipLaunchKernelGGL(k_count_hist<3>, dim3(g), dim3(kThreads), 0, ctx->stream, r, n, min_freq, max_freq, gb); break;
         case 4: hipLaunchKernelGGL(k_count_hist<4>, dim3(g), dim3(kThreads), 0, ctx->stream, r, n, min_freq, max_freq, gb); break;
         case 5: hipLaunchKernelGGL(k_count_hist<5>, dim3(g), dim3(kThreads), 0, ctx->stream, r, n, min_freq, max_freq, gb); break;
-        default: hipLaunchKernelGGL(k_count_hist<6>, dim3(g), dim3(kThreads), 0, ctx->stream, r, n, min_freq, max_freq, gb); break;
+        case 6: hipLaunchKernelGGL(k_count_hist<6>, dim3(g), dim3(kThreads), 0, ctx->stream, r, n, min_freq, max_freq, gb); break;
+        case 9: hipLaunchKernelGGL(k_count_hist<9>, dim3(g), dim3(kThreads), 0, ctx->stream, r, n, min_freq, max_freq, gb); break;
+        default: hipLaunchKernelGGL(k_count_hist<10>, dim3(g), dim3(kThreads), 0, ctx->stream, r, n, min_freq, max_freq, gb); break;
         }
         if (hipGetLastError() != hipSuccess) rc = KC_EDEVICE;
     }
@@ -614,6 +630,8 @@ int kc_sort_range(kc_ctx *ctx, uint32_t *a, uint32_t *b, uint64_t n, uint32_t rw
     case 4: return sort_impl<4>(ctx, a, b, n, lo, hi, in_scratch);
     case 5: return sort_impl<5>(ctx, a, b, n, lo, hi, in_scratch);
     case 6: return sort_impl<6>(ctx, a, b, n, lo, hi, in_scratch);
+    case 9: return sort_impl<9>(ctx, a, b, n, lo, hi, in_scratch);
+    case 10: return sort_impl<10>(ctx, a, b, n, lo, hi, in_scratch);
     default: return KC_EINVAL;
     }
 }

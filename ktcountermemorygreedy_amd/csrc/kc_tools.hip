@@ -66,7 +66,7 @@ template <int NW>
 __global__ __launch_bounds__(kB) void k_pack_low(const uint64_t *__restrict__ keys, uint64_t n, PackSpec ps,
                                                  uint8_t *__restrict__ out)
 {
-    __shared__ uint8_t stage[kExportStage];
+    __shared__ uint8_t stage[export_stage<NW>()];
     const uint64_t n_tiles = (n + kExportTile - 1) / kExportTile;
     for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
         const uint64_t t0 = tile * kExportTile;
@@ -108,7 +108,7 @@ __global__ __launch_bounds__(kB) void k_pack_high(const uint64_t *__restrict__ k
                                                   uint64_t n, PackSpec ps, uint32_t fmin, uint32_t fmax,
                                                   const uint64_t *__restrict__ tile_off, uint8_t *__restrict__ out)
 {
-    __shared__ uint8_t stage[kExportStage];
+    __shared__ uint8_t stage[export_stage<NW>()];
     const uint32_t rec = ps.kmer_bytes + ps.count_bytes;
     const uint64_t n_tiles = (n + kExportTile - 1) / kExportTile;
     for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
@@ -161,7 +161,8 @@ __global__ __launch_bounds__(kB) void k_unpack(const uint8_t *__restrict__ src, 
 }
 
 // Keys [base, base + n) of a FIRST list loaded whole: key g goes to
-// segment g % KC_LOW_SEGS at row g / KC_LOW_SEGS.
+// segment g % KC_LOW_SEGS at row g / KC_LOW_SEGS, widened from the caller's
+// tv.rw words to the table's NW.
 template <int NW>
 __global__ __launch_bounds__(kB) void k_list_load(TableView tv, const uint64_t *__restrict__ keys, uint64_t n,
                                                   uint64_t base)
@@ -169,8 +170,10 @@ __global__ __launch_bounds__(kB) void k_list_load(TableView tv, const uint64_t *
     for (uint64_t i = blockIdx.x * (uint64_t)kB + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kB) {
         const uint64_t g = base + i;
         uint64_t *dst = tv.low + ((g % KC_LOW_SEGS) * tv.seg_cap + g / KC_LOW_SEGS) * NW;
+        uint64_t key[NW];
+        caller_key<NW>(keys, i, tv.rw, key);
 #pragma unroll
-        for (int w = 0; w < NW; ++w) dst[w] = keys[i * NW + w];
+        for (int w = 0; w < NW; ++w) dst[w] = key[w];
     }
 }
 
@@ -240,13 +243,14 @@ int file_size(const std::string &path, uint64_t *size)
 
 extern "C" int kc_export_low(kc_ctx *ctx, const uint64_t *d_keys, uint64_t n, uint32_t k, uint8_t *d_out)
 {
-    if (!ctx || k < 1 || k > 63 || (n && (!d_keys || !d_out))) return KC_EINVAL;
+    if (!ctx || k < 1 || k > 128 || (n && (!d_keys || !d_out))) return KC_EINVAL;
     if (!n) return KC_OK;
     KC_HIP(hipSetDevice(ctx->device));
     const PackSpec ps = kc_pack_spec(k, 0xFF);
     const uint32_t g = (uint32_t)std::min<uint64_t>((n + kExportTile - 1) / kExportTile, (uint64_t)ctx->cu_count * 8);
     if (k <= 32) hipLaunchKernelGGL(k_pack_low<1>, dim3(g), dim3(kB), 0, ctx->stream, d_keys, n, ps, d_out);
-    else hipLaunchKernelGGL(k_pack_low<2>, dim3(g), dim3(kB), 0, ctx->stream, d_keys, n, ps, d_out);
+    else if (k <= 64) hipLaunchKernelGGL(k_pack_low<2>, dim3(g), dim3(kB), 0, ctx->stream, d_keys, n, ps, d_out);
+    else hipLaunchKernelGGL(k_pack_low<4>, dim3(g), dim3(kB), 0, ctx->stream, d_keys, n, ps, d_out);
     KC_HIP(hipGetLastError());
     return KC_OK;
 }
@@ -255,7 +259,7 @@ extern "C" int kc_export_high(kc_ctx *ctx, const uint64_t *d_keys, const uint32_
                               uint32_t count_max, uint32_t filter_min, uint32_t filter_max, uint8_t *d_out,
                               uint64_t out_cap, uint64_t *out_bytes)
 {
-    if (!ctx || !out_bytes || k < 1 || k > 63 || count_max == 0 || (n && (!d_keys || !d_counts))) return KC_EINVAL;
+    if (!ctx || !out_bytes || k < 1 || k > 128 || count_max == 0 || (n && (!d_keys || !d_counts))) return KC_EINVAL;
     *out_bytes = 0;
     if (!n) return KC_OK;
     KC_HIP(hipSetDevice(ctx->device));
@@ -281,8 +285,11 @@ extern "C" int kc_export_high(kc_ctx *ctx, const uint64_t *d_keys, const uint32_
     if (k <= 32)
         hipLaunchKernelGGL(k_pack_high<1>, dim3(g), dim3(kB), 0, ctx->stream, d_keys, d_counts, n, ps, filter_min,
                            filter_max, off.p, d_out);
-    else
+    else if (k <= 64)
         hipLaunchKernelGGL(k_pack_high<2>, dim3(g), dim3(kB), 0, ctx->stream, d_keys, d_counts, n, ps, filter_min,
+                           filter_max, off.p, d_out);
+    else
+        hipLaunchKernelGGL(k_pack_high<4>, dim3(g), dim3(kB), 0, ctx->stream, d_keys, d_counts, n, ps, filter_min,
                            filter_max, off.p, d_out);
     KC_HIP(hipGetLastError());
     KC_HIP(hipStreamSynchronize(ctx->stream));  // the scan scratch is freed on return
@@ -311,7 +318,8 @@ static int stream_records(kc_ctx *ctx, const std::string &path, uint64_t n, uint
         if (std::fread(h.p, rec, m, f.f) != m) return KC_EIO;
         KC_HIP(hipMemcpyAsync(raw.p, h.p, m * rec, hipMemcpyHostToDevice, ctx->stream));
         if (nw == 1) hipLaunchKernelGGL(k_unpack<1>, dim3(grid_of(m, ctx->cu_count)), dim3(kB), 0, ctx->stream, raw.p, m, ps, counted, keys.p, counts.p);
-        else hipLaunchKernelGGL(k_unpack<2>, dim3(grid_of(m, ctx->cu_count)), dim3(kB), 0, ctx->stream, raw.p, m, ps, counted, keys.p, counts.p);
+        else if (nw == 2) hipLaunchKernelGGL(k_unpack<2>, dim3(grid_of(m, ctx->cu_count)), dim3(kB), 0, ctx->stream, raw.p, m, ps, counted, keys.p, counts.p);
+        else hipLaunchKernelGGL(k_unpack<4>, dim3(grid_of(m, ctx->cu_count)), dim3(kB), 0, ctx->stream, raw.p, m, ps, counted, keys.p, counts.p);
         KC_HIP(hipGetLastError());
         const int rc = use(keys.p, counts.p, m, done);
         if (rc != KC_OK) return rc;
@@ -332,10 +340,10 @@ static int load_dir(kc_ctx *ctx, const char *dir, int with_low, kc_table **out)
         if (std::fread(infos, 4, 2, f.f) != 2) return KC_EIO;
     }
     const uint32_t k = infos[0], count_max = infos[1];
-    if (k > 63 && k <= 128) return KC_ENOTSUP;
-    if (k < 1 || k > 63 || count_max == 0) return KC_EINVAL;
+    if (k < 1 || k > 128 || count_max == 0) return KC_EINVAL;
     const PackSpec ps = kc_pack_spec(k, count_max);
-    const uint32_t nw = k <= 32 ? 1 : 2;
+    const uint32_t nw = k <= 32 ? 1 : k <= 64 ? 2 : 4;  // the files' kmer<N> words
+    const uint32_t tnw = k <= 32 ? 1 : k <= 63 ? 2 : 4;  // the table's key words
     uint64_t hsize = 0, lsize = 0;
     int rc = file_size(d + "high.bin", &hsize);
     if (rc != KC_OK) return rc;
@@ -351,7 +359,7 @@ static int load_dir(kc_ctx *ctx, const char *dir, int with_low, kc_table **out)
     cfg.max_distinct = n_high + 1;
     cfg.bloom = with_low ? 1 : 0;  // the FIRST list holds low.bin
     cfg.low_capacity = n_low + 1;
-    cfg.bloom_blocks = with_low ? cfg.capacity / kc_lds_part_slots(nw, true) : 0;
+    cfg.bloom_blocks = with_low ? cfg.capacity / kc_lds_part_slots(tnw, true) : 0;
     cfg.count_path = KC_PATH_DIRECT;
     kc_table *t = nullptr;
     if ((rc = kc_table_create(ctx, &cfg, &t)) != KC_OK) return rc;
@@ -361,8 +369,9 @@ static int load_dir(kc_ctx *ctx, const char *dir, int with_low, kc_table **out)
     if (rc == KC_OK && with_low) {
         const kc::TableView tv = kc_table_view(t);
         rc = stream_records(ctx, d + "low.bin", n_low, nw, ps, false, [&](uint64_t *keys, uint32_t *, uint64_t m, uint64_t base) {
-            if (nw == 1) hipLaunchKernelGGL(k_list_load<1>, dim3(grid_of(m, ctx->cu_count)), dim3(kB), 0, ctx->stream, tv, keys, m, base);
-            else hipLaunchKernelGGL(k_list_load<2>, dim3(grid_of(m, ctx->cu_count)), dim3(kB), 0, ctx->stream, tv, keys, m, base);
+            if (tnw == 1) hipLaunchKernelGGL(k_list_load<1>, dim3(grid_of(m, ctx->cu_count)), dim3(kB), 0, ctx->stream, tv, keys, m, base);
+            else if (tnw == 2) hipLaunchKernelGGL(k_list_load<2>, dim3(grid_of(m, ctx->cu_count)), dim3(kB), 0, ctx->stream, tv, keys, m, base);
+            else hipLaunchKernelGGL(k_list_load<4>, dim3(grid_of(m, ctx->cu_count)), dim3(kB), 0, ctx->stream, tv, keys, m, base);
             return hipGetLastError() == hipSuccess ? KC_OK : KC_EDEVICE;
         });
         if (rc == KC_OK) {

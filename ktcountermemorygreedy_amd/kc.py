@@ -203,7 +203,8 @@ def check(status, what):
 
 
 def words_for_k(k):
-    return 1 if k <= 32 else 2
+    """Words of a k-mer key (kmer<N>, main.cpp:617-633)."""
+    return 1 if k <= 32 else 2 if k <= 64 else 4
 
 
 def _ptr(t):
@@ -301,6 +302,31 @@ class PinnedBuffer:
             pass
 
 
+class _Native:
+    """A native handle and the handles that must be destroyed before it (a
+    table's batches, a context's tables).  Parents hold these records
+    strongly and the Python objects weakly-by-id, so a parent destroys its
+    children first even when the garbage collector finalizes a cycle of
+    them in any order (weak references are cleared before finalizers run);
+    a destroyed record is a no-op afterwards."""
+
+    def __init__(self, h, destroy, parent=None):
+        self.h, self._destroy, self.children, self._parent = h, destroy, {}, parent
+        if parent is not None:
+            parent.children[id(self)] = self
+
+    def destroy(self):
+        if self.h:
+            for c in list(self.children.values()):
+                c.destroy()
+            self.children.clear()
+            self._destroy(self.h)
+            self.h = None
+        if self._parent is not None:
+            self._parent.children.pop(id(self), None)
+            self._parent = None
+
+
 class Context:
     """kc_ctx: one device and one HIP stream."""
 
@@ -309,7 +335,7 @@ class Context:
         check(lib().kc_ctx_create(device, C.byref(h)), "kc_ctx_create")
         self.h = h
         self.device = device
-        self._tables = weakref.WeakSet()  # closed before the context: they use its stream
+        self._tables = _Native(h, lambda _: None)  # tables (and their batches) destroyed before the context
         self._buffers = weakref.WeakSet()  # device and pinned buffers it allocated, freed on close
 
     def sync(self):
@@ -321,8 +347,7 @@ class Context:
 
     def close(self):
         if self.h:
-            for t in list(self._tables):
-                t.close()
+            self._tables.destroy()
             for b in list(self._buffers):
                 b.free()
             lib().kc_ctx_destroy(self.h)
@@ -438,10 +463,13 @@ class Table:
             check(lib().kc_table_create(ctx.h, C.byref(cfg), C.byref(h)), "kc_table_create")
         else:
             h = _handle
-        self.h, self.ctx, self.k, self.nw, self.count_max = h, ctx, k, words_for_k(k), count_max
+        self._native = _Native(h, lambda x: lib().kc_table_destroy(x), ctx._tables)
+        self.ctx, self.k, self.nw, self.count_max = ctx, k, words_for_k(k), count_max
         self.bloom = bool(bloom)
-        self._batches = weakref.WeakSet()  # destroyed before the table: they count into it
-        ctx._tables.add(self)
+
+    @property
+    def h(self):
+        return self._native.h
 
     @classmethod
     def load_dir(cls, ctx: Context, path, with_low=True):
@@ -455,11 +483,7 @@ class Table:
         return cls(ctx, int(k), int(count_max), bloom=with_low, _handle=h)
 
     def close(self):
-        if self.h:
-            for b in list(self._batches):
-                b.close()
-            lib().kc_table_destroy(self.h)
-            self.h = None
+        self._native.destroy()  # its batches first
 
     def __del__(self):
         try:
@@ -600,8 +624,12 @@ class Batch:
             check(lib().kc_batch_create(table.h, C.byref(t), C.byref(h)), "kc_batch_create")
         else:
             check(lib().kc_batch_create(table.h, None, C.byref(h)), "kc_batch_create")
-        self.h, self.table = h, table
-        table._batches.add(self)
+        self._native = _Native(h, lambda x: lib().kc_batch_destroy(x), table._native)
+        self.table = table
+
+    @property
+    def h(self):
+        return self._native.h
 
     def add(self, chunk):
         buf = np.frombuffer(chunk, dtype=np.uint8) if isinstance(chunk, (bytes, bytearray)) else \
@@ -634,9 +662,7 @@ class Batch:
         return {f: getattr(i, f) for f, _ in BatchInfo._fields_}
 
     def close(self):
-        if self.h:
-            lib().kc_batch_destroy(self.h)
-            self.h = None
+        self._native.destroy()
 
     def __del__(self):
         try:

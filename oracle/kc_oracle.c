@@ -13,6 +13,11 @@
  *  - kco_count, kco_merge, kco_histogram, kco_dump_text, kco_histogram_text:
  *    reference goldens (byte-identical dump --sort and histogram text) plus
  *    the merge-rule KAT of test_low_frequency_query_thread_pool.cpp:94-138.
+ *    For k = 65..128 the reference's dump text is not a k-mer spelling (its
+ *    sort key holds 128 bits) and at k = 96 its counter splits counts: there
+ *    the extraction, counts and dump text are pinned by a count done on
+ *    strings (tests/test_oracle.py) and the histograms by the reference's
+ *    goldens at k = 65, 100, 127 and 128.
  *  - kco_pack_kmer / kco_unpack_kmer: the k=33 / k=64 vectors of
  *    test_export_reader.cpp:175-289 and round trips through the reference
  *    tools (their reader unpacks our files).
@@ -112,6 +117,9 @@ static uint64_t extract_impl(const uint8_t *bytes, uint64_t len, uint32_t k, uin
         for (int i = (int)nw - 1; i > 0; --i) fwd[i] = (fwd[i] >> 2) | (fwd[i - 1] << 62);
         fwd[0] = (fwd[0] >> 2) | ((uint64_t)c << 62);
         fwd[last] &= back_mask;
+        /* words after `last` are padding: zero (kmer<4> at k = 65..96 leaves
+         * shifted-out bases there, GetKmer.h:71-73 masks word `last` only) */
+        for (uint32_t i = last + 1; i < nw; ++i) fwd[i] = 0;
         for (uint32_t i = 0; i + 1 < nw; ++i) rev[i] = (rev[i] << 2) | (rev[i + 1] >> 62);
         rev[nw - 1] <<= 2;
         rev[last] |= (uint64_t)(c ^ 3) << rev_shift;
@@ -382,40 +390,45 @@ void kco_histogram(const uint64_t *high_keys, const uint32_t *high_counts,
 /* ------------------------------------------------------------------ */
 /* Text outputs.  dump --sort: lexicographic (A<C<G<T) order of the string
  * (dump_tool.cpp:345-354 make_sort_key, :373-454 sort), line = bases, '\t',
- * decimal count, '\n' (dump_tool.cpp:1172-1193).  Keys with k<=64 only. */
-static u128 string_order_key(const uint64_t *key, uint32_t k)
+ * decimal count, '\n' (dump_tool.cpp:1172-1193).  The string-order key is
+ * a 256-bit number (word 0 most significant) whose low 2k bits hold the
+ * bases, first base most significant; for k > 64 that is wider than the
+ * reference's 128-bit sort key (SortKeyType<4>, dump_tool.cpp:320-324),
+ * which keeps only the last 64 bases of each line. */
+static void string_order_key(const uint64_t *key, uint32_t k, uint64_t sk[4])
 {
     const uint32_t nw = kco_words_for_k(k);
     const uint32_t base0 = 64 * nw - 2 * k;
-    u128 sk = 0;
+    sk[0] = sk[1] = sk[2] = sk[3] = 0;
     for (uint32_t j = 0; j < k; ++j) {
         uint32_t bit = base0 + 2 * j;
         uint64_t c = (key[nw - 1 - bit / 64] >> (bit % 64)) & 3;
-        sk = (sk << 2) | c;
+        uint32_t at = 2 * (k - 1 - j); /* the string key's bit for base j */
+        sk[3 - at / 64] |= c << (at % 64);
     }
-    return sk;
 }
 
 uint64_t kco_dump_text(const uint64_t *keys, const uint32_t *counts, uint64_t n,
                        uint32_t k, char *out)
 {
     static const char B[4] = {'A', 'C', 'G', 'T'};
-    uint64_t *sk = malloc((n ? n : 1) * 2 * 8);
+    uint64_t *sk = malloc((n ? n : 1) * 4 * 8);
     uint64_t *ord = malloc((n ? n : 1) * 8);
     for (uint64_t i = 0; i < n; ++i) {
-        u128 s = string_order_key(keys + i * kco_words_for_k(k), k);
-        sk[2 * i] = (uint64_t)(s >> 64);
-        sk[2 * i + 1] = (uint64_t)s;
+        string_order_key(keys + i * kco_words_for_k(k), k, sk + 4 * i);
         ord[i] = i;
     }
-    radix_sort_words(sk, ord, n, 2);
+    radix_sort_words(sk, ord, n, 4);
     uint64_t w = 0;
     char digits[16];
     for (uint64_t r = 0; r < n; ++r) {
-        u128 s = ((u128)sk[2 * r] << 64) | sk[2 * r + 1];
+        const uint64_t *s = sk + 4 * r;
         uint32_t c = counts[ord[r]];
         if (out) {
-            for (uint32_t j = 0; j < k; ++j) out[w + j] = B[(uint32_t)(s >> (2 * (k - 1 - j))) & 3];
+            for (uint32_t j = 0; j < k; ++j) {
+                uint32_t at = 2 * (k - 1 - j);
+                out[w + j] = B[(s[3 - at / 64] >> (at % 64)) & 3];
+            }
         }
         w += k;
         if (out) out[w] = '\t';

@@ -15,6 +15,9 @@ Outputs (all data, no reference source):
                            histogram_tool min/max ranges over two exports of it
                            (the plain split, and every key in low.bin with
                            count - 1 in high.bin), listed in tiny_counts.json
+  tiny_wide.*              k = 64..128 (kmer<2> and kmer<4>) views of reads
+                           from a small genome, listed in tiny_wide.json
+                           (dump text at k = 64 only)
   hash_kat.txt             hashes and Bloom probes of fixed keys
   sort_kat_<nw>_<bits>.npz records before/after the reference sorts
 """
@@ -135,6 +138,66 @@ def write_tiny():
             f.write(hist)
         cases.append(case)
     with open(os.path.join(HERE, "tiny_cases.json"), "w") as f:
+        json.dump(cases, f, indent=1)
+
+
+def tiny_wide_fastq():
+    """Reads for k = 64..128 (kmer<2> at 64, kmer<4> above): 240 reads of
+    150-260 bases from both strands of a 6 kb random genome (12x, so long
+    k-mers repeat), some lower case, some with an N or a U, and a few reads
+    shorter than k."""
+    rng = np.random.default_rng(20251204)
+    genome = "".join("ACGT"[i] for i in rng.integers(0, 4, 6000))
+    comp = {"A": "T", "C": "G", "G": "C", "T": "A"}
+    recs = []
+    for i in range(240):
+        n = int(rng.integers(150, 261))
+        at = int(rng.integers(0, len(genome) - n))
+        r = genome[at:at + n]
+        if rng.random() < 0.5:
+            r = "".join(comp[c] for c in reversed(r))
+        if i % 17 == 0:
+            j = int(rng.integers(0, n))
+            r = r[:j] + "N" + r[j + 1:]
+        if i % 23 == 0:
+            r = r.lower()
+        if i % 29 == 0:
+            r = r.replace("T", "U", 3)
+        recs.append(r)
+    recs += [genome[:60], genome[100:227], genome[300:428]]
+    return "".join("@w%d\n%s\n+\n%s\n" % (i, r, "I" * len(r)) for i, r in enumerate(recs))
+
+
+WIDE_CASES = [(64, 0xFFFFFFFF), (65, 0xFFFFFFFF), (96, 0xFFFFFFFF), (100, 255), (127, 0xFFFFFFFF),
+              (128, 0xFFFFFFFF)]
+
+
+def write_tiny_wide():
+    """tiny_wide.fastq and the reference views at k = 64..128 (tiny_wide.json).
+    k = 64: dump and histogram files.  k > 64: the histogram file; of the
+    dump only its size, line count, hash and first lines are kept, since the
+    reference's dump --sort does not spell k-mers longer than 64 bases (its
+    sort key holds 128 bits) and at k = 96 its counter splits counts."""
+    path = os.path.join(HERE, "tiny_wide.fastq")
+    with open(path, "w") as f:
+        f.write(tiny_wide_fastq())
+    cases = []
+    for k, cm in WIDE_CASES:
+        dump, hist, _ = ref_views(path, k, cm, hist_max=500)
+        base = "tiny_wide.k%d.c%d" % (k, cm)
+        case = {"input": "tiny_wide.fastq", "k": k, "count_max": cm, "hist_max": 500,
+                "hist": base + ".hist.txt", "dump_sha256": hashlib.sha256(dump).hexdigest(),
+                "dump_bytes": len(dump), "dump_lines": dump.count(b"\n")}
+        if k <= 64:
+            case["dump"] = base + ".dump.txt"
+            with open(os.path.join(HERE, case["dump"]), "wb") as f:
+                f.write(dump)
+        else:
+            case["dump_head"] = dump.decode().splitlines()[:8]
+        with open(os.path.join(HERE, case["hist"]), "wb") as f:
+            f.write(hist)
+        cases.append(case)
+    with open(os.path.join(HERE, "tiny_wide.json"), "w") as f:
         json.dump(cases, f, indent=1)
 
 
@@ -320,4 +383,5 @@ if __name__ == "__main__":
     write_count_5mbp()
     write_repeat()
     write_tiny_counts()
+    write_tiny_wide()
     print("goldens written to", HERE)

@@ -119,8 +119,11 @@ def test_plan_sizes_passes_and_budget():
         last = p["passes"]
     with pytest.raises(kc.KCError):  # even 256 passes do not fit
         kc.plan_sizes(31, 1 << 34, 1 << 30, bloom=True, hbm_budget=1 << 20)
+    # four-word keys (k = 64..128) take 64-byte slots
+    kw4, p4 = kc.plan_sizes(100, 1 << 20, 1 << 20)
+    assert p4["device_bytes"] == kw4["capacity"] * 64
     with pytest.raises(kc.KCError):
-        kc.plan_sizes(64, 1000, 10)
+        kc.plan_sizes(129, 1000, 10)
 
 
 def test_context_close_frees_its_buffers(monkeypatch):

@@ -156,7 +156,7 @@ def test_merge_rule_kat_low_frequency_query():
 def test_rolling_matches_closed_form():
     rng = np.random.default_rng(5)
     alphabet = np.frombuffer(b"ACGTacgtUNN\n", dtype=np.uint8)
-    for k in (1, 5, 15, 16, 21, 31, 32, 33, 47, 63, 64):
+    for k in (1, 5, 15, 16, 21, 31, 32, 33, 47, 63, 64, 65, 96, 100, 127, 128):
         s = alphabet[rng.integers(0, len(alphabet) - 3, 3000)]
         s[rng.integers(0, 3000, 20)] = ord("N")
         s = s.tobytes()
@@ -173,7 +173,7 @@ def test_rolling_matches_closed_form():
 def test_canonical_is_lexicographic_min():
     comp = {ord("A"): "T", ord("C"): "G", ord("G"): "C", ord("T"): "A"}
     rng = np.random.default_rng(6)
-    for k in (7, 31, 45, 63):
+    for k in (7, 31, 45, 63, 64, 65, 96, 127, 128):
         for _ in range(200):
             s = "".join("ACGT"[i] for i in rng.integers(0, 4, k))
             rc = "".join(comp[ord(c)] for c in reversed(s))
@@ -224,3 +224,85 @@ def test_tool_ranges_golden(split):
         lo, hi = map(int, r.split("-"))
         bins = O.histogram(hk, hc, lk, 1, g["count_max"], lo, hi)
         assert O.histogram_text(bins, lo, hi).decode() == want, r
+
+
+def _brute_force_views(fq: bytes, k: int, count_max: int, hist_max: int):
+    """dump --sort and histogram text of canonical k-mer counts found with
+    string operations only: every window of k bases of an ACGT run of a
+    sequence line (upper case, U read as T), canonical = the smaller of the
+    window and its reverse complement as strings (A < C < G < T is the order
+    of the 2-bit codes), counts clamped to count_max."""
+    from collections import Counter
+    comp = str.maketrans("ACGT", "TGCA")
+    counts = Counter()
+    lines = fq.decode().split("\n")[1::4]
+    for line in lines:
+        s = line.upper().replace("U", "T")
+        for run in "".join(c if c in "ACGT" else " " for c in s).split():
+            for i in range(len(run) - k + 1):
+                w = run[i:i + k]
+                counts[min(w, w.translate(comp)[::-1])] += 1
+    clamped = {w: min(c, count_max) for w, c in counts.items()}
+    dump = "".join("%s\t%d\n" % (w, clamped[w]) for w in sorted(clamped)).encode()
+    bins = np.zeros(hist_max, dtype=np.int64)
+    for c in clamped.values():
+        if c <= hist_max:
+            bins[c - 1] += 1
+    return dump, O.histogram_text(bins, 1, hist_max), clamped
+
+
+def _wide_cases():
+    return load_json("tiny_wide.json")
+
+
+@pytest.mark.parametrize("case", _wide_cases(), ids=lambda c: "k%d-c%d" % (c["k"], c["count_max"]))
+def test_wide_oracle_matches_brute_force(case):
+    """k = 64..128 (kmer<2> at 64, kmer<4> above): the oracle's dump and
+    histogram equal a count done on strings, and its keys survive the
+    high.bin packing (ExportWriter.h:296-313) both ways."""
+    fq = read_golden(case["input"])
+    k, cm = case["k"], case["count_max"]
+    dump, hist, _ = O.count_fastq_views(fq, k, cm, 1, case["hist_max"])
+    bdump, bhist, _ = _brute_force_views(fq, k, cm, case["hist_max"])
+    assert dump == bdump
+    assert hist == bhist
+    keys = O.extract(O.fastq_frame(fq), k)[:500]
+    assert np.array_equal(O.unpack(O.pack(keys, k), k), keys)
+
+
+@pytest.mark.parametrize("case", _wide_cases(), ids=lambda c: "k%d-c%d" % (c["k"], c["count_max"]))
+def test_wide_goldens(case):
+    """The reference's own outputs at k = 64..128.  k = 64: dump and
+    histogram byte for byte.  k > 64 the reference departs from the counts
+    in two ways, pinned here so the oracle's (correct) views are known to
+    differ only there:
+      - dump --sort spells only the last 64 bases of each k-mer: its sort
+        key is 128 bits wide (SortKeyType<4>, dump_tool.cpp:320-324,
+        346-353); line count, line lengths and counts are right;
+      - at k = 96 (kmer<4>, three whole words) the counter splits counts:
+        GetKmer.h:71-73 masks word (k-1)/32 only, so word 3 keeps bases
+        shifted out of word 2 and one k-mer is counted under several keys;
+        its histogram then differs as well."""
+    fq = read_golden(case["input"])
+    k, cm = case["k"], case["count_max"]
+    dump, hist, _ = O.count_fastq_views(fq, k, cm, 1, case["hist_max"])
+    lines = dump.decode().splitlines()
+    if k <= 64:
+        assert dump == read_golden(case["dump"])
+        assert hashlib.sha256(dump).hexdigest() == case["dump_sha256"]
+        assert hist == read_golden(case["hist"])
+        return
+    if k == 96:
+        assert case["dump_lines"] > len(lines)
+        assert hist != read_golden(case["hist"])
+        return
+    assert hist == read_golden(case["hist"])
+    assert case["dump_lines"] == len(lines)
+    assert case["dump_bytes"] == len(dump)
+    by_tail = {}
+    for ln in lines:
+        w, c = ln.split("\t")
+        by_tail.setdefault(w[-64:], set()).add(c)
+    for ln in case["dump_head"]:
+        w, c = ln.split("\t")
+        assert len(w) == k and c in by_tail.get(w[-64:], ()), ln
