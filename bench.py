@@ -662,7 +662,7 @@ def bench_passes(ctx, args, table, d_seq, S, k, bloom, est_distinct, K, bins, tu
     t = kc.Table(ctx, k, count_max=0xFFFFFFFF, capacity=cap, max_distinct=md, bloom=bloom,
                  low_capacity=md if bloom else 0)
     if tune:
-        t.set_tune(**{f: tune[f] for f in ("front_blocks", "fine_slices", "path", "piece_bytes", "first_mode")
+        t.set_tune(**{f: tune[f] for f in ("front_blocks", "fine_slices", "path", "piece_bytes", "first_mode", "front_mode")
                       if f in tune})
     bufs = {}
 

@@ -82,6 +82,12 @@ typedef struct kc_tune {
     uint32_t first_mode;    /* FIRST occurrences in the partition pass: 0 = by key width (staged
                                for k <= 32, appended above), 1 = staged per wave and moved to the
                                list at the end, 2 = appended per wave step */
+    uint32_t front_mode;    /* how the partitioned pass places keys in its coarse regions:
+                               0 = auto (speculative on inputs of 32 MB or more), 1 = exact (a
+                               count pass over the input sizes every region first), 2 = speculative
+                               whenever the pass has two scatter levels (regions of a fixed
+                               capacity; a region that overflows makes the same launch sequence
+                               redo the front end exactly) */
 } kc_tune;
 
 const char *kc_strerror(int status);
@@ -165,6 +171,11 @@ int kc_table_query(kc_table *t, const uint64_t *d_keys, uint64_t n,
 int kc_table_query_on(kc_table *t, kc_ctx *ctx, const uint64_t *d_keys, uint64_t n,
                       uint32_t *d_counts);
 int kc_table_stats(kc_table *t, kc_stats *out);
+/* How many partitioned counting passes since the table was created or last
+ * cleared had a speculative coarse region overflow and redid their front end
+ * exactly (kc_tune.front_mode).  No reference counterpart: the reference's
+ * tree grows its nodes on demand (NewKmerTree.h:600-614). */
+int kc_table_front_redos(kc_table *t, uint64_t *out);
 
 /* Multi-pass counting for tables larger than HBM: the k-mers are split into
  * n_passes hash slices (n_passes a power of two <= 256; slice = top

@@ -69,7 +69,7 @@ def tune_shape(table: kc.Table, d_sample, nbytes=None, reps=1, use_cache=True):
         hit = _load().get(key)
         if hit:
             table.set_tune(**{f: hit[f] for f in ("front_blocks", "fine_slices", "batch_bytes", "queue_depth", "path",
-                                                  "piece_bytes", "first_mode") if f in hit})
+                                                  "piece_bytes", "first_mode", "front_mode") if f in hit})
             return dict(hit, cached=True)
     tune, ms = table.autotune(d_sample, n, reps=reps)
     rec = dict(tune, ms=ms)

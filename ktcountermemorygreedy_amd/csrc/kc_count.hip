@@ -1212,6 +1212,16 @@ extern "C" int kc_table_stats(kc_table *t, kc_stats *out)
     return KC_OK;
 }
 
+extern "C" int kc_table_front_redos(kc_table *t, uint64_t *out)
+{
+    if (!t || !out) return KC_EINVAL;
+    HostCounters hc;
+    int rc = read_counters(t, hc);
+    if (rc != KC_OK) return rc;
+    *out = hc.stat[KC_STAT_FRONT_REDOS];
+    return KC_OK;
+}
+
 extern "C" int kc_table_export_device(kc_table *t, uint8_t *d_high, uint64_t high_cap, uint64_t *high_bytes,
                                       uint8_t *d_low, uint64_t low_cap, uint64_t *low_bytes)
 {

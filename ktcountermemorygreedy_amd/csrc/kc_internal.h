@@ -43,7 +43,8 @@ enum kc_stat_idx : int {
     KC_STAT_KMERS = 0,
     KC_STAT_DISTINCT = 1,
     KC_STAT_ERRORS = 2,
-    KC_STAT_N = 3,
+    KC_STAT_FRONT_REDOS = 3,  // speculative front ends redone exactly (kc_table_front_redos)
+    KC_STAT_N = 4,
 };
 constexpr int KC_LANES = 64;
 constexpr int KC_LINE_WORDS = 16;
@@ -61,17 +62,20 @@ struct kc_pipe_shape {
     uint32_t fine_slices;   // blocks per coarse region in the second scatter level
     uint64_t key_cap;
     uint32_t first_mode;    // partition pass FIRST hand-over: 1 staged, 2 per wave step, 3 per wave group of steps
+    uint32_t front_mode;    // kc_tune.front_mode: 0 auto, 1 exact, 2 speculative
 };
 
 // Device scratch of the partitioned counting pass, sized for a shape.
 struct kc_part_scratch {
-    uint64_t *keys_a = nullptr, *keys_b = nullptr;  // key_cap * nw words each
+    uint64_t *keys_a = nullptr, *keys_b = nullptr;  // front_cap / key_cap keys of nw words
     uint32_t *m1 = nullptr, *m2 = nullptr;          // digit counts per block / slice
     uint64_t *o1 = nullptr, *o2 = nullptr;          // their exclusive offsets
     uint64_t *sums = nullptr, *starts = nullptr, *part_off = nullptr;
     uint8_t *dig = nullptr;                         // each key's fine digit (u8, or u16 above 256 digits)
-    uint64_t *gcur = nullptr;                       // front scatter: [coarse][group] cursors, then region ends
+    uint64_t *gcur = nullptr;                       // front scatter: [coarse][group] cursors, region ends,
+                                                    // region starts, then the speculative overflow flag
     uint64_t key_cap = 0;
+    uint64_t front_cap = 0;  // keys of keys_a (and digits): key_cap plus the speculative regions' slack
     uint64_t m1_n = 0, m2_n = 0, sums_n = 0, part_n = 0, dig_bytes = 0;
 };
 

@@ -64,7 +64,24 @@ constexpr int kFrontChunk2 = 16;
 // them; with a region per block, 64 blocks x 512 open runs filled the 4 MB
 // L2 of an XCD and partial lines reached memory.
 constexpr uint32_t kFrontGroups = 8;
-constexpr uint64_t kGroupWords = 2ull * 256 * kFrontGroups;  // cursors, then region ends
+// Group words: [coarse][group] cursors, region ends, region starts, then the
+// overflow flag of a speculative scatter.
+constexpr uint64_t kGcurEnd = 256ull * kFrontGroups;
+constexpr uint64_t kGcurBase = 2ull * 256 * kFrontGroups;
+constexpr uint64_t kGcurFlag = 3ull * 256 * kFrontGroups;
+constexpr uint64_t kGroupWords = kGcurFlag + 1;
+// Speculative front end (kc_tune.front_mode): every (coarse digit, group)
+// region gets the same capacity, a share of key arrays kSpecSlackDiv-th
+// larger than the input's window bound, and the scatter runs without a count
+// pass; a run that does not fit raises the overflow flag, and the exact front
+// end (count pass, scan, region placement, scatter) that follows it in the
+// same launch sequence runs only then.  Hash digits load the regions evenly
+// (Poisson: ~0.1% spread at 1M keys per region), so the redo is for skewed
+// input (a few keys repeated all over).  Auto mode speculates from
+// kSpecMinRegion keys per region (inputs of 32 MB or more).
+constexpr uint64_t kSpecSlackDiv = 8;  // capacity slack: key_cap / 8
+constexpr uint64_t kSpecAlign = 128;   // region capacity in whole lines of keys and digits
+constexpr uint64_t kSpecMinRegion = 1u << 14;
 
 struct Digits {
     uint32_t part_bits;  // L
@@ -158,13 +175,21 @@ __device__ __forceinline__ uint32_t scan_block(uint32_t v, uint32_t *wsum, uint3
     return wbase + x - v;
 }
 
+// The exact front end after a speculative scatter runs only when that
+// scatter overflowed (its flag, `gate`, is nonzero); a null gate always runs.
+// The flag is written by an earlier kernel of the stream and not changed
+// while these run, so every thread of a launch decides alike.
+__device__ __forceinline__ bool gated_off(const unsigned long long *gate) { return gate && *gate == 0; }
+
 // ---- front end over framed bytes (persistent: block b owns tiles b + iG) --
 template <int NW, bool PASS>
 __global__ __launch_bounds__(kParseBlock) void k_front_count_seq(SeqSrc src, uint32_t k, Digits d,
-                                                                 uint32_t *__restrict__ m1)
+                                                                 uint32_t *__restrict__ m1,
+                                                                 const unsigned long long *gate)
 {
     __shared__ ParseLds lds;
     __shared__ uint32_t hist[256];
+    if (gated_off(gate)) return;
     uint64_t n, emit_from;
     seq_bounds(src, n, emit_from);
     hist[threadIdx.x] = 0;
@@ -205,11 +230,16 @@ __device__ __forceinline__ void guard_report(unsigned long long *counters, uint3
 // digit and chunk).  A tile is TB x 64 bytes, the tiles of
 // k_front_count_seq at TB = 256, so the group regions of the count pass hold
 // exactly the keys of their blocks.
+// Speculative (overflow non-null): the regions have a fixed capacity; a run
+// that does not fit raises *overflow instead of counting an error, and the
+// exact front end that follows redoes the placement.  gate: see gated_off.
 template <int NW, int TB, int CB, bool PASS>
 __global__ __launch_bounds__(TB) void k_front_scatter_seq_staged(SeqSrc src, uint32_t k, Digits d,
-                                                                 unsigned long long *__restrict__ gcur,
+                                                                 unsigned long long *gcur,
                                                                  uint64_t *__restrict__ out, uint8_t *__restrict__ dig,
-                                                                 unsigned long long *counters)
+                                                                 unsigned long long *counters,
+                                                                 const unsigned long long *gate,
+                                                                 unsigned long long *overflow)
 {
     constexpr int HW = NW == 1 ? 8 : 16;
     constexpr int kChunks = kParseSeg / CB;
@@ -221,6 +251,7 @@ __global__ __launch_bounds__(TB) void k_front_scatter_seq_staged(SeqSrc src, uin
     __shared__ uint32_t hist[256], loff[256];
     __shared__ uint64_t delta[256];
     uint32_t *wsum = loff + 256 - TB / 64;  // the scan's wave sums: loff is written only after the scan
+    if (gated_off(gate)) return;
     const uint32_t t = threadIdx.x;
     const uint32_t nbins = 1u << d.l1;
     uint32_t n_err = 0;
@@ -228,7 +259,7 @@ __global__ __launch_bounds__(TB) void k_front_scatter_seq_staged(SeqSrc src, uin
     seq_bounds(src, n, emit_from);
     // thread t's digit: its group's cursor and region end
     const uint32_t gslot = t * kFrontGroups + blockIdx.x % kFrontGroups;
-    const uint64_t my_end = t < nbins ? gcur[256 * kFrontGroups + gslot] : 0;
+    const uint64_t my_end = t < nbins ? gcur[kGcurEnd + gslot] : 0;
     hist[t] = 0;
     __syncthreads();
     constexpr uint64_t kTileBytes = (uint64_t)TB * kParseSeg;
@@ -309,33 +340,70 @@ __global__ __launch_bounds__(TB) void k_front_scatter_seq_staged(SeqSrc src, uin
             __syncthreads();
         }
     }
-    guard_report(counters, n_err);
+    if (!overflow) guard_report(counters, n_err);
+    else if (n_err) atomicOr(overflow, 1ull);
 }
 
 // Group cursors of the front scatter: for coarse digit c and group x the
 // region [o1[c][x][0], o1[c][x + 1][0]) of the scan (the next group's start,
-// or the next digit's, or the total), cursors first, region ends after them.
+// or the next digit's, or the total): cursors, region ends, region starts.
+// Under a gate (the redo of a speculative front end) it counts the redo.
 __global__ void k_group_cursors(const uint64_t *o1, const uint32_t *m1, uint32_t G, uint32_t n_coarse,
-                                unsigned long long *gcur)
+                                unsigned long long *gcur, const unsigned long long *gate,
+                                unsigned long long *counters)
 {
+    if (gated_off(gate)) return;
     const uint32_t c = threadIdx.x;
+    if (gate) stat_add(counters, KC_STAT_FRONT_REDOS, c == 0 ? 1u : 0u);  // whole waves: stat_add reduces over the wave
     if (c >= n_coarse) return;
     const uint32_t J = G / kFrontGroups;
     const uint64_t last = (uint64_t)n_coarse * G - 1;
     for (uint32_t x = 0; x < kFrontGroups; ++x) {
         const uint64_t at = (uint64_t)c * G + (uint64_t)x * J, next = at + J;
         gcur[c * kFrontGroups + x] = o1[at];
-        gcur[256 * kFrontGroups + c * kFrontGroups + x] = next <= last ? o1[next] : o1[last] + m1[last];
+        gcur[kGcurBase + c * kFrontGroups + x] = o1[at];
+        gcur[kGcurEnd + c * kFrontGroups + x] = next <= last ? o1[next] : o1[last] + m1[last];
+    }
+}
+
+// Regions of a speculative front scatter: region i = (coarse c, group x),
+// i = c * kFrontGroups + x, holds [i * cap, (i + 1) * cap); the overflow flag
+// is cleared.
+__global__ void k_spec_regions(unsigned long long *gcur, uint32_t n_coarse, uint64_t cap)
+{
+    for (uint32_t i = threadIdx.x; i < n_coarse * kFrontGroups; i += blockDim.x) {
+        const uint64_t b = (uint64_t)i * cap;
+        gcur[i] = b;
+        gcur[kGcurBase + i] = b;
+        gcur[kGcurEnd + i] = b + cap;
+    }
+    if (threadIdx.x == 0) gcur[kGcurFlag] = 0;
+}
+
+// Regions of the key-input front end (one contiguous region per coarse
+// digit, from the coarse starts) in the group words the fine level reads:
+// group 0 holds the region, the other groups are empty.
+__global__ void k_group_from_starts(const uint64_t *starts, uint32_t n_coarse, unsigned long long *gcur)
+{
+    const uint32_t c = threadIdx.x;
+    if (c >= n_coarse) return;
+    for (uint32_t x = 0; x < kFrontGroups; ++x) {
+        const uint32_t i = c * kFrontGroups + x;
+        gcur[kGcurBase + i] = x ? starts[c + 1] : starts[c];
+        gcur[i] = starts[c + 1];
+        gcur[kGcurEnd + i] = starts[c + 1];
     }
 }
 
 // Scatter guard of the grouped front scatter: a region whose cursor did not
 // reach its end holds keys the count pass saw and the scatter did not.
-__global__ void k_group_check(const unsigned long long *gcur, uint32_t n_coarse, unsigned long long *counters)
+__global__ void k_group_check(const unsigned long long *gcur, uint32_t n_coarse, unsigned long long *counters,
+                              const unsigned long long *gate)
 {
+    if (gated_off(gate)) return;
     uint32_t n_err = 0;
     for (uint32_t i = threadIdx.x; i < n_coarse * kFrontGroups; i += blockDim.x)
-        n_err += gcur[i] != gcur[256 * kFrontGroups + i];
+        n_err += gcur[i] != gcur[kGcurEnd + i];
     guard_report(counters, n_err);
 }
 
@@ -412,8 +480,9 @@ __global__ void k_skew_count(uint32_t *m, uint64_t n, int delta)
 
 // Coarse region starts: o1[c * G] for every coarse digit, plus the total.
 __global__ void k_coarse_starts(const uint64_t *o1, const uint32_t *m1, uint32_t G, uint32_t n_coarse,
-                                uint64_t *starts)
+                                uint64_t *starts, const unsigned long long *gate)
 {
+    if (gated_off(gate)) return;
     const uint32_t c = threadIdx.x;
     if (c < n_coarse) starts[c] = o1[(uint64_t)c * G];
     if (c == 0) {
@@ -423,58 +492,84 @@ __global__ void k_coarse_starts(const uint64_t *o1, const uint32_t *m1, uint32_t
 }
 
 // ---- second level: fine digit inside each coarse region -----------------
-// Block b works on slice s = b % S of coarse region c = b / S: the keys
-// [start_c + len_c*s/S, start_c + len_c*(s+1)/S).  Its counts go to
+// The keys of coarse region c are its group regions [start, min(cursor, end))
+// in group order: back to back when the count pass placed them, a fixed
+// capacity apart when the scatter speculated.  Block b works on slice
+// s = b % S of coarse region c = b / S: the part [L*s/S, L*(s+1)/S) of their
+// concatenation (L keys in all), returned as one piece [rb[x], re[x]) per
+// group region in LDS (empty pieces are [0, 0)).  Its counts go to
 // m2[((c << L2) + f) * S + s], so that the scan lays partition (c, f) out as
 // slices 0..S-1 in order.
-__device__ __forceinline__ void slice_of(const uint64_t *starts, uint32_t S, uint32_t &c, uint32_t &s, uint64_t &beg,
-                                         uint64_t &end)
+__device__ __forceinline__ void slice_pieces(const unsigned long long *gcur, uint32_t S, uint32_t &c, uint32_t &s,
+                                             uint64_t *rb, uint64_t *re)
 {
     c = blockIdx.x / S;
     s = blockIdx.x % S;
-    const uint64_t b0 = starts[c], len = starts[c + 1] - b0;
-    beg = b0 + len * s / S;
-    end = b0 + len * (s + 1) / S;
+    if (threadIdx.x == 0) {
+        uint64_t b[kFrontGroups], l[kFrontGroups], L = 0;
+#pragma unroll
+        for (uint32_t x = 0; x < kFrontGroups; ++x) {
+            const uint64_t i = (uint64_t)c * kFrontGroups + x;
+            const uint64_t e = gcur[i] < gcur[kGcurEnd + i] ? gcur[i] : gcur[kGcurEnd + i];
+            b[x] = gcur[kGcurBase + i];
+            l[x] = e > b[x] ? e - b[x] : 0;
+            L += l[x];
+        }
+        const uint64_t lo = L * s / S, hi = L * (s + 1) / S;
+        uint64_t acc = 0;
+#pragma unroll
+        for (uint32_t x = 0; x < kFrontGroups; ++x) {
+            const uint64_t a = lo > acc ? lo : acc, z = hi < acc + l[x] ? hi : acc + l[x];
+            rb[x] = a < z ? b[x] + (a - acc) : 0;
+            re[x] = a < z ? b[x] + (z - acc) : 0;
+            acc += l[x];
+        }
+    }
+    __syncthreads();
 }
 
 // Fine count over the digits the front scatter wrote: 16-byte loads (16 u8
 // or 8 u16 digits), U of them per thread in flight, LDS histogram.
 template <typename DT>
 __global__ __launch_bounds__(kKeyBlock) void k_fine_count_dig(const DT *__restrict__ dig,
-                                                              const uint64_t *__restrict__ starts, Digits d,
+                                                              const unsigned long long *__restrict__ gcur, Digits d,
                                                               uint32_t S, uint32_t *__restrict__ m2)
 {
     __shared__ uint32_t hist[1024];
+    __shared__ uint64_t rb[kFrontGroups], re[kFrontGroups];
     uint32_t c, s;
-    uint64_t beg, end;
-    slice_of(starts, S, c, s, beg, end);
+    slice_pieces(gcur, S, c, s, rb, re);
     const uint32_t nf = 1u << d.l2, fmask = nf - 1;  // the mask keeps any stored value inside hist
     for (uint32_t f = threadIdx.x; f < nf; f += kKeyBlock) hist[f] = 0;
     __syncthreads();
     constexpr uint32_t PER = 16 / sizeof(DT);  // digits per 16-byte load
-    const uint64_t a0 = (beg + PER - 1) / PER * PER < end ? (beg + PER - 1) / PER * PER : end;
-    const uint64_t a1 = end / PER * PER > a0 ? end / PER * PER : a0;
-    for (uint64_t i = beg + threadIdx.x; i < a0; i += kKeyBlock) atomicAdd(&hist[dig[i] & fmask], 1u);
-    for (uint64_t i = a1 + threadIdx.x; i < end; i += kKeyBlock) atomicAdd(&hist[dig[i] & fmask], 1u);
-    const uint4 *v = reinterpret_cast<const uint4 *>(dig);
-    const uint64_t q0 = a0 / PER, q1 = a1 / PER;
-    constexpr uint32_t U = 4;
-    for (uint64_t qb = q0; qb < q1; qb += (uint64_t)U * kKeyBlock) {
-        uint4 x[U];
+    for (uint32_t g = 0; g < kFrontGroups; ++g) {
+        const uint64_t beg = rb[g], end = re[g];
+        if (beg >= end) continue;
+        const uint64_t a0 = (beg + PER - 1) / PER * PER < end ? (beg + PER - 1) / PER * PER : end;
+        const uint64_t a1 = end / PER * PER > a0 ? end / PER * PER : a0;
+        for (uint64_t i = beg + threadIdx.x; i < a0; i += kKeyBlock) atomicAdd(&hist[dig[i] & fmask], 1u);
+        for (uint64_t i = a1 + threadIdx.x; i < end; i += kKeyBlock) atomicAdd(&hist[dig[i] & fmask], 1u);
+        const uint4 *v = reinterpret_cast<const uint4 *>(dig);
+        const uint64_t q0 = a0 / PER, q1 = a1 / PER;
+        constexpr uint32_t U = 4;
+        for (uint64_t qb = q0; qb < q1; qb += (uint64_t)U * kKeyBlock) {
+            uint4 x[U];
 #pragma unroll
-        for (uint32_t u = 0; u < U; ++u) {
-            const uint64_t q = qb + u * kKeyBlock + threadIdx.x;
-            x[u] = v[q < q1 ? q : q0];
-        }
+            for (uint32_t u = 0; u < U; ++u) {
+                const uint64_t q = qb + u * kKeyBlock + threadIdx.x;
+                x[u] = v[q < q1 ? q : q0];
+            }
 #pragma unroll
-        for (uint32_t u = 0; u < U; ++u) {
-            if (qb + u * kKeyBlock + threadIdx.x >= q1) continue;
-            const uint32_t w[4] = {x[u].x, x[u].y, x[u].z, x[u].w};
+            for (uint32_t u = 0; u < U; ++u) {
+                if (qb + u * kKeyBlock + threadIdx.x >= q1) continue;
+                const uint32_t w[4] = {x[u].x, x[u].y, x[u].z, x[u].w};
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
+                for (int j = 0; j < 4; ++j) {
 #pragma unroll
-                for (uint32_t b = 0; b < 4 / sizeof(DT); ++b)
-                    atomicAdd(&hist[(w[j] >> (8 * sizeof(DT) * b)) & fmask], 1u);
+                    for (uint32_t b = 0; b < 4 / sizeof(DT); ++b)
+                        atomicAdd(&hist[(w[j] >> (8 * sizeof(DT) * b)) & fmask], 1u);
+                }
             }
         }
     }
@@ -495,7 +590,7 @@ __global__ __launch_bounds__(kKeyBlock) void k_fine_count_dig(const DT *__restri
 // larger chunks.
 template <int NW, int NF, int TB, int FPT, bool SB = true>
 __global__ __launch_bounds__(TB) void k_fine_scatter_staged(const uint64_t *__restrict__ keys,
-                                                            const uint64_t *__restrict__ starts, Digits d,
+                                                            const unsigned long long *__restrict__ gcur, Digits d,
                                                             uint32_t S, const uint64_t *__restrict__ o2,
                                                             const uint32_t *__restrict__ m2,
                                                             uint64_t *__restrict__ out,
@@ -508,10 +603,10 @@ __global__ __launch_bounds__(TB) void k_fine_scatter_staged(const uint64_t *__re
     __shared__ BinT sbin[SB ? CH : 1];
     __shared__ uint32_t hist[NF], loff[NF];
     __shared__ uint64_t delta[NF];
+    __shared__ uint64_t rb[kFrontGroups], re[kFrontGroups];
     uint32_t *tmp = loff + NF - TB / 64;  // the scan's wave sums: loff is rewritten only after the scan
     uint32_t c, s;
-    uint64_t beg, end;
-    slice_of(starts, S, c, s, beg, end);
+    slice_pieces(gcur, S, c, s, rb, re);
     const uint32_t nf = 1u << d.l2;
     const uint32_t t = threadIdx.x;
     const uint64_t row = (uint64_t)c << d.l2;
@@ -524,7 +619,7 @@ __global__ __launch_bounds__(TB) void k_fine_scatter_staged(const uint64_t *__re
     __syncthreads();
     // a chunk's FPT keys per thread (index clamped into the chunk)
     uint64_t key[FPT][NW];
-    auto load_chunk = [&](uint64_t cs) {
+    auto load_chunk = [&](uint64_t cs, uint64_t end) {
         const uint32_t cn = (uint32_t)(end - cs < CH ? end - cs : CH);
 #pragma unroll
         for (uint32_t j = 0; j < (uint32_t)FPT; ++j) {
@@ -539,8 +634,12 @@ __global__ __launch_bounds__(TB) void k_fine_scatter_staged(const uint64_t *__re
             }
         }
     };
-    if (beg < end) load_chunk(beg);
-    for (uint64_t cs = beg; cs < end; cs += CH) {
+    // chunks of the slice's pieces in order: [cs, end) of piece x
+    uint32_t x = 0;
+    while (x < kFrontGroups && rb[x] >= re[x]) ++x;
+    uint64_t cs = x < kFrontGroups ? rb[x] : 0, end = x < kFrontGroups ? re[x] : 0;
+    if (x < kFrontGroups) load_chunk(cs, end);
+    while (x < kFrontGroups) {
         const uint32_t cn = (uint32_t)(end - cs < CH ? end - cs : CH);
         uint32_t bin[FPT], rank[FPT];
 #pragma unroll
@@ -574,9 +673,18 @@ __global__ __launch_bounds__(TB) void k_fine_scatter_staged(const uint64_t *__re
                 if constexpr (SB) sbin[p] = (BinT)bin[j];
             }
         }
-        // the next chunk's loads stay in flight through the write-out below
-        // (the barriers wait for LDS operations only)
-        if (cs + CH < end) load_chunk(cs + CH);
+        // the next chunk (the rest of this piece, else the next piece's
+        // first): its loads stay in flight through the write-out below (the
+        // barriers wait for LDS operations only)
+        uint32_t nx = x;
+        uint64_t ncs = cs + CH, nend = end;
+        if (ncs >= end) {
+            ++nx;
+            while (nx < kFrontGroups && rb[nx] >= re[nx]) ++nx;
+            ncs = nx < kFrontGroups ? rb[nx] : 0;
+            nend = nx < kFrontGroups ? re[nx] : 0;
+        }
+        if (nx < kFrontGroups) load_chunk(ncs, nend);
         __syncthreads();
         for (uint32_t p = t; p < cn; p += TB) {
             uint32_t f;
@@ -598,19 +706,24 @@ __global__ __launch_bounds__(TB) void k_fine_scatter_staged(const uint64_t *__re
             hist[t] = 0;
         }
         __syncthreads();
+        x = nx;
+        cs = ncs;
+        end = nend;
     }
     n_err += mine && my_left != 0;  // keys the count pass saw and this one did not
     guard_report(counters, n_err);
 }
 
 // Partition p starts at o2[p * S] (two levels) or at the coarse start p (one
-// level); part_off[2^L] = total keys.
-__global__ void k_part_offsets(const uint64_t *o2, const uint64_t *starts, Digits d, uint32_t S, uint64_t *part_off)
+// level); part_off[2^L] = total keys (the fine scan's total, or the last
+// coarse start).
+__global__ void k_part_offsets(const uint64_t *o2, const uint32_t *m2, const uint64_t *starts, Digits d, uint32_t S,
+                               uint64_t *part_off)
 {
     const uint64_t n_parts = 1ull << d.part_bits;
     for (uint64_t p = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; p <= n_parts;
          p += (uint64_t)gridDim.x * blockDim.x) {
-        if (p == n_parts) part_off[p] = starts[1u << d.l1];
+        if (p == n_parts) part_off[p] = d.l2 ? o2[n_parts * S - 1] + m2[n_parts * S - 1] : starts[1u << d.l1];
         else part_off[p] = d.l2 ? o2[p * S] : starts[p];
     }
 }
@@ -972,7 +1085,28 @@ kc_pipe_shape kc_part_shape(const kc_table *t, const kc_tune &tune, uint64_t key
     sh.fine_slices = std::max<uint32_t>(1, std::min<uint32_t>(tune.fine_slices ? tune.fine_slices : 8, 64));
     sh.key_cap = key_cap;
     sh.first_mode = tune.first_mode >= 1 && tune.first_mode <= 3 ? tune.first_mode : kFirstModeDefault;
+    sh.front_mode = tune.front_mode <= 2 ? tune.front_mode : 0;
     return sh;
+}
+
+// Keys the first key array (and the digit array) holds: key_cap plus the
+// speculative regions' slack and one alignment unit per region.
+static uint64_t front_cap_for(uint64_t key_cap)
+{
+    const uint64_t kc = std::max<uint64_t>(key_cap, 1);
+    return kc + kc / kSpecSlackDiv + 256 * kFrontGroups * kSpecAlign;
+}
+
+// Capacity of each speculative region for a front key array of front_cap
+// keys (whole lines), or 0 when this pass places its keys exactly: one
+// scatter level, the count-skew test hook, front_mode 1, or (auto) regions
+// smaller than kSpecMinRegion keys.
+static uint64_t spec_region_cap(const kc_table *t, const kc_pipe_shape &sh, const Digits &d, uint64_t front_cap)
+{
+    if (!d.l2 || sh.front_mode == 1 || t->fault_skew == 1 || t->fault_skew == 2) return 0;
+    const uint64_t cap = front_cap / ((uint64_t)kFrontGroups << d.l1) / kSpecAlign * kSpecAlign;
+    if (cap == 0 || (sh.front_mode == 0 && cap < kSpecMinRegion)) return 0;
+    return cap;
 }
 
 // Sizes the scratch for a shape (reallocating what is too small).
@@ -993,14 +1127,15 @@ int kc_part_reserve(const kc_table *t, kc_part_scratch &s, const kc_pipe_shape &
         have = need;
         return true;
     };
-    uint64_t kc_a = s.key_cap, kc_b = s.key_cap, m1c = s.m1_n, o1c = s.m1_n, m2c = s.m2_n, o2c = s.m2_n;
+    uint64_t kc_a = s.front_cap, kc_b = s.key_cap, m1c = s.m1_n, o1c = s.m1_n, m2c = s.m2_n, o2c = s.m2_n;
     uint64_t sc = s.sums_n, pc = s.part_n, stc = s.starts ? 257 : 0, dgc = s.dig_bytes, gcc = s.gcur ? kGroupWords : 0;
     const uint64_t keys_words = std::max<uint64_t>(sh.key_cap, 1) * nw;
+    const uint64_t front_words = front_cap_for(sh.key_cap) * nw;
     // the digit array (+16: the fine count's last 16-byte load may reach past the final digit)
-    const uint64_t dig_need = d.l2 ? std::max<uint64_t>(sh.key_cap, 1) * (d.dig16 ? 2 : 1) + 16 : 0;
+    const uint64_t dig_need = d.l2 ? front_cap_for(sh.key_cap) * (d.dig16 ? 2 : 1) + 16 : 0;
     kc_a *= nw;
     kc_b *= nw;
-    const bool ok = grow(s.keys_a, kc_a, keys_words) && grow(s.keys_b, kc_b, keys_words) &&
+    const bool ok = grow(s.keys_a, kc_a, front_words) && grow(s.keys_b, kc_b, keys_words) &&
                     grow(s.m1, m1c, m1n) && grow(s.o1, o1c, m1n) && (d.l2 == 0 || grow(s.m2, m2c, m2n)) &&
                     (d.l2 == 0 || grow(s.o2, o2c, m2n)) && grow(s.sums, sc, sumsn) &&
                     grow(s.part_off, pc, n_parts + 1) && grow(s.starts, stc, 257) && grow(s.gcur, gcc, kGroupWords) &&
@@ -1010,6 +1145,7 @@ int kc_part_reserve(const kc_table *t, kc_part_scratch &s, const kc_pipe_shape &
         return KC_ENOMEM;
     }
     s.key_cap = std::min(kc_a, kc_b) / nw;
+    s.front_cap = kc_a / nw;
     s.m1_n = std::min(m1c, o1c);
     s.m2_n = d.l2 ? std::min(m2c, o2c) : s.m2_n;
     s.sums_n = sc;
@@ -1031,20 +1167,38 @@ int kc_part_enqueue(const kc_table *t, kc_part_scratch &s, const kc_pipe_shape &
     // the shape the scratch was sized for must cover this launch
     const uint64_t m1n = (uint64_t)n_coarse * G, m2n = d.l2 ? n_parts * S : 0;
     const uint64_t need_keys = from_seq ? seq.n - std::min(seq.n, seq.hdr ? 0 : seq.emit_from) : n_keys;
-    const uint64_t dig_need = d.l2 ? s.key_cap * (d.dig16 ? 2 : 1) + 16 : 0;
-    if (s.key_cap < need_keys || s.key_cap < sh.key_cap || s.m1_n < m1n || s.m2_n < m2n ||
+    const uint64_t dig_need = d.l2 ? s.front_cap * (d.dig16 ? 2 : 1) + 16 : 0;
+    if (s.front_cap < s.key_cap || s.key_cap < need_keys || s.key_cap < sh.key_cap || s.m1_n < m1n || s.m2_n < m2n ||
         s.sums_n < kc_scan_tmp_words(std::max(m1n, m2n)) || s.part_n < n_parts + 1 || !s.starts || !s.gcur ||
         (d.l2 && (!s.dig || s.dig_bytes < dig_need)))
         return KC_EINVAL;
     uint8_t *dig = d.l2 ? s.dig : nullptr;
+    auto *gcur = reinterpret_cast<unsigned long long *>(s.gcur);
+    // the pass filter (multi-pass counting) is compiled in only where it is used
+    const bool pass = d.pass_bits != 0;
+    auto f1 = pass ? &k_front_scatter_seq_staged<1, kFrontBlock1, kFrontChunk1, true>
+                   : &k_front_scatter_seq_staged<1, kFrontBlock1, kFrontChunk1, false>;
+    auto f2 = pass ? &k_front_scatter_seq_staged<2, kFrontBlock2, kFrontChunk2, true>
+                   : &k_front_scatter_seq_staged<2, kFrontBlock2, kFrontChunk2, false>;
 
-    // level 1: count, scan, coarse starts, scatter
+    // level 1, speculative: fixed-capacity regions, scatter without a count
+    // pass; the exact front end below then runs only if a region overflowed
+    const uint64_t spec_cap = from_seq ? spec_region_cap(t, sh, d, s.front_cap) : 0;
+    const unsigned long long *gate = nullptr;
+    if (spec_cap) {
+        hipLaunchKernelGGL(k_spec_regions, dim3(1), dim3(256), 0, st, gcur, n_coarse, spec_cap);
+        unsigned long long *flag = gcur + kGcurFlag;
+        if (nw == 1) hipLaunchKernelGGL(f1, dim3(G), dim3(kFrontBlock1), 0, st, seq, t->cfg.k, d, gcur, s.keys_a, dig, tv.counters, nullptr, flag);
+        else hipLaunchKernelGGL(f2, dim3(G), dim3(kFrontBlock2), 0, st, seq, t->cfg.k, d, gcur, s.keys_a, dig, tv.counters, nullptr, flag);
+        KC_HIP(hipGetLastError());
+        gate = flag;
+    }
+
+    // level 1, exact: count, scan, coarse starts, scatter
     if (from_seq) {
-        // the pass filter (multi-pass counting) is compiled in only where it is used
-        const bool pass = d.pass_bits != 0;
         auto fc = nw == 1 ? (pass ? &k_front_count_seq<1, true> : &k_front_count_seq<1, false>)
                           : (pass ? &k_front_count_seq<2, true> : &k_front_count_seq<2, false>);
-        hipLaunchKernelGGL(fc, dim3(G), dim3(kParseBlock), 0, st, seq, t->cfg.k, d, s.m1);
+        hipLaunchKernelGGL(fc, dim3(G), dim3(kParseBlock), 0, st, seq, t->cfg.k, d, s.m1, gate);
     } else {
         if (nw == 1) hipLaunchKernelGGL(k_front_count_keys<1>, dim3(G), dim3(kKeyBlock), 0, st, d_keys, n_keys, d, s.m1);
         else hipLaunchKernelGGL(k_front_count_keys<2>, dim3(G), dim3(kKeyBlock), 0, st, d_keys, n_keys, d, s.m1);
@@ -1053,22 +1207,19 @@ int kc_part_enqueue(const kc_table *t, kc_part_scratch &s, const kc_pipe_shape &
     if (t->fault_skew == 1 || t->fault_skew == 2)
         hipLaunchKernelGGL(k_skew_count, dim3(1), dim3(64), 0, st, s.m1, m1n, t->fault_skew == 1 ? -1 : 1);
     int rc;
+    // (the scan is not gated: after a speculative scatter that fit it scans
+    // counts nothing reads)
     if ((rc = kc_scan_u32_on(st, s.m1, m1n, s.o1, s.sums))) return rc;
-    hipLaunchKernelGGL(k_coarse_starts, dim3(1), dim3(256), 0, st, s.o1, s.m1, G, n_coarse, s.starts);
+    hipLaunchKernelGGL(k_coarse_starts, dim3(1), dim3(256), 0, st, s.o1, s.m1, G, n_coarse, s.starts, gate);
     if (from_seq) {
-        auto *gcur = reinterpret_cast<unsigned long long *>(s.gcur);
-        hipLaunchKernelGGL(k_group_cursors, dim3(1), dim3(256), 0, st, s.o1, s.m1, G, n_coarse, gcur);
-        const bool pass = d.pass_bits != 0;
-        auto f1 = pass ? &k_front_scatter_seq_staged<1, kFrontBlock1, kFrontChunk1, true>
-                       : &k_front_scatter_seq_staged<1, kFrontBlock1, kFrontChunk1, false>;
-        auto f2 = pass ? &k_front_scatter_seq_staged<2, kFrontBlock2, kFrontChunk2, true>
-                       : &k_front_scatter_seq_staged<2, kFrontBlock2, kFrontChunk2, false>;
-        if (nw == 1) hipLaunchKernelGGL(f1, dim3(G), dim3(kFrontBlock1), 0, st, seq, t->cfg.k, d, gcur, s.keys_a, dig, tv.counters);
-        else hipLaunchKernelGGL(f2, dim3(G), dim3(kFrontBlock2), 0, st, seq, t->cfg.k, d, gcur, s.keys_a, dig, tv.counters);
-        hipLaunchKernelGGL(k_group_check, dim3(1), dim3(256), 0, st, gcur, n_coarse, tv.counters);
+        hipLaunchKernelGGL(k_group_cursors, dim3(1), dim3(256), 0, st, s.o1, s.m1, G, n_coarse, gcur, gate, tv.counters);
+        if (nw == 1) hipLaunchKernelGGL(f1, dim3(G), dim3(kFrontBlock1), 0, st, seq, t->cfg.k, d, gcur, s.keys_a, dig, tv.counters, gate, nullptr);
+        else hipLaunchKernelGGL(f2, dim3(G), dim3(kFrontBlock2), 0, st, seq, t->cfg.k, d, gcur, s.keys_a, dig, tv.counters, gate, nullptr);
+        hipLaunchKernelGGL(k_group_check, dim3(1), dim3(256), 0, st, gcur, n_coarse, tv.counters, gate);
     } else {
         if (nw == 1) hipLaunchKernelGGL(k_front_scatter_keys<1>, dim3(G), dim3(kKeyBlock), 0, st, d_keys, n_keys, d, s.o1, s.m1, s.keys_a, dig, tv.counters);
         else hipLaunchKernelGGL(k_front_scatter_keys<2>, dim3(G), dim3(kKeyBlock), 0, st, d_keys, n_keys, d, s.o1, s.m1, s.keys_a, dig, tv.counters);
+        hipLaunchKernelGGL(k_group_from_starts, dim3(1), dim3(256), 0, st, s.starts, n_coarse, gcur);
     }
     KC_HIP(hipGetLastError());
 
@@ -1076,27 +1227,27 @@ int kc_part_enqueue(const kc_table *t, kc_part_scratch &s, const kc_pipe_shape &
     const uint64_t *part_keys = s.keys_a;
     if (d.l2) {
         const dim3 g2(n_coarse * S);
-        if (!d.dig16) hipLaunchKernelGGL(k_fine_count_dig<uint8_t>, g2, dim3(kKeyBlock), 0, st, s.dig, s.starts, d, S, s.m2);
-        else hipLaunchKernelGGL(k_fine_count_dig<uint16_t>, g2, dim3(kKeyBlock), 0, st, reinterpret_cast<const uint16_t *>(s.dig), s.starts, d, S, s.m2);
+        if (!d.dig16) hipLaunchKernelGGL(k_fine_count_dig<uint8_t>, g2, dim3(kKeyBlock), 0, st, s.dig, gcur, d, S, s.m2);
+        else hipLaunchKernelGGL(k_fine_count_dig<uint16_t>, g2, dim3(kKeyBlock), 0, st, reinterpret_cast<const uint16_t *>(s.dig), gcur, d, S, s.m2);
         KC_HIP(hipGetLastError());
         if (t->fault_skew == 3 || t->fault_skew == 4)
             hipLaunchKernelGGL(k_skew_count, dim3(1), dim3(64), 0, st, s.m2, m2n, t->fault_skew == 3 ? -1 : 1);
         if ((rc = kc_scan_u32_on(st, s.m2, m2n, s.o2, s.sums))) return rc;
         unsigned long long *ctr = tv.counters;
         if (d.l2 <= 8) {
-            if (nw == 1) hipLaunchKernelGGL((k_fine_scatter_staged<1, 256, 1024, 16>), g2, dim3(1024), 0, st, s.keys_a, s.starts, d, S, s.o2, s.m2, s.keys_b, ctr);
-            else hipLaunchKernelGGL((k_fine_scatter_staged<2, 256, 1024, 8>), g2, dim3(1024), 0, st, s.keys_a, s.starts, d, S, s.o2, s.m2, s.keys_b, ctr);
+            if (nw == 1) hipLaunchKernelGGL((k_fine_scatter_staged<1, 256, 1024, 16>), g2, dim3(1024), 0, st, s.keys_a, gcur, d, S, s.o2, s.m2, s.keys_b, ctr);
+            else hipLaunchKernelGGL((k_fine_scatter_staged<2, 256, 1024, 8>), g2, dim3(1024), 0, st, s.keys_a, gcur, d, S, s.o2, s.m2, s.keys_b, ctr);
         } else {
-            if (nw == 1) hipLaunchKernelGGL((k_fine_scatter_staged<1, 1024, 1024, 16, false>), g2, dim3(1024), 0, st, s.keys_a, s.starts, d, S, s.o2, s.m2, s.keys_b, ctr);
+            if (nw == 1) hipLaunchKernelGGL((k_fine_scatter_staged<1, 1024, 1024, 16, false>), g2, dim3(1024), 0, st, s.keys_a, gcur, d, S, s.o2, s.m2, s.keys_b, ctr);
             else if (d.l2 == 9)  // 512 digits: their u16 copies fit next to the 8192 two-word keys
-                hipLaunchKernelGGL((k_fine_scatter_staged<2, 512, 1024, 8, true>), g2, dim3(1024), 0, st, s.keys_a, s.starts, d, S, s.o2, s.m2, s.keys_b, ctr);
-            else hipLaunchKernelGGL((k_fine_scatter_staged<2, 1024, 1024, 8, false>), g2, dim3(1024), 0, st, s.keys_a, s.starts, d, S, s.o2, s.m2, s.keys_b, ctr);
+                hipLaunchKernelGGL((k_fine_scatter_staged<2, 512, 1024, 8, true>), g2, dim3(1024), 0, st, s.keys_a, gcur, d, S, s.o2, s.m2, s.keys_b, ctr);
+            else hipLaunchKernelGGL((k_fine_scatter_staged<2, 1024, 1024, 8, false>), g2, dim3(1024), 0, st, s.keys_a, gcur, d, S, s.o2, s.m2, s.keys_b, ctr);
         }
         KC_HIP(hipGetLastError());
         part_keys = s.keys_b;
     }
     hipLaunchKernelGGL(k_part_offsets, dim3((uint32_t)std::min<uint64_t>((n_parts + 256) / 256, 4096)), dim3(256), 0,
-                       st, s.o2, s.starts, d, S, s.part_off);
+                       st, s.o2, s.m2, s.starts, d, S, s.part_off);
     KC_HIP(hipGetLastError());
 
     // per-partition LDS counting

@@ -57,7 +57,7 @@ class ReadSpec(C.Structure):
 class Tune(C.Structure):
     _fields_ = [("front_blocks", C.c_uint32), ("fine_slices", C.c_uint32), ("batch_bytes", C.c_uint64),
                 ("queue_depth", C.c_uint32), ("path", C.c_uint32), ("piece_bytes", C.c_uint64),
-                ("first_mode", C.c_uint32)]
+                ("first_mode", C.c_uint32), ("front_mode", C.c_uint32)]
 
 
 class BatchInfo(C.Structure):
@@ -119,6 +119,7 @@ EXPORTS = {
     "kc_table_query": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
     "kc_table_query_on": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
     "kc_table_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
+    "kc_table_front_redos": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
     "kc_table_set_pass": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32]),
     "kc_table_export_dir": (C.c_int, [C.c_void_p, C.c_char_p]),
     "kc_table_export_dir_append": (C.c_int, [C.c_void_p, C.c_char_p]),
@@ -534,6 +535,13 @@ class Table:
         check(lib().kc_table_stats(self.h, C.byref(s)), "kc_table_stats")
         return {f: getattr(s, f) for f, _ in Stats._fields_}
 
+    def front_redos(self):
+        """Partitioned passes since the last clear whose speculative coarse
+        regions overflowed and were redone exactly."""
+        v = C.c_uint64()
+        check(lib().kc_table_front_redos(self.h, C.byref(v)), "kc_table_front_redos")
+        return v.value
+
     def export_dir(self, path, append=False):
         if append:
             check(lib().kc_table_export_dir_append(self.h, path.encode()), "kc_table_export_dir_append")
@@ -620,7 +628,7 @@ class Batch:
             t = Tune(front_blocks=front_blocks or base["front_blocks"], fine_slices=fine_slices or base["fine_slices"],
                      batch_bytes=batch_bytes or base["batch_bytes"], queue_depth=queue_depth or base["queue_depth"],
                      path=base["path"] if path is None else paths[path], piece_bytes=base["piece_bytes"],
-                     first_mode=base["first_mode"])
+                     first_mode=base["first_mode"], front_mode=base["front_mode"])
             check(lib().kc_batch_create(table.h, C.byref(t), C.byref(h)), "kc_batch_create")
         else:
             check(lib().kc_batch_create(table.h, None, C.byref(h)), "kc_batch_create")
@@ -707,7 +715,8 @@ def count_fastq_files(table: "Table", paths, threads=4, batch_bytes=0, queue_dep
         base = table.tune()
         tune = Tune(front_blocks=base["front_blocks"], fine_slices=base["fine_slices"],
                     batch_bytes=batch_bytes or base["batch_bytes"], queue_depth=queue_depth or base["queue_depth"],
-                    path=base["path"], piece_bytes=base["piece_bytes"], first_mode=base["first_mode"])
+                    path=base["path"], piece_bytes=base["piece_bytes"], first_mode=base["first_mode"],
+                    front_mode=base["front_mode"])
     check(lib().kc_count_fastq_files(table.h, arr, len(paths), threads, C.byref(tune) if tune else None,
                                      C.byref(st)), "kc_count_fastq_files")
     return {f: getattr(st, f) for f, _ in IngestStats._fields_}
