@@ -310,8 +310,10 @@ int kc_table_get_tune(kc_table *t, kc_tune *out);
 /* Times kc_table_count_seq over a device-resident sample (reps timed runs
  * per candidate after one warm-up) for each launch shape: partitioned with
  * {2,4,8,16} persistent front blocks per CU x {2,4,8,16} fine slices, and
- * direct; stores the fastest in the table's tune and returns it.  Every
- * trial starts from an empty table and the table is left empty. */
+ * direct; stores the fastest in the table's tune and returns it (with
+ * front_mode exact when that shape's speculative front end had to be redone
+ * on the sample).  Every trial starts from an empty table and the table is
+ * left empty. */
 int kc_autotune(kc_table *t, const uint8_t *d_sample, uint64_t nbytes, uint32_t reps, kc_tune *best,
                 float *best_ms);
 

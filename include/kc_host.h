@@ -31,8 +31,14 @@ void kc_host_free(void *p);
  * tune): the chunks go straight into the batch's page-locked staging slots
  * and each full slot is counted by one graph launch while inflating goes on.
  * Several files inflate in parallel (the reference reads a directory or a
- * comma-separated file list, main.cpp:501-530).  Returns when every read is
- * counted.  stats may be NULL. */
+ * comma-separated file list, main.cpp:501-530), and so do the parts of one
+ * BGZF file (blocked gzip, as htslib's bgzip writes it: gzip members whose
+ * "BC" extra subfield gives their size): threads inflate runs of about 1 MB
+ * of members, each run is framed from the record phase the newline counts of
+ * the runs before it give, and the runs are handed over in file order.  The
+ * reference's gzread inflates such a file as one stream on one thread; the
+ * counts are the same.  Returns when every read is counted.  stats may be
+ * NULL. */
 typedef struct kc_ingest_stats {
     uint64_t files;
     uint64_t threads;
@@ -45,6 +51,13 @@ typedef struct kc_ingest_stats {
 } kc_ingest_stats;
 int kc_count_fastq_files(kc_table *t, const char *const *paths, uint32_t n_paths, uint32_t threads,
                          const kc_tune *tune, kc_ingest_stats *stats);
+/* The same reader without a table: frames the files into one malloc'ed
+ * buffer (released with kc_host_free), each file's reads in order, the files'
+ * chunks in the order the threads finish them, a '\n' after a file's last
+ * read if it lacks one.  Host only; no reference counterpart beyond
+ * FastqReader's framing. */
+int kc_fastq_frame_files(const char *const *paths, uint32_t n_paths, uint32_t threads, uint8_t **out,
+                         uint64_t *out_len, kc_ingest_stats *stats);
 
 /* Out-of-HBM counting: the files are counted in n_passes passes, pass p
  * keeping the k-mers of hash slice p (kc_table_set_pass) in the cleared

@@ -532,6 +532,7 @@ extern "C" int kc_autotune(kc_table *t, const uint8_t *d_sample, uint64_t nbytes
     float best_t = 0;
     int rc = KC_OK;
     kc_tune win = saved;
+    uint64_t win_redos = 0;  // speculative front ends the winning shape had to redo
     for (uint32_t ci = 0; ci < n_cands; ++ci) {
         const Cand &c = cands[ci];
         t->tune = saved;
@@ -555,11 +556,17 @@ extern "C" int kc_autotune(kc_table *t, const uint8_t *d_sample, uint64_t nbytes
         if (st.errors) {  // this shape overflowed: not a candidate
             continue;
         }
+        uint64_t redos = 0;
+        if ((rc = kc_table_front_redos(t, &redos)) != KC_OK) break;
         if (best_t == 0 || tmin < best_t) {
             best_t = tmin;
             win = t->tune;
+            win_redos = redos;
         }
     }
+    // input whose coarse regions overflow the speculative capacity (a few
+    // keys repeated all over) pays the redo on every call: place it exactly
+    if (rc == KC_OK && win_redos && win.front_mode == 0) win.front_mode = 1;
     // the other FIRST hand-overs of the partition pass on the winning shape
     const uint32_t first_default = kc_part_shape(t, win, 0).first_mode;
     for (uint32_t mode = 1; mode <= 3; ++mode) {

@@ -5,6 +5,8 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <condition_variable>
+#include <functional>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -311,17 +313,378 @@ extern "C" int64_t kc_synth_reads(const uint8_t *genome, uint64_t glen, const kc
     });
 }
 
-// ---- streaming ingest: FASTQ(.gz) files -> framed chunks -> kc_batch -------
-// Host threads take files from a shared cursor; each inflates its file with
-// zlib (gzread transparently reads plain files too, as FastqReader.h:575-616
-// does for ".gz"), frames it (line 2 of every record + '\n') into a local
-// buffer, and hands whole reads to the batch in chunks of about kChunk bytes.
-// The batch copies each chunk into its page-locked staging slot, and every
-// full slot is counted by one graph launch while the threads keep inflating.
+// ---- streaming ingest: FASTQ(.gz) files -> framed chunks -> a sink --------
+// Files are cut into segments that host threads take in turn.  A plain or
+// single-member gzip file is one segment: its thread inflates it with zlib
+// (gzread reads plain files too, as FastqReader.h:575-616 does for ".gz")
+// and frames it (line 2 of every record + '\n') as it goes.  A BGZF file
+// (htslib's blocked gzip: a chain of gzip members whose "BC" extra subfield
+// gives each member's size, so members are found without inflating) is cut
+// into runs of members that threads inflate in parallel; the record phase at
+// the start of a run (its line number mod 4) comes from the newline counts
+// of the runs before it, handed on in file order, so each run is framed
+// independently and the framed runs are handed to the sink in file order,
+// whole lines only (a line cut by a run boundary is carried to the next).
+// The sink gets whole reads in chunks (a file's last chunk may lack its
+// final '\n'); chunks of different files may interleave, each file's reads
+// stay in order.
 namespace {
 
 constexpr uint64_t kIngestChunk = 4ull << 20;
 constexpr unsigned kGzRead = 1u << 20;
+constexpr uint64_t kBgzfRun = 1ull << 20;  // compressed bytes per BGZF segment
+
+using Sink = std::function<int(const uint8_t *, uint64_t)>;
+
+struct Member {
+    uint64_t off;   // file offset of the gzip member
+    uint32_t size;  // bytes of the member
+};
+
+uint16_t le16(const uint8_t *p) { return (uint16_t)(p[0] | (p[1] << 8)); }
+uint32_t le32(const uint8_t *p) { return (uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) | ((uint32_t)p[3] << 24); }
+
+// Header of a BGZF member at p (n bytes available): the member's size and
+// where its deflate data starts, or false if it is no BGZF member (FLG must
+// be FEXTRA alone and the extra field must hold a "BC" subfield, SAM/BAM
+// format specification section 4.1).
+bool bgzf_header(const uint8_t *p, uint64_t n, uint32_t &size, uint32_t &data_at)
+{
+    if (n < 18 || p[0] != 0x1f || p[1] != 0x8b || p[2] != 8 || p[3] != 4) return false;
+    const uint32_t xlen = le16(p + 10);
+    if (n < 12ull + xlen) return false;
+    for (uint32_t i = 0; i + 4 <= xlen;) {
+        const uint8_t *sf = p + 12 + i;
+        const uint32_t slen = le16(sf + 2);
+        if (sf[0] == 'B' && sf[1] == 'C' && slen == 2 && i + 6 <= xlen) {
+            size = (uint32_t)le16(sf + 4) + 1;
+            data_at = 12 + xlen;
+            return size >= data_at + 8;
+        }
+        i += 4 + slen;
+    }
+    return false;
+}
+
+// The members of a BGZF file, or false if any part of it is not BGZF.
+bool bgzf_members(FILE *f, uint64_t file_size, std::vector<Member> &out)
+{
+    uint8_t h[12 + 65535];
+    uint64_t off = 0;
+    while (off < file_size) {
+        const uint64_t want = std::min<uint64_t>(sizeof(h), file_size - off);
+        if (fseeko(f, (off_t)off, SEEK_SET) != 0 || std::fread(h, 1, 12, f) != 12) return false;
+        const uint64_t hdr = std::min<uint64_t>(want, 12ull + le16(h + 10));
+        if (hdr > 12 && std::fread(h + 12, 1, hdr - 12, f) != hdr - 12) return false;
+        uint32_t size = 0, data_at = 0;
+        if (!bgzf_header(h, hdr, size, data_at) || off + size > file_size) return false;
+        out.push_back({off, size});
+        off += size;
+    }
+    return !out.empty();
+}
+
+struct Segment {
+    uint32_t file;
+    uint32_t index;   // among the file's segments
+    bool last;        // the file's last segment
+    bool bgzf;        // members [m0, m1) of the file; else the whole file, streamed
+    uint32_t m0, m1;
+};
+
+// Hand-over state of one BGZF file: the next segment whose phase and whose
+// framed bytes are due, the file's newlines so far, the partial line carried.
+struct FileOrder {
+    uint32_t next_phase = 0, next_hand = 0;
+    uint64_t lines = 0;
+    std::vector<uint8_t> carry;
+};
+
+// Frames p[0, n) in place (the framed bytes never pass the bytes read).
+uint64_t frame_in_place(uint8_t *p, uint64_t n, uint32_t state)
+{
+    uint64_t w = 0, i = 0;
+    while (i < n) {
+        const uint8_t *nl = static_cast<const uint8_t *>(std::memchr(p + i, '\n', n - i));
+        const uint64_t end = nl ? (uint64_t)(nl - p) : n;
+        if (state == 1) {
+            std::memmove(p + w, p + i, end - i);
+            w += end - i;
+            if (nl) p[w++] = '\n';
+        }
+        if (!nl) break;
+        i = end + 1;
+        state = (state + 1) & 3;
+    }
+    return w;
+}
+
+struct Ingest {
+    const char *const *paths;
+    uint32_t n_paths;
+    Sink sink;
+    std::vector<std::vector<Member>> members;  // per file (empty: streamed)
+    std::vector<Segment> segs;
+    std::vector<FileOrder> order;
+    std::mutex omu;  // order, the results and the error
+    std::condition_variable ocv;
+    std::atomic<uint64_t> next{0};
+    int first_error = KC_OK;
+    uint64_t in_bytes = 0, framed = 0, chunks = 0;
+    double inflate_s = 0;
+
+    void fail(int e)
+    {
+        std::lock_guard<std::mutex> lk(omu);
+        if (first_error == KC_OK) first_error = e;
+        ocv.notify_all();
+    }
+    bool failed()
+    {
+        std::lock_guard<std::mutex> lk(omu);
+        return first_error != KC_OK;
+    }
+
+    // Plans the segments: member runs of BGZF files, one per other file.
+    int plan(uint32_t threads)
+    {
+        members.resize(n_paths);
+        order.resize(n_paths);
+        for (uint32_t fi = 0; fi < n_paths; ++fi) {
+            FILE *f = std::fopen(paths[fi], "rb");
+            if (!f) return KC_EIO;
+            uint64_t size = 0;
+            if (fseeko(f, 0, SEEK_END) == 0) size = (uint64_t)ftello(f);
+            std::vector<Member> m;
+            // only worth the walk when there are threads to share the file
+            const bool bgzf = threads > 1 && size > kBgzfRun && bgzf_members(f, size, m);
+            std::fclose(f);
+            if (!bgzf) {
+                segs.push_back({fi, 0, true, false, 0, 0});
+                continue;
+            }
+            uint32_t idx = 0;
+            for (uint32_t a = 0; a < m.size();) {
+                uint64_t bytes = 0;
+                uint32_t b = a;
+                while (b < m.size() && (b == a || bytes + m[b].size <= kBgzfRun)) bytes += m[b++].size;
+                segs.push_back({fi, idx++, b == m.size(), true, a, b});
+                a = b;
+            }
+            members[fi] = std::move(m);
+        }
+        return KC_OK;
+    }
+
+    // Whole lines of p[0, n) (after the file's carried partial line) to the
+    // sink; the rest is carried.  The file's last piece also hands over the
+    // carry.  omu is held by the caller (file order).
+    int hand(FileOrder &fo, const uint8_t *p, uint64_t n, bool last, uint64_t &n_chunks)
+    {
+        int rc = KC_OK;
+        const uint8_t *lnl = n ? static_cast<const uint8_t *>(memrchr(p, '\n', n)) : nullptr;
+        if (!lnl) {
+            fo.carry.insert(fo.carry.end(), p, p + n);
+        } else {
+            uint64_t from = 0;
+            const uint64_t cut = (uint64_t)(lnl - p) + 1;
+            if (!fo.carry.empty()) {
+                const uint64_t first = (uint64_t)(static_cast<const uint8_t *>(std::memchr(p, '\n', n)) - p) + 1;
+                fo.carry.insert(fo.carry.end(), p, p + first);
+                rc = sink(fo.carry.data(), fo.carry.size());
+                ++n_chunks;
+                fo.carry.clear();
+                from = first;
+            }
+            if (rc == KC_OK && cut > from) {
+                rc = sink(p + from, cut - from);
+                ++n_chunks;
+            }
+            fo.carry.assign(p + cut, p + n);
+        }
+        if (rc == KC_OK && last && !fo.carry.empty()) {
+            rc = sink(fo.carry.data(), fo.carry.size());  // the file's last read (newline added if missing)
+            ++n_chunks;
+            fo.carry.clear();
+        }
+        return rc;
+    }
+
+    // One BGZF member run: inflate (CRC and size checked, as gzread does),
+    // count its lines, take its phase in file order, frame, hand over in
+    // file order.
+    int run_bgzf(const Segment &sg, std::vector<uint8_t> &cbuf, std::vector<uint8_t> &out, z_stream &z)
+    {
+        const std::vector<Member> &m = members[sg.file];
+        const uint64_t c0 = m[sg.m0].off, c1 = m[sg.m1 - 1].off + m[sg.m1 - 1].size;
+        cbuf.resize(c1 - c0);
+        FILE *f = std::fopen(paths[sg.file], "rb");
+        if (!f) return KC_EIO;
+        const bool rd = fseeko(f, (off_t)c0, SEEK_SET) == 0 && std::fread(cbuf.data(), 1, cbuf.size(), f) == cbuf.size();
+        std::fclose(f);
+        if (!rd) return KC_EIO;
+        uint64_t total = 0;
+        for (uint32_t i = sg.m0; i < sg.m1; ++i) total += le32(cbuf.data() + (m[i].off - c0) + m[i].size - 4);
+        if (total > (4ull << 30)) return KC_EIO;  // not what ~1 MB of BGZF members inflate to
+        out.resize(total);
+        const auto a = std::chrono::steady_clock::now();
+        uint64_t o = 0;
+        for (uint32_t i = sg.m0; i < sg.m1; ++i) {
+            const uint8_t *p = cbuf.data() + (m[i].off - c0);
+            uint32_t size = 0, data_at = 0;
+            if (!bgzf_header(p, m[i].size, size, data_at)) return KC_EIO;
+            const uint32_t isize = le32(p + size - 4), crc = le32(p + size - 8);
+            uint8_t dummy = 0;
+            if (inflateReset(&z) != Z_OK) return KC_EIO;
+            z.next_in = const_cast<Bytef *>(p + data_at);
+            z.avail_in = size - data_at - 8;
+            z.next_out = isize ? out.data() + o : &dummy;
+            z.avail_out = isize;
+            if (inflate(&z, Z_FINISH) != Z_STREAM_END || z.avail_out != 0 || z.total_out != isize) return KC_EIO;
+            if (crc32(0L, isize ? out.data() + o : &dummy, isize) != crc) return KC_EIO;
+            o += isize;
+        }
+        const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
+        const uint64_t lines = (uint64_t)std::count(out.begin(), out.end(), (uint8_t)'\n');
+        FileOrder &fo = order[sg.file];
+        uint32_t state;
+        {
+            std::unique_lock<std::mutex> lk(omu);
+            ocv.wait(lk, [&] { return fo.next_phase == sg.index || first_error != KC_OK; });
+            if (first_error != KC_OK) return first_error;
+            state = (uint32_t)(fo.lines & 3);
+            fo.lines += lines;
+            ++fo.next_phase;
+            ocv.notify_all();
+        }
+        const uint64_t w = frame_in_place(out.data(), out.size(), state);
+        std::unique_lock<std::mutex> lk(omu);
+        ocv.wait(lk, [&] { return fo.next_hand == sg.index || first_error != KC_OK; });
+        if (first_error != KC_OK) return first_error;
+        uint64_t n_chunks = 0;
+        const int rc = hand(fo, out.data(), w, sg.last, n_chunks);
+        ++fo.next_hand;
+        in_bytes += total;
+        framed += w;
+        chunks += n_chunks;
+        inflate_s += secs;
+        ocv.notify_all();
+        return rc;
+    }
+
+    // A whole file streamed through gzread by one thread.
+    int run_stream(const Segment &sg, std::vector<uint8_t> &in, std::vector<uint8_t> &buf)
+    {
+        gzFile g = gzopen(paths[sg.file], "rb");
+        if (!g) return KC_EIO;
+        gzbuffer(g, kGzRead);
+        Framer fr;
+        uint64_t used = 0, my_in = 0, my_framed = 0, my_chunks = 0;
+        double my_inflate = 0;
+        int err = KC_OK;
+        for (;;) {
+            const auto a = std::chrono::steady_clock::now();
+            const int r = gzread(g, in.data(), kGzRead);
+            my_inflate += std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
+            if (r < 0) {
+                err = KC_EIO;
+                break;
+            }
+            if (r == 0) break;
+            my_in += (uint64_t)r;
+            used += fr.feed(in.data(), (uint64_t)r, buf.data() + used);
+            if (used >= kIngestChunk) {
+                // hand over whole reads; a partial sequence line waits for the next block
+                const uint8_t *nl = static_cast<const uint8_t *>(memrchr(buf.data(), '\n', used));
+                const uint64_t cut = nl ? (uint64_t)(nl - buf.data()) + 1 : 0;
+                if (cut) err = sink(buf.data(), cut);
+                if (err != KC_OK) break;
+                std::memmove(buf.data(), buf.data() + cut, used - cut);
+                my_framed += cut;
+                my_chunks += cut ? 1 : 0;
+                used -= cut;
+            }
+            // room for the next block (more than a chunk when one sequence line is longer)
+            if (buf.size() < used + kGzRead + (used > kIngestChunk ? kIngestChunk : 0))
+                buf.resize(used + kGzRead + (used > kIngestChunk ? kIngestChunk : 0));
+        }
+        gzclose(g);
+        if (err == KC_OK && used) {
+            err = sink(buf.data(), used);  // the file's last read (newline added if missing)
+            my_framed += used;
+            ++my_chunks;
+        }
+        std::lock_guard<std::mutex> lk(omu);
+        in_bytes += my_in;
+        framed += my_framed;
+        chunks += my_chunks;
+        inflate_s += my_inflate;
+        return err;
+    }
+
+    void work()
+    {
+        std::vector<uint8_t> in, buf, cbuf, out;
+        z_stream z{};
+        bool z_ok = false;
+        for (;;) {
+            const uint64_t si = next.fetch_add(1);
+            if (si >= segs.size() || failed()) break;
+            const Segment &sg = segs[si];
+            int rc;
+            if (sg.bgzf) {
+                if (!z_ok) {
+                    if (inflateInit2(&z, -15) != Z_OK) {
+                        fail(KC_ENOMEM);
+                        break;
+                    }
+                    z_ok = true;
+                }
+                rc = run_bgzf(sg, cbuf, out, z);
+            } else {
+                if (in.empty()) {
+                    in.resize(kGzRead);
+                    buf.resize(kIngestChunk + kGzRead);
+                }
+                rc = run_stream(sg, in, buf);
+            }
+            if (rc != KC_OK) {
+                fail(rc);
+                break;
+            }
+        }
+        if (z_ok) inflateEnd(&z);
+    }
+
+    // Runs the segments on up to `threads` threads; returns the first error.
+    int run(uint32_t threads)
+    {
+        int rc = plan(threads);
+        if (rc != KC_OK) return rc;
+        // an allocation that fails in a worker ends the ingest with KC_ENOMEM
+        auto worker = [&]() {
+            try {
+                work();
+            } catch (...) {
+                fail(KC_ENOMEM);
+            }
+        };
+        const uint32_t nt = std::max<uint32_t>(
+            1, std::min<uint64_t>(threads ? threads : 1, std::max<uint64_t>(segs.size(), 1)));
+        std::vector<std::thread> pool;
+        try {
+            pool.reserve(nt);
+            for (uint32_t i = 0; i < nt; ++i) pool.emplace_back(worker);
+        } catch (...) {
+            // fewer threads: the ones running take every segment from the cursor
+        }
+        if (pool.empty()) worker();
+        for (auto &th : pool) th.join();
+        used_threads = pool.empty() ? 1 : (uint32_t)pool.size();
+        return first_error;
+    }
+    uint32_t used_threads = 1;
+};
 
 }  // namespace
 
@@ -336,102 +699,15 @@ extern "C" int kc_count_fastq_files(kc_table *t, const char *const *paths, uint3
         int rc = kc_batch_create(t, tune, &b);
         if (rc != KC_OK) return rc;
         const auto t0 = std::chrono::steady_clock::now();
-        std::mutex mu;  // guards the batch and the shared results
-        std::atomic<uint32_t> next{0};
-        int first_error = KC_OK;
-        uint64_t in_bytes = 0, framed = 0, chunks = 0;
-        double inflate_s = 0;
-        auto work = [&]() {
-            std::vector<uint8_t> in(kGzRead), buf(kIngestChunk + kGzRead);
-            for (;;) {
-                const uint32_t f = next.fetch_add(1);
-                if (f >= n_paths) return;
-                {
-                    std::lock_guard<std::mutex> lk(mu);
-                    if (first_error != KC_OK) return;
-                }
-                gzFile g = gzopen(paths[f], "rb");
-                if (!g) {
-                    std::lock_guard<std::mutex> lk(mu);
-                    if (first_error == KC_OK) first_error = KC_EIO;
-                    return;
-                }
-                gzbuffer(g, kGzRead);
-                Framer fr;
-                uint64_t used = 0, my_in = 0, my_framed = 0, my_chunks = 0;
-                double my_inflate = 0;
-                int err = KC_OK;
-                for (;;) {
-                    const auto a = std::chrono::steady_clock::now();
-                    const int r = gzread(g, in.data(), kGzRead);
-                    my_inflate += std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
-                    if (r < 0) {
-                        err = KC_EIO;
-                        break;
-                    }
-                    if (r == 0) break;
-                    my_in += (uint64_t)r;
-                    used += fr.feed(in.data(), (uint64_t)r, buf.data() + used);
-                    if (used >= kIngestChunk) {
-                        // hand over whole reads; a partial sequence line waits for the next block
-                        const uint8_t *nl = static_cast<const uint8_t *>(memrchr(buf.data(), '\n', used));
-                        const uint64_t cut = nl ? (uint64_t)(nl - buf.data()) + 1 : 0;
-                        if (cut) {
-                            std::lock_guard<std::mutex> lk(mu);
-                            err = kc_batch_add(b, buf.data(), cut);
-                        }
-                        if (err != KC_OK) break;
-                        std::memmove(buf.data(), buf.data() + cut, used - cut);
-                        my_framed += cut;
-                        my_chunks += cut ? 1 : 0;
-                        used -= cut;
-                    }
-                    // room for the next block (more than a chunk when one sequence line is longer)
-                    if (buf.size() < used + kGzRead + (used > kIngestChunk ? kIngestChunk : 0)) {
-                        try {
-                            buf.resize(used + kGzRead + (used > kIngestChunk ? kIngestChunk : 0));
-                        } catch (...) {
-                            err = KC_ENOMEM;
-                            break;
-                        }
-                    }
-                }
-                gzclose(g);
-                if (err == KC_OK && used) {
-                    std::lock_guard<std::mutex> lk(mu);
-                    err = kc_batch_add(b, buf.data(), used);  // the file's last read (newline added if missing)
-                    my_framed += used;
-                    ++my_chunks;
-                }
-                std::lock_guard<std::mutex> lk(mu);
-                in_bytes += my_in;
-                framed += my_framed;
-                chunks += my_chunks;
-                inflate_s += my_inflate;
-                if (err != KC_OK && first_error == KC_OK) first_error = err;
-                if (first_error != KC_OK) return;
-            }
+        std::mutex bmu;  // guards the batch
+        Ingest ing;
+        ing.paths = paths;
+        ing.n_paths = n_paths;
+        ing.sink = [&](const uint8_t *p, uint64_t n) {
+            std::lock_guard<std::mutex> lk(bmu);
+            return kc_batch_add(b, p, n);
         };
-        // an allocation that fails in a worker ends the ingest with KC_ENOMEM
-        auto worker = [&]() {
-            try {
-                work();
-            } catch (...) {
-                std::lock_guard<std::mutex> lk(mu);
-                if (first_error == KC_OK) first_error = KC_ENOMEM;
-            }
-        };
-        const uint32_t nt =
-            std::max<uint32_t>(1, std::min<uint32_t>(threads ? threads : 1, std::max<uint32_t>(n_paths, 1)));
-        std::vector<std::thread> pool;
-        try {
-            pool.reserve(nt);
-            for (uint32_t i = 0; i < nt; ++i) pool.emplace_back(worker);
-        } catch (...) {
-            // fewer threads: the ones running take every file from the cursor
-        }
-        if (pool.empty()) worker();
-        for (auto &th : pool) th.join();
+        int first_error = ing.run(threads);
         if (first_error == KC_OK) first_error = kc_batch_flush(b);
         const int rs = kc_batch_sync(b);
         if (first_error == KC_OK) first_error = rs;
@@ -440,15 +716,55 @@ extern "C" int kc_count_fastq_files(kc_table *t, const char *const *paths, uint3
         kc_batch_destroy(b);
         if (stats) {
             stats->files = n_paths;
-            stats->threads = nt;
-            stats->input_bytes = in_bytes;
-            stats->framed_bytes = framed;
-            stats->chunks = chunks;
+            stats->threads = ing.used_threads;
+            stats->input_bytes = ing.in_bytes;
+            stats->framed_bytes = ing.framed;
+            stats->chunks = ing.chunks;
             stats->launches = bi.launches;
-            stats->inflate_seconds = inflate_s;
+            stats->inflate_seconds = ing.inflate_s;
             stats->wall_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         }
         return first_error;
+    });
+}
+
+extern "C" int kc_fastq_frame_files(const char *const *paths, uint32_t n_paths, uint32_t threads, uint8_t **out,
+                                    uint64_t *out_len, kc_ingest_stats *stats)
+{
+    return kc_guard([&]() -> int {
+        if (!out || !out_len || (n_paths && !paths)) return KC_EINVAL;
+        for (uint32_t i = 0; i < n_paths; ++i)
+            if (!paths[i]) return KC_EINVAL;
+        const auto t0 = std::chrono::steady_clock::now();
+        std::mutex bmu;  // guards the output
+        std::vector<uint8_t> acc;
+        Ingest ing;
+        ing.paths = paths;
+        ing.n_paths = n_paths;
+        ing.sink = [&](const uint8_t *p, uint64_t n) {
+            std::lock_guard<std::mutex> lk(bmu);
+            acc.insert(acc.end(), p, p + n);
+            if (n && p[n - 1] != '\n') acc.push_back('\n');  // as a batch completes a chunk's last line
+            return KC_OK;
+        };
+        const int rc = ing.run(threads);
+        if (rc != KC_OK) return rc;
+        uint8_t *buf = static_cast<uint8_t *>(std::malloc(std::max<size_t>(acc.size(), 1)));
+        if (!buf) return KC_ENOMEM;
+        if (!acc.empty()) std::memcpy(buf, acc.data(), acc.size());
+        *out = buf;
+        *out_len = acc.size();
+        if (stats) {
+            stats->files = n_paths;
+            stats->threads = ing.used_threads;
+            stats->input_bytes = ing.in_bytes;
+            stats->framed_bytes = ing.framed;
+            stats->chunks = ing.chunks;
+            stats->launches = 0;
+            stats->inflate_seconds = ing.inflate_s;
+            stats->wall_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        }
+        return KC_OK;
     });
 }
 
@@ -631,13 +947,6 @@ namespace {
 // pass (Bloom split on).
 uint64_t kc_lds_slots(uint32_t nw) { return nw == 1 ? 4096 : 2048; }
 uint64_t kc_lds_bloom(uint32_t) { return 4096; }
-
-uint64_t pow2_at_most(uint64_t v)
-{
-    uint64_t p = 1;
-    while (p * 2 <= v) p <<= 1;
-    return p;
-}
 
 uint64_t pow2_at_least(uint64_t v)
 {

@@ -153,7 +153,9 @@ inline unsigned kc_low_chunks(uint64_t most, uint64_t per_block)
 int kc_launch_fill(kc_ctx *ctx, uint64_t *p, uint64_t n_words, uint64_t value);
 uint64_t kc_scan_tmp_words(uint64_t n);
 int kc_scan_u32(kc_ctx *ctx, const uint32_t *in, uint64_t n, uint64_t *out, uint64_t *sums);
-int kc_scan_u32_on(hipStream_t st, const uint32_t *in, uint64_t n, uint64_t *out, uint64_t *sums);
+// gate non-null: the scan runs only when *gate is nonzero (device flag)
+int kc_scan_u32_on(hipStream_t st, const uint32_t *in, uint64_t n, uint64_t *out, uint64_t *sums,
+                   const unsigned long long *gate = nullptr);
 int kc_sort_range(kc_ctx *ctx, uint32_t *a, uint32_t *b, uint64_t n, uint32_t rw, uint32_t lo, uint32_t hi,
                   int *in_scratch);
 // histogram_tool precise over a table's exported view with caller-provided

@@ -1207,9 +1207,7 @@ int kc_part_enqueue(const kc_table *t, kc_part_scratch &s, const kc_pipe_shape &
     if (t->fault_skew == 1 || t->fault_skew == 2)
         hipLaunchKernelGGL(k_skew_count, dim3(1), dim3(64), 0, st, s.m1, m1n, t->fault_skew == 1 ? -1 : 1);
     int rc;
-    // (the scan is not gated: after a speculative scatter that fit it scans
-    // counts nothing reads)
-    if ((rc = kc_scan_u32_on(st, s.m1, m1n, s.o1, s.sums))) return rc;
+    if ((rc = kc_scan_u32_on(st, s.m1, m1n, s.o1, s.sums, gate))) return rc;
     hipLaunchKernelGGL(k_coarse_starts, dim3(1), dim3(256), 0, st, s.o1, s.m1, G, n_coarse, s.starts, gate);
     if (from_seq) {
         hipLaunchKernelGGL(k_group_cursors, dim3(1), dim3(256), 0, st, s.o1, s.m1, G, n_coarse, gcur, gate, tv.counters);

@@ -146,8 +146,12 @@ __device__ __forceinline__ uint64_t block_excl_scan(uint64_t v, uint64_t &total)
     return r;
 }
 
-__global__ __launch_bounds__(kThreads) void k_scan_reduce(const uint32_t *in, uint64_t n, uint64_t *block_sums)
+// gate (the exact front end after a speculative scatter): the scan runs only
+// when *gate is nonzero; a null gate always runs.
+__global__ __launch_bounds__(kThreads) void k_scan_reduce(const uint32_t *in, uint64_t n, uint64_t *block_sums,
+                                                          const unsigned long long *gate)
 {
+    if (gate && *gate == 0) return;
     const uint64_t base = (uint64_t)blockIdx.x * kScanChunk;
     uint64_t acc = 0;
     for (int j = 0; j < kScanItems; ++j) {
@@ -159,8 +163,9 @@ __global__ __launch_bounds__(kThreads) void k_scan_reduce(const uint32_t *in, ui
     if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
 }
 
-__global__ __launch_bounds__(kThreads) void k_scan_sums(uint64_t *sums, uint64_t nb)
+__global__ __launch_bounds__(kThreads) void k_scan_sums(uint64_t *sums, uint64_t nb, const unsigned long long *gate)
 {
+    if (gate && *gate == 0) return;
     // single block: sequential chunks of kThreads
     uint64_t carry = 0;
     for (uint64_t b = 0; b < nb; b += kThreads) {
@@ -174,8 +179,9 @@ __global__ __launch_bounds__(kThreads) void k_scan_sums(uint64_t *sums, uint64_t
 }
 
 __global__ __launch_bounds__(kThreads) void k_scan_apply(const uint32_t *in, uint64_t n, const uint64_t *block_base,
-                                                         uint64_t *out)
+                                                         uint64_t *out, const unsigned long long *gate)
 {
+    if (gate && *gate == 0) return;
     const uint64_t base = (uint64_t)blockIdx.x * kScanChunk;
     uint64_t acc = 0;
     uint32_t v[kScanItems];
@@ -458,9 +464,9 @@ int sort_pass(kc_ctx *ctx, SortScratch &s, const uint32_t *src, uint32_t *dst, u
                       hipMalloc((void **)&s.sums, nb * 8) != hipSuccess))
         return KC_ENOMEM;
     hipLaunchKernelGGL((k_upsweep<RW, DB>), dim3(grid), dim3(kThreads), 0, ctx->stream, src, n, bit, width, n_tiles, s.counts);
-    hipLaunchKernelGGL(k_scan_reduce, dim3((uint32_t)nb), dim3(kThreads), 0, ctx->stream, s.counts, m, s.sums);
-    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kThreads), 0, ctx->stream, s.sums, nb);
-    hipLaunchKernelGGL(k_scan_apply, dim3((uint32_t)nb), dim3(kThreads), 0, ctx->stream, s.counts, m, s.sums, s.offsets);
+    hipLaunchKernelGGL(k_scan_reduce, dim3((uint32_t)nb), dim3(kThreads), 0, ctx->stream, s.counts, m, s.sums, nullptr);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kThreads), 0, ctx->stream, s.sums, nb, nullptr);
+    hipLaunchKernelGGL(k_scan_apply, dim3((uint32_t)nb), dim3(kThreads), 0, ctx->stream, s.counts, m, s.sums, s.offsets, nullptr);
     hipLaunchKernelGGL((k_downsweep<RW, DB>), dim3(grid), dim3(L::TB), 0, ctx->stream, src, dst, n, bit, width, n_tiles, s.offsets);
     return hipGetLastError() == hipSuccess ? KC_OK : KC_EDEVICE;
 }
@@ -605,13 +611,14 @@ extern "C" int kc_synth_records(kc_ctx *ctx, void *d_recs, uint64_t n, uint32_t 
 // sums must hold kc_scan_tmp_words(n) words.
 uint64_t kc_scan_tmp_words(uint64_t n) { return (n + kScanChunk - 1) / kScanChunk + 1; }
 
-int kc_scan_u32_on(hipStream_t st, const uint32_t *in, uint64_t n, uint64_t *out, uint64_t *sums)
+int kc_scan_u32_on(hipStream_t st, const uint32_t *in, uint64_t n, uint64_t *out, uint64_t *sums,
+                   const unsigned long long *gate)
 {
     if (!n) return KC_OK;
     const uint64_t nb = (n + kScanChunk - 1) / kScanChunk;
-    hipLaunchKernelGGL(k_scan_reduce, dim3((uint32_t)nb), dim3(kThreads), 0, st, in, n, sums);
-    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kThreads), 0, st, sums, nb);
-    hipLaunchKernelGGL(k_scan_apply, dim3((uint32_t)nb), dim3(kThreads), 0, st, in, n, sums, out);
+    hipLaunchKernelGGL(k_scan_reduce, dim3((uint32_t)nb), dim3(kThreads), 0, st, in, n, sums, gate);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kThreads), 0, st, sums, nb, gate);
+    hipLaunchKernelGGL(k_scan_apply, dim3((uint32_t)nb), dim3(kThreads), 0, st, in, n, sums, out, gate);
     KC_HIP(hipGetLastError());
     return KC_OK;
 }

@@ -171,6 +171,8 @@ EXPORTS = {
     "kc_server_resident": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "kc_fastq_frame": (C.c_uint64, [C.c_void_p, C.c_uint64, C.c_void_p]),
     "kc_fastq_frame_file": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p), C.POINTER(C.c_uint64)]),
+    "kc_fastq_frame_files": (C.c_int, [C.POINTER(C.c_char_p), C.c_uint32, C.c_uint32, C.POINTER(C.c_void_p),
+                                       C.POINTER(C.c_uint64), C.c_void_p]),
     "kc_host_free": (None, [C.c_void_p]),
     "kc_count_fastq_files": (C.c_int, [C.c_void_p, C.POINTER(C.c_char_p), C.c_uint32, C.c_uint32,
                                        C.POINTER(Tune), C.POINTER(IngestStats)]),
@@ -703,6 +705,21 @@ def fastq_frame_file(path) -> np.ndarray:
             if n.value else np.zeros(0, np.uint8)
     finally:
         lib().kc_host_free(p)
+
+
+def fastq_frame_files(paths, threads=4):
+    """The ingest reader without a table (kc_fastq_frame_files): the framed
+    reads of the files and the ingest statistics."""
+    arr = (C.c_char_p * max(1, len(paths)))(*[os.fsencode(p) for p in paths])
+    p, n, st = C.c_void_p(), C.c_uint64(), IngestStats()
+    check(lib().kc_fastq_frame_files(arr, len(paths), threads, C.byref(p), C.byref(n), C.byref(st)),
+          "kc_fastq_frame_files")
+    try:
+        data = np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_uint8)), shape=(n.value,)).copy() \
+            if n.value else np.zeros(0, np.uint8)
+    finally:
+        lib().kc_host_free(p)
+    return data, {f: getattr(st, f) for f, _ in IngestStats._fields_}
 
 
 def count_fastq_files(table: "Table", paths, threads=4, batch_bytes=0, queue_depth=0):

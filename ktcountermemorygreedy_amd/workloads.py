@@ -77,3 +77,38 @@ REPEAT_4MBP = ReadSet("repeat_4mbp", 2, 4_000_000, 12, 300_000, n_segments=500, 
                       repeat_fraction=0.30, segment_divergence=0.01)
 REPEAT_4MBP_HOT = ReadSet("repeat_4mbp_hot", 2, 4_000_000, 12, 300_000, n_segments=20, segment_len=1000,
                           repeat_fraction=0.30, segment_divergence=0.01)
+
+
+# BGZF, the blocked gzip of htslib's bgzip (SAM/BAM format specification,
+# section 4.1): gzip members of at most 65280 input bytes, each carrying its
+# own size in a "BC" extra subfield, then an empty member as end-of-file
+# marker.  Any gzip reader inflates it as one stream; kc_count_fastq_files
+# inflates its members on several threads.
+BGZF_BLOCK = 65280
+BGZF_EOF = bytes.fromhex("1f8b08040000000000ff0600424302001b0003000000000000000000")
+
+
+def bgzf_member(data, level=6) -> bytes:
+    import struct
+    import zlib
+    c = zlib.compressobj(level, zlib.DEFLATED, -15)
+    body = c.compress(data) + c.flush()
+    size = 18 + len(body) + 8
+    head = b"\x1f\x8b\x08\x04\x00\x00\x00\x00\x00\xff\x06\x00BC\x02\x00" + struct.pack("<H", size - 1)
+    return head + body + struct.pack("<II", zlib.crc32(data) & 0xFFFFFFFF, len(data))
+
+
+def write_bgzf(path, data, level=6, threads=8):
+    """Writes data (bytes or a uint8 array) to path as BGZF, compressing the
+    blocks on `threads` threads (zlib releases the GIL); returns the file size."""
+    from concurrent.futures import ThreadPoolExecutor
+    mv = memoryview(data).cast("B")
+    spans = [(i, min(i + BGZF_BLOCK, len(mv))) for i in range(0, len(mv), BGZF_BLOCK)]
+    n = 0
+    with open(path, "wb") as f, ThreadPoolExecutor(max(1, threads)) as ex:
+        for i in range(0, len(spans), 4096):  # bounded memory: 4096 blocks at a time
+            for blk in ex.map(lambda s: bgzf_member(mv[s[0]:s[1]], level), spans[i:i + 4096]):
+                f.write(blk)
+                n += len(blk)
+        f.write(BGZF_EOF)
+    return n + len(BGZF_EOF)

@@ -59,3 +59,34 @@ def test_ingest_missing_file_reports_io_error(ctx, tmp_path):
         kc.count_fastq_files(t, [str(tmp_path / "nope.fastq.gz")], threads=2, batch_bytes=1 << 20)
     assert e.value.status == kc.KC_EIO
     t.close()
+
+
+@pytest.mark.parametrize("k", [31, 63])
+def test_ingest_bgzf_single_file_parallel(ctx, tmp_path, k):
+    """One BGZF file (many member runs) counted with 8 host threads sharing
+    it, next to a plain gzip file: counts equal the oracle's over the framed
+    reads; the same file streamed by one thread counts the same."""
+    from ktcountermemorygreedy_amd import kc
+    from ktcountermemorygreedy_amd import workloads as W
+    rs = W.UNEVEN_50MBP.scaled(4000)
+    g = W.ReadSet("g", 5, 2_000_000, 13, 0).genome()
+    fq = rs.reads(fastq=True, genome=g).tobytes()
+    more = _fastq([b"ACGTTGCA" * 20, b"N" * 40, b"ACGT" * 50])
+    pa, pb = tmp_path / "a.fastq.gz", tmp_path / "b.fastq.gz"
+    W.write_bgzf(str(pa), fq, level=1, threads=4)
+    pb.write_bytes(gzip.compress(more, 1))
+    seq = kc.fastq_frame(fq) + kc.fastq_frame(more)
+    nw = O.words_for_k(k)
+    keys = O.extract(seq, k)
+    uniq, cnt = O.count(keys, nw)
+    for threads in (8, 1):
+        t = kc.Table(ctx, k, capacity=1 << 23, bloom=True, max_distinct=len(seq))
+        st = kc.count_fastq_files(t, [str(pa), str(pb)], threads=threads, batch_bytes=4 << 20, queue_depth=3)
+        print("ingest", threads, st, flush=True)
+        assert st["framed_bytes"] == len(seq) and st["input_bytes"] == len(fq) + len(more)
+        if threads > 1:
+            assert st["chunks"] > 4  # the BGZF file went in member runs
+        s = t.stats()
+        assert s["errors"] == 0 and s["kmers"] == keys.shape[0], s
+        assert np.array_equal(t.query(uniq).astype(np.uint64), cnt)
+        t.close()

@@ -28,6 +28,74 @@ def test_frame_file_plain_and_gzip(tmp_path):
     assert kc.fastq_frame_file(str(g)).tobytes() == want
 
 
+def _uneven_fastq(n_reads, seed=5):
+    """FASTQ of reads of uneven length (both short and long), so member runs
+    end in every record line and mid-line."""
+    from ktcountermemorygreedy_amd import workloads as W
+    rs = W.UNEVEN_50MBP.scaled(n_reads)
+    g = W.ReadSet("g", seed, 2_000_000, 13, 0).genome()
+    return rs.reads(fastq=True, genome=g).tobytes()
+
+
+@pytest.mark.parametrize("threads", [1, 3, 8])
+def test_bgzf_parallel_framing_equals_stream(tmp_path, threads):
+    """A BGZF file (many member runs) framed by the parallel reader equals
+    the framing of the decompressed bytes, and the stream reader's."""
+    from ktcountermemorygreedy_amd import kc
+    from ktcountermemorygreedy_amd import workloads as W
+    fq = _uneven_fastq(3000)
+    p = tmp_path / "r.fastq.gz"
+    W.write_bgzf(str(p), fq, level=1, threads=4)
+    want = O.fastq_frame(fq)
+    got, st = kc.fastq_frame_files([str(p)], threads=threads)
+    assert got.tobytes() == want
+    assert st["input_bytes"] == len(fq) and st["framed_bytes"] == len(want)
+    if threads > 1:  # the file was cut into runs of members
+        assert p.stat().st_size > 4 << 20 and st["chunks"] > 4
+    assert kc.fastq_frame_file(str(p)).tobytes() == want
+
+
+def test_bgzf_missing_final_newline_and_mixed_files(tmp_path):
+    """A BGZF file whose last record lacks its final newline, next to a plain
+    gzip and a plain file: each file's reads come out whole and in order."""
+    import gzip
+    from ktcountermemorygreedy_amd import kc
+    from ktcountermemorygreedy_amd import workloads as W
+    a = _uneven_fastq(2500, seed=7)
+    a = a[:a.rstrip(b"\n").rfind(b"\n") + 1] + b"+\n" + b"I" * 10  # cut the last quality line short, no newline
+    b = _uneven_fastq(300, seed=9)
+    c = _uneven_fastq(200, seed=11)
+    pa, pb, pc = tmp_path / "a.fq.gz", tmp_path / "b.fq.gz", tmp_path / "c.fq"
+    W.write_bgzf(str(pa), a, level=1)
+    pb.write_bytes(gzip.compress(b, 1))
+    pc.write_bytes(c)
+    got, st = kc.fastq_frame_files([str(pa), str(pb), str(pc)], threads=6)
+    want_lines = sorted(l for x in (a, b, c) for l in O.fastq_frame(x).split(b"\n")[:-1])
+    assert sorted(got.tobytes().split(b"\n")[:-1]) == want_lines
+    # file a alone keeps its order
+    got_a, _ = kc.fastq_frame_files([str(pa)], threads=6)
+    fa = O.fastq_frame(a)
+    assert got_a.tobytes() == (fa if fa.endswith(b"\n") else fa + b"\n")
+
+
+def test_bgzf_corrupt_member_is_an_error(tmp_path):
+    """A member whose data no longer matches its CRC fails with KC_EIO, as
+    gzread does; nothing hangs."""
+    from ktcountermemorygreedy_amd import kc
+    from ktcountermemorygreedy_amd import workloads as W
+    fq = _uneven_fastq(3000, seed=13)
+    p = tmp_path / "bad.fastq.gz"
+    W.write_bgzf(str(p), fq, level=1)
+    raw = bytearray(p.read_bytes())
+    # flip one byte inside a member's deflate data in the middle of the file
+    at = len(raw) // 2
+    at = raw.rfind(b"\x1f\x8b\x08\x04", 0, at) + 18 + 100
+    raw[at] ^= 0x55
+    p.write_bytes(bytes(raw))
+    with pytest.raises(kc.KCError):
+        kc.fastq_frame_files([str(p)], threads=4)
+
+
 def test_generator_is_deterministic_and_pinned():
     from ktcountermemorygreedy_amd import workloads as W
     g = load_json("count_5mbp.json")
