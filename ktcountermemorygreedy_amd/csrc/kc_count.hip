@@ -223,164 +223,205 @@ __global__ void k_fill(uint64_t *p, uint64_t n, uint64_t v)
 }
 
 // ---- export (packing helpers: kc_pack_dev.h) ---------------------------
-// Classifies slot i: 1 = high record (count in cc), 2 = low record
-// (singleton without an active FIRST list), 0 = nothing.
+// A slot's words in registers: key words, count, and for four-word keys the
+// state word (count_word / kStateWord index them as in the slot).
+template <int NW> constexpr int kSlotRegs = NW == 4 ? 6 : NW + 1;
+
 template <int NW>
-__device__ __forceinline__ int classify_slot(const TableView &tv, uint64_t i, bool singles_to_low,
-                                             uint64_t *key, uint32_t &cc)
+__device__ __forceinline__ void load_slot_regs(const TableView &tv, uint64_t i, uint64_t *r)
 {
     const uint64_t *s = tv.slots + i * slot_words<NW>();
-    if (!slot_used<NW>(s)) return 0;
+    if constexpr (NW == 1) {
+        const ulonglong2 v = *reinterpret_cast<const ulonglong2 *>(s);
+        r[0] = v.x;
+        r[1] = v.y;
+    } else if constexpr (NW == 2) {
+        const ulonglong2 a = *reinterpret_cast<const ulonglong2 *>(s);
+        r[0] = a.x;
+        r[1] = a.y;
+        r[2] = s[2];
+    } else {
 #pragma unroll
-    for (int w = 0; w < NW; ++w) key[w] = s[w];
-    const uint64_t c = s[count_word<NW>()];
+        for (int w = 0; w < kSlotRegs<NW>; ++w) r[w] = s[w];
+    }
+}
+
+// Classifies a slot: 1 = high record (count in cc), 2 = low record
+// (singleton without an active FIRST list), 0 = nothing.
+template <int NW>
+__device__ __forceinline__ int classify_regs(const TableView &tv, const uint64_t *r, bool singles_to_low,
+                                             uint32_t &cc)
+{
+    const bool used = NW == 4 ? r[kStateWord] == kSlotReady : r[0] != KC_EMPTY;
+    if (!used) return 0;
+    const uint64_t c = r[count_word<NW>()];
     if (singles_to_low && c == 1) return 2;
     if (c == 0) return 0;
     cc = (uint32_t)(c < tv.count_max ? c : tv.count_max);
     return export_filtered(cc, tv) ? 0 : 1;
 }
 
-// Block-wide exclusive scan of two per-thread counts (high, low).
-__device__ __forceinline__ void block_scan2(uint32_t &h, uint32_t &l, uint32_t &th, uint32_t &tl)
+// Exclusive scan of one value per thread of a kBlock-thread block: a shuffle
+// scan inside each wave, then the wave totals (two barriers).  The export
+// scans a tile's high and low record counts at once, packed as h | l << 16
+// (each at most kExportTile).
+__device__ __forceinline__ uint32_t block_scan_u32(uint32_t v, uint32_t *wsum, uint32_t &total)
 {
-    __shared__ uint32_t sh[kBlock], sl[kBlock];
-    sh[threadIdx.x] = h;
-    sl[threadIdx.x] = l;
-    __syncthreads();
-    for (int o = 1; o < kBlock; o <<= 1) {
-        const uint32_t ah = threadIdx.x >= (uint32_t)o ? sh[threadIdx.x - o] : 0;
-        const uint32_t al = threadIdx.x >= (uint32_t)o ? sl[threadIdx.x - o] : 0;
-        __syncthreads();
-        sh[threadIdx.x] += ah;
-        sl[threadIdx.x] += al;
-        __syncthreads();
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= (uint32_t)o) x += y;
     }
-    th = sh[kBlock - 1];
-    tl = sl[kBlock - 1];
-    h = sh[threadIdx.x] - h;
-    l = sl[threadIdx.x] - l;
+    if (lane == 63) wsum[wave] = x;
     __syncthreads();
+    uint32_t base = 0;
+    total = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < kBlock / 64; ++w) {
+        const uint32_t sw = wsum[w];
+        base += w < wave ? sw : 0;
+        total += sw;
+    }
+    __syncthreads();
+    return base + x - v;
 }
 
-// Pass 1: per-tile record counts -> tile_counts[2t], [2t+1].
+// Export scan scratch (one allocation of export_scan_words(n_tiles) words):
+// exclusive offsets of the 2 x n_tiles record counts (high counts of every
+// tile, then low counts), then the high and low totals; the u32 counts; the
+// scan's block sums.
+inline uint64_t export_scan_words(uint64_t n_tiles)
+{
+    return 2 * n_tiles + 2 + n_tiles + kc_scan_tmp_words(2 * n_tiles);
+}
+struct ExportScan {
+    uint64_t *off;    // [2 n_tiles + 2]
+    uint32_t *cnt;    // [2 n_tiles]
+    uint64_t *sums;
+};
+inline ExportScan export_scan_of(uint64_t *base, uint64_t n_tiles)
+{
+    ExportScan e;
+    e.off = base;
+    e.cnt = reinterpret_cast<uint32_t *>(base + 2 * n_tiles + 2);
+    e.sums = base + 2 * n_tiles + 2 + n_tiles;
+    return e;
+}
+
+// Pass 1: per-tile record counts -> cnt[t] (high), cnt[n_tiles + t] (low).
 template <int NW>
 __global__ __launch_bounds__(kBlock) void k_export_count(TableView tv, bool singles_to_low, uint64_t n_tiles,
-                                                         uint64_t *tile_counts)
+                                                         uint32_t *cnt)
 {
+    __shared__ uint32_t wsum[kBlock / 64];
     const uint64_t cap = tv.mask + 1;
     for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
-        uint32_t h = 0, l = 0;
-        int kind[kExportPer];
+        uint32_t hl = 0;
+        uint64_t r[kExportPer][kSlotRegs<NW>];
 #pragma unroll
         for (int j = 0; j < kExportPer; ++j) {  // every slot load issued before the first use
             const uint64_t i = tile * kExportTile + (uint64_t)j * kBlock + threadIdx.x;
-            uint64_t key[NW];
-            uint32_t cc;
-            kind[j] = i < cap ? classify_slot<NW>(tv, i, singles_to_low, key, cc) : 0;
+            if (i < cap) load_slot_regs<NW>(tv, i, r[j]);
         }
 #pragma unroll
         for (int j = 0; j < kExportPer; ++j) {
-            h += kind[j] == 1;
-            l += kind[j] == 2;
+            const uint64_t i = tile * kExportTile + (uint64_t)j * kBlock + threadIdx.x;
+            uint32_t cc;
+            const int kind = i < cap ? classify_regs<NW>(tv, r[j], singles_to_low, cc) : 0;
+            hl += kind == 1 ? 1u : kind == 2 ? 1u << 16 : 0u;
         }
-        uint32_t th, tl;
-        block_scan2(h, l, th, tl);
+        uint32_t total;
+        block_scan_u32(hl, wsum, total);
         if (threadIdx.x == 0) {
-            tile_counts[2 * tile] = th;
-            tile_counts[2 * tile + 1] = tl;
+            cnt[tile] = total & 0xFFFF;
+            cnt[n_tiles + tile] = total >> 16;
         }
     }
 }
 
-// Exclusive scan of the per-tile pairs in one block; totals land at the end.
-__global__ __launch_bounds__(1024) void k_scan_tiles(uint64_t *tile_counts, uint64_t n_tiles)
+// Totals after the scan of the counts: high = the offset of the first low
+// count, low = the last low offset + count - high.
+__global__ void k_export_totals(uint64_t *off, const uint32_t *cnt, uint64_t n_tiles)
 {
-    __shared__ uint64_t part_h[1024], part_l[1024];
-    const uint64_t per = (n_tiles + 1023) / 1024;
-    const uint64_t b = threadIdx.x * per, e = b + per < n_tiles ? b + per : n_tiles;
-    uint64_t h = 0, l = 0;
-    for (uint64_t i = b; i < e; ++i) {
-        h += tile_counts[2 * i];
-        l += tile_counts[2 * i + 1];
-    }
-    part_h[threadIdx.x] = h;
-    part_l[threadIdx.x] = l;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint64_t rh = 0, rl = 0;
-        for (int i = 0; i < 1024; ++i) {
-            const uint64_t xh = part_h[i], xl = part_l[i];
-            part_h[i] = rh;
-            part_l[i] = rl;
-            rh += xh;
-            rl += xl;
-        }
-        tile_counts[2 * n_tiles] = rh;
-        tile_counts[2 * n_tiles + 1] = rl;
-    }
-    __syncthreads();
-    h = part_h[threadIdx.x];
-    l = part_l[threadIdx.x];
-    for (uint64_t i = b; i < e; ++i) {
-        const uint64_t xh = tile_counts[2 * i], xl = tile_counts[2 * i + 1];
-        tile_counts[2 * i] = h;
-        tile_counts[2 * i + 1] = l;
-        h += xh;
-        l += xl;
-    }
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    off[2 * n_tiles] = off[n_tiles];
+    off[2 * n_tiles + 1] = off[2 * n_tiles - 1] + cnt[2 * n_tiles - 1] - off[n_tiles];
 }
 
 // Pass 2: pack records at the scanned offsets.  Low records from the table
 // (singletons) follow the n_list FIRST-list records in the low buffer.  Each
 // tile's high records, then its low records, are packed into LDS and copied
-// out as one contiguous byte range.
+// out as one contiguous byte range.  One- and two-word slots of the next tile
+// are loaded while this one is packed and written.
 template <int NW>
 __global__ __launch_bounds__(kBlock) void k_export_write(TableView tv, PackSpec ps, bool singles_to_low,
-                                                         uint64_t n_tiles, const uint64_t *tile_off,
+                                                         uint64_t n_tiles, const uint64_t *off,
                                                          uint8_t *out_high, uint8_t *out_low, uint64_t n_list)
 {
-    __shared__ uint8_t stage[export_stage<NW>()];
+    constexpr bool kPrefetch = NW <= 2;
+    __shared__ __align__(16) uint8_t stage[export_stage<NW>()];
+    __shared__ uint32_t wsum[kBlock / 64];
     const uint64_t cap = tv.mask + 1;
     const uint32_t hrec = ps.kmer_bytes + ps.count_bytes;
-    for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
-        uint64_t key[kExportPer][NW];
-        uint32_t cc[kExportPer];
-        int kind[kExportPer];
-        uint32_t h = 0, l = 0;
+    const uint64_t low0 = off[n_tiles];
+    uint64_t r[kExportPer][kSlotRegs<NW>];
+    auto load_tile = [&](uint64_t tile) {
 #pragma unroll
         for (int j = 0; j < kExportPer; ++j) {
             const uint64_t i = tile * kExportTile + (uint64_t)j * kBlock + threadIdx.x;
-            kind[j] = i < cap ? classify_slot<NW>(tv, i, singles_to_low, key[j], cc[j]) : 0;
-            h += kind[j] == 1;
-            l += kind[j] == 2;
+            if (i < cap) load_slot_regs<NW>(tv, i, r[j]);
         }
-        uint32_t th, tl;
-        block_scan2(h, l, th, tl);
+    };
+    if (kPrefetch && blockIdx.x < n_tiles) load_tile(blockIdx.x);
+    for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+        if (!kPrefetch) load_tile(tile);
+        uint64_t key[kExportPer][NW];
+        uint32_t cc[kExportPer];
+        int kind[kExportPer];
+        uint32_t hl = 0;
+#pragma unroll
+        for (int j = 0; j < kExportPer; ++j) {
+            const uint64_t i = tile * kExportTile + (uint64_t)j * kBlock + threadIdx.x;
+            kind[j] = i < cap ? classify_regs<NW>(tv, r[j], singles_to_low, cc[j]) : 0;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) key[j][w] = r[j][w];
+            hl += kind[j] == 1 ? 1u : kind[j] == 2 ? 1u << 16 : 0u;
+        }
+        if (kPrefetch && tile + gridDim.x < n_tiles) load_tile(tile + gridDim.x);
+        uint32_t total;
+        const uint32_t at0 = block_scan_u32(hl, wsum, total);
+        const uint32_t th = total & 0xFFFF, tl = total >> 16;
         // high records of the tile
-        uint32_t at = h;
+        uint8_t *dh = out_high + off[tile] * hrec;
+        uint8_t *sh = stage + stage_skew(dh);
+        uint32_t at = at0 & 0xFFFF;
 #pragma unroll
         for (int j = 0; j < kExportPer; ++j) {
             if (kind[j] == 1) {
-                uint8_t *d = stage + at * hrec;
+                uint8_t *d = sh + at * hrec;
                 pack_key<NW>(d, key[j], ps);
                 put_bytes(d + ps.kmer_bytes, cc[j], ps.count_bytes, 0);
                 ++at;
             }
         }
         __syncthreads();
-        stage_copy(out_high + tile_off[2 * tile] * hrec, stage, th * hrec);
+        stage_copy(dh, sh, th * hrec);
         __syncthreads();
         if (tl) {
-            at = l;
+            uint8_t *dl = out_low + (n_list + off[n_tiles + tile] - low0) * ps.kmer_bytes;
+            uint8_t *sl = stage + stage_skew(dl);
+            at = at0 >> 16;
 #pragma unroll
             for (int j = 0; j < kExportPer; ++j) {
                 if (kind[j] == 2) {
-                    pack_key<NW>(stage + at * ps.kmer_bytes, key[j], ps);
+                    pack_key<NW>(sl + at * ps.kmer_bytes, key[j], ps);
                     ++at;
                 }
             }
             __syncthreads();
-            stage_copy(out_low + (n_list + tile_off[2 * tile + 1]) * ps.kmer_bytes, stage, tl * ps.kmer_bytes);
+            stage_copy(dl, sl, tl * ps.kmer_bytes);
             __syncthreads();
         }
     }
@@ -392,19 +433,30 @@ __global__ __launch_bounds__(kBlock) void k_export_write(TableView tv, PackSpec 
 template <int NW>
 __global__ __launch_bounds__(kBlock) void k_export_low_list(TableView tv, PackSpec ps, uint8_t *out_low)
 {
-    __shared__ uint8_t stage[export_stage<NW>()];
+    __shared__ __align__(16) uint8_t stage[export_stage<NW>()];
     const uint32_t seg = blockIdx.y;
     const uint64_t s0 = tv.seg_off[seg], cnt = tv.seg_off[seg + 1] - s0;
     for (uint64_t r0 = blockIdx.x * kExportTile; r0 < cnt; r0 += (uint64_t)gridDim.x * kExportTile) {
         const uint32_t rn = (uint32_t)(cnt - r0 < kExportTile ? cnt - r0 : kExportTile);
+        uint8_t *d = out_low + (s0 + r0) * ps.kmer_bytes;
+        uint8_t *sd = stage + stage_skew(d);
+        uint64_t key[kExportPer][NW];
+#pragma unroll
+        for (int j = 0; j < kExportPer; ++j) {  // every row load issued before the first use
+            const uint32_t r = (uint32_t)j * kBlock + threadIdx.x;
+            if (r < rn) {
+                const uint64_t *src = tv.low + ((uint64_t)seg * tv.seg_cap + r0 + r) * NW;
+#pragma unroll
+                for (int w = 0; w < NW; ++w) key[j][w] = src[w];
+            }
+        }
 #pragma unroll
         for (int j = 0; j < kExportPer; ++j) {
             const uint32_t r = (uint32_t)j * kBlock + threadIdx.x;
-            if (r < rn)
-                pack_key<NW>(stage + r * ps.kmer_bytes, tv.low + ((uint64_t)seg * tv.seg_cap + r0 + r) * NW, ps);
+            if (r < rn) pack_key<NW>(sd + r * ps.kmer_bytes, key[j], ps);
         }
         __syncthreads();
-        stage_copy(out_low + (s0 + r0) * ps.kmer_bytes, stage, rn * ps.kmer_bytes);
+        stage_copy(d, sd, rn * ps.kmer_bytes);
         __syncthreads();
     }
 }
@@ -880,7 +932,7 @@ extern "C" int kc_table_create(kc_ctx *ctx, const kc_table_config *cfg, kc_table
     t->n_tiles = (t->cap + kExportTile - 1) / kExportTile;
     const uint64_t slot_bytes = 8ull * (t->nw == 1 ? 2 : t->nw == 2 ? 4 : 8);
     const uint64_t need = t->cap * slot_bytes + 8 * t->bloom_blocks + 8ull * t->nw * t->low_cap +
-                          8 * KC_COUNTER_WORDS + 16 * (t->n_tiles + 1);
+                          8 * KC_COUNTER_WORDS + 8 * export_scan_words(t->n_tiles);
     size_t free_b = 0, total_b = 0;
     KC_HIP(hipMemGetInfo(&free_b, &total_b));
     if (need > free_b) {
@@ -889,7 +941,7 @@ extern "C" int kc_table_create(kc_ctx *ctx, const kc_table_config *cfg, kc_table
     }
     bool ok = hipMalloc((void **)&t->slots, t->cap * slot_bytes) == hipSuccess &&
               hipMalloc((void **)&t->counters, 8 * KC_COUNTER_WORDS) == hipSuccess &&
-              hipMalloc((void **)&t->tile_counts, 16 * (t->n_tiles + 1)) == hipSuccess &&
+              hipMalloc((void **)&t->tile_counts, 8 * export_scan_words(t->n_tiles)) == hipSuccess &&
               hipMalloc((void **)&t->seg_off, 8 * (KC_LOW_SEGS + 1)) == hipSuccess;
     if (ok && t->bloom_blocks) ok = hipMalloc((void **)&t->bloom, 8 * t->bloom_blocks) == hipSuccess;
     if (ok && t->low_cap) ok = hipMalloc((void **)&t->low, 8ull * t->nw * t->low_cap) == hipSuccess;
@@ -1238,21 +1290,25 @@ extern "C" int kc_table_export_device(kc_table *t, uint8_t *d_high, uint64_t hig
     const uint64_t n_list = list ? hc.low : 0;
     const TableView v = view_of(t);
     const uint32_t g = (uint32_t)std::min<uint64_t>(t->n_tiles, (uint64_t)c->cu_count * 8);
-    if (t->nw == 1) hipLaunchKernelGGL(k_export_count<1>, dim3(g), dim3(kBlock), 0, c->stream, v, !list, t->n_tiles, t->tile_counts);
-    else if (t->nw == 2) hipLaunchKernelGGL(k_export_count<2>, dim3(g), dim3(kBlock), 0, c->stream, v, !list, t->n_tiles, t->tile_counts);
-    else hipLaunchKernelGGL(k_export_count<4>, dim3(g), dim3(kBlock), 0, c->stream, v, !list, t->n_tiles, t->tile_counts);
-    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, c->stream, t->tile_counts, t->n_tiles);
+    const uint64_t N = t->n_tiles;
+    const ExportScan es = export_scan_of(t->tile_counts, N);
+    if (t->nw == 1) hipLaunchKernelGGL(k_export_count<1>, dim3(g), dim3(kBlock), 0, c->stream, v, !list, N, es.cnt);
+    else if (t->nw == 2) hipLaunchKernelGGL(k_export_count<2>, dim3(g), dim3(kBlock), 0, c->stream, v, !list, N, es.cnt);
+    else hipLaunchKernelGGL(k_export_count<4>, dim3(g), dim3(kBlock), 0, c->stream, v, !list, N, es.cnt);
+    KC_HIP(hipGetLastError());
+    if ((rc = kc_scan_u32_on(c->stream, es.cnt, 2 * N, es.off, es.sums)) != KC_OK) return rc;
+    hipLaunchKernelGGL(k_export_totals, dim3(1), dim3(64), 0, c->stream, es.off, es.cnt, N);
     KC_HIP(hipGetLastError());
     uint64_t tot[2];
-    KC_HIP(hipMemcpyAsync(tot, t->tile_counts + 2 * t->n_tiles, 16, hipMemcpyDeviceToHost, c->stream));
+    KC_HIP(hipMemcpyAsync(tot, es.off + 2 * N, 16, hipMemcpyDeviceToHost, c->stream));
     KC_HIP(hipStreamSynchronize(c->stream));
     *high_bytes = tot[0] * hrec;
     *low_bytes = (tot[1] + n_list) * ps.kmer_bytes;
     if (!d_high || !d_low) return KC_OK;
     if (*high_bytes > high_cap || *low_bytes > low_cap) return KC_EINVAL;
-    if (t->nw == 1) hipLaunchKernelGGL(k_export_write<1>, dim3(g), dim3(kBlock), 0, c->stream, v, ps, !list, t->n_tiles, t->tile_counts, d_high, d_low, n_list);
-    else if (t->nw == 2) hipLaunchKernelGGL(k_export_write<2>, dim3(g), dim3(kBlock), 0, c->stream, v, ps, !list, t->n_tiles, t->tile_counts, d_high, d_low, n_list);
-    else hipLaunchKernelGGL(k_export_write<4>, dim3(g), dim3(kBlock), 0, c->stream, v, ps, !list, t->n_tiles, t->tile_counts, d_high, d_low, n_list);
+    if (t->nw == 1) hipLaunchKernelGGL(k_export_write<1>, dim3(g), dim3(kBlock), 0, c->stream, v, ps, !list, N, es.off, d_high, d_low, n_list);
+    else if (t->nw == 2) hipLaunchKernelGGL(k_export_write<2>, dim3(g), dim3(kBlock), 0, c->stream, v, ps, !list, N, es.off, d_high, d_low, n_list);
+    else hipLaunchKernelGGL(k_export_write<4>, dim3(g), dim3(kBlock), 0, c->stream, v, ps, !list, N, es.off, d_high, d_low, n_list);
     KC_HIP(hipGetLastError());
     if (n_list) {
         // chunks per segment: enough blocks for the fullest segment, capped

@@ -96,7 +96,7 @@ struct kc_table {
     uint64_t *low;          // low_cap * nw words, segment s at s * seg_cap
     uint64_t *seg_off;      // KC_LOW_SEGS + 1 exclusive offsets (device)
     uint64_t *counters;     // KC_COUNTER_WORDS device u64
-    uint64_t *tile_counts;  // export scan scratch: 2 per tile (+2)
+    uint64_t *tile_counts;  // export scan scratch (export_scan_words(n_tiles) words)
     uint64_t n_tiles;
     // host staging for kc_table_count_seq_host
     uint8_t *pinned[2];

@@ -70,14 +70,42 @@ constexpr int kPackBlock = 256;
 constexpr int kExportPer = 8;
 constexpr uint64_t kExportTile = (uint64_t)kPackBlock * kExportPer;
 // LDS staging of a tile's records by key words: 20 bytes hold a k <= 64 high
-// record (16 key bytes + 4 count bytes), 36 a k <= 128 one.
-template <int NW> constexpr uint32_t export_stage() { return (uint32_t)kExportTile * (NW == 4 ? 36 : 20); }
+// record (16 key bytes + 4 count bytes), 36 a k <= 128 one; 16 more for the
+// skew below.  Declare the staging buffer __align__(16).
+template <int NW> constexpr uint32_t export_stage() { return (uint32_t)kExportTile * (NW == 4 ? 36 : 20) + 16; }
 
-// Copies nbytes of LDS staging to dst (any alignment): byte stores up to a
-// 4-byte boundary, then 4-byte stores, then the tail; consecutive threads
-// take consecutive words, so every wave store is one contiguous segment.
+// Records for dst are packed into the staging buffer from byte stage_skew(dst)
+// on, so that LDS and destination bytes agree modulo 16 and stage_copy moves
+// aligned 16-byte words.
+__device__ __forceinline__ uint32_t stage_skew(const uint8_t *dst)
+{
+    return (uint32_t)(reinterpret_cast<uintptr_t>(dst) & 15);
+}
+
+// Copies nbytes of LDS staging at lds to dst.  When lds and dst agree modulo
+// 16 (lds = buffer + stage_skew(dst)): the head bytes up to a 16-byte
+// boundary, then whole 16-byte words (consecutive threads take consecutive
+// words, so every wave store is one contiguous 1 KB segment), then the tail
+// bytes.  Otherwise bytes up to a 4-byte boundary, then 4-byte stores.
 __device__ __forceinline__ void stage_copy(uint8_t *dst, const uint8_t *lds, uint32_t nbytes)
 {
+    const uintptr_t a = reinterpret_cast<uintptr_t>(dst) & 15;
+    if (a == (reinterpret_cast<uintptr_t>(lds) & 15)) {
+        const uint32_t end = (uint32_t)a + nbytes;
+        const uint32_t q0 = ((uint32_t)a + 15) / 16, q1 = end / 16;  // whole words [q0, q1) from the aligned base
+        if (q0 > q1) {  // inside one word
+            if (threadIdx.x < nbytes) dst[threadIdx.x] = lds[threadIdx.x];
+            return;
+        }
+        const uint32_t head = q0 * 16 - (uint32_t)a, tail = end - q1 * 16;
+        if (threadIdx.x < head) dst[threadIdx.x] = lds[threadIdx.x];
+        const uint32_t t0 = q1 * 16 - (uint32_t)a;  // first tail byte
+        if (threadIdx.x < tail) dst[t0 + threadIdx.x] = lds[t0 + threadIdx.x];
+        uint4 *dw = reinterpret_cast<uint4 *>(dst - a);
+        const uint4 *sw = reinterpret_cast<const uint4 *>(lds - a);
+        for (uint32_t q = q0 + threadIdx.x; q < q1; q += kPackBlock) dw[q] = sw[q];
+        return;
+    }
     const uint32_t head = (uint32_t)((4 - (reinterpret_cast<uintptr_t>(dst) & 3)) & 3);
     const uint32_t h = head < nbytes ? head : nbytes;
     if (threadIdx.x < h) dst[threadIdx.x] = lds[threadIdx.x];

@@ -66,18 +66,20 @@ template <int NW>
 __global__ __launch_bounds__(kB) void k_pack_low(const uint64_t *__restrict__ keys, uint64_t n, PackSpec ps,
                                                  uint8_t *__restrict__ out)
 {
-    __shared__ uint8_t stage[export_stage<NW>()];
+    __shared__ __align__(16) uint8_t stage[export_stage<NW>()];
     const uint64_t n_tiles = (n + kExportTile - 1) / kExportTile;
     for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
         const uint64_t t0 = tile * kExportTile;
         const uint32_t rn = (uint32_t)(n - t0 < kExportTile ? n - t0 : kExportTile);
+        uint8_t *d = out + t0 * ps.kmer_bytes;
+        uint8_t *sd = stage + stage_skew(d);
 #pragma unroll
         for (int j = 0; j < kExportPer; ++j) {
             const uint32_t r = threadIdx.x * kExportPer + j;
-            if (r < rn) pack_key<NW>(stage + r * ps.kmer_bytes, keys + (t0 + r) * NW, ps);
+            if (r < rn) pack_key<NW>(sd + r * ps.kmer_bytes, keys + (t0 + r) * NW, ps);
         }
         __syncthreads();
-        stage_copy(out + t0 * ps.kmer_bytes, stage, rn * ps.kmer_bytes);
+        stage_copy(d, sd, rn * ps.kmer_bytes);
         __syncthreads();
     }
 }
@@ -108,7 +110,7 @@ __global__ __launch_bounds__(kB) void k_pack_high(const uint64_t *__restrict__ k
                                                   uint64_t n, PackSpec ps, uint32_t fmin, uint32_t fmax,
                                                   const uint64_t *__restrict__ tile_off, uint8_t *__restrict__ out)
 {
-    __shared__ uint8_t stage[export_stage<NW>()];
+    __shared__ __align__(16) uint8_t stage[export_stage<NW>()];
     const uint32_t rec = ps.kmer_bytes + ps.count_bytes;
     const uint64_t n_tiles = (n + kExportTile - 1) / kExportTile;
     for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
@@ -125,16 +127,18 @@ __global__ __launch_bounds__(kB) void k_pack_high(const uint64_t *__restrict__ k
         }
         uint32_t total;
         uint32_t at = block_scan(mine, total);
+        uint8_t *dt = out + tile_off[tile] * rec;
+        uint8_t *st = stage + stage_skew(dt);
 #pragma unroll
         for (int j = 0; j < kExportPer; ++j) {
             if (!keep[j]) continue;
-            uint8_t *d = stage + at * rec;
+            uint8_t *d = st + at * rec;
             pack_key<NW>(d, keys + (t0 + threadIdx.x * kExportPer + j) * NW, ps);
             put_bytes(d + ps.kmer_bytes, c[j], ps.count_bytes, 0);
             ++at;
         }
         __syncthreads();
-        stage_copy(out + tile_off[tile] * rec, stage, total * rec);
+        stage_copy(dt, st, total * rec);
         __syncthreads();
     }
 }
