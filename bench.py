@@ -61,6 +61,9 @@ def parse_args():
     ap.add_argument("--passes", type=int, default=1,
                     help="also time multi-pass counting (a power of two): the same input counted in P hash-slice "
                          "passes into a table of 1/P the slots (the out-of-HBM fallback); a 'multipass' object")
+    ap.add_argument("--gz-format", default="bgzf", choices=["bgzf", "gzip"],
+                    help="ingest_bgzf: the one file as BGZF (members inflated on all host threads) or as one "
+                         "plain gzip member (one inflate stream, as gzip -6 writes it)")
     ap.add_argument("--no-host-batch", action="store_true",
                     help="skip the host-batched (PCIe-inclusive) line of uneven_50mbp")
     return ap.parse_args()
@@ -238,12 +241,14 @@ def bench_sort(args):
 
 def bench_ingest_bgzf(args):
     """End-to-end from ONE gzip FASTQ file: the whole repeat_100mbp read set
-    (default 20M reads) as a single BGZF .fastq.gz (level 6), counted by
-    kc_count_fastq_files with 16 host threads inflating its member runs in
-    parallel -- host inflate, framing, PCIe and counting all inside the timed
-    region, so it is never the headline.  Checked against the same reads
-    counted from HBM (equal histograms and k-mer totals).  The reference
-    Tree binary counts the same file on the same host beside it."""
+    (default 20M reads) as a single .fastq.gz (level 6), counted by
+    kc_count_fastq_files with 16 host threads -- host inflate, framing, PCIe
+    and counting all inside the timed region, so it is never the headline.
+    --gz-format bgzf: BGZF, whose member runs the threads inflate in
+    parallel; gzip: one plain gzip member, one inflate stream (a second thread
+    frames).  Checked against the same reads counted from HBM (equal
+    histograms and k-mer totals).  The reference Tree binary counts a sample
+    of the same format on the same host beside it."""
     import shutil
 
     import numpy as np
@@ -253,17 +258,19 @@ def bench_ingest_bgzf(args):
 
     n_reads = args.reads or W.REPEAT_100MBP.n_reads
     threads = host_threads()
-    ws = W.REPEAT_100MBP.scaled(n_reads, "repeat_100mbp_bgzf_%d" % n_reads)
+    fmt = args.gz_format
+    write = W.write_bgzf if fmt == "bgzf" else W.write_gzip_parallel
+    ws = W.REPEAT_100MBP.scaled(n_reads, "repeat_100mbp_%s_%d" % (fmt, n_reads))
     genome = ws.genome()
     t0 = time.time()
     tmp = tempfile.mkdtemp(prefix="kcbgzf_")
     path = os.path.join(tmp, "reads.fastq.gz")
     fq = ws.reads(fastq=True, genome=genome)
     fq_bytes = int(fq.nbytes)
-    gz_bytes = W.write_bgzf(path, fq, level=6, threads=threads)
+    gz_bytes = write(path, fq, level=6, threads=threads)
     del fq
     prep_s = time.time() - t0
-    print("prep: %d FASTQ bytes -> %d BGZF bytes in %.1f s" % (fq_bytes, gz_bytes, prep_s), flush=True)
+    print("prep: %d FASTQ bytes -> %d %s bytes in %.1f s" % (fq_bytes, gz_bytes, fmt, prep_s), flush=True)
 
     ctx = kc.Context(0)
     est_distinct = int(ws.genome_len * 1.1 + n_reads * 120 * 0.15)
@@ -305,7 +312,7 @@ def bench_ingest_bgzf(args):
         # a bounded sample: the first reads of the workload as their own BGZF file
         sample_reads = min(n_reads, args.cpu_sample_reads or 4_000_000)
         spath = os.path.join(tmp, "sample.fastq.gz")
-        W.write_bgzf(spath, ws.scaled(sample_reads).reads(fastq=True, genome=genome), level=6, threads=threads)
+        write(spath, ws.scaled(sample_reads).reads(fastq=True, genome=genome), level=6, threads=threads)
         out_dir = os.path.join(tmp, "out")
         os.makedirs(out_dir)
         t1 = time.time()
@@ -318,9 +325,9 @@ def bench_ingest_bgzf(args):
         if km and el:
             kmers, us = int(km.group(1)), int(el.group(1))
             cpu = {"value": kmers / (us * 1e-6), "unit": "k-mers/s", "cores": threads, "kind": "reference",
-                   "sample": "first %d reads as one BGZF file, Tree <file> 31 %d 16 <tmp> 1024 2 4294967295 "
+                   "sample": "first %d reads as one %s file, Tree <file> 31 %d 16 <tmp> 1024 2 4294967295 "
                              "4294967295 (its reader inflates the file as one gzip stream); %d k-mers, elapsed "
-                             "%.2f s (wall %.2f s)" % (sample_reads, threads, kmers, us * 1e-6, ref_wall),
+                             "%.2f s (wall %.2f s)" % (sample_reads, fmt, threads, kmers, us * 1e-6, ref_wall),
                    "host": host_info()}
         else:
             cpu = {"value": None, "error": "reference run printed no totals", "output_tail": out[-2000:]}
@@ -330,7 +337,8 @@ def bench_ingest_bgzf(args):
         "value": stats["kmers"] / wall, "unit": "k-mers/s", "n_gpus": 1, "steps": len(runs), "warmup": 0,
         "ms_per_step": wall * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "u64", "data": "synthetic",
-        "config": {"workload": ws.name, "reads": n_reads, "files": 1, "format": "BGZF", "gzip_level": 6,
+        "config": {"workload": ws.name, "reads": n_reads, "files": 1,
+                   "format": "BGZF" if fmt == "bgzf" else "gzip (one member)", "gzip_level": 6,
                    "fastq_bytes": fq_bytes, "gz_bytes": gz_bytes, "kmers": stats["kmers"], "host_threads": threads,
                    "batch_bytes": 256 << 20, "queue_depth": 3},
         "ingest": st, "roofline": None, "cpu_baseline": cpu, "prep_seconds": prep_s, "checks": checks,

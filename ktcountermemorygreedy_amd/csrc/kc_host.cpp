@@ -429,6 +429,7 @@ struct Ingest {
     std::mutex omu;  // order, the results and the error
     std::condition_variable ocv;
     std::atomic<uint64_t> next{0};
+    std::atomic<int64_t> spare_threads{0};
     int first_error = KC_OK;
     uint64_t in_bytes = 0, framed = 0, chunks = 0;
     double inflate_s = 0;
@@ -572,7 +573,36 @@ struct Ingest {
         return rc;
     }
 
-    // A whole file streamed through gzread by one thread.
+    // Frames one inflated block into buf (at used), handing whole reads to the
+    // sink once a chunk has gathered.
+    int frame_block(Framer &fr, const uint8_t *p, uint64_t r, std::vector<uint8_t> &buf, uint64_t &used,
+                    uint64_t &my_framed, uint64_t &my_chunks)
+    {
+        used += fr.feed(p, r, buf.data() + used);
+        if (used >= kIngestChunk) {
+            // hand over whole reads; a partial sequence line waits for the next block
+            const uint8_t *nl = static_cast<const uint8_t *>(memrchr(buf.data(), '\n', used));
+            const uint64_t cut = nl ? (uint64_t)(nl - buf.data()) + 1 : 0;
+            if (cut) {
+                const int err = sink(buf.data(), cut);
+                if (err != KC_OK) return err;
+            }
+            std::memmove(buf.data(), buf.data() + cut, used - cut);
+            my_framed += cut;
+            my_chunks += cut ? 1 : 0;
+            used -= cut;
+        }
+        // room for the next block (more than a chunk when one sequence line is longer)
+        if (buf.size() < used + kGzRead + (used > kIngestChunk ? kIngestChunk : 0))
+            buf.resize(used + kGzRead + (used > kIngestChunk ? kIngestChunk : 0));
+        return KC_OK;
+    }
+
+    // A whole file streamed through gzread.  With a spare thread (helper),
+    // this thread only inflates, into a ring of kRing blocks, and the helper
+    // frames the blocks in order and hands the reads to the sink, so a single
+    // gzip stream inflates back to back; otherwise one thread does both.
+    static constexpr unsigned kRing = 4;
     int run_stream(const Segment &sg, std::vector<uint8_t> &in, std::vector<uint8_t> &buf)
     {
         gzFile g = gzopen(paths[sg.file], "rb");
@@ -582,31 +612,89 @@ struct Ingest {
         uint64_t used = 0, my_in = 0, my_framed = 0, my_chunks = 0;
         double my_inflate = 0;
         int err = KC_OK;
-        for (;;) {
-            const auto a = std::chrono::steady_clock::now();
-            const int r = gzread(g, in.data(), kGzRead);
-            my_inflate += std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
-            if (r < 0) {
-                err = KC_EIO;
-                break;
+        bool helper = false;
+        if (spare_threads.fetch_sub(1) > 0) helper = true;
+        else spare_threads.fetch_add(1);
+        if (helper) {
+            struct Ring {
+                std::mutex mu;
+                std::condition_variable cv;
+                std::vector<std::vector<uint8_t>> blk;
+                int len[kRing] = {};
+                uint64_t head = 0, tail = 0;  // blocks inflated / framed
+                bool done = false;
+                int err = KC_OK;
+            } ring;
+            std::thread th;
+            try {
+                ring.blk.assign(kRing, std::vector<uint8_t>(kGzRead));
+                th = std::thread([&]() {
+                    int e = KC_OK;
+                    try {
+                        for (;;) {
+                            std::unique_lock<std::mutex> lk(ring.mu);
+                            ring.cv.wait(lk, [&] { return ring.tail < ring.head || ring.done || ring.err; });
+                            if (ring.err || ring.tail == ring.head) break;  // failed, or inflated and framed
+                            const unsigned b = (unsigned)(ring.tail % kRing);
+                            const int r = ring.len[b];
+                            lk.unlock();
+                            e = frame_block(fr, ring.blk[b].data(), (uint64_t)r, buf, used, my_framed, my_chunks);
+                            lk.lock();
+                            ++ring.tail;
+                            if (e != KC_OK) ring.err = e;
+                            ring.cv.notify_all();
+                            if (e != KC_OK) break;
+                        }
+                    } catch (...) {
+                        std::lock_guard<std::mutex> lk(ring.mu);
+                        ring.err = KC_ENOMEM;
+                        ring.cv.notify_all();
+                    }
+                });
+            } catch (...) {
+                helper = false;
             }
-            if (r == 0) break;
-            my_in += (uint64_t)r;
-            used += fr.feed(in.data(), (uint64_t)r, buf.data() + used);
-            if (used >= kIngestChunk) {
-                // hand over whole reads; a partial sequence line waits for the next block
-                const uint8_t *nl = static_cast<const uint8_t *>(memrchr(buf.data(), '\n', used));
-                const uint64_t cut = nl ? (uint64_t)(nl - buf.data()) + 1 : 0;
-                if (cut) err = sink(buf.data(), cut);
-                if (err != KC_OK) break;
-                std::memmove(buf.data(), buf.data() + cut, used - cut);
-                my_framed += cut;
-                my_chunks += cut ? 1 : 0;
-                used -= cut;
+            if (helper) {
+                for (;;) {
+                    unsigned b;
+                    {
+                        std::unique_lock<std::mutex> lk(ring.mu);
+                        ring.cv.wait(lk, [&] { return ring.head - ring.tail < kRing || ring.err; });
+                        if (ring.err) break;
+                        b = (unsigned)(ring.head % kRing);
+                    }
+                    const auto a = std::chrono::steady_clock::now();
+                    const int r = gzread(g, ring.blk[b].data(), kGzRead);
+                    my_inflate += std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
+                    std::lock_guard<std::mutex> lk(ring.mu);
+                    if (r < 0) ring.err = KC_EIO;
+                    else if (r == 0) ring.done = true;
+                    else {
+                        my_in += (uint64_t)r;
+                        ring.len[b] = r;
+                        ++ring.head;
+                    }
+                    ring.cv.notify_all();
+                    if (r <= 0) break;
+                }
+                th.join();
+                err = ring.err;
             }
-            // room for the next block (more than a chunk when one sequence line is longer)
-            if (buf.size() < used + kGzRead + (used > kIngestChunk ? kIngestChunk : 0))
-                buf.resize(used + kGzRead + (used > kIngestChunk ? kIngestChunk : 0));
+            spare_threads.fetch_add(1);
+        }
+        if (!helper) {
+            for (;;) {
+                const auto a = std::chrono::steady_clock::now();
+                const int r = gzread(g, in.data(), kGzRead);
+                my_inflate += std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
+                if (r < 0) {
+                    err = KC_EIO;
+                    break;
+                }
+                if (r == 0) break;
+                my_in += (uint64_t)r;
+                if ((err = frame_block(fr, in.data(), (uint64_t)r, buf, used, my_framed, my_chunks)) != KC_OK) break;
+            }
         }
         gzclose(g);
         if (err == KC_OK && used) {
@@ -671,6 +759,8 @@ struct Ingest {
         };
         const uint32_t nt = std::max<uint32_t>(
             1, std::min<uint64_t>(threads ? threads : 1, std::max<uint64_t>(segs.size(), 1)));
+        // threads beyond one per segment frame streamed files beside their inflating thread
+        spare_threads = (int64_t)(threads ? threads : 1) - (int64_t)nt;
         std::vector<std::thread> pool;
         try {
             pool.reserve(nt);

@@ -112,3 +112,33 @@ def write_bgzf(path, data, level=6, threads=8):
                 n += len(blk)
         f.write(BGZF_EOF)
     return n + len(BGZF_EOF)
+
+
+def write_gzip_parallel(path, data, level=6, threads=8, block=4 << 20):
+    """Writes data as ONE gzip member (a single deflate stream, as gzip -6
+    writes it, so readers must inflate it in order), compressing 4 MB blocks
+    on `threads` threads the way pigz does: each block primed with the 32 KB
+    before it as dictionary and ended by a sync flush, the last one by the
+    final flush; the CRC-32 runs over the whole input.  Returns the file size."""
+    import struct
+    import zlib
+    from concurrent.futures import ThreadPoolExecutor
+    mv = memoryview(data).cast("B")
+    n = len(mv)
+    spans = [(i, min(i + block, n)) for i in range(0, n, block)] or [(0, 0)]
+
+    def comp(sp):
+        a, b = sp
+        c = zlib.compressobj(level, zlib.DEFLATED, -15, 9, zlib.Z_DEFAULT_STRATEGY,
+                             *([bytes(mv[max(0, a - 32768):a])] if a else []))
+        return c.compress(mv[a:b]) + c.flush(zlib.Z_FINISH if b == n else zlib.Z_SYNC_FLUSH)
+
+    size = 0
+    with open(path, "wb") as f, ThreadPoolExecutor(max(1, threads)) as ex:
+        f.write(b"\x1f\x8b\x08\x00\x00\x00\x00\x00\x00\xff")
+        for i in range(0, len(spans), 64):  # bounded memory: 64 blocks at a time
+            for part in ex.map(comp, spans[i:i + 64]):
+                f.write(part)
+                size += len(part)
+        f.write(struct.pack("<II", zlib.crc32(mv) & 0xFFFFFFFF, n & 0xFFFFFFFF))
+    return size + 18

@@ -78,6 +78,29 @@ def test_bgzf_missing_final_newline_and_mixed_files(tmp_path):
     assert got_a.tobytes() == (fa if fa.endswith(b"\n") else fa + b"\n")
 
 
+@pytest.mark.parametrize("threads", [1, 2, 5])
+def test_gzip_stream_with_framing_helper(tmp_path, threads):
+    """A single-member gzip file (no members to split) with spare threads:
+    one thread inflates into a ring of blocks while a helper frames them; the
+    framed reads equal the stream framing, also without a final newline and
+    next to a second file that takes a thread of its own."""
+    import gzip
+    from ktcountermemorygreedy_amd import kc
+    a = _uneven_fastq(1500, seed=21)
+    a = a[:-1]  # no final newline
+    b = _uneven_fastq(200, seed=23)
+    pa, pb = tmp_path / "a.fastq.gz", tmp_path / "b.fastq.gz"
+    pa.write_bytes(gzip.compress(a, 1))
+    pb.write_bytes(gzip.compress(b, 1))
+    got, st = kc.fastq_frame_files([str(pa)], threads=threads)
+    want = O.fastq_frame(a)
+    assert got.tobytes() == (want if want.endswith(b"\n") else want + b"\n")
+    assert st["input_bytes"] == len(a)
+    got2, _ = kc.fastq_frame_files([str(pa), str(pb)], threads=threads)
+    lines = sorted(got2.tobytes().split(b"\n")[:-1])
+    assert lines == sorted(O.fastq_frame(a).split(b"\n")[:-1] + O.fastq_frame(b).split(b"\n")[:-1])
+
+
 def test_bgzf_corrupt_member_is_an_error(tmp_path):
     """A member whose data no longer matches its CRC fails with KC_EIO, as
     gzread does; nothing hangs."""
