@@ -106,3 +106,48 @@ def test_synth_records_sort_and_histogram_properties(ctx):
     pos = {int(x): i for i, x in enumerate(k0[:1000])}
     m = {int(x): int(cc) for x, cc in zip(k1, after[:, 2]) if int(x) in pos}
     assert all(m[int(x)] == int(before[i, 2]) for x, i in pos.items())
+
+
+def _sorted_stably(keys, counts, out_keys, out_counts, k, cpb=10):
+    """The output holds the input's (key, count) pairs, ordered by sort-key
+    bits [2(32-k), 2(32-cpb)) of word 0 (k <= 32), equal sort keys in input
+    order (counts are the input positions)."""
+    lo, hi = 64 - 2 * k, 64 - 2 * cpb
+    sk = (out_keys[:, 0] >> np.uint64(lo)) & np.uint64((1 << (hi - lo)) - 1)
+    assert np.all(sk[1:] >= sk[:-1])
+    eq = sk[1:] == sk[:-1]
+    assert np.all(out_counts[1:][eq] > out_counts[:-1][eq])
+    assert np.array_equal(keys[out_counts, 0], out_keys[:, 0])
+
+
+def test_sort_keys_concentrated_on_few_prefixes(ctx):
+    """Half of 300k keys on eight values of the top sort-key bits (digits
+    with ~19k records in every tile's run) and the rest spread, with equal
+    keys: equal to the oracle."""
+    rng = np.random.default_rng(77)
+    n = 300_000
+    keys = rng.integers(0, 2**62, size=(n, 1), dtype=np.uint64) << np.uint64(2)
+    hot = rng.random(n) < 0.5
+    keys[hot, 0] = (keys[hot, 0] & np.uint64((1 << 40) - 1)) | np.uint64(0x5A5 << 52)
+    keys[n // 2: n // 2 + 5000] = keys[:5000]  # equal keys: stability
+    counts = np.arange(n, dtype=np.uint32)
+    want = O.sort_records12(keys, counts, 31, 10)
+    got = device_sort(ctx, keys, counts, 1, 31, 12, 12)
+    assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])
+
+
+def test_sort_large_skewed_prefixes(ctx):
+    """2^24 keys whose sort-key bits [32, 44) take 1100 of their 4096 values:
+    sorted, stable, the same (key, count) pairs."""
+    rng = np.random.default_rng(78)
+    n = 1 << 24
+    # sort-key bits [32, 44) on 1100 of their 4096 values (~15k records
+    # each); bits [2, 32) and [44, 64) random
+    pv = rng.choice(4096, size=1100, replace=False).astype(np.uint64)
+    pre = pv[rng.integers(0, 1100, size=n)] << np.uint64(32)
+    low = rng.integers(0, 2**30, size=n, dtype=np.uint64) << np.uint64(2)
+    top = rng.integers(0, 2**20, size=n, dtype=np.uint64) << np.uint64(44)
+    keys = (top | pre | low).reshape(n, 1)
+    counts = np.arange(n, dtype=np.uint32)
+    got = device_sort(ctx, keys, counts, 1, 31, 12, 12)
+    _sorted_stably(keys, counts, got[0], got[1], 31)
