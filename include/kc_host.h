@@ -4,7 +4,8 @@
  * FASTQ framing keeps line 2 of every 4-line record followed by '\n', the
  * byte stream the reference's reader hands to its parsers
  * (FastqReader.h:478-487, 657-687); gzip input is detected by the ".gz"
- * suffix as in FastqReader.h:423,575 and inflated with zlib on the host.
+ * suffix as in FastqReader.h:423,575 (any other file is read as it is, also
+ * when it holds gzip bytes) and inflated with zlib on the host.
  *
  * The synthetic generator has no counterpart in the reference; it produces
  * the seeded workloads the engine is measured and checked on.

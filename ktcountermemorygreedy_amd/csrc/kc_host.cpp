@@ -66,6 +66,46 @@ struct Framer {
     }
 };
 
+bool has_gz_suffix(const char *path)
+{
+    const size_t n = std::strlen(path);
+    return n >= 3 && std::strcmp(path + n - 3, ".gz") == 0;
+}
+
+// An input file as the reference's reader opens it (FastqReader.h:423,575):
+// a name ending in ".gz" is inflated with zlib (gzread passes plain bytes
+// through as well), any other file is read as it is -- gzip bytes in a file
+// without the suffix are framed as text, as there.
+struct InFile {
+    gzFile g = nullptr;
+    FILE *f = nullptr;
+    InFile() = default;
+    InFile(const InFile &) = delete;
+    InFile &operator=(const InFile &) = delete;
+    bool open(const char *path)
+    {
+        if (has_gz_suffix(path)) {
+            g = gzopen(path, "rb");
+            if (g) gzbuffer(g, 1u << 20);
+            return g != nullptr;
+        }
+        f = std::fopen(path, "rb");
+        return f != nullptr;
+    }
+    // bytes read (0 at the end), or -1 on an error
+    int read(uint8_t *p, unsigned n)
+    {
+        if (g) return gzread(g, p, n);
+        const size_t r = std::fread(p, 1, n, f);
+        return r == 0 && std::ferror(f) ? -1 : (int)r;
+    }
+    ~InFile()
+    {
+        if (g) gzclose(g);
+        if (f) std::fclose(f);
+    }
+};
+
 }  // namespace
 
 extern "C" uint64_t kc_fastq_frame(const uint8_t *fq, uint64_t len, uint8_t *out)
@@ -80,24 +120,16 @@ extern "C" int kc_fastq_frame_file(const char *path, uint8_t **out, uint64_t *ou
 {
     return kc_guard([&]() -> int {
         if (!path || !out || !out_len) return KC_EINVAL;
-        const std::string f(path);
-        const bool gz = f.size() >= 3 && f.compare(f.size() - 3, 3, ".gz") == 0;
         std::vector<uint8_t> in(1u << 22);
-        gzFile g = gzopen(path, "rb");  // reads plain files transparently too
-        if (!g) return KC_EIO;
-        gzbuffer(g, 1u << 20);
-        (void)gz;
+        InFile g;
+        if (!g.open(path)) return KC_EIO;
         uint64_t cap = 1ull << 24, used = 0;
         uint8_t *buf = static_cast<uint8_t *>(std::malloc(cap));
-        if (!buf) {
-            gzclose(g);
-            return KC_ENOMEM;
-        }
+        if (!buf) return KC_ENOMEM;
         Framer fr;
         for (;;) {
-            const int r = gzread(g, in.data(), (unsigned)in.size());
+            const int r = g.read(in.data(), (unsigned)in.size());
             if (r < 0) {
-                gzclose(g);
                 std::free(buf);
                 return KC_EIO;
             }
@@ -106,7 +138,6 @@ extern "C" int kc_fastq_frame_file(const char *path, uint8_t **out, uint64_t *ou
                 while (used + (uint64_t)r > cap) cap *= 2;
                 uint8_t *nb = static_cast<uint8_t *>(std::realloc(buf, cap));
                 if (!nb) {
-                    gzclose(g);
                     std::free(buf);
                     return KC_ENOMEM;
                 }
@@ -114,7 +145,6 @@ extern "C" int kc_fastq_frame_file(const char *path, uint8_t **out, uint64_t *ou
             }
             used += fr.feed(in.data(), (uint64_t)r, buf + used);
         }
-        gzclose(g);
         *out = buf;
         *out_len = used;
         return KC_OK;
@@ -315,9 +345,9 @@ extern "C" int64_t kc_synth_reads(const uint8_t *genome, uint64_t glen, const kc
 
 // ---- streaming ingest: FASTQ(.gz) files -> framed chunks -> a sink --------
 // Files are cut into segments that host threads take in turn.  A plain or
-// single-member gzip file is one segment: its thread inflates it with zlib
-// (gzread reads plain files too, as FastqReader.h:575-616 does for ".gz")
-// and frames it (line 2 of every record + '\n') as it goes.  A BGZF file
+// single-member gzip file is one segment: its thread reads it (a ".gz" name
+// inflated with zlib, as FastqReader.h:423,575-616 does) and frames it (line
+// 2 of every record + '\n') as it goes.  A BGZF file
 // (htslib's blocked gzip: a chain of gzip members whose "BC" extra subfield
 // gives each member's size, so members are found without inflating) is cut
 // into runs of members that threads inflate in parallel; the record phase at
@@ -458,7 +488,7 @@ struct Ingest {
             if (fseeko(f, 0, SEEK_END) == 0) size = (uint64_t)ftello(f);
             std::vector<Member> m;
             // only worth the walk when there are threads to share the file
-            const bool bgzf = threads > 1 && size > kBgzfRun && bgzf_members(f, size, m);
+            const bool bgzf = threads > 1 && size > kBgzfRun && has_gz_suffix(paths[fi]) && bgzf_members(f, size, m);
             std::fclose(f);
             if (!bgzf) {
                 segs.push_back({fi, 0, true, false, 0, 0});
@@ -598,16 +628,15 @@ struct Ingest {
         return KC_OK;
     }
 
-    // A whole file streamed through gzread.  With a spare thread (helper),
+    // A whole file streamed (InFile).  With a spare thread (helper),
     // this thread only inflates, into a ring of kRing blocks, and the helper
     // frames the blocks in order and hands the reads to the sink, so a single
     // gzip stream inflates back to back; otherwise one thread does both.
     static constexpr unsigned kRing = 4;
     int run_stream(const Segment &sg, std::vector<uint8_t> &in, std::vector<uint8_t> &buf)
     {
-        gzFile g = gzopen(paths[sg.file], "rb");
-        if (!g) return KC_EIO;
-        gzbuffer(g, kGzRead);
+        InFile g;
+        if (!g.open(paths[sg.file])) return KC_EIO;
         Framer fr;
         uint64_t used = 0, my_in = 0, my_framed = 0, my_chunks = 0;
         double my_inflate = 0;
@@ -664,7 +693,7 @@ struct Ingest {
                         b = (unsigned)(ring.head % kRing);
                     }
                     const auto a = std::chrono::steady_clock::now();
-                    const int r = gzread(g, ring.blk[b].data(), kGzRead);
+                    const int r = g.read(ring.blk[b].data(), kGzRead);
                     my_inflate += std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
                     std::lock_guard<std::mutex> lk(ring.mu);
                     if (r < 0) ring.err = KC_EIO;
@@ -685,7 +714,7 @@ struct Ingest {
         if (!helper) {
             for (;;) {
                 const auto a = std::chrono::steady_clock::now();
-                const int r = gzread(g, in.data(), kGzRead);
+                const int r = g.read(in.data(), kGzRead);
                 my_inflate += std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
                 if (r < 0) {
                     err = KC_EIO;
@@ -696,7 +725,6 @@ struct Ingest {
                 if ((err = frame_block(fr, in.data(), (uint64_t)r, buf, used, my_framed, my_chunks)) != KC_OK) break;
             }
         }
-        gzclose(g);
         if (err == KC_OK && used) {
             err = sink(buf.data(), used);  // the file's last read (newline added if missing)
             my_framed += used;

@@ -119,6 +119,25 @@ def test_bgzf_corrupt_member_is_an_error(tmp_path):
         kc.fastq_frame_files([str(p)], threads=4)
 
 
+def test_gzip_detected_by_suffix_as_the_reader_does(tmp_path):
+    """FastqReader.h:423,575: a ".gz" name is inflated, any other file is read
+    as it is (gzip bytes without the suffix are framed as text), in both
+    host readers."""
+    import gzip
+    from ktcountermemorygreedy_amd import kc
+    fq = read_golden("tiny_mixed.fastq")
+    z = gzip.compress(fq)
+    p = tmp_path / "no_suffix.fastq"
+    p.write_bytes(z)
+    want = O.fastq_frame(z)
+    assert kc.fastq_frame_file(str(p)).tobytes() == want
+    got, _ = kc.fastq_frame_files([str(p)], threads=2)
+    assert got.tobytes() == (want if not want or want.endswith(b"\n") else want + b"\n")
+    g = tmp_path / "plain_named.fastq.gz"
+    g.write_bytes(fq)  # a plain file with the suffix: zlib passes it through
+    assert kc.fastq_frame_file(str(g)).tobytes() == O.fastq_frame(fq)
+
+
 def test_generator_is_deterministic_and_pinned():
     from ktcountermemorygreedy_amd import workloads as W
     g = load_json("count_5mbp.json")
