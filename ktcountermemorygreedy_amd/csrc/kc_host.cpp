@@ -1,5 +1,8 @@
 // Host ingest (FASTQ framing, zlib inflate, streaming multi-file ingest into
 // batches) and the seeded synthetic workload generator.
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <unistd.h>
 #include <zlib.h>
 
 #include <algorithm>
@@ -10,6 +13,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <string_view>
@@ -19,6 +23,7 @@
 #include "../../include/kc_api.h"
 #include "../../include/kc_host.h"
 #include "kc_guard.h"
+#include "kc_inflate.h"
 
 namespace {
 
@@ -414,21 +419,90 @@ bool bgzf_members(FILE *f, uint64_t file_size, std::vector<Member> &out)
     return !out.empty();
 }
 
+enum SegmentKind : uint8_t { kStreamed = 0, kBgzfMembers = 1, kGzipChunk = 2 };
+
 struct Segment {
     uint32_t file;
     uint32_t index;   // among the file's segments
     bool last;        // the file's last segment
-    bool bgzf;        // members [m0, m1) of the file; else the whole file, streamed
+    uint8_t kind;     // kBgzfMembers: members [m0, m1); kGzipChunk: chunk `index`; kStreamed: the whole file
     uint32_t m0, m1;
 };
 
-// Hand-over state of one BGZF file: the next segment whose phase and whose
-// framed bytes are due, the file's newlines so far, the partial line carried.
+// Hand-over state of a file cut into segments: the next segment whose phase
+// and whose framed bytes are due, the file's newlines so far, the partial
+// line carried.
 struct FileOrder {
     uint32_t next_phase = 0, next_hand = 0;
     uint64_t lines = 0;
     std::vector<uint8_t> carry;
 };
+
+// One gzip member inflated in chunks on several threads (kc_inflate.h).
+// The deflate stream is cut into equal shares; chunk 0 decodes from the
+// stream's start, chunk i > 0 from the first dynamic block it finds in its
+// share (one that decodes to its end as text with the 32 KB before it
+// unknown).  Every chunk decodes block by block until a block starts in the
+// next share (or the member ends).  In chunk order, each chunk's start is
+// checked against where the output before it ends: equal, its markers are
+// answered by the last 32 KB of that output; different (the guess was no
+// block of the stream, or nothing was found), the chunk decodes its share
+// again from there with that window.  So the output is the stream's, byte
+// for byte, and the member's CRC-32 and size are checked as gzread checks
+// them; members after the first go through zlib in order.
+constexpr uint64_t kGzipChunkMin = 64ull << 20;  // compressed bytes from which a member is cut
+
+// Test-only override (KC_TEST_GZIP_CHUNK=<bytes>): files from that size are
+// cut into shares of that size, so small test files take this path.
+uint64_t test_gzip_chunk()
+{
+    const char *e = std::getenv("KC_TEST_GZIP_CHUNK");
+    return e ? std::strtoull(e, nullptr, 10) : 0;
+}
+
+struct GzipFile {
+    const uint8_t *map = nullptr;
+    uint64_t size = 0;
+    uint64_t d0 = 0;     // first byte of the deflate stream
+    uint64_t chunk = 0;  // compressed bytes per share
+    uint32_t n_chunks = 0;
+    // in chunk order (Ingest::omu): the next chunk due, where the output so
+    // far ends (bit), whether the member ended, the output's last <= 32 KB,
+    // the CRC-32 and size of the output so far
+    uint32_t next_window = 0;
+    uint64_t stop_bit = 0;
+    bool ended = false;
+    std::vector<uint8_t> window;
+    uint32_t crc = 0;
+    uint64_t isize = 0;
+    GzipFile() = default;
+    GzipFile(const GzipFile &) = delete;
+    GzipFile &operator=(const GzipFile &) = delete;
+    ~GzipFile()
+    {
+        if (map) munmap(const_cast<uint8_t *>(map), size);
+    }
+};
+
+// The first byte of the deflate stream after a gzip header at p, 0 if p
+// holds no gzip header.
+uint64_t gzip_header_end(const uint8_t *p, uint64_t n)
+{
+    if (n < 18 || p[0] != 0x1f || p[1] != 0x8b || p[2] != 8 || (p[3] & 0xE0)) return 0;
+    const uint8_t flg = p[3];
+    uint64_t o = 10;
+    if (flg & 4) {
+        if (o + 2 > n) return 0;
+        o += 2 + (uint64_t)(p[o] | (p[o + 1] << 8));
+    }
+    for (uint8_t bit : {(uint8_t)8, (uint8_t)16})  // file name, comment: zero-terminated
+        if (flg & bit) {
+            while (o < n && p[o]) ++o;
+            ++o;
+        }
+    if (flg & 2) o += 2;
+    return o < n ? o : 0;
+}
 
 // Frames p[0, n) in place (the framed bytes never pass the bytes read).
 uint64_t frame_in_place(uint8_t *p, uint64_t n, uint32_t state)
@@ -454,6 +528,7 @@ struct Ingest {
     uint32_t n_paths;
     Sink sink;
     std::vector<std::vector<Member>> members;  // per file (empty: streamed)
+    std::vector<std::unique_ptr<GzipFile>> gzip;  // per file cut into gzip shares
     std::vector<Segment> segs;
     std::vector<FileOrder> order;
     std::mutex omu;  // order, the results and the error
@@ -476,11 +551,36 @@ struct Ingest {
         return first_error != KC_OK;
     }
 
-    // Plans the segments: member runs of BGZF files, one per other file.
+    // A large gzip file: mapped, its deflate stream cut into shares of 4 to
+    // 32 MB (about four per thread).  False when it has no gzip header.
+    bool plan_gzip_chunks(uint32_t fi, uint64_t size, uint32_t threads, uint64_t tchunk)
+    {
+        const int fd = ::open(paths[fi], O_RDONLY);
+        if (fd < 0) return false;
+        void *m = mmap(nullptr, size, PROT_READ, MAP_PRIVATE, fd, 0);
+        ::close(fd);
+        if (m == MAP_FAILED) return false;
+        auto g = std::make_unique<GzipFile>();
+        g->map = static_cast<const uint8_t *>(m);
+        g->size = size;
+        g->d0 = gzip_header_end(g->map, size);
+        if (!g->d0) return false;
+        const uint64_t body = size - g->d0;
+        g->chunk = tchunk ? tchunk : std::max<uint64_t>(4ull << 20, std::min<uint64_t>(32ull << 20, body / (4ull * threads)));
+        g->n_chunks = (uint32_t)std::max<uint64_t>(1, (body + g->chunk - 1) / g->chunk);
+        g->stop_bit = g->d0 * 8;
+        for (uint32_t i = 0; i < g->n_chunks; ++i) segs.push_back({fi, i, i + 1 == g->n_chunks, kGzipChunk, 0, 0});
+        gzip[fi] = std::move(g);
+        return true;
+    }
+
+    // Plans the segments: member runs of BGZF files, shares of large gzip
+    // files, one per other file.
     int plan(uint32_t threads)
     {
         members.resize(n_paths);
         order.resize(n_paths);
+        gzip.resize(n_paths);
         for (uint32_t fi = 0; fi < n_paths; ++fi) {
             FILE *f = std::fopen(paths[fi], "rb");
             if (!f) return KC_EIO;
@@ -488,10 +588,14 @@ struct Ingest {
             if (fseeko(f, 0, SEEK_END) == 0) size = (uint64_t)ftello(f);
             std::vector<Member> m;
             // only worth the walk when there are threads to share the file
-            const bool bgzf = threads > 1 && size > kBgzfRun && has_gz_suffix(paths[fi]) && bgzf_members(f, size, m);
+            const bool gz = threads > 1 && has_gz_suffix(paths[fi]);
+            const bool bgzf = gz && size > kBgzfRun && bgzf_members(f, size, m);
             std::fclose(f);
+            const uint64_t tchunk = test_gzip_chunk();
+            if (!bgzf && gz && size >= (tchunk ? tchunk : kGzipChunkMin) && plan_gzip_chunks(fi, size, threads, tchunk))
+                continue;
             if (!bgzf) {
-                segs.push_back({fi, 0, true, false, 0, 0});
+                segs.push_back({fi, 0, true, kStreamed, 0, 0});
                 continue;
             }
             uint32_t idx = 0;
@@ -499,7 +603,7 @@ struct Ingest {
                 uint64_t bytes = 0;
                 uint32_t b = a;
                 while (b < m.size() && (b == a || bytes + m[b].size <= kBgzfRun)) bytes += m[b++].size;
-                segs.push_back({fi, idx++, b == m.size(), true, a, b});
+                segs.push_back({fi, idx++, b == m.size(), kBgzfMembers, a, b});
                 a = b;
             }
             members[fi] = std::move(m);
@@ -738,9 +842,201 @@ struct Ingest {
         return err;
     }
 
+    // Members after the first of a gzip file cut into shares: inflated in
+    // order with zlib (gzread reads concatenated members too, and stops at
+    // bytes that start no gzip header), framed on from the file's line phase
+    // and handed over.  omu held (file order).
+    int rest_members(const GzipFile &g, uint64_t from, FileOrder &fo, uint64_t &n_in, uint64_t &n_framed,
+                     uint64_t &n_chunks)
+    {
+        z_stream z{};
+        if (inflateInit2(&z, 16 + MAX_WBITS) != Z_OK) return KC_ENOMEM;
+        std::vector<uint8_t> out(kGzRead), framed(kGzRead);
+        Framer fr;
+        fr.state = (uint32_t)(fo.lines & 3);
+        int rc = KC_OK;
+        uint64_t at = from;
+        while (rc == KC_OK && at + 18 <= g.size && g.map[at] == 0x1f && g.map[at + 1] == 0x8b) {
+            inflateReset(&z);
+            z.next_in = const_cast<Bytef *>(g.map + at);
+            z.avail_in = (uInt)std::min<uint64_t>(g.size - at, 1u << 30);
+            int zr = Z_OK;
+            while (zr == Z_OK) {
+                z.next_out = out.data();
+                z.avail_out = (uInt)out.size();
+                zr = inflate(&z, Z_NO_FLUSH);
+                if (zr != Z_OK && zr != Z_STREAM_END) {
+                    rc = KC_EIO;
+                    break;
+                }
+                const uint64_t got = out.size() - z.avail_out;
+                n_in += got;
+                const uint64_t w = fr.feed(out.data(), got, framed.data());
+                n_framed += w;
+                rc = hand(fo, framed.data(), w, false, n_chunks);
+                if (rc != KC_OK) break;
+                if (zr == Z_OK && z.avail_in == 0) {
+                    if (at + (uint64_t)(z.next_in - (g.map + at)) >= g.size) {
+                        rc = KC_EIO;  // a member cut short
+                        break;
+                    }
+                    z.avail_in = (uInt)std::min<uint64_t>(g.size - (uint64_t)(z.next_in - g.map), 1u << 30);
+                }
+            }
+            at = (uint64_t)(z.next_in - g.map);
+        }
+        inflateEnd(&z);
+        return rc;
+    }
+
+    // Share i of a gzip member (GzipFile): speculative decode, then in chunk
+    // order the start check (and a redo from the known output if it fails),
+    // the resolve of the symbols to bytes, the record phase, framing, and
+    // the hand-over in file order.
+    int run_gzip_chunk(const Segment &sg, kcz::Out16 &o, std::vector<uint8_t> &prev)
+    {
+        GzipFile &g = *gzip[sg.file];
+        FileOrder &fo = order[sg.file];
+        const uint32_t i = sg.index;
+        const uint64_t total_bits = g.size * 8;
+        const uint64_t sbit = (g.d0 + (uint64_t)i * g.chunk) * 8;
+        const uint64_t nbit = i + 1 < g.n_chunks ? (g.d0 + (uint64_t)(i + 1) * g.chunk) * 8 : UINT64_MAX;
+        const auto a = std::chrono::steady_clock::now();
+        kcz::Decoder dec;
+        dec.in.base = g.map;
+        dec.in.nbytes = g.size;
+        o.n = 0;
+        uint64_t start = sbit, stop = 0;
+        bool ok = true, fin = false;
+        if (i > 0) {
+            const uint64_t lim = std::min(nbit, total_bits);
+            start = kcz::find_block_start(g.map, g.size, sbit, lim);
+            ok = start < lim;
+            dec.unknown_window = true;
+        }
+        // blocks from `from` until one starts in the next share or the member ends
+        auto decode_from = [&](uint64_t from) {
+            dec.in.pos = from;
+            for (;;) {
+                if (dec.in.pos >= nbit) {
+                    stop = dec.in.pos;
+                    return true;
+                }
+                const int r = dec.block(o);
+                if (r == kcz::kBlockFinal) {
+                    fin = true;
+                    stop = dec.in.pos;
+                    return true;
+                }
+                if (r != kcz::kBlockOk) return false;
+            }
+        };
+        if (ok) ok = decode_from(start);
+        double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
+        uint64_t skip = 0;  // leading symbols of o that are the window (a redo)
+        bool dead = false;
+        {
+            std::unique_lock<std::mutex> lk(omu);
+            ocv.wait(lk, [&] { return g.next_window == i || first_error != KC_OK; });
+            if (first_error != KC_OK) return first_error;
+            if (g.ended) {
+                dead = true;  // the member ended in an earlier share (more members follow it)
+            } else {
+                if (!ok || start != g.stop_bit) {
+                    // no block of the stream starts there: this share again,
+                    // from where the output ends, with its window known
+                    const auto b = std::chrono::steady_clock::now();
+                    o.n = 0;
+                    o.reserve_more(g.window.size());
+                    for (uint8_t c : g.window) o.v[o.n++] = c;
+                    skip = g.window.size();
+                    dec.unknown_window = false;
+                    dec.text_only = false;
+                    fin = false;
+                    if (!decode_from(g.stop_bit)) {
+                        first_error = KC_EIO;
+                        ocv.notify_all();
+                        return KC_EIO;
+                    }
+                    secs += std::chrono::duration<double>(std::chrono::steady_clock::now() - b).count();
+                }
+                // the markers answer to the output before this share; the next
+                // share's to the last 32 KB of the output with this one
+                prev = g.window;
+                const uint64_t n = o.n - skip;
+                std::vector<uint8_t> nw;
+                bool rok;
+                if (n >= kcz::kWindow) {
+                    nw.resize(kcz::kWindow);
+                    rok = kcz::resolve(o.v.data() + o.n - kcz::kWindow, kcz::kWindow, prev.data(), (uint32_t)prev.size(),
+                                       nw.data());
+                } else {
+                    const uint64_t keep = std::min<uint64_t>(prev.size(), kcz::kWindow - n);
+                    nw.assign(prev.end() - keep, prev.end());
+                    nw.resize(keep + n);
+                    rok = kcz::resolve(o.v.data() + skip, n, prev.data(), (uint32_t)prev.size(), nw.data() + keep);
+                }
+                if (!rok) {
+                    first_error = KC_EIO;
+                    ocv.notify_all();
+                    return KC_EIO;
+                }
+                g.window.swap(nw);
+                g.stop_bit = stop;
+                if (fin) g.ended = true;
+            }
+            ++g.next_window;
+            ocv.notify_all();
+        }
+        // the share's bytes, in place over its symbols
+        uint8_t *bytes = reinterpret_cast<uint8_t *>(o.v.data());
+        const uint64_t n = dead ? 0 : o.n - skip;
+        if (!dead && !kcz::resolve(o.v.data() + skip, n, prev.data(), (uint32_t)prev.size(), bytes)) return KC_EIO;
+        uint32_t crc_n = 0;
+        for (uint64_t k = 0; k < n; k += 1u << 30)
+            crc_n = (uint32_t)crc32(crc_n, bytes + k, (uInt)std::min<uint64_t>(n - k, 1u << 30));
+        const uint64_t lines = (uint64_t)std::count(bytes, bytes + n, (uint8_t)'\n');
+        uint32_t state;
+        {
+            std::unique_lock<std::mutex> lk(omu);
+            ocv.wait(lk, [&] { return fo.next_phase == i || first_error != KC_OK; });
+            if (first_error != KC_OK) return first_error;
+            state = (uint32_t)(fo.lines & 3);
+            fo.lines += lines;
+            ++fo.next_phase;
+            ocv.notify_all();
+        }
+        const uint64_t w = frame_in_place(bytes, n, state);
+        std::unique_lock<std::mutex> lk(omu);
+        ocv.wait(lk, [&] { return fo.next_hand == i || first_error != KC_OK; });
+        if (first_error != KC_OK) return first_error;
+        uint64_t n_chunks = 0, n_in = n, n_framed = w;
+        int rc = KC_OK;
+        if (!dead) {
+            g.crc = (uint32_t)crc32_combine(g.crc, crc_n, (z_off_t)n);
+            g.isize += n;
+            rc = hand(fo, bytes, w, false, n_chunks);
+            if (rc == KC_OK && fin) {
+                // the member's trailer: CRC-32 and size (mod 2^32) of its bytes
+                const uint64_t t = (stop + 7) / 8;
+                if (t + 8 > g.size || le32(g.map + t) != g.crc || le32(g.map + t + 4) != (uint32_t)g.isize) rc = KC_EIO;
+                if (rc == KC_OK && t + 8 < g.size) rc = rest_members(g, t + 8, fo, n_in, n_framed, n_chunks);
+                if (rc == KC_OK) rc = hand(fo, nullptr, 0, true, n_chunks);  // the file's last read
+            }
+        }
+        ++fo.next_hand;
+        in_bytes += n_in;
+        framed += n_framed;
+        chunks += n_chunks;
+        inflate_s += secs;
+        ocv.notify_all();
+        return rc;
+    }
+
     void work()
     {
-        std::vector<uint8_t> in, buf, cbuf, out;
+        std::vector<uint8_t> in, buf, cbuf, out, prev;
+        kcz::Out16 sym;
         z_stream z{};
         bool z_ok = false;
         for (;;) {
@@ -748,7 +1044,9 @@ struct Ingest {
             if (si >= segs.size() || failed()) break;
             const Segment &sg = segs[si];
             int rc;
-            if (sg.bgzf) {
+            if (sg.kind == kGzipChunk) {
+                rc = run_gzip_chunk(sg, sym, prev);
+            } else if (sg.kind == kBgzfMembers) {
                 if (!z_ok) {
                     if (inflateInit2(&z, -15) != Z_OK) {
                         fail(KC_ENOMEM);

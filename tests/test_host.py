@@ -101,6 +101,67 @@ def test_gzip_stream_with_framing_helper(tmp_path, threads):
     assert lines == sorted(O.fastq_frame(a).split(b"\n")[:-1] + O.fastq_frame(b).split(b"\n")[:-1])
 
 
+def _gzip_fixed_codes(data):
+    """A gzip member of fixed-Huffman blocks only (no dynamic block to find)."""
+    import zlib
+    c = zlib.compressobj(6, zlib.DEFLATED, 31, 9, zlib.Z_FIXED)
+    return c.compress(data) + c.flush()
+
+
+@pytest.mark.parametrize("kind", ["gzip6", "parallel_writer", "fixed_codes", "members", "binary"])
+def test_gzip_member_in_shares(tmp_path, monkeypatch, kind):
+    """One gzip member cut into shares inflated on several threads (shares of
+    64 KB here): the framed reads equal the one-stream reader's, for a gzip -6
+    member, the parallel writer's sync-flushed blocks, fixed-code blocks (no
+    dynamic block to start from: every share decoded again in order), two
+    members (the second through zlib) and bytes that are not text."""
+    import gzip
+    import os
+    from ktcountermemorygreedy_amd import kc
+    from ktcountermemorygreedy_amd import workloads as W
+    fq = _uneven_fastq(2000, seed=31)
+    p = tmp_path / "r.fastq.gz"
+    if kind == "gzip6":
+        p.write_bytes(gzip.compress(fq, 6))
+    elif kind == "parallel_writer":
+        W.write_gzip_parallel(str(p), fq, level=6, threads=4, block=256 << 10)
+    elif kind == "fixed_codes":
+        p.write_bytes(_gzip_fixed_codes(fq))
+    elif kind == "members":
+        h = len(fq) // 3
+        h = fq.index(b"\n@", h) + 1
+        p.write_bytes(gzip.compress(fq[:h], 6) + gzip.compress(fq[h:], 1))
+    else:
+        fq = os.urandom(1 << 20) + fq[:1 << 20]
+        p.write_bytes(gzip.compress(fq, 6))
+    want = O.fastq_frame(fq)
+    want = want if not want or want.endswith(b"\n") else want + b"\n"
+    one, _ = kc.fastq_frame_files([str(p)], threads=1)
+    assert one.tobytes() == want
+    monkeypatch.setenv("KC_TEST_GZIP_CHUNK", str(64 << 10))
+    got, st = kc.fastq_frame_files([str(p)], threads=6)
+    assert got.tobytes() == want
+    assert st["input_bytes"] == len(fq) and st["threads"] == 6
+
+
+def test_gzip_member_in_shares_errors(tmp_path, monkeypatch):
+    """A member whose bytes no longer match its CRC, and a member cut short:
+    KC_EIO, as gzread reports them."""
+    import gzip
+    from ktcountermemorygreedy_amd import kc
+    fq = _uneven_fastq(1500, seed=33)
+    z = bytearray(gzip.compress(fq, 6))
+    monkeypatch.setenv("KC_TEST_GZIP_CHUNK", str(64 << 10))
+    bad = bytearray(z)
+    bad[-8] ^= 0x01  # the stored CRC-32
+    (tmp_path / "crc.fastq.gz").write_bytes(bytes(bad))
+    (tmp_path / "cut.fastq.gz").write_bytes(bytes(z[:len(z) * 2 // 3]))
+    for name in ("crc.fastq.gz", "cut.fastq.gz"):
+        with pytest.raises(kc.KCError) as e:
+            kc.fastq_frame_files([str(tmp_path / name)], threads=4)
+        assert e.value.status == kc.KC_EIO
+
+
 def test_bgzf_corrupt_member_is_an_error(tmp_path):
     """A member whose data no longer matches its CRC fails with KC_EIO, as
     gzread does; nothing hangs."""
