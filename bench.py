@@ -63,7 +63,8 @@ def parse_args():
                          "passes into a table of 1/P the slots (the out-of-HBM fallback); a 'multipass' object")
     ap.add_argument("--gz-format", default="bgzf", choices=["bgzf", "gzip"],
                     help="ingest_bgzf: the one file as BGZF (members inflated on all host threads) or as one "
-                         "plain gzip member (one inflate stream, as gzip -6 writes it)")
+                         "plain gzip member as gzip -6 writes it (its deflate stream decoded in shares on all host "
+                         "threads)")
     ap.add_argument("--no-host-batch", action="store_true",
                     help="skip the host-batched (PCIe-inclusive) line of uneven_50mbp")
     return ap.parse_args()
@@ -245,8 +246,8 @@ def bench_ingest_bgzf(args):
     kc_count_fastq_files with 16 host threads -- host inflate, framing, PCIe
     and counting all inside the timed region, so it is never the headline.
     --gz-format bgzf: BGZF, whose member runs the threads inflate in
-    parallel; gzip: one plain gzip member, one inflate stream (a second thread
-    frames).  Checked against the same reads counted from HBM (equal
+    parallel; gzip: one plain gzip member, whose deflate stream the threads
+    decode in shares from guessed block starts, joined in order.  Checked against the same reads counted from HBM (equal
     histograms and k-mer totals).  The reference Tree binary counts a sample
     of the same format on the same host beside it."""
     import shutil

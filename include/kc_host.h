@@ -36,9 +36,12 @@ void kc_host_free(void *p);
  * BGZF file (blocked gzip, as htslib's bgzip writes it: gzip members whose
  * "BC" extra subfield gives their size): threads inflate runs of about 1 MB
  * of members, each run is framed from the record phase the newline counts of
- * the runs before it give, and the runs are handed over in file order.  The
- * reference's gzread inflates such a file as one stream on one thread; the
- * counts are the same.  Returns when every read is counted.  stats may be
+ * the runs before it give, and the runs are handed over in file order.  A
+ * single gzip member of 64 MB or more is cut into shares of its deflate
+ * stream that threads decode from guessed block starts, checked and joined in
+ * order (the CRC-32 and size checked against the trailer).  The reference's
+ * gzread inflates either file as one stream on one thread; the counts are the
+ * same.  Returns when every read is counted.  stats may be
  * NULL. */
 typedef struct kc_ingest_stats {
     uint64_t files;
