@@ -219,38 +219,118 @@ struct Decoder {
         } else {
             return kBlockError;
         }
-        for (;;) {
-            if (in.past_end()) return kBlockError;
-            const int s = lit.decode(in);
-            if (s < 0) return kBlockError;
-            if (s < 256) {
-                if (text_only && (s < 9 || s > 126)) return kBlockError;
-                out.reserve_more(1);
-                out.v[out.n++] = (uint16_t)s;
-                continue;
-            }
-            if (s == 256) break;
-            const int li = s - 257;
-            if (li >= 29) return kBlockError;
-            const uint32_t len = kLenBase[li] + (uint32_t)in.bits(kLenExtra[li]);
-            const int ds = dist.decode(in);
-            if (ds < 0 || ds >= 30) return kBlockError;
-            const uint32_t d = kDistBase[ds] + (uint32_t)in.bits(kDistExtra[ds]);
-            out.reserve_more(len);
-            uint16_t *o = out.v.data();
-            const int64_t src = (int64_t)out.n - (int64_t)d;
-            if (src >= 0) {
-                for (uint32_t k = 0; k < len; ++k) o[out.n + k] = o[src + k];  // overlapping copies repeat
+        if (!codes(out)) return kBlockError;
+        return final_block ? kBlockFinal : kBlockOk;
+    }
+
+    // The symbols of a Huffman-coded block up to its end-of-block code, with
+    // a 64-bit bit buffer refilled once per symbol: a literal/length code,
+    // its extra bits, a distance code and its extra bits take at most 48
+    // bits, and a refill leaves at least 56.
+    bool codes(Out16 &out)
+    {
+        const uint8_t *base = in.base;
+        const uint64_t nbytes = in.nbytes;
+        uint64_t bp = in.pos >> 3;  // next byte to load (past the end: zeros)
+        uint64_t buf = 0;
+        int cnt = 0;
+        auto refill = [&]() {
+            if (bp + 8 <= nbytes) {
+                uint64_t v;
+                std::memcpy(&v, base + bp, 8);
+                buf |= v << cnt;
+                bp += (uint64_t)((63 - cnt) >> 3);
+                cnt |= 56;
             } else {
-                if (!unknown_window || src < -(int64_t)kWindow) return kBlockError;
-                for (uint32_t k = 0; k < len; ++k) {
-                    const int64_t p = src + (int64_t)k;
-                    o[out.n + k] = p < 0 ? (uint16_t)(kMarker + kWindow + p) : o[p];
+                while (cnt <= 56) {
+                    if (bp < nbytes) buf |= (uint64_t)base[bp] << cnt;
+                    ++bp;
+                    cnt += 8;
                 }
             }
-            out.n += len;
+        };
+        refill();
+        buf >>= (in.pos & 7);
+        cnt -= (int)(in.pos & 7);
+        auto take = [&](int n) {
+            const uint64_t v = buf & ((1ull << n) - 1);
+            buf >>= n;
+            cnt -= n;
+            return (uint32_t)v;
+        };
+        // the next symbol of h from the buffer, or -1 for no code
+        auto sym_of = [&](const Huff &h) -> int {
+            const uint16_t e = h.fast[buf & ((1u << Huff::FB) - 1)];
+            if (e) {
+                const int l = e >> 12;
+                buf >>= l;
+                cnt -= l;
+                return e & 0xFFF;
+            }
+            int code = 0, first = 0, index = 0;
+            for (int l = 1; l < 16; ++l) {
+                code |= (int)((buf >> (l - 1)) & 1);
+                const int c = h.count[l];
+                if (code - c < first) {
+                    buf >>= l;
+                    cnt -= l;
+                    return h.sym[index + (code - first)];
+                }
+                index += c;
+                first += c;
+                first <<= 1;
+                code <<= 1;
+            }
+            return -1;
+        };
+        uint64_t n = out.n;
+        uint16_t *o = out.v.data();
+        uint64_t cap = out.v.size();
+        const uint64_t end_bits = nbytes * 8;
+        bool ok = false;
+        for (;;) {
+            refill();
+            if (bp * 8 - (uint64_t)cnt > end_bits) break;  // read past the input
+            if (n + 258 > cap) {
+                out.n = n;
+                out.reserve_more(258);
+                o = out.v.data();
+                cap = out.v.size();
+            }
+            const int s = sym_of(lit);
+            if (s < 0) break;
+            if (s < 256) {
+                if (text_only && (s < 9 || s > 126)) break;
+                o[n++] = (uint16_t)s;
+                continue;
+            }
+            if (s == 256) {
+                ok = true;
+                break;
+            }
+            const int li = s - 257;
+            if (li >= 29) break;
+            const uint32_t len = kLenBase[li] + take(kLenExtra[li]);
+            const int ds = sym_of(dist);
+            if (ds < 0 || ds >= 30) break;
+            const uint32_t d = kDistBase[ds] + take(kDistExtra[ds]);
+            const int64_t src = (int64_t)n - (int64_t)d;
+            if (src >= 0) {
+                if (d >= len) std::memcpy(o + n, o + src, (size_t)len * 2);
+                else
+                    for (uint32_t k = 0; k < len; ++k) o[n + k] = o[src + k];  // overlapping copies repeat
+            } else {
+                if (!unknown_window || src < -(int64_t)kWindow) break;
+                for (uint32_t k = 0; k < len; ++k) {
+                    const int64_t q = src + (int64_t)k;
+                    o[n + k] = q < 0 ? (uint16_t)(kMarker + kWindow + q) : o[q];
+                }
+            }
+            n += len;
         }
-        return final_block ? kBlockFinal : kBlockOk;
+        out.n = n;
+        in.pos = bp * 8 - (uint64_t)cnt;
+        return ok && !in.past_end();
     }
 };
 
