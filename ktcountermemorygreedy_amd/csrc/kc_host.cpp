@@ -913,6 +913,9 @@ struct Ingest {
             start = kcz::find_block_start(g.map, g.size, sbit, lim);
             ok = start < lim;
             dec.unknown_window = true;
+            // a guess gone wrong must not decode without bound: FASTQ inflates
+            // a few times its size; more than 64x is redone from the known output
+            dec.max_out = 64 * g.chunk + (1u << 20);
         }
         // blocks from `from` until one starts in the next share or the member ends
         auto decode_from = [&](uint64_t from) {
@@ -952,6 +955,7 @@ struct Ingest {
                     skip = g.window.size();
                     dec.unknown_window = false;
                     dec.text_only = false;
+                    dec.max_out = ~0ull;
                     fin = false;
                     if (!decode_from(g.stop_bit)) {
                         first_error = KC_EIO;

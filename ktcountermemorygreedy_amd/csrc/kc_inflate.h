@@ -136,6 +136,7 @@ struct Decoder {
     Huff lit, dist;
     bool unknown_window = false;  // back-references before the first output become markers
     bool text_only = false;       // literals must be text bytes (9..126): a plausibility check for a guessed start
+    uint64_t max_out = ~0ull;     // more output symbols than this fail the block (a guessed start gone wrong)
 
     bool read_dynamic_tables()
     {
@@ -203,7 +204,7 @@ struct Decoder {
             if (b + 4 > in.nbytes) return kBlockError;
             const uint16_t len = (uint16_t)(in.base[b] | (in.base[b + 1] << 8));
             const uint16_t nlen = (uint16_t)(in.base[b + 2] | (in.base[b + 3] << 8));
-            if ((uint16_t)~len != nlen || b + 4 + len > in.nbytes) return kBlockError;
+            if ((uint16_t)~len != nlen || b + 4 + len > in.nbytes || out.n + len > max_out) return kBlockError;
             out.reserve_more(len);
             for (uint32_t k = 0; k < len; ++k) {
                 const uint8_t c = in.base[b + 4 + k];
@@ -291,6 +292,7 @@ struct Decoder {
         for (;;) {
             refill();
             if (bp * 8 - (uint64_t)cnt > end_bits) break;  // read past the input
+            if (n > max_out) break;
             if (n + 258 > cap) {
                 out.n = n;
                 out.reserve_more(258);
