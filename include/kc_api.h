@@ -94,6 +94,12 @@ const char *kc_strerror(int status);
 
 /* ---- context ---------------------------------------------------------- */
 int kc_ctx_create(int device, kc_ctx **out);
+/* A context whose stream leaves compute units 0 .. left_cus-1 to other
+ * streams (a CU mask, hipExtStreamCreateWithCUMask); a plain context where the
+ * runtime has no CU masks or the device has no more than 2*left_cus CUs.  No
+ * reference counterpart: the serving loop runs its count jobs on such
+ * contexts so that lookups never wait for a count kernel's blocks. */
+int kc_ctx_create_leaving_cus(int device, uint32_t left_cus, kc_ctx **out);
 void kc_ctx_destroy(kc_ctx *ctx);
 int kc_ctx_sync(kc_ctx *ctx);
 /* hipStream_t of the context, for callers that record events on it. */

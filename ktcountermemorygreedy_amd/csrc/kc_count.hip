@@ -710,6 +710,26 @@ extern "C" int kc_ctx_create(int device, kc_ctx **out)
     return KC_OK;
 }
 
+extern "C" int kc_ctx_create_leaving_cus(int device, uint32_t left_cus, kc_ctx **out)
+{
+    if (!out) return KC_EINVAL;
+    int rc = kc_ctx_create(device, out);
+    kc_ctx *c = *out;
+    // a mask only where enough CUs stay on the stream to be worth it
+    if (rc != KC_OK || !left_cus || c->cu_count <= 2 * left_cus) return rc;
+    std::vector<uint32_t> mask((c->cu_count + 31) / 32, 0u);
+    for (uint32_t cu = left_cus; cu < c->cu_count; ++cu) mask[cu / 32] |= 1u << (cu % 32);
+    hipStream_t ms = nullptr;
+    if (hipExtStreamCreateWithCUMask(&ms, (uint32_t)mask.size(), mask.data()) == hipSuccess) {
+        // the masked stream replaces the plain one; a runtime without CU masks keeps it
+        (void)hipStreamDestroy(c->stream);
+        c->stream = ms;
+    } else {
+        (void)hipGetLastError();
+    }
+    return KC_OK;
+}
+
 extern "C" int kc_dev_alloc(kc_ctx *ctx, uint64_t bytes, void **out)
 {
     if (!ctx || !out) return KC_EINVAL;

@@ -10,7 +10,10 @@
 //   default) and work on one hardware queue runs in order, so a lookup on a
 //   stream that shares its queue with a running job's stream waits for that
 //   job's kernels; a lookup stream next to a bounded number of job streams
-//   keeps lookups off the jobs' queues.  Requests on one table are
+//   keeps lookups off the jobs' queues.  The job streams also carry a CU
+//   mask that leaves kLookupCUs compute units to the lookup stream
+//   (kc_ctx_create_leaving_cus), so a lookup launch never waits for a count
+//   kernel's blocks to drain from every CU.  Requests on one table are
 //   serialized by the table's mutex.
 // * Worker threads take requests from one FIFO.  A count job is admitted
 //   only when its device footprint fits the HBM measured free at that moment
@@ -55,6 +58,8 @@ using Clock = std::chrono::steady_clock;
 constexpr uint64_t kMergeKeys = 4ull << 20;
 // Count jobs running at once (one job context each).
 constexpr uint32_t kJobStreams = 3;
+// Compute units the job streams leave to lookups (CU masks on the job streams).
+constexpr uint32_t kLookupCUs = 16;
 
 struct Table {
     uint64_t id = 0;
@@ -396,7 +401,7 @@ extern "C" int kc_server_create(int device, uint32_t workers, uint64_t hbm_budge
         rc = kc_ctx_create(device, &s->lookup_ctx);
         for (uint32_t i = 0; i < jobs && rc == KC_OK; ++i) {
             kc_ctx *c = nullptr;
-            rc = kc_ctx_create(device, &c);
+            rc = kc_ctx_create_leaving_cus(device, kLookupCUs, &c);
             if (rc == KC_OK) {
                 try {
                     s->job_ctx.push_back(c);

@@ -95,6 +95,7 @@ _lib = None
 EXPORTS = {
     "kc_strerror": (C.c_char_p, [C.c_int]),
     "kc_ctx_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "kc_ctx_create_leaving_cus": (C.c_int, [C.c_int, C.c_uint32, C.POINTER(C.c_void_p)]),
     "kc_ctx_destroy": (None, [C.c_void_p]),
     "kc_ctx_sync": (C.c_int, [C.c_void_p]),
     "kc_ctx_stream": (C.c_void_p, [C.c_void_p]),
@@ -331,11 +332,16 @@ class _Native:
 
 
 class Context:
-    """kc_ctx: one device and one HIP stream."""
+    """kc_ctx: one device and one HIP stream.  left_cus > 0 keeps the
+    stream's kernels off compute units 0 .. left_cus-1 (a CU mask), leaving
+    them to other streams."""
 
-    def __init__(self, device=0):
+    def __init__(self, device=0, left_cus=0):
         h = C.c_void_p()
-        check(lib().kc_ctx_create(device, C.byref(h)), "kc_ctx_create")
+        if left_cus:
+            check(lib().kc_ctx_create_leaving_cus(device, left_cus, C.byref(h)), "kc_ctx_create_leaving_cus")
+        else:
+            check(lib().kc_ctx_create(device, C.byref(h)), "kc_ctx_create")
         self.h = h
         self.device = device
         self._tables = _Native(h, lambda _: None)  # tables (and their batches) destroyed before the context

@@ -238,3 +238,21 @@ def test_export_filter_wraps_at_u32_max(ctx, filter_min):
     assert t.dump().count(b"\n") == len(want)
     assert t.query(keys).tolist() == [0xFFFFFFFF, 0xFFFFFFFE, 7, 0xFFFFFFFF]
     t.close()
+
+
+@pytest.mark.parametrize("path", ["direct", "partitioned"])
+def test_count_on_context_leaving_cus(path):
+    """A context whose stream leaves 16 CUs to other streams (the serving
+    loop's job contexts) counts the 5 Mbp reads to the same golden dump."""
+    from ktcountermemorygreedy_amd import kc
+    from ktcountermemorygreedy_amd import workloads as W
+    g = load_json("count_5mbp.json")
+    seq = W.COUNT_5MBP.reads(fastq=False).tobytes()
+    c = kc.Context(0, left_cus=16)
+    try:
+        dump, dev_hist, _, st = gpu_views(c, seq, 31, g["count_max"], path=path)
+    finally:
+        c.close()
+    assert st["kmers"] == g["kmers"]
+    assert hashlib.sha256(dump).hexdigest() == g["dump_sha256"]
+    assert hashlib.sha256(dev_hist).hexdigest() == g["hist_sha256"]
