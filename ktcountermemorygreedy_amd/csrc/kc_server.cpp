@@ -36,6 +36,7 @@
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <memory>
@@ -58,8 +59,18 @@ using Clock = std::chrono::steady_clock;
 constexpr uint64_t kMergeKeys = 4ull << 20;
 // Count jobs running at once (one job context each).
 constexpr uint32_t kJobStreams = 3;
-// Compute units the job streams leave to lookups (CU masks on the job streams).
+// Compute units the job streams leave to lookups (CU masks on the job streams);
+// KC_LOOKUP_CUS overrides it (0 = no mask).
 constexpr uint32_t kLookupCUs = 16;
+
+uint32_t lookup_cus()
+{
+    const char *e = std::getenv("KC_LOOKUP_CUS");
+    if (!e || !*e) return kLookupCUs;
+    char *end = nullptr;
+    unsigned long v = std::strtoul(e, &end, 10);
+    return (end && !*end && v <= 1024) ? (uint32_t)v : kLookupCUs;
+}
 
 struct Table {
     uint64_t id = 0;
@@ -399,9 +410,10 @@ extern "C" int kc_server_create(int device, uint32_t workers, uint64_t hbm_budge
         // the lookup context first, then the job contexts
         const uint32_t jobs = std::min(workers, kJobStreams);
         rc = kc_ctx_create(device, &s->lookup_ctx);
+        const uint32_t left = lookup_cus();
         for (uint32_t i = 0; i < jobs && rc == KC_OK; ++i) {
             kc_ctx *c = nullptr;
-            rc = kc_ctx_create_leaving_cus(device, kLookupCUs, &c);
+            rc = kc_ctx_create_leaving_cus(device, left, &c);
             if (rc == KC_OK) {
                 try {
                     s->job_ctx.push_back(c);
