@@ -30,7 +30,9 @@
 //   staging directly: no copy-engine transfer, which would queue behind the
 //   count jobs' host-to-device streaming.
 // * A count job with the Bloom split builds its table's FIRST-list index
-//   before it completes, so lookups never pay for it.
+//   before it completes, and every job allocates its table's page-locked
+//   lookup staging (kStagingKeys keys), so lookups pay for neither; only a
+//   merged lookup larger than the staging grows it.
 // * Latency = completion time - submit time, per request.
 #include <algorithm>
 #include <atomic>
@@ -59,6 +61,10 @@ using Clock = std::chrono::steady_clock;
 constexpr uint64_t kMergeKeys = 4ull << 20;
 // Count jobs running at once (one job context each).
 constexpr uint32_t kJobStreams = 3;
+// Keys of lookup staging a table gets when its job completes.
+constexpr uint64_t kStagingKeys = 65536;
+
+uint64_t next_pow2(uint64_t v);
 // Compute units the job streams leave to lookups (CU masks on the job streams);
 // KC_LOOKUP_CUS overrides it (0 = no mask).
 constexpr uint32_t kLookupCUs = 16;
@@ -86,6 +92,19 @@ struct Table {
     // the lookup kernel reads and writes in place (at their device addresses)
     void *hkeys = nullptr, *hcounts = nullptr, *dkeys = nullptr, *dcounts = nullptr;
     uint64_t qcap = 0;
+    // Page-locked lookup staging for at least `keys` keys (65536 at least).
+    int reserve_staging(uint64_t keys)
+    {
+        free_staging();
+        const uint64_t cap = std::max<uint64_t>(next_pow2(keys), kStagingKeys);
+        int rc = kc_host_alloc_pinned(sctx, cap * 8 * nw, &hkeys);
+        if (rc == KC_OK) rc = kc_host_alloc_pinned(sctx, cap * 4, &hcounts);
+        if (rc == KC_OK) rc = kc_host_device_ptr(sctx, hkeys, &dkeys);
+        if (rc == KC_OK) rc = kc_host_device_ptr(sctx, hcounts, &dcounts);
+        if (rc == KC_OK) qcap = cap;
+        else free_staging();
+        return rc;
+    }
     void free_staging()
     {
         if (hkeys) kc_host_free_pinned(sctx, hkeys);
@@ -290,6 +309,8 @@ struct kc_server {
         if (rc == KC_OK) rc = kc_table_stats(tab->t, &st);
         if (rc == KC_OK && st.errors) rc = KC_EFULL;
         if (rc == KC_OK && r->bloom) rc = kc_table_query(tab->t, nullptr, 0, nullptr);  // builds the FIRST-list index
+        // the lookup staging too, so that a table's first lookup allocates nothing
+        if (rc == KC_OK) rc = tab->reserve_staging(kStagingKeys);
         if (rc == KC_OK) table_bytes = r->resident;
         r->res.kmers = st.kmers;
         r->res.distinct = st.distinct;
@@ -320,16 +341,7 @@ struct kc_server {
         uint64_t total = 0;
         for (Request *r : batch) total += r->n;
         int rc = KC_OK;
-        if (total > tab->qcap) {
-            tab->free_staging();
-            const uint64_t cap = std::max<uint64_t>(next_pow2(total), 65536);
-            rc = kc_host_alloc_pinned(tab->sctx, cap * 8 * nw, &tab->hkeys);
-            if (rc == KC_OK) rc = kc_host_alloc_pinned(tab->sctx, cap * 4, &tab->hcounts);
-            if (rc == KC_OK) rc = kc_host_device_ptr(tab->sctx, tab->hkeys, &tab->dkeys);
-            if (rc == KC_OK) rc = kc_host_device_ptr(tab->sctx, tab->hcounts, &tab->dcounts);
-            if (rc == KC_OK) tab->qcap = cap;
-            else tab->free_staging();
-        }
+        if (total > tab->qcap) rc = tab->reserve_staging(total);
         if (rc == KC_OK && total) {
             uint64_t at = 0;
             for (Request *r : batch) {
